@@ -1,0 +1,304 @@
+/*
+ * kp_image.h -- C ABI of the MI355X-native KPolaris per-pixel imaging kernel.
+ *
+ * Every entry point is plain C: structs of scalars, caller-owned pointers,
+ * a status code, and no C++ types or exceptions across the boundary.
+ * Per-pixel arrays use the reference's pixel order p = iy * nx + ix
+ * (reference: src/camera/camera.hpp:96-99).
+ *
+ * Struct layouts mirror the reference's by-value parameter objects so that a
+ * reference-side binding is a field-by-field copy:
+ *   kp_camera_params      <- kpolaris::CameraParams      (src/camera/camera.hpp:12-25)
+ *   kp_emission_selection <- kpolaris::EmissionSelection (src/radiation/emission_selection.hpp:11-19)
+ *   kp_pass_a_params      <- kpolaris::PassAParams       (src/geodesic/pass_a_core.hpp:10-36)
+ *   kp_riaf_model         <- kpolaris::RIAFAnalyticRadiationModel (src/model/riaf.hpp:17-52)
+ *   kp_image_data         <- ImageHostData arrays         (src/image/result.hpp:9-31)
+ */
+#ifndef KP_IMAGE_H
+#define KP_IMAGE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KP_ABI_VERSION 1
+
+/* Status codes.  The reference throws std::runtime_error / std::invalid_argument
+ * (src/image/riaf.cpp:15-18, src/image/batch_driver.hpp:40-41); here no
+ * exception crosses the ABI and kp_last_error() holds the message. */
+typedef enum kp_status {
+    KP_OK = 0,
+    KP_INVALID_ARGUMENT = 1,
+    KP_UNSUPPORTED = 2,
+    KP_HIP_ERROR = 3,
+    KP_OUT_OF_MEMORY = 4
+} kp_status;
+
+/* Per-pixel termination codes, same values and meanings as
+ * kpolaris::TerminationReason (src/common/types.hpp:37-48). */
+typedef enum kp_termination_reason {
+    KP_REASON_NONE = 0,
+    KP_REASON_REACHED_CAMERA = 1,
+    KP_REASON_REACHED_INNER_BOUNDARY = 2,
+    KP_REASON_ESCAPED_DOMAIN = 3,
+    KP_REASON_MAX_STEPS = 4,
+    KP_REASON_INVALID_STATE = 5,
+    KP_REASON_SLOW_LIGHT_TIME_EXHAUSTED = 6,
+    KP_REASON_ADAPTIVE_STEP_UNDERFLOW = 7,
+    KP_REASON_METRIC_TIME_EXHAUSTED = 8,
+    KP_REASON_REACHED_DIRECT_TURN = 9
+} kp_termination_reason;
+
+/* kpolaris::CoordinateSystem (src/common/types.hpp:50-56). */
+typedef enum kp_coordinate_system {
+    KP_COORD_CARTESIAN_KS = 0,
+    KP_COORD_BOYER_LINDQUIST = 1,
+    KP_COORD_SPHERICAL_KS = 2,
+    KP_COORD_FMKS = 3,
+    KP_COORD_MKS = 4
+} kp_coordinate_system;
+
+/* kpolaris::CameraModel (src/camera/camera.hpp:7-10). */
+typedef enum kp_camera_model {
+    KP_CAMERA_PINHOLE = 0,
+    KP_CAMERA_PARALLEL_PLANE = 1
+} kp_camera_model;
+
+/* Thermal fits (src/radiation/thermal_synchrotron.hpp:9-17) and the
+ * nonthermal distributions (src/radiation/nonthermal_synchrotron.hpp). */
+typedef enum kp_emission_type {
+    KP_EMISSION_THERMAL_PANDYA = 1,
+    KP_EMISSION_KAPPA = 2,
+    KP_EMISSION_POWER_LAW = 3,
+    KP_EMISSION_THERMAL_DEXTER = 4
+} kp_emission_type;
+
+/* Arithmetic of the per-pixel kernel. */
+typedef enum kp_precision {
+    /* FP64 throughout: the parity kernel. */
+    KP_PRECISION_FP64 = 0,
+    /* FP32 stepping and coefficients with FP64 kept for near-horizon and
+     * turning-point steps, the step-error estimate and large Faraday depth. */
+    KP_PRECISION_MIXED = 1
+} kp_precision;
+
+typedef struct kp_camera_params {
+    int32_t nx;
+    int32_t ny;
+    double radius;
+    double inclination;
+    double fov;
+    double fov_y;
+    double xspan;
+    double yspan;
+    double x_offset;
+    double y_offset;
+    int32_t model;      /* kp_camera_model */
+    int32_t reserved0;
+} kp_camera_params;
+
+typedef struct kp_emission_selection {
+    double equatorial_h_over_r;
+    int32_t equatorial_samples;
+    int32_t faraday_rotation;
+} kp_emission_selection;
+
+typedef struct kp_pass_a_params {
+    kp_camera_params camera;
+    kp_emission_selection emission_selection;
+    double mass;
+    double spin;
+    double inner_radius;
+    double outer_radius;
+    double step;
+    int32_t max_steps;
+    int32_t adaptive;
+    int32_t direct_only;
+    int32_t coordinate_system; /* kp_coordinate_system */
+    double adaptive_tolerance;
+    double min_step;
+    double max_step;
+    double max_radiation_step;
+    double max_radiation_depth;
+    double max_absorption_depth;
+    double max_faraday_depth;
+    double fmks_startx1;
+    double fmks_hslope;
+    double fmks_mks_smooth;
+    double fmks_poly_alpha;
+    double fmks_poly_xt;
+    double fmks_poly_norm;
+} kp_pass_a_params;
+
+typedef struct kp_riaf_model {
+    double nth0;
+    double Te0;
+    double disk_h;
+    double pow_nth;
+    double pow_T;
+    double ne_unit;
+    double te_unit;
+    double b_unit;
+    double freq_cgs;
+    double mbh_solar;
+    double r_min;
+    double r_max;
+    double horizon_buffer;
+    double emission_scale;
+    double absorption_scale;
+    double faraday_scale;
+    double max_pol_frac;
+    double nonthermal_kappa;
+    int32_t variable_kappa;
+    int32_t emission_type;    /* kp_emission_type */
+    double variable_kappa_min;
+    double variable_kappa_interp_start;
+    double variable_kappa_max;
+    double powerlaw_p;
+    double powerlaw_eta;
+    double powerlaw_gamma_min;
+    double powerlaw_gamma_max;
+    double powerlaw_gamma_cutoff;
+    double keplerian_factor;
+    double infall_factor;
+    double min_frequency_scale;
+    double min_sin_theta;
+    double min_thetae;
+    double min_radius;
+} kp_riaf_model;
+
+/* Per-pixel outputs.  Every pointer may be NULL (that output is skipped).
+ * Stokes values are invariant (I/nu^3), in the camera basis, not EVPA-flipped
+ * and not masked by reason: the host packing rules apply afterwards, exactly
+ * as for ImageHostData. */
+typedef struct kp_image_data {
+    double* image_i;
+    double* image_q;
+    double* image_u;
+    double* image_v;
+    double* closure_x;
+    double* closure_k;
+    double* final_null;
+    double* frame_error;
+    double* det_r;
+    double* overlap_r11;
+    double* overlap_r12;
+    double* overlap_r21;
+    double* overlap_r22;
+    double* basis_identity_error;
+    double* basis_rotation_angle;
+    int32_t* pass_a_steps;
+    int32_t* steps;
+    int32_t* reason;
+} kp_image_data;
+
+/* Launch configuration; no reference counterpart (Kokkos chose it there). */
+typedef struct kp_run_options {
+    int32_t precision;        /* kp_precision */
+    int32_t split_transport;  /* 1: pass A and pass B kernels (reference default) */
+    int32_t block_size;       /* rays per workgroup; 0 = tuned default */
+    int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread */
+    int32_t waves_per_cu;     /* persistent grid size; 0 = occupancy-derived */
+    int32_t reserved0;
+    void* stream;             /* hipStream_t, NULL = default stream */
+} kp_run_options;
+
+/* Work counters and device timings of one run.  The unit counts are what the
+ * roofline's algorithmic FLOPs are priced on. */
+typedef struct kp_image_stats {
+    double pass_a_ms;
+    double pass_b_ms;
+    double kernel_ms;
+    int64_t rays;
+    int64_t pass_a_steps;       /* accepted steps, pass A */
+    int64_t pass_b_steps;       /* accepted steps, pass B */
+    int64_t rhs_evals;          /* transport_rhs evaluations, both passes */
+    int64_t coefficient_evals;  /* plasma coefficient evaluations */
+    int64_t transfer_steps;     /* Strang transfer steps */
+    int64_t fp64_guard_steps;   /* mixed precision: steps taken in FP64 */
+} kp_image_stats;
+
+/* ABI version and last error message of the calling thread. */
+int32_t kp_abi_version(void);
+const char* kp_last_error(void);
+
+/* Number of visible HIP devices; 0 when none (never an error). */
+int32_t kp_device_count(void);
+
+/*
+ * RIAF image in Boyer-Lindquist coordinates: pass A + pass B per pixel.
+ * Replaces run_riaf_image(const PassAParams&, RIAFAnalyticRadiationModel,
+ * int radiation_substeps, int timing, int split_transport)
+ * (src/image/riaf.hpp:12-17, dispatch src/image/riaf.cpp:21-45,
+ * kernels src/image/split_driver.hpp:41-104).
+ * `out` holds DEVICE pointers; work is stream-ordered on options->stream and
+ * the call returns after the kernels are queued unless `stats` is non-NULL,
+ * in which case it synchronises to read counters and event times.
+ * radiation_substeps is accepted and ignored, as on the reference's segment
+ * path (src/geodesic/pass_b.hpp:526).
+ */
+kp_status kp_run_riaf_image(const kp_pass_a_params* params,
+                            const kp_riaf_model* model,
+                            int32_t radiation_substeps,
+                            const kp_run_options* options,
+                            kp_image_data* out,
+                            kp_image_stats* stats);
+
+/* Same computation with HOST output pointers; synchronous. */
+kp_status kp_run_riaf_image_host(const kp_pass_a_params* params,
+                                 const kp_riaf_model* model,
+                                 int32_t radiation_substeps,
+                                 const kp_run_options* options,
+                                 kp_image_data* out,
+                                 kp_image_stats* stats);
+
+/* Defaults of the reference's structs (field initialisers of CameraParams,
+ * PassAParams and RIAFAnalyticRadiationModel) and of kp_run_options. */
+void kp_pass_a_params_default(kp_pass_a_params* params);
+void kp_riaf_model_default(kp_riaf_model* model);
+void kp_run_options_default(kp_run_options* options);
+
+/* ---- Host option layer (the reference CLI's rules) ------------------------
+ * Mirrors the image tool's Options struct subset on the hot path
+ * (tools/kpolaris_model_image_options.hpp:18-99), key=value parsing with the
+ * same key names (tools/kpolaris_model_image_options.cpp), camera-extent
+ * resolution (tools/kpolaris_model_image_options.cpp:136-180), PassAParams
+ * assembly (tools/kpolaris_model_image.cpp:3940-3990), RIAF model fill and
+ * radius defaults (tools/kpolaris_model_image.cpp:4925-4962), and the flux
+ * scale (tools/kpolaris_model_image.cpp:558-595). */
+typedef struct kp_options kp_options;
+
+kp_options* kp_options_create(void);
+void kp_options_destroy(kp_options* options);
+/* Set one key (e.g. "spin", "0.94").  Unknown keys return KP_UNSUPPORTED. */
+kp_status kp_options_set(kp_options* options, const char* key, const char* value);
+/* Parse a parameter file of key=value lines ('#' comments). */
+kp_status kp_options_load_file(kp_options* options, const char* path);
+/* Resolve camera extents and radii, then fill PassAParams and the RIAF model.
+ * intensity_to_flux_jy_per_pixel receives the flux scale (NaN if invalid). */
+kp_status kp_options_resolve_riaf(kp_options* options,
+                                  kp_pass_a_params* params,
+                                  kp_riaf_model* model,
+                                  double* intensity_to_flux_jy_per_pixel,
+                                  int32_t* evpa_qu_sign);
+
+/* Packing rules of pack_transport_frequency_image
+ * (tools/kpolaris_model_image.cpp:2668-2692, tools/kpolaris_evpa.hpp:15-17):
+ * pixels with reason != 1 are zeroed, Q and U take evpa_qu_sign, and the
+ * nu^3 factor turns invariant into observed intensity.  Host arrays.
+ * Any of i_nu..v_nu may be NULL; flux_jy[4] receives the I,Q,U,V fluxes. */
+kp_status kp_pack_image(const double* image_i, const double* image_q,
+                        const double* image_u, const double* image_v,
+                        const int32_t* reason, int64_t npix,
+                        double frequency, int32_t evpa_qu_sign,
+                        double intensity_to_flux_jy_per_pixel,
+                        double* i_nu, double* q_nu, double* u_nu, double* v_nu,
+                        double* flux_jy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KP_IMAGE_H */

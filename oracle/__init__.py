@@ -1,0 +1,154 @@
+"""TEST INFRASTRUCTURE ONLY: loader for the CPU oracle (oracle/kp_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this package, and only as the checker.  The product library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from kpolaris_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libkp_oracle.so"
+
+_lib = None
+
+
+class State(C.Structure):
+    _fields_ = [("v", C.c_double * 16)]
+
+
+class Coeffs(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("jI", "jQ", "jU", "jV", "aI", "aQ", "aU", "aV", "rQ", "rU", "rV")]
+
+    def as_array(self):
+        return np.array([getattr(self, n) for n, _ in self._fields_])
+
+
+class Stokes(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("I", "Q", "U", "V")]
+
+
+class Perturbation(C.Structure):
+    _fields_ = [("parameter", C.c_int32), ("reserved0", C.c_int32), ("log_scale", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in
+                ("rhs_evals", "coefficient_evals", "transfer_steps", "pass_a_steps",
+                 "pass_b_steps")]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    L = C.CDLL(str(LIB_PATH))
+    d, i32 = C.c_double, C.c_int32
+    dp = C.POINTER(C.c_double)
+    sp = C.POINTER(State)
+    L.kpo_bl_transport_rhs.argtypes = [d, d, sp, sp]
+    L.kpo_bl_transport_rhs_generic.argtypes = [d, d, sp, sp]
+    L.kpo_bl_gcov.argtypes = [d, d, dp, dp]
+    L.kpo_bl_gcon.argtypes = [d, d, dp, dp]
+    L.kpo_bl_dgcov.argtypes = [d, d, dp, dp]
+    L.kpo_bl_connection.argtypes = [d, d, dp, dp]
+    L.kpo_bl_dot.argtypes = [d, d, dp, dp, dp]
+    L.kpo_bl_dot.restype = d
+    L.kpo_bl_rk4_step.argtypes = [d, d, sp, d, sp]
+    L.kpo_bl_adaptive_rk4_step.argtypes = [d, d, sp, d, d, d, d, i32, sp, sp, dp, dp, dp]
+    L.kpo_bl_adaptive_rk4_step.restype = i32
+    L.kpo_state_error_norm.argtypes = [sp, sp]
+    L.kpo_state_error_norm.restype = d
+    L.kpo_pixel_offsets.argtypes = [i32, C.POINTER(abi.CameraParams), dp, dp]
+    L.kpo_bl_camera_ray.argtypes = [C.POINTER(abi.PassAParams), i32, sp]
+    L.kpo_bl_camera_tetrad.argtypes = [C.POINTER(abi.PassAParams), dp]
+    L.kpo_thermal_coefficients.argtypes = [i32] + [d] * 12 + [C.POINTER(Coeffs)]
+    L.kpo_symphony_rho_v.argtypes = [d] * 6
+    L.kpo_symphony_rho_v.restype = d
+    L.kpo_bessel_k012.argtypes = [d, dp, dp, dp]
+    L.kpo_riaf_coefficients.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), sp,
+                                        C.POINTER(Perturbation), C.POINTER(Coeffs)]
+    L.kpo_riaf_fluid_velocity.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel),
+                                          dp, dp]
+    L.kpo_riaf_isco_radius.argtypes = [d]
+    L.kpo_riaf_isco_radius.restype = d
+    L.kpo_semi_analytic_step.argtypes = [C.POINTER(Stokes), C.POINTER(Coeffs), d]
+    L.kpo_transform_to_observer_basis.argtypes = [C.POINTER(Stokes), d, d, d, d, C.POINTER(Stokes)]
+    L.kpo_radiation_substep_count.argtypes = [C.POINTER(Coeffs), d, d, i32, d, d, d, d]
+    L.kpo_radiation_substep_count.restype = i32
+    L.kpo_bl_pass_a_endpoint.argtypes = [C.POINTER(abi.PassAParams), i32, sp,
+                                         C.POINTER(i32), C.POINTER(i32)]
+    L.kpo_bl_pass_a_endpoint.restype = i32
+    L.kpo_run_riaf_image.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel),
+                                     C.POINTER(Perturbation), i32, i32, i32,
+                                     C.POINTER(abi.ImageData), C.POINTER(Stats)]
+    L.kpo_run_riaf_image.restype = i32
+    _lib = L
+    return L
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def state(values) -> State:
+    s = State()
+    for i, v in enumerate(values):
+        s.v[i] = float(v)
+    return s
+
+
+def state_array(s: State) -> np.ndarray:
+    return np.array(list(s.v), dtype=np.float64)
+
+
+def alloc_image(npix: int) -> dict:
+    out = {n: np.zeros(npix, dtype=np.float64) for n in abi.IMAGE_REAL_FIELDS}
+    out.update({n: np.zeros(npix, dtype=np.int32) for n in abi.IMAGE_INT_FIELDS})
+    return out
+
+
+def image_struct(arrays: dict) -> abi.ImageData:
+    img = abi.ImageData()
+    for n in abi.IMAGE_REAL_FIELDS:
+        if n in arrays:
+            setattr(img, n, arrays[n].ctypes.data_as(C.POINTER(C.c_double)))
+    for n in abi.IMAGE_INT_FIELDS:
+        if n in arrays:
+            setattr(img, n, arrays[n].ctypes.data_as(C.POINTER(C.c_int32)))
+    return img
+
+
+def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, pixel_begin: int = 0,
+                   pixel_end: int = -1, nthreads: int = 0, perturbation=None):
+    """Oracle image: dict of per-pixel arrays (full npix; the selected range filled)
+    plus work counters."""
+    L = lib()
+    npix = params.camera.nx * params.camera.ny
+    arrays = alloc_image(npix)
+    img = image_struct(arrays)
+    st = Stats()
+    pert = perturbation if perturbation is not None else Perturbation()
+    if nthreads <= 0:
+        nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
+    rc = L.kpo_run_riaf_image(C.byref(params), C.byref(model), C.byref(pert), pixel_begin,
+                              pixel_end, nthreads, C.byref(img), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
+    stats = {n: getattr(st, n) for n, _ in Stats._fields_}
+    return arrays, stats

@@ -1,0 +1,1725 @@
+/*
+ * kp_oracle.c -- TEST INFRASTRUCTURE ONLY: the CPU oracle of the imaging path.
+ *
+ * A plain-C11 restatement, in IEEE double with the C library's math, of the
+ * reference algorithm for the Boyer-Lindquist RIAF image: camera rays, Kerr BL
+ * connection, fixed and step-doubling adaptive RK4, pass A segment endpoints,
+ * pass B transport with camera crossing and depth control, the RIAF plasma
+ * model, thermal synchrotron fits, the Strang-split Stokes step and the
+ * observer-basis transform.  Each function cites the reference file:line it
+ * follows.  It is the checker for the HIP kernels (tests, smoke, bench's CPU
+ * baseline); the product never links or calls it.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   transfer step, observer transform   closed forms of tests/test_core.cpp:247-335
+ *                                       (pure emission, absorption+emission,
+ *                                       pi/2 Faraday rotation, Jones-vector oracle)
+ *   thermal Pandya coefficients         Symphony fast-fit checkpoints,
+ *                                       tests/test_core.cpp:2188-2237 (rel 1e-5)
+ *   thermal rho_V, Dexter fit           PARITY UNPINNED by reference vectors; checked
+ *                                       only by Kirchhoff's law a=j/B_nu, sign and
+ *                                       theta-symmetry invariants
+ *   Bessel K0/K1/K2                     scipy.special.kv (NR polynomial accuracy 1e-7)
+ *   BL metric, connection, RHS          g g^-1 = I, finite differences of gcov,
+ *                                       specialised == generic connection
+ *   RK4 / adaptive RK4                  convergence order on Kerr geodesics, null
+ *                                       and frame invariants, tolerance monotonicity
+ *   camera                              -u.k = 1, k.k = 0, e_i.e_j = delta_ij
+ *   whole RIAF image (pass A + B)       end-to-end 8x8 golden fluxes of
+ *                                       tests/test_hdf5_contract.py:671-681 (rel 2e-6)
+ */
+#include "kp_oracle.h"
+
+#include <math.h>
+#include <stddef.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TINY 1e-300
+#define LARGE 1e300
+#define PI_D 3.141592653589793238462643383279502884
+
+static inline double absv(double x) { return x < 0.0 ? -x : x; }
+static inline double minv(double a, double b) { return a < b ? a : b; }
+static inline double maxv(double a, double b) { return a > b ? a : b; }
+static inline double clampv(double x, double lo, double hi) { return minv(maxv(x, lo), hi); }
+static inline double sq(double x) { return x * x; }
+
+/* ======================================================================
+ * Boyer-Lindquist metric (src/geometry/kerr_boyer_lindquist.hpp)
+ * ==================================================================== */
+typedef struct bl_work {
+    double r, th, a, a2, m, sinth, costh, sin2, cos2, sigma, delta;
+    double dsigma_dr, dsigma_dth, ddelta_dr, dsin2_dth;
+} bl_work;
+
+/* KerrBoyerLindquistMetric::build_work (kerr_boyer_lindquist.hpp:28-45). */
+static bl_work bl_build_work(double mass, double spin, const double* X) {
+    bl_work w;
+    w.r = maxv(X[1], TINY);
+    w.th = X[2];
+    w.a = spin;
+    w.a2 = spin * spin;
+    w.m = mass;
+    w.sinth = sin(w.th);
+    w.costh = cos(w.th);
+    w.sin2 = maxv(w.sinth * w.sinth, TINY);
+    w.cos2 = w.costh * w.costh;
+    w.sigma = w.r * w.r + w.a2 * w.cos2;
+    w.delta = w.r * w.r - 2.0 * w.m * w.r + w.a2;
+    w.dsigma_dr = 2.0 * w.r;
+    w.dsigma_dth = -2.0 * w.a2 * w.sinth * w.costh;
+    w.ddelta_dr = 2.0 * (w.r - w.m);
+    w.dsin2_dth = 2.0 * w.sinth * w.costh;
+    return w;
+}
+
+/* gcov(mu,nu) element formulas (kerr_boyer_lindquist.hpp:47-71). */
+void kpo_bl_gcov(double mass, double spin, const double x[4], double g[16]) {
+    const bl_work w = bl_build_work(mass, spin, x);
+    for (int i = 0; i < 16; ++i) g[i] = 0.0;
+    g[0] = -(1.0 - 2.0 * w.m * w.r / w.sigma);
+    g[3] = g[12] = -2.0 * w.m * w.a * w.r * w.sin2 / w.sigma;
+    g[5] = w.sigma / w.delta;
+    g[10] = w.sigma;
+    g[15] = w.sin2 * (w.r * w.r + w.a2 + 2.0 * w.m * w.a2 * w.r * w.sin2 / w.sigma);
+}
+
+/* gcon(mu,nu) element formulas (kerr_boyer_lindquist.hpp:73-99). */
+void kpo_bl_gcon(double mass, double spin, const double x[4], double g[16]) {
+    const bl_work w = bl_build_work(mass, spin, x);
+    for (int i = 0; i < 16; ++i) g[i] = 0.0;
+    const double inv_sigma_delta = 1.0 / (w.sigma * w.delta);
+    const double r2pa2 = w.r * w.r + w.a2;
+    g[0] = -(r2pa2 * r2pa2 - w.a2 * w.delta * w.sin2) * inv_sigma_delta;
+    g[3] = g[12] = -2.0 * w.m * w.a * w.r * inv_sigma_delta;
+    g[5] = w.delta / w.sigma;
+    g[10] = 1.0 / w.sigma;
+    g[15] = (w.delta - w.a2 * w.sin2) / (w.sigma * w.delta * w.sin2);
+}
+
+/* KerrBoyerLindquistMetric::dot (kerr_boyer_lindquist.hpp:117-127). */
+double kpo_bl_dot(double mass, double spin, const double x[4],
+                  const double u[4], const double v[4]) {
+    double g[16];
+    kpo_bl_gcov(mass, spin, x, g);
+    double out = 0.0;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int nu = 0; nu < 4; ++nu)
+            out += g[mu * 4 + nu] * u[mu] * v[nu];
+    return out;
+}
+
+/* KerrBoyerLindquistMetric::dgcov (kerr_boyer_lindquist.hpp:129-177); dg[mu][nu][alpha]. */
+void kpo_bl_dgcov(double mass, double spin, const double x[4], double dg[64]) {
+    const bl_work w = bl_build_work(mass, spin, x);
+    for (int i = 0; i < 64; ++i) dg[i] = 0.0;
+    const double sig = w.sigma, sig2 = sig * sig, del = w.delta, del2 = del * del;
+    const double r = w.r, m = w.m, a = w.a, a2 = w.a2, s2 = w.sin2, ds2 = w.dsin2_dth;
+    const double sig_r = w.dsigma_dr, sig_t = w.dsigma_dth, del_r = w.ddelta_dr;
+#define SETSYM(mu, nu, al, val) do { dg[((mu)*4+(nu))*4+(al)] = (val); dg[((nu)*4+(mu))*4+(al)] = (val); } while (0)
+    SETSYM(0, 0, 1, 2.0 * m * (sig - r * sig_r) / sig2);
+    SETSYM(0, 0, 2, -2.0 * m * r * sig_t / sig2);
+    SETSYM(0, 3, 1, -2.0 * m * a * s2 * (sig - r * sig_r) / sig2);
+    SETSYM(0, 3, 2, -2.0 * m * a * r * (ds2 * sig - s2 * sig_t) / sig2);
+    SETSYM(1, 1, 1, (sig_r * del - sig * del_r) / del2);
+    SETSYM(1, 1, 2, sig_t / del);
+    SETSYM(2, 2, 1, sig_r);
+    SETSYM(2, 2, 2, sig_t);
+    const double extra = 2.0 * m * a2 * r * s2 / sig;
+    const double aterm = r * r + a2 + extra;
+    const double extra_r = 2.0 * m * a2 * s2 * (sig - r * sig_r) / sig2;
+    const double extra_t = 2.0 * m * a2 * r * (ds2 * sig - s2 * sig_t) / sig2;
+    SETSYM(3, 3, 1, s2 * (2.0 * r + extra_r));
+    SETSYM(3, 3, 2, ds2 * aterm + s2 * extra_t);
+#undef SETSYM
+}
+
+/* fill_connection_from_metric_derivs (src/geometry/connection.hpp:7-24). */
+void kpo_bl_connection(double mass, double spin, const double x[4], double gamma[64]) {
+    double gcon[16], dg[64];
+    kpo_bl_gcon(mass, spin, x, gcon);
+    kpo_bl_dgcov(mass, spin, x, dg);
+    for (int mu = 0; mu < 4; ++mu)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                double comp = 0.0;
+                for (int nu = 0; nu < 4; ++nu)
+                    comp += 0.5 * gcon[mu * 4 + nu] *
+                            (dg[(nu * 4 + b) * 4 + a] + dg[(nu * 4 + a) * 4 + b] -
+                             dg[(a * 4 + b) * 4 + nu]);
+                gamma[(mu * 4 + a) * 4 + b] = comp;
+            }
+}
+
+/* Generic transport_rhs (src/geodesic/rk4.hpp:7-34): test oracle for the
+ * specialised contraction. */
+void kpo_bl_transport_rhs_generic(double mass, double spin, const kpo_state* s, kpo_state* d) {
+    double gamma[64];
+    kpo_bl_connection(mass, spin, s->v, gamma);
+    for (int mu = 0; mu < 4; ++mu) d->v[mu] = s->v[4 + mu];
+    for (int mu = 0; mu < 4; ++mu) {
+        double dk = 0.0, de1 = 0.0, de2 = 0.0;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                const double g = gamma[(mu * 4 + a) * 4 + b];
+                dk -= g * s->v[4 + a] * s->v[4 + b];
+                de1 -= g * s->v[4 + a] * s->v[8 + b];
+                de2 -= g * s->v[4 + a] * s->v[12 + b];
+            }
+        d->v[4 + mu] = dk;
+        d->v[8 + mu] = de1;
+        d->v[12 + mu] = de2;
+    }
+}
+
+typedef struct bl_conn {
+    double dr00, dr03, dr11, dr22, dr33;
+    double dt00, dt03, dt11, dt22, dt33;
+    double g00, g03, g11, g22, g33;
+} bl_conn;
+
+/* build_bl_connection_work (kerr_boyer_lindquist.hpp:229-272). */
+static bl_conn bl_connection_work(double mass, double spin, const double* X) {
+    const bl_work w = bl_build_work(mass, spin, X);
+    bl_conn c;
+    const double sig = w.sigma, sig2 = sig * sig, del = w.delta, del2 = del * del;
+    const double r = w.r, m = w.m, a = w.a, a2 = w.a2, s2 = w.sin2, ds2 = w.dsin2_dth;
+    const double sig_r = w.dsigma_dr, sig_t = w.dsigma_dth, del_r = w.ddelta_dr;
+    c.dr00 = 2.0 * m * (sig - r * sig_r) / sig2;
+    c.dt00 = -2.0 * m * r * sig_t / sig2;
+    c.dr03 = -2.0 * m * a * s2 * (sig - r * sig_r) / sig2;
+    c.dt03 = -2.0 * m * a * r * (ds2 * sig - s2 * sig_t) / sig2;
+    c.dr11 = (sig_r * del - sig * del_r) / del2;
+    c.dt11 = sig_t / del;
+    c.dr22 = sig_r;
+    c.dt22 = sig_t;
+    const double extra = 2.0 * m * a2 * r * s2 / sig;
+    const double aterm = r * r + a2 + extra;
+    const double extra_r = 2.0 * m * a2 * s2 * (sig - r * sig_r) / sig2;
+    const double extra_t = 2.0 * m * a2 * r * (ds2 * sig - s2 * sig_t) / sig2;
+    c.dr33 = s2 * (2.0 * r + extra_r);
+    c.dt33 = ds2 * aterm + s2 * extra_t;
+    const double inv_sigma_delta = 1.0 / (sig * del);
+    const double r2pa2 = r * r + a2;
+    c.g00 = -(r2pa2 * r2pa2 - a2 * del * s2) * inv_sigma_delta;
+    c.g03 = -2.0 * m * a * r * inv_sigma_delta;
+    c.g11 = del / sig;
+    c.g22 = 1.0 / sig;
+    c.g33 = (del - a2 * s2) / (sig * del * s2);
+    return c;
+}
+
+/* bl_derivative_row_dot (kerr_boyer_lindquist.hpp:180-194). */
+static inline double row_dot(double d00, double d03, double d11, double d22, double d33,
+                             int row, const double* v) {
+    if (row == 0) return d00 * v[0] + d03 * v[3];
+    if (row == 1) return d11 * v[1];
+    if (row == 2) return d22 * v[2];
+    return d03 * v[0] + d33 * v[3];
+}
+
+/* bl_derivative_bilinear (kerr_boyer_lindquist.hpp:196-206). */
+static inline double bilinear(double d00, double d03, double d11, double d22, double d33,
+                              const double* p, const double* q) {
+    return d00 * p[0] * q[0] + d03 * (p[0] * q[3] + p[3] * q[0]) + d11 * p[1] * q[1] +
+           d22 * p[2] * q[2] + d33 * p[3] * q[3];
+}
+
+/* bl_connection_contract: Gamma^mu_ab p^a q^b (kerr_boyer_lindquist.hpp:274-295). */
+static void bl_contract(const bl_conn* c, const double* p, const double* q, double* out) {
+    double t[4];
+    for (int nu = 0; nu < 4; ++nu) {
+        t[nu] = p[1] * row_dot(c->dr00, c->dr03, c->dr11, c->dr22, c->dr33, nu, q) +
+                p[2] * row_dot(c->dt00, c->dt03, c->dt11, c->dt22, c->dt33, nu, q) +
+                q[1] * row_dot(c->dr00, c->dr03, c->dr11, c->dr22, c->dr33, nu, p) +
+                q[2] * row_dot(c->dt00, c->dt03, c->dt11, c->dt22, c->dt33, nu, p);
+    }
+    t[1] -= bilinear(c->dr00, c->dr03, c->dr11, c->dr22, c->dr33, p, q);
+    t[2] -= bilinear(c->dt00, c->dt03, c->dt11, c->dt22, c->dt33, p, q);
+    out[0] = 0.5 * (c->g00 * t[0] + c->g03 * t[3]);
+    out[1] = 0.5 * c->g11 * t[1];
+    out[2] = 0.5 * c->g22 * t[2];
+    out[3] = 0.5 * (c->g03 * t[0] + c->g33 * t[3]);
+}
+
+/* Specialised BL transport_rhs (kerr_boyer_lindquist.hpp:306-322). */
+void kpo_bl_transport_rhs(double mass, double spin, const kpo_state* s, kpo_state* d) {
+    const bl_conn c = bl_connection_work(mass, spin, s->v);
+    double ck[4], ce1[4], ce2[4];
+    bl_contract(&c, s->v + 4, s->v + 4, ck);
+    bl_contract(&c, s->v + 4, s->v + 8, ce1);
+    bl_contract(&c, s->v + 4, s->v + 12, ce2);
+    for (int mu = 0; mu < 4; ++mu) {
+        d->v[mu] = s->v[4 + mu];
+        d->v[4 + mu] = -ck[mu];
+        d->v[8 + mu] = -ce1[mu];
+        d->v[12 + mu] = -ce2[mu];
+    }
+}
+
+/* ======================================================================
+ * Integrators (src/geodesic/rk4.hpp)
+ * ==================================================================== */
+typedef struct metric_ctx {
+    double mass, spin;
+    int64_t* rhs_counter;
+} metric_ctx;
+
+static inline void rhs(const metric_ctx* m, const kpo_state* s, kpo_state* d) {
+    if (m->rhs_counter) ++*m->rhs_counter;
+    kpo_bl_transport_rhs(m->mass, m->spin, s, d);
+}
+
+/* axpy(y, a, x) = y + x * a (src/geodesic/state.hpp:39-44). */
+static inline void axpy(const kpo_state* y, double a, const kpo_state* x, kpo_state* out) {
+    for (int i = 0; i < 16; ++i) out->v[i] = y->v[i] + x->v[i] * a;
+}
+
+/* rk4_step_from_k1 (rk4.hpp:49-59): s + (k1 + 2 k2 + 2 k3 + k4) * (h / 6). */
+static void rk4_from_k1(const metric_ctx* m, const kpo_state* s, double h,
+                        const kpo_state* k1, kpo_state* out) {
+    kpo_state tmp, k2, k3, k4;
+    axpy(s, 0.5 * h, k1, &tmp);
+    rhs(m, &tmp, &k2);
+    axpy(s, 0.5 * h, &k2, &tmp);
+    rhs(m, &tmp, &k3);
+    axpy(s, h, &k3, &tmp);
+    rhs(m, &tmp, &k4);
+    const double h6 = h / 6.0;
+    for (int i = 0; i < 16; ++i)
+        out->v[i] = s->v[i] + (k1->v[i] + 2.0 * k2.v[i] + 2.0 * k3.v[i] + k4.v[i]) * h6;
+}
+
+/* rk4_step (rk4.hpp:36-46). */
+static void rk4(const metric_ctx* m, const kpo_state* s, double h, kpo_state* out) {
+    kpo_state k1;
+    rhs(m, s, &k1);
+    rk4_from_k1(m, s, h, &k1, out);
+}
+
+void kpo_bl_rk4_step(double mass, double spin, const kpo_state* s, double h, kpo_state* out) {
+    metric_ctx m = {mass, spin, NULL};
+    rk4(&m, s, h, out);
+}
+
+/* state_error_norm (rk4.hpp:74-89). */
+double kpo_state_error_norm(const kpo_state* a, const kpo_state* b) {
+    double err = 0.0;
+    for (int mu = 0; mu < 4; ++mu) {
+        const double sx = maxv(1.0, maxv(absv(a->v[mu]), absv(b->v[mu])));
+        const double sk = maxv(1.0, maxv(absv(a->v[4 + mu]), absv(b->v[4 + mu])));
+        const double se1 = maxv(1.0, maxv(absv(a->v[8 + mu]), absv(b->v[8 + mu])));
+        const double se2 = maxv(1.0, maxv(absv(a->v[12 + mu]), absv(b->v[12 + mu])));
+        err = maxv(err, absv(a->v[mu] - b->v[mu]) / sx);
+        err = maxv(err, absv(a->v[4 + mu] - b->v[4 + mu]) / sk);
+        err = maxv(err, absv(a->v[8 + mu] - b->v[8 + mu]) / se1);
+        err = maxv(err, absv(a->v[12 + mu] - b->v[12 + mu]) / se2);
+    }
+    return err;
+}
+
+typedef struct adaptive_control {
+    double tolerance, min_step, max_step;
+    int max_attempts;
+} adaptive_control;
+
+typedef struct adaptive_result {
+    kpo_state state, mid_state;
+    double used_h, next_h, error;
+    int accepted;
+} adaptive_result;
+
+/* adaptive_rk4_step: step doubling (rk4.hpp:109-161); tolerance floor
+ * 8 * DBL_EPSILON (src/common/math.hpp:56-60). */
+static void adaptive_rk4(const metric_ctx* m, const kpo_state* state, double requested_h,
+                         adaptive_control control, adaptive_result* out) {
+    double h = minv(maxv(absv(requested_h), control.min_step), control.max_step);
+    if (requested_h < 0.0) h = -h;
+    double last_err = LARGE;
+    const double tol_floor = 8.0 * 2.220446049250313080847263336181640625e-16;
+    for (int attempt = 0; attempt < control.max_attempts; ++attempt) {
+        kpo_state k1, full, half, two_half;
+        rhs(m, state, &k1);
+        rk4_from_k1(m, state, h, &k1, &full);
+        rk4_from_k1(m, state, h * 0.5, &k1, &half);
+        rk4(m, &half, h * 0.5, &two_half);
+        const double err = kpo_state_error_norm(&full, &two_half);
+        last_err = err;
+        const double tol = maxv(control.tolerance, tol_floor);
+        if (err <= tol) {
+            double factor = 2.0;
+            if (err > 0.0) {
+                factor = 0.9 * pow(tol / err, 0.2);
+                factor = minv(2.0, maxv(0.2, factor));
+            }
+            out->state = two_half;
+            out->mid_state = half;
+            out->used_h = h;
+            out->next_h = minv(control.max_step, maxv(control.min_step, absv(h) * factor));
+            if (requested_h < 0.0) out->next_h = -out->next_h;
+            out->error = err;
+            out->accepted = 1;
+            return;
+        }
+        if (absv(h) <= control.min_step * 1.0001) break;
+        double factor = 0.9 * pow(tol / maxv(err, TINY), 0.25);
+        factor = minv(0.8, maxv(0.1, factor));
+        h *= factor;
+    }
+    const double next_abs = maxv(control.min_step, absv(h) * 0.5);
+    out->state = *state;
+    out->mid_state = *state;
+    out->used_h = 0.0;
+    out->next_h = requested_h < 0.0 ? -next_abs : next_abs;
+    out->error = last_err;
+    out->accepted = 0;
+}
+
+int32_t kpo_bl_adaptive_rk4_step(double mass, double spin, const kpo_state* s,
+                                 double requested_h, double tolerance,
+                                 double min_step, double max_step, int32_t max_attempts,
+                                 kpo_state* state, kpo_state* mid_state,
+                                 double* used_h, double* next_h, double* error) {
+    metric_ctx m = {mass, spin, NULL};
+    adaptive_control c = {tolerance, min_step, max_step, max_attempts};
+    adaptive_result r;
+    adaptive_rk4(&m, s, requested_h, c, &r);
+    if (state) *state = r.state;
+    if (mid_state) *mid_state = r.mid_state;
+    if (used_h) *used_h = r.used_h;
+    if (next_h) *next_h = r.next_h;
+    if (error) *error = r.error;
+    return r.accepted;
+}
+
+/* ======================================================================
+ * Camera (src/camera/camera.hpp), BL pinhole only
+ * ==================================================================== */
+static void raise_cov(const double* gcon, const double* cov, double* out) {
+    for (int mu = 0; mu < 4; ++mu) {
+        double sum = 0.0;
+        for (int nu = 0; nu < 4; ++nu) sum += gcon[mu * 4 + nu] * cov[nu];
+        out[mu] = sum;
+    }
+}
+
+static double dot_g(const double* g, const double* u, const double* v) {
+    double out = 0.0;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int nu = 0; nu < 4; ++nu) out += g[mu * 4 + nu] * u[mu] * v[nu];
+    return out;
+}
+
+/* project_out_camera (camera.hpp:353-363). */
+static void project_out(const double* g, double* a, const double* b) {
+    const double b2 = dot_g(g, b, b);
+    if (absv(b2) > TINY) {
+        const double f = dot_g(g, a, b) / b2;
+        for (int i = 0; i < 4; ++i) a[i] = a[i] - b[i] * f;
+    }
+}
+
+/* normalize_camera_basis_vector (camera.hpp:365-372). */
+static void normalize_basis(const double* g, double* v) {
+    const double n2 = dot_g(g, v, v);
+    const double n = sqrt(maxv(absv(n2), TINY));
+    const double inv = 1.0 / n;
+    for (int i = 0; i < 4; ++i) v[i] = v[i] * inv;
+}
+
+/* levi_civita4 + tetrad_orientation (camera.hpp:374-407). */
+static int levi_civita4(int i, int j, int k, int l) {
+    if (i == j || i == k || i == l || j == k || j == l || k == l) return 0;
+    int a[4] = {i, j, k, l};
+    int sign = 1;
+    for (int m = 0; m < 4; ++m)
+        for (int n = m + 1; n < 4; ++n)
+            if (a[m] > a[n]) sign = -sign;
+    return sign;
+}
+
+static double tetrad_orientation(const double* e0, const double* e1, const double* e2,
+                                 const double* e3) {
+    double out = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l)
+                    out += (double)levi_civita4(i, j, k, l) * e0[i] * e1[j] * e2[k] * e3[l];
+    return out;
+}
+
+/* make_ipole_camera_tetrad for BL (camera.hpp:415-453) with the BL branches
+ * of ipole_camera_observer_velocity (:396-413), ipole_centering_covector
+ * (:350-360) and native_vector_phi0_to_metric (:300-320).
+ * tetrad[0..3]=e0, [4..7]=e1, [8..11]=e2, [12..15]=e3. */
+void kpo_bl_camera_tetrad(const kp_pass_a_params* p, double tetrad[16]) {
+    const double mass = p->mass, spin = p->spin;
+    const double x[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
+    double gcov[16], gcon[16];
+    kpo_bl_gcov(mass, spin, x, gcov);
+    kpo_bl_gcon(mass, spin, x, gcon);
+    double* e0 = tetrad;
+    double* e1 = tetrad + 4;
+    double* e2 = tetrad + 8;
+    double* e3 = tetrad + 12;
+    {
+        const double r = maxv(x[1], TINY);
+        const double delta = maxv(r * r - 2.0 * mass * r + spin * spin, TINY);
+        const double cov[4] = {-1.0, -2.0 * mass * r / delta, 0.0, 0.0};
+        raise_cov(gcon, cov, e0);
+        const double center[4] = {1.0, 1.0 / r + 2.0 * mass * r / delta, 0.0, 0.0};
+        raise_cov(gcon, center, e3);
+    }
+    {
+        const double r = maxv(p->camera.radius, TINY);
+        const double delta = maxv(r * r - 2.0 * mass * r + spin * spin, TINY);
+        /* e2 trial: native (0,0,1,0) */
+        double vr = r * 0.0;
+        e2[0] = 0.0 - 2.0 * mass * r * vr / delta;
+        e2[1] = vr;
+        e2[2] = PI_D * 1.0;
+        e2[3] = 0.0 - spin * vr / delta;
+        /* e1 trial: native (1,1,1,1) */
+        vr = r * 1.0;
+        e1[0] = 1.0 - 2.0 * mass * r * vr / delta;
+        e1[1] = vr;
+        e1[2] = PI_D * 1.0;
+        e1[3] = 1.0 - spin * vr / delta;
+    }
+    normalize_basis(gcov, e0);
+    project_out(gcov, e3, e0);
+    project_out(gcov, e3, e0);
+    normalize_basis(gcov, e3);
+
+    project_out(gcov, e2, e0);
+    project_out(gcov, e2, e3);
+    project_out(gcov, e2, e0);
+    project_out(gcov, e2, e3);
+    normalize_basis(gcov, e2);
+
+    project_out(gcov, e1, e0);
+    project_out(gcov, e1, e2);
+    project_out(gcov, e1, e3);
+    project_out(gcov, e1, e0);
+    project_out(gcov, e1, e2);
+    project_out(gcov, e1, e3);
+    normalize_basis(gcov, e1);
+
+    if (tetrad_orientation(e0, e1, e2, e3) < 0.0)
+        for (int i = 0; i < 4; ++i) e1[i] = e1[i] * -1.0;
+}
+
+/* pixel_offsets (camera.hpp:96-106). */
+void kpo_pixel_offsets(int32_t pixel, const kp_camera_params* cam, double* sx, double* sy) {
+    const int i = pixel % cam->nx;
+    const int j = pixel / cam->nx;
+    const double x_extent = cam->xspan > 0.0 ? 2.0 * cam->xspan : cam->fov;
+    const double y_extent = cam->yspan > 0.0 ? 2.0 * cam->yspan
+                          : (cam->xspan > 0.0 ? 2.0 * cam->xspan * (double)cam->ny / (double)cam->nx
+                                              : (cam->fov_y > 0.0 ? cam->fov_y : cam->fov));
+    *sx = (((double)i + 0.5) / (double)cam->nx - 0.5) * x_extent + cam->x_offset;
+    *sy = (((double)j + 0.5) / (double)cam->ny - 0.5) * y_extent + cam->y_offset;
+}
+
+/* initialize_pinhole_camera_ray (camera.hpp:484-513) with
+ * normalize_spacelike (:304-313) and orthogonalize_spacelike (:315-325). */
+static void camera_ray_from_tetrad(const kp_pass_a_params* p, const double* tetrad,
+                                   const double* gcov, int32_t pixel, kpo_state* out) {
+    const double* e0 = tetrad;
+    const double* e1 = tetrad + 4;
+    const double* e2 = tetrad + 8;
+    const double* e3 = tetrad + 12;
+    double sx = 0.0, sy = 0.0;
+    kpo_pixel_offsets(pixel, &p->camera, &sx, &sy);
+    const double spatial_norm = sqrt(1.0 + sx * sx + sy * sy);
+    const double kt0 = 1.0;
+    const double kt1 = sx / spatial_norm;
+    const double kt2 = sy / spatial_norm;
+    const double kt3 = 1.0 / spatial_norm;
+    double sd[4];
+    for (int i = 0; i < 4; ++i) sd[i] = e1[i] * kt1 + e2[i] * kt2 + e3[i] * kt3;
+    out->v[0] = 0.0;
+    out->v[1] = p->camera.radius;
+    out->v[2] = p->camera.inclination;
+    out->v[3] = 0.0;
+    for (int i = 0; i < 4; ++i) out->v[4 + i] = e0[i] * kt0 + sd[i];
+    double a[4], b[4];
+    for (int i = 0; i < 4; ++i) a[i] = e1[i] - sd[i] * kt1;
+    {
+        const double n2 = dot_g(gcov, a, a);
+        if (n2 > 0.0) {
+            const double inv = 1.0 / sqrt(n2);
+            for (int i = 0; i < 4; ++i) a[i] = a[i] * inv;
+        }
+    }
+    for (int i = 0; i < 4; ++i) b[i] = e2[i] - sd[i] * kt2;
+    {
+        const double denom = dot_g(gcov, a, a);
+        if (absv(denom) > TINY) {
+            const double f = dot_g(gcov, b, a) / denom;
+            for (int i = 0; i < 4; ++i) b[i] = b[i] - a[i] * f;
+        }
+        const double n2 = dot_g(gcov, b, b);
+        if (n2 > 0.0) {
+            const double inv = 1.0 / sqrt(n2);
+            for (int i = 0; i < 4; ++i) b[i] = b[i] * inv;
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        out->v[8 + i] = a[i];
+        out->v[12 + i] = b[i];
+    }
+}
+
+void kpo_bl_camera_ray(const kp_pass_a_params* p, int32_t pixel, kpo_state* out) {
+    double tetrad[16], gcov[16];
+    const double x[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
+    kpo_bl_camera_tetrad(p, tetrad);
+    kpo_bl_gcov(p->mass, p->spin, x, gcov);
+    camera_ray_from_tetrad(p, tetrad, gcov, pixel, out);
+}
+
+/* ======================================================================
+ * Frame diagnostics (src/geodesic/state.hpp:44-76)
+ * ==================================================================== */
+static double max_frame_error(double mass, double spin, const kpo_state* s) {
+    double g[16];
+    kpo_bl_gcov(mass, spin, s->v, g);
+    const double* k = s->v + 4;
+    const double* e1 = s->v + 8;
+    const double* e2 = s->v + 12;
+    double out = absv(dot_g(g, k, k));
+    out = maxv(out, absv(dot_g(g, e1, k)));
+    out = maxv(out, absv(dot_g(g, e2, k)));
+    out = maxv(out, absv(dot_g(g, e1, e1) - 1.0));
+    out = maxv(out, absv(dot_g(g, e2, e2) - 1.0));
+    out = maxv(out, absv(dot_g(g, e1, e2)));
+    return out;
+}
+
+/* ======================================================================
+ * Thermal synchrotron (src/radiation/thermal_synchrotron.hpp)
+ * ==================================================================== */
+#define EE 4.803204712570263e-10
+#define ME 9.1093837015e-28
+#define CL 2.99792458e10
+#define HPL 6.62607015e-27
+#define MP 1.67262192369e-24
+
+/* Numerical-Recipes polynomial Bessel functions (thermal_synchrotron.hpp:60-165). */
+static double bessel_i0(double ax) {
+    if (ax < 3.75) {
+        double y = ax / 3.75;
+        y *= y;
+        double p = 0.0045813;
+        p = 0.0360768 + y * p;
+        p = 0.2659732 + y * p;
+        p = 1.2067492 + y * p;
+        p = 3.0899424 + y * p;
+        p = 3.5156229 + y * p;
+        return 1.0 + y * p;
+    }
+    const double y = 3.75 / ax;
+    double p = 0.00392377;
+    p = -0.01647633 + y * p;
+    p = 0.02635537 + y * p;
+    p = -0.02057706 + y * p;
+    p = 0.00916281 + y * p;
+    p = -0.00157565 + y * p;
+    p = 0.00225319 + y * p;
+    p = 0.01328592 + y * p;
+    p = 0.39894228 + y * p;
+    return exp(ax) / sqrt(ax) * p;
+}
+
+static double bessel_i1(double ax) {
+    if (ax < 3.75) {
+        double y = ax / 3.75;
+        y *= y;
+        double p = 0.00032411;
+        p = 0.00301532 + y * p;
+        p = 0.02658733 + y * p;
+        p = 0.15084934 + y * p;
+        p = 0.51498869 + y * p;
+        p = 0.87890594 + y * p;
+        p = 0.5 + y * p;
+        return ax * p;
+    }
+    const double y = 3.75 / ax;
+    double p = -0.00420059;
+    p = 0.01787654 + y * p;
+    p = -0.02895312 + y * p;
+    p = 0.02282967 + y * p;
+    p = -0.01031555 + y * p;
+    p = 0.00163801 + y * p;
+    p = -0.00362018 + y * p;
+    p = -0.03988024 + y * p;
+    p = 0.39894228 + y * p;
+    return exp(ax) / sqrt(ax) * p;
+}
+
+static double bessel_k0(double x) {
+    if (x <= 0.0) return LARGE;
+    if (x <= 2.0) {
+        const double y = x * x * 0.25;
+        double p = 0.00000740;
+        p = 0.00010750 + y * p;
+        p = 0.00262698 + y * p;
+        p = 0.03488590 + y * p;
+        p = 0.23069756 + y * p;
+        p = 0.42278420 + y * p;
+        p = -0.57721566 + y * p;
+        return -log(x * 0.5) * bessel_i0(x) + p;
+    }
+    const double y = 2.0 / x;
+    double p = 0.00053208;
+    p = -0.00251540 + y * p;
+    p = 0.00587872 + y * p;
+    p = -0.01062446 + y * p;
+    p = 0.02189568 + y * p;
+    p = -0.07832358 + y * p;
+    p = 1.25331414 + y * p;
+    return exp(-x) / sqrt(x) * p;
+}
+
+static double bessel_k1(double x) {
+    if (x <= 0.0) return LARGE;
+    if (x <= 2.0) {
+        const double y = x * x * 0.25;
+        double p = -0.00004686;
+        p = -0.00110404 + y * p;
+        p = -0.01919402 + y * p;
+        p = -0.18156897 + y * p;
+        p = -0.67278579 + y * p;
+        p = 0.15443144 + y * p;
+        p = 1.0 + y * p;
+        return log(x * 0.5) * bessel_i1(x) + p / x;
+    }
+    const double y = 2.0 / x;
+    double p = -0.00068245;
+    p = 0.00325614 + y * p;
+    p = -0.00780353 + y * p;
+    p = 0.01504268 + y * p;
+    p = -0.03655620 + y * p;
+    p = 0.23498619 + y * p;
+    p = 1.25331414 + y * p;
+    return exp(-x) / sqrt(x) * p;
+}
+
+/* thermal_bessel_k0k1k2 (thermal_synchrotron.hpp:211-231): K2 from the recurrence. */
+void kpo_bessel_k012(double x, double* k0, double* k1, double* k2) {
+    const double k0c = bessel_k0(x);
+    const double k1c = bessel_k1(x);
+    const double k2c = maxv(k0c + 2.0 * k1c / maxv(x, TINY), TINY);
+    if (k0) *k0 = k0c;
+    if (k1) *k1 = k1c;
+    if (k2) *k2 = k2c;
+}
+
+/* symphony_bnu_inv (thermal_synchrotron.hpp:233-251): B_nu / nu^3. */
+static double bnu_inv(double nu, double thetae) {
+    if (!(thetae > 0.0)) return 0.0;
+    const double x = HPL * nu / (ME * CL * CL * thetae);
+    if (x < 2e-3) {
+        const double denom = x / 24.0 * (24.0 + x * (12.0 + x * (4.0 + x)));
+        return (2.0 * HPL / maxv(denom, TINY)) / (CL * CL);
+    }
+    const double denom = exp(x) - 1.0;
+    return (2.0 * HPL / maxv(denom, TINY)) / (CL * CL);
+}
+
+/* symphony_pandya_* (thermal_synchrotron.hpp:311-395). */
+static double pandya_nu_c(double b) { return EE * b / (2.0 * PI_D * ME * CL); }
+
+static double pandya_x(double nu, double thetae, double b, double sin_theta) {
+    const double nu_c = pandya_nu_c(b);
+    const double nu_s = 2.0 / 9.0 * nu_c * sin_theta * thetae * thetae;
+    return maxv(nu / maxv(nu_s, TINY), TINY);
+}
+
+static double pandya_jI(double nu, double ne, double thetae, double b, double sin_theta) {
+    if (!(thetae > 0.0) || !(ne > 0.0) || !(b > 0.0) || !(sin_theta > 0.0)) return 0.0;
+    const double nu_c = pandya_nu_c(b);
+    const double x = pandya_x(nu, thetae, b, sin_theta);
+    const double prefactor = ne * EE * EE * nu_c / CL;
+    const double term1 = sqrt(2.0) * PI_D / 27.0 * sin_theta;
+    const double x13 = cbrt(x);
+    const double shape = sqrt(x) + 1.8877486253633869933 * sqrt(x13);
+    const double term2 = shape * shape;
+    return prefactor * term1 * term2 * exp(-x13);
+}
+
+static double pandya_jQ(double nu, double ne, double thetae, double b, double sin_theta) {
+    if (!(thetae > 0.0) || !(ne > 0.0) || !(b > 0.0) || !(sin_theta > 0.0)) return 0.0;
+    const double nu_c = pandya_nu_c(b);
+    const double x = pandya_x(nu, thetae, b, sin_theta);
+    const double t2425 = pow(thetae, 24.0 / 25.0);
+    const double prefactor = ne * EE * EE * nu_c / CL;
+    const double term1 = sqrt(2.0) * PI_D / 27.0 * sin_theta;
+    const double term2 = (7.0 * t2425 + 35.0) / (10.0 * t2425 + 75.0);
+    const double x13 = cbrt(x);
+    const double shape = sqrt(x) + term2 * 1.8877486253633869933 * sqrt(x13);
+    const double term3 = shape * shape;
+    return prefactor * term1 * term3 * exp(-x13);
+}
+
+static double pandya_jV(double nu, double ne, double thetae, double b, double sin_theta,
+                        double cos_theta) {
+    if (!(thetae > 0.0) || !(ne > 0.0) || !(b > 0.0) || !(sin_theta > 0.0)) return 0.0;
+    const double nu_c = pandya_nu_c(b);
+    const double x = pandya_x(nu, thetae, b, sin_theta);
+    const double prefactor = ne * EE * EE * nu_c / CL;
+    const double sin_shift = sin_theta * 0.4356824462767121 - cos_theta * 0.9001004421765051;
+    const double term1 = (37.0 - 87.0 * sin_shift) / (100.0 * (thetae + 1.0));
+    const double term2 = pow(1.0 + (pow(thetae, 3.0 / 5.0) / 25.0 + 7.0 / 10.0) *
+                                       pow(x, 9.0 / 25.0),
+                             5.0 / 3.0);
+    return prefactor * term1 * term2 * exp(-cbrt(x));
+}
+
+/* Faraday x-variable shared by rho_Q/rho_V (thermal_synchrotron.hpp:409-411). */
+static double faraday_x(double thetae, double sin_theta, double omega0, double two_pi_nu) {
+    return maxv(thetae * sqrt(maxv(0.0, sqrt(2.0) * sin_theta *
+                                            (1e3 * omega0 / maxv(two_pi_nu, TINY)))),
+                1e-12);
+}
+
+/* symphony_rho_V (thermal_synchrotron.hpp:431-459). */
+double kpo_symphony_rho_v(double nu, double ne, double thetae, double b, double theta,
+                          double min_sin_theta) {
+    const double sin_theta = maxv(sin(theta), min_sin_theta);
+    const double omega0 = EE * b / (ME * CL);
+    const double wp2 = 4.0 * PI_D * ne * EE * EE / ME;
+    const double two_pi_nu = 2.0 * PI_D * nu;
+    const double x = faraday_x(thetae, sin_theta, omega0, two_pi_nu);
+    double k0 = 0.0, k2 = 0.0;
+    kpo_bessel_k012(1.0 / thetae, &k0, NULL, &k2);
+    const double shgm = 1.0 - 0.11 * log(1.0 + 0.035 * x);
+    const double k_ratio = k2 > 0.0 ? k0 / k2 : 1.0;
+    const double fit_factor = k_ratio * shgm;
+    const double safe = maxv(two_pi_nu, TINY);
+    double eps12 = wp2 * omega0;
+    eps12 /= safe;
+    eps12 /= safe;
+    eps12 /= safe;
+    eps12 *= fit_factor * cos(theta);
+    return 2.0 * PI_D * nu / CL * eps12;
+}
+
+/* symphony_rho_QV (thermal_synchrotron.hpp:461-506). */
+static void rho_qv(double nu, double ne, double thetae, double b, double sin_theta,
+                   double cos_theta, double* rhoQ, double* rhoV) {
+    const double omega0 = EE * b / (ME * CL);
+    const double wp2 = 4.0 * PI_D * ne * EE * EE / ME;
+    const double two_pi_nu = 2.0 * PI_D * nu;
+    const double x = faraday_x(thetae, sin_theta, omega0, two_pi_nu);
+    double k0 = 0.0, k1 = 0.0, k2 = 0.0;
+    kpo_bessel_k012(1.0 / thetae, &k0, &k1, &k2);
+    const double exp_x_47 = exp(-x / 47.2);
+    const double extraterm = (0.011 * exp_x_47 - 369.69342869090815 * pow(x + 1e-16, -8.0 / 3.0)) *
+                             (0.5 + 0.5 * tanh((log(x) - 4.787491742782046) / 0.1));
+    const double jffunc = 2.011 * exp(-pow(x, 1.035) / 4.7) -
+                          cos(x / 2.0) * exp(-pow(x, 1.2) / 2.73) - 0.011 * exp_x_47 + extraterm;
+    const double k1_ratio = k2 > 0.0 ? k1 / k2 : 1.0;
+    const double safe = maxv(two_pi_nu, TINY);
+    double eps11m22 = jffunc * wp2 * omega0 * omega0;
+    eps11m22 /= safe;
+    eps11m22 /= safe;
+    eps11m22 /= safe;
+    eps11m22 /= safe;
+    eps11m22 *= (k1_ratio + 6.0 * thetae) * sin_theta * sin_theta;
+    *rhoQ = PI_D * nu / CL * eps11m22;
+
+    const double shgm = 1.0 - 0.11 * log(1.0 + 0.035 * x);
+    const double k0_ratio = k2 > 0.0 ? k0 / k2 : 1.0;
+    const double fit_factor = k0_ratio * shgm;
+    double eps12 = wp2 * omega0;
+    eps12 /= safe;
+    eps12 /= safe;
+    eps12 /= safe;
+    eps12 *= fit_factor * cos_theta;
+    *rhoV = 2.0 * PI_D * nu / CL * eps12;
+}
+
+/* thermal_synchrotron_magnetic_basis_coefficients_fit (thermal_synchrotron.hpp:512-616):
+ * invariant coefficients in the magnetic screen basis; fit 1 = Pandya, 4 = Dexter. */
+void kpo_thermal_coefficients(int32_t fit, double nu_in, double ne, double thetae, double b,
+                              double theta, double sin_theta_in, double cos_theta,
+                              double max_pol_frac, double min_sin_theta, double emission_scale,
+                              double absorption_scale, double faraday_scale, kpo_coeffs* c) {
+    memset(c, 0, sizeof(*c));
+    (void)theta;
+    if (!(nu_in > 0.0) || !(ne > 0.0) || !(thetae > 0.0) || !(b > 0.0)) return;
+    const double nu = maxv(nu_in, 1.0);
+    const double sin_theta = maxv(sin_theta_in, min_sin_theta);
+    double jI = 0.0, jQ = 0.0, jV = 0.0;
+    if (fit == KP_EMISSION_THERMAL_DEXTER) {
+        const double nus = 3.0 * EE * b * sin_theta / (4.0 * PI_D * ME * CL) * thetae * thetae + 1.0;
+        const double x = maxv(nu / maxv(nus, TINY), TINY);
+        const double xm13 = 1.0 / cbrt(x);
+        const double exp_shape = exp(-1.8899 / xm13);
+        const double pref = ne * EE * EE * nu / (2.0 * sqrt(3.0) * CL * thetae * thetae);
+        const double i_val = 2.5651 * (1.0 + 1.92 * xm13 + 0.9977 * xm13 * xm13) * exp_shape;
+        const double iq = 2.5651 * (1.0 + 0.93193 * xm13 + 0.499873 * xm13 * xm13) * exp_shape;
+        jI = pref * i_val;
+        jQ = pref * iq;
+        const double nus_v = 3.0 * EE * b * sin_theta / (4.0 * PI_D * ME * CL) * thetae * thetae + 1.0;
+        const double xv = maxv(nu / maxv(nus_v, TINY), TINY);
+        const double xv_m13 = 1.0 / cbrt(xv);
+        const double iv = (1.81384 / xv + 3.42319 * xv_m13 * xv_m13 + 0.0292545 / sqrt(xv) +
+                           2.03773 * xv_m13) * exp(-1.8899 / xv_m13);
+        double cos_for_tan = cos_theta;
+        if (absv(cos_for_tan) < 1e-10) cos_for_tan = cos_for_tan >= 0.0 ? 1e-10 : -1e-10;
+        const double tan_theta = sin_theta / cos_for_tan;
+        jV = 2.0 * ne * EE * EE * nu /
+             (tan_theta * 3.0 * sqrt(3.0) * CL * thetae * thetae * thetae) * iv;
+    } else {
+        jI = pandya_jI(nu, ne, thetae, b, sin_theta);
+        jQ = pandya_jQ(nu, ne, thetae, b, sin_theta);
+        jV = pandya_jV(nu, ne, thetae, b, sin_theta, cos_theta);
+    }
+    const double nusq = nu * nu;
+    double jI_inv = jI / maxv(nusq, TINY);
+    double jQ_inv = jQ / maxv(nusq, TINY);
+    double jV_inv = jV / maxv(nusq, TINY);
+    const double jP_scale = maxv(absv(jQ_inv), absv(jV_inv));
+    const double jP = jP_scale > 0.0
+                          ? jP_scale * sqrt(sq(jQ_inv / jP_scale) + sq(jV_inv / jP_scale))
+                          : 0.0;
+    if (jI_inv > 0.0 && jP > 0.0 && jI_inv < jP / maxv(max_pol_frac, 1e-30)) {
+        const double scale = jI_inv / jP * max_pol_frac;
+        jQ_inv *= scale;
+        jV_inv *= scale;
+    }
+    double aI = 0.0, aQ = 0.0, aV = 0.0;
+    const double bi = bnu_inv(nu, thetae);
+    if (bi > 0.0) {
+        aI = jI_inv / bi;
+        aQ = jQ_inv / bi;
+        aV = jV_inv / bi;
+    }
+    double rhoQ = 0.0, rhoV = 0.0;
+    rho_qv(nu, ne, thetae, b, sin_theta, cos_theta, &rhoQ, &rhoV);
+    c->jI = emission_scale * jI_inv;
+    c->jQ = emission_scale * jQ_inv;
+    c->jV = emission_scale * jV_inv;
+    c->aI = absorption_scale * aI;
+    c->aQ = absorption_scale * aQ;
+    c->aV = absorption_scale * aV;
+    c->rQ = faraday_scale * rhoQ * nu;
+    c->rV = faraday_scale * rhoV * nu;
+}
+
+/* ======================================================================
+ * RIAF model (src/model/riaf.hpp), BL metric branches
+ * ==================================================================== */
+/* density_profile_bl (riaf.hpp:54-60). */
+static double riaf_density(const kp_riaf_model* m, double r, double th) {
+    const double zc = r * cos(th);
+    const double rc = maxv(r * sin(th), m->min_radius);
+    const double h = maxv(m->disk_h, 1e-6);
+    const double vertical = exp(-(zc * zc) / (2.0 * rc * rc * h * h));
+    return m->nth0 * vertical * pow(maxv(r, m->min_radius), m->pow_nth);
+}
+
+/* thetae_profile (riaf.hpp:67-74). */
+static double riaf_thetae(const kp_riaf_model* m, double r) {
+    const double kbol = 1.380649e-16;
+    const double thetae_unit = m->te_unit * kbol / (ME * CL * CL);
+    return maxv(m->Te0 * pow(maxv(r, m->min_radius), m->pow_T) * thetae_unit, m->min_thetae);
+}
+
+/* length_unit_cgs / dlambda_scale (riaf.hpp:76-85). */
+static double riaf_dlambda_scale(const kp_riaf_model* m) {
+    const double l = 6.67430e-8 * m->mbh_solar * 1.98847e33 / (CL * CL);
+    return l / maxv(m->freq_cgs, 1.0);
+}
+
+/* magnetic_field_cgs (riaf.hpp:87-96). */
+static double riaf_bfield(const kp_riaf_model* m, double ne_cgs, double r) {
+    const double b = sqrt(maxv(8.0 * PI_D * 0.1 * ne_cgs * MP * CL * CL /
+                                   (6.0 * maxv(r, m->min_radius)),
+                               0.0));
+    return maxv(b, 1e-12);
+}
+
+/* bl_metric (riaf.hpp:137-166): the model's own unit-mass BL metric. */
+static void riaf_bl_metric(double r, double th, double spin, double* gcov, double* gcon) {
+    const double a = spin, a2 = a * a;
+    const double sinth = sin(th);
+    const double sin2 = maxv(sinth * sinth, 1e-30);
+    const double costh = cos(th);
+    const double sigma = r * r + a2 * costh * costh;
+    const double delta = r * r - 2.0 * r + a2;
+    for (int i = 0; i < 16; ++i) gcov[i] = gcon[i] = 0.0;
+    gcov[0] = -(1.0 - 2.0 * r / sigma);
+    gcov[3] = gcov[12] = -2.0 * a * r * sin2 / sigma;
+    gcov[5] = sigma / maxv(delta, TINY);
+    gcov[10] = sigma;
+    gcov[15] = (r * r + a2 + 2.0 * a2 * r * sin2 / sigma) * sin2;
+    const double isd = 1.0 / maxv(sigma * delta, TINY);
+    gcon[0] = -((r * r + a2) * (r * r + a2) - a2 * delta * sin2) * isd;
+    gcon[3] = gcon[12] = -2.0 * a * r * isd;
+    gcon[5] = delta / sigma;
+    gcon[10] = 1.0 / sigma;
+    gcon[15] = (delta - a2 * sin2) / maxv(sigma * delta * sin2, TINY);
+}
+
+/* isco_radius (riaf.hpp:168-176). */
+double kpo_riaf_isco_radius(double spin) {
+    const double a = spin;
+    const double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+    const double z2 = sqrt(3.0 * a * a + z1 * z1);
+    const double sign = a < 0.0 ? -1.0 : 1.0;
+    return 3.0 + z2 - sign * sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
+
+static double signed_nonzero(double x) {
+    if (absv(x) > TINY) return x;
+    return x < 0.0 ? -TINY : TINY;
+}
+
+/* fluid_four_velocity_from_bl_coords, BL branch (riaf.hpp:210-273). */
+static void riaf_fluid_u(const kp_riaf_model* m, double mass, double spin, double r, double th,
+                         const double* metric_gcov, double* u) {
+    const double rh = mass + sqrt(maxv(mass * mass - spin * spin, 0.0));
+    const double risco = kpo_riaf_isco_radius(spin);
+    double gcov[16], gcon[16];
+    riaf_bl_metric(r, th, spin, gcov, gcon);
+    double vt = 1.0, vr = 0.0, vth = 0.0, vphi = 0.0;
+    if (r < rh) {
+        vt = 1.0 / sqrt(maxv(-gcon[0], TINY));
+    } else if (r < risco) {
+        double gcov_i[16], gcon_i[16];
+        riaf_bl_metric(risco, th, spin, gcov_i, gcon_i);
+        const double omega_isco = 1.0 / (risco * sqrt(risco) + spin);
+        const double qi = gcov_i[0] + 2.0 * omega_isco * gcov_i[3] +
+                          omega_isco * omega_isco * gcov_i[15];
+        const double ut_i = 1.0 / sqrt(maxv(-qi, TINY));
+        const double e = (gcov_i[0] + gcov_i[3] * omega_isco) * ut_i;
+        const double l = (gcov_i[12] + gcov_i[15] * omega_isco) * ut_i;
+        const double k_con = gcon[0] * e * e + 2.0 * gcon[3] * e * l + gcon[15] * l * l;
+        const double ur_cov = -sqrt(maxv(-(1.0 + k_con) / maxv(gcon[5], TINY), 0.0));
+        const double vr_tmp = gcon[5] * ur_cov;
+        const double vt_tmp = gcon[0] * e + gcon[3] * l;
+        const double vphi_tmp = gcon[12] * e + gcon[15] * l;
+        const double omega_k = vphi_tmp / maxv(vt_tmp, TINY);
+        const double omega_ff = gcon[3] / signed_nonzero(gcon[0]);
+        const double omega = omega_k + (1.0 - m->keplerian_factor) * (omega_ff - omega_k);
+        const double ur_ff = -sqrt(maxv(-(1.0 + gcon[0]) * gcon[5], 0.0));
+        vr = vr_tmp + m->infall_factor * (ur_ff - vr_tmp);
+        const double q = gcov[0] + 2.0 * omega * gcov[3] + omega * omega * gcov[15];
+        vt = sqrt(maxv(-(1.0 + vr * vr * gcov[5]) / q, 0.0));
+        vphi = omega * vt;
+    } else {
+        const double r_eff = maxv(r, m->min_radius);
+        const double omega_k = 1.0 / (r_eff * sqrt(r_eff) + spin);
+        const double omega_ff = gcon[3] / signed_nonzero(gcon[0]);
+        const double omega = omega_k + (1.0 - m->keplerian_factor) * (omega_ff - omega_k);
+        vr = m->infall_factor * -sqrt(maxv(-(1.0 + gcon[0]) * gcon[5], 0.0));
+        const double q = gcov[0] + 2.0 * omega * gcov[3] + omega * omega * gcov[15];
+        vt = sqrt(maxv(-(1.0 + vr * vr * gcov[5]) / q, 0.0));
+        vphi = omega * vt;
+    }
+    u[0] = vt;
+    u[1] = vr;
+    u[2] = vth;
+    u[3] = vphi;
+    const double norm2 = dot_g(metric_gcov, u, u);
+    if (norm2 < 0.0) {
+        const double f = 1.0 / sqrt(maxv(-norm2, TINY));
+        for (int i = 0; i < 4; ++i) u[i] = u[i] * f;
+    }
+}
+
+void kpo_riaf_fluid_velocity(const kp_pass_a_params* p, const kp_riaf_model* m,
+                             const double x[4], double u[4]) {
+    double g[16];
+    kpo_bl_gcov(p->mass, p->spin, x, g);
+    riaf_fluid_u(m, p->mass, p->spin, maxv(x[1], m->min_radius), x[2], g, u);
+}
+
+static void lower_g(const double* g, const double* v, double* out) {
+    for (int mu = 0; mu < 4; ++mu) {
+        double s = 0.0;
+        for (int nu = 0; nu < 4; ++nu) s += g[mu * 4 + nu] * v[nu];
+        out[mu] = s;
+    }
+}
+
+/* magnetic_unit_four_vector_with_gcov, BL branch (riaf.hpp:350-386). */
+static void riaf_b_unit(const double* gcov, const double* ucon, double* bcon) {
+    double btrial[4] = {0.0, 0.0, 0.0, 1.0};
+    double udotb = dot_g(gcov, ucon, btrial);
+    for (int i = 0; i < 4; ++i) bcon[i] = btrial[i] + ucon[i] * udotb;
+    double bsq = dot_g(gcov, bcon, bcon);
+    if (!(bsq > 1e-30)) {
+        const double bt[4] = {0.0, 0.0, 0.0, 1.0};
+        const double udotz = dot_g(gcov, ucon, bt);
+        for (int i = 0; i < 4; ++i) bcon[i] = bt[i] + ucon[i] * udotz;
+        bsq = dot_g(gcov, bcon, bcon);
+    }
+    if (!(bsq > 1e-30)) {
+        const double bt[4] = {0.0, 1.0, 0.0, 0.0};
+        const double udotx = dot_g(gcov, ucon, bt);
+        for (int i = 0; i < 4; ++i) bcon[i] = bt[i] + ucon[i] * udotx;
+        bsq = dot_g(gcov, bcon, bcon);
+    }
+    if (bsq > 1e-30) {
+        const double f = 1.0 / sqrt(bsq);
+        for (int i = 0; i < 4; ++i) bcon[i] = bcon[i] * f;
+    }
+}
+
+/* screen_inner_product_with_gcov (riaf.hpp:320-336). */
+static double screen_inner(const double* g, const double* u, const double* k, const double* a,
+                           const double* b) {
+    const double uk = dot_g(g, u, k);
+    if (absv(uk) <= TINY) return dot_g(g, a, b);
+    double eK[4];
+    for (int i = 0; i < 4; ++i) eK[i] = k[i] * (-1.0 / uk) - u[i];
+    return dot_g(g, a, b) + dot_g(g, u, a) * dot_g(g, u, b) - dot_g(g, eK, a) * dot_g(g, eK, b);
+}
+
+/* PlasmaPerturbation::apply (plasma_perturbation.hpp:17-32). */
+static void apply_perturbation(const kpo_perturbation* p, double* ne, double* thetae, double* b) {
+    if (!p || p->log_scale == 0.0) return;
+    const double f = exp(p->log_scale);
+    if (p->parameter == 1) {
+        *ne *= f;
+        *b *= exp(p->log_scale * 0.5);
+    } else if (p->parameter == 2) {
+        *thetae *= f;
+    } else if (p->parameter == 3) {
+        *b *= f;
+    }
+}
+
+/* RIAFAnalyticRadiationModel::coefficients, BL metric, compiled Pandya fit by
+ * default and Dexter when emission_type == 4 (riaf.hpp:410-549). */
+void kpo_riaf_coefficients(const kp_pass_a_params* p, const kp_riaf_model* m,
+                           const kpo_state* s, const kpo_perturbation* pert, kpo_coeffs* c) {
+    memset(c, 0, sizeof(*c));
+    const double* x = s->v;
+    const double* k = s->v + 4;
+    const double r_coord = x[1], th = x[2];
+    const double r_bl = maxv(r_coord, m->min_radius);
+    const double rh = p->mass + sqrt(maxv(p->mass * p->mass - p->spin * p->spin, 0.0));
+    if (r_bl <= rh + m->horizon_buffer || r_bl < m->r_min || r_bl > m->r_max) return;
+    const double ne_norm = riaf_density(m, r_bl, th);
+    if (!(ne_norm > 0.0)) return;
+    double ne = ne_norm * m->ne_unit;
+    double thetae = riaf_thetae(m, r_bl);
+    double b = riaf_bfield(m, ne, r_bl);
+    apply_perturbation(pert, &ne, &thetae, &b);
+
+    double g[16];
+    kpo_bl_gcov(p->mass, p->spin, x, g);
+    double ucon[4], ucov[4];
+    riaf_fluid_u(m, p->mass, p->spin, r_bl, th, g, ucon);
+    lower_g(g, ucon, ucov);
+    double uk = 0.0;
+    for (int mu = 0; mu < 4; ++mu) uk += ucov[mu] * k[mu];
+    const double nu_scale = maxv(absv(-uk), m->min_frequency_scale);
+
+    double bcon[4], bcov[4];
+    riaf_b_unit(g, ucon, bcon);
+    lower_g(g, bcon, bcov);
+    double kdotb = 0.0;
+    for (int mu = 0; mu < 4; ++mu) kdotb += k[mu] * bcov[mu];
+    const double cos_theta = clampv(kdotb / maxv(nu_scale, 1e-30), -1.0, 1.0);
+    const double sin_theta = maxv(sqrt(maxv(0.0, 1.0 - cos_theta * cos_theta)), m->min_sin_theta);
+    const double theta = acos(clampv(cos_theta, -1.0, 1.0));
+    const double nu = maxv(m->freq_cgs * nu_scale, 1.0);
+
+    if (theta <= 0.0 || theta >= PI_D) {
+        c->rV = m->faraday_scale * kpo_symphony_rho_v(nu, ne, thetae, b, theta, m->min_sin_theta) * nu;
+        return;
+    }
+    const int fit = m->emission_type == KP_EMISSION_THERMAL_DEXTER ? KP_EMISSION_THERMAL_DEXTER
+                                                                    : KP_EMISSION_THERMAL_PANDYA;
+    kpo_coeffs mc;
+    kpo_thermal_coefficients(fit, nu, ne, thetae, b, theta, sin_theta, cos_theta, m->max_pol_frac,
+                             m->min_sin_theta, m->emission_scale, m->absorption_scale,
+                             m->faraday_scale, &mc);
+    const double b1 = screen_inner(g, ucon, k, s->v + 8, bcon);
+    const double b2 = screen_inner(g, ucon, k, s->v + 12, bcon);
+    const double bproj2 = b1 * b1 + b2 * b2;
+    c->jI = mc.jI;
+    c->jV = mc.jV;
+    c->aI = mc.aI;
+    c->aV = mc.aV;
+    c->rV = mc.rV;
+    if (bproj2 > 1e-30) {
+        const double cos2chi = (b2 * b2 - b1 * b1) / bproj2;
+        const double sin2chi = -2.0 * b1 * b2 / bproj2;
+        c->jQ = mc.jQ * cos2chi;
+        c->jU = mc.jQ * sin2chi;
+        c->aQ = mc.aQ * cos2chi;
+        c->aU = mc.aQ * sin2chi;
+        c->rQ = mc.rQ * cos2chi;
+        c->rU = mc.rQ * sin2chi;
+    }
+}
+
+/* ======================================================================
+ * Transfer (src/radiation/semi_analytic.hpp, src/common/math.hpp:62-86)
+ * ==================================================================== */
+static double stable_attenuation(double kappa, double dl) {
+    const double x = kappa * dl;
+    if (x > 700.0) return 0.0;
+    return exp(-x);
+}
+
+static double stable_source_factor(double kappa, double dl) {
+    const double x = kappa * dl;
+    if (absv(x) < 1e-6) return dl * (1.0 - x / 2.0 + x * x / 6.0 - x * x * x / 24.0);
+    return (1.0 - exp(-x)) / kappa;
+}
+
+/* Scaled Euclidean norm (src/common/vec.hpp:63-71). */
+static double norm3(double x, double y, double z) {
+    const double scale = maxv(absv(x), maxv(absv(y), absv(z)));
+    if (scale == 0.0) return 0.0;
+    const double a = x / scale, b = y / scale, c = z / scale;
+    return scale * sqrt(a * a + b * b + c * c);
+}
+
+/* faraday_rotate (semi_analytic.hpp:9-37). */
+static void faraday_rotate(kpo_stokes* s, const kpo_coeffs* c, double dl) {
+    const double px = s->Q, py = s->U, pz = s->V;
+    const double rx = c->rQ, ry = c->rU, rz = c->rV;
+    const double rn = norm3(rx, ry, rz);
+    if (rn == 0.0 || dl == 0.0) return;
+    const double theta = rn * dl;
+    double ox, oy, oz;
+    if (absv(theta) < 1e-8) {
+        ox = px + (ry * pz - rz * py) * dl;
+        oy = py + (rz * px - rx * pz) * dl;
+        oz = pz + (rx * py - ry * px) * dl;
+    } else {
+        const double inv = 1.0 / rn;
+        const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+        const double ct = cos(theta), st = sin(theta);
+        const double nd = (nx * px + ny * py + nz * pz) * (1.0 - ct);
+        ox = px * ct + nx * nd + (ny * pz - nz * py) * st;
+        oy = py * ct + ny * nd + (nz * px - nx * pz) * st;
+        oz = pz * ct + nz * nd + (nx * py - ny * px) * st;
+    }
+    s->Q = ox;
+    s->U = oy;
+    s->V = oz;
+}
+
+/* absorb_emit (semi_analytic.hpp:39-91). */
+static void absorb_emit(kpo_stokes* s, const kpo_coeffs* c, double dl) {
+    const double px = s->Q, py = s->U, pz = s->V;
+    const double an = norm3(c->aQ, c->aU, c->aV);
+    if (an < TINY) {
+        const double e = stable_attenuation(c->aI, dl);
+        const double f = stable_source_factor(c->aI, dl);
+        s->I = e * s->I + f * c->jI;
+        s->Q = e * s->Q + f * c->jQ;
+        s->U = e * s->U + f * c->jU;
+        s->V = e * s->V + f * c->jV;
+        return;
+    }
+    const double inv = 1.0 / an;
+    const double nx = c->aQ * inv, ny = c->aU * inv, nz = c->aV * inv;
+    const double ppar = nx * px + ny * py + nz * pz;
+    const double pperp_x = px - nx * ppar, pperp_y = py - ny * ppar, pperp_z = pz - nz * ppar;
+    const double jpar = nx * c->jQ + ny * c->jU + nz * c->jV;
+    const double jperp_x = c->jQ - nx * jpar, jperp_y = c->jU - ny * jpar, jperp_z = c->jV - nz * jpar;
+    const double s_plus = s->I + ppar, s_minus = s->I - ppar;
+    const double j_plus = c->jI + jpar, j_minus = c->jI - jpar;
+    const double k_plus = c->aI + an, k_minus = c->aI - an;
+    const double sp = stable_attenuation(k_plus, dl) * s_plus + stable_source_factor(k_plus, dl) * j_plus;
+    const double sm = stable_attenuation(k_minus, dl) * s_minus + stable_source_factor(k_minus, dl) * j_minus;
+    const double ei = stable_attenuation(c->aI, dl);
+    const double fi = stable_source_factor(c->aI, dl);
+    const double i_out = 0.5 * (sp + sm);
+    const double ppar_out = 0.5 * (sp - sm);
+    s->I = i_out;
+    s->Q = nx * ppar_out + (pperp_x * ei + jperp_x * fi);
+    s->U = ny * ppar_out + (pperp_y * ei + jperp_y * fi);
+    s->V = nz * ppar_out + (pperp_z * ei + jperp_z * fi);
+}
+
+/* semi_analytic_stokes_step (semi_analytic.hpp:93-100). */
+void kpo_semi_analytic_step(kpo_stokes* s, const kpo_coeffs* c, double dl) {
+    faraday_rotate(s, c, 0.5 * dl);
+    absorb_emit(s, c, dl);
+    faraday_rotate(s, c, 0.5 * dl);
+}
+
+/* transform_to_observer_basis (src/frame/stokes_rotation.hpp:39-57). */
+void kpo_transform_to_observer_basis(const kpo_stokes* in, double r11, double r12, double r21,
+                                     double r22, kpo_stokes* out) {
+    kpo_stokes o = *in;
+    o.Q = 0.5 * in->Q * (r11 * r11 - r12 * r12 - r21 * r21 + r22 * r22) + in->U * (r11 * r12 - r21 * r22);
+    o.U = in->Q * (r11 * r21 - r12 * r22) + in->U * (r11 * r22 + r12 * r21);
+    o.V = (r11 * r22 - r12 * r21) * in->V;
+    *out = o;
+}
+
+/* radiation_substep_count with its rate helpers (src/geodesic/pass_b.hpp:126-179). */
+int32_t kpo_radiation_substep_count(const kpo_coeffs* c, double geometric_step,
+                                    double dlambda_scale, int32_t requested,
+                                    double max_radiation_step, double max_radiation_depth,
+                                    double max_absorption_depth, double max_faraday_depth) {
+    int nsub = requested > 0 ? requested : 1;
+    const double abs_h = absv(geometric_step);
+    if (max_radiation_step > 0.0)
+        while (abs_h / (double)nsub > max_radiation_step && nsub < 4096) nsub *= 2;
+    const double dl = abs_h * absv(dlambda_scale);
+    const double alim = max_absorption_depth >= 0.0 ? max_absorption_depth : max_radiation_depth;
+    if (alim > 0.0) {
+        const double depth = (absv(c->aI) + sqrt(sq(c->aQ) + sq(c->aU) + sq(c->aV))) * dl;
+        while (depth / (double)nsub > alim && nsub < 4096) nsub *= 2;
+    }
+    const double flim = max_faraday_depth >= 0.0 ? max_faraday_depth : max_radiation_depth;
+    if (flim > 0.0) {
+        const double depth = sqrt(sq(c->rQ) + sq(c->rU) + sq(c->rV)) * dl;
+        while (depth / (double)nsub > flim && nsub < 4096) nsub *= 2;
+    }
+    return nsub;
+}
+
+/* ======================================================================
+ * Passes (src/geodesic/pass_b.hpp, src/geodesic/pass_a_core.hpp)
+ * ==================================================================== */
+static int finite_state(const kpo_state* s) {
+    for (int i = 0; i < 16; ++i) {
+        const double v = s->v[i];
+        if (!(v == v) || absv(v) >= LARGE) return 0;
+    }
+    return 1;
+}
+
+/* metric_surface_path_state (pass_a_core.hpp:73-84). */
+static void surface_path_state(const metric_ctx* m, const kpo_state* first, double h, int composed,
+                               kpo_state* out) {
+    if (composed) {
+        kpo_state half;
+        rk4(m, first, h * 0.5, &half);
+        rk4(m, &half, h * 0.5, out);
+        return;
+    }
+    rk4(m, first, h, out);
+}
+
+/* locate_metric_surface_crossing for a radial surface (pass_a_core.hpp:94-197).
+ * For BL the inner surface value and the domain radius are both x[1]. */
+static void locate_crossing(const metric_ctx* m, const kpo_state* first, const kpo_state* second,
+                            double full_h, double surface, int composed, kpo_state* out_state,
+                            double* out_h) {
+    *out_state = *first;
+    *out_h = 0.0;
+    double lo = 0.0, hi = 1.0;
+    double flo = first->v[1] - surface;
+    if (flo == 0.0) return;
+    if (!finite_state(second)) return;
+    kpo_state hi_state;
+    surface_path_state(m, first, full_h, composed, &hi_state);
+    if (!finite_state(&hi_state)) return;
+    double fhi = hi_state.v[1] - surface;
+    if (!(fhi == fhi) || absv(fhi) >= LARGE) return;
+    if (fhi == 0.0) {
+        *out_state = hi_state;
+        *out_h = full_h;
+        return;
+    }
+    if ((flo > 0.0) == (fhi > 0.0)) return;
+    int use_secant = 1;
+    for (int iter = 0; iter < 12; ++iter) {
+        const double width = hi - lo;
+        double trial = (lo + hi) * 0.5;
+        const double denom = flo - fhi;
+        if (use_secant && absv(denom) > TINY) {
+            const double secant = lo + width * flo / denom;
+            const double guard = width * 0.1;
+            if (secant > lo + guard && secant < hi - guard) trial = secant;
+        }
+        kpo_state ts;
+        surface_path_state(m, first, full_h * trial, composed, &ts);
+        if (!finite_state(&ts)) {
+            hi = trial;
+            use_secant = 0;
+            continue;
+        }
+        const double ft = ts.v[1] - surface;
+        if (!(ft == ft) || absv(ft) >= LARGE) {
+            hi = trial;
+            use_secant = 0;
+            continue;
+        }
+        if (ft == 0.0) {
+            *out_state = ts;
+            *out_h = full_h * trial;
+            return;
+        }
+        if ((flo > 0.0) == (ft > 0.0)) {
+            lo = trial;
+            flo = ft;
+            *out_state = ts;
+            *out_h = full_h * trial;
+        } else {
+            hi = trial;
+            fhi = ft;
+        }
+    }
+}
+
+/* trace_pass_a_segment_endpoint_pixel_metric (pass_b.hpp:337-486), BL hooks
+ * reduced per metric_domain.hpp:159-223 (no time domain, no step limits). */
+static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const kpo_state* init,
+                           kpo_state* state_out, int32_t* steps_out, int32_t* reason_out) {
+    kpo_state state = *init;
+    int reason = KP_REASON_MAX_STEPS;
+    int steps = 0;
+    adaptive_control ac = {p->adaptive_tolerance, p->min_step, p->max_step, 10};
+    double h_current = -absv(p->step);
+    int entered = 0;
+    int steps_inside = 0;
+    for (int n = 0; n < p->max_steps; ++n) {
+        const double r_bl = state.v[1];
+        if (!entered) {
+            if (r_bl <= p->outer_radius) {
+                entered = 1;
+                steps_inside = 0;
+            }
+        } else {
+            if (r_bl - p->inner_radius <= 0.0) {
+                reason = KP_REASON_REACHED_INNER_BOUNDARY;
+                break;
+            }
+            if (steps_inside > 0 && r_bl >= p->outer_radius) {
+                reason = KP_REASON_ESCAPED_DOMAIN;
+                break;
+            }
+        }
+        const kpo_state old = state;
+        const double eff_min = absv(p->min_step);
+        const double old_r = r_bl;
+        const double old_inner = old.v[1] - p->inner_radius;
+        double used_h = h_current;
+        kpo_state next;
+        if (p->adaptive) {
+            adaptive_control tc = ac;
+            tc.min_step = eff_min;
+            if (absv(used_h) < tc.min_step) tc.min_step = absv(used_h);
+            adaptive_result st;
+            adaptive_rk4(m, &old, used_h, tc, &st);
+            if (!st.accepted) {
+                h_current = st.next_h;
+                if (absv(h_current) <= eff_min * 1.0001) {
+                    reason = KP_REASON_ADAPTIVE_STEP_UNDERFLOW;
+                    break;
+                }
+                continue;
+            }
+            next = st.state;
+            used_h = st.used_h;
+            h_current = st.next_h;
+        } else {
+            rk4(m, &state, used_h, &next);
+        }
+        const double next_r = next.v[1];
+        const double next_inner = next.v[1] - p->inner_radius;
+        if (!entered && next_r <= p->outer_radius) {
+            entered = 1;
+            steps_inside = 0;
+        } else if (!entered && next_r >= old_r) {
+            state = next;
+            reason = KP_REASON_ESCAPED_DOMAIN;
+            steps += 1;
+            break;
+        }
+        int crossing = KP_REASON_NONE;
+        if (entered && old_inner > 0.0 && next_inner <= 0.0) {
+            crossing = KP_REASON_REACHED_INNER_BOUNDARY;
+        } else if (entered && steps_inside > 0 && old_r < p->outer_radius &&
+                   next_r >= p->outer_radius) {
+            crossing = KP_REASON_ESCAPED_DOMAIN;
+        }
+        if (crossing != KP_REASON_NONE) {
+            const int inner = crossing == KP_REASON_REACHED_INNER_BOUNDARY;
+            const double surface = inner ? p->inner_radius : p->outer_radius;
+            kpo_state ev;
+            double ev_h;
+            locate_crossing(m, &old, &next, used_h, surface, p->adaptive, &ev, &ev_h);
+            next = ev;
+            used_h = ev_h;
+            state = next;
+            reason = crossing;
+            steps += 1;
+            break;
+        }
+        state = next;
+        steps += 1;
+        if (entered) steps_inside += 1;
+    }
+    const int valid = entered && (reason == KP_REASON_REACHED_INNER_BOUNDARY ||
+                                  reason == KP_REASON_ESCAPED_DOMAIN);
+    *state_out = state;
+    *steps_out = steps;
+    *reason_out = reason;
+    return valid;
+}
+
+int32_t kpo_bl_pass_a_endpoint(const kp_pass_a_params* p, int32_t pixel, kpo_state* endpoint,
+                               int32_t* steps, int32_t* reason) {
+    metric_ctx m = {p->mass, p->spin, NULL};
+    kpo_state init;
+    kpo_bl_camera_ray(p, pixel, &init);
+    return pass_a_endpoint(p, &m, &init, endpoint, steps, reason);
+}
+
+static double spatial_distance(const double* a, const double* b) {
+    return sqrt(sq(a[1] - b[1]) + sq(a[2] - b[2]) + sq(a[3] - b[3]));
+}
+
+static double max_abs_diff4(const double* a, const double* b) {
+    double out = absv(a[0] - b[0]);
+    for (int i = 1; i < 4; ++i) out = maxv(out, absv(a[i] - b[i]));
+    return out;
+}
+
+typedef struct pixel_result {
+    kpo_stokes observed;
+    double r11, r12, r21, r22;
+    double closure_x, closure_k, final_null, frame_error;
+    int pass_a_steps, steps, reason;
+} pixel_result;
+
+/* Evaluate the plasma coefficients on the sample state, with the
+ * emission-selection Faraday switch (emission_selection.hpp:35-42). */
+static void sample_coefficients(const kp_pass_a_params* p, const kp_riaf_model* model,
+                                const kpo_state* s, const kpo_perturbation* pert, kpo_coeffs* c,
+                                kpo_stats* stats) {
+    if (stats) stats->coefficient_evals++;
+    kpo_riaf_coefficients(p, model, s, pert, c);
+    if (!p->emission_selection.faraday_rotation) c->rV = 0.0;
+}
+
+/* trace_pass_b_segment_model_endpoint_pixel_metric (pass_b.hpp:515-722). */
+static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
+                   const kpo_perturbation* pert, const metric_ctx* m, const kpo_state* camera,
+                   const kpo_state* endpoint, int pass_a_steps, int pass_a_reason, int valid,
+                   pixel_result* out, kpo_stats* stats) {
+    memset(out, 0, sizeof(*out));
+    out->r11 = 1.0;
+    out->r22 = 1.0;
+    out->pass_a_steps = pass_a_steps;
+    out->reason = pass_a_reason;
+    kpo_state state = *endpoint;
+    if (!valid) {
+        out->closure_x = spatial_distance(state.v, camera->v);
+        out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
+        out->final_null = kpo_bl_dot(p->mass, p->spin, state.v, state.v + 4, state.v + 4);
+        out->frame_error = max_frame_error(p->mass, p->spin, &state);
+        return;
+    }
+    const adaptive_control tcontrol = {p->adaptive_tolerance, p->min_step, p->max_step, 10};
+    kpo_stokes stokes = {0.0, 0.0, 0.0, 0.0};
+    const double dls = riaf_dlambda_scale(model);
+    int reached = 0;
+    int steps = 0;
+    int reason = pass_a_reason;
+    double h_current = p->step;
+    double cap = p->max_radiation_step > 0.0 ? p->max_radiation_step : p->max_step;
+    while (!reached && steps < p->max_steps) {
+        double h = h_current;
+        const kpo_state old = state;
+        const double eff_min = absv(p->min_step);
+        const double start_r = old.v[1];
+        if (start_r <= p->outer_radius && cap > 0.0) h = minv(h, cap);
+        kpo_state next, sample;
+        if (p->adaptive) {
+            adaptive_control tc = tcontrol;
+            tc.min_step = eff_min;
+            adaptive_result pr;
+            adaptive_rk4(m, &old, h, tc, &pr);
+            if (!pr.accepted) {
+                h_current = absv(pr.next_h);
+                if (h_current <= eff_min * 1.0001) {
+                    reason = KP_REASON_ADAPTIVE_STEP_UNDERFLOW;
+                    break;
+                }
+                continue;
+            }
+            h = absv(pr.used_h);
+            h_current = absv(pr.next_h);
+            next = pr.state;
+            sample = pr.mid_state;
+        } else {
+            rk4(m, &old, h, &next);
+            sample = next;
+        }
+        const double s0 = old.v[1] - p->camera.radius;
+        const double s1 = next.v[1] - p->camera.radius;
+        const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1
+                            : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+        if (crosses) {
+            const double denom = s0 - s1;
+            const double frac = absv(denom) <= TINY ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
+            const double h_cross = h * frac;
+            if (absv(h_cross) <= 1e-14) {
+                h = 0.0;
+                next = old;
+                sample = old;
+            } else if (frac < 0.999999999999) {
+                h = h_cross;
+                if (p->adaptive) {
+                    adaptive_control cc = tcontrol;
+                    cc.min_step = minv(eff_min, absv(h));
+                    cc.max_step = maxv(cc.min_step, absv(h));
+                    adaptive_result cr;
+                    adaptive_rk4(m, &old, h, cc, &cr);
+                    if (!cr.accepted) {
+                        h_current = absv(cr.next_h);
+                        continue;
+                    }
+                    h = absv(cr.used_h);
+                    next = cr.state;
+                    sample = cr.mid_state;
+                } else {
+                    rk4(m, &old, h, &next);
+                    sample = next;
+                }
+            }
+        }
+        const double old_r = old.v[1], sample_r = sample.v[1], next_r = next.v[1];
+        kpo_coeffs c;
+        memset(&c, 0, sizeof(c));
+        if (sample.v[1] - p->inner_radius > 0.0 && sample_r < p->outer_radius)
+            sample_coefficients(p, model, &sample, pert, &c, stats);
+        const double local_max_rs =
+            (p->max_radiation_step > 0.0 &&
+             (old_r <= p->outer_radius || sample_r <= p->outer_radius || next_r <= p->outer_radius))
+                ? p->max_radiation_step
+                : 0.0;
+        const int required = kpo_radiation_substep_count(&c, h, dls, 1, local_max_rs,
+                                                         p->max_radiation_depth,
+                                                         p->max_absorption_depth,
+                                                         p->max_faraday_depth);
+        if (required > 1 && h > eff_min * 1.0001) {
+            h_current = maxv(eff_min, h / (double)required);
+            cap = h_current;
+            continue;
+        }
+        if (local_max_rs > 0.0 || p->max_radiation_depth > 0.0 || p->max_absorption_depth > 0.0 ||
+            p->max_faraday_depth > 0.0) {
+            const int predictive = kpo_radiation_substep_count(&c, h_current, dls, 1, local_max_rs,
+                                                               p->max_radiation_depth,
+                                                               p->max_absorption_depth,
+                                                               p->max_faraday_depth);
+            cap = predictive > 1 ? maxv(eff_min, h_current / (double)predictive) : p->max_step;
+        } else {
+            cap = p->max_step;
+        }
+        if (h > 0.0) {
+            if (stats) stats->transfer_steps++;
+            kpo_semi_analytic_step(&stokes, &c, absv(h) * dls);
+        }
+        state = next;
+        steps += 1;
+        if (crosses) reached = 1;
+    }
+    /* screen_overlap (pass_b.hpp:87-98) at the final position. */
+    double g[16];
+    kpo_bl_gcov(p->mass, p->spin, state.v, g);
+    out->r11 = dot_g(g, camera->v + 8, state.v + 8);
+    out->r12 = dot_g(g, camera->v + 8, state.v + 12);
+    out->r21 = dot_g(g, camera->v + 12, state.v + 8);
+    out->r22 = dot_g(g, camera->v + 12, state.v + 12);
+    kpo_transform_to_observer_basis(&stokes, out->r11, out->r12, out->r21, out->r22, &out->observed);
+    out->closure_x = spatial_distance(state.v, camera->v);
+    out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
+    out->final_null = dot_g(g, state.v + 4, state.v + 4);
+    out->frame_error = max_frame_error(p->mass, p->spin, &state);
+    out->steps = steps;
+    if (reached) {
+        reason = KP_REASON_REACHED_CAMERA;
+    } else if (reason != KP_REASON_ADAPTIVE_STEP_UNDERFLOW &&
+               reason != KP_REASON_METRIC_TIME_EXHAUSTED) {
+        reason = KP_REASON_MAX_STEPS;
+    }
+    out->reason = reason;
+}
+
+static void store_pixel(kp_image_data* o, int p, const pixel_result* r) {
+#define PUT(field, val) do { if (o->field) o->field[p] = (val); } while (0)
+    PUT(image_i, r->observed.I);
+    PUT(image_q, r->observed.Q);
+    PUT(image_u, r->observed.U);
+    PUT(image_v, r->observed.V);
+    PUT(closure_x, r->closure_x);
+    PUT(closure_k, r->closure_k);
+    PUT(final_null, r->final_null);
+    PUT(frame_error, r->frame_error);
+    PUT(det_r, r->r11 * r->r22 - r->r12 * r->r21);
+    PUT(overlap_r11, r->r11);
+    PUT(overlap_r12, r->r12);
+    PUT(overlap_r21, r->r21);
+    PUT(overlap_r22, r->r22);
+    PUT(basis_identity_error, maxv(maxv(absv(r->r11 - 1.0), absv(r->r22 - 1.0)),
+                                   maxv(absv(r->r12), absv(r->r21))));
+    PUT(basis_rotation_angle, atan2(r->r21 - r->r12, r->r11 + r->r22));
+    PUT(pass_a_steps, r->pass_a_steps);
+    PUT(steps, r->steps);
+    PUT(reason, r->reason);
+#undef PUT
+}
+
+/* Split pass A / pass B image (src/image/split_driver.hpp:41-104), each pixel
+ * independent; OpenMP static schedule as the reference's RangePolicy. */
+kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* model,
+                             const kpo_perturbation* pert, int32_t pixel_begin,
+                             int32_t pixel_end, int32_t nthreads, kp_image_data* out,
+                             kpo_stats* stats) {
+    if (!p || !model || !out) return KP_INVALID_ARGUMENT;
+    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST || p->camera.model != KP_CAMERA_PINHOLE)
+        return KP_UNSUPPORTED;
+    const int npix = p->camera.nx * p->camera.ny;
+    const int begin = pixel_begin < 0 ? 0 : pixel_begin;
+    const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
+    double tetrad[16], gcam[16];
+    const double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
+    kpo_bl_camera_tetrad(p, tetrad);
+    kpo_bl_gcov(p->mass, p->spin, xcam, gcam);
+    int64_t rhs_total = 0, coeff_total = 0, transfer_total = 0, a_total = 0, b_total = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(+ : rhs_total, coeff_total, transfer_total, a_total, b_total)
+#endif
+    for (int pix = begin; pix < end; ++pix) {
+        int64_t rhs_count = 0;
+        metric_ctx m = {p->mass, p->spin, &rhs_count};
+        kpo_stats local = {0, 0, 0, 0, 0};
+        kpo_state camera, endpoint;
+        camera_ray_from_tetrad(p, tetrad, gcam, pix, &camera);
+        int32_t a_steps = 0, a_reason = 0;
+        const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
+        pixel_result r;
+        pass_b(p, model, pert, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
+        store_pixel(out, pix, &r);
+        rhs_total += rhs_count;
+        coeff_total += local.coefficient_evals;
+        transfer_total += local.transfer_steps;
+        a_total += a_steps;
+        b_total += r.steps;
+    }
+    if (stats) {
+        stats->rhs_evals = rhs_total;
+        stats->coefficient_evals = coeff_total;
+        stats->transfer_steps = transfer_total;
+        stats->pass_a_steps = a_total;
+        stats->pass_b_steps = b_total;
+    }
+    return KP_OK;
+}

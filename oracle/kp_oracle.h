@@ -1,0 +1,112 @@
+/*
+ * kp_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C11, IEEE double, libm) of the KPolaris imaging
+ * path.  It is the checker for the HIP kernels: only tests, the smoke check
+ * and the benchmark's CPU-baseline leg may load it.  The product library
+ * never links, loads or falls back to it.
+ *
+ * Parity status of every function is recorded in the header of kp_oracle.c.
+ */
+#ifndef KP_ORACLE_H
+#define KP_ORACLE_H
+
+#include <stdint.h>
+#include "../include/kp_image.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* State layout: x[0..3], k[0..3], e1[0..3], e2[0..3] (TransportState). */
+typedef struct kpo_state { double v[16]; } kpo_state;
+
+typedef struct kpo_coeffs {
+    double jI, jQ, jU, jV, aI, aQ, aU, aV, rQ, rU, rV;
+} kpo_coeffs;
+
+typedef struct kpo_stokes { double I, Q, U, V; } kpo_stokes;
+
+/* Plasma perturbation (src/radiation/plasma_perturbation.hpp:9-33). */
+typedef struct kpo_perturbation {
+    int32_t parameter;   /* 0 none, 1 density, 2 temperature, 3 magnetic */
+    int32_t reserved0;
+    double log_scale;
+} kpo_perturbation;
+
+typedef struct kpo_stats {
+    int64_t rhs_evals;
+    int64_t coefficient_evals;
+    int64_t transfer_steps;
+    int64_t pass_a_steps;
+    int64_t pass_b_steps;
+} kpo_stats;
+
+/* ---- geometry -------------------------------------------------------- */
+void kpo_bl_gcov(double mass, double spin, const double x[4], double g[16]);
+void kpo_bl_gcon(double mass, double spin, const double x[4], double g[16]);
+void kpo_bl_dgcov(double mass, double spin, const double x[4], double dg[64]);
+/* Generic connection from gcon and dgcov (src/geometry/connection.hpp:7-24). */
+void kpo_bl_connection(double mass, double spin, const double x[4], double gamma[64]);
+void kpo_bl_transport_rhs(double mass, double spin, const kpo_state* s, kpo_state* d);
+void kpo_bl_transport_rhs_generic(double mass, double spin, const kpo_state* s, kpo_state* d);
+double kpo_bl_dot(double mass, double spin, const double x[4],
+                  const double u[4], const double v[4]);
+
+/* ---- integrators ----------------------------------------------------- */
+void kpo_bl_rk4_step(double mass, double spin, const kpo_state* s, double h, kpo_state* out);
+/* Returns accepted flag; fills state, mid_state, used_h, next_h, error. */
+int32_t kpo_bl_adaptive_rk4_step(double mass, double spin, const kpo_state* s,
+                                 double requested_h, double tolerance,
+                                 double min_step, double max_step, int32_t max_attempts,
+                                 kpo_state* state, kpo_state* mid_state,
+                                 double* used_h, double* next_h, double* error);
+double kpo_state_error_norm(const kpo_state* a, const kpo_state* b);
+
+/* ---- camera ---------------------------------------------------------- */
+void kpo_pixel_offsets(int32_t pixel, const kp_camera_params* cam, double* sx, double* sy);
+void kpo_bl_camera_ray(const kp_pass_a_params* params, int32_t pixel, kpo_state* out);
+void kpo_bl_camera_tetrad(const kp_pass_a_params* params, double tetrad[16]);
+
+/* ---- radiation ------------------------------------------------------- */
+void kpo_thermal_coefficients(int32_t fit, double nu, double ne, double thetae,
+                              double b_cgs, double theta, double sin_theta,
+                              double cos_theta, double max_pol_frac,
+                              double min_sin_theta, double emission_scale,
+                              double absorption_scale, double faraday_scale,
+                              kpo_coeffs* out);
+double kpo_symphony_rho_v(double nu, double ne, double thetae, double b_cgs,
+                          double theta, double min_sin_theta);
+void kpo_bessel_k012(double x, double* k0, double* k1, double* k2);
+void kpo_riaf_coefficients(const kp_pass_a_params* params, const kp_riaf_model* model,
+                           const kpo_state* s, const kpo_perturbation* pert,
+                           kpo_coeffs* out);
+void kpo_riaf_fluid_velocity(const kp_pass_a_params* params, const kp_riaf_model* model,
+                             const double x[4], double u[4]);
+double kpo_riaf_isco_radius(double spin);
+void kpo_semi_analytic_step(kpo_stokes* s, const kpo_coeffs* c, double dlambda);
+void kpo_transform_to_observer_basis(const kpo_stokes* in, double r11, double r12,
+                                     double r21, double r22, kpo_stokes* out);
+int32_t kpo_radiation_substep_count(const kpo_coeffs* c, double geometric_step,
+                                    double dlambda_scale, int32_t requested_substeps,
+                                    double max_radiation_step, double max_radiation_depth,
+                                    double max_absorption_depth, double max_faraday_depth);
+
+/* ---- passes and images ----------------------------------------------- */
+/* Pass A segment endpoint for one pixel (BL). Returns valid flag. */
+int32_t kpo_bl_pass_a_endpoint(const kp_pass_a_params* params, int32_t pixel,
+                               kpo_state* endpoint, int32_t* steps, int32_t* reason);
+
+/* Whole image: pass A then pass B per pixel, OpenMP static schedule over
+ * pixels.  pixel_begin/pixel_end select a sub-range (end<0: all).  Output
+ * arrays are indexed by absolute pixel and may be NULL. */
+kp_status kpo_run_riaf_image(const kp_pass_a_params* params, const kp_riaf_model* model,
+                             const kpo_perturbation* pert,
+                             int32_t pixel_begin, int32_t pixel_end,
+                             int32_t nthreads, kp_image_data* out, kpo_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
