@@ -215,6 +215,7 @@ typedef struct kp_image_stats {
     int64_t pass_a_steps;       /* accepted steps, pass A */
     int64_t pass_b_steps;       /* accepted steps, pass B */
     int64_t rhs_evals;          /* transport_rhs evaluations, both passes */
+    int64_t pass_a_rhs_evals;   /* transport_rhs evaluations in pass A */
     int64_t coefficient_evals;  /* plasma coefficient evaluations */
     int64_t transfer_steps;     /* Strang transfer steps */
     int64_t fp64_guard_steps;   /* mixed precision: steps taken in FP64 */

@@ -144,7 +144,8 @@ class ImageStats(C.Structure):
     _fields_ = [
         ("pass_a_ms", C.c_double), ("pass_b_ms", C.c_double), ("kernel_ms", C.c_double),
         ("rays", C.c_int64), ("pass_a_steps", C.c_int64), ("pass_b_steps", C.c_int64),
-        ("rhs_evals", C.c_int64), ("coefficient_evals", C.c_int64),
+        ("rhs_evals", C.c_int64), ("pass_a_rhs_evals", C.c_int64),
+        ("coefficient_evals", C.c_int64),
         ("transfer_steps", C.c_int64), ("fp64_guard_steps", C.c_int64),
     ]
 

@@ -1,0 +1,723 @@
+// kp_host.cpp -- host side of the imaging library: error state, defaults,
+// camera tetrad, model constants, the option layer of the reference image
+// tool, and the output packing rules.  Plain C++; compiled by hipcc with the
+// kernels into one shared library.
+#include "kp_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <string>
+
+namespace kp_internal {
+
+static thread_local std::string g_error;
+
+void clear_error() { g_error.clear(); }
+
+kp_status fail(kp_status status, const std::string& message) {
+    g_error = message;
+    return status;
+}
+
+// --- BL metric at a point (src/geometry/kerr_boyer_lindquist.hpp:47-99) -----
+namespace {
+struct Metric4 {
+    double g[4][4];
+    double c[4][4];
+};
+
+Metric4 bl_metric(double m, double a, double r_in, double th) {
+    Metric4 M;
+    std::memset(&M, 0, sizeof(M));
+    const double r = std::max(r_in, 1e-300);
+    const double sn = std::sin(th), cs = std::cos(th);
+    const double a2 = a * a;
+    const double sin2 = std::max(sn * sn, 1e-300);
+    const double sig = r * r + a2 * cs * cs;
+    const double del = r * r - 2.0 * m * r + a2;
+    M.g[0][0] = -(1.0 - 2.0 * m * r / sig);
+    M.g[0][3] = M.g[3][0] = -2.0 * m * a * r * sin2 / sig;
+    M.g[1][1] = sig / del;
+    M.g[2][2] = sig;
+    M.g[3][3] = sin2 * (r * r + a2 + 2.0 * m * a2 * r * sin2 / sig);
+    const double isd = 1.0 / (sig * del);
+    const double r2pa2 = r * r + a2;
+    M.c[0][0] = -(r2pa2 * r2pa2 - a2 * del * sin2) * isd;
+    M.c[0][3] = M.c[3][0] = -2.0 * m * a * r * isd;
+    M.c[1][1] = del / sig;
+    M.c[2][2] = 1.0 / sig;
+    M.c[3][3] = (del - a2 * sin2) / (sig * del * sin2);
+    return M;
+}
+
+double mdot(const Metric4& M, const double* u, const double* v) {
+    double s = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) s += M.g[i][j] * u[i] * v[j];
+    return s;
+}
+
+void raise(const Metric4& M, const double* cov, double* out) {
+    for (int i = 0; i < 4; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 4; ++j) s += M.c[i][j] * cov[j];
+        out[i] = s;
+    }
+}
+
+void project_out(const Metric4& M, double* a, const double* b) {
+    const double b2 = mdot(M, b, b);
+    if (std::fabs(b2) > 1e-300) {
+        const double f = mdot(M, a, b) / b2;
+        for (int i = 0; i < 4; ++i) a[i] -= b[i] * f;
+    }
+}
+
+void normalise(const Metric4& M, double* v) {
+    const double n = std::sqrt(std::max(std::fabs(mdot(M, v, v)), 1e-300));
+    for (int i = 0; i < 4; ++i) v[i] *= 1.0 / n;
+}
+
+// Sign of the Levi-Civita contraction eps_{ijkl} e0^i e1^j e2^k e3^l, i.e. det[e0;e1;e2;e3].
+double orientation(const double* e0, const double* e1, const double* e2, const double* e3) {
+    const double* rows[4] = {e0, e1, e2, e3};
+    double a[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) a[i][j] = rows[i][j];
+    double det = 1.0;
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return 0.0;
+        if (piv != col) {
+            for (int j = 0; j < 4; ++j) std::swap(a[piv][j], a[col][j]);
+            det = -det;
+        }
+        det *= a[col][col];
+        for (int r = col + 1; r < 4; ++r) {
+            const double f = a[r][col] / a[col][col];
+            for (int j = col; j < 4; ++j) a[r][j] -= f * a[col][j];
+        }
+    }
+    return det;
+}
+}  // namespace
+
+// make_ipole_camera_tetrad, BL branch (src/camera/camera.hpp:300-320, 350-360,
+// 396-453): KS normal observer, native-MKS radial centring covector, native
+// theta and (1,1,1,1) trial vectors, Gram-Schmidt with repeated projections.
+void bl_camera_tetrad(double mass, double spin, double radius, double inclination, double tetrad[16],
+                      double gcov5[5]) {
+    const Metric4 M = bl_metric(mass, spin, radius, inclination);
+    double* e0 = tetrad;
+    double* e1 = tetrad + 4;
+    double* e2 = tetrad + 8;
+    double* e3 = tetrad + 12;
+    const double r = std::max(radius, 1e-300);
+    const double delta = std::max(r * r - 2.0 * mass * r + spin * spin, 1e-300);
+    const double obs_cov[4] = {-1.0, -2.0 * mass * r / delta, 0.0, 0.0};
+    raise(M, obs_cov, e0);
+    const double centre_cov[4] = {1.0, 1.0 / r + 2.0 * mass * r / delta, 0.0, 0.0};
+    raise(M, centre_cov, e3);
+    const double pi = 3.141592653589793238462643383279502884;
+    e2[0] = 0.0;
+    e2[1] = 0.0;
+    e2[2] = pi;
+    e2[3] = 0.0;
+    e1[0] = 1.0 - 2.0 * mass * r * r / delta;
+    e1[1] = r;
+    e1[2] = pi;
+    e1[3] = 1.0 - spin * r / delta;
+    normalise(M, e0);
+    project_out(M, e3, e0);
+    project_out(M, e3, e0);
+    normalise(M, e3);
+    for (int rep = 0; rep < 2; ++rep) {
+        project_out(M, e2, e0);
+        project_out(M, e2, e3);
+    }
+    normalise(M, e2);
+    for (int rep = 0; rep < 2; ++rep) {
+        project_out(M, e1, e0);
+        project_out(M, e1, e2);
+        project_out(M, e1, e3);
+    }
+    normalise(M, e1);
+    if (orientation(e0, e1, e2, e3) < 0.0)
+        for (int i = 0; i < 4; ++i) e1[i] = -e1[i];
+    gcov5[0] = M.g[0][0];
+    gcov5[1] = M.g[0][3];
+    gcov5[2] = M.g[1][1];
+    gcov5[3] = M.g[2][2];
+    gcov5[4] = M.g[3][3];
+}
+
+// isco_radius (src/model/riaf.hpp:168-176): Bardeen-Press-Teukolsky.
+double isco_radius(double a) {
+    const double z1 = 1.0 + std::cbrt(1.0 - a * a) * (std::cbrt(1.0 + a) + std::cbrt(1.0 - a));
+    const double z2 = std::sqrt(3.0 * a * a + z1 * z1);
+    const double sign = a < 0.0 ? -1.0 : 1.0;
+    return 3.0 + z2 - sign * std::sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
+
+void riaf_constants(const kp_pass_a_params& p, const kp_riaf_model& m, int perturbation, double q,
+                    kpd::RiafDev<double>& o) {
+    const double kbol = 1.380649e-16, me = 9.1093837015e-28, cl = 2.99792458e10;
+    o.nth0 = m.nth0;
+    o.Te0 = m.Te0;
+    o.disk_h = m.disk_h;
+    o.pow_nth = m.pow_nth;
+    o.pow_T = m.pow_T;
+    o.ne_unit = m.ne_unit;
+    o.thetae_unit = m.te_unit * kbol / (me * cl * cl);
+    o.r_min = m.r_min;
+    o.r_max = m.r_max;
+    o.horizon_buffer = m.horizon_buffer;
+    o.emission_scale = m.emission_scale;
+    o.absorption_scale = m.absorption_scale;
+    o.faraday_scale = m.faraday_scale;
+    o.max_pol_frac = m.max_pol_frac;
+    o.keplerian_factor = m.keplerian_factor;
+    o.infall_factor = m.infall_factor;
+    o.min_frequency_scale = m.min_frequency_scale;
+    o.min_sin_theta = m.min_sin_theta;
+    o.min_thetae = m.min_thetae;
+    o.min_radius = m.min_radius;
+    o.freq_cgs = m.freq_cgs;
+    // length_unit_cgs / dlambda_scale (src/model/riaf.hpp:76-85)
+    const double l_unit = 6.67430e-8 * m.mbh_solar * 1.98847e33 / (cl * cl);
+    o.dlambda_scale = l_unit / std::max(m.freq_cgs, 1.0);
+    o.rh = p.mass + std::sqrt(std::max(p.mass * p.mass - p.spin * p.spin, 0.0));
+    o.risco = isco_radius(p.spin);
+    o.ne_factor = 1.0;
+    o.thetae_factor = 1.0;
+    o.b_factor = 1.0;
+    // PlasmaPerturbation::apply (src/radiation/plasma_perturbation.hpp:17-32)
+    if (q != 0.0) {
+        if (perturbation == 1) {
+            o.ne_factor = std::exp(q);
+            o.b_factor = std::exp(q * 0.5);
+        } else if (perturbation == 2) {
+            o.thetae_factor = std::exp(q);
+        } else if (perturbation == 3) {
+            o.b_factor = std::exp(q);
+        }
+    }
+    o.mass = p.mass;
+    o.spin = p.spin;
+    o.fit = m.emission_type == KP_EMISSION_THERMAL_DEXTER ? 4 : 1;
+    o.faraday_rotation = p.emission_selection.faraday_rotation;
+}
+
+}  // namespace kp_internal
+
+// ===========================================================================
+// Public C ABI: version, errors, defaults
+// ===========================================================================
+extern "C" int32_t kp_abi_version(void) { return KP_ABI_VERSION; }
+
+extern "C" const char* kp_last_error(void) { return kp_internal::g_error.c_str(); }
+
+extern "C" void kp_pass_a_params_default(kp_pass_a_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->camera.nx = 64;
+    p->camera.ny = 64;
+    p->camera.radius = 100.0;
+    p->camera.inclination = 1.04719755119659774615;
+    p->camera.fov = 10.0;
+    p->camera.fov_y = -1.0;
+    p->camera.xspan = -1.0;
+    p->camera.yspan = -1.0;
+    p->camera.model = KP_CAMERA_PINHOLE;
+    p->emission_selection.equatorial_h_over_r = 0.0;
+    p->emission_selection.equatorial_samples = 8;
+    p->emission_selection.faraday_rotation = 1;
+    p->mass = 1.0;
+    p->spin = 0.0;
+    p->inner_radius = 2.2;
+    p->outer_radius = 200.0;
+    p->step = 0.05;
+    p->max_steps = 4096;
+    p->adaptive = 0;
+    p->direct_only = 0;
+    p->coordinate_system = KP_COORD_CARTESIAN_KS;
+    p->adaptive_tolerance = 1e-7;
+    p->min_step = 1e-5;
+    p->max_step = 1.0;
+    p->max_radiation_step = 1.0;
+    p->max_radiation_depth = 1.0;
+    p->max_absorption_depth = -1.0;
+    p->max_faraday_depth = -1.0;
+    p->fmks_startx1 = 0.0;
+    p->fmks_hslope = 0.3;
+    p->fmks_mks_smooth = 0.5;
+    p->fmks_poly_alpha = 14.0;
+    p->fmks_poly_xt = 0.82;
+    p->fmks_poly_norm = 1.0;
+}
+
+extern "C" void kp_riaf_model_default(kp_riaf_model* m) {
+    if (!m) return;
+    std::memset(m, 0, sizeof(*m));
+    m->nth0 = 1.0;
+    m->Te0 = 1.0;
+    m->disk_h = 0.35;
+    m->pow_nth = -1.1;
+    m->pow_T = -0.84;
+    m->ne_unit = 5.0e6;
+    m->te_unit = 1.0e11;
+    m->b_unit = 30.0;
+    m->freq_cgs = 230.0e9;
+    m->mbh_solar = 4.3e6;
+    m->r_min = 1.0;
+    m->r_max = 100.0;
+    m->horizon_buffer = 0.1;
+    m->emission_scale = 1.0;
+    m->absorption_scale = 1.0;
+    m->faraday_scale = 1.0;
+    m->max_pol_frac = 0.99;
+    m->nonthermal_kappa = 3.5;
+    m->variable_kappa = 0;
+    m->emission_type = KP_EMISSION_THERMAL_PANDYA;
+    m->variable_kappa_min = 3.1;
+    m->variable_kappa_interp_start = 1e20;
+    m->variable_kappa_max = 7.0;
+    m->powerlaw_p = 3.25;
+    m->powerlaw_eta = 0.02;
+    m->powerlaw_gamma_min = 1e2;
+    m->powerlaw_gamma_max = 1e5;
+    m->powerlaw_gamma_cutoff = 1e10;
+    m->keplerian_factor = 1.0;
+    m->infall_factor = 0.0;
+    m->min_frequency_scale = 1e-6;
+    m->min_sin_theta = 1e-4;
+    m->min_thetae = 1e-3;
+    m->min_radius = 1e-6;
+}
+
+extern "C" void kp_run_options_default(kp_run_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = KP_PRECISION_FP64;
+    o->split_transport = 1;
+    o->block_size = 0;
+    o->persistent = 1;
+    o->waves_per_cu = 0;
+    o->stream = nullptr;
+}
+
+// ===========================================================================
+// Option layer: the image tool's Options subset on the hot path.
+// ===========================================================================
+struct kp_options {
+    std::string model = "riaf";
+    std::string evpa_0 = "N";
+    std::string camera = "pinhole";
+    std::string coordinate = "cartesian_ks";
+    int nx = 96, ny = 96;
+    double radius = 35.0;
+    double inclination = 1.04719755119659774615;
+    double fov = 0.08, fovy = -1.0, xspan = -1.0, yspan = -1.0;
+    double dsource_pc = -1.0, fovx_dsource = -1.0, fovy_dsource = -1.0;
+    double image_width_x = -1.0, image_width_y = -1.0;
+    double spin = 0.9375;
+    double inner_radius = -1.0;
+    bool inner_radius_explicit = false;
+    double outer_radius = -1.0;
+    double step = 0.025;
+    int max_steps = 4096;
+    int adaptive = 1;
+    int direct_only = 0;
+    double equatorial_h_over_r = 0.0;
+    int equatorial_samples = 8;
+    int faraday_rotation = 1;
+    double adaptive_tolerance = 1e-10;
+    double min_step = 1e-5, max_step = 1.0;
+    double max_radiation_step = 1.0, max_radiation_depth = 1.0, max_absorption_depth = 1.0,
+           max_faraday_depth = 4.0;
+    double freq = 230.0e9;
+    double riaf_r_min = 1.0, riaf_r_max = 100.0, riaf_nth0 = 1.0, riaf_Te0 = 1.0, riaf_disk_h = 0.35;
+    double riaf_pow_nth = -1.1, riaf_pow_T = -0.84, riaf_ne_unit = 5.0e6, riaf_te_unit = 1.0e11;
+    double riaf_mbh_solar = 4.3e6, riaf_keplerian_factor = 1.0, riaf_infall_factor = 0.0;
+    int emission_type = 0;
+    double x_offset = 0.0, y_offset = 0.0;
+    int use_pinhole_pixel_bias = 0;
+    double pinhole_pixel_bias = -0.01;
+    int radiation_substeps = 2;
+    int split_transport = 1;
+};
+
+namespace {
+
+std::string trim(const std::string& s) {
+    size_t b = s.find_first_not_of(" \t\r\n");
+    if (b == std::string::npos) return "";
+    size_t e = s.find_last_not_of(" \t\r\n");
+    return s.substr(b, e - b + 1);
+}
+
+bool parse_double(const std::string& v, double& out) {
+    try {
+        size_t used = 0;
+        out = std::stod(v, &used);
+        return used == v.size();
+    } catch (...) {
+        return false;
+    }
+}
+
+bool parse_int(const std::string& v, int& out) {
+    try {
+        size_t used = 0;
+        out = std::stoi(v, &used);
+        return used == v.size();
+    } catch (...) {
+        return false;
+    }
+}
+
+}  // namespace
+
+extern "C" kp_options* kp_options_create(void) { return new (std::nothrow) kp_options(); }
+
+extern "C" void kp_options_destroy(kp_options* o) { delete o; }
+
+// Key semantics of apply_option (tools/kpolaris_model_image_options.cpp:489-600).
+extern "C" kp_status kp_options_set(kp_options* o, const char* key_c, const char* value_c) {
+    kp_internal::clear_error();
+    if (!o || !key_c || !value_c) return kp_internal::fail(KP_INVALID_ARGUMENT, "null argument");
+    std::string key = trim(key_c);
+    while (!key.empty() && key[0] == '-') key.erase(0, 1);
+    for (char& ch : key)
+        if (ch == '-') ch = '_';
+    const std::string v = trim(value_c);
+    double d = 0.0;
+    int i = 0;
+    auto bad = [&]() { return kp_internal::fail(KP_INVALID_ARGUMENT, "invalid value for " + key + ": " + v); };
+#define KP_DBL(name, field)                 \
+    if (key == name) {                      \
+        if (!parse_double(v, d)) return bad(); \
+        o->field = d;                       \
+        return KP_OK;                       \
+    }
+#define KP_INT(name, field)                 \
+    if (key == name) {                      \
+        if (!parse_int(v, i)) return bad(); \
+        o->field = i;                       \
+        return KP_OK;                       \
+    }
+    if (key == "model") { o->model = v; return KP_OK; }
+    if (key == "evpa_0") {
+        std::string u = v;
+        for (char& ch : u) ch = (char)std::toupper((unsigned char)ch);
+        if (u == "NORTH") u = "N";
+        if (u == "WEST") u = "W";
+        if (u != "N" && u != "W") return bad();
+        o->evpa_0 = u;
+        return KP_OK;
+    }
+    if (key == "camera") { o->camera = v; return KP_OK; }
+    if (key == "coordinate") {
+        std::string u = v;
+        for (char& ch : u) ch = (char)std::tolower((unsigned char)ch);
+        if (u == "bl") u = "boyer_lindquist";
+        o->coordinate = u;
+        return KP_OK;
+    }
+    if (key == "nx") { if (!parse_int(v, i)) return bad(); o->nx = i; return KP_OK; }
+    if (key == "ny") { if (!parse_int(v, i)) return bad(); o->ny = i; return KP_OK; }
+    if (key == "inclination_deg") {
+        if (!parse_double(v, d)) return bad();
+        o->inclination = d * 3.141592653589793238462643383279502884 / 180.0;
+        return KP_OK;
+    }
+    if (key == "inclination_rad") { if (!parse_double(v, d)) return bad(); o->inclination = d; return KP_OK; }
+    if (key == "fov") {
+        if (!parse_double(v, d)) return bad();
+        o->fov = d;
+        o->fovy = o->xspan = o->yspan = o->fovx_dsource = o->fovy_dsource = -1.0;
+        return KP_OK;
+    }
+    if (key == "fovy" || key == "fov_y") {
+        if (!parse_double(v, d)) return bad();
+        o->fovy = d;
+        o->fovy_dsource = -1.0;
+        return KP_OK;
+    }
+    if (key == "xspan") { if (!parse_double(v, d)) return bad(); o->xspan = d; o->fovx_dsource = -1.0; return KP_OK; }
+    if (key == "yspan") { if (!parse_double(v, d)) return bad(); o->yspan = d; o->fovy_dsource = -1.0; return KP_OK; }
+    if (key == "dsource" || key == "dsource_pc") { if (!parse_double(v, d)) return bad(); o->dsource_pc = d; return KP_OK; }
+    if (key == "fovx_dsource" || key == "fovx_muas" || key == "fovx_uas") {
+        if (!parse_double(v, d)) return bad();
+        o->fovx_dsource = d;
+        o->xspan = -1.0;
+        return KP_OK;
+    }
+    if (key == "fovy_dsource" || key == "fovy_muas" || key == "fovy_uas") {
+        if (!parse_double(v, d)) return bad();
+        o->fovy_dsource = d;
+        o->yspan = -1.0;
+        return KP_OK;
+    }
+    if (key == "inner_radius") {
+        if (!parse_double(v, d)) return bad();
+        o->inner_radius = d;
+        o->inner_radius_explicit = true;
+        return KP_OK;
+    }
+    if (key == "max_radiation_depth" || key == "radiation_sampling_depth") {
+        if (!parse_double(v, d)) return bad();
+        o->max_radiation_depth = o->max_absorption_depth = o->max_faraday_depth = d;
+        return KP_OK;
+    }
+    if (key == "faraday_rotation" || key == "direct_only") {
+        if (v != "0" && v != "1") return bad();
+        (key == "faraday_rotation" ? o->faraday_rotation : o->direct_only) = v == "1";
+        return KP_OK;
+    }
+    KP_DBL("radius", radius)
+    KP_DBL("spin", spin)
+    KP_DBL("outer_radius", outer_radius)
+    KP_DBL("step", step)
+    KP_INT("max_steps", max_steps)
+    KP_INT("adaptive", adaptive)
+    KP_DBL("equatorial_h_over_r", equatorial_h_over_r)
+    KP_INT("equatorial_samples", equatorial_samples)
+    KP_DBL("adaptive_tolerance", adaptive_tolerance)
+    KP_DBL("min_step", min_step)
+    KP_DBL("max_step", max_step)
+    KP_DBL("max_radiation_step", max_radiation_step)
+    KP_DBL("radiation_sampling_interval", max_radiation_step)
+    KP_DBL("max_absorption_depth", max_absorption_depth)
+    KP_DBL("absorption_sampling_depth", max_absorption_depth)
+    KP_DBL("max_faraday_depth", max_faraday_depth)
+    KP_DBL("faraday_sampling_depth", max_faraday_depth)
+    KP_DBL("freq", freq)
+    KP_DBL("riaf_r_min", riaf_r_min)
+    KP_DBL("riaf_r_max", riaf_r_max)
+    KP_DBL("riaf_nth0", riaf_nth0)
+    KP_DBL("riaf_Te0", riaf_Te0)
+    KP_DBL("riaf_disk_h", riaf_disk_h)
+    KP_DBL("riaf_pow_nth", riaf_pow_nth)
+    KP_DBL("riaf_pow_T", riaf_pow_T)
+    KP_DBL("riaf_ne_unit", riaf_ne_unit)
+    KP_DBL("riaf_te_unit", riaf_te_unit)
+    KP_DBL("riaf_mbh_solar", riaf_mbh_solar)
+    KP_DBL("riaf_keplerian_factor", riaf_keplerian_factor)
+    KP_DBL("riaf_infall_factor", riaf_infall_factor)
+    KP_INT("emission_type", emission_type)
+    KP_DBL("x_offset", x_offset)
+    KP_DBL("y_offset", y_offset)
+    KP_INT("use_pinhole_pixel_bias", use_pinhole_pixel_bias)
+    KP_DBL("pinhole_pixel_bias", pinhole_pixel_bias)
+    KP_INT("substeps", radiation_substeps)
+    KP_INT("radiation_substeps", radiation_substeps)
+    KP_INT("split_transport", split_transport)
+#undef KP_DBL
+#undef KP_INT
+    if (key == "format" || key == "output" || key == "parameter_output" || key == "timing" ||
+        key == "analysis_mode")
+        return KP_OK;  // output-side keys with no effect on the kernel
+    return kp_internal::fail(KP_UNSUPPORTED, "unsupported option: " + key);
+}
+
+extern "C" kp_status kp_options_load_file(kp_options* o, const char* path) {
+    kp_internal::clear_error();
+    if (!o || !path) return kp_internal::fail(KP_INVALID_ARGUMENT, "null argument");
+    std::ifstream in(path);
+    if (!in) return kp_internal::fail(KP_INVALID_ARGUMENT, std::string("cannot open parameter file ") + path);
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        line = trim(line);
+        if (line.empty()) continue;
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "parameter line without '=': " + line);
+        const kp_status st = kp_options_set(o, line.substr(0, eq).c_str(), line.substr(eq + 1).c_str());
+        if (st != KP_OK) return st;
+    }
+    return KP_OK;
+}
+
+namespace {
+
+// Length unit of the image tool (tools/kpolaris_model_image_options.cpp:119-124);
+// its CGS constants differ from the RIAF model's.
+double tool_length_unit(double mbh) { return 6.6742e-8 * mbh * 1.989e33 / (2.99792458e10 * 2.99792458e10); }
+
+// resolve_camera_extents (tools/kpolaris_model_image_options.cpp:136-180).
+kp_status resolve_extents(kp_options& o) {
+    const double pc_cgs = 3.085678e18, muas_per_rad = 2.06265e11;
+    const bool source_fov = o.fovx_dsource > 0.0 || o.fovy_dsource > 0.0;
+    if (source_fov) {
+        if (o.dsource_pc <= 0.0)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "fovx_dsource/fovy_dsource require dsource in parsec");
+        if (o.fovx_dsource <= 0.0 && o.fovy_dsource > 0.0) o.fovx_dsource = o.fovy_dsource;
+        if (o.fovy_dsource <= 0.0 && o.fovx_dsource > 0.0) o.fovy_dsource = o.fovx_dsource;
+        const double fov_to_m = o.dsource_pc * pc_cgs / tool_length_unit(o.riaf_mbh_solar) / muas_per_rad;
+        o.image_width_x = o.fovx_dsource * fov_to_m;
+        o.image_width_y = o.fovy_dsource * fov_to_m;
+        o.fov = o.image_width_x / o.radius;
+        o.fovy = o.image_width_y / o.radius;
+        if (o.camera == "parallel_plane") {
+            o.xspan = o.image_width_x / 2.0;
+            o.yspan = o.image_width_y / 2.0;
+        } else {
+            o.xspan = o.yspan = -1.0;
+        }
+        return KP_OK;
+    }
+    if (o.camera != "parallel_plane" && o.xspan > 0.0) {
+        o.fov = 2.0 * o.xspan / o.radius;
+        if (o.yspan > 0.0) o.fovy = 2.0 * o.yspan / o.radius;
+    }
+    if (o.fovy <= 0.0) o.fovy = o.fov;
+    if (o.camera == "parallel_plane") {
+        if (o.xspan > 0.0 && o.yspan <= 0.0) o.yspan = o.xspan * double(o.ny) / double(o.nx);
+        o.image_width_x = o.xspan > 0.0 ? 2.0 * o.xspan : o.fov;
+        o.image_width_y = o.yspan > 0.0 ? 2.0 * o.yspan : o.fovy;
+    } else {
+        o.image_width_x = o.fov * o.radius;
+        o.image_width_y = o.fovy * o.radius;
+    }
+    return KP_OK;
+}
+
+}  // namespace
+
+// Camera extents, radii, PassAParams assembly (tools/kpolaris_model_image.cpp:3940-3985),
+// RIAF model fill and radius defaults (:4925-4962), flux scale (:568-588).
+extern "C" kp_status kp_options_resolve_riaf(kp_options* options, kp_pass_a_params* p, kp_riaf_model* m,
+                                             double* flux_scale, int32_t* qu_sign) {
+    kp_internal::clear_error();
+    if (!options || !p || !m) return kp_internal::fail(KP_INVALID_ARGUMENT, "null argument");
+    if (options->model != "riaf") return kp_internal::fail(KP_UNSUPPORTED, "only model=riaf is resolved here");
+    if (options->nx <= 0 || options->ny <= 0)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "nx and ny must be positive");
+    kp_options o = *options;
+    kp_status st = resolve_extents(o);
+    if (st != KP_OK) return st;
+    const double horizon = 1.0 + std::sqrt(std::max(0.0, 1.0 - o.spin * o.spin));
+    if (o.inner_radius <= 0.0) o.inner_radius = horizon * 1.05;
+    if (o.outer_radius <= 0.0) o.outer_radius = o.riaf_r_max * 1.15;
+    kp_pass_a_params_default(p);
+    p->camera.nx = o.nx;
+    p->camera.ny = o.ny;
+    p->camera.radius = o.radius;
+    p->camera.inclination = o.inclination;
+    p->camera.fov = o.fov;
+    p->camera.fov_y = o.fovy > 0.0 ? o.fovy : o.fov;
+    p->camera.x_offset = o.x_offset;
+    p->camera.y_offset = o.y_offset;
+    if (o.camera == "pinhole" && o.use_pinhole_pixel_bias)
+        p->camera.x_offset += o.pinhole_pixel_bias / double(o.nx) * o.fov;
+    p->camera.xspan = o.camera == "parallel_plane" ? o.xspan : -1.0;
+    p->camera.yspan = (o.camera == "parallel_plane" && o.xspan > 0.0)
+                          ? (o.yspan > 0.0 ? o.yspan : o.xspan * double(o.ny) / double(o.nx))
+                          : -1.0;
+    p->camera.model = o.camera == "parallel_plane" ? KP_CAMERA_PARALLEL_PLANE : KP_CAMERA_PINHOLE;
+    if (o.coordinate == "boyer_lindquist")
+        p->coordinate_system = KP_COORD_BOYER_LINDQUIST;
+    else if (o.coordinate == "fmks")
+        p->coordinate_system = KP_COORD_FMKS;
+    else if (o.coordinate == "spherical_ks" || o.coordinate == "ks")
+        p->coordinate_system = KP_COORD_SPHERICAL_KS;
+    else if (o.coordinate == "mks")
+        p->coordinate_system = KP_COORD_MKS;
+    else
+        p->coordinate_system = KP_COORD_CARTESIAN_KS;
+    p->mass = 1.0;
+    p->spin = o.spin;
+    p->inner_radius = o.inner_radius;
+    p->outer_radius = o.outer_radius;
+    p->step = o.step;
+    p->max_steps = o.max_steps;
+    p->adaptive = o.adaptive;
+    p->direct_only = o.direct_only;
+    p->emission_selection.equatorial_h_over_r = o.equatorial_h_over_r;
+    p->emission_selection.equatorial_samples = o.equatorial_samples;
+    p->emission_selection.faraday_rotation = o.faraday_rotation;
+    p->adaptive_tolerance = o.adaptive_tolerance;
+    p->min_step = o.min_step;
+    p->max_step = o.max_step;
+    p->max_radiation_step = o.max_radiation_step;
+    p->max_radiation_depth = o.max_radiation_depth;
+    p->max_absorption_depth = o.max_absorption_depth;
+    p->max_faraday_depth = o.max_faraday_depth;
+    kp_riaf_model_default(m);
+    m->freq_cgs = o.freq;
+    m->r_min = o.riaf_r_min;
+    m->r_max = o.riaf_r_max;
+    m->nth0 = o.riaf_nth0;
+    m->Te0 = o.riaf_Te0;
+    m->disk_h = o.riaf_disk_h;
+    m->pow_nth = o.riaf_pow_nth;
+    m->pow_T = o.riaf_pow_T;
+    m->ne_unit = o.riaf_ne_unit;
+    m->te_unit = o.riaf_te_unit;
+    m->mbh_solar = o.riaf_mbh_solar;
+    m->keplerian_factor = o.riaf_keplerian_factor;
+    m->infall_factor = o.riaf_infall_factor;
+    // The RIAF fit is a build-time choice (Pandya) in the reference tool; a
+    // conflicting runtime request is an error (tools/kpolaris_model_image.cpp:504-516).
+    if (o.emission_type != 0 && o.emission_type != KP_EMISSION_THERMAL_PANDYA)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "emission_type conflicts with the RIAF fit (Pandya)");
+    m->emission_type = KP_EMISSION_THERMAL_PANDYA;
+    if (flux_scale) {
+        const double pc_cgs = 3.085678e18, jy = 1.0e-23;
+        double scale = std::numeric_limits<double>::quiet_NaN();
+        if (o.dsource_pc > 0.0 && o.image_width_x > 0.0 && o.image_width_y > 0.0) {
+            const double l = tool_length_unit(o.riaf_mbh_solar);
+            const double d = o.dsource_pc * pc_cgs;
+            const double px = o.image_width_x * l / double(o.nx);
+            const double py = o.image_width_y * l / double(o.ny);
+            scale = (px * py / (d * d)) / jy;
+        }
+        *flux_scale = scale;
+    }
+    if (qu_sign) *qu_sign = o.evpa_0 == "N" ? -1 : 1;
+    return KP_OK;
+}
+
+// pack_transport_frequency_image (tools/kpolaris_model_image.cpp:2668-2692) and
+// flux_density_jy (:590-595).
+extern "C" kp_status kp_pack_image(const double* image_i, const double* image_q, const double* image_u,
+                                   const double* image_v, const int32_t* reason, int64_t npix,
+                                   double frequency, int32_t qu_sign, double scale, double* i_nu,
+                                   double* q_nu, double* u_nu, double* v_nu, double* flux_jy) {
+    kp_internal::clear_error();
+    if (!image_i || !image_q || !image_u || !image_v || !reason || npix < 0)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "null input array");
+    const double nu3 = frequency * frequency * frequency;
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t p = 0; p < npix; ++p) {
+        const bool ok = reason[p] == KP_REASON_REACHED_CAMERA;
+        const double iv = ok ? image_i[p] : 0.0;
+        const double qv = ok ? double(qu_sign) * image_q[p] : 0.0;
+        const double uv = ok ? double(qu_sign) * image_u[p] : 0.0;
+        const double vv = ok ? image_v[p] : 0.0;
+        if (i_nu) i_nu[p] = iv * nu3;
+        if (q_nu) q_nu[p] = qv * nu3;
+        if (u_nu) u_nu[p] = uv * nu3;
+        if (v_nu) v_nu[p] = vv * nu3;
+        sum[0] += iv;
+        sum[1] += qv;
+        sum[2] += uv;
+        sum[3] += vv;
+    }
+    if (flux_jy) {
+        for (int c = 0; c < 4; ++c)
+            flux_jy[c] = std::isfinite(scale) && scale > 0.0
+                             ? sum[c] * frequency * frequency * frequency * scale
+                             : std::numeric_limits<double>::quiet_NaN();
+    }
+    return KP_OK;
+}

@@ -1,0 +1,957 @@
+// kp_image.hip -- pass A / pass B imaging kernels for gfx950 and the C ABI.
+//
+// Design (one ray per lane, state in VGPRs):
+//  * Each kernel is a per-lane state machine whose "tick" performs exactly one
+//    integration attempt (one step-doubling attempt, 11 RHS, or one RK4 step)
+//    followed by cheap per-lane bookkeeping, so the 64 lanes of a wavefront
+//    execute the same heavy instruction stream every tick even when their
+//    rays are at different points of the reference's accept/reject control.
+//  * Persistent mode: a grid sized to the resident wave count; a lane whose
+//    ray terminates refills from a global pixel queue with one wave-level
+//    atomic (ballot + popcount), so long photon-ring rays do not hold idle
+//    lanes hostage.  This is the wavefront compaction of terminated rays.
+//  * Pass A writes endpoints as structure-of-arrays [16][npix] (coalesced);
+//    pass B reads them back, like the reference's split driver
+//    (src/image/split_driver.hpp:33-104).
+#include "kp_device.h"
+#include "kp_riaf.h"
+#include "kp_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace kpd;
+
+namespace {
+
+enum : int {
+    R_NONE = 0,
+    R_CAMERA = 1,
+    R_INNER = 2,
+    R_ESCAPED = 3,
+    R_MAX_STEPS = 4,
+    R_UNDERFLOW = 7,
+    R_TIME = 8,
+};
+
+enum Counter : int {
+    C_RHS_A = 0,
+    C_RHS_B = 1,
+    C_COEFF = 2,
+    C_TRANSFER = 3,
+    C_STEPS_A = 4,
+    C_STEPS_B = 5,
+    C_GUARD = 6,
+    C_COUNT = 8
+};
+
+struct Params {
+    // camera (CameraParams + the per-image tetrad built on the host)
+    int32_t nx, ny, npix, adaptive;
+    int32_t max_steps, persistent;
+    double radius, inclination, x_extent, y_extent, x_offset, y_offset;
+    double tetrad[16];
+    BLGcov<double> gcam;
+    // integrator (PassAParams scalars)
+    double mass, spin, inner, outer, step, tol, min_step, max_step;
+    double max_rs, max_rd, max_ad, max_fd;
+    RiafDev<double> riaf;
+    // workspace
+    double* ep;
+    int32_t* ep_valid;
+    int32_t* ep_steps;
+    int32_t* ep_reason;
+    uint32_t* queue;
+    unsigned long long* counters;
+    // outputs
+    kp_image_data out;
+};
+
+// ---------------------------------------------------------------------------
+// Camera ray for one pixel from the host-built ipole tetrad
+// (src/camera/camera.hpp:96-106, 484-513).
+// ---------------------------------------------------------------------------
+KP_DEV void camera_ray(const Params& P, int pixel, St<double>& s) {
+    const int i = pixel % P.nx;
+    const int j = pixel / P.nx;
+    const double sx = ((double(i) + 0.5) / double(P.nx) - 0.5) * P.x_extent + P.x_offset;
+    const double sy = ((double(j) + 0.5) / double(P.ny) - 0.5) * P.y_extent + P.y_offset;
+    const double sn = sqrt(1.0 + sx * sx + sy * sy);
+    const double kt1 = sx / sn, kt2 = sy / sn, kt3 = 1.0 / sn;
+    const double* e0 = P.tetrad;
+    const double* e1 = P.tetrad + 4;
+    const double* e2 = P.tetrad + 8;
+    const double* e3 = P.tetrad + 12;
+    double sd[4], a[4], b[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) sd[m] = e1[m] * kt1 + e2[m] * kt2 + e3[m] * kt3;
+    s.v[0] = 0.0;
+    s.v[1] = P.radius;
+    s.v[2] = P.inclination;
+    s.v[3] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        s.v[4 + m] = e0[m] + sd[m];
+        a[m] = e1[m] - sd[m] * kt1;
+        b[m] = e2[m] - sd[m] * kt2;
+    }
+    const double na = gdot(P.gcam, a, a);
+    if (na > 0.0) {
+        const double f = 1.0 / sqrt(na);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[m] *= f;
+    }
+    const double den = gdot(P.gcam, a, a);
+    if (absv(den) > 1e-300) {
+        const double f = gdot(P.gcam, b, a) / den;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) b[m] = b[m] - a[m] * f;
+    }
+    const double nb = gdot(P.gcam, b, b);
+    if (nb > 0.0) {
+        const double f = 1.0 / sqrt(nb);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) b[m] *= f;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        s.v[8 + m] = a[m];
+        s.v[12 + m] = b[m];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-aggregated queue fetch: one atomic per wavefront.
+// ---------------------------------------------------------------------------
+KP_DEV bool fetch_pixel(uint32_t* counter, bool need, uint32_t npix, int& pixel) {
+    const unsigned long long mask = __ballot(need);
+    if (mask == 0ull) return false;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!need) return false;
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    const uint32_t p = base + rank;
+    if (p >= npix) return false;
+    pixel = (int)p;
+    return true;
+}
+
+// Wave-level sum of per-lane counters, one atomic per counter per wave.
+KP_DEV void flush_counters(const Params& P, int rhs_slot, uint32_t rhs, uint32_t coeff,
+                           uint32_t transfer, uint32_t steps_a, uint32_t steps_b) {
+    unsigned long long v[5] = {rhs, coeff, transfer, steps_a, steps_b};
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        unsigned long long x = v[c];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        v[c] = x;
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&P.counters[rhs_slot], v[0]);
+        atomicAdd(&P.counters[C_COEFF], v[1]);
+        atomicAdd(&P.counters[C_TRANSFER], v[2]);
+        atomicAdd(&P.counters[C_STEPS_A], v[3]);
+        atomicAdd(&P.counters[C_STEPS_B], v[4]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass A: trace_pass_a_segment_endpoint_pixel_metric (src/geodesic/pass_b.hpp:337-486)
+// with the BL metric-domain hooks reduced (src/geodesic/metric_domain.hpp:159-223).
+// ---------------------------------------------------------------------------
+struct LaneA {
+    St<double> s;
+    double h_current, h_try, call_min;
+    int n, steps, steps_inside, entered, att, in_call, pixel, reason;
+};
+
+KP_DEV void lane_a_init(const Params& P, int pixel, LaneA& L) {
+    camera_ray(P, pixel, L.s);
+    L.h_current = -absv(P.step);
+    L.h_try = 0.0;
+    L.call_min = 0.0;
+    L.n = 0;
+    L.steps = 0;
+    L.steps_inside = 0;
+    L.entered = 0;
+    L.att = 0;
+    L.in_call = 0;
+    L.pixel = pixel;
+    L.reason = R_MAX_STEPS;
+}
+
+// locate_metric_surface_crossing for a radial surface (src/geodesic/pass_a_core.hpp:94-197).
+KP_DEV void locate_crossing(const Geo<double>& G, const St<double>& first, const St<double>& second,
+                            double full_h, double surface, int composed, St<double>& out) {
+    st_copy(out, first);
+    double lo = 0.0, hi = 1.0;
+    double flo = first.v[1] - surface;
+    if (flo == 0.0 || !st_finite(second)) return;
+    St<double> t;
+    auto path = [&](double h, St<double>& o) {
+        if (composed) {
+            St<double> half;
+            rk4(G, first, h * 0.5, half);
+            rk4(G, half, h * 0.5, o);
+        } else {
+            rk4(G, first, h, o);
+        }
+    };
+    path(full_h, t);
+    if (!st_finite(t)) return;
+    double fhi = t.v[1] - surface;
+    if (!(fhi == fhi) || absv(fhi) >= 1e300) return;
+    if (fhi == 0.0) {
+        st_copy(out, t);
+        return;
+    }
+    if ((flo > 0.0) == (fhi > 0.0)) return;
+    int use_secant = 1;
+    for (int iter = 0; iter < 12; ++iter) {
+        const double width = hi - lo;
+        double trial = (lo + hi) * 0.5;
+        const double denom = flo - fhi;
+        if (use_secant && absv(denom) > 1e-300) {
+            const double secant = lo + width * flo / denom;
+            const double guard = width * 0.1;
+            if (secant > lo + guard && secant < hi - guard) trial = secant;
+        }
+        path(full_h * trial, t);
+        if (!st_finite(t)) {
+            hi = trial;
+            use_secant = 0;
+            continue;
+        }
+        const double ft = t.v[1] - surface;
+        if (!(ft == ft) || absv(ft) >= 1e300) {
+            hi = trial;
+            use_secant = 0;
+            continue;
+        }
+        if (ft == 0.0) {
+            st_copy(out, t);
+            return;
+        }
+        if ((flo > 0.0) == (ft > 0.0)) {
+            lo = trial;
+            flo = ft;
+            st_copy(out, t);
+        } else {
+            hi = trial;
+            fhi = ft;
+        }
+    }
+}
+
+// One tick of pass A; returns true when the ray has terminated (L.reason set).
+KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_t& steps_acc) {
+    const double eff_min = absv(P.min_step);
+    if (!L.in_call) {
+        if (L.n >= P.max_steps) {
+            L.reason = R_MAX_STEPS;
+            return true;
+        }
+        const double r = L.s.v[1];
+        if (!L.entered) {
+            if (r <= P.outer) {
+                L.entered = 1;
+                L.steps_inside = 0;
+            }
+        } else {
+            if (r - P.inner <= 0.0) {
+                L.reason = R_INNER;
+                return true;
+            }
+            if (L.steps_inside > 0 && r >= P.outer) {
+                L.reason = R_ESCAPED;
+                return true;
+            }
+        }
+        if (P.adaptive) {
+            const double req = L.h_current;
+            L.call_min = eff_min;
+            if (absv(req) < L.call_min) L.call_min = absv(req);
+            double h = minv(maxv(absv(req), L.call_min), P.max_step);
+            L.h_try = req < 0.0 ? -h : h;
+            L.att = 0;
+            L.in_call = 1;
+        }
+    }
+    St<double> next;
+    double used_h;
+    if (P.adaptive) {
+        // adaptive_rk4_step (src/geodesic/rk4.hpp:109-161), one attempt per tick
+        St<double> half;
+        const double err = doubling_attempt(G, L.s, L.h_try, half, next);
+        const double tol = maxv(P.tol, 8.0 * eps<double>());
+        const double req_sign = L.h_current < 0.0 ? -1.0 : 1.0;
+        if (err <= tol) {
+            double factor = 2.0;
+            if (err > 0.0) factor = minv(2.0, maxv(0.2, 0.9 * pow(tol / err, 0.2)));
+            const double nh = minv(P.max_step, maxv(L.call_min, absv(L.h_try) * factor));
+            used_h = L.h_try;
+            L.h_current = req_sign * nh;
+            L.in_call = 0;
+        } else {
+            bool fail = false;
+            if (absv(L.h_try) <= L.call_min * 1.0001) {
+                fail = true;
+            } else {
+                const double f = minv(0.8, maxv(0.1, 0.9 * pow(tol / maxv(err, 1e-300), 0.25)));
+                L.h_try *= f;
+                if (++L.att >= 10) fail = true;
+            }
+            if (fail) {
+                L.h_current = req_sign * maxv(L.call_min, absv(L.h_try) * 0.5);
+                L.in_call = 0;
+                L.n += 1;
+                if (absv(L.h_current) <= eff_min * 1.0001) {
+                    L.reason = R_UNDERFLOW;
+                    return true;
+                }
+            }
+            return false;
+        }
+    } else {
+        used_h = L.h_current;
+        rk4(G, L.s, used_h, next);
+    }
+    const double old_r = L.s.v[1];
+    const double next_r = next.v[1];
+    const double old_inner = old_r - P.inner;
+    const double next_inner = next_r - P.inner;
+    if (!L.entered && next_r <= P.outer) {
+        L.entered = 1;
+        L.steps_inside = 0;
+    } else if (!L.entered && next_r >= old_r) {
+        st_copy(L.s, next);
+        L.reason = R_ESCAPED;
+        L.steps += 1;
+        steps_acc += 1;
+        return true;
+    }
+    int crossing = R_NONE;
+    if (L.entered && old_inner > 0.0 && next_inner <= 0.0)
+        crossing = R_INNER;
+    else if (L.entered && L.steps_inside > 0 && old_r < P.outer && next_r >= P.outer)
+        crossing = R_ESCAPED;
+    if (crossing != R_NONE) {
+        St<double> ev;
+        locate_crossing(G, L.s, next, used_h, crossing == R_INNER ? P.inner : P.outer,
+                        P.adaptive, ev);
+        st_copy(L.s, ev);
+        L.reason = crossing;
+        L.steps += 1;
+        steps_acc += 1;
+        return true;
+    }
+    st_copy(L.s, next);
+    L.steps += 1;
+    steps_acc += 1;
+    if (L.entered) L.steps_inside += 1;
+    L.n += 1;
+    return false;
+}
+
+KP_DEV void lane_a_store(const Params& P, const LaneA& L) {
+    const size_t np = (size_t)P.npix;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = L.s.v[c];
+    const int valid = L.entered && (L.reason == R_INNER || L.reason == R_ESCAPED);
+    P.ep_valid[L.pixel] = valid;
+    P.ep_steps[L.pixel] = L.steps;
+    P.ep_reason[L.pixel] = L.reason;
+}
+
+template <bool PERSISTENT>
+__global__ __launch_bounds__(256) void pass_a_kernel(Params P) {
+    uint32_t rhs_count = 0, steps_a = 0;
+    Geo<double> G{P.mass, P.spin, &rhs_count};
+    LaneA L;
+    bool has = false;
+    if (PERSISTENT) {
+        bool exhausted = false;
+        while (true) {
+            int pix = 0;
+            const bool need = !has && !exhausted;
+            if (__ballot(need)) {
+                if (fetch_pixel(&P.queue[0], need, (uint32_t)P.npix, pix)) {
+                    lane_a_init(P, pix, L);
+                    has = true;
+                } else if (need) {
+                    exhausted = true;
+                }
+            }
+            if (!__ballot(has)) break;
+            if (has && pass_a_tick(P, G, L, steps_a)) {
+                lane_a_store(P, L);
+                has = false;
+            }
+        }
+    } else {
+        const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix < P.npix) {
+            lane_a_init(P, pix, L);
+            while (!pass_a_tick(P, G, L, steps_a)) {
+            }
+            lane_a_store(P, L);
+        }
+    }
+    flush_counters(P, C_RHS_A, rhs_count, 0, 0, steps_a, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Pass B: trace_pass_b_segment_model_endpoint_pixel_metric (src/geodesic/pass_b.hpp:515-722).
+// ---------------------------------------------------------------------------
+enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2 };
+
+struct LaneB {
+    St<double> s;
+    Stokes<double> st;
+    double h_current, cap, h, h_try, call_min, call_max;
+    int steps, reason, att, phase, crosses, reached, pixel, pass_a_steps;
+};
+
+KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
+    St<double> cam;
+    camera_ray(P, L.pixel, cam);
+    double sn, cs;
+    sincos(L.s.v[2], &sn, &cs);
+    const BLGcov<double> g = bl_gcov(P.mass, P.spin, L.s.v[1], sn, cs);
+    double r11 = 1.0, r12 = 0.0, r21 = 0.0, r22 = 1.0;
+    Stokes<double> o = {0.0, 0.0, 0.0, 0.0};
+    int reason = L.reason;
+    if (valid) {
+        // screen_overlap (pass_b.hpp:87-98) and transform_to_observer_basis
+        // (src/frame/stokes_rotation.hpp:39-57)
+        r11 = gdot(g, cam.v + 8, L.s.v + 8);
+        r12 = gdot(g, cam.v + 8, L.s.v + 12);
+        r21 = gdot(g, cam.v + 12, L.s.v + 8);
+        r22 = gdot(g, cam.v + 12, L.s.v + 12);
+        o.I = L.st.I;
+        o.Q = 0.5 * L.st.Q * (r11 * r11 - r12 * r12 - r21 * r21 + r22 * r22) + L.st.U * (r11 * r12 - r21 * r22);
+        o.U = L.st.Q * (r11 * r21 - r12 * r22) + L.st.U * (r11 * r22 + r12 * r21);
+        o.V = (r11 * r22 - r12 * r21) * L.st.V;
+        if (L.reached)
+            reason = R_CAMERA;
+        else if (reason != R_UNDERFLOW && reason != R_TIME)
+            reason = R_MAX_STEPS;
+    }
+    const kp_image_data& O = P.out;
+    const int p = L.pixel;
+    if (O.image_i) O.image_i[p] = o.I;
+    if (O.image_q) O.image_q[p] = o.Q;
+    if (O.image_u) O.image_u[p] = o.U;
+    if (O.image_v) O.image_v[p] = o.V;
+    if (O.closure_x)
+        O.closure_x[p] = sqrt(sq(L.s.v[1] - cam.v[1]) + sq(L.s.v[2] - cam.v[2]) + sq(L.s.v[3] - cam.v[3]));
+    if (O.closure_k) {
+        double ck = absv(L.s.v[4] - cam.v[4]);
+        for (int m = 1; m < 4; ++m) ck = maxv(ck, absv(L.s.v[4 + m] - cam.v[4 + m]));
+        O.closure_k[p] = ck;
+    }
+    if (O.final_null) O.final_null[p] = gdot(g, L.s.v + 4, L.s.v + 4);
+    if (O.frame_error) O.frame_error[p] = frame_error(g, L.s);
+    if (O.det_r) O.det_r[p] = r11 * r22 - r12 * r21;
+    if (O.overlap_r11) O.overlap_r11[p] = r11;
+    if (O.overlap_r12) O.overlap_r12[p] = r12;
+    if (O.overlap_r21) O.overlap_r21[p] = r21;
+    if (O.overlap_r22) O.overlap_r22[p] = r22;
+    if (O.basis_identity_error)
+        O.basis_identity_error[p] = maxv(maxv(absv(r11 - 1.0), absv(r22 - 1.0)), maxv(absv(r12), absv(r21)));
+    if (O.basis_rotation_angle) O.basis_rotation_angle[p] = atan2(r21 - r12, r11 + r22);
+    if (O.pass_a_steps) O.pass_a_steps[p] = L.pass_a_steps;
+    if (O.steps) O.steps[p] = valid ? L.steps : 0;
+    if (O.reason) O.reason[p] = reason;
+}
+
+// Loads a pass A endpoint; returns false (after writing the pixel's outputs)
+// when the endpoint is not a valid radiating segment.
+KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
+    const size_t np = (size_t)P.npix;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) L.s.v[c] = P.ep[(size_t)c * np + pixel];
+    L.pixel = pixel;
+    L.pass_a_steps = P.ep_steps[pixel];
+    L.reason = P.ep_reason[pixel];
+    L.st = {0.0, 0.0, 0.0, 0.0};
+    L.h_current = P.step;
+    L.cap = P.max_rs > 0.0 ? P.max_rs : P.max_step;
+    L.h = 0.0;
+    L.h_try = 0.0;
+    L.call_min = 0.0;
+    L.call_max = 0.0;
+    L.steps = 0;
+    L.att = 0;
+    L.phase = PH_IDLE;
+    L.crosses = 0;
+    L.reached = 0;
+    if (!P.ep_valid[pixel]) {
+        lane_b_finish(P, L, 0);
+        return false;
+    }
+    return true;
+}
+
+KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
+    L.call_min = cmin;
+    L.call_max = cmax;
+    L.h_try = minv(maxv(absv(req), cmin), cmax);
+    L.att = 0;
+}
+
+// One tick of pass B; returns true when the ray has terminated.
+KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_t& coeff_acc,
+                        uint32_t& transfer_acc, uint32_t& steps_acc) {
+    const double eff_min = absv(P.min_step);
+    if (L.phase == PH_IDLE) {
+        if (L.reached || L.steps >= P.max_steps) return true;
+        double h = L.h_current;
+        if (L.s.v[1] <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
+        L.h = h;
+        if (P.adaptive) start_call(L, h, eff_min, P.max_step);
+        L.phase = PH_MAIN;
+    }
+    St<double> next, sample;
+    if (P.adaptive) {
+        const double err = doubling_attempt(G, L.s, L.h_try, sample, next);
+        const double tol = maxv(P.tol, 8.0 * eps<double>());
+        if (err <= tol) {
+            double factor = 2.0;
+            if (err > 0.0) factor = minv(2.0, maxv(0.2, 0.9 * pow(tol / err, 0.2)));
+            L.h = absv(L.h_try);
+            if (L.phase == PH_MAIN) L.h_current = minv(L.call_max, maxv(L.call_min, absv(L.h_try) * factor));
+        } else {
+            bool fail = false;
+            if (absv(L.h_try) <= L.call_min * 1.0001) {
+                fail = true;
+            } else {
+                L.h_try *= minv(0.8, maxv(0.1, 0.9 * pow(tol / maxv(err, 1e-300), 0.25)));
+                if (++L.att >= 10) fail = true;
+            }
+            if (fail) {
+                L.h_current = maxv(L.call_min, absv(L.h_try) * 0.5);
+                const bool main = L.phase == PH_MAIN;
+                L.phase = PH_IDLE;
+                if (main && L.h_current <= eff_min * 1.0001) {
+                    L.reason = R_UNDERFLOW;
+                    return true;
+                }
+            }
+            return false;
+        }
+    } else {
+        rk4(G, L.s, L.h, next);
+        st_copy(sample, next);
+    }
+    if (L.phase == PH_MAIN) {
+        // camera_surface_value/crossed/crossing_fraction for the pinhole sphere
+        // (pass_b.hpp:54-85, 605-637)
+        const double s0 = L.s.v[1] - P.radius;
+        const double s1 = next.v[1] - P.radius;
+        const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+        L.crosses = crosses;
+        if (crosses) {
+            const double denom = s0 - s1;
+            const double frac = absv(denom) <= 1e-300 ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
+            const double h_cross = L.h * frac;
+            if (absv(h_cross) <= 1e-14) {
+                L.h = 0.0;
+                st_copy(next, L.s);
+                st_copy(sample, L.s);
+            } else if (frac < 0.999999999999) {
+                L.h = h_cross;
+                if (P.adaptive) {
+                    const double cmin = minv(eff_min, absv(h_cross));
+                    start_call(L, h_cross, cmin, maxv(cmin, absv(h_cross)));
+                    L.phase = PH_CROSS;
+                    return false;
+                }
+                rk4(G, L.s, L.h, next);
+                st_copy(sample, next);
+            }
+        }
+    }
+    L.phase = PH_IDLE;
+    const double h = L.h;
+    const double old_r = L.s.v[1], sample_r = sample.v[1], next_r = next.v[1];
+    Coeffs<double> c;
+    coeffs_zero(c);
+    if (sample_r - P.inner > 0.0 && sample_r < P.outer) {
+        riaf_coefficients(P.riaf, sample, c);
+        coeff_acc += 1;
+    }
+    const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
+                            ? P.max_rs : 0.0;
+    const double dls = P.riaf.dlambda_scale;
+    const int required = substep_count(c, h, dls, lmrs, P.max_rd, P.max_ad, P.max_fd);
+    if (required > 1 && h > eff_min * 1.0001) {
+        L.h_current = maxv(eff_min, h / double(required));
+        L.cap = L.h_current;
+        return false;
+    }
+    if (lmrs > 0.0 || P.max_rd > 0.0 || P.max_ad > 0.0 || P.max_fd > 0.0) {
+        const int pred = substep_count(c, L.h_current, dls, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        L.cap = pred > 1 ? maxv(eff_min, L.h_current / double(pred)) : P.max_step;
+    } else {
+        L.cap = P.max_step;
+    }
+    if (h > 0.0) {
+        strang_step(L.st, c, absv(h) * dls);
+        transfer_acc += 1;
+    }
+    st_copy(L.s, next);
+    L.steps += 1;
+    steps_acc += 1;
+    if (L.crosses) L.reached = 1;
+    return false;
+}
+
+template <bool PERSISTENT>
+__global__ __launch_bounds__(256) void pass_b_kernel(Params P) {
+    uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0;
+    Geo<double> G{P.mass, P.spin, &rhs_count};
+    LaneB L;
+    bool has = false;
+    if (PERSISTENT) {
+        bool exhausted = false;
+        while (true) {
+            // refill until every lane holds a radiating ray or the queue is dry
+            while (true) {
+                const bool need = !has && !exhausted;
+                if (!__ballot(need)) break;
+                int pix = 0;
+                if (fetch_pixel(&P.queue[1], need, (uint32_t)P.npix, pix)) {
+                    has = lane_b_init(P, pix, L);
+                } else if (need) {
+                    exhausted = true;
+                }
+            }
+            if (!__ballot(has)) break;
+            if (has && pass_b_tick(P, G, L, coeff, transfer, steps_b)) {
+                lane_b_finish(P, L, 1);
+                has = false;
+            }
+        }
+    } else {
+        const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix < P.npix && lane_b_init(P, pix, L)) {
+            while (!pass_b_tick(P, G, L, coeff, transfer, steps_b)) {
+            }
+            lane_b_finish(P, L, 1);
+        }
+    }
+    flush_counters(P, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b);
+}
+
+__global__ void zero_kernel(uint32_t* q, unsigned long long* c) {
+    const int t = threadIdx.x;
+    if (t < 2) q[t] = 0u;
+    if (t < C_COUNT) c[t] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct DeviceWorkspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+std::mutex g_ws_mutex;
+std::vector<DeviceWorkspace> g_ws;
+
+kp_status workspace_for(int device, size_t bytes, void** out) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if ((int)g_ws.size() <= device) g_ws.resize(device + 1);
+    DeviceWorkspace& w = g_ws[device];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            if (hipDeviceSynchronize() != hipSuccess) return kp_internal::fail(KP_HIP_ERROR, "hipDeviceSynchronize failed");
+            (void)hipFree(w.ptr);
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+            return kp_internal::fail(KP_OUT_OF_MEMORY, "hipMalloc of the image workspace failed");
+        }
+        w.bytes = bytes;
+    }
+    *out = w.ptr;
+    return KP_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+#define KP_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return kp_internal::fail(KP_HIP_ERROR, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Occupancy {
+    int cus = 0;
+    int blocks_a = 0, blocks_b = 0;
+};
+
+std::mutex g_occ_mutex;
+std::vector<Occupancy> g_occ;
+
+kp_status occupancy_for(int device, int block, Occupancy& occ) {
+    std::lock_guard<std::mutex> lock(g_occ_mutex);
+    if ((int)g_occ.size() <= device) g_occ.resize(device + 1);
+    Occupancy& o = g_occ[device];
+    if (o.cus == 0) {
+        hipDeviceProp_t prop;
+        KP_HIP(hipGetDeviceProperties(&prop, device));
+        o.cus = prop.multiProcessorCount;
+        KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_kernel<true>, block, 0));
+        KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_kernel<true>, block, 0));
+        o.blocks_a = std::max(o.blocks_a, 1);
+        o.blocks_b = std::max(o.blocks_b, 1);
+    }
+    occ = o;
+    return KP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Parameter preparation (host, double): tetrad, extents, model constants.
+// ---------------------------------------------------------------------------
+static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Params& P) {
+    if (!p || !m) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
+    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST)
+        return kp_internal::fail(KP_UNSUPPORTED, "the RIAF image is built only in Boyer-Lindquist coordinates");
+    if (p->camera.model != KP_CAMERA_PINHOLE)
+        return kp_internal::fail(KP_UNSUPPORTED, "only the pinhole camera is defined in Boyer-Lindquist coordinates");
+    if (p->direct_only)
+        return kp_internal::fail(KP_UNSUPPORTED, "direct_only vertical-turn truncation is not implemented");
+    const double hr = p->emission_selection.equatorial_h_over_r;
+    if (hr > 0.0 && hr < 1.0)
+        return kp_internal::fail(KP_UNSUPPORTED, "equatorial emission wedges are not implemented");
+    if (m->emission_type != KP_EMISSION_THERMAL_PANDYA && m->emission_type != KP_EMISSION_THERMAL_DEXTER)
+        return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
+    if (p->camera.nx <= 0 || p->camera.ny <= 0)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "camera nx and ny must be positive");
+    const long long npix = (long long)p->camera.nx * (long long)p->camera.ny;
+    if (npix > (1ll << 30)) return kp_internal::fail(KP_INVALID_ARGUMENT, "image too large");
+    if (p->max_steps < 0) return kp_internal::fail(KP_INVALID_ARGUMENT, "max_steps must be non-negative");
+    std::memset(&P, 0, sizeof(P));
+    P.nx = p->camera.nx;
+    P.ny = p->camera.ny;
+    P.npix = (int32_t)npix;
+    P.adaptive = p->adaptive ? 1 : 0;
+    P.max_steps = p->max_steps;
+    P.radius = p->camera.radius;
+    P.inclination = p->camera.inclination;
+    const kp_camera_params& c = p->camera;
+    P.x_extent = c.xspan > 0.0 ? 2.0 * c.xspan : c.fov;
+    P.y_extent = c.yspan > 0.0 ? 2.0 * c.yspan
+               : (c.xspan > 0.0 ? 2.0 * c.xspan * double(c.ny) / double(c.nx) : (c.fov_y > 0.0 ? c.fov_y : c.fov));
+    P.x_offset = c.x_offset;
+    P.y_offset = c.y_offset;
+    double g5[5];
+    kp_internal::bl_camera_tetrad(p->mass, p->spin, c.radius, c.inclination, P.tetrad, g5);
+    P.gcam.g00 = g5[0];
+    P.gcam.g03 = g5[1];
+    P.gcam.g11 = g5[2];
+    P.gcam.g22 = g5[3];
+    P.gcam.g33 = g5[4];
+    P.mass = p->mass;
+    P.spin = p->spin;
+    P.inner = p->inner_radius;
+    P.outer = p->outer_radius;
+    P.step = p->step;
+    P.tol = p->adaptive_tolerance;
+    P.min_step = p->min_step;
+    P.max_step = p->max_step;
+    P.max_rs = p->max_radiation_step;
+    P.max_rd = p->max_radiation_depth;
+    P.max_ad = p->max_absorption_depth;
+    P.max_fd = p->max_faraday_depth;
+    kp_internal::riaf_constants(*p, *m, 0, 0.0, P.riaf);
+    return KP_OK;
+}
+
+extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                          const kp_run_options* options, kp_image_data* out,
+                                          kp_image_stats* stats, int perturbation, double log_scale) {
+    kp_internal::clear_error();
+    if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
+    kp_run_options opt;
+    kp_run_options_default(&opt);
+    if (options) opt = *options;
+    if (opt.precision != KP_PRECISION_FP64)
+        return kp_internal::fail(KP_UNSUPPORTED, "mixed precision is not built in this library version");
+    Params P;
+    kp_status st = prepare(params, model, P);
+    if (st != KP_OK) return st;
+    if (perturbation) kp_internal::riaf_constants(*params, *model, perturbation, log_scale, P.riaf);
+    P.out = *out;
+    P.persistent = opt.persistent ? 1 : 0;
+    int device = 0;
+    KP_HIP(hipGetDevice(&device));
+    const size_t np = (size_t)P.npix;
+    const size_t off_ep = 0;
+    const size_t off_valid = align_up(off_ep + 16 * np * sizeof(double));
+    const size_t off_steps = align_up(off_valid + np * sizeof(int32_t));
+    const size_t off_reason = align_up(off_steps + np * sizeof(int32_t));
+    const size_t off_queue = align_up(off_reason + np * sizeof(int32_t));
+    const size_t off_counters = align_up(off_queue + 2 * sizeof(uint32_t));
+    const size_t total = align_up(off_counters + C_COUNT * sizeof(unsigned long long));
+    void* ws = nullptr;
+    st = workspace_for(device, total, &ws);
+    if (st != KP_OK) return st;
+    char* base = static_cast<char*>(ws);
+    P.ep = reinterpret_cast<double*>(base + off_ep);
+    P.ep_valid = reinterpret_cast<int32_t*>(base + off_valid);
+    P.ep_steps = reinterpret_cast<int32_t*>(base + off_steps);
+    P.ep_reason = reinterpret_cast<int32_t*>(base + off_reason);
+    P.queue = reinterpret_cast<uint32_t*>(base + off_queue);
+    P.counters = reinterpret_cast<unsigned long long*>(base + off_counters);
+
+    hipStream_t stream = static_cast<hipStream_t>(opt.stream);
+    const int block = opt.block_size > 0 ? opt.block_size : 256;
+    if (block % 64 != 0 || block > 256)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
+    Occupancy occ;
+    st = occupancy_for(device, 256, occ);
+    if (st != KP_OK) return st;
+    const int nblocks_flat = (int)((np + block - 1) / block);
+    int grid_a = nblocks_flat, grid_b = nblocks_flat;
+    if (P.persistent) {
+        const int waves_per_block = block / 64;
+        const int per_cu_a = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block)
+                                                  : occ.blocks_a * (256 / block);
+        const int per_cu_b = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block)
+                                                  : occ.blocks_b * (256 / block);
+        grid_a = std::min(nblocks_flat, occ.cus * per_cu_a);
+        grid_b = std::min(nblocks_flat, occ.cus * per_cu_b);
+        grid_a = std::max(grid_a, 1);
+        grid_b = std::max(grid_b, 1);
+    }
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (stats) {
+        for (int i = 0; i < 3; ++i) KP_HIP(hipEventCreate(&ev[i]));
+    }
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, P.queue, P.counters);
+    KP_HIP(hipGetLastError());
+    if (stats) KP_HIP(hipEventRecord(ev[0], stream));
+    if (P.npix > 0) {
+        if (P.persistent)
+            hipLaunchKernelGGL(pass_a_kernel<true>, dim3(grid_a), dim3(block), 0, stream, P);
+        else
+            hipLaunchKernelGGL(pass_a_kernel<false>, dim3(grid_a), dim3(block), 0, stream, P);
+        KP_HIP(hipGetLastError());
+    }
+    if (stats) KP_HIP(hipEventRecord(ev[1], stream));
+    if (P.npix > 0) {
+        if (P.persistent)
+            hipLaunchKernelGGL(pass_b_kernel<true>, dim3(grid_b), dim3(block), 0, stream, P);
+        else
+            hipLaunchKernelGGL(pass_b_kernel<false>, dim3(grid_b), dim3(block), 0, stream, P);
+        KP_HIP(hipGetLastError());
+    }
+    if (stats) {
+        KP_HIP(hipEventRecord(ev[2], stream));
+        KP_HIP(hipEventSynchronize(ev[2]));
+        float a_ms = 0.f, b_ms = 0.f;
+        KP_HIP(hipEventElapsedTime(&a_ms, ev[0], ev[1]));
+        KP_HIP(hipEventElapsedTime(&b_ms, ev[1], ev[2]));
+        unsigned long long cnt[C_COUNT];
+        KP_HIP(hipMemcpy(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->pass_a_ms = a_ms;
+        stats->pass_b_ms = b_ms;
+        stats->kernel_ms = double(a_ms) + double(b_ms);
+        stats->rays = P.npix;
+        stats->rhs_evals = (int64_t)(cnt[C_RHS_A] + cnt[C_RHS_B]);
+        stats->pass_a_rhs_evals = (int64_t)cnt[C_RHS_A];
+        stats->coefficient_evals = (int64_t)cnt[C_COEFF];
+        stats->transfer_steps = (int64_t)cnt[C_TRANSFER];
+        stats->pass_a_steps = (int64_t)cnt[C_STEPS_A];
+        stats->pass_b_steps = (int64_t)cnt[C_STEPS_B];
+        stats->fp64_guard_steps = (int64_t)cnt[C_GUARD];
+        for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
+    }
+    return KP_OK;
+}
+
+extern "C" int32_t kp_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" kp_status kp_run_riaf_image(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                       int32_t radiation_substeps, const kp_run_options* options,
+                                       kp_image_data* out, kp_image_stats* stats) {
+    (void)radiation_substeps;
+    return kp_internal_run_riaf(params, model, options, out, stats, 0, 0.0);
+}
+
+extern "C" kp_status kp_run_riaf_image_host(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                            int32_t radiation_substeps, const kp_run_options* options,
+                                            kp_image_data* out, kp_image_stats* stats) {
+    (void)radiation_substeps;
+    kp_internal::clear_error();
+    if (!params || !out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null argument");
+    const size_t np = (size_t)std::max(0, params->camera.nx) * (size_t)std::max(0, params->camera.ny);
+    double* const* host_real[15] = {&out->image_i, &out->image_q, &out->image_u, &out->image_v,
+                                    &out->closure_x, &out->closure_k, &out->final_null, &out->frame_error,
+                                    &out->det_r, &out->overlap_r11, &out->overlap_r12, &out->overlap_r21,
+                                    &out->overlap_r22, &out->basis_identity_error, &out->basis_rotation_angle};
+    int32_t* const* host_int[3] = {&out->pass_a_steps, &out->steps, &out->reason};
+    const size_t bytes = np * (15 * sizeof(double) + 3 * sizeof(int32_t));
+    void* dbuf = nullptr;
+    if (np > 0 && hipMalloc(&dbuf, bytes) != hipSuccess)
+        return kp_internal::fail(KP_OUT_OF_MEMORY, "hipMalloc of device outputs failed");
+    kp_image_data dev;
+    std::memset(&dev, 0, sizeof(dev));
+    double** dev_real[15] = {&dev.image_i, &dev.image_q, &dev.image_u, &dev.image_v, &dev.closure_x,
+                             &dev.closure_k, &dev.final_null, &dev.frame_error, &dev.det_r,
+                             &dev.overlap_r11, &dev.overlap_r12, &dev.overlap_r21, &dev.overlap_r22,
+                             &dev.basis_identity_error, &dev.basis_rotation_angle};
+    int32_t** dev_int[3] = {&dev.pass_a_steps, &dev.steps, &dev.reason};
+    char* b = static_cast<char*>(dbuf);
+    for (int i = 0; i < 15; ++i) {
+        if (*host_real[i]) *dev_real[i] = reinterpret_cast<double*>(b);
+        b += np * sizeof(double);
+    }
+    for (int i = 0; i < 3; ++i) {
+        if (*host_int[i]) *dev_int[i] = reinterpret_cast<int32_t*>(b);
+        b += np * sizeof(int32_t);
+    }
+    kp_run_options opt;
+    kp_run_options_default(&opt);
+    if (options) opt = *options;
+    kp_status st = kp_internal_run_riaf(params, model, &opt, &dev, stats, 0, 0.0);
+    if (st == KP_OK) {
+        if (hipStreamSynchronize(static_cast<hipStream_t>(opt.stream)) != hipSuccess) {
+            st = kp_internal::fail(KP_HIP_ERROR, "stream synchronisation failed");
+        } else {
+            for (int i = 0; i < 15 && st == KP_OK; ++i)
+                if (*host_real[i] && hipMemcpy(*host_real[i], *dev_real[i], np * sizeof(double),
+                                               hipMemcpyDeviceToHost) != hipSuccess)
+                    st = kp_internal::fail(KP_HIP_ERROR, "device to host copy failed");
+            for (int i = 0; i < 3 && st == KP_OK; ++i)
+                if (*host_int[i] && hipMemcpy(*host_int[i], *dev_int[i], np * sizeof(int32_t),
+                                              hipMemcpyDeviceToHost) != hipSuccess)
+                    st = kp_internal::fail(KP_HIP_ERROR, "device to host copy failed");
+        }
+    }
+    if (dbuf) (void)hipFree(dbuf);
+    return st;
+}

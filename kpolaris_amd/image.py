@@ -1,0 +1,200 @@
+"""Python host layer over the C ABI (include/kp_image.h).
+
+Mirrors the reference's image entry point
+``run_riaf_image(pass_a, model, radiation_substeps, timing, split_transport)``
+(src/image/riaf.hpp:12-17) returning the ImageHostData arrays
+(src/image/result.hpp:9-31), plus the image tool's option layer
+(tools/kpolaris_model_image_options.cpp) through ``Options``.
+
+Device memory and streams come from torch (plumbing only); every computation
+runs in the HIP library.  If the library is missing this module raises: there
+is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "_lib" / "libkpolaris_amd.so"
+HEADER = PKG.parent / "include" / "kp_image.h"
+
+_lib = None
+
+
+class KPError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{abi.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def library():
+    """Load libkpolaris_amd.so (built by kpolaris_amd.build_lib)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m kpolaris_amd.build_lib` "
+            "(or __graft_entry__.build()); the imaging kernel has no CPU fallback")
+    L = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    L.kp_abi_version.restype = C.c_int32
+    L.kp_last_error.restype = C.c_char_p
+    L.kp_device_count.restype = C.c_int32
+    for fn in (L.kp_run_riaf_image, L.kp_run_riaf_image_host):
+        fn.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(abi.RunOptions),
+                       P(abi.ImageData), P(abi.ImageStats)]
+        fn.restype = C.c_int32
+    L.kp_pass_a_params_default.argtypes = [P(abi.PassAParams)]
+    L.kp_riaf_model_default.argtypes = [P(abi.RIAFModel)]
+    L.kp_run_options_default.argtypes = [P(abi.RunOptions)]
+    L.kp_options_create.restype = C.c_void_p
+    L.kp_options_destroy.argtypes = [C.c_void_p]
+    L.kp_options_set.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
+    L.kp_options_set.restype = C.c_int32
+    L.kp_options_load_file.argtypes = [C.c_void_p, C.c_char_p]
+    L.kp_options_load_file.restype = C.c_int32
+    L.kp_options_resolve_riaf.argtypes = [C.c_void_p, P(abi.PassAParams), P(abi.RIAFModel),
+                                          P(C.c_double), P(C.c_int32)]
+    L.kp_options_resolve_riaf.restype = C.c_int32
+    dp, ip = P(C.c_double), P(C.c_int32)
+    L.kp_pack_image.argtypes = [dp, dp, dp, dp, ip, C.c_int64, C.c_double, C.c_int32, C.c_double,
+                                dp, dp, dp, dp, dp]
+    L.kp_pack_image.restype = C.c_int32
+    _lib = L
+    return L
+
+
+def header_functions() -> list[str]:
+    """Names of every function the public header declares."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kp_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(status: int):
+    if status != abi.KP_OK:
+        msg = library().kp_last_error().decode(errors="replace")
+        raise KPError(status, msg)
+
+
+class Options:
+    """The image tool's option set (key=value, parameter files)."""
+
+    def __init__(self, params: dict | None = None, parameter_file: str | None = None):
+        self._h = library().kp_options_create()
+        if not self._h:
+            raise MemoryError("kp_options_create")
+        if parameter_file:
+            _check(library().kp_options_load_file(self._h, str(parameter_file).encode()))
+        for k, v in (params or {}).items():
+            self.set(k, v)
+
+    def set(self, key: str, value) -> None:
+        _check(library().kp_options_set(self._h, str(key).encode(), str(value).encode()))
+
+    def resolve_riaf(self):
+        """Returns (PassAParams, RIAFModel, intensity_to_flux_jy_per_pixel, qu_sign)."""
+        p, m = abi.PassAParams(), abi.RIAFModel()
+        scale, sign = C.c_double(), C.c_int32()
+        _check(library().kp_options_resolve_riaf(self._h, C.byref(p), C.byref(m), C.byref(scale),
+                                                 C.byref(sign)))
+        return p, m, scale.value, sign.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                library().kp_options_destroy(self._h)
+        except Exception:
+            pass
+
+
+def pack_image(image: dict, frequency: float, qu_sign: int, scale: float):
+    """Host packing: (I_nu, Q_nu, U_nu, V_nu, [flux_I, Q, U, V]) as numpy arrays."""
+    L = library()
+    arrs = [np.ascontiguousarray(np.asarray(image[n], dtype=np.float64))
+            for n in ("image_i", "image_q", "image_u", "image_v")]
+    reason = np.ascontiguousarray(np.asarray(image["reason"], dtype=np.int32))
+    n = reason.size
+    outs = [np.zeros(n) for _ in range(4)]
+    flux = np.zeros(4)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    _check(L.kp_pack_image(*(dp(a) for a in arrs), reason.ctypes.data_as(C.POINTER(C.c_int32)), n,
+                           frequency, qu_sign, scale, *(dp(a) for a in outs), dp(flux)))
+    return outs[0], outs[1], outs[2], outs[3], flux
+
+
+def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
+                waves_per_cu: int = 0, stream=None) -> abi.RunOptions:
+    o = abi.RunOptions()
+    library().kp_run_options_default(C.byref(o))
+    o.precision = precision
+    o.persistent = 1 if persistent else 0
+    o.block_size = block_size
+    o.waves_per_cu = waves_per_cu
+    o.stream = stream
+    return o
+
+
+def alloc_device_image(npix: int, device="cuda", fields=abi.IMAGE_FIELDS):
+    import torch
+    out = {}
+    for n in fields:
+        dt = torch.int32 if n in abi.IMAGE_INT_FIELDS else torch.float64
+        out[n] = torch.empty(npix, dtype=dt, device=device)
+    return out
+
+
+def image_struct_from_tensors(arrays: dict) -> abi.ImageData:
+    img = abi.ImageData()
+    for n, t in arrays.items():
+        ptr = t.data_ptr()
+        if n in abi.IMAGE_INT_FIELDS:
+            setattr(img, n, C.cast(C.c_void_p(ptr), C.POINTER(C.c_int32)))
+        else:
+            setattr(img, n, C.cast(C.c_void_p(ptr), C.POINTER(C.c_double)))
+    return img
+
+
+def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_substeps: int = 1,
+                   precision: int = abi.PRECISION_FP64, persistent: bool = True,
+                   block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
+                   fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+    """Run pass A + pass B on the current HIP device; returns (dict of device
+    tensors, stats dict or None).  Work is queued on torch's current stream."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    if out is None:
+        out = alloc_device_image(npix, fields=fields)
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream))
+    st = abi.ImageStats()
+    _check(library().kp_run_riaf_image(C.byref(params), C.byref(model), radiation_substeps,
+                                       C.byref(opts), C.byref(img),
+                                       C.byref(st) if with_stats else None))
+    return out, (st.as_dict() if with_stats else None)
+
+
+def run_riaf_image_host(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_substeps: int = 1,
+                        precision: int = abi.PRECISION_FP64, persistent: bool = True,
+                        block_size: int = 0):
+    """Same computation with host (numpy) outputs; synchronous."""
+    npix = params.camera.nx * params.camera.ny
+    arrays = {n: np.zeros(npix, dtype=np.int32 if n in abi.IMAGE_INT_FIELDS else np.float64)
+              for n in abi.IMAGE_FIELDS}
+    img = abi.ImageData()
+    for n, a in arrays.items():
+        ct = C.c_int32 if n in abi.IMAGE_INT_FIELDS else C.c_double
+        setattr(img, n, a.ctypes.data_as(C.POINTER(ct)))
+    opts = run_options(precision, persistent, block_size)
+    st = abi.ImageStats()
+    _check(library().kp_run_riaf_image_host(C.byref(params), C.byref(model), radiation_substeps,
+                                            C.byref(opts), C.byref(img), C.byref(st)))
+    return arrays, st.as_dict()

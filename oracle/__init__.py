@@ -98,6 +98,10 @@ def lib():
                                      C.POINTER(Perturbation), i32, i32, i32,
                                      C.POINTER(abi.ImageData), C.POINTER(Stats)]
     L.kpo_run_riaf_image.restype = i32
+    L.kpo_run_riaf_image_strided.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel),
+                                             C.POINTER(Perturbation), i32, i32, i32, i32,
+                                             C.POINTER(abi.ImageData), C.POINTER(Stats)]
+    L.kpo_run_riaf_image_strided.restype = i32
     _lib = L
     return L
 
@@ -152,3 +156,20 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, pixel_begin: i
         raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
     stats = {n: getattr(st, n) for n, _ in Stats._fields_}
     return arrays, stats
+
+
+def run_riaf_image_strided(params: abi.PassAParams, model: abi.RIAFModel, offset: int, stride: int,
+                           count: int, nthreads: int = 0, perturbation=None):
+    """Oracle on pixels offset + i*stride (i < count); compact output arrays."""
+    L = lib()
+    arrays = alloc_image(count)
+    img = image_struct(arrays)
+    st = Stats()
+    pert = perturbation if perturbation is not None else Perturbation()
+    if nthreads <= 0:
+        nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
+    rc = L.kpo_run_riaf_image_strided(C.byref(params), C.byref(model), C.byref(pert), offset,
+                                      stride, count, nthreads, C.byref(img), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
+    return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}

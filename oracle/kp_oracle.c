@@ -1723,3 +1723,52 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
     }
     return KP_OK;
 }
+
+/* Strided pixel sample pixel = offset + i * stride, i < count, outputs compact
+ * (index i): bounded CPU baselines and spot checks of full-size images. */
+kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* p, const kp_riaf_model* model,
+                                     const kpo_perturbation* pert, int32_t offset, int32_t stride,
+                                     int32_t count, int32_t nthreads, kp_image_data* out,
+                                     kpo_stats* stats) {
+    if (!p || !model || !out || stride <= 0 || count < 0 || offset < 0) return KP_INVALID_ARGUMENT;
+    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST || p->camera.model != KP_CAMERA_PINHOLE)
+        return KP_UNSUPPORTED;
+    const int64_t npix = (int64_t)p->camera.nx * p->camera.ny;
+    if ((int64_t)offset + (int64_t)(count > 0 ? count - 1 : 0) * stride >= npix && count > 0)
+        return KP_INVALID_ARGUMENT;
+    double tetrad[16], gcam[16];
+    const double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
+    kpo_bl_camera_tetrad(p, tetrad);
+    kpo_bl_gcov(p->mass, p->spin, xcam, gcam);
+    int64_t rhs_total = 0, coeff_total = 0, transfer_total = 0, a_total = 0, b_total = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(+ : rhs_total, coeff_total, transfer_total, a_total, b_total)
+#endif
+    for (int i = 0; i < count; ++i) {
+        const int pix = offset + i * stride;
+        int64_t rhs_count = 0;
+        metric_ctx m = {p->mass, p->spin, &rhs_count};
+        kpo_stats local = {0, 0, 0, 0, 0};
+        kpo_state camera, endpoint;
+        camera_ray_from_tetrad(p, tetrad, gcam, pix, &camera);
+        int32_t a_steps = 0, a_reason = 0;
+        const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
+        pixel_result r;
+        pass_b(p, model, pert, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
+        store_pixel(out, i, &r);
+        rhs_total += rhs_count;
+        coeff_total += local.coefficient_evals;
+        transfer_total += local.transfer_steps;
+        a_total += a_steps;
+        b_total += r.steps;
+    }
+    if (stats) {
+        stats->rhs_evals = rhs_total;
+        stats->coefficient_evals = coeff_total;
+        stats->transfer_steps = transfer_total;
+        stats->pass_a_steps = a_total;
+        stats->pass_b_steps = b_total;
+    }
+    return KP_OK;
+}

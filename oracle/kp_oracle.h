@@ -105,6 +105,11 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* params, const kp_riaf_model
                              int32_t pixel_begin, int32_t pixel_end,
                              int32_t nthreads, kp_image_data* out, kpo_stats* stats);
 
+kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                     const kpo_perturbation* pert, int32_t offset, int32_t stride,
+                                     int32_t count, int32_t nthreads, kp_image_data* out,
+                                     kpo_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,184 @@
+"""GPU parity of the plain-FP64 HIP imaging kernel against the CPU oracle,
+through the C ABI, plus the reference's own 8x8 golden.
+
+Bar: per pixel |gpu - oracle| <= 1e-10 |oracle| + 1e-8 * peak(I) on I/Q/U/V,
+and identical termination reasons.
+"""
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+PARAMS = ROOT / "kpolaris_amd" / "params"
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+REL = 1e-10
+PEAK_FLOOR = 1e-8
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.skip("no HIP device")
+
+
+def _options(par: str, **overrides):
+    from kpolaris_amd import image
+    o = image.Options(parameter_file=str(PARAMS / par))
+    for k, v in overrides.items():
+        o.set(k, v)
+    return o
+
+
+def _gpu_image(p, m, **kw):
+    import torch
+    from kpolaris_amd import image
+    out, stats = image.run_riaf_image(p, m, **kw)
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in out.items()}, stats
+
+
+def _assert_parity(gpu, ref, label, rel=REL, floor=PEAK_FLOOR):
+    assert np.array_equal(gpu["reason"], ref["reason"]), (
+        f"{label}: reasons differ at {np.flatnonzero(gpu['reason'] != ref['reason'])[:10]}")
+    peak = float(np.max(np.abs(ref["image_i"]))) if ref["image_i"].size else 0.0
+    for name in ("image_i", "image_q", "image_u", "image_v"):
+        err = np.abs(gpu[name] - ref[name])
+        bound = rel * np.abs(ref[name]) + floor * peak
+        bad = np.flatnonzero(err > bound)
+        assert bad.size == 0, (
+            f"{label} {name}: {bad.size} pixels beyond tolerance, worst rel "
+            f"{float(np.max(err / (np.abs(ref[name]) + 1e-300))):.3e} at {bad[:5]}")
+    np.testing.assert_array_equal(gpu["pass_a_steps"], ref["pass_a_steps"], err_msg=label)
+    np.testing.assert_array_equal(gpu["steps"], ref["steps"], err_msg=label)
+    for name in ("closure_x", "closure_k", "final_null", "frame_error", "det_r", "overlap_r11",
+                 "overlap_r12", "overlap_r21", "overlap_r22", "basis_rotation_angle"):
+        np.testing.assert_allclose(gpu[name], ref[name], rtol=1e-8, atol=1e-9, err_msg=f"{label} {name}")
+
+
+def test_gpu_reference_golden_8x8():
+    """tests/test_hdf5_contract.py:671-681 through the HIP kernel and the C++ host layer."""
+    import oracle
+    from kpolaris_amd import image
+    print("case: golden 8x8 RIAF BL", flush=True)
+    g = json.loads((GOLDEN / "riaf_bl_8x8_golden.json").read_text())
+    o = image.Options()
+    for kv in g["parameters"]:
+        k, _, v = kv.partition("=")
+        o.set(k, v)
+    p, m, scale, sign = o.resolve_riaf()
+    gpu, _ = _gpu_image(p, m)
+    I, Q, U, V, flux = image.pack_image(gpu, m.freq_cgs, sign, scale)
+    I2 = I.reshape(8, 8)
+    got = {"flux_I_Jy": flux[0], "flux_Q_Jy": flux[1], "flux_U_Jy": flux[2], "flux_V_Jy": flux[3],
+           "I_sum": float(I2.sum()), "I_max": float(I2.max()), "I_4_4": float(I2[4, 4])}
+    for key, (expected, abs_tol) in g["expected"].items():
+        assert math.isclose(got[key], expected, rel_tol=g["rel_tolerance"], abs_tol=abs_tol), (key, got[key])
+    ref, _ = oracle.run_riaf_image(p, m)
+    _assert_parity(gpu, ref, "golden8x8")
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_gpu_fp64_parity_adaptive_riaf(persistent):
+    """M87 RIAF, adaptive step doubling at tolerance 1e-6, 40x40 over the full field."""
+    import oracle
+    print(f"case: adaptive RIAF 40x40 persistent={persistent}", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=40, ny=40).resolve_riaf()
+    gpu, stats = _gpu_image(p, m, persistent=persistent)
+    ref, ostats = oracle.run_riaf_image(p, m)
+    _assert_parity(gpu, ref, "adaptive40")
+    assert stats["rays"] == 1600
+    assert stats["pass_a_steps"] == int(ref["pass_a_steps"].sum())
+    assert stats["pass_b_steps"] == int(ref["steps"].sum())
+    assert stats["rhs_evals"] == ostats["rhs_evals"]
+    assert stats["coefficient_evals"] == ostats["coefficient_evals"]
+    assert stats["transfer_steps"] == ostats["transfer_steps"]
+
+
+def test_gpu_fp64_parity_fixed_step_riaf():
+    """M87 RIAF with fixed-step RK4 (step 0.05), 12x12 over the full field."""
+    import oracle
+    print("case: fixed-step RIAF 12x12", flush=True)
+    p, m, _, _ = _options("riaf_m87_c1.par", nx=12, ny=12).resolve_riaf()
+    gpu, _ = _gpu_image(p, m)
+    ref, _ = oracle.run_riaf_image(p, m)
+    _assert_parity(gpu, ref, "fixed12")
+
+
+def test_gpu_full_size_c2_spot_parity_and_invariants():
+    """1024x1024 adaptive image: a strided spot sample against the oracle, and
+    size-independent invariants on every pixel."""
+    import oracle
+    print("case: 1024x1024 adaptive RIAF spot parity", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par").resolve_riaf()
+    gpu, stats = _gpu_image(p, m)
+    npix = 1024 * 1024
+    stride, offset = 4099, 17
+    count = (npix - offset + stride - 1) // stride
+    ref, _ = oracle.run_riaf_image_strided(p, m, offset, stride, count)
+    sel = offset + stride * np.arange(count)
+    _assert_parity({k: v[sel] for k, v in gpu.items()}, ref, "c2-spot")
+    ok = gpu["reason"] == 1
+    assert ok.mean() > 0.99
+    for name in ("image_i", "image_q", "image_u", "image_v"):
+        assert np.all(np.isfinite(gpu[name]))
+    lp = np.sqrt(gpu["image_q"] ** 2 + gpu["image_u"] ** 2 + gpu["image_v"] ** 2)
+    assert np.all(lp <= gpu["image_i"] * (1 + 1e-9) + 1e-300)
+    assert np.all(np.abs(gpu["det_r"][ok] - 1.0) < 1e-6)
+    assert stats["rays"] == npix
+
+
+def test_gpu_repeat_is_bitwise_deterministic():
+    """tests/test_repeat_images.py:25-34: repeated images are bitwise equal."""
+    print("case: repeat determinism 32x32", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=32, ny=32).resolve_riaf()
+    a, _ = _gpu_image(p, m)
+    b, _ = _gpu_image(p, m)
+    c, _ = _gpu_image(p, m, persistent=False, block_size=64)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+        np.testing.assert_array_equal(a[k], c[k], err_msg=k)
+
+
+def test_gpu_edge_shapes_and_max_steps():
+    """Ragged image sizes (not a multiple of the wave) and max_steps exhaustion."""
+    import oracle
+    for nx, ny, max_steps in ((1, 1, 200000), (7, 3, 200000), (13, 11, 300)):
+        print(f"case: edge {nx}x{ny} max_steps={max_steps}", flush=True)
+        p, m, _, _ = _options("riaf_m87_c2.par", nx=nx, ny=ny, max_steps=max_steps).resolve_riaf()
+        gpu, _ = _gpu_image(p, m)
+        ref, _ = oracle.run_riaf_image(p, m)
+        _assert_parity(gpu, ref, f"edge{nx}x{ny}")
+    assert (gpu["reason"] == 4).any(), "max_steps=300 should exhaust some rays"
+
+
+def test_gpu_host_entry_point_matches_device_entry_point():
+    from kpolaris_amd import image
+    print("case: host entry point 9x9", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=9, ny=9).resolve_riaf()
+    dev, _ = _gpu_image(p, m)
+    host, _ = image.run_riaf_image_host(p, m)
+    for k in dev:
+        np.testing.assert_array_equal(dev[k], host[k], err_msg=k)
+
+
+def test_gpu_perturbed_dexter_and_faraday_off():
+    """Dexter fit and faraday_rotation=0 through the same kernel."""
+    import oracle
+    from kpolaris_amd import abi
+    print("case: dexter + faraday off 10x10", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=10, ny=10).resolve_riaf()
+    m.emission_type = abi.EMISSION_THERMAL_DEXTER
+    p.emission_selection.faraday_rotation = 0
+    gpu, _ = _gpu_image(p, m)
+    ref, _ = oracle.run_riaf_image(p, m)
+    _assert_parity(gpu, ref, "dexter")
