@@ -288,11 +288,12 @@ template <class T> struct Stokes {
     T I, Q, U, V;
 };
 
+// Scaled Euclidean norm (src/common/vec.hpp:63-71).  Divides by the scale: a
+// reciprocal of a subnormal scale overflows to inf and turns 0 into NaN.
 template <class T> KP_DEV T norm3(T x, T y, T z) {
     const T scale = maxv(absv(x), maxv(absv(y), absv(z)));
     if (scale == T(0)) return T(0);
-    const T is = T(1) / scale;
-    const T a = x * is, b = y * is, c = z * is;
+    const T a = x / scale, b = y / scale, c = z / scale;
     return scale * sqrt(a * a + b * b + c * c);
 }
 
@@ -601,10 +602,10 @@ KP_DEV void thermal_coeffs(int fit, T nu_in, T ne, T thetae, T b, T sin_theta_in
     T aI = T(0), aQ = T(0), aV = T(0);
     const T bi = bnu_inv(nu, thetae);
     if (bi > T(0)) {
-        const T ibi = T(1) / bi;
-        aI = jI_inv * ibi;
-        aQ = jQ_inv * ibi;
-        aV = jV_inv * ibi;
+        // true divisions: B_nu/nu^3 underflows to subnormals at large h nu / kT
+        aI = jI_inv / bi;
+        aQ = jQ_inv / bi;
+        aV = jV_inv / bi;
     }
     T rhoQ, rhoV;
     rho_qv(nu, ne, thetae, b, sin_theta, cos_theta, rhoQ, rhoV);
