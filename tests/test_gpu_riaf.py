@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import json
 import math
+import os
 from pathlib import Path
 
 import numpy as np
@@ -126,6 +127,11 @@ def test_gpu_full_size_c2_spot_parity_and_invariants():
     p, m, _, _ = _options("riaf_m87_c2.par").resolve_riaf()
     gpu, stats = _gpu_image(p, m)
     npix = 1024 * 1024
+    dump = os.environ.get("KP_TEST_DUMP_DIR")
+    if dump:
+        np.savez_compressed(Path(dump) / "c2_fp64_image.npz",
+                            **{k: gpu[k] for k in ("image_i", "image_q", "image_u", "image_v",
+                                                   "reason", "closure_x", "pass_a_steps", "steps")})
     peak = float(np.max(np.abs(gpu["image_i"])))
     stride, offset = 257, 17
     count = (npix - offset + stride - 1) // stride
@@ -147,7 +153,7 @@ def test_gpu_full_size_c2_spot_parity_and_invariants():
     # chaotically, so they are compared with the oracle at 1e-6.
     lp = np.sqrt(gpu["image_q"] ** 2 + gpu["image_u"] ** 2 + gpu["image_v"] ** 2)
     over = np.flatnonzero(lp > gpu["image_i"] * (1 + 1e-9) + 1e-12 * peak)
-    unclosed = ok & (gpu["closure_x"] >= 1e-3)
+    unclosed = ok & (gpu["closure_x"] >= 0.05)
     print(f"unclosed rays: {int(unclosed.sum())} of {npix}; |P|>I rays: {over.size}", flush=True)
     assert np.all(unclosed[over]), "|P| > I only on rays that did not return to the camera"
     worst = over[np.argsort(-(lp[over] / np.maximum(gpu["image_i"][over], 1e-300)))][:6]
@@ -157,7 +163,7 @@ def test_gpu_full_size_c2_spot_parity_and_invariants():
               f"|P|={lp[pix]:.6e} oracle I={ref1['image_i'][0]:.6e}", flush=True)
         _assert_parity({k: v[pix:pix + 1] for k, v in gpu.items()}, ref1, f"pixel{pix}", rel=1e-6,
                        peak=peak)
-    closed = ok & (gpu["closure_x"] < 1e-3)
+    closed = ok & ~unclosed
     assert closed.mean() > 0.99
     assert np.all(np.abs(gpu["det_r"][closed] - 1.0) < 1e-6)
     assert stats["rays"] == npix
