@@ -165,7 +165,8 @@ def test_gpu_full_size_c2_spot_parity_and_invariants():
                        peak=peak)
     closed = ok & ~unclosed
     assert closed.mean() > 0.99
-    assert np.all(np.abs(gpu["det_r"][closed] - 1.0) < 1e-6)
+    # R is evaluated with the metric at the end point, so det R - 1 ~ closure_x
+    assert np.all(np.abs(gpu["det_r"][closed] - 1.0) < 1e-6 + 10.0 * gpu["closure_x"][closed])
     assert stats["rays"] == npix
 
 
