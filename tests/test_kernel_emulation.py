@@ -12,6 +12,7 @@ import pytest
 
 import oracle
 from kpolaris_amd import abi, image
+from tests import parity
 
 ROOT = Path(__file__).resolve().parent.parent
 PARAMS = ROOT / "kpolaris_amd" / "params"
@@ -38,18 +39,9 @@ def _run_emul(L, p, m):
     return arrays, rhs.value
 
 
-def _compare(a, b, rel=1e-10):
-    np.testing.assert_array_equal(a["reason"], b["reason"])
-    np.testing.assert_array_equal(a["pass_a_steps"], b["pass_a_steps"])
-    np.testing.assert_array_equal(a["steps"], b["steps"])
-    peak = float(np.max(np.abs(b["image_i"])))
-    for n in ("image_i", "image_q", "image_u", "image_v"):
-        np.testing.assert_array_less(np.abs(a[n] - b[n]), rel * np.abs(b[n]) + 1e-10 * peak + 1e-300, err_msg=n)
-    for n in ("closure_x", "closure_k", "final_null", "frame_error", "overlap_r11", "overlap_r12",
-              "overlap_r21", "overlap_r22", "det_r", "basis_rotation_angle", "basis_identity_error"):
-        # diagnostics of rays that wind near the photon ring are chaotic in the
-        # end position; they are checked loosely (the parity bar is on I/Q/U/V)
-        np.testing.assert_allclose(a[n], b[n], rtol=1e-4, atol=1e-8, err_msg=n)
+def _compare(a, b):
+    """b: oracle_with_sensitivity result; the parity bar of tests/parity.py."""
+    parity.assert_parity(a, b, "emulated kernel")
 
 
 def _opts(par, **kw):
@@ -68,7 +60,7 @@ def test_emulated_kernel_golden_8x8(emul):
         o.set(k, v)
     p, m, scale, sign = o.resolve_riaf()
     e, _ = _run_emul(emul, p, m)
-    r, _ = oracle.run_riaf_image(p, m)
+    r = parity.oracle_with_sensitivity(p, m)
     _compare(e, r)
     flux = image.pack_image(e, m.freq_cgs, sign, scale)[4]
     assert flux[0] == pytest.approx(2.3672187308355426, rel=2e-6)
@@ -77,15 +69,24 @@ def test_emulated_kernel_golden_8x8(emul):
 def test_emulated_kernel_adaptive_c2(emul):
     p, m, _, _ = _opts("riaf_m87_c2.par", nx=18, ny=18)
     e, rhs = _run_emul(emul, p, m)
-    r, st = oracle.run_riaf_image(p, m)
+    r = parity.oracle_with_sensitivity(p, m)
     _compare(e, r)
+    _, st = oracle.run_riaf_image(p, m)
     assert rhs == st["rhs_evals"]
 
 
 def test_emulated_kernel_fixed_c1(emul):
     p, m, _, _ = _opts("riaf_m87_c1.par", nx=5, ny=5)
     e, _ = _run_emul(emul, p, m)
-    r, _ = oracle.run_riaf_image(p, m)
+    r = parity.oracle_with_sensitivity(p, m)
+    _compare(e, r)
+
+
+def test_emulated_kernel_edge_7x3(emul):
+    """A ray that winds once around the hole (ill-conditioned) and ragged shape."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=7, ny=3)
+    e, _ = _run_emul(emul, p, m)
+    r = parity.oracle_with_sensitivity(p, m)
     _compare(e, r)
 
 
@@ -94,6 +95,6 @@ def test_emulated_kernel_dexter_faraday_off_and_max_steps(emul):
     m.emission_type = abi.EMISSION_THERMAL_DEXTER
     p.emission_selection.faraday_rotation = 0
     e, _ = _run_emul(emul, p, m)
-    r, _ = oracle.run_riaf_image(p, m)
+    r = parity.oracle_with_sensitivity(p, m)
     _compare(e, r)
     assert (r["reason"] == abi.REASON_MAX_STEPS).any()
