@@ -82,7 +82,10 @@ def assert_parity(got, ref, label, peak=None, diagnostics=True):
     assert not steps_bad.any(), (
         f"{label}: step counts differ on well-conditioned pixels {np.flatnonzero(steps_bad)[:8]}")
     if diagnostics:
-        wc = ~ill
+        # end-state diagnostics are compared on well-conditioned rays that
+        # reached the camera; rays stopped by max_steps close to the horizon
+        # (where the Boyer-Lindquist k diverges) have no well-defined end state
+        wc = ~ill & (ref["reason"] == 1)
         for n in ("closure_x", "closure_k", "final_null", "frame_error", "det_r", "overlap_r11",
                   "overlap_r12", "overlap_r21", "overlap_r22", "basis_rotation_angle"):
             np.testing.assert_allclose(got[n][wc], ref[n][wc], rtol=1e-4, atol=1e-8,
