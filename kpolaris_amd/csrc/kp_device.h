@@ -33,6 +33,50 @@ template <class T> KP_DEV T sq(T x) { return x * x; }
 
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
+// Sine and cosine for the bounded arguments of the transport equations
+// (|x| < 1e5; the Boyer-Lindquist polar angle stays within a few pi).
+// Cody-Waite reduction by pi/2 with a three-part split of pi/2 (the first two
+// parts carry 33 significant bits, so n * P1 and n * P2 are exact for
+// |n| < 2^20), then the Taylor series of sin and cos on |r| <= pi/4 through
+// r^19 and r^18 in Horner form; the truncation error there is below 1e-19.
+// The library routines also handle arbitrarily large arguments, which costs
+// registers this kernel needs for its state.
+KP_DEV void sincos_bounded(double x, double* s_out, double* c_out) {
+    const double n = rint(x * 0.6366197723675814);
+    double r = fma(-n, 1.5707963267341256, x);
+    r = fma(-n, 6.077100506303966e-11, r);
+    r = fma(-n, 2.0222662487959506e-21, r);
+    const double r2 = r * r;
+    double ps = -8.22063524662433e-18;
+    ps = fma(ps, r2, 2.8114572543455206e-15);
+    ps = fma(ps, r2, -7.647163731819816e-13);
+    ps = fma(ps, r2, 1.6059043836821613e-10);
+    ps = fma(ps, r2, -2.505210838544172e-08);
+    ps = fma(ps, r2, 2.7557319223985893e-06);
+    ps = fma(ps, r2, -0.0001984126984126984);
+    ps = fma(ps, r2, 0.008333333333333333);
+    ps = fma(ps, r2, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = -1.5619206968586225e-16;
+    pc = fma(pc, r2, 4.779477332387385e-14);
+    pc = fma(pc, r2, -1.1470745597729725e-11);
+    pc = fma(pc, r2, 2.08767569878681e-09);
+    pc = fma(pc, r2, -2.755731922398589e-07);
+    pc = fma(pc, r2, 2.48015873015873e-05);
+    pc = fma(pc, r2, -0.001388888888888889);
+    pc = fma(pc, r2, 0.041666666666666664);
+    pc = fma(pc, r2, -0.5);
+    const double cr = fma(r2, pc, 1.0);
+    const int q = int(n) & 3;
+    const double s_ = (q & 1) ? cr : sr;
+    const double c_ = (q & 1) ? sr : cr;
+    *s_out = (q & 2) ? -s_ : s_;
+    *c_out = ((q + 1) & 2) ? -c_ : c_;
+}
+
+KP_DEV void ksincos(double x, double* s, double* c) { sincos_bounded(x, s, c); }
+KP_DEV void ksincos(float x, float* s, float* c) { sincosf(x, s, c); }
+
 // ---------------------------------------------------------------------------
 // Transport state: x, k, e1, e2 (src/geodesic/state.hpp:8-16) as 16 registers.
 // ---------------------------------------------------------------------------
@@ -68,7 +112,7 @@ template <class T> KP_DEV BLConn<T> bl_conn(T m, T a, T x1, T x2) {
     BLConn<T> c;
     const T r = maxv(x1, tiny<T>());
     T sn, cs;
-    sincos(x2, &sn, &cs);
+    ksincos(x2, &sn, &cs);
     const T a2 = a * a;
     const T sin2 = maxv(sn * sn, tiny<T>());
     const T cos2 = cs * cs;
@@ -216,38 +260,68 @@ template <class T> KP_DEV void rhs(const Geo<T>& g, const St<T>& s, St<T>& d) {
     bl_rhs(g.m, g.a, s, d);
 }
 
-// s + (k1 + 2 k2 + 2 k3 + k4) * (h / 6), with k1 supplied.
-template <class T>
-KP_DEV void rk4_from_k1(const Geo<T>& g, const St<T>& s, T h, const St<T>& k1, St<T>& out) {
-    St<T> acc, tmp, k;
-    const T hh = T(0.5) * h;
+// BL transport RHS delivered component by component: sink(i, f_i) is called
+// for i = 0..15 in order, and every component of y a later f_j depends on is
+// read before sink(i, .) runs, so a sink may overwrite y[i] in place.  This
+// lets an RK stage fold f into its accumulator and build the next stage state
+// without holding a separate derivative vector.
+template <class T, class Sink> KP_DEV void bl_rhs_apply(T m, T a, const T* y, Sink&& sink) {
+    const BLConn<T> c = bl_conn(m, a, y[1], y[2]);
+    const T k[4] = {y[4], y[5], y[6], y[7]};
+    T Ak[4], Bk[4];
+    rowdot4(c.dr00, c.dr03, c.dr11, c.dr22, c.dr33, k, Ak);
+    rowdot4(c.dt00, c.dt03, c.dt11, c.dt22, c.dt33, k, Bk);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        acc.v[i] = k1.v[i];
-        tmp.v[i] = s.v[i] + k1.v[i] * hh;
-    }
-    rhs(g, tmp, k);
+    for (int mu = 0; mu < 4; ++mu) sink(mu, k[mu]);
+    T out[4];
+    bl_contract_k(c, k, Ak, Bk, k, out);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        acc.v[i] = acc.v[i] + T(2) * k.v[i];
-        tmp.v[i] = s.v[i] + k.v[i] * hh;
-    }
-    rhs(g, tmp, k);
+    for (int mu = 0; mu < 4; ++mu) sink(4 + mu, -out[mu]);
+    const T e1[4] = {y[8], y[9], y[10], y[11]};
+    bl_contract_k(c, k, Ak, Bk, e1, out);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        acc.v[i] = acc.v[i] + T(2) * k.v[i];
-        tmp.v[i] = s.v[i] + k.v[i] * h;
-    }
-    rhs(g, tmp, k);
-    const T h6 = h / T(6);
+    for (int mu = 0; mu < 4; ++mu) sink(8 + mu, -out[mu]);
+    const T e2[4] = {y[12], y[13], y[14], y[15]};
+    bl_contract_k(c, k, Ak, Bk, e2, out);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) out.v[i] = s.v[i] + (acc.v[i] + k.v[i]) * h6;
+    for (int mu = 0; mu < 4; ++mu) sink(12 + mu, -out[mu]);
 }
 
+// RK4 stages (src/geodesic/rk4.hpp:36-59).  With acc = k1 + 2 k2 + 2 k3 built
+// in that order, the step is base + (acc + k4) * (h / 6) exactly as the
+// reference sums it.
+// f = f(y); acc += 2 f; y = base + f * c
+template <class T> KP_DEV void stage_mid(const Geo<T>& g, St<T>& y, const St<T>& base, St<T>& acc, T c) {
+    ++*g.rhs_count;
+    bl_rhs_apply(g.m, g.a, y.v, [&](int i, T f) {
+        acc.v[i] = acc.v[i] + T(2) * f;
+        y.v[i] = base.v[i] + f * c;
+    });
+}
+
+// f = f(y); y = base + (acc + f) * c
+template <class T> KP_DEV void stage_last(const Geo<T>& g, St<T>& y, const St<T>& base, const St<T>& acc, T c) {
+    ++*g.rhs_count;
+    bl_rhs_apply(g.m, g.a, y.v, [&](int i, T f) { y.v[i] = base.v[i] + (acc.v[i] + f) * c; });
+}
+
+// f = f(src); acc = f; y = src + f * c
+template <class T> KP_DEV void stage_first(const Geo<T>& g, const St<T>& src, St<T>& acc, St<T>& y, T c) {
+    ++*g.rhs_count;
+    bl_rhs_apply(g.m, g.a, src.v, [&](int i, T f) {
+        acc.v[i] = f;
+        y.v[i] = src.v[i] + f * c;
+    });
+}
+
+// RK4 step s -> out (rk4.hpp:36-46).
 template <class T> KP_DEV void rk4(const Geo<T>& g, const St<T>& s, T h, St<T>& out) {
-    St<T> k1;
-    rhs(g, s, k1);
-    rk4_from_k1(g, s, h, k1, out);
+    St<T> acc;
+    const T hh = T(0.5) * h;
+    stage_first(g, s, acc, out, hh);
+    stage_mid(g, out, s, acc, hh);
+    stage_mid(g, out, s, acc, h);
+    stage_last(g, out, s, acc, h / T(6));
 }
 
 // state_error_norm (rk4.hpp:74-89).
@@ -261,16 +335,50 @@ template <class T> KP_DEV T err_norm(const St<T>& a, const St<T>& b) {
     return err;
 }
 
-// One step-doubling attempt of adaptive_rk4_step (rk4.hpp:121-128): the
-// full step and two half steps share k1; returns the error norm.
+// One step-doubling attempt of adaptive_rk4_step (rk4.hpp:121-128): the full
+// step and the two half steps share k1.  The half steps are integrated first;
+// the full step's last stage is compared with the two-half-step state as it is
+// formed, so the full-step state is never held.  Returns the error norm.
 template <class T>
 KP_DEV T doubling_attempt(const Geo<T>& g, const St<T>& s, T h, St<T>& half, St<T>& two_half) {
-    St<T> k1, full;
+    St<T> k1, acc;
     rhs(g, s, k1);
-    rk4_from_k1(g, s, h, k1, full);
-    rk4_from_k1(g, s, h * T(0.5), k1, half);
-    rk4(g, half, h * T(0.5), two_half);
-    return err_norm(full, two_half);
+    const T h2 = h * T(0.5);
+    const T q2 = T(0.5) * h2;
+    // first half step from s with k1
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        acc.v[i] = k1.v[i];
+        half.v[i] = s.v[i] + k1.v[i] * q2;
+    }
+    stage_mid(g, half, s, acc, q2);
+    stage_mid(g, half, s, acc, h2);
+    stage_last(g, half, s, acc, h2 / T(6));
+    // second half step from the mid state
+    stage_first(g, half, acc, two_half, q2);
+    stage_mid(g, two_half, half, acc, q2);
+    stage_mid(g, two_half, half, acc, h2);
+    stage_last(g, two_half, half, acc, h2 / T(6));
+    // full step from s with k1, compared on the fly
+    St<T> y;
+    const T hh = T(0.5) * h;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        acc.v[i] = k1.v[i];
+        y.v[i] = s.v[i] + k1.v[i] * hh;
+    }
+    stage_mid(g, y, s, acc, hh);
+    stage_mid(g, y, s, acc, h);
+    T err = T(0);
+    const T h6 = h / T(6);
+    ++*g.rhs_count;
+    bl_rhs_apply(g.m, g.a, y.v, [&](int i, T f) {
+        const T full = s.v[i] + (acc.v[i] + f) * h6;
+        const T two = two_half.v[i];
+        const T sc = maxv(T(1), maxv(absv(full), absv(two)));
+        err = maxv(err, absv(full - two) / sc);
+    });
+    return err;
 }
 
 // ---------------------------------------------------------------------------

@@ -426,7 +426,7 @@ KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
     St<double> cam;
     camera_ray(P, L.pixel, cam);
     double sn, cs;
-    sincos(L.s.v[2], &sn, &cs);
+    ksincos(L.s.v[2], &sn, &cs);
     const BLGcov<double> g = bl_gcov(P.mass, P.spin, L.s.v[1], sn, cs);
     double r11 = 1.0, r12 = 0.0, r21 = 0.0, r22 = 1.0;
     Stokes<double> o = {0.0, 0.0, 0.0, 0.0};

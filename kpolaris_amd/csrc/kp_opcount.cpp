@@ -86,6 +86,9 @@ static inline void sincos(CD a, CD* s, CD* c) {
 #include "kp_device.h"
 #include "kp_riaf.h"
 
+// the kernels' bounded-argument sine/cosine pair counts as two transcendentals
+KP_DEV void ksincos(CD a, CD* s, CD* c) { sincos(a, s, c); }
+
 namespace kpd {
 template <> KP_DEV CD tiny<CD>() { return CD(1e-300); }
 template <> KP_DEV CD large<CD>() { return CD(1e300); }

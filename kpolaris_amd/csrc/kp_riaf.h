@@ -153,7 +153,7 @@ template <class T> KP_DEV void riaf_coefficients(const RiafDev<T>& M, const St<T
     const T r = maxv(x[1], M.min_radius);
     if (r <= M.rh + M.horizon_buffer || r < M.r_min || r > M.r_max) return;
     T sn, cs;
-    sincos(x[2], &sn, &cs);
+    ksincos(x[2], &sn, &cs);
     // density_profile_bl (riaf.hpp:54-60)
     const T zc = r * cs;
     const T rc = maxv(r * sn, M.min_radius);

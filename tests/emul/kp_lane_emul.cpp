@@ -40,3 +40,6 @@ extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp
     if (rhs_evals) *rhs_evals = total;
     return KP_OK;
 }
+
+// The kernels' bounded-argument sine/cosine, for its accuracy test.
+extern "C" void kp_emul_sincos(double x, double* s, double* c) { kpd::sincos_bounded(x, s, c); }
