@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define KP_ABI_VERSION 1
+#define KP_ABI_VERSION 2
 
 /* Status codes.  The reference throws std::runtime_error / std::invalid_argument
  * (src/image/riaf.cpp:15-18, src/image/batch_driver.hpp:40-41); here no
@@ -79,8 +79,11 @@ typedef enum kp_emission_type {
 typedef enum kp_precision {
     /* FP64 throughout: the parity kernel. */
     KP_PRECISION_FP64 = 0,
-    /* FP32 stepping and coefficients with FP64 kept for near-horizon and
-     * turning-point steps, the step-error estimate and large Faraday depth. */
+    /* FP64 ray state; FP32 transport derivatives, RK stage sums, plasma
+     * coefficients and Stokes transfer (in nu_obs^3-scaled units).  FP64 is
+     * kept for integration attempts inside kp_run_options.fp64_radius, the
+     * step-error estimate, surface-crossing refinement and Faraday rotation
+     * angles above one radian. */
     KP_PRECISION_MIXED = 1
 } kp_precision;
 
@@ -202,6 +205,7 @@ typedef struct kp_run_options {
     int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread */
     int32_t waves_per_cu;     /* persistent grid size; 0 = occupancy-derived */
     int32_t reserved0;
+    double fp64_radius;       /* mixed: attempts starting at r < fp64_radius run in FP64 */
     void* stream;             /* hipStream_t, NULL = default stream */
 } kp_run_options;
 

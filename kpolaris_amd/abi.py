@@ -51,6 +51,11 @@ EMISSION_THERMAL_DEXTER = 4
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+# mixed precision: integration attempts that start inside this radius run in FP64
+FP64_RADIUS_DEFAULT = 2.5
+
+# include/kp_image.h KP_ABI_VERSION this mirror matches
+ABI_VERSION = 2
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)
@@ -136,6 +141,7 @@ class RunOptions(C.Structure):
         ("precision", C.c_int32), ("split_transport", C.c_int32),
         ("block_size", C.c_int32), ("persistent", C.c_int32),
         ("waves_per_cu", C.c_int32), ("reserved0", C.c_int32),
+        ("fp64_radius", C.c_double),
         ("stream", C.c_void_p),
     ]
 
@@ -231,5 +237,6 @@ def run_options_defaults() -> RunOptions:
     o.block_size = 0
     o.persistent = 1
     o.waves_per_cu = 0
+    o.fp64_radius = FP64_RADIUS_DEFAULT
     o.stream = None
     return o

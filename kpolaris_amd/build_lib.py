@@ -39,28 +39,35 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the imaging library needs ROCm's hipcc")
 
 
-def build(verbose: bool = False, resource_usage: bool = False) -> Path:
+def build(verbose: bool = False, resource_usage: bool = False, defines: dict | None = None,
+          lib: Path | None = None) -> Path:
+    """Compile and link the library.  `defines` sets the kernel's tuning macros
+    (KP_WAVES_PER_EU, KP_LANE_LDS); `lib` redirects the output for a tuning
+    build that must not replace the in-tree library."""
     OUT_DIR.mkdir(parents=True, exist_ok=True)
+    lib = Path(lib) if lib else LIB
+    dflags = [f"-D{k}={v}" for k, v in (defines or {}).items()]
     objs = []
     for src in SOURCES:
         obj = OUT_DIR / (Path(src).stem + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [hipcc(), *HIPCC_FLAGS, *dflags, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
         if resource_usage and src.endswith(".hip"):
             cmd.append("-Rpass-analysis=kernel-resource-usage")
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(str(obj))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     for o in objs:
         Path(o).unlink(missing_ok=True)
-    build_opcount(verbose)
-    return LIB
+    if lib == LIB:
+        build_opcount(verbose)
+    return lib
 
 
 def build_opcount(verbose: bool = False) -> Path:

@@ -33,6 +33,20 @@ template <class T> KP_DEV T sq(T x) { return x * x; }
 
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
+// Reciprocal and quotient: exact IEEE division in double; in float on the
+// device the hardware reciprocal (1 ulp), which the mixed-precision path
+// accepts in place of the multi-instruction correctly rounded division.
+KP_DEV double krcp(double x) { return 1.0 / x; }
+KP_DEV double kdiv(double a, double b) { return a / b; }
+KP_DEV float krcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+KP_DEV float kdiv(float a, float b) { return a * krcp(b); }
+
 // Sine and cosine for the bounded arguments of the transport equations
 // (|x| < 1e5; the Boyer-Lindquist polar angle stays within a few pi).
 // Cody-Waite reduction by pi/2 with a three-part split of pi/2 (the first two
@@ -76,18 +90,44 @@ KP_DEV void sincos_bounded(double x, double* s_out, double* c_out) {
 
 KP_DEV void ksincos(double x, double* s, double* c) { sincos_bounded(x, s, c); }
 KP_DEV void ksincos(float x, float* s, float* c) { sincosf(x, s, c); }
+// Library sine/cosine for unbounded arguments (Faraday rotation angles).
+KP_DEV void lsincos(double x, double* s, double* c) { sincos(x, s, c); }
+KP_DEV void lsincos(float x, float* s, float* c) { sincosf(x, s, c); }
 
 // ---------------------------------------------------------------------------
 // Transport state: x, k, e1, e2 (src/geodesic/state.hpp:8-16) as 16 registers.
 // ---------------------------------------------------------------------------
 template <class T> struct St {
     T v[16];
+    KP_DEV T operator[](int i) const { return v[i]; }
+};
+
+// A 16-component vector held in LDS for one lane: component c at p[c * stride]
+// (component-major, so a wavefront's 64 lanes touch 64 consecutive words).
+template <class T> struct LdsVec {
+    T* p;
+    int stride;
+    KP_DEV T operator[](int i) const { return p[i * stride]; }
+    KP_DEV void set(int i, T x) const { p[i * stride] = x; }
 };
 
 template <class T> KP_DEV void st_copy(St<T>& d, const St<T>& s) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) d.v[i] = s.v[i];
 }
+
+template <class T> KP_DEV void st_load(St<T>& d, const LdsVec<T>& s) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) d.v[i] = s[i];
+}
+
+template <class T> KP_DEV void st_store(const LdsVec<T>& d, const St<T>& s) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) d.set(i, s.v[i]);
+}
+
+template <class T> KP_DEV void st_load(St<T>& d, const St<T>& s) { st_copy(d, s); }
+template <class T> KP_DEV void st_store(St<T>& d, const St<T>& s) { st_copy(d, s); }
 
 template <class T> KP_DEV bool st_finite(const St<T>& s) {
     bool ok = true;
@@ -122,8 +162,8 @@ template <class T> KP_DEV BLConn<T> bl_conn(T m, T a, T x1, T x2) {
     const T sig_t = -T(2) * a2 * sn * cs;
     const T del_r = T(2) * (r - m);
     const T ds2 = T(2) * sn * cs;
-    const T isig = T(1) / sig;
-    const T idel = T(1) / del;
+    const T isig = krcp(sig);
+    const T idel = krcp(del);
     const T isig2 = isig * isig;
     const T A = (sig - r * sig_r) * isig2;
     const T B = (ds2 * sig - sin2 * sig_t) * isig2;
@@ -147,7 +187,7 @@ template <class T> KP_DEV BLConn<T> bl_conn(T m, T a, T x1, T x2) {
     c.g03 = -T(2) * m * a * r * isd;
     c.g11 = del * isig;
     c.g22 = isig;
-    c.g33 = (del - a2 * sin2) * isd / sin2;
+    c.g33 = kdiv((del - a2 * sin2) * isd, sin2);
     return c;
 }
 
@@ -264,9 +304,10 @@ template <class T> KP_DEV void rhs(const Geo<T>& g, const St<T>& s, St<T>& d) {
 // for i = 0..15 in order, and every component of y a later f_j depends on is
 // read before sink(i, .) runs, so a sink may overwrite y[i] in place.  This
 // lets an RK stage fold f into its accumulator and build the next stage state
-// without holding a separate derivative vector.
-template <class T, class Sink> KP_DEV void bl_rhs_apply(T m, T a, const T* y, Sink&& sink) {
-    const BLConn<T> c = bl_conn(m, a, y[1], y[2]);
+// without holding a separate derivative vector.  Y is any indexable vector
+// (registers or LDS).
+template <class T, class Y, class Sink> KP_DEV void bl_rhs_apply(T m, T a, const Y& y, Sink&& sink) {
+    const BLConn<T> c = bl_conn(m, a, T(y[1]), T(y[2]));
     const T k[4] = {y[4], y[5], y[6], y[7]};
     T Ak[4], Bk[4];
     rowdot4(c.dr00, c.dr03, c.dr11, c.dr22, c.dr33, k, Ak);
@@ -287,35 +328,40 @@ template <class T, class Sink> KP_DEV void bl_rhs_apply(T m, T a, const T* y, Si
     for (int mu = 0; mu < 4; ++mu) sink(12 + mu, -out[mu]);
 }
 
+template <class T> KP_DEV void vset(St<T>& v, int i, T x) { v.v[i] = x; }
+template <class T> KP_DEV void vset(const LdsVec<T>& v, int i, T x) { v.set(i, x); }
+
 // RK4 stages (src/geodesic/rk4.hpp:36-59).  With acc = k1 + 2 k2 + 2 k3 built
 // in that order, the step is base + (acc + k4) * (h / 6) exactly as the
-// reference sums it.
+// reference sums it.  The base vector may live in registers or in LDS.
 // f = f(y); acc += 2 f; y = base + f * c
-template <class T> KP_DEV void stage_mid(const Geo<T>& g, St<T>& y, const St<T>& base, St<T>& acc, T c) {
+template <class T, class B>
+KP_DEV void stage_mid(const Geo<T>& g, St<T>& y, const B& base, St<T>& acc, T c) {
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, y.v, [&](int i, T f) {
+    bl_rhs_apply(g.m, g.a, y, [&](int i, T f) {
         acc.v[i] = acc.v[i] + T(2) * f;
-        y.v[i] = base.v[i] + f * c;
+        y.v[i] = base[i] + f * c;
     });
 }
 
 // f = f(y); y = base + (acc + f) * c
-template <class T> KP_DEV void stage_last(const Geo<T>& g, St<T>& y, const St<T>& base, const St<T>& acc, T c) {
+template <class T, class B>
+KP_DEV void stage_last(const Geo<T>& g, St<T>& y, const B& base, const St<T>& acc, T c) {
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, y.v, [&](int i, T f) { y.v[i] = base.v[i] + (acc.v[i] + f) * c; });
+    bl_rhs_apply(g.m, g.a, y, [&](int i, T f) { y.v[i] = base[i] + (acc.v[i] + f) * c; });
 }
 
 // f = f(src); acc = f; y = src + f * c
-template <class T> KP_DEV void stage_first(const Geo<T>& g, const St<T>& src, St<T>& acc, St<T>& y, T c) {
+template <class T, class B> KP_DEV void stage_first(const Geo<T>& g, const B& src, St<T>& acc, St<T>& y, T c) {
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, src.v, [&](int i, T f) {
+    bl_rhs_apply(g.m, g.a, src, [&](int i, T f) {
         acc.v[i] = f;
-        y.v[i] = src.v[i] + f * c;
+        y.v[i] = src[i] + f * c;
     });
 }
 
 // RK4 step s -> out (rk4.hpp:36-46).
-template <class T> KP_DEV void rk4(const Geo<T>& g, const St<T>& s, T h, St<T>& out) {
+template <class T, class B> KP_DEV void rk4(const Geo<T>& g, const B& s, T h, St<T>& out) {
     St<T> acc;
     const T hh = T(0.5) * h;
     stage_first(g, s, acc, out, hh);
@@ -338,18 +384,21 @@ template <class T> KP_DEV T err_norm(const St<T>& a, const St<T>& b) {
 // One step-doubling attempt of adaptive_rk4_step (rk4.hpp:121-128): the full
 // step and the two half steps share k1.  The half steps are integrated first;
 // the full step's last stage is compared with the two-half-step state as it is
-// formed, so the full-step state is never held.  Returns the error norm.
-template <class T>
-KP_DEV T doubling_attempt(const Geo<T>& g, const St<T>& s, T h, St<T>& half, St<T>& two_half) {
-    St<T> k1, acc;
-    rhs(g, s, k1);
+// formed, so the full-step state is never held.  s (read only) and the k1
+// scratch K may live in registers or LDS.  Returns the error norm.
+template <class T, class S, class K>
+KP_DEV T doubling_attempt(const Geo<T>& g, const S& s, T h, K& k1, St<T>& half, St<T>& two_half) {
+    St<T> acc;
+    ++*g.rhs_count;
+    bl_rhs_apply(g.m, g.a, s, [&](int i, T f) { vset(k1, i, f); });
     const T h2 = h * T(0.5);
     const T q2 = T(0.5) * h2;
     // first half step from s with k1
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-        acc.v[i] = k1.v[i];
-        half.v[i] = s.v[i] + k1.v[i] * q2;
+        const T k = k1[i];
+        acc.v[i] = k;
+        half.v[i] = s[i] + k * q2;
     }
     stage_mid(g, half, s, acc, q2);
     stage_mid(g, half, s, acc, h2);
@@ -364,21 +413,28 @@ KP_DEV T doubling_attempt(const Geo<T>& g, const St<T>& s, T h, St<T>& half, St<
     const T hh = T(0.5) * h;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-        acc.v[i] = k1.v[i];
-        y.v[i] = s.v[i] + k1.v[i] * hh;
+        const T k = k1[i];
+        acc.v[i] = k;
+        y.v[i] = s[i] + k * hh;
     }
     stage_mid(g, y, s, acc, hh);
     stage_mid(g, y, s, acc, h);
     T err = T(0);
     const T h6 = h / T(6);
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, y.v, [&](int i, T f) {
-        const T full = s.v[i] + (acc.v[i] + f) * h6;
+    bl_rhs_apply(g.m, g.a, y, [&](int i, T f) {
+        const T full = s[i] + (acc.v[i] + f) * h6;
         const T two = two_half.v[i];
         const T sc = maxv(T(1), maxv(absv(full), absv(two)));
         err = maxv(err, absv(full - two) / sc);
     });
     return err;
+}
+
+template <class T>
+KP_DEV T doubling_attempt(const Geo<T>& g, const St<T>& s, T h, St<T>& half, St<T>& two_half) {
+    St<T> k1;
+    return doubling_attempt(g, s, h, k1, half, two_half);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,7 +488,7 @@ template <class T> KP_DEV void faraday_rotate(Stokes<T>& s, const Coeffs<T>& c, 
     const T inv = T(1) / rn;
     const T nx = c.rQ * inv, ny = c.rU * inv, nz = c.rV * inv;
     T st, ct;
-    sincos(theta, &st, &ct);
+    lsincos(theta, &st, &ct);
     const T nd = (nx * px + ny * py + nz * pz) * (T(1) - ct);
     s.Q = px * ct + nx * nd + (ny * pz - nz * py) * st;
     s.U = py * ct + ny * nd + (nz * px - nx * pz) * st;

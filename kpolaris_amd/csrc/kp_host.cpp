@@ -309,6 +309,9 @@ extern "C" void kp_run_options_default(kp_run_options* o) {
     o->block_size = 0;
     o->persistent = 1;
     o->waves_per_cu = 0;
+    // Delta = r^2 - 2r + a^2 loses about -log2(Delta / r^2) bits in FP32;
+    // below r = 2.5 (all spins) attempts run in FP64.
+    o->fp64_radius = 2.5;
     o->stream = nullptr;
 }
 

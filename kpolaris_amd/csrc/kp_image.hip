@@ -13,8 +13,13 @@
 //  * Pass A writes endpoints as structure-of-arrays [16][npix] (coalesced);
 //    pass B reads them back, like the reference's split driver
 //    (src/image/split_driver.hpp:33-104).
+//  * Each kernel is instantiated twice: plain FP64 (the parity path) and
+//    mixed (FP64 state, FP32 derivatives, coefficients and transfer; the
+//    geodesic attempt falls back to FP64 inside fp64_radius, where the
+//    Boyer-Lindquist Delta cancels).
 #include "kp_device.h"
 #include "kp_riaf.h"
+#include "kp_mixed.h"
 #include "kp_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -29,7 +34,24 @@
 
 using namespace kpd;
 
+// Target occupancy of the imaging kernels (waves per SIMD): the register
+// budget the compiler may use is 512 / KP_WAVES_PER_EU per lane.
+#ifndef KP_WAVES_PER_EU
+#define KP_WAVES_PER_EU 1
+#endif
+// 1: each lane's state and k1 scratch live in LDS ([32][blockDim] doubles per
+// block) instead of registers, trading LDS traffic for register headroom.
+#ifndef KP_LANE_LDS
+#define KP_LANE_LDS 0
+#endif
+
 namespace {
+
+#if KP_LANE_LDS
+using LaneVec = LdsVec<double>;
+#else
+using LaneVec = St<double>;
+#endif
 
 enum : int {
     R_NONE = 0,
@@ -63,6 +85,10 @@ struct Params {
     double mass, spin, inner, outer, step, tol, min_step, max_step;
     double max_rs, max_rd, max_ad, max_fd;
     RiafDev<double> riaf;
+    // mixed precision: FP32 model constants, FP64 guard radius, 1 / nu_obs^3
+    RiafMixed rm;
+    float mass_f, spin_f;
+    double fp64_radius, inv_nu3;
     // workspace
     double* ep;
     int32_t* ep_valid;
@@ -148,10 +174,10 @@ KP_DEV bool fetch_pixel(uint32_t* counter, bool need, uint32_t npix, int& pixel)
 
 // Wave-level sum of per-lane counters, one atomic per counter per wave.
 KP_DEV void flush_counters(const Params& P, int rhs_slot, uint32_t rhs, uint32_t coeff,
-                           uint32_t transfer, uint32_t steps_a, uint32_t steps_b) {
-    unsigned long long v[5] = {rhs, coeff, transfer, steps_a, steps_b};
+                           uint32_t transfer, uint32_t steps_a, uint32_t steps_b, uint32_t guard) {
+    unsigned long long v[6] = {rhs, coeff, transfer, steps_a, steps_b, guard};
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < 6; ++c) {
         unsigned long long x = v[c];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
         v[c] = x;
@@ -162,6 +188,7 @@ KP_DEV void flush_counters(const Params& P, int rhs_slot, uint32_t rhs, uint32_t
         atomicAdd(&P.counters[C_TRANSFER], v[2]);
         atomicAdd(&P.counters[C_STEPS_A], v[3]);
         atomicAdd(&P.counters[C_STEPS_B], v[4]);
+        atomicAdd(&P.counters[C_GUARD], v[5]);
     }
 }
 
@@ -170,13 +197,15 @@ KP_DEV void flush_counters(const Params& P, int rhs_slot, uint32_t rhs, uint32_t
 // with the BL metric-domain hooks reduced (src/geodesic/metric_domain.hpp:159-223).
 // ---------------------------------------------------------------------------
 struct LaneA {
-    St<double> s;
+    LaneVec s, k1;  // the lane's state and its k1 scratch
     double h_current, h_try, call_min;
     int n, steps, steps_inside, entered, att, in_call, pixel, reason;
 };
 
 KP_DEV void lane_a_init(const Params& P, int pixel, LaneA& L) {
-    camera_ray(P, pixel, L.s);
+    St<double> cam;
+    camera_ray(P, pixel, cam);
+    st_store(L.s, cam);
     L.h_current = -absv(P.step);
     L.h_try = 0.0;
     L.call_min = 0.0;
@@ -253,15 +282,42 @@ KP_DEV void locate_crossing(const Geo<double>& G, const St<double>& first, const
     }
 }
 
+// One integration attempt from the lane state: FP64, or with FP32 derivatives
+// outside the guard radius when MIXED.  guard counts the FP64 attempts a
+// mixed lane takes.
+template <bool MIXED>
+KP_DEV double attempt(const Params& P, const Geo<double>& G, const LaneVec& s, LaneVec& k1, double h,
+                      St<double>& half, St<double>& next, uint32_t& guard) {
+    if (MIXED) {
+        if (!(s[1] < P.fp64_radius)) return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count);
+        ++guard;
+    }
+    return doubling_attempt(G, s, h, k1, half, next);
+}
+
+template <bool MIXED>
+KP_DEV void step_rk4(const Params& P, const Geo<double>& G, const LaneVec& s, double h, St<double>& next,
+                     uint32_t& guard) {
+    if (MIXED) {
+        if (!(s[1] < P.fp64_radius)) {
+            rk4_mixed(P.mass_f, P.spin_f, s, h, next, G.rhs_count);
+            return;
+        }
+        ++guard;
+    }
+    rk4(G, s, h, next);
+}
+
 // One tick of pass A; returns true when the ray has terminated (L.reason set).
-KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_t& steps_acc) {
+template <bool MIXED>
+KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_t& steps_acc, uint32_t& guard) {
     const double eff_min = absv(P.min_step);
     if (!L.in_call) {
         if (L.n >= P.max_steps) {
             L.reason = R_MAX_STEPS;
             return true;
         }
-        const double r = L.s.v[1];
+        const double r = L.s[1];
         if (!L.entered) {
             if (r <= P.outer) {
                 L.entered = 1;
@@ -292,7 +348,7 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
     if (P.adaptive) {
         // adaptive_rk4_step (src/geodesic/rk4.hpp:109-161), one attempt per tick
         St<double> half;
-        const double err = doubling_attempt(G, L.s, L.h_try, half, next);
+        const double err = attempt<MIXED>(P, G, L.s, L.k1, L.h_try, half, next, guard);
         const double tol = maxv(P.tol, 8.0 * eps<double>());
         const double req_sign = L.h_current < 0.0 ? -1.0 : 1.0;
         if (err <= tol) {
@@ -324,9 +380,9 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
         }
     } else {
         used_h = L.h_current;
-        rk4(G, L.s, used_h, next);
+        step_rk4<MIXED>(P, G, L.s, used_h, next, guard);
     }
-    const double old_r = L.s.v[1];
+    const double old_r = L.s[1];
     const double next_r = next.v[1];
     const double old_inner = old_r - P.inner;
     const double next_inner = next_r - P.inner;
@@ -334,7 +390,7 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
         L.entered = 1;
         L.steps_inside = 0;
     } else if (!L.entered && next_r >= old_r) {
-        st_copy(L.s, next);
+        st_store(L.s, next);
         L.reason = R_ESCAPED;
         L.steps += 1;
         steps_acc += 1;
@@ -346,16 +402,17 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
     else if (L.entered && L.steps_inside > 0 && old_r < P.outer && next_r >= P.outer)
         crossing = R_ESCAPED;
     if (crossing != R_NONE) {
-        St<double> ev;
-        locate_crossing(G, L.s, next, used_h, crossing == R_INNER ? P.inner : P.outer,
+        St<double> first, ev;
+        st_load(first, L.s);
+        locate_crossing(G, first, next, used_h, crossing == R_INNER ? P.inner : P.outer,
                         P.adaptive, ev);
-        st_copy(L.s, ev);
+        st_store(L.s, ev);
         L.reason = crossing;
         L.steps += 1;
         steps_acc += 1;
         return true;
     }
-    st_copy(L.s, next);
+    st_store(L.s, next);
     L.steps += 1;
     steps_acc += 1;
     if (L.entered) L.steps_inside += 1;
@@ -366,18 +423,28 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
 KP_DEV void lane_a_store(const Params& P, const LaneA& L) {
     const size_t np = (size_t)P.npix;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = L.s.v[c];
+    for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = L.s[c];
     const int valid = L.entered && (L.reason == R_INNER || L.reason == R_ESCAPED);
     P.ep_valid[L.pixel] = valid;
     P.ep_steps[L.pixel] = L.steps;
     P.ep_reason[L.pixel] = L.reason;
 }
 
-template <bool PERSISTENT>
-__global__ __launch_bounds__(256) void pass_a_kernel(Params P) {
-    uint32_t rhs_count = 0, steps_a = 0;
+// Per-lane LDS slots: [32][blockDim] doubles, components 0-15 the state and
+// 16-31 the k1 scratch of the step-doubling attempt.
+KP_DEV void lane_lds(LdsVec<double>& s, LdsVec<double>& k1) {
+    extern __shared__ double kp_lds[];
+    s = LdsVec<double>{kp_lds + threadIdx.x, (int)blockDim.x};
+    k1 = LdsVec<double>{kp_lds + 16 * blockDim.x + threadIdx.x, (int)blockDim.x};
+}
+KP_DEV void lane_lds(St<double>&, St<double>&) {}
+
+template <bool PERSISTENT, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_a_kernel(Params P) {
+    uint32_t rhs_count = 0, steps_a = 0, guard = 0;
     Geo<double> G{P.mass, P.spin, &rhs_count};
     LaneA L;
+    lane_lds(L.s, L.k1);
     bool has = false;
     if (PERSISTENT) {
         bool exhausted = false;
@@ -393,7 +460,7 @@ __global__ __launch_bounds__(256) void pass_a_kernel(Params P) {
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_a_tick(P, G, L, steps_a)) {
+            if (has && pass_a_tick<MIXED>(P, G, L, steps_a, guard)) {
                 lane_a_store(P, L);
                 has = false;
             }
@@ -402,12 +469,12 @@ __global__ __launch_bounds__(256) void pass_a_kernel(Params P) {
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
         if (pix < P.npix) {
             lane_a_init(P, pix, L);
-            while (!pass_a_tick(P, G, L, steps_a)) {
+            while (!pass_a_tick<MIXED>(P, G, L, steps_a, guard)) {
             }
             lane_a_store(P, L);
         }
     }
-    flush_counters(P, C_RHS_A, rhs_count, 0, 0, steps_a, 0);
+    flush_counters(P, C_RHS_A, rhs_count, 0, 0, steps_a, 0, guard);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,32 +483,37 @@ __global__ __launch_bounds__(256) void pass_a_kernel(Params P) {
 enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2 };
 
 struct LaneB {
-    St<double> s;
+    LaneVec s, k1;  // the lane's state and its k1 scratch
     Stokes<double> st;
     double h_current, cap, h, h_try, call_min, call_max;
     int steps, reason, att, phase, crosses, reached, pixel, pass_a_steps;
 };
 
-KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
-    St<double> cam;
+// Writes the pixel's outputs.  A mixed-precision lane carries its Stokes
+// vector in nu_obs^3-scaled units; SCALED converts back to invariant ones.
+template <bool SCALED> KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
+    St<double> cam, fin;
     camera_ray(P, L.pixel, cam);
+    st_load(fin, L.s);
     double sn, cs;
-    ksincos(L.s.v[2], &sn, &cs);
-    const BLGcov<double> g = bl_gcov(P.mass, P.spin, L.s.v[1], sn, cs);
+    ksincos(fin.v[2], &sn, &cs);
+    const BLGcov<double> g = bl_gcov(P.mass, P.spin, fin.v[1], sn, cs);
     double r11 = 1.0, r12 = 0.0, r21 = 0.0, r22 = 1.0;
     Stokes<double> o = {0.0, 0.0, 0.0, 0.0};
     int reason = L.reason;
     if (valid) {
         // screen_overlap (pass_b.hpp:87-98) and transform_to_observer_basis
         // (src/frame/stokes_rotation.hpp:39-57)
-        r11 = gdot(g, cam.v + 8, L.s.v + 8);
-        r12 = gdot(g, cam.v + 8, L.s.v + 12);
-        r21 = gdot(g, cam.v + 12, L.s.v + 8);
-        r22 = gdot(g, cam.v + 12, L.s.v + 12);
-        o.I = L.st.I;
-        o.Q = 0.5 * L.st.Q * (r11 * r11 - r12 * r12 - r21 * r21 + r22 * r22) + L.st.U * (r11 * r12 - r21 * r22);
-        o.U = L.st.Q * (r11 * r21 - r12 * r22) + L.st.U * (r11 * r22 + r12 * r21);
-        o.V = (r11 * r22 - r12 * r21) * L.st.V;
+        r11 = gdot(g, cam.v + 8, fin.v + 8);
+        r12 = gdot(g, cam.v + 8, fin.v + 12);
+        r21 = gdot(g, cam.v + 12, fin.v + 8);
+        r22 = gdot(g, cam.v + 12, fin.v + 12);
+        Stokes<double> in = L.st;
+        if (SCALED) in = {in.I * P.inv_nu3, in.Q * P.inv_nu3, in.U * P.inv_nu3, in.V * P.inv_nu3};
+        o.I = in.I;
+        o.Q = 0.5 * in.Q * (r11 * r11 - r12 * r12 - r21 * r21 + r22 * r22) + in.U * (r11 * r12 - r21 * r22);
+        o.U = in.Q * (r11 * r21 - r12 * r22) + in.U * (r11 * r22 + r12 * r21);
+        o.V = (r11 * r22 - r12 * r21) * in.V;
         if (L.reached)
             reason = R_CAMERA;
         else if (reason != R_UNDERFLOW && reason != R_TIME)
@@ -454,14 +526,14 @@ KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
     if (O.image_u) O.image_u[p] = o.U;
     if (O.image_v) O.image_v[p] = o.V;
     if (O.closure_x)
-        O.closure_x[p] = sqrt(sq(L.s.v[1] - cam.v[1]) + sq(L.s.v[2] - cam.v[2]) + sq(L.s.v[3] - cam.v[3]));
+        O.closure_x[p] = sqrt(sq(fin.v[1] - cam.v[1]) + sq(fin.v[2] - cam.v[2]) + sq(fin.v[3] - cam.v[3]));
     if (O.closure_k) {
-        double ck = absv(L.s.v[4] - cam.v[4]);
-        for (int m = 1; m < 4; ++m) ck = maxv(ck, absv(L.s.v[4 + m] - cam.v[4 + m]));
+        double ck = absv(fin.v[4] - cam.v[4]);
+        for (int m = 1; m < 4; ++m) ck = maxv(ck, absv(fin.v[4 + m] - cam.v[4 + m]));
         O.closure_k[p] = ck;
     }
-    if (O.final_null) O.final_null[p] = gdot(g, L.s.v + 4, L.s.v + 4);
-    if (O.frame_error) O.frame_error[p] = frame_error(g, L.s);
+    if (O.final_null) O.final_null[p] = gdot(g, fin.v + 4, fin.v + 4);
+    if (O.frame_error) O.frame_error[p] = frame_error(g, fin);
     if (O.det_r) O.det_r[p] = r11 * r22 - r12 * r21;
     if (O.overlap_r11) O.overlap_r11[p] = r11;
     if (O.overlap_r12) O.overlap_r12[p] = r12;
@@ -480,7 +552,7 @@ KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
 KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
     const size_t np = (size_t)P.npix;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) L.s.v[c] = P.ep[(size_t)c * np + pixel];
+    for (int c = 0; c < 16; ++c) vset(L.s, c, P.ep[(size_t)c * np + pixel]);
     L.pixel = pixel;
     L.pass_a_steps = P.ep_steps[pixel];
     L.reason = P.ep_reason[pixel];
@@ -497,7 +569,7 @@ KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
     L.crosses = 0;
     L.reached = 0;
     if (!P.ep_valid[pixel]) {
-        lane_b_finish(P, L, 0);
+        lane_b_finish<false>(P, L, 0);
         return false;
     }
     return true;
@@ -510,21 +582,56 @@ KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
     L.att = 0;
 }
 
+// Coefficients at the sample state and one Strang transfer step over the
+// affine length |h|.  FP64: invariant coefficients over dl = |h| dl/dlambda.
+// Mixed: FP32 coefficients already per unit |h| and in nu_obs^3-scaled
+// intensity units (the Stokes state is kept scaled until the ray ends).
+template <bool MIXED> struct Radiation;
+
+template <> struct Radiation<false> {
+    Coeffs<double> c;
+    KP_DEV void eval(const Params& P, const St<double>& sample) { riaf_coefficients(P.riaf, sample, c); }
+    KP_DEV void zero() { coeffs_zero(c); }
+    KP_DEV int substeps(const Params& P, double h, double lmrs) const {
+        return substep_count(c, h, P.riaf.dlambda_scale, lmrs, P.max_rd, P.max_ad, P.max_fd);
+    }
+    KP_DEV void transfer(const Params& P, Stokes<double>& st, double h) const {
+        strang_step(st, c, absv(h) * P.riaf.dlambda_scale);
+    }
+};
+
+template <> struct Radiation<true> {
+    Coeffs<float> c;
+    KP_DEV void eval(const Params& P, const St<double>& sample) { riaf_coefficients_scaled(P.rm, sample, c); }
+    KP_DEV void zero() { coeffs_zero(c); }
+    KP_DEV int substeps(const Params& P, double h, double lmrs) const {
+        const Coeffs<double> d = {c.jI, c.jQ, c.jU, c.jV, c.aI, c.aQ, c.aU, c.aV, c.rQ, c.rU, c.rV};
+        return substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd);
+    }
+    KP_DEV void transfer(const Params& P, Stokes<double>& st, double h) const {
+        (void)P;
+        Stokes<float> f = {float(st.I), float(st.Q), float(st.U), float(st.V)};
+        strang_step32(f, c, float(absv(h)));
+        st = {f.I, f.Q, f.U, f.V};
+    }
+};
+
 // One tick of pass B; returns true when the ray has terminated.
+template <bool MIXED>
 KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_t& coeff_acc,
-                        uint32_t& transfer_acc, uint32_t& steps_acc) {
+                        uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     const double eff_min = absv(P.min_step);
     if (L.phase == PH_IDLE) {
         if (L.reached || L.steps >= P.max_steps) return true;
         double h = L.h_current;
-        if (L.s.v[1] <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
+        if (L.s[1] <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
         L.h = h;
         if (P.adaptive) start_call(L, h, eff_min, P.max_step);
         L.phase = PH_MAIN;
     }
     St<double> next, sample;
     if (P.adaptive) {
-        const double err = doubling_attempt(G, L.s, L.h_try, sample, next);
+        const double err = attempt<MIXED>(P, G, L.s, L.k1, L.h_try, sample, next, guard);
         const double tol = maxv(P.tol, 8.0 * eps<double>());
         if (err <= tol) {
             double factor = 2.0;
@@ -551,13 +658,13 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
             return false;
         }
     } else {
-        rk4(G, L.s, L.h, next);
+        step_rk4<MIXED>(P, G, L.s, L.h, next, guard);
         st_copy(sample, next);
     }
     if (L.phase == PH_MAIN) {
         // camera_surface_value/crossed/crossing_fraction for the pinhole sphere
         // (pass_b.hpp:54-85, 605-637)
-        const double s0 = L.s.v[1] - P.radius;
+        const double s0 = L.s[1] - P.radius;
         const double s1 = next.v[1] - P.radius;
         const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
         L.crosses = crosses;
@@ -567,8 +674,8 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
             const double h_cross = L.h * frac;
             if (absv(h_cross) <= 1e-14) {
                 L.h = 0.0;
-                st_copy(next, L.s);
-                st_copy(sample, L.s);
+                st_load(next, L.s);
+                st_load(sample, L.s);
             } else if (frac < 0.999999999999) {
                 L.h = h_cross;
                 if (P.adaptive) {
@@ -577,51 +684,51 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
                     L.phase = PH_CROSS;
                     return false;
                 }
-                rk4(G, L.s, L.h, next);
+                step_rk4<MIXED>(P, G, L.s, L.h, next, guard);
                 st_copy(sample, next);
             }
         }
     }
     L.phase = PH_IDLE;
     const double h = L.h;
-    const double old_r = L.s.v[1], sample_r = sample.v[1], next_r = next.v[1];
-    Coeffs<double> c;
-    coeffs_zero(c);
+    const double old_r = L.s[1], sample_r = sample.v[1], next_r = next.v[1];
+    Radiation<MIXED> rad;
+    rad.zero();
     if (sample_r - P.inner > 0.0 && sample_r < P.outer) {
-        riaf_coefficients(P.riaf, sample, c);
+        rad.eval(P, sample);
         coeff_acc += 1;
     }
     const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
                             ? P.max_rs : 0.0;
-    const double dls = P.riaf.dlambda_scale;
-    const int required = substep_count(c, h, dls, lmrs, P.max_rd, P.max_ad, P.max_fd);
+    const int required = rad.substeps(P, h, lmrs);
     if (required > 1 && h > eff_min * 1.0001) {
         L.h_current = maxv(eff_min, h / double(required));
         L.cap = L.h_current;
         return false;
     }
     if (lmrs > 0.0 || P.max_rd > 0.0 || P.max_ad > 0.0 || P.max_fd > 0.0) {
-        const int pred = substep_count(c, L.h_current, dls, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        const int pred = rad.substeps(P, L.h_current, lmrs);
         L.cap = pred > 1 ? maxv(eff_min, L.h_current / double(pred)) : P.max_step;
     } else {
         L.cap = P.max_step;
     }
     if (h > 0.0) {
-        strang_step(L.st, c, absv(h) * dls);
+        rad.transfer(P, L.st, h);
         transfer_acc += 1;
     }
-    st_copy(L.s, next);
+    st_store(L.s, next);
     L.steps += 1;
     steps_acc += 1;
     if (L.crosses) L.reached = 1;
     return false;
 }
 
-template <bool PERSISTENT>
-__global__ __launch_bounds__(256) void pass_b_kernel(Params P) {
-    uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0;
+template <bool PERSISTENT, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
+    uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
     Geo<double> G{P.mass, P.spin, &rhs_count};
     LaneB L;
+    lane_lds(L.s, L.k1);
     bool has = false;
     if (PERSISTENT) {
         bool exhausted = false;
@@ -638,20 +745,20 @@ __global__ __launch_bounds__(256) void pass_b_kernel(Params P) {
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_b_tick(P, G, L, coeff, transfer, steps_b)) {
-                lane_b_finish(P, L, 1);
+            if (has && pass_b_tick<MIXED>(P, G, L, coeff, transfer, steps_b, guard)) {
+                lane_b_finish<MIXED>(P, L, 1);
                 has = false;
             }
         }
     } else {
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
         if (pix < P.npix && lane_b_init(P, pix, L)) {
-            while (!pass_b_tick(P, G, L, coeff, transfer, steps_b)) {
+            while (!pass_b_tick<MIXED>(P, G, L, coeff, transfer, steps_b, guard)) {
             }
-            lane_b_finish(P, L, 1);
+            lane_b_finish<MIXED>(P, L, 1);
         }
     }
-    flush_counters(P, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b);
+    flush_counters(P, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
 }
 
 __global__ void zero_kernel(uint32_t* q, unsigned long long* c) {
@@ -705,22 +812,36 @@ struct Occupancy {
     int blocks_a = 0, blocks_b = 0;
 };
 
-std::mutex g_occ_mutex;
-std::vector<Occupancy> g_occ;
+template <bool PERSISTENT> const void* pass_a_fn(bool mixed) {
+    return mixed ? (const void*)pass_a_kernel<PERSISTENT, true> : (const void*)pass_a_kernel<PERSISTENT, false>;
+}
+template <bool PERSISTENT> const void* pass_b_fn(bool mixed) {
+    return mixed ? (const void*)pass_b_kernel<PERSISTENT, true> : (const void*)pass_b_kernel<PERSISTENT, false>;
+}
 
-kp_status occupancy_for(int device, int block, Occupancy& occ) {
+// LDS bytes per block: 32 doubles per lane (state + k1 scratch).
+size_t lds_bytes(int block) { return KP_LANE_LDS ? size_t(32) * size_t(block) * sizeof(double) : 0; }
+
+std::mutex g_occ_mutex;
+std::vector<std::pair<long long, Occupancy>> g_occ;  // key: (device * 4096 + block) * 2 + mixed
+
+kp_status occupancy_for(int device, int block, bool mixed, Occupancy& occ) {
     std::lock_guard<std::mutex> lock(g_occ_mutex);
-    if ((int)g_occ.size() <= device) g_occ.resize(device + 1);
-    Occupancy& o = g_occ[device];
-    if (o.cus == 0) {
-        hipDeviceProp_t prop;
-        KP_HIP(hipGetDeviceProperties(&prop, device));
-        o.cus = prop.multiProcessorCount;
-        KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_kernel<true>, block, 0));
-        KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_kernel<true>, block, 0));
-        o.blocks_a = std::max(o.blocks_a, 1);
-        o.blocks_b = std::max(o.blocks_b, 1);
-    }
+    const long long key = ((long long)device * 4096 + block) * 2 + (mixed ? 1 : 0);
+    for (auto& e : g_occ)
+        if (e.first == key) {
+            occ = e.second;
+            return KP_OK;
+        }
+    Occupancy o;
+    hipDeviceProp_t prop;
+    KP_HIP(hipGetDeviceProperties(&prop, device));
+    o.cus = prop.multiProcessorCount;
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed), block, lds_bytes(block)));
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed), block, lds_bytes(block)));
+    o.blocks_a = std::max(o.blocks_a, 1);
+    o.blocks_b = std::max(o.blocks_b, 1);
+    g_occ.push_back({key, o});
     occ = o;
     return KP_OK;
 }
@@ -785,6 +906,32 @@ static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Para
     return KP_OK;
 }
 
+// Single-precision model constants and the rescaled-unit factors of the
+// mixed-precision radiation path (kp_mixed.h).
+static void prepare_mixed(Params& P) {
+    const RiafDev<double>& d = P.riaf;
+    RiafDev<float>& f = P.rm.f;
+    f.nth0 = float(d.nth0); f.Te0 = float(d.Te0); f.disk_h = float(d.disk_h);
+    f.pow_nth = float(d.pow_nth); f.pow_T = float(d.pow_T); f.ne_unit = float(d.ne_unit);
+    f.thetae_unit = float(d.thetae_unit); f.r_min = float(d.r_min); f.r_max = float(d.r_max);
+    f.horizon_buffer = float(d.horizon_buffer); f.emission_scale = float(d.emission_scale);
+    f.absorption_scale = float(d.absorption_scale); f.faraday_scale = float(d.faraday_scale);
+    f.max_pol_frac = float(d.max_pol_frac); f.keplerian_factor = float(d.keplerian_factor);
+    f.infall_factor = float(d.infall_factor); f.min_frequency_scale = float(d.min_frequency_scale);
+    f.min_sin_theta = float(d.min_sin_theta); f.min_thetae = float(d.min_thetae);
+    f.min_radius = float(d.min_radius); f.freq_cgs = float(d.freq_cgs); f.dlambda_scale = float(d.dlambda_scale);
+    f.rh = float(d.rh); f.risco = float(d.risco); f.ne_factor = float(d.ne_factor);
+    f.thetae_factor = float(d.thetae_factor); f.b_factor = float(d.b_factor);
+    f.mass = float(d.mass); f.spin = float(d.spin); f.fit = d.fit; f.faraday_rotation = d.faraday_rotation;
+    const double nu_obs = std::max(d.freq_cgs, 1.0);
+    P.rm.L = float(d.dlambda_scale * nu_obs);
+    P.rm.bconst = float(2.0 * KP_HPL * nu_obs * nu_obs * nu_obs / (KP_CL * KP_CL));
+    P.rm.x0 = float(KP_HPL / (KP_ME * KP_CL * KP_CL));
+    P.inv_nu3 = 1.0 / (nu_obs * nu_obs * nu_obs);
+    P.mass_f = float(P.mass);
+    P.spin_f = float(P.spin);
+}
+
 extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const kp_riaf_model* model,
                                           const kp_run_options* options, kp_image_data* out,
                                           kp_image_stats* stats, int perturbation, double log_scale) {
@@ -793,12 +940,19 @@ extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const 
     kp_run_options opt;
     kp_run_options_default(&opt);
     if (options) opt = *options;
-    if (opt.precision != KP_PRECISION_FP64)
-        return kp_internal::fail(KP_UNSUPPORTED, "mixed precision is not built in this library version");
+    if (opt.precision != KP_PRECISION_FP64 && opt.precision != KP_PRECISION_MIXED)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "precision must be KP_PRECISION_FP64 or KP_PRECISION_MIXED");
+    const bool mixed = opt.precision == KP_PRECISION_MIXED;
     Params P;
     kp_status st = prepare(params, model, P);
     if (st != KP_OK) return st;
     if (perturbation) kp_internal::riaf_constants(*params, *model, perturbation, log_scale, P.riaf);
+    if (mixed) {
+        if (!(opt.fp64_radius >= 0.0))
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
+        prepare_mixed(P);
+        P.fp64_radius = opt.fp64_radius;
+    }
     P.out = *out;
     P.persistent = opt.persistent ? 1 : 0;
     int device = 0;
@@ -823,24 +977,21 @@ extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const 
     P.counters = reinterpret_cast<unsigned long long*>(base + off_counters);
 
     hipStream_t stream = static_cast<hipStream_t>(opt.stream);
-    const int block = opt.block_size > 0 ? opt.block_size : 256;
+    const int block = opt.block_size > 0 ? opt.block_size : 128;
     if (block % 64 != 0 || block > 256)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
     Occupancy occ;
-    st = occupancy_for(device, 256, occ);
+    st = occupancy_for(device, block, mixed, occ);
     if (st != KP_OK) return st;
+    const size_t smem = lds_bytes(block);
     const int nblocks_flat = (int)((np + block - 1) / block);
     int grid_a = nblocks_flat, grid_b = nblocks_flat;
     if (P.persistent) {
         const int waves_per_block = block / 64;
-        const int per_cu_a = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block)
-                                                  : occ.blocks_a * (256 / block);
-        const int per_cu_b = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block)
-                                                  : occ.blocks_b * (256 / block);
-        grid_a = std::min(nblocks_flat, occ.cus * per_cu_a);
-        grid_b = std::min(nblocks_flat, occ.cus * per_cu_b);
-        grid_a = std::max(grid_a, 1);
-        grid_b = std::max(grid_b, 1);
+        const int per_cu_a = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block) : occ.blocks_a;
+        const int per_cu_b = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block) : occ.blocks_b;
+        grid_a = std::max(1, std::min(nblocks_flat, occ.cus * per_cu_a));
+        grid_b = std::max(1, std::min(nblocks_flat, occ.cus * per_cu_b));
     }
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (stats) {
@@ -849,20 +1000,15 @@ extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const 
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, P.queue, P.counters);
     KP_HIP(hipGetLastError());
     if (stats) KP_HIP(hipEventRecord(ev[0], stream));
+    void* args[] = {&P};
     if (P.npix > 0) {
-        if (P.persistent)
-            hipLaunchKernelGGL(pass_a_kernel<true>, dim3(grid_a), dim3(block), 0, stream, P);
-        else
-            hipLaunchKernelGGL(pass_a_kernel<false>, dim3(grid_a), dim3(block), 0, stream, P);
-        KP_HIP(hipGetLastError());
+        const void* fn = P.persistent ? pass_a_fn<true>(mixed) : pass_a_fn<false>(mixed);
+        KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem, stream));
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
     if (P.npix > 0) {
-        if (P.persistent)
-            hipLaunchKernelGGL(pass_b_kernel<true>, dim3(grid_b), dim3(block), 0, stream, P);
-        else
-            hipLaunchKernelGGL(pass_b_kernel<false>, dim3(grid_b), dim3(block), 0, stream, P);
-        KP_HIP(hipGetLastError());
+        const void* fn = P.persistent ? pass_b_fn<true>(mixed) : pass_b_fn<false>(mixed);
+        KP_HIP(hipLaunchKernel(fn, dim3(grid_b), dim3(block), args, smem, stream));
     }
     if (stats) {
         KP_HIP(hipEventRecord(ev[2], stream));

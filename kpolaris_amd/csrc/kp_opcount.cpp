@@ -88,6 +88,10 @@ static inline void sincos(CD a, CD* s, CD* c) {
 
 // the kernels' bounded-argument sine/cosine pair counts as two transcendentals
 KP_DEV void ksincos(CD a, CD* s, CD* c) { sincos(a, s, c); }
+KP_DEV void lsincos(CD a, CD* s, CD* c) { sincos(a, s, c); }
+// a reciprocal or quotient counts as one division
+KP_DEV CD krcp(CD x) { return CD(1.0) / x; }
+KP_DEV CD kdiv(CD a, CD b) { return a / b; }
 
 namespace kpd {
 template <> KP_DEV CD tiny<CD>() { return CD(1e-300); }

@@ -67,6 +67,9 @@ def library():
     L.kp_pack_image.argtypes = [dp, dp, dp, dp, ip, C.c_int64, C.c_double, C.c_int32, C.c_double,
                                 dp, dp, dp, dp, dp]
     L.kp_pack_image.restype = C.c_int32
+    if L.kp_abi_version() != abi.ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
+                           f"{abi.ABI_VERSION}: rebuild the library")
     _lib = L
     return L
 
@@ -131,13 +134,15 @@ def pack_image(image: dict, frequency: float, qu_sign: int, scale: float):
 
 
 def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
-                waves_per_cu: int = 0, stream=None) -> abi.RunOptions:
+                waves_per_cu: int = 0, stream=None, fp64_radius: float | None = None) -> abi.RunOptions:
     o = abi.RunOptions()
     library().kp_run_options_default(C.byref(o))
     o.precision = precision
     o.persistent = 1 if persistent else 0
     o.block_size = block_size
     o.waves_per_cu = waves_per_cu
+    if fp64_radius is not None:
+        o.fp64_radius = fp64_radius
     o.stream = stream
     return o
 
@@ -165,7 +170,7 @@ def image_struct_from_tensors(arrays: dict) -> abi.ImageData:
 def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_substeps: int = 1,
                    precision: int = abi.PRECISION_FP64, persistent: bool = True,
                    block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
-                   fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+                   fields=abi.IMAGE_FIELDS, with_stats: bool = True, fp64_radius: float | None = None):
     """Run pass A + pass B on the current HIP device; returns (dict of device
     tensors, stats dict or None).  Work is queued on torch's current stream."""
     import torch
@@ -174,7 +179,7 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
         out = alloc_device_image(npix, fields=fields)
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream))
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius)
     st = abi.ImageStats()
     _check(library().kp_run_riaf_image(C.byref(params), C.byref(model), radiation_substeps,
                                        C.byref(opts), C.byref(img),
@@ -184,7 +189,7 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
 
 def run_riaf_image_host(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_substeps: int = 1,
                         precision: int = abi.PRECISION_FP64, persistent: bool = True,
-                        block_size: int = 0):
+                        block_size: int = 0, fp64_radius: float | None = None):
     """Same computation with host (numpy) outputs; synchronous."""
     npix = params.camera.nx * params.camera.ny
     arrays = {n: np.zeros(npix, dtype=np.int32 if n in abi.IMAGE_INT_FIELDS else np.float64)
@@ -193,7 +198,7 @@ def run_riaf_image_host(params: abi.PassAParams, model: abi.RIAFModel, *, radiat
     for n, a in arrays.items():
         ct = C.c_int32 if n in abi.IMAGE_INT_FIELDS else C.c_double
         setattr(img, n, a.ctypes.data_as(C.POINTER(ct)))
-    opts = run_options(precision, persistent, block_size)
+    opts = run_options(precision, persistent, block_size, fp64_radius=fp64_radius)
     st = abi.ImageStats()
     _check(library().kp_run_riaf_image_host(C.byref(params), C.byref(model), radiation_substeps,
                                             C.byref(opts), C.byref(img), C.byref(st)))

@@ -7,11 +7,16 @@
 
 #include <vector>
 
-extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp_riaf_model* m,
-                                            kp_image_data* out, int64_t* rhs_evals) {
+template <bool MIXED>
+static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, double fp64_radius,
+                          kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps) {
     Params P;
     kp_status st = prepare(pa, m, P);
     if (st != KP_OK) return st;
+    if (MIXED) {
+        prepare_mixed(P);
+        P.fp64_radius = fp64_radius;
+    }
     const int np = P.npix;
     std::vector<double> ep(16 * (size_t)np);
     std::vector<int32_t> valid(np), steps(np), reason(np);
@@ -20,25 +25,40 @@ extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp
     P.ep_steps = steps.data();
     P.ep_reason = reason.data();
     P.out = *out;
-    int64_t total = 0;
+    int64_t total = 0, guard_total = 0;
     for (int pix = 0; pix < np; ++pix) {
-        uint32_t rc = 0, sa = 0, c1 = 0, c2 = 0, sb = 0;
+        uint32_t rc = 0, sa = 0, c1 = 0, c2 = 0, sb = 0, guard = 0;
         Geo<double> G{P.mass, P.spin, &rc};
         LaneA A;
         lane_a_init(P, pix, A);
-        while (!pass_a_tick(P, G, A, sa)) {
+        while (!pass_a_tick<MIXED>(P, G, A, sa, guard)) {
         }
         lane_a_store(P, A);
         LaneB B;
         if (lane_b_init(P, pix, B)) {
-            while (!pass_b_tick(P, G, B, c1, c2, sb)) {
+            while (!pass_b_tick<MIXED>(P, G, B, c1, c2, sb, guard)) {
             }
-            lane_b_finish(P, B, 1);
+            lane_b_finish<MIXED>(P, B, 1);
         }
         total += rc;
+        guard_total += guard;
     }
     if (rhs_evals) *rhs_evals = total;
+    if (guard_steps) *guard_steps = guard_total;
     return KP_OK;
+}
+
+extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp_riaf_model* m,
+                                            kp_image_data* out, int64_t* rhs_evals) {
+    return emul_run<false>(pa, m, 0.0, out, rhs_evals, nullptr);
+}
+
+// The mixed-precision lanes (FP32 arithmetic on the host's IEEE float; the
+// device's 1-ulp hardware reciprocal is a correctly rounded division here).
+extern "C" kp_status kp_emul_run_riaf_image_mixed(const kp_pass_a_params* pa, const kp_riaf_model* m,
+                                                  double fp64_radius, kp_image_data* out, int64_t* rhs_evals,
+                                                  int64_t* guard_steps) {
+    return emul_run<true>(pa, m, fp64_radius, out, rhs_evals, guard_steps);
 }
 
 // The kernels' bounded-argument sine/cosine, for its accuracy test.

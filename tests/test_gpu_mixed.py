@@ -1,0 +1,130 @@
+"""Mixed-precision HIP kernel (FP64 ray state; FP32 derivatives, coefficients
+and transfer; FP64 attempts inside fp64_radius) through the C ABI.
+
+Mixed precision is held to image-level bars against the FP64 image of the
+same inputs (tests/parity.py: total I within 1e-4, LP/CP fractions within 1e-3
+relative, every pixel within 1e-2 of peak I, termination reasons matching on
+all but 0.1% of pixels).  The FP64 kernel itself is pinned to the oracle by
+test_gpu_riaf.py; the small cases here also compare with the oracle directly.
+"""
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests import parity
+from tests.conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+PARAMS = ROOT / "kpolaris_amd" / "params"
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.skip("no HIP device")
+
+
+def _options(par: str, **overrides):
+    from kpolaris_amd import image
+    o = image.Options(parameter_file=str(PARAMS / par))
+    for k, v in overrides.items():
+        o.set(k, v)
+    return o
+
+
+def _gpu_image(p, m, **kw):
+    import torch
+    from kpolaris_amd import image
+    out, stats = image.run_riaf_image(p, m, **kw)
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in out.items()}, stats
+
+
+def test_gpu_mixed_full_image_c2():
+    """The bench workload (1024x1024 adaptive RIAF): mixed vs FP64 on the device."""
+    from kpolaris_amd import abi
+    print("case: mixed vs fp64, RIAF 1024x1024 adaptive", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par").resolve_riaf()
+    ref, st64 = _gpu_image(p, m, precision=abi.PRECISION_FP64)
+    got, stm = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
+    e = parity.assert_mixed(got, ref, "mixed 1024")
+    print(e, "fp64 ms", st64["kernel_ms"], "mixed ms", stm["kernel_ms"], "guard", stm["fp64_guard_steps"],
+          flush=True)
+    assert stm["fp64_guard_steps"] > 0
+    assert st64["fp64_guard_steps"] == 0
+
+
+def test_gpu_mixed_golden_8x8():
+    """The reference's 8x8 golden (tests/test_hdf5_contract.py:671-681) within the mixed bar."""
+    from kpolaris_amd import abi, image
+    print("case: mixed golden 8x8", flush=True)
+    g = json.loads((GOLDEN / "riaf_bl_8x8_golden.json").read_text())
+    o = image.Options()
+    for kv in g["parameters"]:
+        k, _, v = kv.partition("=")
+        o.set(k, v)
+    p, m, scale, sign = o.resolve_riaf()
+    got, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
+    flux = image.pack_image(got, m.freq_cgs, sign, scale)[4]
+    assert math.isclose(flux[0], g["expected"]["flux_I_Jy"][0], rel_tol=parity.MIXED_TOTAL_I)
+    import oracle
+    ref, _ = oracle.run_riaf_image(p, m)
+    parity.assert_mixed(got, ref, "mixed golden8x8")
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_gpu_mixed_vs_oracle_adaptive(persistent):
+    """40x40 adaptive RIAF against the FP64 oracle, both launch modes."""
+    import oracle
+    from kpolaris_amd import abi
+    print(f"case: mixed vs oracle 40x40 persistent={persistent}", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=40, ny=40).resolve_riaf()
+    got, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED, persistent=persistent)
+    ref, _ = oracle.run_riaf_image(p, m)
+    print(parity.assert_mixed(got, ref, "mixed40"), flush=True)
+
+
+def test_gpu_mixed_dexter_fixed_step_unguarded():
+    """Fixed-step RK4, Dexter fit, Faraday off, no FP64 guard radius."""
+    import oracle
+    from kpolaris_amd import abi
+    print("case: mixed dexter fixed-step 16x12 unguarded", flush=True)
+    p, m, _, _ = _options("riaf_m87_c1.par", nx=16, ny=12).resolve_riaf()
+    m.emission_type = abi.EMISSION_THERMAL_DEXTER
+    p.emission_selection.faraday_rotation = 0
+    got, st = _gpu_image(p, m, precision=abi.PRECISION_MIXED, fp64_radius=0.0)
+    assert st["fp64_guard_steps"] == 0
+    ref, _ = oracle.run_riaf_image(p, m)
+    parity.assert_mixed(got, ref, "mixed dexter c1")
+
+
+def test_gpu_mixed_edge_shapes_and_repeat():
+    """1x1 and ragged 7x3 images; a repeated run gives identical bits."""
+    from kpolaris_amd import abi
+    for nx, ny in ((1, 1), (7, 3)):
+        print(f"case: mixed edge {nx}x{ny}", flush=True)
+        p, m, _, _ = _options("riaf_m87_c2.par", nx=nx, ny=ny).resolve_riaf()
+        a, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
+        b, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+        ref, _ = _gpu_image(p, m, precision=abi.PRECISION_FP64)
+        assert np.array_equal(a["reason"], ref["reason"])
+        assert np.all(np.isfinite(a["image_i"]))
+
+
+def test_gpu_invalid_precision_fails_loudly():
+    from kpolaris_amd import image
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=4, ny=4).resolve_riaf()
+    with pytest.raises(image.KPError):
+        image.run_riaf_image(p, m, precision=7)
+    with pytest.raises(image.KPError):
+        image.run_riaf_image(p, m, precision=1, fp64_radius=-1.0)

@@ -27,6 +27,10 @@ def emul():
     L.kp_emul_run_riaf_image.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel),
                                          C.POINTER(abi.ImageData), C.POINTER(C.c_int64)]
     L.kp_emul_run_riaf_image.restype = C.c_int32
+    L.kp_emul_run_riaf_image_mixed.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.c_double,
+                                               C.POINTER(abi.ImageData), C.POINTER(C.c_int64),
+                                               C.POINTER(C.c_int64)]
+    L.kp_emul_run_riaf_image_mixed.restype = C.c_int32
     return L
 
 
@@ -37,6 +41,16 @@ def _run_emul(L, p, m):
     rhs = C.c_int64()
     assert L.kp_emul_run_riaf_image(C.byref(p), C.byref(m), C.byref(img), C.byref(rhs)) == 0
     return arrays, rhs.value
+
+
+def _run_emul_mixed(L, p, m, fp64_radius=abi.FP64_RADIUS_DEFAULT):
+    npix = p.camera.nx * p.camera.ny
+    arrays = oracle.alloc_image(npix)
+    img = oracle.image_struct(arrays)
+    rhs, guard = C.c_int64(), C.c_int64()
+    assert L.kp_emul_run_riaf_image_mixed(C.byref(p), C.byref(m), fp64_radius, C.byref(img), C.byref(rhs),
+                                          C.byref(guard)) == 0
+    return arrays, rhs.value, guard.value
 
 
 def _compare(a, b):
@@ -98,3 +112,32 @@ def test_emulated_kernel_dexter_faraday_off_and_max_steps(emul):
     r = parity.oracle_with_sensitivity(p, m)
     _compare(e, r)
     assert (r["reason"] == abi.REASON_MAX_STEPS).any()
+
+
+def test_emulated_mixed_precision_c2(emul):
+    """The mixed-precision lanes against the FP64 oracle: image-level bars."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=20, ny=20)
+    e, _, guard = _run_emul_mixed(emul, p, m)
+    r, _ = oracle.run_riaf_image(p, m)
+    err = parity.assert_mixed(e, r, "emulated mixed c2")
+    print(err)
+    assert guard > 0  # rays reach r < 2.5 and take FP64 attempts there
+
+
+def test_emulated_mixed_precision_dexter_fixed_step(emul):
+    """Dexter fit, Faraday rotation off, fixed-step RK4: the other branches."""
+    p, m, _, _ = _opts("riaf_m87_c1.par", nx=6, ny=6)
+    m.emission_type = abi.EMISSION_THERMAL_DEXTER
+    p.emission_selection.faraday_rotation = 0
+    e, _, _ = _run_emul_mixed(emul, p, m)
+    r, _ = oracle.run_riaf_image(p, m)
+    parity.assert_mixed(e, r, "emulated mixed dexter c1")
+
+
+def test_emulated_mixed_precision_unguarded(emul):
+    """Without the FP64 guard radius every attempt is FP32-derived; still in bounds."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=12, ny=12)
+    e, _, guard = _run_emul_mixed(emul, p, m, fp64_radius=0.0)
+    assert guard == 0
+    r, _ = oracle.run_riaf_image(p, m)
+    parity.assert_mixed(e, r, "emulated mixed unguarded")
