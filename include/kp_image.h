@@ -25,6 +25,10 @@ extern "C" {
 
 #define KP_ABI_VERSION 2
 
+/* Largest frequency list of one shared-geodesic image (the reference's
+ * KPOLARIS_MAX_FREQUENCIES build option, CMakeLists.txt:14). */
+#define KP_MAX_FREQUENCIES 16
+
 /* Status codes.  The reference throws std::runtime_error / std::invalid_argument
  * (src/image/riaf.cpp:15-18, src/image/batch_driver.hpp:40-41); here no
  * exception crosses the ABI and kp_last_error() holds the message. */
@@ -258,6 +262,45 @@ kp_status kp_run_riaf_image_host(const kp_pass_a_params* params,
                                  const kp_run_options* options,
                                  kp_image_data* out,
                                  kp_image_stats* stats);
+
+/*
+ * Multi-frequency RIAF image on shared geodesics: the geodesic of each pixel
+ * is integrated once and nfreq Stokes vectors are transported along it.
+ * Replaces run_riaf_multifrequency_image(const PassAParams&,
+ * RIAFAnalyticRadiationModel, int radiation_substeps,
+ * const std::vector<double>& frequencies,
+ * const std::vector<double>& step_control_frequencies, int timing,
+ * int split_transport) (src/image/riaf.hpp:31-39, src/image/riaf.cpp:80-105,
+ * src/image/multifrequency.hpp:152-280; per-pixel algorithm
+ * src/geodesic/pass_b.hpp:1130-1500 and, for distinct control lists,
+ * :1505-1895).  frequencies and step_control_frequencies are HOST arrays of
+ * 1..KP_MAX_FREQUENCIES values (Hz); the substep control uses the largest
+ * count over the control frequencies.  out->image_i/q/u/v point to
+ * [nfreq][npix] arrays (frequency-major); the other outputs are per pixel.
+ * model->freq_cgs is not used.
+ */
+kp_status kp_run_riaf_multifrequency_image(const kp_pass_a_params* params,
+                                           const kp_riaf_model* model,
+                                           int32_t radiation_substeps,
+                                           const double* frequencies,
+                                           int32_t nfreq,
+                                           const double* step_control_frequencies,
+                                           int32_t step_control_nfreq,
+                                           const kp_run_options* options,
+                                           kp_image_data* out,
+                                           kp_image_stats* stats);
+
+/* Multi-frequency image with HOST output pointers; synchronous. */
+kp_status kp_run_riaf_multifrequency_image_host(const kp_pass_a_params* params,
+                                                const kp_riaf_model* model,
+                                                int32_t radiation_substeps,
+                                                const double* frequencies,
+                                                int32_t nfreq,
+                                                const double* step_control_frequencies,
+                                                int32_t step_control_nfreq,
+                                                const kp_run_options* options,
+                                                kp_image_data* out,
+                                                kp_image_stats* stats);
 
 /* Defaults of the reference's structs (field initialisers of CameraParams,
  * PassAParams and RIAFAnalyticRadiationModel) and of kp_run_options. */

@@ -128,6 +128,8 @@ IMAGE_REAL_FIELDS = (
 )
 IMAGE_INT_FIELDS = ("pass_a_steps", "steps", "reason")
 IMAGE_FIELDS = IMAGE_REAL_FIELDS + IMAGE_INT_FIELDS
+# per-frequency outputs of a multi-frequency image ([nfreq][npix])
+STOKES_FIELDS = ("image_i", "image_q", "image_u", "image_v")
 
 
 class ImageData(C.Structure):

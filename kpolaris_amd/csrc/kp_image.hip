@@ -89,6 +89,14 @@ struct Params {
     RiafMixed rm;
     float mass_f, spin_f;
     double fp64_radius, inv_nu3;
+    // multi-frequency pass B (shared geodesics): output and step-control
+    // frequency lists with their dl/dlambda, mixed-precision Planck constants
+    // 2 h nu^3 / c^2 and 1 / nu^3
+    int32_t nfreq, nctrl;
+    double freqs[KP_MAX_FREQUENCIES], ctrl[KP_MAX_FREQUENCIES];
+    double dls_f[KP_MAX_FREQUENCIES], dls_c[KP_MAX_FREQUENCIES];
+    double inv_nu3_f[KP_MAX_FREQUENCIES];
+    float bconst_f[KP_MAX_FREQUENCIES], bconst_c[KP_MAX_FREQUENCIES];
     // workspace
     double* ep;
     int32_t* ep_valid;
@@ -485,13 +493,25 @@ enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2 };
 struct LaneB {
     LaneVec s, k1;  // the lane's state and its k1 scratch
     Stokes<double> st;
+    LdsVec<double> mf;  // multi-frequency: Stokes component 4 f + c of this lane, in LDS
     double h_current, cap, h, h_try, call_min, call_max;
     int steps, reason, att, phase, crosses, reached, pixel, pass_a_steps;
 };
 
+// transform_to_observer_basis (src/frame/stokes_rotation.hpp:39-57).
+KP_DEV Stokes<double> observer_basis(const Stokes<double>& in, double r11, double r12, double r21, double r22) {
+    Stokes<double> o;
+    o.I = in.I;
+    o.Q = 0.5 * in.Q * (r11 * r11 - r12 * r12 - r21 * r21 + r22 * r22) + in.U * (r11 * r12 - r21 * r22);
+    o.U = in.Q * (r11 * r21 - r12 * r22) + in.U * (r11 * r22 + r12 * r21);
+    o.V = (r11 * r22 - r12 * r21) * in.V;
+    return o;
+}
+
 // Writes the pixel's outputs.  A mixed-precision lane carries its Stokes
 // vector in nu_obs^3-scaled units; SCALED converts back to invariant ones.
-template <bool SCALED> KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
+// MF: nfreq Stokes vectors from LDS to the [nfreq][npix] image arrays.
+template <bool SCALED, bool MF> KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
     St<double> cam, fin;
     camera_ray(P, L.pixel, cam);
     st_load(fin, L.s);
@@ -510,10 +530,7 @@ template <bool SCALED> KP_DEV void lane_b_finish(const Params& P, const LaneB& L
         r22 = gdot(g, cam.v + 12, fin.v + 12);
         Stokes<double> in = L.st;
         if (SCALED) in = {in.I * P.inv_nu3, in.Q * P.inv_nu3, in.U * P.inv_nu3, in.V * P.inv_nu3};
-        o.I = in.I;
-        o.Q = 0.5 * in.Q * (r11 * r11 - r12 * r12 - r21 * r21 + r22 * r22) + in.U * (r11 * r12 - r21 * r22);
-        o.U = in.Q * (r11 * r21 - r12 * r22) + in.U * (r11 * r22 + r12 * r21);
-        o.V = (r11 * r22 - r12 * r21) * in.V;
+        o = observer_basis(in, r11, r12, r21, r22);
         if (L.reached)
             reason = R_CAMERA;
         else if (reason != R_UNDERFLOW && reason != R_TIME)
@@ -521,10 +538,27 @@ template <bool SCALED> KP_DEV void lane_b_finish(const Params& P, const LaneB& L
     }
     const kp_image_data& O = P.out;
     const int p = L.pixel;
-    if (O.image_i) O.image_i[p] = o.I;
-    if (O.image_q) O.image_q[p] = o.Q;
-    if (O.image_u) O.image_u[p] = o.U;
-    if (O.image_v) O.image_v[p] = o.V;
+    if (MF) {
+        for (int f = 0; f < P.nfreq; ++f) {
+            Stokes<double> of = {0.0, 0.0, 0.0, 0.0};
+            if (valid) {
+                const double sc = SCALED ? P.inv_nu3_f[f] : 1.0;
+                const Stokes<double> in = {L.mf[4 * f] * sc, L.mf[4 * f + 1] * sc, L.mf[4 * f + 2] * sc,
+                                           L.mf[4 * f + 3] * sc};
+                of = observer_basis(in, r11, r12, r21, r22);
+            }
+            const size_t q = (size_t)f * (size_t)P.npix + (size_t)p;
+            if (O.image_i) O.image_i[q] = of.I;
+            if (O.image_q) O.image_q[q] = of.Q;
+            if (O.image_u) O.image_u[q] = of.U;
+            if (O.image_v) O.image_v[q] = of.V;
+        }
+    } else {
+        if (O.image_i) O.image_i[p] = o.I;
+        if (O.image_q) O.image_q[p] = o.Q;
+        if (O.image_u) O.image_u[p] = o.U;
+        if (O.image_v) O.image_v[p] = o.V;
+    }
     if (O.closure_x)
         O.closure_x[p] = sqrt(sq(fin.v[1] - cam.v[1]) + sq(fin.v[2] - cam.v[2]) + sq(fin.v[3] - cam.v[3]));
     if (O.closure_k) {
@@ -549,7 +583,7 @@ template <bool SCALED> KP_DEV void lane_b_finish(const Params& P, const LaneB& L
 
 // Loads a pass A endpoint; returns false (after writing the pixel's outputs)
 // when the endpoint is not a valid radiating segment.
-KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
+template <bool MF> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
     const size_t np = (size_t)P.npix;
 #pragma unroll
     for (int c = 0; c < 16; ++c) vset(L.s, c, P.ep[(size_t)c * np + pixel]);
@@ -557,6 +591,8 @@ KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
     L.pass_a_steps = P.ep_steps[pixel];
     L.reason = P.ep_reason[pixel];
     L.st = {0.0, 0.0, 0.0, 0.0};
+    if (MF)
+        for (int j = 0; j < 4 * P.nfreq; ++j) L.mf.set(j, 0.0);
     L.h_current = P.step;
     L.cap = P.max_rs > 0.0 ? P.max_rs : P.max_step;
     L.h = 0.0;
@@ -569,7 +605,7 @@ KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
     L.crosses = 0;
     L.reached = 0;
     if (!P.ep_valid[pixel]) {
-        lane_b_finish<false>(P, L, 0);
+        lane_b_finish<false, MF>(P, L, 0);
         return false;
     }
     return true;
@@ -582,42 +618,138 @@ KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
     L.att = 0;
 }
 
-// Coefficients at the sample state and one Strang transfer step over the
-// affine length |h|.  FP64: invariant coefficients over dl = |h| dl/dlambda.
-// Mixed: FP32 coefficients already per unit |h| and in nu_obs^3-scaled
-// intensity units (the Stokes state is kept scaled until the ray ends).
-template <bool MIXED> struct Radiation;
+// Coefficients at the sample state, the substep control they drive and one
+// Strang transfer step over the affine length |h|.
+//  * FP64: invariant coefficients over dl = |h| dl/dlambda.
+//  * Mixed: FP32 coefficients already per unit |h| and in nu_obs^3-scaled
+//    intensity units (the Stokes state is kept scaled until the ray ends).
+//  * Multi-frequency (MF): the frequency-independent plasma state is evaluated
+//    once; each control frequency's coefficients drive the substep count (the
+//    largest wins, pass_b.hpp:1416-1460) and each output frequency's the
+//    transfer of its own Stokes vector, held in LDS.
+template <bool MIXED, bool MF> struct Radiation;
 
-template <> struct Radiation<false> {
+KP_DEV void mf_strang(const Params& P, LaneB& L, int f, const Coeffs<double>& c, double dl) {
+    Stokes<double> st = {L.mf[4 * f], L.mf[4 * f + 1], L.mf[4 * f + 2], L.mf[4 * f + 3]};
+    strang_step(st, c, dl);
+    L.mf.set(4 * f, st.I);
+    L.mf.set(4 * f + 1, st.Q);
+    L.mf.set(4 * f + 2, st.U);
+    L.mf.set(4 * f + 3, st.V);
+}
+
+KP_DEV void mf_strang32(const Params& P, LaneB& L, int f, const Coeffs<float>& c, float dl) {
+    Stokes<float> st = {float(L.mf[4 * f]), float(L.mf[4 * f + 1]), float(L.mf[4 * f + 2]), float(L.mf[4 * f + 3])};
+    strang_step32(st, c, dl);
+    L.mf.set(4 * f, st.I);
+    L.mf.set(4 * f + 1, st.Q);
+    L.mf.set(4 * f + 2, st.U);
+    L.mf.set(4 * f + 3, st.V);
+}
+
+KP_DEV Coeffs<double> widen(const Coeffs<float>& c) {
+    return {c.jI, c.jQ, c.jU, c.jV, c.aI, c.aQ, c.aU, c.aV, c.rQ, c.rU, c.rV};
+}
+
+template <> struct Radiation<false, false> {
     Coeffs<double> c;
-    KP_DEV void eval(const Params& P, const St<double>& sample) { riaf_coefficients(P.riaf, sample, c); }
     KP_DEV void zero() { coeffs_zero(c); }
-    KP_DEV int substeps(const Params& P, double h, double lmrs) const {
-        return substep_count(c, h, P.riaf.dlambda_scale, lmrs, P.max_rd, P.max_ad, P.max_fd);
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        riaf_coefficients(P.riaf, sample, c);
+        return 1;
     }
-    KP_DEV void transfer(const Params& P, Stokes<double>& st, double h) const {
-        strang_step(st, c, absv(h) * P.riaf.dlambda_scale);
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        req = substep_count(c, h, P.riaf.dlambda_scale, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        pred = substep_count(c, h_next, P.riaf.dlambda_scale, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        return 0;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        strang_step(L.st, c, absv(h) * P.riaf.dlambda_scale);
+        return 0;
     }
 };
 
-template <> struct Radiation<true> {
+template <> struct Radiation<true, false> {
     Coeffs<float> c;
-    KP_DEV void eval(const Params& P, const St<double>& sample) { riaf_coefficients_scaled(P.rm, sample, c); }
     KP_DEV void zero() { coeffs_zero(c); }
-    KP_DEV int substeps(const Params& P, double h, double lmrs) const {
-        const Coeffs<double> d = {c.jI, c.jQ, c.jU, c.jV, c.aI, c.aQ, c.aU, c.aV, c.rQ, c.rU, c.rV};
-        return substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd);
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        riaf_coefficients_scaled(P.rm, sample, c);
+        return 1;
     }
-    KP_DEV void transfer(const Params& P, Stokes<double>& st, double h) const {
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        const Coeffs<double> d = widen(c);
+        req = substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        pred = substep_count(d, h_next, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        return 0;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         (void)P;
-        Stokes<float> f = {float(st.I), float(st.Q), float(st.U), float(st.V)};
+        Stokes<float> f = {float(L.st.I), float(L.st.Q), float(L.st.U), float(L.st.V)};
         strang_step32(f, c, float(absv(h)));
-        st = {f.I, f.Q, f.U, f.V};
+        L.st = {f.I, f.Q, f.U, f.V};
+        return 0;
+    }
+};
+
+template <> struct Radiation<false, true> {
+    RiafLocal<double> loc;
+    KP_DEV void zero() { loc.valid = 0; }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        riaf_local(P.riaf, sample, loc);
+        return 0;
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        req = 1;
+        pred = 1;
+        for (int f = 0; f < P.nctrl; ++f) {
+            Coeffs<double> c;
+            riaf_coeffs_at(P.riaf, loc, P.ctrl[f], c);
+            req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+            pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+        }
+        return loc.valid ? (uint32_t)P.nctrl : 0u;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        for (int f = 0; f < P.nfreq; ++f) {
+            Coeffs<double> c;
+            riaf_coeffs_at(P.riaf, loc, P.freqs[f], c);
+            mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
+        }
+        return loc.valid ? (uint32_t)P.nfreq : 0u;
+    }
+};
+
+template <> struct Radiation<true, true> {
+    RiafLocal<float> loc;
+    KP_DEV void zero() { loc.valid = 0; }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        riaf_local_scaled(P.rm, sample, loc);
+        return 0;
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        req = 1;
+        pred = 1;
+        for (int f = 0; f < P.nctrl; ++f) {
+            Coeffs<float> c;
+            riaf_coeffs_scaled_at(P.rm, loc, float(P.ctrl[f]), P.bconst_c[f], c);
+            const Coeffs<double> d = widen(c);
+            req = maxv(req, substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
+            pred = maxv(pred, substep_count(d, h_next, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
+        }
+        return loc.valid ? (uint32_t)P.nctrl : 0u;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        for (int f = 0; f < P.nfreq; ++f) {
+            Coeffs<float> c;
+            riaf_coeffs_scaled_at(P.rm, loc, float(P.freqs[f]), P.bconst_f[f], c);
+            mf_strang32(P, L, f, c, float(absv(h)));
+        }
+        return loc.valid ? (uint32_t)P.nfreq : 0u;
     }
 };
 
 // One tick of pass B; returns true when the ray has terminated.
-template <bool MIXED>
+template <bool MIXED, bool MF>
 KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_t& coeff_acc,
                         uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     const double eff_min = absv(P.min_step);
@@ -692,28 +824,24 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
     L.phase = PH_IDLE;
     const double h = L.h;
     const double old_r = L.s[1], sample_r = sample.v[1], next_r = next.v[1];
-    Radiation<MIXED> rad;
+    Radiation<MIXED, MF> rad;
     rad.zero();
-    if (sample_r - P.inner > 0.0 && sample_r < P.outer) {
-        rad.eval(P, sample);
-        coeff_acc += 1;
-    }
+    if (sample_r - P.inner > 0.0 && sample_r < P.outer) coeff_acc += rad.eval(P, sample);
     const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
                             ? P.max_rs : 0.0;
-    const int required = rad.substeps(P, h, lmrs);
+    int required, pred;
+    coeff_acc += rad.control(P, h, L.h_current, lmrs, required, pred);
     if (required > 1 && h > eff_min * 1.0001) {
         L.h_current = maxv(eff_min, h / double(required));
         L.cap = L.h_current;
         return false;
     }
-    if (lmrs > 0.0 || P.max_rd > 0.0 || P.max_ad > 0.0 || P.max_fd > 0.0) {
-        const int pred = rad.substeps(P, L.h_current, lmrs);
+    if (lmrs > 0.0 || P.max_rd > 0.0 || P.max_ad > 0.0 || P.max_fd > 0.0)
         L.cap = pred > 1 ? maxv(eff_min, L.h_current / double(pred)) : P.max_step;
-    } else {
+    else
         L.cap = P.max_step;
-    }
     if (h > 0.0) {
-        rad.transfer(P, L.st, h);
+        coeff_acc += rad.transfer(P, L, h);
         transfer_acc += 1;
     }
     st_store(L.s, next);
@@ -723,12 +851,16 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
     return false;
 }
 
-template <bool PERSISTENT, bool MIXED>
+template <bool PERSISTENT, bool MIXED, bool MF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
     uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
     Geo<double> G{P.mass, P.spin, &rhs_count};
     LaneB L;
     lane_lds(L.s, L.k1);
+    if (MF) {
+        extern __shared__ double kp_lds[];
+        L.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
+    }
     bool has = false;
     if (PERSISTENT) {
         bool exhausted = false;
@@ -739,23 +871,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
                 if (!__ballot(need)) break;
                 int pix = 0;
                 if (fetch_pixel(&P.queue[1], need, (uint32_t)P.npix, pix)) {
-                    has = lane_b_init(P, pix, L);
+                    has = lane_b_init<MF>(P, pix, L);
                 } else if (need) {
                     exhausted = true;
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_b_tick<MIXED>(P, G, L, coeff, transfer, steps_b, guard)) {
-                lane_b_finish<MIXED>(P, L, 1);
+            if (has && pass_b_tick<MIXED, MF>(P, G, L, coeff, transfer, steps_b, guard)) {
+                lane_b_finish<MIXED, MF>(P, L, 1);
                 has = false;
             }
         }
     } else {
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-        if (pix < P.npix && lane_b_init(P, pix, L)) {
-            while (!pass_b_tick<MIXED>(P, G, L, coeff, transfer, steps_b, guard)) {
+        if (pix < P.npix && lane_b_init<MF>(P, pix, L)) {
+            while (!pass_b_tick<MIXED, MF>(P, G, L, coeff, transfer, steps_b, guard)) {
             }
-            lane_b_finish<MIXED>(P, L, 1);
+            lane_b_finish<MIXED, MF>(P, L, 1);
         }
     }
     flush_counters(P, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
@@ -815,19 +947,28 @@ struct Occupancy {
 template <bool PERSISTENT> const void* pass_a_fn(bool mixed) {
     return mixed ? (const void*)pass_a_kernel<PERSISTENT, true> : (const void*)pass_a_kernel<PERSISTENT, false>;
 }
-template <bool PERSISTENT> const void* pass_b_fn(bool mixed) {
-    return mixed ? (const void*)pass_b_kernel<PERSISTENT, true> : (const void*)pass_b_kernel<PERSISTENT, false>;
+template <bool PERSISTENT> const void* pass_b_fn(bool mixed, bool mf) {
+    if (mf)
+        return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, true>
+                     : (const void*)pass_b_kernel<PERSISTENT, false, true>;
+    return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, false>
+                 : (const void*)pass_b_kernel<PERSISTENT, false, false>;
 }
 
-// LDS bytes per block: 32 doubles per lane (state + k1 scratch).
-size_t lds_bytes(int block) { return KP_LANE_LDS ? size_t(32) * size_t(block) * sizeof(double) : 0; }
+// LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
+// plus 4 doubles per lane and output frequency for multi-frequency Stokes.
+size_t lds_bytes(int block, int nfreq_mf) {
+    return (size_t(KP_LANE_LDS ? 32 : 0) + size_t(4 * nfreq_mf)) * size_t(block) * sizeof(double);
+}
 
 std::mutex g_occ_mutex;
-std::vector<std::pair<long long, Occupancy>> g_occ;  // key: (device * 4096 + block) * 2 + mixed
+std::vector<std::pair<long long, Occupancy>> g_occ;  // key: ((device * 4096 + block) * 2 + mixed) * 32 + nfreq_mf
 
-kp_status occupancy_for(int device, int block, bool mixed, Occupancy& occ) {
+// nfreq_mf: output frequencies of a multi-frequency pass B, 0 for the
+// single-frequency kernel.
+kp_status occupancy_for(int device, int block, bool mixed, int nfreq_mf, Occupancy& occ) {
     std::lock_guard<std::mutex> lock(g_occ_mutex);
-    const long long key = ((long long)device * 4096 + block) * 2 + (mixed ? 1 : 0);
+    const long long key = (((long long)device * 4096 + block) * 2 + (mixed ? 1 : 0)) * 32 + nfreq_mf;
     for (auto& e : g_occ)
         if (e.first == key) {
             occ = e.second;
@@ -837,8 +978,9 @@ kp_status occupancy_for(int device, int block, bool mixed, Occupancy& occ) {
     hipDeviceProp_t prop;
     KP_HIP(hipGetDeviceProperties(&prop, device));
     o.cus = prop.multiProcessorCount;
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed), block, lds_bytes(block)));
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed), block, lds_bytes(block)));
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed), block, lds_bytes(block, 0)));
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed, nfreq_mf > 0), block,
+                                                        lds_bytes(block, nfreq_mf)));
     o.blocks_a = std::max(o.blocks_a, 1);
     o.blocks_b = std::max(o.blocks_b, 1);
     g_occ.push_back({key, o});
@@ -932,9 +1074,44 @@ static void prepare_mixed(Params& P) {
     P.spin_f = float(P.spin);
 }
 
-extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const kp_riaf_model* model,
-                                          const kp_run_options* options, kp_image_data* out,
-                                          kp_image_stats* stats, int perturbation, double log_scale) {
+// Frequency lists of a multi-frequency image (host copies); nfreq = 0 for
+// the single-frequency image at model->freq_cgs.
+struct FreqLists {
+    int nfreq = 0, nctrl = 0;
+    const double* freqs = nullptr;
+    const double* ctrl = nullptr;
+};
+
+static kp_status prepare_frequencies(const kp_riaf_model& m, const FreqLists& fl, Params& P) {
+    if (fl.nfreq < 1 || fl.nfreq > KP_MAX_FREQUENCIES || !fl.freqs)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "frequencies: 1 to KP_MAX_FREQUENCIES values required");
+    if (fl.nctrl < 1 || fl.nctrl > KP_MAX_FREQUENCIES || !fl.ctrl)
+        return kp_internal::fail(KP_INVALID_ARGUMENT,
+                                 "step-control frequencies: 1 to KP_MAX_FREQUENCIES values required");
+    P.nfreq = fl.nfreq;
+    P.nctrl = fl.nctrl;
+    const double cl = KP_CL;
+    const double l_unit = 6.67430e-8 * m.mbh_solar * 1.98847e33 / (cl * cl);
+    auto fill = [&](const double* src, int n, double* f, double* dls, float* bconst, double* inv3) {
+        for (int i = 0; i < n; ++i) {
+            if (!(src[i] > 0.0) || !std::isfinite(src[i])) return false;
+            const double nu = std::max(src[i], 1.0);
+            f[i] = src[i];
+            dls[i] = l_unit / nu;  // the model copy's dlambda_scale (src/model/riaf.hpp:76-85)
+            bconst[i] = float(2.0 * KP_HPL * nu * nu * nu / (cl * cl));
+            if (inv3) inv3[i] = 1.0 / (nu * nu * nu);
+        }
+        return true;
+    };
+    if (!fill(fl.freqs, fl.nfreq, P.freqs, P.dls_f, P.bconst_f, P.inv_nu3_f) ||
+        !fill(fl.ctrl, fl.nctrl, P.ctrl, P.dls_c, P.bconst_c, nullptr))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "frequencies must be positive and finite");
+    return KP_OK;
+}
+
+static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl,
+                           const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
+                           int perturbation, double log_scale) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -952,6 +1129,11 @@ extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const 
             return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
         prepare_mixed(P);
         P.fp64_radius = opt.fp64_radius;
+    }
+    const bool mf = fl.nfreq > 0;
+    if (mf) {
+        st = prepare_frequencies(*model, fl, P);
+        if (st != KP_OK) return st;
     }
     P.out = *out;
     P.persistent = opt.persistent ? 1 : 0;
@@ -981,9 +1163,10 @@ extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const 
     if (block % 64 != 0 || block > 256)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
     Occupancy occ;
-    st = occupancy_for(device, block, mixed, occ);
+    st = occupancy_for(device, block, mixed, mf ? P.nfreq : 0, occ);
     if (st != KP_OK) return st;
-    const size_t smem = lds_bytes(block);
+    const size_t smem_a = lds_bytes(block, 0);
+    const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0);
     const int nblocks_flat = (int)((np + block - 1) / block);
     int grid_a = nblocks_flat, grid_b = nblocks_flat;
     if (P.persistent) {
@@ -1003,12 +1186,12 @@ extern "C" kp_status kp_internal_run_riaf(const kp_pass_a_params* params, const 
     void* args[] = {&P};
     if (P.npix > 0) {
         const void* fn = P.persistent ? pass_a_fn<true>(mixed) : pass_a_fn<false>(mixed);
-        KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem, stream));
+        KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem_a, stream));
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
     if (P.npix > 0) {
-        const void* fn = P.persistent ? pass_b_fn<true>(mixed) : pass_b_fn<false>(mixed);
-        KP_HIP(hipLaunchKernel(fn, dim3(grid_b), dim3(block), args, smem, stream));
+        const void* fn = P.persistent ? pass_b_fn<true>(mixed, mf) : pass_b_fn<false>(mixed, mf);
+        KP_HIP(hipLaunchKernel(fn, dim3(grid_b), dim3(block), args, smem_b, stream));
     }
     if (stats) {
         KP_HIP(hipEventRecord(ev[2], stream));
@@ -1045,22 +1228,41 @@ extern "C" kp_status kp_run_riaf_image(const kp_pass_a_params* params, const kp_
                                        int32_t radiation_substeps, const kp_run_options* options,
                                        kp_image_data* out, kp_image_stats* stats) {
     (void)radiation_substeps;
-    return kp_internal_run_riaf(params, model, options, out, stats, 0, 0.0);
+    return run_image(params, model, FreqLists{}, options, out, stats, 0, 0.0);
 }
 
-extern "C" kp_status kp_run_riaf_image_host(const kp_pass_a_params* params, const kp_riaf_model* model,
-                                            int32_t radiation_substeps, const kp_run_options* options,
-                                            kp_image_data* out, kp_image_stats* stats) {
+extern "C" kp_status kp_run_riaf_multifrequency_image(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                                      int32_t radiation_substeps, const double* frequencies,
+                                                      int32_t nfreq, const double* step_control_frequencies,
+                                                      int32_t step_control_nfreq, const kp_run_options* options,
+                                                      kp_image_data* out, kp_image_stats* stats) {
     (void)radiation_substeps;
+    FreqLists fl;
+    fl.nfreq = nfreq;
+    fl.freqs = frequencies;
+    fl.nctrl = step_control_nfreq;
+    fl.ctrl = step_control_frequencies;
+    if (nfreq < 1) return kp_internal::fail(KP_INVALID_ARGUMENT, "nfreq must be at least 1");
+    return run_image(params, model, fl, options, out, stats, 0, 0.0);
+}
+
+// Host-output wrapper shared by the single- and multi-frequency entries:
+// device buffers for the selected outputs, the run, copies back.
+static kp_status run_image_host(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl,
+                                const kp_run_options* options, kp_image_data* out, kp_image_stats* stats) {
     kp_internal::clear_error();
     if (!params || !out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null argument");
     const size_t np = (size_t)std::max(0, params->camera.nx) * (size_t)std::max(0, params->camera.ny);
+    const size_t nf = fl.nfreq > 0 ? (size_t)std::min(fl.nfreq, KP_MAX_FREQUENCIES) : 1;
     double* const* host_real[15] = {&out->image_i, &out->image_q, &out->image_u, &out->image_v,
                                     &out->closure_x, &out->closure_k, &out->final_null, &out->frame_error,
                                     &out->det_r, &out->overlap_r11, &out->overlap_r12, &out->overlap_r21,
                                     &out->overlap_r22, &out->basis_identity_error, &out->basis_rotation_angle};
     int32_t* const* host_int[3] = {&out->pass_a_steps, &out->steps, &out->reason};
-    const size_t bytes = np * (15 * sizeof(double) + 3 * sizeof(int32_t));
+    auto real_len = [&](int i) { return i < 4 ? nf * np : np; };
+    size_t bytes = 0;
+    for (int i = 0; i < 15; ++i) bytes += real_len(i) * sizeof(double);
+    bytes += 3 * np * sizeof(int32_t);
     void* dbuf = nullptr;
     if (np > 0 && hipMalloc(&dbuf, bytes) != hipSuccess)
         return kp_internal::fail(KP_OUT_OF_MEMORY, "hipMalloc of device outputs failed");
@@ -1074,7 +1276,7 @@ extern "C" kp_status kp_run_riaf_image_host(const kp_pass_a_params* params, cons
     char* b = static_cast<char*>(dbuf);
     for (int i = 0; i < 15; ++i) {
         if (*host_real[i]) *dev_real[i] = reinterpret_cast<double*>(b);
-        b += np * sizeof(double);
+        b += real_len(i) * sizeof(double);
     }
     for (int i = 0; i < 3; ++i) {
         if (*host_int[i]) *dev_int[i] = reinterpret_cast<int32_t*>(b);
@@ -1083,13 +1285,13 @@ extern "C" kp_status kp_run_riaf_image_host(const kp_pass_a_params* params, cons
     kp_run_options opt;
     kp_run_options_default(&opt);
     if (options) opt = *options;
-    kp_status st = kp_internal_run_riaf(params, model, &opt, &dev, stats, 0, 0.0);
+    kp_status st = run_image(params, model, fl, &opt, &dev, stats, 0, 0.0);
     if (st == KP_OK) {
         if (hipStreamSynchronize(static_cast<hipStream_t>(opt.stream)) != hipSuccess) {
             st = kp_internal::fail(KP_HIP_ERROR, "stream synchronisation failed");
         } else {
             for (int i = 0; i < 15 && st == KP_OK; ++i)
-                if (*host_real[i] && hipMemcpy(*host_real[i], *dev_real[i], np * sizeof(double),
+                if (*host_real[i] && hipMemcpy(*host_real[i], *dev_real[i], real_len(i) * sizeof(double),
                                                hipMemcpyDeviceToHost) != hipSuccess)
                     st = kp_internal::fail(KP_HIP_ERROR, "device to host copy failed");
             for (int i = 0; i < 3 && st == KP_OK; ++i)
@@ -1100,4 +1302,25 @@ extern "C" kp_status kp_run_riaf_image_host(const kp_pass_a_params* params, cons
     }
     if (dbuf) (void)hipFree(dbuf);
     return st;
+}
+
+extern "C" kp_status kp_run_riaf_image_host(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                            int32_t radiation_substeps, const kp_run_options* options,
+                                            kp_image_data* out, kp_image_stats* stats) {
+    (void)radiation_substeps;
+    return run_image_host(params, model, FreqLists{}, options, out, stats);
+}
+
+extern "C" kp_status kp_run_riaf_multifrequency_image_host(
+    const kp_pass_a_params* params, const kp_riaf_model* model, int32_t radiation_substeps,
+    const double* frequencies, int32_t nfreq, const double* step_control_frequencies, int32_t step_control_nfreq,
+    const kp_run_options* options, kp_image_data* out, kp_image_stats* stats) {
+    (void)radiation_substeps;
+    if (nfreq < 1) return kp_internal::fail(KP_INVALID_ARGUMENT, "nfreq must be at least 1");
+    FreqLists fl;
+    fl.nfreq = nfreq;
+    fl.freqs = frequencies;
+    fl.nctrl = step_control_nfreq;
+    fl.ctrl = step_control_frequencies;
+    return run_image_host(params, model, fl, options, out, stats);
 }

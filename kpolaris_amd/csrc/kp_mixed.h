@@ -124,11 +124,12 @@ struct RiafMixed {
 // bconst / expm1(x) (Kirchhoff's law in the rescaled units), and
 // rho_h = rho_phys * g * L.  Follows
 // thermal_synchrotron_magnetic_basis_coefficients_fit (thermal_synchrotron.hpp:512-616).
-KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float g, float ne, float thetae, float b,
+KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float freq, float bconst, float g, float ne,
+                                  float thetae, float b,
                                   float sin_theta_in, float cos_theta, Coeffs<float>& c) {
     coeffs_zero(c);
     const RiafDev<float>& M = R.f;
-    const float nu = fmaxf(M.freq_cgs * g, 1.0f);
+    const float nu = fmaxf(freq * g, 1.0f);
     if (!(ne > 0.0f) || !(thetae > 0.0f) || !(b > 0.0f)) return;
     const float pi = float(PI_D);
     const float ee = float(KP_EE), me = float(KP_ME), cl = float(KP_CL);
@@ -184,7 +185,7 @@ KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float g, float ne, float t
     float aI = 0.0f, aQ = 0.0f, aV = 0.0f;
     {
         const float x = R.x0 * nu / thetae;
-        const float sp = R.bconst / expm1f(x);
+        const float sp = bconst / expm1f(x);
         if (sp > 0.0f) {
             aI = jIh / sp;
             aQ = jQh / sp;
@@ -227,9 +228,12 @@ KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float g, float ne, float t
     c.rV = M.faraday_scale * rhoV * rscale;
 }
 
-// RIAF coefficients in FP32, rescaled (riaf.hpp:410-549).
-KP_DEV void riaf_coefficients_scaled(const RiafMixed& R, const St<double>& sd, Coeffs<float>& c) {
-    coeffs_zero(c);
+// The frequency-independent part of the RIAF coefficients in FP32
+// (riaf.hpp:410-500, 529-547; the FP64 form is riaf_local in kp_riaf.h).
+KP_DEV void riaf_local_scaled(const RiafMixed& R, const St<double>& sd, RiafLocal<float>& L) {
+    L.valid = 0;
+    L.aligned = 0;
+    L.has_chi = 0;
     const RiafDev<float>& M = R.f;
     St<float> s;
 #pragma unroll
@@ -247,10 +251,8 @@ KP_DEV void riaf_coefficients_scaled(const RiafMixed& R, const St<double>& sd, C
     if (!(ne_norm > 0.0f)) return;
     float ne = ne_norm * M.ne_unit;
     float thetae = fmaxf(M.Te0 * powf(r, M.pow_T) * M.thetae_unit, M.min_thetae);
-    const float mp = 1.67262192369e-24f;
-    // 8 pi 0.1 m_p c^2 / 6 folded into one constant to stay in range
+    // 8 pi 0.1 m_p c^2 / 6 folded into one constant (in range for FP32)
     const float bk = float(8.0 * PI_D * 0.1 * 1.67262192369e-24 * KP_CL * KP_CL / 6.0);
-    (void)mp;
     float b = fmaxf(sqrtf(fmaxf(bk * ne / r, 0.0f)), 1e-12f);
     ne *= M.ne_factor;
     thetae *= M.thetae_factor;
@@ -267,36 +269,62 @@ KP_DEV void riaf_coefficients_scaled(const RiafMixed& R, const St<double>& sd, C
     glower(g, bc, bcov);
     const float kdotb = k[0] * bcov[0] + k[1] * bcov[1] + k[2] * bcov[2] + k[3] * bcov[3];
     const float cos_theta = clampv(kdotb / fmaxf(gscale, 1e-30f), -1.0f, 1.0f);
-    const float sin_theta = fmaxf(sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta)), M.min_sin_theta);
-    if (cos_theta == 1.0f || cos_theta == -1.0f) {
-        // exactly field-aligned: only rho_V survives; negligible measure, kept for completeness
-        Coeffs<float> tmp;
-        thermal_coeffs_scaled(R, gscale, ne, thetae, b, M.min_sin_theta, cos_theta, tmp);
-        c.rV = M.faraday_rotation ? tmp.rV : 0.0f;
-        return;
-    }
-    Coeffs<float> mc;
-    thermal_coeffs_scaled(R, gscale, ne, thetae, b, sin_theta, cos_theta, mc);
+    L.valid = 1;
+    L.ne = ne;
+    L.thetae = thetae;
+    L.b = b;
+    L.nu_scale = gscale;
+    L.cos_theta = cos_theta;
+    L.sin_theta = fmaxf(sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta)), M.min_sin_theta);
+    L.aligned = cos_theta == 1.0f || cos_theta == -1.0f;
+    if (L.aligned) return;
     const float ukc = gdot(g, u, k);
     const float b1 = screen_inner(g, u, k, s.v + 8, bc, ukc);
     const float b2 = screen_inner(g, u, k, s.v + 12, bc, ukc);
     const float bp2 = b1 * b1 + b2 * b2;
+    if (bp2 > 1e-30f) {
+        const float ib = 1.0f / bp2;
+        L.has_chi = 1;
+        L.cos2chi = (b2 * b2 - b1 * b1) * ib;
+        L.sin2chi = -2.0f * b1 * b2 * ib;
+    }
+}
+
+// Rescaled FP32 coefficients at observing frequency freq; bconst = 2 h freq^3 / c^2.
+KP_DEV void riaf_coeffs_scaled_at(const RiafMixed& R, const RiafLocal<float>& L, float freq, float bconst,
+                                  Coeffs<float>& c) {
+    coeffs_zero(c);
+    if (!L.valid) return;
+    const RiafDev<float>& M = R.f;
+    if (L.aligned) {
+        // exactly field-aligned: only rho_V survives (riaf.hpp:505-512)
+        Coeffs<float> tmp;
+        thermal_coeffs_scaled(R, freq, bconst, L.nu_scale, L.ne, L.thetae, L.b, M.min_sin_theta, L.cos_theta, tmp);
+        c.rV = M.faraday_rotation ? tmp.rV : 0.0f;
+        return;
+    }
+    Coeffs<float> mc;
+    thermal_coeffs_scaled(R, freq, bconst, L.nu_scale, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, mc);
     c.jI = mc.jI;
     c.jV = mc.jV;
     c.aI = mc.aI;
     c.aV = mc.aV;
     c.rV = M.faraday_rotation ? mc.rV : 0.0f;
-    if (bp2 > 1e-30f) {
-        const float ib = 1.0f / bp2;
-        const float cos2chi = (b2 * b2 - b1 * b1) * ib;
-        const float sin2chi = -2.0f * b1 * b2 * ib;
-        c.jQ = mc.jQ * cos2chi;
-        c.jU = mc.jQ * sin2chi;
-        c.aQ = mc.aQ * cos2chi;
-        c.aU = mc.aQ * sin2chi;
-        c.rQ = mc.rQ * cos2chi;
-        c.rU = mc.rQ * sin2chi;
+    if (L.has_chi) {
+        c.jQ = mc.jQ * L.cos2chi;
+        c.jU = mc.jQ * L.sin2chi;
+        c.aQ = mc.aQ * L.cos2chi;
+        c.aU = mc.aQ * L.sin2chi;
+        c.rQ = mc.rQ * L.cos2chi;
+        c.rU = mc.rQ * L.sin2chi;
     }
+}
+
+// RIAF coefficients in FP32, rescaled (riaf.hpp:410-549).
+KP_DEV void riaf_coefficients_scaled(const RiafMixed& R, const St<double>& sd, Coeffs<float>& c) {
+    RiafLocal<float> L;
+    riaf_local_scaled(R, sd, L);
+    riaf_coeffs_scaled_at(R, L, R.f.freq_cgs, R.bconst, c);
 }
 
 // FP32 Strang step with expm1-based attenuation/source factors (exact for all

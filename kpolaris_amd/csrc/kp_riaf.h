@@ -145,9 +145,24 @@ KP_DEV T screen_inner(const BLGcov<T>& g, const T* u, const T* k, const T* a, co
     return gdot(g, a, b) + gdot(g, u, a) * gdot(g, u, b) - gdot(g, eK, a) * gdot(g, eK, b);
 }
 
-// RIAFAnalyticRadiationModel::coefficients for the BL metric (riaf.hpp:410-549).
-template <class T> KP_DEV void riaf_coefficients(const RiafDev<T>& M, const St<T>& s, Coeffs<T>& c) {
-    coeffs_zero(c);
+// The frequency-independent part of RIAFAnalyticRadiationModel::coefficients
+// at one sample state: plasma scalars, fluid-frame frequency ratio, field
+// angle and the screen rotation of the field.  Only nu = freq * nu_scale and
+// dlambda_scale depend on the observing frequency, so a multi-frequency step
+// evaluates this once and riaf_coeffs_at once per frequency, with results
+// identical to re-running the whole model per frequency (pass_b.hpp:1422-1430).
+template <class T> struct RiafLocal {
+    int valid;    // inside the model domain with positive density
+    int aligned;  // k exactly along b: only rho_V survives (riaf.hpp:505-512)
+    int has_chi;  // the field has a screen projection
+    T ne, thetae, b, nu_scale, cos_theta, sin_theta, cos2chi, sin2chi;
+};
+
+// riaf.hpp:410-500 and 529-547 without the frequency.
+template <class T> KP_DEV void riaf_local(const RiafDev<T>& M, const St<T>& s, RiafLocal<T>& L) {
+    L.valid = 0;
+    L.aligned = 0;
+    L.has_chi = 0;
     const T* x = s.v;
     const T* k = s.v + 4;
     const T r = maxv(x[1], M.min_radius);
@@ -183,39 +198,64 @@ template <class T> KP_DEV void riaf_coefficients(const RiafDev<T>& M, const St<T
     glower(g, bc, bcov);
     const T kdotb = k[0] * bcov[0] + k[1] * bcov[1] + k[2] * bcov[2] + k[3] * bcov[3];
     const T cos_theta = clampv(kdotb / maxv(nu_scale, T(1e-30)), T(-1), T(1));
-    const T sin_theta = maxv(sqrt(maxv(T(0), T(1) - cos_theta * cos_theta)), M.min_sin_theta);
-    const T nu = maxv(M.freq_cgs * nu_scale, T(1));
+    L.valid = 1;
+    L.ne = ne;
+    L.thetae = thetae;
+    L.b = b;
+    L.nu_scale = nu_scale;
+    L.cos_theta = cos_theta;
+    L.sin_theta = maxv(sqrt(maxv(T(0), T(1) - cos_theta * cos_theta)), M.min_sin_theta);
     // theta = acos(cos_theta) is 0 or pi exactly when cos_theta = +-1.
-    if (cos_theta == T(1) || cos_theta == T(-1)) {
-        const T theta = cos_theta == T(1) ? T(0) : T(PI_D);
-        c.rV = M.faraday_scale * rho_v_aligned(nu, ne, thetae, b, theta, M.min_sin_theta) * nu;
-        if (!M.faraday_rotation) c.rV = T(0);
-        return;
-    }
-    Coeffs<T> mc;
-    thermal_coeffs(M.fit, nu, ne, thetae, b, sin_theta, cos_theta, M.max_pol_frac, M.min_sin_theta,
-                   M.emission_scale, M.absorption_scale, M.faraday_scale, mc);
+    L.aligned = cos_theta == T(1) || cos_theta == T(-1);
+    if (L.aligned) return;
     // screen projections of the field: e2 along projected B (riaf.hpp:529-547)
     const T ukc = gdot(g, u, k);
     const T b1 = screen_inner(g, u, k, s.v + 8, bc, ukc);
     const T b2 = screen_inner(g, u, k, s.v + 12, bc, ukc);
     const T bp2 = b1 * b1 + b2 * b2;
+    if (bp2 > T(1e-30)) {
+        const T ib = T(1) / bp2;
+        L.has_chi = 1;
+        L.cos2chi = (b2 * b2 - b1 * b1) * ib;
+        L.sin2chi = -T(2) * b1 * b2 * ib;
+    }
+}
+
+// riaf.hpp:500-549 at observing frequency freq.
+template <class T>
+KP_DEV void riaf_coeffs_at(const RiafDev<T>& M, const RiafLocal<T>& L, T freq, Coeffs<T>& c) {
+    coeffs_zero(c);
+    if (!L.valid) return;
+    const T nu = maxv(freq * L.nu_scale, T(1));
+    if (L.aligned) {
+        const T theta = L.cos_theta == T(1) ? T(0) : T(PI_D);
+        c.rV = M.faraday_scale * rho_v_aligned(nu, L.ne, L.thetae, L.b, theta, M.min_sin_theta) * nu;
+        if (!M.faraday_rotation) c.rV = T(0);
+        return;
+    }
+    Coeffs<T> mc;
+    thermal_coeffs(M.fit, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, M.max_pol_frac, M.min_sin_theta,
+                   M.emission_scale, M.absorption_scale, M.faraday_scale, mc);
     c.jI = mc.jI;
     c.jV = mc.jV;
     c.aI = mc.aI;
     c.aV = mc.aV;
     c.rV = M.faraday_rotation ? mc.rV : T(0);
-    if (bp2 > T(1e-30)) {
-        const T ib = T(1) / bp2;
-        const T cos2chi = (b2 * b2 - b1 * b1) * ib;
-        const T sin2chi = -T(2) * b1 * b2 * ib;
-        c.jQ = mc.jQ * cos2chi;
-        c.jU = mc.jQ * sin2chi;
-        c.aQ = mc.aQ * cos2chi;
-        c.aU = mc.aQ * sin2chi;
-        c.rQ = mc.rQ * cos2chi;
-        c.rU = mc.rQ * sin2chi;
+    if (L.has_chi) {
+        c.jQ = mc.jQ * L.cos2chi;
+        c.jU = mc.jQ * L.sin2chi;
+        c.aQ = mc.aQ * L.cos2chi;
+        c.aU = mc.aQ * L.sin2chi;
+        c.rQ = mc.rQ * L.cos2chi;
+        c.rU = mc.rQ * L.sin2chi;
     }
+}
+
+// RIAFAnalyticRadiationModel::coefficients for the BL metric (riaf.hpp:410-549).
+template <class T> KP_DEV void riaf_coefficients(const RiafDev<T>& M, const St<T>& s, Coeffs<T>& c) {
+    RiafLocal<T> L;
+    riaf_local(M, s, L);
+    riaf_coeffs_at(M, L, M.freq_cgs, c);
 }
 
 }  // namespace kpd

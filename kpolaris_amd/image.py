@@ -51,6 +51,10 @@ def library():
         fn.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(abi.RunOptions),
                        P(abi.ImageData), P(abi.ImageStats)]
         fn.restype = C.c_int32
+    for fn in (L.kp_run_riaf_multifrequency_image, L.kp_run_riaf_multifrequency_image_host):
+        fn.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(C.c_double), C.c_int32,
+                       P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData), P(abi.ImageStats)]
+        fn.restype = C.c_int32
     L.kp_pass_a_params_default.argtypes = [P(abi.PassAParams)]
     L.kp_riaf_model_default.argtypes = [P(abi.RIAFModel)]
     L.kp_run_options_default.argtypes = [P(abi.RunOptions)]
@@ -147,12 +151,14 @@ def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, bl
     return o
 
 
-def alloc_device_image(npix: int, device="cuda", fields=abi.IMAGE_FIELDS):
+def alloc_device_image(npix: int, device="cuda", fields=abi.IMAGE_FIELDS, nfreq: int = 1):
+    """Device output tensors; Stokes fields hold nfreq * npix values ([nfreq][npix])."""
     import torch
     out = {}
     for n in fields:
         dt = torch.int32 if n in abi.IMAGE_INT_FIELDS else torch.float64
-        out[n] = torch.empty(npix, dtype=dt, device=device)
+        size = npix * nfreq if n in abi.STOKES_FIELDS else npix
+        out[n] = torch.empty(size, dtype=dt, device=device)
     return out
 
 
@@ -203,3 +209,30 @@ def run_riaf_image_host(params: abi.PassAParams, model: abi.RIAFModel, *, radiat
     _check(library().kp_run_riaf_image_host(C.byref(params), C.byref(model), radiation_substeps,
                                             C.byref(opts), C.byref(img), C.byref(st)))
     return arrays, st.as_dict()
+
+
+def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel, frequencies,
+                                  step_control_frequencies=None, *, radiation_substeps: int = 1,
+                                  precision: int = abi.PRECISION_FP64, persistent: bool = True,
+                                  block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
+                                  fields=abi.IMAGE_FIELDS, with_stats: bool = True,
+                                  fp64_radius: float | None = None):
+    """Multi-frequency image on shared geodesics on the current HIP device.
+    Stokes tensors are [nfreq * npix] (frequency-major); step control uses
+    step_control_frequencies (default: the output frequencies)."""
+    import torch
+    f = np.ascontiguousarray(frequencies, dtype=np.float64)
+    c = np.ascontiguousarray(frequencies if step_control_frequencies is None else step_control_frequencies,
+                             dtype=np.float64)
+    npix = params.camera.nx * params.camera.ny
+    if out is None:
+        out = alloc_device_image(npix, fields=fields, nfreq=len(f))
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius)
+    st = abi.ImageStats()
+    dp = C.POINTER(C.c_double)
+    _check(library().kp_run_riaf_multifrequency_image(
+        C.byref(params), C.byref(model), radiation_substeps, f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp),
+        len(c), C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
+    return out, (st.as_dict() if with_stats else None)

@@ -102,6 +102,11 @@ def lib():
                                              C.POINTER(Perturbation), i32, i32, i32, i32,
                                              C.POINTER(abi.ImageData), C.POINTER(Stats)]
     L.kpo_run_riaf_image_strided.restype = i32
+    dp = C.POINTER(C.c_double)
+    L.kpo_run_riaf_multifrequency_image.argtypes = [
+        C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.POINTER(Perturbation), dp, i32, dp, i32,
+        i32, i32, i32, C.POINTER(abi.ImageData), C.POINTER(Stats)]
+    L.kpo_run_riaf_multifrequency_image.restype = i32
     _lib = L
     return L
 
@@ -170,6 +175,36 @@ def run_riaf_image_strided(params: abi.PassAParams, model: abi.RIAFModel, offset
         nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
     rc = L.kpo_run_riaf_image_strided(C.byref(params), C.byref(model), C.byref(pert), offset,
                                       stride, count, nthreads, C.byref(img), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
+    return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}
+
+
+MAX_FREQUENCIES = 16
+
+
+def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel, frequencies,
+                                  step_control_frequencies=None, pixel_begin: int = 0, pixel_end: int = -1,
+                                  nthreads: int = 0, perturbation=None):
+    """Oracle multi-frequency image: Stokes arrays of shape [nfreq * npix]
+    (frequency-major), per-pixel diagnostics [npix]; plus work counters."""
+    L = lib()
+    npix = params.camera.nx * params.camera.ny
+    f = np.ascontiguousarray(frequencies, dtype=np.float64)
+    c = np.ascontiguousarray(frequencies if step_control_frequencies is None else step_control_frequencies,
+                             dtype=np.float64)
+    arrays = alloc_image(npix)
+    for n in ("image_i", "image_q", "image_u", "image_v"):
+        arrays[n] = np.zeros(len(f) * npix, dtype=np.float64)
+    img = image_struct(arrays)
+    st = Stats()
+    pert = perturbation if perturbation is not None else Perturbation()
+    if nthreads <= 0:
+        nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
+    dp = C.POINTER(C.c_double)
+    rc = L.kpo_run_riaf_multifrequency_image(C.byref(params), C.byref(model), C.byref(pert),
+                                             f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp), len(c),
+                                             pixel_begin, pixel_end, nthreads, C.byref(img), C.byref(st))
     if rc != 0:
         raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
     return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}

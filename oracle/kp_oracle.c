@@ -1490,7 +1490,7 @@ static double max_abs_diff4(const double* a, const double* b) {
 }
 
 typedef struct pixel_result {
-    kpo_stokes observed;
+    kpo_stokes observed[KPO_MAX_FREQUENCIES];
     double r11, r12, r21, r22;
     double closure_x, closure_k, final_null, frame_error;
     int pass_a_steps, steps, reason;
@@ -1506,11 +1506,35 @@ static void sample_coefficients(const kp_pass_a_params* p, const kp_riaf_model* 
     if (!p->emission_selection.faraday_rotation) c->rV = 0.0;
 }
 
+/* The frequency lists of one pass B: output frequencies and the frequencies
+ * whose coefficients drive the substep control.  A single-frequency image is
+ * the list {freq_cgs} for both (pass_b.hpp:515-722 is the nfreq = 1 case of
+ * pass_b.hpp:1130-1500); distinct lists follow the control variant
+ * (pass_b.hpp:1505-1895). */
+typedef struct freq_plan {
+    const double* freqs;
+    int nfreq;
+    const double* ctrl;
+    int nctrl;
+    int same;  /* ctrl == freqs elementwise: coefficients are reused */
+} freq_plan;
+
+/* Coefficients of the model copy with freq_cgs = f, and its dlambda_scale. */
+static double coeffs_at_frequency(const kp_pass_a_params* p, const kp_riaf_model* model, double f,
+                                  const kpo_state* sample, int inside, const kpo_perturbation* pert,
+                                  kpo_coeffs* c, kpo_stats* stats) {
+    kp_riaf_model mf = *model;
+    mf.freq_cgs = f;
+    memset(c, 0, sizeof(*c));
+    if (inside) sample_coefficients(p, &mf, sample, pert, c, stats);
+    return riaf_dlambda_scale(&mf);
+}
+
 /* trace_pass_b_segment_model_endpoint_pixel_metric (pass_b.hpp:515-722). */
 static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
-                   const kpo_perturbation* pert, const metric_ctx* m, const kpo_state* camera,
-                   const kpo_state* endpoint, int pass_a_steps, int pass_a_reason, int valid,
-                   pixel_result* out, kpo_stats* stats) {
+                   const kpo_perturbation* pert, const freq_plan* fp, const metric_ctx* m,
+                   const kpo_state* camera, const kpo_state* endpoint, int pass_a_steps,
+                   int pass_a_reason, int valid, pixel_result* out, kpo_stats* stats) {
     memset(out, 0, sizeof(*out));
     out->r11 = 1.0;
     out->r22 = 1.0;
@@ -1525,8 +1549,8 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
         return;
     }
     const adaptive_control tcontrol = {p->adaptive_tolerance, p->min_step, p->max_step, 10};
-    kpo_stokes stokes = {0.0, 0.0, 0.0, 0.0};
-    const double dls = riaf_dlambda_scale(model);
+    kpo_stokes stokes[KPO_MAX_FREQUENCIES];
+    memset(stokes, 0, sizeof(stokes));
     int reached = 0;
     int steps = 0;
     int reason = pass_a_reason;
@@ -1594,19 +1618,23 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
             }
         }
         const double old_r = old.v[1], sample_r = sample.v[1], next_r = next.v[1];
-        kpo_coeffs c;
-        memset(&c, 0, sizeof(c));
-        if (sample.v[1] - p->inner_radius > 0.0 && sample_r < p->outer_radius)
-            sample_coefficients(p, model, &sample, pert, &c, stats);
+        const int inside = sample.v[1] - p->inner_radius > 0.0 && sample_r < p->outer_radius;
         const double local_max_rs =
             (p->max_radiation_step > 0.0 &&
              (old_r <= p->outer_radius || sample_r <= p->outer_radius || next_r <= p->outer_radius))
                 ? p->max_radiation_step
                 : 0.0;
-        const int required = kpo_radiation_substep_count(&c, h, dls, 1, local_max_rs,
-                                                         p->max_radiation_depth,
-                                                         p->max_absorption_depth,
-                                                         p->max_faraday_depth);
+        /* step control: the largest substep count over the control frequencies */
+        kpo_coeffs cc[KPO_MAX_FREQUENCIES];
+        double dls_c[KPO_MAX_FREQUENCIES];
+        int required = 1;
+        for (int f = 0; f < fp->nctrl; ++f) {
+            dls_c[f] = coeffs_at_frequency(p, model, fp->ctrl[f], &sample, inside, pert, &cc[f], stats);
+            const int n = kpo_radiation_substep_count(&cc[f], h, dls_c[f], 1, local_max_rs,
+                                                      p->max_radiation_depth, p->max_absorption_depth,
+                                                      p->max_faraday_depth);
+            required = n > required ? n : required;
+        }
         if (required > 1 && h > eff_min * 1.0001) {
             h_current = maxv(eff_min, h / (double)required);
             cap = h_current;
@@ -1614,17 +1642,31 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
         }
         if (local_max_rs > 0.0 || p->max_radiation_depth > 0.0 || p->max_absorption_depth > 0.0 ||
             p->max_faraday_depth > 0.0) {
-            const int predictive = kpo_radiation_substep_count(&c, h_current, dls, 1, local_max_rs,
-                                                               p->max_radiation_depth,
-                                                               p->max_absorption_depth,
-                                                               p->max_faraday_depth);
+            int predictive = 1;
+            for (int f = 0; f < fp->nctrl; ++f) {
+                const int n = kpo_radiation_substep_count(&cc[f], h_current, dls_c[f], 1, local_max_rs,
+                                                          p->max_radiation_depth,
+                                                          p->max_absorption_depth,
+                                                          p->max_faraday_depth);
+                predictive = n > predictive ? n : predictive;
+            }
             cap = predictive > 1 ? maxv(eff_min, h_current / (double)predictive) : p->max_step;
         } else {
             cap = p->max_step;
         }
         if (h > 0.0) {
             if (stats) stats->transfer_steps++;
-            kpo_semi_analytic_step(&stokes, &c, absv(h) * dls);
+            for (int f = 0; f < fp->nfreq; ++f) {
+                kpo_coeffs c;
+                double dls;
+                if (fp->same) {
+                    c = cc[f];
+                    dls = dls_c[f];
+                } else {
+                    dls = coeffs_at_frequency(p, model, fp->freqs[f], &sample, inside, pert, &c, stats);
+                }
+                kpo_semi_analytic_step(&stokes[f], &c, absv(h) * dls);
+            }
         }
         state = next;
         steps += 1;
@@ -1637,7 +1679,9 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
     out->r12 = dot_g(g, camera->v + 8, state.v + 12);
     out->r21 = dot_g(g, camera->v + 12, state.v + 8);
     out->r22 = dot_g(g, camera->v + 12, state.v + 12);
-    kpo_transform_to_observer_basis(&stokes, out->r11, out->r12, out->r21, out->r22, &out->observed);
+    for (int f = 0; f < fp->nfreq; ++f)
+        kpo_transform_to_observer_basis(&stokes[f], out->r11, out->r12, out->r21, out->r22,
+                                        &out->observed[f]);
     out->closure_x = spatial_distance(state.v, camera->v);
     out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
     out->final_null = dot_g(g, state.v + 4, state.v + 4);
@@ -1652,12 +1696,16 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
     out->reason = reason;
 }
 
-static void store_pixel(kp_image_data* o, int p, const pixel_result* r) {
+/* Stokes arrays are [nfreq][nout] (frequency-major, src/image/multifrequency.hpp). */
+static void store_pixel(kp_image_data* o, int p, int nout, int nfreq, const pixel_result* r) {
 #define PUT(field, val) do { if (o->field) o->field[p] = (val); } while (0)
-    PUT(image_i, r->observed.I);
-    PUT(image_q, r->observed.Q);
-    PUT(image_u, r->observed.U);
-    PUT(image_v, r->observed.V);
+    for (int f = 0; f < nfreq; ++f) {
+        const size_t q = (size_t)f * (size_t)nout + (size_t)p;
+        if (o->image_i) o->image_i[q] = r->observed[f].I;
+        if (o->image_q) o->image_q[q] = r->observed[f].Q;
+        if (o->image_u) o->image_u[q] = r->observed[f].U;
+        if (o->image_v) o->image_v[q] = r->observed[f].V;
+    }
     PUT(closure_x, r->closure_x);
     PUT(closure_k, r->closure_k);
     PUT(final_null, r->final_null);
@@ -1676,18 +1724,14 @@ static void store_pixel(kp_image_data* o, int p, const pixel_result* r) {
 #undef PUT
 }
 
-/* Split pass A / pass B image (src/image/split_driver.hpp:41-104), each pixel
- * independent; OpenMP static schedule as the reference's RangePolicy. */
-kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* model,
-                             const kpo_perturbation* pert, int32_t pixel_begin,
-                             int32_t pixel_end, int32_t nthreads, kp_image_data* out,
-                             kpo_stats* stats) {
-    if (!p || !model || !out) return KP_INVALID_ARGUMENT;
-    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST || p->camera.model != KP_CAMERA_PINHOLE)
-        return KP_UNSUPPORTED;
-    const int npix = p->camera.nx * p->camera.ny;
-    const int begin = pixel_begin < 0 ? 0 : pixel_begin;
-    const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
+/* One pixel set: pixel = first + i * stride for i < count, outputs at index
+ * i (compact) or at the pixel index.  Split pass A / pass B per pixel
+ * (src/image/split_driver.hpp:41-104); OpenMP static schedule as the
+ * reference's RangePolicy. */
+static kp_status run_pixels(const kp_pass_a_params* p, const kp_riaf_model* model,
+                            const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
+                            int count, int compact, int nout, int nthreads, kp_image_data* out,
+                            kpo_stats* stats) {
     double tetrad[16], gcam[16];
     const double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
     kpo_bl_camera_tetrad(p, tetrad);
@@ -1697,7 +1741,8 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(static) reduction(+ : rhs_total, coeff_total, transfer_total, a_total, b_total)
 #endif
-    for (int pix = begin; pix < end; ++pix) {
+    for (int i = 0; i < count; ++i) {
+        const int pix = first + i * stride;
         int64_t rhs_count = 0;
         metric_ctx m = {p->mass, p->spin, &rhs_count};
         kpo_stats local = {0, 0, 0, 0, 0};
@@ -1706,8 +1751,8 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
         int32_t a_steps = 0, a_reason = 0;
         const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
         pixel_result r;
-        pass_b(p, model, pert, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
-        store_pixel(out, pix, &r);
+        pass_b(p, model, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
+        store_pixel(out, compact ? i : pix, nout, fp->nfreq, &r);
         rhs_total += rhs_count;
         coeff_total += local.coefficient_evals;
         transfer_total += local.transfer_steps;
@@ -1724,51 +1769,65 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
     return KP_OK;
 }
 
+static kp_status check_params(const kp_pass_a_params* p, const kp_riaf_model* model, const void* out) {
+    if (!p || !model || !out) return KP_INVALID_ARGUMENT;
+    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST || p->camera.model != KP_CAMERA_PINHOLE)
+        return KP_UNSUPPORTED;
+    return KP_OK;
+}
+
+kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* model,
+                             const kpo_perturbation* pert, int32_t pixel_begin,
+                             int32_t pixel_end, int32_t nthreads, kp_image_data* out,
+                             kpo_stats* stats) {
+    const kp_status st = check_params(p, model, out);
+    if (st != KP_OK) return st;
+    const int npix = p->camera.nx * p->camera.ny;
+    const int begin = pixel_begin < 0 ? 0 : pixel_begin;
+    const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
+    const double f0 = model->freq_cgs;
+    const freq_plan fp = {&f0, 1, &f0, 1, 1};
+    return run_pixels(p, model, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
+                      out, stats);
+}
+
 /* Strided pixel sample pixel = offset + i * stride, i < count, outputs compact
  * (index i): bounded CPU baselines and spot checks of full-size images. */
 kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* p, const kp_riaf_model* model,
                                      const kpo_perturbation* pert, int32_t offset, int32_t stride,
                                      int32_t count, int32_t nthreads, kp_image_data* out,
                                      kpo_stats* stats) {
-    if (!p || !model || !out || stride <= 0 || count < 0 || offset < 0) return KP_INVALID_ARGUMENT;
-    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST || p->camera.model != KP_CAMERA_PINHOLE)
-        return KP_UNSUPPORTED;
+    kp_status st = check_params(p, model, out);
+    if (st != KP_OK) return st;
+    if (stride <= 0 || count < 0 || offset < 0) return KP_INVALID_ARGUMENT;
     const int64_t npix = (int64_t)p->camera.nx * p->camera.ny;
     if ((int64_t)offset + (int64_t)(count > 0 ? count - 1 : 0) * stride >= npix && count > 0)
         return KP_INVALID_ARGUMENT;
-    double tetrad[16], gcam[16];
-    const double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
-    kpo_bl_camera_tetrad(p, tetrad);
-    kpo_bl_gcov(p->mass, p->spin, xcam, gcam);
-    int64_t rhs_total = 0, coeff_total = 0, transfer_total = 0, a_total = 0, b_total = 0;
-#ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(static) reduction(+ : rhs_total, coeff_total, transfer_total, a_total, b_total)
-#endif
-    for (int i = 0; i < count; ++i) {
-        const int pix = offset + i * stride;
-        int64_t rhs_count = 0;
-        metric_ctx m = {p->mass, p->spin, &rhs_count};
-        kpo_stats local = {0, 0, 0, 0, 0};
-        kpo_state camera, endpoint;
-        camera_ray_from_tetrad(p, tetrad, gcam, pix, &camera);
-        int32_t a_steps = 0, a_reason = 0;
-        const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
-        pixel_result r;
-        pass_b(p, model, pert, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
-        store_pixel(out, i, &r);
-        rhs_total += rhs_count;
-        coeff_total += local.coefficient_evals;
-        transfer_total += local.transfer_steps;
-        a_total += a_steps;
-        b_total += r.steps;
-    }
-    if (stats) {
-        stats->rhs_evals = rhs_total;
-        stats->coefficient_evals = coeff_total;
-        stats->transfer_steps = transfer_total;
-        stats->pass_a_steps = a_total;
-        stats->pass_b_steps = b_total;
-    }
-    return KP_OK;
+    const double f0 = model->freq_cgs;
+    const freq_plan fp = {&f0, 1, &f0, 1, 1};
+    return run_pixels(p, model, pert, &fp, offset, stride, count, 1, count, nthreads, out, stats);
+}
+
+/* Multi-frequency image on shared geodesics: Stokes outputs [nfreq][npix]
+ * (trace_pass_b_segment_model_multifrequency_pixel_metric, pass_b.hpp:1130-1500,
+ * and the separate step-control variant, pass_b.hpp:1505-1895; driver
+ * src/image/multifrequency.hpp:152-280). */
+kp_status kpo_run_riaf_multifrequency_image(const kp_pass_a_params* p, const kp_riaf_model* model,
+                                            const kpo_perturbation* pert, const double* freqs,
+                                            int32_t nfreq, const double* ctrl, int32_t nctrl,
+                                            int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
+                                            kp_image_data* out, kpo_stats* stats) {
+    const kp_status st = check_params(p, model, out);
+    if (st != KP_OK) return st;
+    if (!freqs || nfreq < 1 || nfreq > KPO_MAX_FREQUENCIES || !ctrl || nctrl < 1 ||
+        nctrl > KPO_MAX_FREQUENCIES)
+        return KP_INVALID_ARGUMENT;
+    int same = nctrl == nfreq;
+    for (int f = 0; same && f < nfreq; ++f) same = freqs[f] == ctrl[f];
+    const freq_plan fp = {freqs, nfreq, ctrl, nctrl, same};
+    const int npix = p->camera.nx * p->camera.ny;
+    const int begin = pixel_begin < 0 ? 0 : pixel_begin;
+    const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
+    return run_pixels(p, model, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
+                      out, stats);
 }

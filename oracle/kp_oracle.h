@@ -19,6 +19,9 @@ extern "C" {
 #endif
 
 /* State layout: x[0..3], k[0..3], e1[0..3], e2[0..3] (TransportState). */
+/* Largest frequency list of one shared-geodesic image. */
+#define KPO_MAX_FREQUENCIES 16
+
 typedef struct kpo_state { double v[16]; } kpo_state;
 
 typedef struct kpo_coeffs {
@@ -104,6 +107,14 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* params, const kp_riaf_model
                              const kpo_perturbation* pert,
                              int32_t pixel_begin, int32_t pixel_end,
                              int32_t nthreads, kp_image_data* out, kpo_stats* stats);
+
+/* Multi-frequency image on shared geodesics; image_i/q/u/v point to
+ * [nfreq][npix] arrays.  ctrl lists the step-control frequencies. */
+kp_status kpo_run_riaf_multifrequency_image(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                            const kpo_perturbation* pert, const double* freqs,
+                                            int32_t nfreq, const double* ctrl, int32_t nctrl,
+                                            int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
+                                            kp_image_data* out, kpo_stats* stats);
 
 kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* params, const kp_riaf_model* model,
                                      const kpo_perturbation* pert, int32_t offset, int32_t stride,

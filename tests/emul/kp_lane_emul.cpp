@@ -7,9 +7,9 @@
 
 #include <vector>
 
-template <bool MIXED>
+template <bool MIXED, bool MF>
 static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, double fp64_radius,
-                          kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps) {
+                          const FreqLists& fl, kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps) {
     Params P;
     kp_status st = prepare(pa, m, P);
     if (st != KP_OK) return st;
@@ -17,6 +17,11 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         prepare_mixed(P);
         P.fp64_radius = fp64_radius;
     }
+    if (MF) {
+        st = prepare_frequencies(*m, fl, P);
+        if (st != KP_OK) return st;
+    }
+    double mfbuf[4 * KP_MAX_FREQUENCIES];
     const int np = P.npix;
     std::vector<double> ep(16 * (size_t)np);
     std::vector<int32_t> valid(np), steps(np), reason(np);
@@ -35,10 +40,11 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         }
         lane_a_store(P, A);
         LaneB B;
-        if (lane_b_init(P, pix, B)) {
-            while (!pass_b_tick<MIXED>(P, G, B, c1, c2, sb, guard)) {
+        B.mf = LdsVec<double>{mfbuf, 1};
+        if (lane_b_init<MF>(P, pix, B)) {
+            while (!pass_b_tick<MIXED, MF>(P, G, B, c1, c2, sb, guard)) {
             }
-            lane_b_finish<MIXED>(P, B, 1);
+            lane_b_finish<MIXED, MF>(P, B, 1);
         }
         total += rc;
         guard_total += guard;
@@ -50,7 +56,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
 
 extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp_riaf_model* m,
                                             kp_image_data* out, int64_t* rhs_evals) {
-    return emul_run<false>(pa, m, 0.0, out, rhs_evals, nullptr);
+    return emul_run<false, false>(pa, m, 0.0, FreqLists{}, out, rhs_evals, nullptr);
 }
 
 // The mixed-precision lanes (FP32 arithmetic on the host's IEEE float; the
@@ -58,7 +64,22 @@ extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp
 extern "C" kp_status kp_emul_run_riaf_image_mixed(const kp_pass_a_params* pa, const kp_riaf_model* m,
                                                   double fp64_radius, kp_image_data* out, int64_t* rhs_evals,
                                                   int64_t* guard_steps) {
-    return emul_run<true>(pa, m, fp64_radius, out, rhs_evals, guard_steps);
+    return emul_run<true, false>(pa, m, fp64_radius, FreqLists{}, out, rhs_evals, guard_steps);
+}
+
+// Multi-frequency lanes (Stokes arrays [nfreq][npix]); mixed != 0 selects the
+// mixed-precision lanes.
+extern "C" kp_status kp_emul_run_riaf_multifrequency_image(const kp_pass_a_params* pa, const kp_riaf_model* m,
+                                                           int32_t mixed, double fp64_radius, const double* freqs,
+                                                           int32_t nfreq, const double* ctrl, int32_t nctrl,
+                                                           kp_image_data* out, int64_t* rhs_evals) {
+    FreqLists fl;
+    fl.nfreq = nfreq;
+    fl.freqs = freqs;
+    fl.nctrl = nctrl;
+    fl.ctrl = ctrl;
+    if (mixed) return emul_run<true, true>(pa, m, fp64_radius, fl, out, rhs_evals, nullptr);
+    return emul_run<false, true>(pa, m, 0.0, fl, out, rhs_evals, nullptr);
 }
 
 // The kernels' bounded-argument sine/cosine, for its accuracy test.

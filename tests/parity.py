@@ -139,3 +139,33 @@ def assert_mixed(got, ref, label):
     for n in STOKES:
         assert np.all(np.isfinite(got[n])), (label, n)
     return e
+
+
+def split_frequencies(img, nfreq):
+    """Per-frequency views of a multi-frequency image ([nfreq][npix] Stokes,
+    per-pixel diagnostics shared)."""
+    npix = img["reason"].size
+    out = []
+    for f in range(nfreq):
+        d = {k: v for k, v in img.items() if not k.startswith("_") and k not in STOKES}
+        for n in STOKES:
+            d[n] = img[n][f * npix:(f + 1) * npix]
+        out.append(d)
+    return out
+
+
+def assert_parity_multifrequency(got, ref_mf, nfreq, ref_single, label):
+    """Each frequency slice of got against the oracle's multi-frequency image at
+    the parity bar, with ill-conditioned pixels taken from ref_single (an
+    oracle_with_sensitivity run of the same geodesics: the conditioning of a
+    ray is a property of its geodesic)."""
+    g = split_frequencies(got, nfreq)
+    r = split_frequencies(ref_mf, nfreq)
+    res = []
+    for f in range(nfreq):
+        rf = dict(r[f])
+        rf["_sens"] = ref_single["_sens"]
+        rf["_moved_steps"] = ref_single["_moved_steps"]
+        res.append(assert_parity(g[f], rf, f"{label} f{f}",
+                                 peak=float(np.max(np.abs(rf["image_i"]))), diagnostics=(f == 0)))
+    return res

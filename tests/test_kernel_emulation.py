@@ -31,6 +31,11 @@ def emul():
                                                C.POINTER(abi.ImageData), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64)]
     L.kp_emul_run_riaf_image_mixed.restype = C.c_int32
+    dp = C.POINTER(C.c_double)
+    L.kp_emul_run_riaf_multifrequency_image.argtypes = [
+        C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.c_int32, C.c_double, dp, C.c_int32, dp,
+        C.c_int32, C.POINTER(abi.ImageData), C.POINTER(C.c_int64)]
+    L.kp_emul_run_riaf_multifrequency_image.restype = C.c_int32
     return L
 
 
@@ -141,3 +146,49 @@ def test_emulated_mixed_precision_unguarded(emul):
     assert guard == 0
     r, _ = oracle.run_riaf_image(p, m)
     parity.assert_mixed(e, r, "emulated mixed unguarded")
+
+
+def _run_emul_mf(L, p, m, freqs, ctrl=None, mixed=False):
+    npix = p.camera.nx * p.camera.ny
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    c = np.ascontiguousarray(freqs if ctrl is None else ctrl, dtype=np.float64)
+    arrays = oracle.alloc_image(npix)
+    for n in parity.STOKES:
+        arrays[n] = np.zeros(len(f) * npix)
+    img = oracle.image_struct(arrays)
+    rhs = C.c_int64()
+    dp = C.POINTER(C.c_double)
+    assert L.kp_emul_run_riaf_multifrequency_image(
+        C.byref(p), C.byref(m), int(mixed), abi.FP64_RADIUS_DEFAULT, f.ctypes.data_as(dp), len(f),
+        c.ctypes.data_as(dp), len(c), C.byref(img), C.byref(rhs)) == 0
+    return arrays
+
+
+MF_FREQS = [86e9, 230e9, 345e9]
+
+
+def test_emulated_multifrequency_depth_limited(emul):
+    """Three frequencies sharing geodesics, with absorption/Faraday depth limits
+    tight enough that the step control depends on the frequency."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=10, ny=10, max_absorption_depth=1e-3, max_faraday_depth=1e-2)
+    e = _run_emul_mf(emul, p, m, MF_FREQS)
+    r, _ = oracle.run_riaf_multifrequency_image(p, m, MF_FREQS)
+    single = parity.oracle_with_sensitivity(p, m)
+    parity.assert_parity_multifrequency(e, r, 3, single, "emulated mf")
+
+
+def test_emulated_multifrequency_separate_control(emul):
+    """Output frequencies with a different step-control list (the control variant)."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=8, ny=8, max_absorption_depth=1e-3, max_faraday_depth=1e-2)
+    e = _run_emul_mf(emul, p, m, MF_FREQS, ctrl=[86e9])
+    r, _ = oracle.run_riaf_multifrequency_image(p, m, MF_FREQS, [86e9])
+    single = parity.oracle_with_sensitivity(p, m)
+    parity.assert_parity_multifrequency(e, r, 3, single, "emulated mf ctrl")
+
+
+def test_emulated_multifrequency_mixed(emul):
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=12, ny=12)
+    e = _run_emul_mf(emul, p, m, MF_FREQS, mixed=True)
+    r, _ = oracle.run_riaf_multifrequency_image(p, m, MF_FREQS)
+    for f, (gf, rf) in enumerate(zip(parity.split_frequencies(e, 3), parity.split_frequencies(r, 3))):
+        parity.assert_mixed(gf, rf, f"emulated mixed mf f{f}")

@@ -6,6 +6,7 @@ index, a transposed operand) in the oracle fails it.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import json
 import math
@@ -412,3 +413,57 @@ def test_riaf_fluid_velocity_normalised():
         L.kpo_riaf_fluid_velocity(C.byref(p), C.byref(m), oracle.dptr(x), oracle.dptr(u))
         g = _mat("kpo_bl_gcov", 1.0, 0.94, x, 16).reshape(4, 4)
         assert u @ g @ u == pytest.approx(-1.0, abs=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# Multi-frequency pass B on shared geodesics (pass_b.hpp:1130-1895)
+# ---------------------------------------------------------------------------
+def _mf_opts(**kw):
+    from kpolaris_amd import image
+    o = image.Options(parameter_file=str(Path(__file__).resolve().parent.parent / "kpolaris_amd" / "params"
+                                         / "riaf_m87_c2.par"))
+    for k, v in kw.items():
+        o.set(k, str(v))
+    return o.resolve_riaf()
+
+
+def test_oracle_multifrequency_single_list_is_the_single_frequency_image():
+    p, m, _, _ = _mf_opts(nx=6, ny=6)
+    single, s1 = oracle.run_riaf_image(p, m)
+    mf, s2 = oracle.run_riaf_multifrequency_image(p, m, [m.freq_cgs])
+    for k in single:
+        assert np.array_equal(single[k], mf[k]), k
+    assert s1 == s2
+
+
+def test_oracle_multifrequency_slice_equals_single_frequency_run_under_its_own_control():
+    """With the step control on frequency f_k alone, slice k is the
+    single-frequency image at f_k, bit for bit, even with depth limits that
+    make the step sequence frequency dependent."""
+    freqs = [86e9, 230e9, 345e9]
+    p, m, _, _ = _mf_opts(nx=5, ny=5, max_absorption_depth=1e-3, max_faraday_depth=1e-2)
+    npix = 25
+    for k, fk in enumerate(freqs):
+        mf, _ = oracle.run_riaf_multifrequency_image(p, m, freqs, [fk])
+        mk = copy.deepcopy(m)
+        mk.freq_cgs = fk
+        single, _ = oracle.run_riaf_image(p, mk)
+        for n in ("image_i", "image_q", "image_u", "image_v"):
+            assert np.array_equal(mf[n][k * npix:(k + 1) * npix], single[n]), (k, n)
+        assert np.array_equal(mf["steps"], single["steps"])
+
+
+def test_oracle_multifrequency_control_is_the_max_over_the_set():
+    """The control list enters only through the largest substep count: a
+    reordered list (the recompute path) equals the output list itself (the
+    reuse path), and the 86 GHz depth limits dominate this model."""
+    freqs = [86e9, 230e9, 345e9]
+    p, m, _, _ = _mf_opts(nx=5, ny=5, max_absorption_depth=1e-3, max_faraday_depth=1e-2)
+    a, _ = oracle.run_riaf_multifrequency_image(p, m, freqs)
+    b, _ = oracle.run_riaf_multifrequency_image(p, m, freqs, freqs[::-1])
+    c, _ = oracle.run_riaf_multifrequency_image(p, m, freqs, [86e9])
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
+    d, _ = oracle.run_riaf_multifrequency_image(p, m, freqs, [345e9])
+    assert d["steps"].sum() < a["steps"].sum()
