@@ -204,7 +204,10 @@ typedef struct kp_image_data {
 /* Launch configuration; no reference counterpart (Kokkos chose it there). */
 typedef struct kp_run_options {
     int32_t precision;        /* kp_precision */
-    int32_t split_transport;  /* 1: pass A and pass B kernels (reference default) */
+    int32_t split_transport;  /* 1: pass A and pass B kernels (reference default);
+                                 0: one kernel, pass A then pass B per lane
+                                 (identical results; stats report the whole
+                                 time as pass_a_ms) */
     int32_t block_size;       /* rays per workgroup; 0 = tuned default */
     int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread */
     int32_t waves_per_cu;     /* persistent grid size; 0 = occupancy-derived */
@@ -301,6 +304,40 @@ kp_status kp_run_riaf_multifrequency_image_host(const kp_pass_a_params* params,
                                                 const kp_run_options* options,
                                                 kp_image_data* out,
                                                 kp_image_stats* stats);
+
+/*
+ * Batched images.  Replaces run_image_batch_metric(jobs, ImageBatchOptions,
+ * ImageBatchTiming*) and ImageBatchJob (src/image/batch_driver.hpp:15-21,
+ * 126-187), which run one unsplit kernel per job on partitioned execution
+ * instances.  Here every job's rays share one persistent pass A launch and
+ * one pass B launch with a single work queue, recorded once as a HIP graph
+ * (kp_batch_create) and replayed by kp_batch_launch.  Jobs may differ in
+ * camera, resolution, spin, inclination and model; all share one precision
+ * and launch configuration (kp_run_options).  Each job's `out` holds DEVICE
+ * pointers owned by the caller, which must stay valid while the batch exists.
+ */
+typedef struct kp_batch_job {
+    kp_pass_a_params params;
+    kp_riaf_model model;
+    int32_t radiation_substeps; /* must be positive, otherwise unused (pass_b.hpp:526) */
+    int32_t reserved0;
+    kp_image_data out;
+} kp_batch_job;
+
+typedef struct kp_batch kp_batch;
+
+/* Validates every job, uploads the job table, allocates the endpoint
+ * workspace and instantiates the graph. */
+kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, const kp_run_options* options,
+                          kp_batch** batch);
+/* Replays the graph on `stream` (hipStream_t; NULL = default stream). */
+kp_status kp_batch_launch(kp_batch* batch, void* stream);
+/* Work counters of the last completed launch (the caller synchronises first). */
+kp_status kp_batch_read_stats(kp_batch* batch, kp_image_stats* stats);
+void kp_batch_destroy(kp_batch* batch);
+/* create + launch + wait + destroy; stats->kernel_ms is the graph's device time. */
+kp_status kp_run_riaf_batch(const kp_batch_job* jobs, int32_t njobs, const kp_run_options* options,
+                            kp_image_stats* stats);
 
 /* Defaults of the reference's structs (field initialisers of CameraParams,
  * PassAParams and RIAFAnalyticRadiationModel) and of kp_run_options. */

@@ -138,6 +138,14 @@ class ImageData(C.Structure):
     ]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [
+        ("params", PassAParams), ("model", RIAFModel),
+        ("radiation_substeps", C.c_int32), ("reserved0", C.c_int32),
+        ("out", ImageData),
+    ]
+
+
 class RunOptions(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("split_transport", C.c_int32),

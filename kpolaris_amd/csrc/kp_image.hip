@@ -181,7 +181,7 @@ KP_DEV bool fetch_pixel(uint32_t* counter, bool need, uint32_t npix, int& pixel)
 }
 
 // Wave-level sum of per-lane counters, one atomic per counter per wave.
-KP_DEV void flush_counters(const Params& P, int rhs_slot, uint32_t rhs, uint32_t coeff,
+KP_DEV void flush_counters(unsigned long long* counters, int rhs_slot, uint32_t rhs, uint32_t coeff,
                            uint32_t transfer, uint32_t steps_a, uint32_t steps_b, uint32_t guard) {
     unsigned long long v[6] = {rhs, coeff, transfer, steps_a, steps_b, guard};
 #pragma unroll
@@ -191,12 +191,12 @@ KP_DEV void flush_counters(const Params& P, int rhs_slot, uint32_t rhs, uint32_t
         v[c] = x;
     }
     if (__lane_id() == 0) {
-        atomicAdd(&P.counters[rhs_slot], v[0]);
-        atomicAdd(&P.counters[C_COEFF], v[1]);
-        atomicAdd(&P.counters[C_TRANSFER], v[2]);
-        atomicAdd(&P.counters[C_STEPS_A], v[3]);
-        atomicAdd(&P.counters[C_STEPS_B], v[4]);
-        atomicAdd(&P.counters[C_GUARD], v[5]);
+        atomicAdd(&counters[rhs_slot], v[0]);
+        atomicAdd(&counters[C_COEFF], v[1]);
+        atomicAdd(&counters[C_TRANSFER], v[2]);
+        atomicAdd(&counters[C_STEPS_A], v[3]);
+        atomicAdd(&counters[C_STEPS_B], v[4]);
+        atomicAdd(&counters[C_GUARD], v[5]);
     }
 }
 
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
             lane_a_store(P, L);
         }
     }
-    flush_counters(P, C_RHS_A, rhs_count, 0, 0, steps_a, 0, guard);
+    flush_counters(P.counters, C_RHS_A, rhs_count, 0, 0, steps_a, 0, guard);
 }
 
 // ---------------------------------------------------------------------------
@@ -890,7 +890,145 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
             lane_b_finish<MIXED, MF>(P, L, 1);
         }
     }
-    flush_counters(P, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
+    flush_counters(P.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
+}
+
+// ---------------------------------------------------------------------------
+// Unsplit transport (split_transport = 0): one kernel in which each lane
+// integrates pass A and then pass B of the same pixel, as the reference's
+// trace_pass_b_segment_model_pixel_metric (pass_b.hpp:768-1114) does; the
+// results are those of the split kernels.  The endpoint goes through the
+// workspace (written and read back by the same lane).  Lanes fetch pixels
+// from one queue, one wave-level atomic per refill; a wave refills only when
+// all of its lanes have finished both passes.
+// ---------------------------------------------------------------------------
+template <bool MIXED, bool MF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void unsplit_kernel(Params P) {
+    uint32_t rhs_a = 0, rhs_b = 0, coeff = 0, transfer = 0, steps_a = 0, steps_b = 0, guard = 0;
+    while (true) {
+        int pix = 0;
+        const bool has = fetch_pixel(&P.queue[0], true, (uint32_t)P.npix, pix);
+        if (!__ballot(has)) break;
+        if (has) {
+            {
+                Geo<double> G{P.mass, P.spin, &rhs_a};
+                LaneA A;
+                lane_lds(A.s, A.k1);
+                lane_a_init(P, pix, A);
+                while (!pass_a_tick<MIXED>(P, G, A, steps_a, guard)) {
+                }
+                lane_a_store(P, A);
+            }
+            Geo<double> G{P.mass, P.spin, &rhs_b};
+            LaneB B;
+            lane_lds(B.s, B.k1);
+            if (MF) {
+                extern __shared__ double kp_lds[];
+                B.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
+            }
+            if (lane_b_init<MF>(P, pix, B)) {
+                while (!pass_b_tick<MIXED, MF>(P, G, B, coeff, transfer, steps_b, guard)) {
+                }
+                lane_b_finish<MIXED, MF>(P, B, 1);
+            }
+        }
+    }
+    flush_counters(P.counters, C_RHS_A, rhs_a, 0, 0, steps_a, 0, guard);
+    flush_counters(P.counters, C_RHS_B, rhs_b, coeff, transfer, 0, steps_b, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Batched images: many jobs (each its own camera, spin, model and outputs) in
+// one persistent launch per pass.  One queue runs over the concatenated rays
+// of all jobs; a lane that fetches global ray index g finds its job by binary
+// search over the jobs' first indices and then reads that job's parameters
+// from the device job table (lanes of one wave may serve different jobs).
+// The reference runs one kernel per job on partitioned execution instances
+// (src/image/batch_driver.hpp:126-187).
+// ---------------------------------------------------------------------------
+struct BatchArgs {
+    const Params* jobs;        // [njobs] device job table
+    const uint32_t* start;     // [njobs + 1] first global ray index of each job
+    int32_t njobs;
+    uint32_t total;            // rays over all jobs
+    uint32_t* queue;           // [2] pass A / pass B fetch counters
+    unsigned long long* counters;
+};
+
+KP_DEV int find_job(const BatchArgs& B, uint32_t g) {
+    int lo = 0, hi = B.njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.start[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_a_batch_kernel(BatchArgs B) {
+    uint32_t rhs_count = 0, steps_a = 0, guard = 0;
+    LaneA L;
+    lane_lds(L.s, L.k1);
+    const Params* P = B.jobs;
+    bool has = false, exhausted = false;
+    while (true) {
+        int g = 0;
+        const bool need = !has && !exhausted;
+        if (__ballot(need)) {
+            if (fetch_pixel(&B.queue[0], need, B.total, g)) {
+                const int j = find_job(B, (uint32_t)g);
+                P = &B.jobs[j];
+                lane_a_init(*P, g - (int)B.start[j], L);
+                has = true;
+            } else if (need) {
+                exhausted = true;
+            }
+        }
+        if (!__ballot(has)) break;
+        if (has) {
+            const Geo<double> G{P->mass, P->spin, &rhs_count};
+            if (pass_a_tick<MIXED>(*P, G, L, steps_a, guard)) {
+                lane_a_store(*P, L);
+                has = false;
+            }
+        }
+    }
+    flush_counters(B.counters, C_RHS_A, rhs_count, 0, 0, steps_a, 0, guard);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_batch_kernel(BatchArgs B) {
+    uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
+    LaneB L;
+    lane_lds(L.s, L.k1);
+    const Params* P = B.jobs;
+    bool has = false, exhausted = false;
+    while (true) {
+        while (true) {
+            const bool need = !has && !exhausted;
+            if (!__ballot(need)) break;
+            int g = 0;
+            if (fetch_pixel(&B.queue[1], need, B.total, g)) {
+                const int j = find_job(B, (uint32_t)g);
+                P = &B.jobs[j];
+                has = lane_b_init<false>(*P, g - (int)B.start[j], L);
+            } else if (need) {
+                exhausted = true;
+            }
+        }
+        if (!__ballot(has)) break;
+        if (has) {
+            const Geo<double> G{P->mass, P->spin, &rhs_count};
+            if (pass_b_tick<MIXED, false>(*P, G, L, coeff, transfer, steps_b, guard)) {
+                lane_b_finish<MIXED, false>(*P, L, 1);
+                has = false;
+            }
+        }
+    }
+    flush_counters(B.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
 }
 
 __global__ void zero_kernel(uint32_t* q, unsigned long long* c) {
@@ -941,7 +1079,7 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Occupancy {
     int cus = 0;
-    int blocks_a = 0, blocks_b = 0;
+    int blocks_a = 0, blocks_b = 0, blocks_u = 0;
 };
 
 template <bool PERSISTENT> const void* pass_a_fn(bool mixed) {
@@ -953,6 +1091,12 @@ template <bool PERSISTENT> const void* pass_b_fn(bool mixed, bool mf) {
                      : (const void*)pass_b_kernel<PERSISTENT, false, true>;
     return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, false>
                  : (const void*)pass_b_kernel<PERSISTENT, false, false>;
+}
+
+const void* unsplit_fn(bool mixed, bool mf) {
+    if (mf)
+        return mixed ? (const void*)unsplit_kernel<true, true> : (const void*)unsplit_kernel<false, true>;
+    return mixed ? (const void*)unsplit_kernel<true, false> : (const void*)unsplit_kernel<false, false>;
 }
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
@@ -981,8 +1125,11 @@ kp_status occupancy_for(int device, int block, bool mixed, int nfreq_mf, Occupan
     KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed), block, lds_bytes(block, 0)));
     KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed, nfreq_mf > 0), block,
                                                         lds_bytes(block, nfreq_mf)));
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_u, unsplit_fn(mixed, nfreq_mf > 0), block,
+                                                        lds_bytes(block, nfreq_mf)));
     o.blocks_a = std::max(o.blocks_a, 1);
     o.blocks_b = std::max(o.blocks_b, 1);
+    o.blocks_u = std::max(o.blocks_u, 1);
     g_occ.push_back({key, o});
     occ = o;
     return KP_OK;
@@ -1168,14 +1315,17 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const size_t smem_a = lds_bytes(block, 0);
     const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0);
     const int nblocks_flat = (int)((np + block - 1) / block);
-    int grid_a = nblocks_flat, grid_b = nblocks_flat;
+    int grid_a = nblocks_flat, grid_b = nblocks_flat, grid_u = nblocks_flat;
     if (P.persistent) {
         const int waves_per_block = block / 64;
-        const int per_cu_a = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block) : occ.blocks_a;
-        const int per_cu_b = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block) : occ.blocks_b;
-        grid_a = std::max(1, std::min(nblocks_flat, occ.cus * per_cu_a));
-        grid_b = std::max(1, std::min(nblocks_flat, occ.cus * per_cu_b));
+        auto per_cu = [&](int occ_blocks) {
+            return opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block) : occ_blocks;
+        };
+        grid_a = std::max(1, std::min(nblocks_flat, occ.cus * per_cu(occ.blocks_a)));
+        grid_b = std::max(1, std::min(nblocks_flat, occ.cus * per_cu(occ.blocks_b)));
+        grid_u = std::max(1, std::min(nblocks_flat, occ.cus * per_cu(occ.blocks_u)));
     }
+    const bool unsplit = opt.split_transport == 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (stats) {
         for (int i = 0; i < 3; ++i) KP_HIP(hipEventCreate(&ev[i]));
@@ -1184,12 +1334,14 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     KP_HIP(hipGetLastError());
     if (stats) KP_HIP(hipEventRecord(ev[0], stream));
     void* args[] = {&P};
-    if (P.npix > 0) {
+    if (P.npix > 0 && unsplit)
+        KP_HIP(hipLaunchKernel(unsplit_fn(mixed, mf), dim3(grid_u), dim3(block), args, smem_b, stream));
+    if (P.npix > 0 && !unsplit) {
         const void* fn = P.persistent ? pass_a_fn<true>(mixed) : pass_a_fn<false>(mixed);
         KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem_a, stream));
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
-    if (P.npix > 0) {
+    if (P.npix > 0 && !unsplit) {
         const void* fn = P.persistent ? pass_b_fn<true>(mixed, mf) : pass_b_fn<false>(mixed, mf);
         KP_HIP(hipLaunchKernel(fn, dim3(grid_b), dim3(block), args, smem_b, stream));
     }
@@ -1216,6 +1368,232 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
     }
     return KP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch: job table, shared endpoint workspace and a two-pass graph.
+// ---------------------------------------------------------------------------
+struct kp_batch {
+    int device = 0;
+    bool mixed = false;
+    int block = 128;
+    int grid_a = 1, grid_b = 1;
+    int32_t njobs = 0;
+    int64_t rays = 0;
+    void* dmem = nullptr;
+    BatchArgs args{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void batch_release(kp_batch* b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->dmem) (void)hipFree(b->dmem);
+    delete b;
+}
+
+static kp_status batch_build_graph(kp_batch* b) {
+    KP_HIP(hipGraphCreate(&b->graph, 0));
+    void* zargs[] = {&b->args.queue, &b->args.counters};
+    void* bargs[] = {&b->args};
+    hipKernelNodeParams zp{};
+    zp.func = (void*)zero_kernel;
+    zp.gridDim = dim3(1);
+    zp.blockDim = dim3(64);
+    zp.kernelParams = zargs;
+    hipKernelNodeParams ap{};
+    ap.func = b->mixed ? (void*)pass_a_batch_kernel<true> : (void*)pass_a_batch_kernel<false>;
+    ap.gridDim = dim3(b->grid_a);
+    ap.blockDim = dim3(b->block);
+    ap.sharedMemBytes = (unsigned)lds_bytes(b->block, 0);
+    ap.kernelParams = bargs;
+    hipKernelNodeParams bp = ap;
+    bp.func = b->mixed ? (void*)pass_b_batch_kernel<true> : (void*)pass_b_batch_kernel<false>;
+    bp.gridDim = dim3(b->grid_b);
+    hipGraphNode_t nz, na, nb;
+    KP_HIP(hipGraphAddKernelNode(&nz, b->graph, nullptr, 0, &zp));
+    KP_HIP(hipGraphAddKernelNode(&na, b->graph, &nz, 1, &ap));
+    KP_HIP(hipGraphAddKernelNode(&nb, b->graph, &na, 1, &bp));
+    KP_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return KP_OK;
+}
+
+extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, const kp_run_options* options,
+                                     kp_batch** out) {
+    kp_internal::clear_error();
+    if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null batch handle");
+    *out = nullptr;
+    if (!jobs || njobs < 1) return kp_internal::fail(KP_INVALID_ARGUMENT, "a batch needs at least one job");
+    kp_run_options opt;
+    kp_run_options_default(&opt);
+    if (options) opt = *options;
+    if (opt.precision != KP_PRECISION_FP64 && opt.precision != KP_PRECISION_MIXED)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "precision must be KP_PRECISION_FP64 or KP_PRECISION_MIXED");
+    const bool mixed = opt.precision == KP_PRECISION_MIXED;
+    if (mixed && !(opt.fp64_radius >= 0.0))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
+    const int block = opt.block_size > 0 ? opt.block_size : 128;
+    if (block % 64 != 0 || block > 256)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
+    // validate every job before any allocation (batch_driver.hpp:137-146)
+    std::vector<Params> table((size_t)njobs);
+    std::vector<uint32_t> start((size_t)njobs + 1, 0);
+    uint64_t total = 0;
+    for (int j = 0; j < njobs; ++j) {
+        if (jobs[j].radiation_substeps < 1)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "batch radiation_substeps must be positive");
+        kp_status st = prepare(&jobs[j].params, &jobs[j].model, table[j]);
+        if (st != KP_OK) return st;
+        if (mixed) {
+            prepare_mixed(table[j]);
+            table[j].fp64_radius = opt.fp64_radius;
+        }
+        table[j].out = jobs[j].out;
+        table[j].persistent = 1;
+        start[j] = (uint32_t)total;
+        total += (uint64_t)table[j].npix;
+        if (total > 0x7fffffffull) return kp_internal::fail(KP_INVALID_ARGUMENT, "batch exceeds 2^31 rays");
+    }
+    start[njobs] = (uint32_t)total;
+    int device = 0;
+    KP_HIP(hipGetDevice(&device));
+    // device layout: job table | start | queue | counters | per-job endpoints
+    const size_t off_table = 0;
+    const size_t off_start = align_up(off_table + table.size() * sizeof(Params));
+    const size_t off_queue = align_up(off_start + start.size() * sizeof(uint32_t));
+    const size_t off_counters = align_up(off_queue + 2 * sizeof(uint32_t));
+    size_t off = align_up(off_counters + C_COUNT * sizeof(unsigned long long));
+    std::vector<size_t> ep_off((size_t)njobs);
+    for (int j = 0; j < njobs; ++j) {
+        ep_off[j] = off;
+        const size_t np = (size_t)table[j].npix;
+        off = align_up(off + 16 * np * sizeof(double));
+        off = align_up(off + 3 * np * sizeof(int32_t));
+    }
+    kp_batch* b = new kp_batch();
+    b->device = device;
+    b->mixed = mixed;
+    b->block = block;
+    b->njobs = njobs;
+    b->rays = (int64_t)total;
+    if (hipMalloc(&b->dmem, off) != hipSuccess) {
+        batch_release(b);
+        return kp_internal::fail(KP_OUT_OF_MEMORY, "hipMalloc of the batch workspace failed");
+    }
+    char* base = static_cast<char*>(b->dmem);
+    for (int j = 0; j < njobs; ++j) {
+        const size_t np = (size_t)table[j].npix;
+        char* e = base + ep_off[j];
+        table[j].ep = reinterpret_cast<double*>(e);
+        char* iv = base + align_up(ep_off[j] + 16 * np * sizeof(double));
+        table[j].ep_valid = reinterpret_cast<int32_t*>(iv);
+        table[j].ep_steps = table[j].ep_valid + np;
+        table[j].ep_reason = table[j].ep_steps + np;
+        table[j].queue = nullptr;
+        table[j].counters = nullptr;
+    }
+    b->args.jobs = reinterpret_cast<const Params*>(base + off_table);
+    b->args.start = reinterpret_cast<const uint32_t*>(base + off_start);
+    b->args.njobs = njobs;
+    b->args.total = (uint32_t)total;
+    b->args.queue = reinterpret_cast<uint32_t*>(base + off_queue);
+    b->args.counters = reinterpret_cast<unsigned long long*>(base + off_counters);
+    kp_status st = KP_OK;
+    if (hipMemcpy(base + off_table, table.data(), table.size() * sizeof(Params), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(base + off_start, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+            hipSuccess)
+        st = kp_internal::fail(KP_HIP_ERROR, "upload of the batch job table failed");
+    if (st == KP_OK) {
+        hipDeviceProp_t prop;
+        int ba = 1, bb = 1;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &ba, mixed ? (const void*)pass_a_batch_kernel<true> : (const void*)pass_a_batch_kernel<false>, block,
+                lds_bytes(block, 0)) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &bb, mixed ? (const void*)pass_b_batch_kernel<true> : (const void*)pass_b_batch_kernel<false>, block,
+                lds_bytes(block, 0)) != hipSuccess) {
+            st = kp_internal::fail(KP_HIP_ERROR, "occupancy query for the batch kernels failed");
+        } else {
+            const int wpb = block / 64;
+            const int per_a = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / wpb) : std::max(ba, 1);
+            const int per_b = opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / wpb) : std::max(bb, 1);
+            const int64_t flat = (int64_t)((total + block - 1) / block);
+            b->grid_a = (int)std::max<int64_t>(1, std::min<int64_t>(flat, (int64_t)prop.multiProcessorCount * per_a));
+            b->grid_b = (int)std::max<int64_t>(1, std::min<int64_t>(flat, (int64_t)prop.multiProcessorCount * per_b));
+            st = batch_build_graph(b);
+        }
+    }
+    if (st != KP_OK) {
+        batch_release(b);
+        return st;
+    }
+    *out = b;
+    return KP_OK;
+}
+
+extern "C" kp_status kp_batch_launch(kp_batch* b, void* stream) {
+    kp_internal::clear_error();
+    if (!b || !b->exec) return kp_internal::fail(KP_INVALID_ARGUMENT, "invalid batch handle");
+    KP_HIP(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
+    return KP_OK;
+}
+
+extern "C" kp_status kp_batch_read_stats(kp_batch* b, kp_image_stats* stats) {
+    kp_internal::clear_error();
+    if (!b || !stats) return kp_internal::fail(KP_INVALID_ARGUMENT, "null argument");
+    unsigned long long cnt[C_COUNT];
+    KP_HIP(hipMemcpy(cnt, b->args.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    std::memset(stats, 0, sizeof(*stats));
+    stats->rays = b->rays;
+    stats->rhs_evals = (int64_t)(cnt[C_RHS_A] + cnt[C_RHS_B]);
+    stats->pass_a_rhs_evals = (int64_t)cnt[C_RHS_A];
+    stats->coefficient_evals = (int64_t)cnt[C_COEFF];
+    stats->transfer_steps = (int64_t)cnt[C_TRANSFER];
+    stats->pass_a_steps = (int64_t)cnt[C_STEPS_A];
+    stats->pass_b_steps = (int64_t)cnt[C_STEPS_B];
+    stats->fp64_guard_steps = (int64_t)cnt[C_GUARD];
+    return KP_OK;
+}
+
+extern "C" void kp_batch_destroy(kp_batch* b) {
+    if (!b) return;
+    (void)hipDeviceSynchronize();
+    batch_release(b);
+}
+
+extern "C" kp_status kp_run_riaf_batch(const kp_batch_job* jobs, int32_t njobs, const kp_run_options* options,
+                                       kp_image_stats* stats) {
+    kp_batch* b = nullptr;
+    kp_status st = kp_batch_create(jobs, njobs, options, &b);
+    if (st != KP_OK) return st;
+    hipStream_t stream = options ? static_cast<hipStream_t>(options->stream) : nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (stats && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        kp_batch_destroy(b);
+        return kp_internal::fail(KP_HIP_ERROR, "hipEventCreate failed");
+    }
+    if (stats) (void)hipEventRecord(e0, stream);
+    st = kp_batch_launch(b, stream);
+    if (st == KP_OK && stats) {
+        (void)hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess) st = kp_internal::fail(KP_HIP_ERROR, "batch did not complete");
+        float ms = 0.f;
+        if (st == KP_OK) {
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            st = kp_batch_read_stats(b, stats);
+            stats->kernel_ms = ms;
+            stats->pass_a_ms = ms;
+        }
+    } else if (st == KP_OK && hipStreamSynchronize(stream) != hipSuccess) {
+        st = kp_internal::fail(KP_HIP_ERROR, "batch did not complete");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    kp_batch_destroy(b);
+    return st;
 }
 
 extern "C" int32_t kp_device_count(void) {

@@ -71,6 +71,16 @@ def library():
     L.kp_pack_image.argtypes = [dp, dp, dp, dp, ip, C.c_int64, C.c_double, C.c_int32, C.c_double,
                                 dp, dp, dp, dp, dp]
     L.kp_pack_image.restype = C.c_int32
+    L.kp_batch_create.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.RunOptions), P(C.c_void_p)]
+    L.kp_batch_create.restype = C.c_int32
+    L.kp_batch_launch.argtypes = [C.c_void_p, C.c_void_p]
+    L.kp_batch_launch.restype = C.c_int32
+    L.kp_batch_read_stats.argtypes = [C.c_void_p, P(abi.ImageStats)]
+    L.kp_batch_read_stats.restype = C.c_int32
+    L.kp_batch_destroy.argtypes = [C.c_void_p]
+    L.kp_batch_destroy.restype = None
+    L.kp_run_riaf_batch.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.RunOptions), P(abi.ImageStats)]
+    L.kp_run_riaf_batch.restype = C.c_int32
     if L.kp_abi_version() != abi.ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
                            f"{abi.ABI_VERSION}: rebuild the library")
@@ -138,7 +148,8 @@ def pack_image(image: dict, frequency: float, qu_sign: int, scale: float):
 
 
 def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
-                waves_per_cu: int = 0, stream=None, fp64_radius: float | None = None) -> abi.RunOptions:
+                waves_per_cu: int = 0, stream=None, fp64_radius: float | None = None,
+                split_transport: int = 1) -> abi.RunOptions:
     o = abi.RunOptions()
     library().kp_run_options_default(C.byref(o))
     o.precision = precision
@@ -147,6 +158,7 @@ def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, bl
     o.waves_per_cu = waves_per_cu
     if fp64_radius is not None:
         o.fp64_radius = fp64_radius
+    o.split_transport = split_transport
     o.stream = stream
     return o
 
@@ -176,7 +188,8 @@ def image_struct_from_tensors(arrays: dict) -> abi.ImageData:
 def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_substeps: int = 1,
                    precision: int = abi.PRECISION_FP64, persistent: bool = True,
                    block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
-                   fields=abi.IMAGE_FIELDS, with_stats: bool = True, fp64_radius: float | None = None):
+                   fields=abi.IMAGE_FIELDS, with_stats: bool = True, fp64_radius: float | None = None,
+                   split_transport: int = 1):
     """Run pass A + pass B on the current HIP device; returns (dict of device
     tensors, stats dict or None).  Work is queued on torch's current stream."""
     import torch
@@ -185,7 +198,8 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
         out = alloc_device_image(npix, fields=fields)
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius)
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius,
+                       split_transport)
     st = abi.ImageStats()
     _check(library().kp_run_riaf_image(C.byref(params), C.byref(model), radiation_substeps,
                                        C.byref(opts), C.byref(img),
@@ -216,7 +230,7 @@ def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel,
                                   precision: int = abi.PRECISION_FP64, persistent: bool = True,
                                   block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
                                   fields=abi.IMAGE_FIELDS, with_stats: bool = True,
-                                  fp64_radius: float | None = None):
+                                  fp64_radius: float | None = None, split_transport: int = 1):
     """Multi-frequency image on shared geodesics on the current HIP device.
     Stokes tensors are [nfreq * npix] (frequency-major); step control uses
     step_control_frequencies (default: the output frequencies)."""
@@ -229,10 +243,57 @@ def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel,
         out = alloc_device_image(npix, fields=fields, nfreq=len(f))
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius)
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius,
+                       split_transport)
     st = abi.ImageStats()
     dp = C.POINTER(C.c_double)
     _check(library().kp_run_riaf_multifrequency_image(
         C.byref(params), C.byref(model), radiation_substeps, f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp),
         len(c), C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)
+
+
+class Batch:
+    """Many RIAF images in one persistent two-pass HIP graph (kp_batch_*).
+
+    jobs: list of (PassAParams, RIAFModel); outputs are allocated here as
+    device tensors (``self.outputs[j]``) and stay owned by this object."""
+
+    def __init__(self, jobs, *, precision: int = abi.PRECISION_FP64, block_size: int = 0,
+                 waves_per_cu: int = 0, fp64_radius: float | None = None, fields=abi.IMAGE_FIELDS):
+        self._h = None
+        n = len(jobs)
+        arr = (abi.BatchJob * n)()
+        self.outputs = []
+        for j, (p, m) in enumerate(jobs):
+            out = alloc_device_image(p.camera.nx * p.camera.ny, fields=fields)
+            self.outputs.append(out)
+            arr[j].params = p
+            arr[j].model = m
+            arr[j].radiation_substeps = 1
+            arr[j].out = image_struct_from_tensors(out)
+        opts = run_options(precision, True, block_size, waves_per_cu, None, fp64_radius)
+        h = C.c_void_p()
+        _check(library().kp_batch_create(arr, n, C.byref(opts), C.byref(h)))
+        self._h = h
+
+    def launch(self, stream=None):
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        _check(library().kp_batch_launch(self._h, C.c_void_p(s)))
+
+    def stats(self) -> dict:
+        st = abi.ImageStats()
+        _check(library().kp_batch_read_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h is not None:
+            library().kp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

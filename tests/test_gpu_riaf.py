@@ -196,3 +196,20 @@ def test_gpu_unsupported_inputs_fail_loudly():
     with pytest.raises(image.KPError) as e:
         image.run_riaf_image(p, m)
     assert e.value.status == abi.KP_UNSUPPORTED
+
+
+def test_gpu_unsplit_transport():
+    """split_transport = 0 (one kernel, pass A then pass B per lane) against the
+    oracle at the parity bar (pass_b.hpp:768-1114 computes what
+    split_driver.hpp:41-104 does), and its mixed-precision form against the
+    split mixed kernels at the mixed bars."""
+    from kpolaris_amd import abi
+    print("case: unsplit 32x32 fp64 and mixed", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=32, ny=32).resolve_riaf()
+    got, st = _gpu_image(p, m, split_transport=0)
+    ref = parity.oracle_with_sensitivity(p, m)
+    print(parity.assert_parity(got, ref, "unsplit"), flush=True)
+    assert st["pass_b_ms"] == 0.0 and st["pass_a_ms"] > 0.0
+    a, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
+    b, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED, split_transport=0)
+    parity.assert_mixed(b, a, "unsplit mixed")
