@@ -306,6 +306,29 @@ kp_status kp_run_riaf_multifrequency_image_host(const kp_pass_a_params* params,
                                                 kp_image_stats* stats);
 
 /*
+ * RIAF image together with its derivatives with respect to the plasma
+ * log-scale parameters of PlasmaPerturbation (src/radiation/plasma_perturbation.hpp:10-33):
+ * q_density (n_e x e^q, B x e^(q/2)), q_temperature (Theta_e x e^q) and
+ * q_magnetic (B x e^q), evaluated at log_scales = {q_density, q_temperature,
+ * q_magnetic} (HOST array of 3; zeros = the unperturbed model) along the
+ * image's geodesics.  The reference estimates these by central differences
+ * of the coefficient blocks (src/diagnostics/response.hpp:118-182); here the
+ * transfer is differentiated exactly in forward (tangent) mode.  grad is a
+ * DEVICE array [3][4][npix]: grad[(k * 4 + s) * npix + p] = dS_s/dq_k for
+ * Stokes s = I, Q, U, V in the observer basis (invariant units, like
+ * out->image_*).  FP64 single-frequency only (options->precision must be
+ * KP_PRECISION_FP64); out receives the image at the same point.
+ */
+kp_status kp_run_riaf_image_gradient(const kp_pass_a_params* params,
+                                     const kp_riaf_model* model,
+                                     int32_t radiation_substeps,
+                                     const double* log_scales,
+                                     const kp_run_options* options,
+                                     kp_image_data* out,
+                                     double* grad,
+                                     kp_image_stats* stats);
+
+/*
  * Batched images.  Replaces run_image_batch_metric(jobs, ImageBatchOptions,
  * ImageBatchTiming*) and ImageBatchJob (src/image/batch_driver.hpp:15-21,
  * 126-187), which run one unsplit kernel per job on partitioned execution

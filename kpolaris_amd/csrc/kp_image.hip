@@ -20,6 +20,7 @@
 #include "kp_device.h"
 #include "kp_riaf.h"
 #include "kp_mixed.h"
+#include "kp_dual.h"
 #include "kp_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -63,6 +64,11 @@ enum : int {
     R_TIME = 8,
 };
 
+// Pass B variants: one frequency, several frequencies on shared geodesics, or
+// one frequency with the tangent (plasma-parameter derivatives) of its Stokes
+// vector carried alongside.
+enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2 };
+
 enum Counter : int {
     C_RHS_A = 0,
     C_RHS_B = 1,
@@ -97,6 +103,8 @@ struct Params {
     double dls_f[KP_MAX_FREQUENCIES], dls_c[KP_MAX_FREQUENCIES];
     double inv_nu3_f[KP_MAX_FREQUENCIES];
     float bconst_f[KP_MAX_FREQUENCIES], bconst_c[KP_MAX_FREQUENCIES];
+    // tangent mode: d(I,Q,U,V)/d(q_density, q_temperature, q_magnetic), [3][4][npix]
+    double* grad;
     // workspace
     double* ep;
     int32_t* ep_valid;
@@ -494,6 +502,7 @@ struct LaneB {
     LaneVec s, k1;  // the lane's state and its k1 scratch
     Stokes<double> st;
     LdsVec<double> mf;  // multi-frequency: Stokes component 4 f + c of this lane, in LDS
+    Stokes<D3> sd;      // tangent mode: Stokes vector with its three parameter derivatives
     double h_current, cap, h, h_try, call_min, call_max;
     int steps, reason, att, phase, crosses, reached, pixel, pass_a_steps;
 };
@@ -511,7 +520,7 @@ KP_DEV Stokes<double> observer_basis(const Stokes<double>& in, double r11, doubl
 // Writes the pixel's outputs.  A mixed-precision lane carries its Stokes
 // vector in nu_obs^3-scaled units; SCALED converts back to invariant ones.
 // MF: nfreq Stokes vectors from LDS to the [nfreq][npix] image arrays.
-template <bool SCALED, bool MF> KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
+template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, const LaneB& L, int valid) {
     St<double> cam, fin;
     camera_ray(P, L.pixel, cam);
     st_load(fin, L.s);
@@ -538,7 +547,7 @@ template <bool SCALED, bool MF> KP_DEV void lane_b_finish(const Params& P, const
     }
     const kp_image_data& O = P.out;
     const int p = L.pixel;
-    if (MF) {
+    if (MODE == M_MF) {
         for (int f = 0; f < P.nfreq; ++f) {
             Stokes<double> of = {0.0, 0.0, 0.0, 0.0};
             if (valid) {
@@ -552,6 +561,24 @@ template <bool SCALED, bool MF> KP_DEV void lane_b_finish(const Params& P, const
             if (O.image_q) O.image_q[q] = of.Q;
             if (O.image_u) O.image_u[q] = of.U;
             if (O.image_v) O.image_v[q] = of.V;
+        }
+    } else if (MODE == M_TANGENT) {
+        Stokes<double> ov = {0.0, 0.0, 0.0, 0.0};
+        if (valid) ov = observer_basis({L.sd.I.v, L.sd.Q.v, L.sd.U.v, L.sd.V.v}, r11, r12, r21, r22);
+        if (O.image_i) O.image_i[p] = ov.I;
+        if (O.image_q) O.image_q[p] = ov.Q;
+        if (O.image_u) O.image_u[p] = ov.U;
+        if (O.image_v) O.image_v[p] = ov.V;
+        const size_t np = (size_t)P.npix;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Stokes<double> g = {0.0, 0.0, 0.0, 0.0};
+            if (valid) g = observer_basis({L.sd.I.d[k], L.sd.Q.d[k], L.sd.U.d[k], L.sd.V.d[k]}, r11, r12, r21, r22);
+            double* gk = P.grad + (size_t)k * 4 * np + (size_t)p;
+            gk[0] = g.I;
+            gk[np] = g.Q;
+            gk[2 * np] = g.U;
+            gk[3 * np] = g.V;
         }
     } else {
         if (O.image_i) O.image_i[p] = o.I;
@@ -583,7 +610,7 @@ template <bool SCALED, bool MF> KP_DEV void lane_b_finish(const Params& P, const
 
 // Loads a pass A endpoint; returns false (after writing the pixel's outputs)
 // when the endpoint is not a valid radiating segment.
-template <bool MF> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
+template <int MODE> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L) {
     const size_t np = (size_t)P.npix;
 #pragma unroll
     for (int c = 0; c < 16; ++c) vset(L.s, c, P.ep[(size_t)c * np + pixel]);
@@ -591,8 +618,9 @@ template <bool MF> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L)
     L.pass_a_steps = P.ep_steps[pixel];
     L.reason = P.ep_reason[pixel];
     L.st = {0.0, 0.0, 0.0, 0.0};
-    if (MF)
+    if (MODE == M_MF)
         for (int j = 0; j < 4 * P.nfreq; ++j) L.mf.set(j, 0.0);
+    if (MODE == M_TANGENT) L.sd = {D3(0.0), D3(0.0), D3(0.0), D3(0.0)};
     L.h_current = P.step;
     L.cap = P.max_rs > 0.0 ? P.max_rs : P.max_step;
     L.h = 0.0;
@@ -605,7 +633,7 @@ template <bool MF> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L)
     L.crosses = 0;
     L.reached = 0;
     if (!P.ep_valid[pixel]) {
-        lane_b_finish<false, MF>(P, L, 0);
+        lane_b_finish<false, MODE>(P, L, 0);
         return false;
     }
     return true;
@@ -627,7 +655,7 @@ KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
 //    once; each control frequency's coefficients drive the substep count (the
 //    largest wins, pass_b.hpp:1416-1460) and each output frequency's the
 //    transfer of its own Stokes vector, held in LDS.
-template <bool MIXED, bool MF> struct Radiation;
+template <bool MIXED, int MODE> struct Radiation;
 
 KP_DEV void mf_strang(const Params& P, LaneB& L, int f, const Coeffs<double>& c, double dl) {
     Stokes<double> st = {L.mf[4 * f], L.mf[4 * f + 1], L.mf[4 * f + 2], L.mf[4 * f + 3]};
@@ -651,7 +679,7 @@ KP_DEV Coeffs<double> widen(const Coeffs<float>& c) {
     return {c.jI, c.jQ, c.jU, c.jV, c.aI, c.aQ, c.aU, c.aV, c.rQ, c.rU, c.rV};
 }
 
-template <> struct Radiation<false, false> {
+template <> struct Radiation<false, M_SINGLE> {
     Coeffs<double> c;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
@@ -669,7 +697,7 @@ template <> struct Radiation<false, false> {
     }
 };
 
-template <> struct Radiation<true, false> {
+template <> struct Radiation<true, M_SINGLE> {
     Coeffs<float> c;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
@@ -691,7 +719,7 @@ template <> struct Radiation<true, false> {
     }
 };
 
-template <> struct Radiation<false, true> {
+template <> struct Radiation<false, M_MF> {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
@@ -719,7 +747,7 @@ template <> struct Radiation<false, true> {
     }
 };
 
-template <> struct Radiation<true, true> {
+template <> struct Radiation<true, M_MF> {
     RiafLocal<float> loc;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
@@ -748,8 +776,71 @@ template <> struct Radiation<true, true> {
     }
 };
 
+// Tangent mode: the plasma scalars carry their derivatives with respect to
+// the log-scale parameters of PlasmaPerturbation (plasma_perturbation.hpp:12-33):
+// q_density scales n_e and B^2 together (d ln n_e = dq, d ln B = dq / 2),
+// q_temperature scales Theta_e, q_magnetic scales B.  The coefficients and
+// the Strang step are then evaluated in Dual<3> arithmetic; the substep
+// control uses the coefficient values, so the step sequence (the "fixed
+// geodesic") is the FP64 kernel's.
+KP_DEV void tangent_coeffs(const RiafDev<double>& M, const RiafLocal<double>& L, Coeffs<D3>& c) {
+    coeffs_zero(c);
+    if (!L.valid) return;
+    D3 ne(L.ne), th(L.thetae), b(L.b);
+    ne.d[0] = L.ne;
+    b.d[0] = 0.5 * L.b;
+    th.d[1] = L.thetae;
+    b.d[2] = L.b;
+    const D3 nu(maxv(M.freq_cgs * L.nu_scale, 1.0));
+    if (L.aligned) {
+        if (M.faraday_rotation) {
+            const D3 theta(L.cos_theta == 1.0 ? 0.0 : PI_D);
+            c.rV = D3(M.faraday_scale) * rho_v_aligned(nu, ne, th, b, theta, D3(M.min_sin_theta)) * nu;
+        }
+        return;
+    }
+    Coeffs<D3> mc;
+    thermal_coeffs(M.fit, nu, ne, th, b, D3(L.sin_theta), D3(L.cos_theta), D3(M.max_pol_frac), D3(M.min_sin_theta),
+                   D3(M.emission_scale), D3(M.absorption_scale), D3(M.faraday_scale), mc);
+    c.jI = mc.jI;
+    c.jV = mc.jV;
+    c.aI = mc.aI;
+    c.aV = mc.aV;
+    if (M.faraday_rotation) c.rV = mc.rV;
+    if (L.has_chi) {
+        const D3 c2(L.cos2chi), s2(L.sin2chi);
+        c.jQ = mc.jQ * c2;
+        c.jU = mc.jQ * s2;
+        c.aQ = mc.aQ * c2;
+        c.aU = mc.aQ * s2;
+        c.rQ = mc.rQ * c2;
+        c.rU = mc.rQ * s2;
+    }
+}
+
+template <> struct Radiation<false, M_TANGENT> {
+    Coeffs<D3> c;
+    KP_DEV void zero() { coeffs_zero(c); }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        RiafLocal<double> L;
+        riaf_local(P.riaf, sample, L);
+        tangent_coeffs(P.riaf, L, c);
+        return 1;
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        const Coeffs<double> v = value_of(c);
+        req = substep_count(v, h, P.riaf.dlambda_scale, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        pred = substep_count(v, h_next, P.riaf.dlambda_scale, lmrs, P.max_rd, P.max_ad, P.max_fd);
+        return 0;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        strang_step(L.sd, c, D3(absv(h) * P.riaf.dlambda_scale));
+        return 0;
+    }
+};
+
 // One tick of pass B; returns true when the ray has terminated.
-template <bool MIXED, bool MF>
+template <bool MIXED, int MODE>
 KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_t& coeff_acc,
                         uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     const double eff_min = absv(P.min_step);
@@ -824,7 +915,7 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
     L.phase = PH_IDLE;
     const double h = L.h;
     const double old_r = L.s[1], sample_r = sample.v[1], next_r = next.v[1];
-    Radiation<MIXED, MF> rad;
+    Radiation<MIXED, MODE> rad;
     rad.zero();
     if (sample_r - P.inner > 0.0 && sample_r < P.outer) coeff_acc += rad.eval(P, sample);
     const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
@@ -851,13 +942,13 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
     return false;
 }
 
-template <bool PERSISTENT, bool MIXED, bool MF>
+template <bool PERSISTENT, bool MIXED, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
     uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
     Geo<double> G{P.mass, P.spin, &rhs_count};
     LaneB L;
     lane_lds(L.s, L.k1);
-    if (MF) {
+    if (MODE == M_MF) {
         extern __shared__ double kp_lds[];
         L.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
     }
@@ -871,23 +962,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
                 if (!__ballot(need)) break;
                 int pix = 0;
                 if (fetch_pixel(&P.queue[1], need, (uint32_t)P.npix, pix)) {
-                    has = lane_b_init<MF>(P, pix, L);
+                    has = lane_b_init<MODE>(P, pix, L);
                 } else if (need) {
                     exhausted = true;
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_b_tick<MIXED, MF>(P, G, L, coeff, transfer, steps_b, guard)) {
-                lane_b_finish<MIXED, MF>(P, L, 1);
+            if (has && pass_b_tick<MIXED, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+                lane_b_finish<MIXED, MODE>(P, L, 1);
                 has = false;
             }
         }
     } else {
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-        if (pix < P.npix && lane_b_init<MF>(P, pix, L)) {
-            while (!pass_b_tick<MIXED, MF>(P, G, L, coeff, transfer, steps_b, guard)) {
+        if (pix < P.npix && lane_b_init<MODE>(P, pix, L)) {
+            while (!pass_b_tick<MIXED, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
             }
-            lane_b_finish<MIXED, MF>(P, L, 1);
+            lane_b_finish<MIXED, MODE>(P, L, 1);
         }
     }
     flush_counters(P.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
@@ -902,7 +993,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
 // from one queue, one wave-level atomic per refill; a wave refills only when
 // all of its lanes have finished both passes.
 // ---------------------------------------------------------------------------
-template <bool MIXED, bool MF>
+template <bool MIXED, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void unsplit_kernel(Params P) {
     uint32_t rhs_a = 0, rhs_b = 0, coeff = 0, transfer = 0, steps_a = 0, steps_b = 0, guard = 0;
     while (true) {
@@ -922,14 +1013,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
             Geo<double> G{P.mass, P.spin, &rhs_b};
             LaneB B;
             lane_lds(B.s, B.k1);
-            if (MF) {
+            if (MODE == M_MF) {
                 extern __shared__ double kp_lds[];
                 B.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
             }
-            if (lane_b_init<MF>(P, pix, B)) {
-                while (!pass_b_tick<MIXED, MF>(P, G, B, coeff, transfer, steps_b, guard)) {
+            if (lane_b_init<MODE>(P, pix, B)) {
+                while (!pass_b_tick<MIXED, MODE>(P, G, B, coeff, transfer, steps_b, guard)) {
                 }
-                lane_b_finish<MIXED, MF>(P, B, 1);
+                lane_b_finish<MIXED, MODE>(P, B, 1);
             }
         }
     }
@@ -1014,7 +1105,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
             if (fetch_pixel(&B.queue[1], need, B.total, g)) {
                 const int j = find_job(B, (uint32_t)g);
                 P = &B.jobs[j];
-                has = lane_b_init<false>(*P, g - (int)B.start[j], L);
+                has = lane_b_init<M_SINGLE>(*P, g - (int)B.start[j], L);
             } else if (need) {
                 exhausted = true;
             }
@@ -1022,8 +1113,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         if (!__ballot(has)) break;
         if (has) {
             const Geo<double> G{P->mass, P->spin, &rhs_count};
-            if (pass_b_tick<MIXED, false>(*P, G, L, coeff, transfer, steps_b, guard)) {
-                lane_b_finish<MIXED, false>(*P, L, 1);
+            if (pass_b_tick<MIXED, M_SINGLE>(*P, G, L, coeff, transfer, steps_b, guard)) {
+                lane_b_finish<MIXED, M_SINGLE>(*P, L, 1);
                 has = false;
             }
         }
@@ -1085,18 +1176,20 @@ struct Occupancy {
 template <bool PERSISTENT> const void* pass_a_fn(bool mixed) {
     return mixed ? (const void*)pass_a_kernel<PERSISTENT, true> : (const void*)pass_a_kernel<PERSISTENT, false>;
 }
-template <bool PERSISTENT> const void* pass_b_fn(bool mixed, bool mf) {
-    if (mf)
-        return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, true>
-                     : (const void*)pass_b_kernel<PERSISTENT, false, true>;
-    return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, false>
-                 : (const void*)pass_b_kernel<PERSISTENT, false, false>;
+template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int mode) {
+    if (mode == M_TANGENT) return (const void*)pass_b_kernel<PERSISTENT, false, M_TANGENT>;
+    if (mode == M_MF)
+        return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, M_MF>
+                     : (const void*)pass_b_kernel<PERSISTENT, false, M_MF>;
+    return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, M_SINGLE>
+                 : (const void*)pass_b_kernel<PERSISTENT, false, M_SINGLE>;
 }
 
-const void* unsplit_fn(bool mixed, bool mf) {
-    if (mf)
-        return mixed ? (const void*)unsplit_kernel<true, true> : (const void*)unsplit_kernel<false, true>;
-    return mixed ? (const void*)unsplit_kernel<true, false> : (const void*)unsplit_kernel<false, false>;
+const void* unsplit_fn(bool mixed, int mode) {
+    if (mode == M_TANGENT) return (const void*)unsplit_kernel<false, M_TANGENT>;
+    if (mode == M_MF)
+        return mixed ? (const void*)unsplit_kernel<true, M_MF> : (const void*)unsplit_kernel<false, M_MF>;
+    return mixed ? (const void*)unsplit_kernel<true, M_SINGLE> : (const void*)unsplit_kernel<false, M_SINGLE>;
 }
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
@@ -1106,13 +1199,13 @@ size_t lds_bytes(int block, int nfreq_mf) {
 }
 
 std::mutex g_occ_mutex;
-std::vector<std::pair<long long, Occupancy>> g_occ;  // key: ((device * 4096 + block) * 2 + mixed) * 32 + nfreq_mf
+std::vector<std::pair<long long, Occupancy>> g_occ;  // key: (((device * 4096 + block) * 2 + mixed) * 4 + mode) * 32 + nfreq_mf
 
-// nfreq_mf: output frequencies of a multi-frequency pass B, 0 for the
-// single-frequency kernel.
-kp_status occupancy_for(int device, int block, bool mixed, int nfreq_mf, Occupancy& occ) {
+// mode: pass B variant (Mode); nfreq_mf: output frequencies of a
+// multi-frequency pass B, 0 otherwise.
+kp_status occupancy_for(int device, int block, bool mixed, int mode, int nfreq_mf, Occupancy& occ) {
     std::lock_guard<std::mutex> lock(g_occ_mutex);
-    const long long key = (((long long)device * 4096 + block) * 2 + (mixed ? 1 : 0)) * 32 + nfreq_mf;
+    const long long key = ((((long long)device * 4096 + block) * 2 + (mixed ? 1 : 0)) * 4 + mode) * 32 + nfreq_mf;
     for (auto& e : g_occ)
         if (e.first == key) {
             occ = e.second;
@@ -1123,9 +1216,9 @@ kp_status occupancy_for(int device, int block, bool mixed, int nfreq_mf, Occupan
     KP_HIP(hipGetDeviceProperties(&prop, device));
     o.cus = prop.multiProcessorCount;
     KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed), block, lds_bytes(block, 0)));
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed, nfreq_mf > 0), block,
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed, mode), block,
                                                         lds_bytes(block, nfreq_mf)));
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_u, unsplit_fn(mixed, nfreq_mf > 0), block,
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_u, unsplit_fn(mixed, mode), block,
                                                         lds_bytes(block, nfreq_mf)));
     o.blocks_a = std::max(o.blocks_a, 1);
     o.blocks_b = std::max(o.blocks_b, 1);
@@ -1256,9 +1349,16 @@ static kp_status prepare_frequencies(const kp_riaf_model& m, const FreqLists& fl
     return KP_OK;
 }
 
+// Tangent request: evaluation point (q_density, q_temperature, q_magnetic)
+// and the [3][4][npix] device array for the derivatives.
+struct Tangent {
+    const double* log_scales = nullptr;
+    double* grad = nullptr;
+};
+
 static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl,
                            const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
-                           int perturbation, double log_scale) {
+                           int perturbation, double log_scale, const Tangent* tangent = nullptr) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -1281,6 +1381,22 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (mf) {
         st = prepare_frequencies(*model, fl, P);
         if (st != KP_OK) return st;
+    }
+    int mode = mf ? M_MF : M_SINGLE;
+    if (tangent) {
+        if (mixed || mf)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "parameter gradients are built for single-frequency FP64");
+        if (!tangent->log_scales || !tangent->grad)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "null log_scales or gradient array");
+        const double qd = tangent->log_scales[0], qt = tangent->log_scales[1], qm = tangent->log_scales[2];
+        if (!std::isfinite(qd) || !std::isfinite(qt) || !std::isfinite(qm))
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "log_scales must be finite");
+        // PlasmaPerturbation::apply per family (plasma_perturbation.hpp:17-32), composed
+        P.riaf.ne_factor = std::exp(qd);
+        P.riaf.thetae_factor = std::exp(qt);
+        P.riaf.b_factor = std::exp(0.5 * qd) * std::exp(qm);
+        P.grad = tangent->grad;
+        mode = M_TANGENT;
     }
     P.out = *out;
     P.persistent = opt.persistent ? 1 : 0;
@@ -1310,7 +1426,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (block % 64 != 0 || block > 256)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
     Occupancy occ;
-    st = occupancy_for(device, block, mixed, mf ? P.nfreq : 0, occ);
+    st = occupancy_for(device, block, mixed, mode, mf ? P.nfreq : 0, occ);
     if (st != KP_OK) return st;
     const size_t smem_a = lds_bytes(block, 0);
     const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0);
@@ -1335,14 +1451,14 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (stats) KP_HIP(hipEventRecord(ev[0], stream));
     void* args[] = {&P};
     if (P.npix > 0 && unsplit)
-        KP_HIP(hipLaunchKernel(unsplit_fn(mixed, mf), dim3(grid_u), dim3(block), args, smem_b, stream));
+        KP_HIP(hipLaunchKernel(unsplit_fn(mixed, mode), dim3(grid_u), dim3(block), args, smem_b, stream));
     if (P.npix > 0 && !unsplit) {
         const void* fn = P.persistent ? pass_a_fn<true>(mixed) : pass_a_fn<false>(mixed);
         KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem_a, stream));
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
     if (P.npix > 0 && !unsplit) {
-        const void* fn = P.persistent ? pass_b_fn<true>(mixed, mf) : pass_b_fn<false>(mixed, mf);
+        const void* fn = P.persistent ? pass_b_fn<true>(mixed, mode) : pass_b_fn<false>(mixed, mode);
         KP_HIP(hipLaunchKernel(fn, dim3(grid_b), dim3(block), args, smem_b, stream));
     }
     if (stats) {
@@ -1622,6 +1738,17 @@ extern "C" kp_status kp_run_riaf_multifrequency_image(const kp_pass_a_params* pa
     fl.ctrl = step_control_frequencies;
     if (nfreq < 1) return kp_internal::fail(KP_INVALID_ARGUMENT, "nfreq must be at least 1");
     return run_image(params, model, fl, options, out, stats, 0, 0.0);
+}
+
+extern "C" kp_status kp_run_riaf_image_gradient(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                                int32_t radiation_substeps, const double* log_scales,
+                                                const kp_run_options* options, kp_image_data* out, double* grad,
+                                                kp_image_stats* stats) {
+    (void)radiation_substeps;
+    Tangent t;
+    t.log_scales = log_scales;
+    t.grad = grad;
+    return run_image(params, model, FreqLists{}, options, out, stats, 0, 0.0, &t);
 }
 
 // Host-output wrapper shared by the single- and multi-frequency entries:

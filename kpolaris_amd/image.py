@@ -71,6 +71,9 @@ def library():
     L.kp_pack_image.argtypes = [dp, dp, dp, dp, ip, C.c_int64, C.c_double, C.c_int32, C.c_double,
                                 dp, dp, dp, dp, dp]
     L.kp_pack_image.restype = C.c_int32
+    L.kp_run_riaf_image_gradient.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(C.c_double),
+                                             P(abi.RunOptions), P(abi.ImageData), P(C.c_double), P(abi.ImageStats)]
+    L.kp_run_riaf_image_gradient.restype = C.c_int32
     L.kp_batch_create.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.RunOptions), P(C.c_void_p)]
     L.kp_batch_create.restype = C.c_int32
     L.kp_batch_launch.argtypes = [C.c_void_p, C.c_void_p]
@@ -297,3 +300,27 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def run_riaf_image_gradient(params: abi.PassAParams, model: abi.RIAFModel, log_scales=(0.0, 0.0, 0.0), *,
+                            persistent: bool = True, block_size: int = 0, out: dict | None = None,
+                            fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+    """Image at plasma log-scales (q_density, q_temperature, q_magnetic) and its
+    derivatives d(I,Q,U,V)/dq as a [3, 4, npix] device tensor (float64)."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    if out is None:
+        out = alloc_device_image(npix, fields=fields)
+    grad = torch.empty((3, 4, npix), dtype=torch.float64, device="cuda")
+    q = np.ascontiguousarray(log_scales, dtype=np.float64)
+    if q.shape != (3,):
+        raise ValueError("log_scales must hold 3 values")
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, persistent, block_size, 0, C.c_void_p(stream))
+    st = abi.ImageStats()
+    dp = C.POINTER(C.c_double)
+    _check(library().kp_run_riaf_image_gradient(
+        C.byref(params), C.byref(model), 1, q.ctypes.data_as(dp), C.byref(opts), C.byref(img),
+        C.cast(C.c_void_p(grad.data_ptr()), dp), C.byref(st) if with_stats else None))
+    return out, grad, (st.as_dict() if with_stats else None)

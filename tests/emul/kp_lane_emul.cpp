@@ -7,9 +7,10 @@
 
 #include <vector>
 
-template <bool MIXED, bool MF>
+template <bool MIXED, int MODE>
 static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, double fp64_radius,
-                          const FreqLists& fl, kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps) {
+                          const FreqLists& fl, kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps,
+                          const double* log_scales = nullptr, double* grad = nullptr) {
     Params P;
     kp_status st = prepare(pa, m, P);
     if (st != KP_OK) return st;
@@ -17,9 +18,15 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         prepare_mixed(P);
         P.fp64_radius = fp64_radius;
     }
-    if (MF) {
+    if (MODE == M_MF) {
         st = prepare_frequencies(*m, fl, P);
         if (st != KP_OK) return st;
+    }
+    if (MODE == M_TANGENT) {
+        P.riaf.ne_factor = std::exp(log_scales[0]);
+        P.riaf.thetae_factor = std::exp(log_scales[1]);
+        P.riaf.b_factor = std::exp(0.5 * log_scales[0]) * std::exp(log_scales[2]);
+        P.grad = grad;
     }
     double mfbuf[4 * KP_MAX_FREQUENCIES];
     const int np = P.npix;
@@ -41,10 +48,10 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         lane_a_store(P, A);
         LaneB B;
         B.mf = LdsVec<double>{mfbuf, 1};
-        if (lane_b_init<MF>(P, pix, B)) {
-            while (!pass_b_tick<MIXED, MF>(P, G, B, c1, c2, sb, guard)) {
+        if (lane_b_init<MODE>(P, pix, B)) {
+            while (!pass_b_tick<MIXED, MODE>(P, G, B, c1, c2, sb, guard)) {
             }
-            lane_b_finish<MIXED, MF>(P, B, 1);
+            lane_b_finish<MIXED, MODE>(P, B, 1);
         }
         total += rc;
         guard_total += guard;
@@ -84,3 +91,9 @@ extern "C" kp_status kp_emul_run_riaf_multifrequency_image(const kp_pass_a_param
 
 // The kernels' bounded-argument sine/cosine, for its accuracy test.
 extern "C" void kp_emul_sincos(double x, double* s, double* c) { kpd::sincos_bounded(x, s, c); }
+
+// Tangent (plasma-parameter derivative) lanes; grad is [3][4][npix].
+extern "C" kp_status kp_emul_run_riaf_image_gradient(const kp_pass_a_params* pa, const kp_riaf_model* m,
+                                                     const double* log_scales, kp_image_data* out, double* grad) {
+    return emul_run<false, M_TANGENT>(pa, m, 0.0, FreqLists{}, out, nullptr, nullptr, log_scales, grad);
+}

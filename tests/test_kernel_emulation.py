@@ -36,6 +36,9 @@ def emul():
         C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.c_int32, C.c_double, dp, C.c_int32, dp,
         C.c_int32, C.POINTER(abi.ImageData), C.POINTER(C.c_int64)]
     L.kp_emul_run_riaf_multifrequency_image.restype = C.c_int32
+    L.kp_emul_run_riaf_image_gradient.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), dp,
+                                                  C.POINTER(abi.ImageData), dp]
+    L.kp_emul_run_riaf_image_gradient.restype = C.c_int32
     return L
 
 
@@ -192,3 +195,39 @@ def test_emulated_multifrequency_mixed(emul):
     r, _ = oracle.run_riaf_multifrequency_image(p, m, MF_FREQS)
     for f, (gf, rf) in enumerate(zip(parity.split_frequencies(e, 3), parity.split_frequencies(r, 3))):
         parity.assert_mixed(gf, rf, f"emulated mixed mf f{f}")
+
+
+def oracle_central_difference(p, m, parameter: int, h: float = 1e-4):
+    """(S(q=+h) - S(q=-h)) / 2h of the oracle for one PlasmaPerturbation family,
+    and the mask of pixels whose step sequence is the same in both runs."""
+    pp, pm = oracle.Perturbation(), oracle.Perturbation()
+    pp.parameter = pm.parameter = parameter
+    pp.log_scale, pm.log_scale = h, -h
+    rp, _ = oracle.run_riaf_image(p, m, perturbation=pp)
+    rm, _ = oracle.run_riaf_image(p, m, perturbation=pm)
+    fd = {n: (rp[n] - rm[n]) / (2 * h) for n in parity.STOKES}
+    same = (rp["steps"] == rm["steps"]) & (rp["pass_a_steps"] == rm["pass_a_steps"])
+    return fd, same
+
+
+def test_emulated_tangent_matches_oracle_central_differences(emul):
+    """The tangent lanes' dS/dq against central differences of the oracle
+    (h = 1e-4 in q: truncation ~h^2 = 1e-8 relative)."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=10, ny=10)
+    n = 100
+    arrays = oracle.alloc_image(n)
+    img = oracle.image_struct(arrays)
+    g = np.zeros(3 * 4 * n)
+    q = np.zeros(3)
+    dp = C.POINTER(C.c_double)
+    assert emul.kp_emul_run_riaf_image_gradient(C.byref(p), C.byref(m), q.ctypes.data_as(dp), C.byref(img),
+                                                g.ctypes.data_as(dp)) == 0
+    ref = parity.oracle_with_sensitivity(p, m)
+    parity.assert_parity(arrays, ref, "emulated tangent value")
+    g = g.reshape(3, 4, n)
+    for k in range(3):
+        fd, same = oracle_central_difference(p, m, k + 1)
+        assert same.all()
+        for s, name in enumerate(parity.STOKES):
+            scale = np.abs(fd[name]).max()
+            np.testing.assert_allclose(g[k, s], fd[name], rtol=0, atol=1e-6 * scale, err_msg=f"q{k} {name}")
