@@ -137,6 +137,72 @@ def summarize(p, m, per_step, wall, steps) -> dict:
     }
 
 
+def bench_grmhd(args) -> int:
+    """512x512 image of a seeded 288x128x128 FMKS snapshot (FM torus r_in = 6,
+    r_max = 12, a = 0.9375, log-normal perturbations, beta_min = 100,
+    R_high = 40) at 8 frequencies log-spaced 86-345 GHz sharing the
+    geodesics, inclination 163 deg; FP64.  Besides rays/s it reports the
+    derived-scalar pass against HBM bandwidth (32 bytes read + 40 written per cell)."""
+    import numpy as np
+    import torch
+    from kpolaris_amd import grmhd_synth as gs, image
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    cells, m = gs.make_snapshot(288, 128, 128)
+    gen_s = time.perf_counter() - t0
+    grid = image.GRMHDGrid(m, cells, derive=False)
+    ncell = cells.size
+    # derived-scalar pass: HIP events around repeated launches on the current stream
+    for _ in range(2):
+        grid.derive()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        grid.derive()
+    e1.record()
+    torch.cuda.synchronize()
+    derive_ms = e0.elapsed_time(e1) / reps
+    derive_gbs = ncell * 72 / (derive_ms * 1e-3) / 1e9
+    p = gs.pass_a_params(m, nx=512, ny=512)
+    freqs = list(np.geomspace(86e9, 345e9, 8))
+    npix = p.camera.nx * p.camera.ny
+    out = None
+    for _ in range(args.warmup):
+        out, _ = image.run_grmhd_image(p, grid, freqs, out=out)
+    torch.cuda.synchronize()
+    per_step = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st = image.run_grmhd_image(p, grid, freqs, out=out)
+        per_step.append(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = statistics.median(s["kernel_ms"] for s in per_step)
+    st = per_step[-1]
+    print(json.dumps({
+        "metric": "rays/s on 1 MI355X, synthetic GRMHD (FMKS) image, 8 frequencies, FP64",
+        "value": args.steps * npix / wall, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Fishbone-Moncrief torus snapshot)",
+        "config": {"workload": "GRMHD 288x128x128 FMKS grid, 512x512 pinhole, 8 frequencies 86-345 GHz, "
+                               "i=163 deg, adaptive step-doubling RK4 rtol 1e-6",
+                   "rays_per_step": npix, "snapshot_generation_s": gen_s,
+                   "kernel_ms_median": kms, "pass_a_ms": st["pass_a_ms"], "pass_b_ms": st["pass_b_ms"],
+                   "steps_per_s": (st["pass_a_steps"] + st["pass_b_steps"]) / (kms * 1e-3),
+                   "coefficient_evals": st["coefficient_evals"],
+                   # one grid sample (8 cell records, 576 bytes) per radiating step; the
+                   # coefficient counter counts 8 control + 8 output frequencies per step
+                   "grid_sample_bytes_per_s": st["coefficient_evals"] / 16 * 576 / (st["pass_b_ms"] * 1e-3)},
+        "roofline": {"bound": "hbm", "kernel": "derive_scalars", "achieved": derive_gbs, "peak": 8000.0,
+                     "unit": "GB/s", "frac": derive_gbs / 8000.0, "traffic": None,
+                     "algorithmic_bytes_per_launch": ncell * 72, "launch_ms": derive_ms},
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,11 +212,16 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="both (default): the metric's mixed run plus the FP64 run it is compared with")
+    ap.add_argument("--workload", choices=["riaf", "grmhd"], default="riaf",
+                    help="riaf (default): the headline metric; grmhd: the 512x512, 8-frequency synthetic "
+                         "GRMHD snapshot on a 288x128x128 FMKS grid (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the imaging kernel runs on a single MI355X "
               "(pixels shard trivially across GPUs, but no multi-GPU path is built)", file=sys.stderr)
         return 2
+    if args.workload == "grmhd":
+        return bench_grmhd(args)
     import torch
     from kpolaris_amd import abi, image
     assert torch.cuda.is_available(), "bench.py needs a HIP device"

@@ -201,6 +201,41 @@ typedef struct kp_image_data {
     int32_t* reason;
 } kp_image_data;
 
+/*
+ * GRMHD grid model on FMKS data: GRMHDGridRadiationModel
+ * (src/model/grmhd_grid.hpp:31-99) with primitives and derived scalars
+ * (has_derived_scalars = 1, the iharm loader's default).  Field defaults are
+ * the reference's (kp_grmhd_model_default).  `cells` is a DEVICE array of
+ * n1*n2*n3 kp_grmhd_cell records in (i, j, k) row-major order, k fastest.
+ */
+typedef struct kp_grmhd_model {
+    int32_t n1, n2, n3;
+    int32_t emission_type;      /* 4: thermal Dexter (default), 1: thermal Pandya */
+    double spin, gam;
+    double startx1, startx2, startx3, dx1, dx2, dx3;
+    double r_in, r_out;
+    double hslope, mks_smooth, poly_alpha, poly_xt, poly_norm;
+    double freq_cgs, mbh_solar, M_unit;
+    double trat_small, trat_large, beta_crit;   /* R_low, R_high, beta_crit */
+    double sigma_cut, sigma_cut_high;
+    double ne_factor;
+    double emission_scale, absorption_scale, faraday_scale, max_pol_frac;
+    const void* cells;          /* kp_grmhd_cell[n1 * n2 * n3], device */
+} kp_grmhd_model;
+
+/*
+ * One cell of the device grid: the 8 primitives as read from the dump
+ * (rho, u, U1, U2, U3, B1, B2, B3 in native FMKS components, single
+ * precision as the iharm reader reads them, src/grmhd/iharm_loader.cpp:262-396)
+ * and the 5 derived scalars (n_e [cm^-3], Theta_e, B [G], sigma, beta;
+ * iharm_loader.cpp:420-505) in double.  A trilinear sample reads 8 whole
+ * records (72 bytes each) instead of 8 entries of 13 separate arrays.
+ */
+typedef struct kp_grmhd_cell {
+    float prim[8];
+    double derived[5];
+} kp_grmhd_cell;
+
 /* Launch configuration; no reference counterpart (Kokkos chose it there). */
 typedef struct kp_run_options {
     int32_t precision;        /* kp_precision */
@@ -362,11 +397,39 @@ void kp_batch_destroy(kp_batch* batch);
 kp_status kp_run_riaf_batch(const kp_batch_job* jobs, int32_t njobs, const kp_run_options* options,
                             kp_image_stats* stats);
 
+/*
+ * GRMHD grid image on FMKS geodesics (params->coordinate_system must be
+ * KP_COORD_FMKS with params->fmks_* describing the geodesic coordinates).
+ * Replaces run_grmhd_image (src/image/grmhd.hpp:12-17, FMKS backend
+ * src/image/grmhd.cpp) for FMKS dumps with derived scalars; FP64 only.
+ * `out` holds DEVICE pointers, as for kp_run_riaf_image.
+ */
+kp_status kp_run_grmhd_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                             int32_t radiation_substeps, const kp_run_options* options,
+                             kp_image_data* out, kp_image_stats* stats);
+
+/* Multi-frequency GRMHD image on shared geodesics; replaces
+ * run_grmhd_multifrequency_image (src/image/grmhd.hpp:31-38).  Arrays and
+ * frequency lists as for kp_run_riaf_multifrequency_image. */
+kp_status kp_run_grmhd_multifrequency_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                            int32_t radiation_substeps, const double* frequencies,
+                                            int32_t nfreq, const double* step_control_frequencies,
+                                            int32_t step_control_nfreq, const kp_run_options* options,
+                                            kp_image_data* out, kp_image_stats* stats);
+
+/* Fills the derived scalars of every cell of model->cells (DEVICE records
+ * whose primitives are set) on `stream`: the iharm loader's derived-scalar
+ * pass (src/grmhd/iharm_loader.cpp:420-505).  No reference ABI counterpart
+ * (the loader runs it inside load_iharm_model_from_hdf5). */
+kp_status kp_grmhd_derive_scalars(const kp_grmhd_model* model, void* stream);
+
 /* Defaults of the reference's structs (field initialisers of CameraParams,
  * PassAParams and RIAFAnalyticRadiationModel) and of kp_run_options. */
 void kp_pass_a_params_default(kp_pass_a_params* params);
 void kp_riaf_model_default(kp_riaf_model* model);
 void kp_run_options_default(kp_run_options* options);
+/* GRMHDGridRadiationModel field initialisers (src/model/grmhd_grid.hpp:53-80). */
+void kp_grmhd_model_default(kp_grmhd_model* model);
 
 /* ---- Host option layer (the reference CLI's rules) ------------------------
  * Mirrors the image tool's Options struct subset on the hot path

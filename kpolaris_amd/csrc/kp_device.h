@@ -273,6 +273,47 @@ template <class T> KP_DEV void glower(const BLGcov<T>& g, const T* v, T* out) {
     out[3] = g.g03 * v[0] + g.g33 * v[3];
 }
 
+// A general symmetric 4x4 metric (FMKS has g01, g12, g13 off-diagonals).
+template <class T> struct Metric4 {
+    T g00, g01, g02, g03, g11, g12, g13, g22, g23, g33;
+};
+
+template <class T> KP_DEV Metric4<T> metric4_of(const BLGcov<T>& b) {
+    Metric4<T> g;
+    g.g00 = b.g00;
+    g.g01 = T(0);
+    g.g02 = T(0);
+    g.g03 = b.g03;
+    g.g11 = b.g11;
+    g.g12 = T(0);
+    g.g13 = T(0);
+    g.g22 = b.g22;
+    g.g23 = T(0);
+    g.g33 = b.g33;
+    return g;
+}
+
+// metric.dot(x, u, v) (e.g. kerr_fmks.hpp:196-204).
+template <class T> KP_DEV T dot4(const Metric4<T>& g, const T* u, const T* v) {
+    return g.g00 * u[0] * v[0] + g.g11 * u[1] * v[1] + g.g22 * u[2] * v[2] + g.g33 * u[3] * v[3] +
+           g.g01 * (u[0] * v[1] + u[1] * v[0]) + g.g02 * (u[0] * v[2] + u[2] * v[0]) +
+           g.g03 * (u[0] * v[3] + u[3] * v[0]) + g.g12 * (u[1] * v[2] + u[2] * v[1]) +
+           g.g13 * (u[1] * v[3] + u[3] * v[1]) + g.g23 * (u[2] * v[3] + u[3] * v[2]);
+}
+
+template <class T> KP_DEV T frame_error4(const Metric4<T>& g, const St<T>& s) {
+    const T* k = s.v + 4;
+    const T* e1 = s.v + 8;
+    const T* e2 = s.v + 12;
+    T out = absv(dot4(g, k, k));
+    out = maxv(out, absv(dot4(g, e1, k)));
+    out = maxv(out, absv(dot4(g, e2, k)));
+    out = maxv(out, absv(dot4(g, e1, e1) - T(1)));
+    out = maxv(out, absv(dot4(g, e2, e2) - T(1)));
+    out = maxv(out, absv(dot4(g, e1, e2)));
+    return out;
+}
+
 // max_frame_error(frame_errors(metric, s)) (src/geodesic/state.hpp:44-76).
 template <class T> KP_DEV T frame_error(const BLGcov<T>& g, const St<T>& s) {
     const T* k = s.v + 4;
@@ -290,10 +331,7 @@ template <class T> KP_DEV T frame_error(const BLGcov<T>& g, const St<T>& s) {
 // ---------------------------------------------------------------------------
 // RK4 family (src/geodesic/rk4.hpp:36-59, 74-89).
 // ---------------------------------------------------------------------------
-template <class T> struct Geo {
-    T m, a;
-    uint32_t* rhs_count;
-};
+template <class T> struct Geo;
 
 template <class T> KP_DEV void rhs(const Geo<T>& g, const St<T>& s, St<T>& d) {
     ++*g.rhs_count;
@@ -328,6 +366,15 @@ template <class T, class Y, class Sink> KP_DEV void bl_rhs_apply(T m, T a, const
     for (int mu = 0; mu < 4; ++mu) sink(12 + mu, -out[mu]);
 }
 
+// Integration context for the Boyer-Lindquist metric: mass, spin and the
+// lane's RHS counter.  apply() delivers the transport RHS component-wise.
+template <class T> struct Geo {
+    using real = T;
+    T m, a;
+    uint32_t* rhs_count;
+    template <class Y, class Sink> KP_DEV void apply(const Y& y, Sink&& sink) const { bl_rhs_apply(m, a, y, sink); }
+};
+
 template <class T> KP_DEV void vset(St<T>& v, int i, T x) { v.v[i] = x; }
 template <class T> KP_DEV void vset(const LdsVec<T>& v, int i, T x) { v.set(i, x); }
 
@@ -335,33 +382,33 @@ template <class T> KP_DEV void vset(const LdsVec<T>& v, int i, T x) { v.set(i, x
 // in that order, the step is base + (acc + k4) * (h / 6) exactly as the
 // reference sums it.  The base vector may live in registers or in LDS.
 // f = f(y); acc += 2 f; y = base + f * c
-template <class T, class B>
-KP_DEV void stage_mid(const Geo<T>& g, St<T>& y, const B& base, St<T>& acc, T c) {
+template <class G, class T, class B>
+KP_DEV void stage_mid(const G& g, St<T>& y, const B& base, St<T>& acc, T c) {
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, y, [&](int i, T f) {
+    g.apply(y, [&](int i, T f) {
         acc.v[i] = acc.v[i] + T(2) * f;
         y.v[i] = base[i] + f * c;
     });
 }
 
 // f = f(y); y = base + (acc + f) * c
-template <class T, class B>
-KP_DEV void stage_last(const Geo<T>& g, St<T>& y, const B& base, const St<T>& acc, T c) {
+template <class G, class T, class B>
+KP_DEV void stage_last(const G& g, St<T>& y, const B& base, const St<T>& acc, T c) {
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, y, [&](int i, T f) { y.v[i] = base[i] + (acc.v[i] + f) * c; });
+    g.apply(y, [&](int i, T f) { y.v[i] = base[i] + (acc.v[i] + f) * c; });
 }
 
 // f = f(src); acc = f; y = src + f * c
-template <class T, class B> KP_DEV void stage_first(const Geo<T>& g, const B& src, St<T>& acc, St<T>& y, T c) {
+template <class G, class T, class B> KP_DEV void stage_first(const G& g, const B& src, St<T>& acc, St<T>& y, T c) {
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, src, [&](int i, T f) {
+    g.apply(src, [&](int i, T f) {
         acc.v[i] = f;
         y.v[i] = src[i] + f * c;
     });
 }
 
 // RK4 step s -> out (rk4.hpp:36-46).
-template <class T, class B> KP_DEV void rk4(const Geo<T>& g, const B& s, T h, St<T>& out) {
+template <class G, class T, class B> KP_DEV void rk4(const G& g, const B& s, T h, St<T>& out) {
     St<T> acc;
     const T hh = T(0.5) * h;
     stage_first(g, s, acc, out, hh);
@@ -386,11 +433,11 @@ template <class T> KP_DEV T err_norm(const St<T>& a, const St<T>& b) {
 // the full step's last stage is compared with the two-half-step state as it is
 // formed, so the full-step state is never held.  s (read only) and the k1
 // scratch K may live in registers or LDS.  Returns the error norm.
-template <class T, class S, class K>
-KP_DEV T doubling_attempt(const Geo<T>& g, const S& s, T h, K& k1, St<T>& half, St<T>& two_half) {
+template <class G, class T, class S, class K>
+KP_DEV T doubling_attempt(const G& g, const S& s, T h, K& k1, St<T>& half, St<T>& two_half) {
     St<T> acc;
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, s, [&](int i, T f) { vset(k1, i, f); });
+    g.apply(s, [&](int i, T f) { vset(k1, i, f); });
     const T h2 = h * T(0.5);
     const T q2 = T(0.5) * h2;
     // first half step from s with k1
@@ -422,7 +469,7 @@ KP_DEV T doubling_attempt(const Geo<T>& g, const S& s, T h, K& k1, St<T>& half, 
     T err = T(0);
     const T h6 = h / T(6);
     ++*g.rhs_count;
-    bl_rhs_apply(g.m, g.a, y, [&](int i, T f) {
+    g.apply(y, [&](int i, T f) {
         const T full = s[i] + (acc.v[i] + f) * h6;
         const T two = two_half.v[i];
         const T sc = maxv(T(1), maxv(absv(full), absv(two)));
@@ -431,8 +478,8 @@ KP_DEV T doubling_attempt(const Geo<T>& g, const S& s, T h, K& k1, St<T>& half, 
     return err;
 }
 
-template <class T>
-KP_DEV T doubling_attempt(const Geo<T>& g, const St<T>& s, T h, St<T>& half, St<T>& two_half) {
+template <class G, class T>
+KP_DEV T doubling_attempt(const G& g, const St<T>& s, T h, St<T>& half, St<T>& two_half) {
     St<T> k1;
     return doubling_attempt(g, s, h, k1, half, two_half);
 }

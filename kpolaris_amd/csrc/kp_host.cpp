@@ -157,6 +157,147 @@ void bl_camera_tetrad(double mass, double spin, double radius, double inclinatio
     gcov5[4] = M.g[3][3];
 }
 
+namespace {
+// FMKS theta(x1, x2) and its first derivatives (src/geometry/kerr_fmks.hpp:37-80).
+double fmks_pow(double x, double p) {
+    const int n = (int)(p + 0.5);
+    if (std::fabs(p - (double)n) < 1e-10 && n >= 0) {
+        double y = 1.0;
+        const double ax = std::fabs(x);
+        for (int i = 0; i < n; ++i) y *= ax;
+        if (x < 0.0 && (n % 2) == 1) y = -y;
+        return y;
+    }
+    return std::pow(std::fabs(x), p);
+}
+
+void fmks_theta_d(const double* f, double x1, double x2, double& th, double& d1, double& d2) {
+    const double pi = 3.141592653589793238462643383279502884;
+    const double hs = f[1], ms = f[2], pa = f[3], pxt = f[4], pn = f[5];
+    const double thG = pi * x2 + ((1.0 - hs) / 2.0) * std::sin(2.0 * pi * x2);
+    const double y = 2.0 * x2 - 1.0;
+    const double thJ = pn * y * (1.0 + fmks_pow(y / pxt, pa) / (pa + 1.0)) + 0.5 * pi;
+    const double ex = std::exp(ms * (f[0] - x1));
+    th = thG + ex * (thJ - thG);
+    d1 = -ms * ex * (thJ - thG);
+    const double dthG = pi + (1.0 - hs) * pi * std::cos(2.0 * pi * x2);
+    const double dthJ = 2.0 * pn * (1.0 + fmks_pow(y / pxt, pa) / (pa + 1.0)) +
+                        (2.0 * pa * pn * y * fmks_pow(y / pxt, pa - 1.0)) / ((pa + 1.0) * pxt);
+    d2 = dthG + ex * (dthJ - dthG);
+}
+
+// KerrFMKSMetric::gcov_matrix (J^T g_KS J, kerr_fmks.hpp:125-157) and gcon by
+// Gauss-Jordan inversion with a tiny-pivot floor (:159-188).
+Metric4 fmks_metric(double mass, double spin, const double* f, const double* x) {
+    Metric4 M;
+    double th, d1, d2;
+    fmks_theta_d(f, x[1], x[2], th, d1, d2);
+    const double r = std::exp(x[1]);
+    const double sth = std::sin(th), cth = std::cos(th);
+    const double s2 = sth * sth, a2 = spin * spin;
+    const double rho2 = r * r + a2 * cth * cth;
+    const double fr = 2.0 * mass * r / rho2;
+    double gks[4][4] = {};
+    gks[0][0] = -1.0 + fr;
+    gks[0][1] = gks[1][0] = fr;
+    gks[0][3] = gks[3][0] = -spin * fr * s2;
+    gks[1][1] = 1.0 + fr;
+    gks[1][3] = gks[3][1] = -spin * s2 * (1.0 + fr);
+    gks[2][2] = rho2;
+    gks[3][3] = s2 * (rho2 + a2 * s2 * (1.0 + fr));
+    double jac[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    jac[1][1] = r;
+    jac[2][1] = d1;
+    jac[2][2] = d2;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int nu = 0; nu < 4; ++nu) {
+            double sum = 0.0;
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) sum += gks[a][b] * jac[a][mu] * jac[b][nu];
+            M.g[mu][nu] = sum;
+        }
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            m[i][j] = M.g[i][j];
+            m[i][j + 4] = i == j ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        double pivot = m[col][col];
+        if (std::fabs(pivot) < 1e-300) pivot = pivot < 0.0 ? -1e-300 : 1e-300;
+        const double ip = 1.0 / pivot;
+        for (int j = 0; j < 8; ++j) m[col][j] *= ip;
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            const double fac = m[row][col];
+            for (int j = 0; j < 8; ++j) m[row][j] -= fac * m[col][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) M.c[i][j] = m[i][j + 4];
+    return M;
+}
+}  // namespace
+
+// make_ipole_camera_tetrad with the FMKS branches (camera.hpp:303-315, 396-453):
+// position (0, ln R, x2(ln R, inclination), 0) with x2 by 64 bisections
+// (kerr_fmks.hpp:107-121), normal observer, centring covector (1, 1, 0, 0),
+// trial vectors (0, 0, 1, 0) and (1, 1, 1, 1).
+void fmks_camera_tetrad(double mass, double spin, const double fm[6], double radius, double inclination,
+                        double xcam[4], double tetrad[16], double gcov16[16]) {
+    const double x1 = std::log(std::max(radius, 1e-300));
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        double th, d1, d2;
+        fmks_theta_d(fm, x1, mid, th, d1, d2);
+        if (th < inclination)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    xcam[0] = 0.0;
+    xcam[1] = x1;
+    xcam[2] = 0.5 * (lo + hi);
+    xcam[3] = 0.0;
+    const Metric4 M = fmks_metric(mass, spin, fm, xcam);
+    double* e0 = tetrad;
+    double* e1 = tetrad + 4;
+    double* e2 = tetrad + 8;
+    double* e3 = tetrad + 12;
+    for (int mu = 0; mu < 4; ++mu) e0[mu] = -M.c[mu][0];
+    {
+        const double n2 = mdot(M, e0, e0);
+        if (n2 < 0.0)
+            for (int mu = 0; mu < 4; ++mu) e0[mu] *= 1.0 / std::sqrt(-n2);
+    }
+    const double centre_cov[4] = {1.0, 1.0, 0.0, 0.0};
+    raise(M, centre_cov, e3);
+    for (int i = 0; i < 4; ++i) {
+        e2[i] = i == 2 ? 1.0 : 0.0;
+        e1[i] = 1.0;
+    }
+    normalise(M, e0);
+    project_out(M, e3, e0);
+    project_out(M, e3, e0);
+    normalise(M, e3);
+    for (int rep = 0; rep < 2; ++rep) {
+        project_out(M, e2, e0);
+        project_out(M, e2, e3);
+    }
+    normalise(M, e2);
+    for (int rep = 0; rep < 2; ++rep) {
+        project_out(M, e1, e0);
+        project_out(M, e1, e2);
+        project_out(M, e1, e3);
+    }
+    normalise(M, e1);
+    if (orientation(e0, e1, e2, e3) < 0.0)
+        for (int i = 0; i < 4; ++i) e1[i] = -e1[i];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) gcov16[i * 4 + j] = M.g[i][j];
+}
+
 // isco_radius (src/model/riaf.hpp:168-176): Bardeen-Press-Teukolsky.
 double isco_radius(double a) {
     const double z1 = 1.0 + std::cbrt(1.0 - a * a) * (std::cbrt(1.0 + a) + std::cbrt(1.0 - a));
@@ -299,6 +440,35 @@ extern "C" void kp_riaf_model_default(kp_riaf_model* m) {
     m->min_sin_theta = 1e-4;
     m->min_thetae = 1e-3;
     m->min_radius = 1e-6;
+}
+
+// GRMHDGridRadiationModel field defaults (src/model/grmhd_grid.hpp:53-80).
+extern "C" void kp_grmhd_model_default(kp_grmhd_model* m) {
+    if (!m) return;
+    std::memset(m, 0, sizeof(*m));
+    m->emission_type = KP_EMISSION_THERMAL_DEXTER;
+    m->spin = 0.5;
+    m->gam = 4.0 / 3.0;
+    m->dx1 = m->dx2 = m->dx3 = 1.0;
+    m->r_in = 1.0;
+    m->r_out = 1000.0;
+    m->hslope = 0.3;
+    m->mks_smooth = 0.5;
+    m->poly_alpha = 14.0;
+    m->poly_xt = 0.82;
+    m->poly_norm = 1.0;
+    m->freq_cgs = 230.0e9;
+    m->mbh_solar = 6.2e9;
+    m->M_unit = 3.0e25;
+    m->trat_small = 1.0;
+    m->trat_large = 20.0;
+    m->beta_crit = 1.0;
+    m->sigma_cut = 1.0;
+    m->sigma_cut_high = -1.0;
+    m->ne_factor = 1.0;
+    m->emission_scale = m->absorption_scale = m->faraday_scale = 1.0;
+    m->max_pol_frac = 0.99;
+    m->cells = nullptr;
 }
 
 extern "C" void kp_run_options_default(kp_run_options* o) {

@@ -21,6 +21,8 @@
 #include "kp_riaf.h"
 #include "kp_mixed.h"
 #include "kp_dual.h"
+#include "kp_fmks.h"
+#include "kp_grmhd.h"
 #include "kp_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -67,7 +69,13 @@ enum : int {
 // Pass B variants: one frequency, several frequencies on shared geodesics, or
 // one frequency with the tangent (plasma-parameter derivatives) of its Stokes
 // vector carried alongside.
-enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2 };
+enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3 };
+
+// Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
+// (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
+enum Geom : int { G_BL = 0, G_FMKS = 1 };
+template <int MODE> constexpr int geom_of() { return MODE == M_GRMHD ? G_FMKS : G_BL; }
+template <int MODE> constexpr bool lds_stokes() { return MODE == M_MF || MODE == M_GRMHD; }
 
 enum Counter : int {
     C_RHS_A = 0,
@@ -86,7 +94,10 @@ struct Params {
     int32_t max_steps, persistent;
     double radius, inclination, x_extent, y_extent, x_offset, y_offset;
     double tetrad[16];
-    BLGcov<double> gcam;
+    double xcam[4];         // camera position in the geodesic coordinates
+    Metric4<double> gcam;   // metric there
+    FmksMap fmks;           // FMKS map of the geodesic coordinates
+    GrmhdDev grmhd;         // GRMHD grid model
     // integrator (PassAParams scalars)
     double mass, spin, inner, outer, step, tol, min_step, max_step;
     double max_rs, max_rd, max_ad, max_fd;
@@ -134,29 +145,27 @@ KP_DEV void camera_ray(const Params& P, int pixel, St<double>& s) {
     double sd[4], a[4], b[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) sd[m] = e1[m] * kt1 + e2[m] * kt2 + e3[m] * kt3;
-    s.v[0] = 0.0;
-    s.v[1] = P.radius;
-    s.v[2] = P.inclination;
-    s.v[3] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) s.v[m] = P.xcam[m];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         s.v[4 + m] = e0[m] + sd[m];
         a[m] = e1[m] - sd[m] * kt1;
         b[m] = e2[m] - sd[m] * kt2;
     }
-    const double na = gdot(P.gcam, a, a);
+    const double na = dot4(P.gcam, a, a);
     if (na > 0.0) {
         const double f = 1.0 / sqrt(na);
 #pragma unroll
         for (int m = 0; m < 4; ++m) a[m] *= f;
     }
-    const double den = gdot(P.gcam, a, a);
+    const double den = dot4(P.gcam, a, a);
     if (absv(den) > 1e-300) {
-        const double f = gdot(P.gcam, b, a) / den;
+        const double f = dot4(P.gcam, b, a) / den;
 #pragma unroll
         for (int m = 0; m < 4; ++m) b[m] = b[m] - a[m] * f;
     }
-    const double nb = gdot(P.gcam, b, b);
+    const double nb = dot4(P.gcam, b, b);
     if (nb > 0.0) {
         const double f = 1.0 / sqrt(nb);
 #pragma unroll
@@ -168,6 +177,33 @@ KP_DEV void camera_ray(const Params& P, int pixel, St<double>& s) {
         s.v[12 + m] = b[m];
     }
 }
+
+// ---------------------------------------------------------------------------
+// Geometry of the geodesic coordinates: integration context, domain radius
+// (metric_domain_radius, src/geodesic/metric_domain.hpp:159-172) and the
+// metric at a point.
+// ---------------------------------------------------------------------------
+template <int GEOM> struct GeomT;
+
+template <> struct GeomT<G_BL> {
+    using GeoType = Geo<double>;
+    static KP_DEV GeoType geo(const Params& P, uint32_t* cnt) { return GeoType{P.mass, P.spin, cnt}; }
+    static KP_DEV double radius(double x1) { return x1; }
+    static KP_DEV Metric4<double> metric(const Params& P, const double* x) {
+        double sn, cs;
+        ksincos(x[2], &sn, &cs);
+        return metric4_of(bl_gcov(P.mass, P.spin, x[1], sn, cs));
+    }
+};
+
+template <> struct GeomT<G_FMKS> {
+    using GeoType = GeoFmks<double>;
+    static KP_DEV GeoType geo(const Params& P, uint32_t* cnt) { return GeoType{P.mass, P.spin, cnt, P.fmks}; }
+    static KP_DEV double radius(double x1) { return exp(x1); }
+    static KP_DEV Metric4<double> metric(const Params& P, const double* x) {
+        return fmks_gcov(P.mass, P.spin, P.fmks, x[1], x[2]);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Wave-aggregated queue fetch: one atomic per wavefront.
@@ -236,11 +272,13 @@ KP_DEV void lane_a_init(const Params& P, int pixel, LaneA& L) {
 }
 
 // locate_metric_surface_crossing for a radial surface (src/geodesic/pass_a_core.hpp:94-197).
-KP_DEV void locate_crossing(const Geo<double>& G, const St<double>& first, const St<double>& second,
-                            double full_h, double surface, int composed, St<double>& out) {
+template <int GEOM, class GG>
+KP_DEV void locate_crossing(const GG& G, const St<double>& first, const St<double>& second, double full_h,
+                            double surface, int composed, St<double>& out) {
+    using GT = GeomT<GEOM>;
     st_copy(out, first);
     double lo = 0.0, hi = 1.0;
-    double flo = first.v[1] - surface;
+    double flo = GT::radius(first.v[1]) - surface;
     if (flo == 0.0 || !st_finite(second)) return;
     St<double> t;
     auto path = [&](double h, St<double>& o) {
@@ -254,7 +292,7 @@ KP_DEV void locate_crossing(const Geo<double>& G, const St<double>& first, const
     };
     path(full_h, t);
     if (!st_finite(t)) return;
-    double fhi = t.v[1] - surface;
+    double fhi = GT::radius(t.v[1]) - surface;
     if (!(fhi == fhi) || absv(fhi) >= 1e300) return;
     if (fhi == 0.0) {
         st_copy(out, t);
@@ -277,7 +315,7 @@ KP_DEV void locate_crossing(const Geo<double>& G, const St<double>& first, const
             use_secant = 0;
             continue;
         }
-        const double ft = t.v[1] - surface;
+        const double ft = GT::radius(t.v[1]) - surface;
         if (!(ft == ft) || absv(ft) >= 1e300) {
             hi = trial;
             use_secant = 0;
@@ -301,8 +339,8 @@ KP_DEV void locate_crossing(const Geo<double>& G, const St<double>& first, const
 // One integration attempt from the lane state: FP64, or with FP32 derivatives
 // outside the guard radius when MIXED.  guard counts the FP64 attempts a
 // mixed lane takes.
-template <bool MIXED>
-KP_DEV double attempt(const Params& P, const Geo<double>& G, const LaneVec& s, LaneVec& k1, double h,
+template <bool MIXED, class GG>
+KP_DEV double attempt(const Params& P, const GG& G, const LaneVec& s, LaneVec& k1, double h,
                       St<double>& half, St<double>& next, uint32_t& guard) {
     if (MIXED) {
         if (!(s[1] < P.fp64_radius)) return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count);
@@ -311,8 +349,8 @@ KP_DEV double attempt(const Params& P, const Geo<double>& G, const LaneVec& s, L
     return doubling_attempt(G, s, h, k1, half, next);
 }
 
-template <bool MIXED>
-KP_DEV void step_rk4(const Params& P, const Geo<double>& G, const LaneVec& s, double h, St<double>& next,
+template <bool MIXED, class GG>
+KP_DEV void step_rk4(const Params& P, const GG& G, const LaneVec& s, double h, St<double>& next,
                      uint32_t& guard) {
     if (MIXED) {
         if (!(s[1] < P.fp64_radius)) {
@@ -325,15 +363,17 @@ KP_DEV void step_rk4(const Params& P, const Geo<double>& G, const LaneVec& s, do
 }
 
 // One tick of pass A; returns true when the ray has terminated (L.reason set).
-template <bool MIXED>
-KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_t& steps_acc, uint32_t& guard) {
+template <bool MIXED, int GEOM>
+KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G, LaneA& L, uint32_t& steps_acc,
+                        uint32_t& guard) {
+    using GT = GeomT<GEOM>;
     const double eff_min = absv(P.min_step);
     if (!L.in_call) {
         if (L.n >= P.max_steps) {
             L.reason = R_MAX_STEPS;
             return true;
         }
-        const double r = L.s[1];
+        const double r = GT::radius(L.s[1]);
         if (!L.entered) {
             if (r <= P.outer) {
                 L.entered = 1;
@@ -398,8 +438,8 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
         used_h = L.h_current;
         step_rk4<MIXED>(P, G, L.s, used_h, next, guard);
     }
-    const double old_r = L.s[1];
-    const double next_r = next.v[1];
+    const double old_r = GT::radius(L.s[1]);
+    const double next_r = GT::radius(next.v[1]);
     const double old_inner = old_r - P.inner;
     const double next_inner = next_r - P.inner;
     if (!L.entered && next_r <= P.outer) {
@@ -420,8 +460,7 @@ KP_DEV bool pass_a_tick(const Params& P, const Geo<double>& G, LaneA& L, uint32_
     if (crossing != R_NONE) {
         St<double> first, ev;
         st_load(first, L.s);
-        locate_crossing(G, first, next, used_h, crossing == R_INNER ? P.inner : P.outer,
-                        P.adaptive, ev);
+        locate_crossing<GEOM>(G, first, next, used_h, crossing == R_INNER ? P.inner : P.outer, P.adaptive, ev);
         st_store(L.s, ev);
         L.reason = crossing;
         L.steps += 1;
@@ -455,10 +494,10 @@ KP_DEV void lane_lds(LdsVec<double>& s, LdsVec<double>& k1) {
 }
 KP_DEV void lane_lds(St<double>&, St<double>&) {}
 
-template <bool PERSISTENT, bool MIXED>
+template <bool PERSISTENT, bool MIXED, int GEOM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_a_kernel(Params P) {
     uint32_t rhs_count = 0, steps_a = 0, guard = 0;
-    Geo<double> G{P.mass, P.spin, &rhs_count};
+    const auto G = GeomT<GEOM>::geo(P, &rhs_count);
     LaneA L;
     lane_lds(L.s, L.k1);
     bool has = false;
@@ -476,7 +515,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_a_tick<MIXED>(P, G, L, steps_a, guard)) {
+            if (has && pass_a_tick<MIXED, GEOM>(P, G, L, steps_a, guard)) {
                 lane_a_store(P, L);
                 has = false;
             }
@@ -485,7 +524,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
         if (pix < P.npix) {
             lane_a_init(P, pix, L);
-            while (!pass_a_tick<MIXED>(P, G, L, steps_a, guard)) {
+            while (!pass_a_tick<MIXED, GEOM>(P, G, L, steps_a, guard)) {
             }
             lane_a_store(P, L);
         }
@@ -524,19 +563,17 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
     St<double> cam, fin;
     camera_ray(P, L.pixel, cam);
     st_load(fin, L.s);
-    double sn, cs;
-    ksincos(fin.v[2], &sn, &cs);
-    const BLGcov<double> g = bl_gcov(P.mass, P.spin, fin.v[1], sn, cs);
+    const Metric4<double> g = GeomT<geom_of<MODE>()>::metric(P, fin.v);
     double r11 = 1.0, r12 = 0.0, r21 = 0.0, r22 = 1.0;
     Stokes<double> o = {0.0, 0.0, 0.0, 0.0};
     int reason = L.reason;
     if (valid) {
         // screen_overlap (pass_b.hpp:87-98) and transform_to_observer_basis
         // (src/frame/stokes_rotation.hpp:39-57)
-        r11 = gdot(g, cam.v + 8, fin.v + 8);
-        r12 = gdot(g, cam.v + 8, fin.v + 12);
-        r21 = gdot(g, cam.v + 12, fin.v + 8);
-        r22 = gdot(g, cam.v + 12, fin.v + 12);
+        r11 = dot4(g, cam.v + 8, fin.v + 8);
+        r12 = dot4(g, cam.v + 8, fin.v + 12);
+        r21 = dot4(g, cam.v + 12, fin.v + 8);
+        r22 = dot4(g, cam.v + 12, fin.v + 12);
         Stokes<double> in = L.st;
         if (SCALED) in = {in.I * P.inv_nu3, in.Q * P.inv_nu3, in.U * P.inv_nu3, in.V * P.inv_nu3};
         o = observer_basis(in, r11, r12, r21, r22);
@@ -547,7 +584,7 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
     }
     const kp_image_data& O = P.out;
     const int p = L.pixel;
-    if (MODE == M_MF) {
+    if (lds_stokes<MODE>()) {
         for (int f = 0; f < P.nfreq; ++f) {
             Stokes<double> of = {0.0, 0.0, 0.0, 0.0};
             if (valid) {
@@ -593,8 +630,8 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
         for (int m = 1; m < 4; ++m) ck = maxv(ck, absv(fin.v[4 + m] - cam.v[4 + m]));
         O.closure_k[p] = ck;
     }
-    if (O.final_null) O.final_null[p] = gdot(g, fin.v + 4, fin.v + 4);
-    if (O.frame_error) O.frame_error[p] = frame_error(g, fin);
+    if (O.final_null) O.final_null[p] = dot4(g, fin.v + 4, fin.v + 4);
+    if (O.frame_error) O.frame_error[p] = frame_error4(g, fin);
     if (O.det_r) O.det_r[p] = r11 * r22 - r12 * r21;
     if (O.overlap_r11) O.overlap_r11[p] = r11;
     if (O.overlap_r12) O.overlap_r12[p] = r12;
@@ -618,7 +655,7 @@ template <int MODE> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L
     L.pass_a_steps = P.ep_steps[pixel];
     L.reason = P.ep_reason[pixel];
     L.st = {0.0, 0.0, 0.0, 0.0};
-    if (MODE == M_MF)
+    if (lds_stokes<MODE>())
         for (int j = 0; j < 4 * P.nfreq; ++j) L.mf.set(j, 0.0);
     if (MODE == M_TANGENT) L.sd = {D3(0.0), D3(0.0), D3(0.0), D3(0.0)};
     L.h_current = P.step;
@@ -776,6 +813,34 @@ template <> struct Radiation<true, M_MF> {
     }
 };
 
+template <> struct Radiation<false, M_GRMHD> {
+    RiafLocal<double> loc;
+    KP_DEV void zero() { loc.valid = 0; }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        grmhd_local(P.grmhd, sample, loc);
+        return 0;
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        req = 1;
+        pred = 1;
+        for (int f = 0; f < P.nctrl; ++f) {
+            Coeffs<double> c;
+            grmhd_coeffs_at(P.grmhd, loc, P.ctrl[f], c);
+            req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+            pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+        }
+        return loc.valid ? (uint32_t)P.nctrl : 0u;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        for (int f = 0; f < P.nfreq; ++f) {
+            Coeffs<double> c;
+            grmhd_coeffs_at(P.grmhd, loc, P.freqs[f], c);
+            mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
+        }
+        return loc.valid ? (uint32_t)P.nfreq : 0u;
+    }
+};
+
 // Tangent mode: the plasma scalars carry their derivatives with respect to
 // the log-scale parameters of PlasmaPerturbation (plasma_perturbation.hpp:12-33):
 // q_density scales n_e and B^2 together (d ln n_e = dq, d ln B = dq / 2),
@@ -841,13 +906,14 @@ template <> struct Radiation<false, M_TANGENT> {
 
 // One tick of pass B; returns true when the ray has terminated.
 template <bool MIXED, int MODE>
-KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_t& coeff_acc,
-                        uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
+KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::GeoType& G, LaneB& L,
+                        uint32_t& coeff_acc, uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
+    using GT = GeomT<geom_of<MODE>()>;
     const double eff_min = absv(P.min_step);
     if (L.phase == PH_IDLE) {
         if (L.reached || L.steps >= P.max_steps) return true;
         double h = L.h_current;
-        if (L.s[1] <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
+        if (GT::radius(L.s[1]) <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
         L.h = h;
         if (P.adaptive) start_call(L, h, eff_min, P.max_step);
         L.phase = PH_MAIN;
@@ -887,8 +953,8 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
     if (L.phase == PH_MAIN) {
         // camera_surface_value/crossed/crossing_fraction for the pinhole sphere
         // (pass_b.hpp:54-85, 605-637)
-        const double s0 = L.s[1] - P.radius;
-        const double s1 = next.v[1] - P.radius;
+        const double s0 = GT::radius(L.s[1]) - P.radius;
+        const double s1 = GT::radius(next.v[1]) - P.radius;
         const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
         L.crosses = crosses;
         if (crosses) {
@@ -914,7 +980,7 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
     }
     L.phase = PH_IDLE;
     const double h = L.h;
-    const double old_r = L.s[1], sample_r = sample.v[1], next_r = next.v[1];
+    const double old_r = GT::radius(L.s[1]), sample_r = GT::radius(sample.v[1]), next_r = GT::radius(next.v[1]);
     Radiation<MIXED, MODE> rad;
     rad.zero();
     if (sample_r - P.inner > 0.0 && sample_r < P.outer) coeff_acc += rad.eval(P, sample);
@@ -945,10 +1011,10 @@ KP_DEV bool pass_b_tick(const Params& P, const Geo<double>& G, LaneB& L, uint32_
 template <bool PERSISTENT, bool MIXED, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
     uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
-    Geo<double> G{P.mass, P.spin, &rhs_count};
+    const auto G = GeomT<geom_of<MODE>()>::geo(P, &rhs_count);
     LaneB L;
     lane_lds(L.s, L.k1);
-    if (MODE == M_MF) {
+    if (lds_stokes<MODE>()) {
         extern __shared__ double kp_lds[];
         L.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
     }
@@ -1001,19 +1067,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         const bool has = fetch_pixel(&P.queue[0], true, (uint32_t)P.npix, pix);
         if (!__ballot(has)) break;
         if (has) {
+            constexpr int GEOM = geom_of<MODE>();
             {
-                Geo<double> G{P.mass, P.spin, &rhs_a};
+                const auto G = GeomT<GEOM>::geo(P, &rhs_a);
                 LaneA A;
                 lane_lds(A.s, A.k1);
                 lane_a_init(P, pix, A);
-                while (!pass_a_tick<MIXED>(P, G, A, steps_a, guard)) {
+                while (!pass_a_tick<MIXED, GEOM>(P, G, A, steps_a, guard)) {
                 }
                 lane_a_store(P, A);
             }
-            Geo<double> G{P.mass, P.spin, &rhs_b};
+            const auto G = GeomT<GEOM>::geo(P, &rhs_b);
             LaneB B;
             lane_lds(B.s, B.k1);
-            if (MODE == M_MF) {
+            if (lds_stokes<MODE>()) {
                 extern __shared__ double kp_lds[];
                 B.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
             }
@@ -1081,7 +1148,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         if (!__ballot(has)) break;
         if (has) {
             const Geo<double> G{P->mass, P->spin, &rhs_count};
-            if (pass_a_tick<MIXED>(*P, G, L, steps_a, guard)) {
+            if (pass_a_tick<MIXED, G_BL>(*P, G, L, steps_a, guard)) {
                 lane_a_store(*P, L);
                 has = false;
             }
@@ -1120,6 +1187,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         }
     }
     flush_counters(B.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
+}
+
+// Derived scalars of every GRMHD cell from its primitives, one cell per
+// thread: reads 32 and writes 40 bytes per cell (HBM-bound).
+__global__ __launch_bounds__(256) void derive_scalars_kernel(GrmhdDev G, kp_grmhd_cell* cells, int64_t ncell) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int64_t per_i = (int64_t)G.n2 * G.n3;
+    const int i = (int)(c / per_i);
+    const int j = (int)((c / G.n3) % G.n2);
+    grmhd_derive_cell(G, i, j, cells[c]);
 }
 
 __global__ void zero_kernel(uint32_t* q, unsigned long long* c) {
@@ -1173,11 +1251,13 @@ struct Occupancy {
     int blocks_a = 0, blocks_b = 0, blocks_u = 0;
 };
 
-template <bool PERSISTENT> const void* pass_a_fn(bool mixed) {
-    return mixed ? (const void*)pass_a_kernel<PERSISTENT, true> : (const void*)pass_a_kernel<PERSISTENT, false>;
+template <bool PERSISTENT> const void* pass_a_fn(bool mixed, int geom = G_BL) {
+    if (geom == G_FMKS) return (const void*)pass_a_kernel<PERSISTENT, false, G_FMKS>;
+    return mixed ? (const void*)pass_a_kernel<PERSISTENT, true, G_BL> : (const void*)pass_a_kernel<PERSISTENT, false, G_BL>;
 }
 template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int mode) {
     if (mode == M_TANGENT) return (const void*)pass_b_kernel<PERSISTENT, false, M_TANGENT>;
+    if (mode == M_GRMHD) return (const void*)pass_b_kernel<PERSISTENT, false, M_GRMHD>;
     if (mode == M_MF)
         return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, M_MF>
                      : (const void*)pass_b_kernel<PERSISTENT, false, M_MF>;
@@ -1187,10 +1267,13 @@ template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int mode) {
 
 const void* unsplit_fn(bool mixed, int mode) {
     if (mode == M_TANGENT) return (const void*)unsplit_kernel<false, M_TANGENT>;
+    if (mode == M_GRMHD) return (const void*)unsplit_kernel<false, M_GRMHD>;
     if (mode == M_MF)
         return mixed ? (const void*)unsplit_kernel<true, M_MF> : (const void*)unsplit_kernel<false, M_MF>;
     return mixed ? (const void*)unsplit_kernel<true, M_SINGLE> : (const void*)unsplit_kernel<false, M_SINGLE>;
 }
+
+int geom_of_mode(int mode) { return mode == M_GRMHD ? G_FMKS : G_BL; }
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
 // plus 4 doubles per lane and output frequency for multi-frequency Stokes.
@@ -1215,7 +1298,8 @@ kp_status occupancy_for(int device, int block, bool mixed, int mode, int nfreq_m
     hipDeviceProp_t prop;
     KP_HIP(hipGetDeviceProperties(&prop, device));
     o.cus = prop.multiProcessorCount;
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed), block, lds_bytes(block, 0)));
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed, geom_of_mode(mode)), block,
+                                                        lds_bytes(block, 0)));
     KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed, mode), block,
                                                         lds_bytes(block, nfreq_mf)));
     KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_u, unsplit_fn(mixed, mode), block,
@@ -1233,19 +1317,15 @@ kp_status occupancy_for(int device, int block, bool mixed, int mode, int nfreq_m
 // ---------------------------------------------------------------------------
 // Parameter preparation (host, double): tetrad, extents, model constants.
 // ---------------------------------------------------------------------------
-static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Params& P) {
-    if (!p || !m) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
-    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST)
-        return kp_internal::fail(KP_UNSUPPORTED, "the RIAF image is built only in Boyer-Lindquist coordinates");
+// Camera extents and integrator scalars shared by every model.
+static kp_status prepare_common(const kp_pass_a_params* p, Params& P) {
     if (p->camera.model != KP_CAMERA_PINHOLE)
-        return kp_internal::fail(KP_UNSUPPORTED, "only the pinhole camera is defined in Boyer-Lindquist coordinates");
+        return kp_internal::fail(KP_UNSUPPORTED, "only the pinhole camera is built");
     if (p->direct_only)
         return kp_internal::fail(KP_UNSUPPORTED, "direct_only vertical-turn truncation is not implemented");
     const double hr = p->emission_selection.equatorial_h_over_r;
     if (hr > 0.0 && hr < 1.0)
         return kp_internal::fail(KP_UNSUPPORTED, "equatorial emission wedges are not implemented");
-    if (m->emission_type != KP_EMISSION_THERMAL_PANDYA && m->emission_type != KP_EMISSION_THERMAL_DEXTER)
-        return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
     if (p->camera.nx <= 0 || p->camera.ny <= 0)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "camera nx and ny must be positive");
     const long long npix = (long long)p->camera.nx * (long long)p->camera.ny;
@@ -1265,13 +1345,6 @@ static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Para
                : (c.xspan > 0.0 ? 2.0 * c.xspan * double(c.ny) / double(c.nx) : (c.fov_y > 0.0 ? c.fov_y : c.fov));
     P.x_offset = c.x_offset;
     P.y_offset = c.y_offset;
-    double g5[5];
-    kp_internal::bl_camera_tetrad(p->mass, p->spin, c.radius, c.inclination, P.tetrad, g5);
-    P.gcam.g00 = g5[0];
-    P.gcam.g03 = g5[1];
-    P.gcam.g11 = g5[2];
-    P.gcam.g22 = g5[3];
-    P.gcam.g33 = g5[4];
     P.mass = p->mass;
     P.spin = p->spin;
     P.inner = p->inner_radius;
@@ -1284,7 +1357,91 @@ static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Para
     P.max_rd = p->max_radiation_depth;
     P.max_ad = p->max_absorption_depth;
     P.max_fd = p->max_faraday_depth;
+    return KP_OK;
+}
+
+static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Params& P) {
+    if (!p || !m) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
+    if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST)
+        return kp_internal::fail(KP_UNSUPPORTED, "the RIAF image is built only in Boyer-Lindquist coordinates");
+    if (p->camera.model != KP_CAMERA_PINHOLE)
+        return kp_internal::fail(KP_UNSUPPORTED, "only the pinhole camera is defined in Boyer-Lindquist coordinates");
+    if (m->emission_type != KP_EMISSION_THERMAL_PANDYA && m->emission_type != KP_EMISSION_THERMAL_DEXTER)
+        return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
+    kp_status st = prepare_common(p, P);
+    if (st != KP_OK) return st;
+    double g5[5];
+    kp_internal::bl_camera_tetrad(p->mass, p->spin, p->camera.radius, p->camera.inclination, P.tetrad, g5);
+    P.xcam[0] = 0.0;
+    P.xcam[1] = p->camera.radius;
+    P.xcam[2] = p->camera.inclination;
+    P.xcam[3] = 0.0;
+    BLGcov<double> b;
+    b.g00 = g5[0];
+    b.g03 = g5[1];
+    b.g11 = g5[2];
+    b.g22 = g5[3];
+    b.g33 = g5[4];
+    P.gcam = metric4_of(b);
     kp_internal::riaf_constants(*p, *m, 0, 0.0, P.riaf);
+    return KP_OK;
+}
+
+// GRMHD grid image on FMKS geodesics: camera in FMKS, model constants.
+static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* g, Params& P) {
+    if (!p || !g) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
+    if (p->coordinate_system != KP_COORD_FMKS)
+        return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built in FMKS coordinates");
+    if (g->emission_type != KP_EMISSION_THERMAL_PANDYA && g->emission_type != KP_EMISSION_THERMAL_DEXTER)
+        return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
+    if (g->n1 < 1 || g->n2 < 1 || g->n3 < 2 || !g->cells)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid dimensions and cells are required");
+    if (!(g->dx1 > 0.0) || !(g->dx2 > 0.0) || !(g->dx3 > 0.0))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid spacings must be positive");
+    kp_status st = prepare_common(p, P);
+    if (st != KP_OK) return st;
+    P.fmks = FmksMap{p->fmks_startx1, p->fmks_hslope, p->fmks_mks_smooth, p->fmks_poly_alpha, p->fmks_poly_xt,
+                     p->fmks_poly_norm};
+    double fm[6] = {p->fmks_startx1, p->fmks_hslope, p->fmks_mks_smooth, p->fmks_poly_alpha, p->fmks_poly_xt,
+                    p->fmks_poly_norm};
+    double g16[16];
+    kp_internal::fmks_camera_tetrad(p->mass, p->spin, fm, p->camera.radius, p->camera.inclination, P.xcam, P.tetrad,
+                                    g16);
+    P.gcam = Metric4<double>{g16[0], g16[1], g16[2], g16[3], g16[5], g16[6], g16[7], g16[10], g16[11], g16[15]};
+    GrmhdDev& G = P.grmhd;
+    G.n1 = g->n1;
+    G.n2 = g->n2;
+    G.n3 = g->n3;
+    G.fit = g->emission_type;
+    G.startx1 = g->startx1;
+    G.startx2 = g->startx2;
+    G.startx3 = g->startx3;
+    G.dx1 = g->dx1;
+    G.dx2 = g->dx2;
+    G.dx3 = g->dx3;
+    G.r_in = g->r_in;
+    G.r_out = g->r_out;
+    G.spin = g->spin;
+    G.gam = g->gam;
+    G.map = FmksMap{g->startx1, g->hslope, g->mks_smooth, g->poly_alpha, g->poly_xt, g->poly_norm};
+    G.sigma_cut = g->sigma_cut;
+    G.sigma_cut_high = g->sigma_cut_high;
+    G.emission_scale = g->emission_scale;
+    G.absorption_scale = g->absorption_scale;
+    G.faraday_scale = g->faraday_scale;
+    G.max_pol_frac = g->max_pol_frac;
+    G.freq_cgs = g->freq_cgs;
+    // length_unit_cgs, rho_unit_cgs, b_unit_cgs (grmhd_grid.hpp:140-156)
+    const double cl = KP_CL;
+    G.length_unit = 6.67430e-8 * g->mbh_solar * 1.98847e33 / (cl * cl);
+    G.rho_unit = g->M_unit / std::max(G.length_unit * G.length_unit * G.length_unit, 1e-300);
+    G.b_unit = cl * std::sqrt(4.0 * PI_D * G.rho_unit);
+    G.ne_factor = g->ne_factor;
+    G.trat_small = g->trat_small;
+    G.trat_large = g->trat_large;
+    G.beta_crit = g->beta_crit;
+    G.faraday_rotation = p->emission_selection.faraday_rotation ? 1 : 0;
+    G.cells = static_cast<const kp_grmhd_cell*>(g->cells);
     return KP_OK;
 }
 
@@ -1322,7 +1479,8 @@ struct FreqLists {
     const double* ctrl = nullptr;
 };
 
-static kp_status prepare_frequencies(const kp_riaf_model& m, const FreqLists& fl, Params& P) {
+// mbh_solar: the model's black-hole mass, which sets its dl/dlambda.
+static kp_status prepare_frequencies(double mbh_solar, const FreqLists& fl, Params& P) {
     if (fl.nfreq < 1 || fl.nfreq > KP_MAX_FREQUENCIES || !fl.freqs)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "frequencies: 1 to KP_MAX_FREQUENCIES values required");
     if (fl.nctrl < 1 || fl.nctrl > KP_MAX_FREQUENCIES || !fl.ctrl)
@@ -1331,7 +1489,7 @@ static kp_status prepare_frequencies(const kp_riaf_model& m, const FreqLists& fl
     P.nfreq = fl.nfreq;
     P.nctrl = fl.nctrl;
     const double cl = KP_CL;
-    const double l_unit = 6.67430e-8 * m.mbh_solar * 1.98847e33 / (cl * cl);
+    const double l_unit = 6.67430e-8 * mbh_solar * 1.98847e33 / (cl * cl);
     auto fill = [&](const double* src, int n, double* f, double* dls, float* bconst, double* inv3) {
         for (int i = 0; i < n; ++i) {
             if (!(src[i] > 0.0) || !std::isfinite(src[i])) return false;
@@ -1356,9 +1514,11 @@ struct Tangent {
     double* grad = nullptr;
 };
 
-static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl,
+// grmhd != NULL selects the GRMHD grid model on FMKS geodesics (model unused).
+static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl_in,
                            const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
-                           int perturbation, double log_scale, const Tangent* tangent = nullptr) {
+                           int perturbation, double log_scale, const Tangent* tangent = nullptr,
+                           const kp_grmhd_model* grmhd = nullptr) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -1368,9 +1528,22 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         return kp_internal::fail(KP_INVALID_ARGUMENT, "precision must be KP_PRECISION_FP64 or KP_PRECISION_MIXED");
     const bool mixed = opt.precision == KP_PRECISION_MIXED;
     Params P;
-    kp_status st = prepare(params, model, P);
-    if (st != KP_OK) return st;
-    if (perturbation) kp_internal::riaf_constants(*params, *model, perturbation, log_scale, P.riaf);
+    FreqLists fl = fl_in;
+    kp_status st;
+    if (grmhd) {
+        if (mixed || tangent || perturbation)
+            return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built for FP64 without parameter gradients");
+        st = prepare_grmhd(params, grmhd, P);
+        if (st != KP_OK) return st;
+        if (fl.nfreq == 0) {
+            fl.nfreq = fl.nctrl = 1;
+            fl.freqs = fl.ctrl = &grmhd->freq_cgs;
+        }
+    } else {
+        st = prepare(params, model, P);
+        if (st != KP_OK) return st;
+        if (perturbation) kp_internal::riaf_constants(*params, *model, perturbation, log_scale, P.riaf);
+    }
     if (mixed) {
         if (!(opt.fp64_radius >= 0.0))
             return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
@@ -1379,10 +1552,10 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     }
     const bool mf = fl.nfreq > 0;
     if (mf) {
-        st = prepare_frequencies(*model, fl, P);
+        st = prepare_frequencies(grmhd ? grmhd->mbh_solar : model->mbh_solar, fl, P);
         if (st != KP_OK) return st;
     }
-    int mode = mf ? M_MF : M_SINGLE;
+    int mode = grmhd ? M_GRMHD : (mf ? M_MF : M_SINGLE);
     if (tangent) {
         if (mixed || mf)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "parameter gradients are built for single-frequency FP64");
@@ -1453,7 +1626,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (P.npix > 0 && unsplit)
         KP_HIP(hipLaunchKernel(unsplit_fn(mixed, mode), dim3(grid_u), dim3(block), args, smem_b, stream));
     if (P.npix > 0 && !unsplit) {
-        const void* fn = P.persistent ? pass_a_fn<true>(mixed) : pass_a_fn<false>(mixed);
+        const void* fn = P.persistent ? pass_a_fn<true>(mixed, geom_of_mode(mode)) : pass_a_fn<false>(mixed, geom_of_mode(mode));
         KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem_a, stream));
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
@@ -1749,6 +1922,53 @@ extern "C" kp_status kp_run_riaf_image_gradient(const kp_pass_a_params* params, 
     t.log_scales = log_scales;
     t.grad = grad;
     return run_image(params, model, FreqLists{}, options, out, stats, 0, 0.0, &t);
+}
+
+extern "C" kp_status kp_run_grmhd_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                        int32_t radiation_substeps, const kp_run_options* options,
+                                        kp_image_data* out, kp_image_stats* stats) {
+    (void)radiation_substeps;
+    if (!model) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model");
+    return run_image(params, nullptr, FreqLists{}, options, out, stats, 0, 0.0, nullptr, model);
+}
+
+extern "C" kp_status kp_run_grmhd_multifrequency_image(const kp_pass_a_params* params,
+                                                       const kp_grmhd_model* model, int32_t radiation_substeps,
+                                                       const double* frequencies, int32_t nfreq,
+                                                       const double* step_control_frequencies,
+                                                       int32_t step_control_nfreq, const kp_run_options* options,
+                                                       kp_image_data* out, kp_image_stats* stats) {
+    (void)radiation_substeps;
+    if (!model) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model");
+    if (nfreq < 1) return kp_internal::fail(KP_INVALID_ARGUMENT, "nfreq must be at least 1");
+    FreqLists fl;
+    fl.nfreq = nfreq;
+    fl.freqs = frequencies;
+    fl.nctrl = step_control_nfreq;
+    fl.ctrl = step_control_frequencies;
+    return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, model);
+}
+
+extern "C" kp_status kp_grmhd_derive_scalars(const kp_grmhd_model* model, void* stream) {
+    kp_internal::clear_error();
+    if (!model || !model->cells || model->n1 < 1 || model->n2 < 1 || model->n3 < 1)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD model with cells and positive dimensions required");
+    kp_pass_a_params p;
+    kp_pass_a_params_default(&p);
+    p.coordinate_system = KP_COORD_FMKS;
+    p.camera.nx = p.camera.ny = 1;
+    p.camera.radius = 100.0;
+    Params P;
+    kp_status st = prepare_grmhd(&p, model, P);
+    if (st != KP_OK) return st;
+    const long long ncell = (long long)model->n1 * model->n2 * model->n3;
+    if (ncell > (1ll << 31)) return kp_internal::fail(KP_INVALID_ARGUMENT, "grid too large");
+    const int block = 256;
+    const int grid = (int)((ncell + block - 1) / block);
+    hipLaunchKernelGGL(derive_scalars_kernel, dim3(grid), dim3(block), 0, static_cast<hipStream_t>(stream), P.grmhd,
+                       const_cast<kp_grmhd_cell*>(P.grmhd.cells), (int64_t)ncell);
+    KP_HIP(hipGetLastError());
+    return KP_OK;
 }
 
 // Host-output wrapper shared by the single- and multi-frequency entries:

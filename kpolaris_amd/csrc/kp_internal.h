@@ -24,4 +24,9 @@ void riaf_constants(const kp_pass_a_params& p, const kp_riaf_model& m, int pertu
 
 double isco_radius(double spin);
 
+// ipole camera position, tetrad and metric there for the FMKS pinhole camera;
+// fm = (startx1, hslope, mks_smooth, poly_alpha, poly_xt, poly_norm).
+void fmks_camera_tetrad(double mass, double spin, const double fm[6], double radius, double inclination,
+                        double xcam[4], double tetrad[16], double gcov16[16]);
+
 }  // namespace kp_internal

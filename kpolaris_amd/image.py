@@ -74,6 +74,16 @@ def library():
     L.kp_run_riaf_image_gradient.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(C.c_double),
                                              P(abi.RunOptions), P(abi.ImageData), P(C.c_double), P(abi.ImageStats)]
     L.kp_run_riaf_image_gradient.restype = C.c_int32
+    L.kp_run_grmhd_image.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), C.c_int32, P(abi.RunOptions),
+                                     P(abi.ImageData), P(abi.ImageStats)]
+    L.kp_run_grmhd_image.restype = C.c_int32
+    L.kp_run_grmhd_multifrequency_image.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), C.c_int32,
+                                                    P(C.c_double), C.c_int32, P(C.c_double), C.c_int32,
+                                                    P(abi.RunOptions), P(abi.ImageData), P(abi.ImageStats)]
+    L.kp_run_grmhd_multifrequency_image.restype = C.c_int32
+    L.kp_grmhd_derive_scalars.argtypes = [P(abi.GRMHDModel), C.c_void_p]
+    L.kp_grmhd_derive_scalars.restype = C.c_int32
+    L.kp_grmhd_model_default.argtypes = [P(abi.GRMHDModel)]
     L.kp_batch_create.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.RunOptions), P(C.c_void_p)]
     L.kp_batch_create.restype = C.c_int32
     L.kp_batch_launch.argtypes = [C.c_void_p, C.c_void_p]
@@ -324,3 +334,54 @@ def run_riaf_image_gradient(params: abi.PassAParams, model: abi.RIAFModel, log_s
         C.byref(params), C.byref(model), 1, q.ctypes.data_as(dp), C.byref(opts), C.byref(img),
         C.cast(C.c_void_p(grad.data_ptr()), dp), C.byref(st) if with_stats else None))
     return out, grad, (st.as_dict() if with_stats else None)
+
+
+class GRMHDGrid:
+    """A GRMHD snapshot resident on the device: kp_grmhd_cell records
+    (72 bytes each) in one uint8 tensor, with the model parameters.
+
+    cells: numpy structured array of kp_grmhd_cell records (primitives set);
+    the derived scalars are computed on the device (kp_grmhd_derive_scalars)."""
+
+    def __init__(self, model: abi.GRMHDModel, cells: np.ndarray, derive: bool = True):
+        import torch
+        assert cells.dtype.itemsize == 72 and cells.size == model.n1 * model.n2 * model.n3
+        raw = np.ascontiguousarray(cells).view(np.uint8).reshape(-1)
+        self.buffer = torch.from_numpy(raw.copy()).to("cuda")
+        self.model = abi.GRMHDModel.from_buffer_copy(model)
+        self.model.cells = C.c_void_p(self.buffer.data_ptr())
+        if derive:
+            self.derive()
+
+    def derive(self, stream=None):
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        _check(library().kp_grmhd_derive_scalars(C.byref(self.model), C.c_void_p(s)))
+
+    def host_cells(self) -> np.ndarray:
+        from kpolaris_amd.grmhd_synth import CELL_DTYPE
+        return self.buffer.cpu().numpy().view(CELL_DTYPE)
+
+
+def run_grmhd_image(params: abi.PassAParams, grid: "GRMHDGrid", frequencies=None, step_control_frequencies=None,
+                    *, persistent: bool = True, block_size: int = 0, waves_per_cu: int = 0,
+                    split_transport: int = 1, out: dict | None = None, fields=abi.IMAGE_FIELDS,
+                    with_stats: bool = True):
+    """GRMHD image on FMKS geodesics; frequencies None = the model's freq_cgs.
+    Stokes tensors are [nfreq * npix]."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    f = np.ascontiguousarray([grid.model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    c = np.ascontiguousarray(f if step_control_frequencies is None else step_control_frequencies, dtype=np.float64)
+    if out is None:
+        out = alloc_device_image(npix, fields=fields, nfreq=len(f))
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, persistent, block_size, waves_per_cu, C.c_void_p(stream), None,
+                       split_transport)
+    st = abi.ImageStats()
+    dp = C.POINTER(C.c_double)
+    _check(library().kp_run_grmhd_multifrequency_image(
+        C.byref(params), C.byref(grid.model), 1, f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp), len(c),
+        C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
+    return out, (st.as_dict() if with_stats else None)

@@ -107,6 +107,14 @@ def lib():
         C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.POINTER(Perturbation), dp, i32, dp, i32,
         i32, i32, i32, C.POINTER(abi.ImageData), C.POINTER(Stats)]
     L.kpo_run_riaf_multifrequency_image.restype = i32
+    L.kpo_grmhd_derive_scalars.argtypes = [C.POINTER(abi.GRMHDModel), C.c_void_p]
+    L.kpo_grmhd_derive_scalars.restype = None
+    L.kpo_run_grmhd_image.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.GRMHDModel), C.POINTER(Perturbation),
+                                      dp, i32, dp, i32, i32, i32, i32, C.POINTER(abi.ImageData), C.POINTER(Stats)]
+    L.kpo_run_grmhd_image.restype = i32
+    L.kpo_grmhd_coefficients.argtypes = [C.POINTER(abi.GRMHDModel), C.POINTER(State), C.POINTER(Perturbation),
+                                         i32, C.POINTER(Coeffs)]
+    L.kpo_grmhd_coefficients.restype = None
     _lib = L
     return L
 
@@ -205,6 +213,37 @@ def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel,
     rc = L.kpo_run_riaf_multifrequency_image(C.byref(params), C.byref(model), C.byref(pert),
                                              f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp), len(c),
                                              pixel_begin, pixel_end, nthreads, C.byref(img), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
+    return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}
+
+
+def grmhd_derive_scalars(model: abi.GRMHDModel, cells: np.ndarray) -> None:
+    """Fill cells["derived"] in place (host cells, kp_grmhd_cell records)."""
+    assert cells.dtype.itemsize == 72 and cells.flags["C_CONTIGUOUS"]
+    lib().kpo_grmhd_derive_scalars(C.byref(model), C.c_void_p(cells.ctypes.data))
+
+
+def run_grmhd_image(params: abi.PassAParams, model: abi.GRMHDModel, cells: np.ndarray, frequencies=None,
+                    step_control_frequencies=None, pixel_begin: int = 0, pixel_end: int = -1, nthreads: int = 0):
+    """Oracle GRMHD image on FMKS geodesics; Stokes arrays [nfreq * npix]."""
+    L = lib()
+    m = abi.GRMHDModel.from_buffer_copy(model)
+    m.cells = C.c_void_p(cells.ctypes.data)
+    npix = params.camera.nx * params.camera.ny
+    f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    c = np.ascontiguousarray(f if step_control_frequencies is None else step_control_frequencies, dtype=np.float64)
+    arrays = alloc_image(npix)
+    for n in ("image_i", "image_q", "image_u", "image_v"):
+        arrays[n] = np.zeros(len(f) * npix, dtype=np.float64)
+    img = image_struct(arrays)
+    st = Stats()
+    if nthreads <= 0:
+        nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
+    dp = C.POINTER(C.c_double)
+    rc = L.kpo_run_grmhd_image(C.byref(params), C.byref(m), None, f.ctypes.data_as(dp), len(f),
+                               c.ctypes.data_as(dp), len(c), pixel_begin, pixel_end, nthreads, C.byref(img),
+                               C.byref(st))
     if rc != 0:
         raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
     return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}

@@ -261,16 +261,369 @@ void kpo_bl_transport_rhs(double mass, double spin, const kpo_state* s, kpo_stat
 }
 
 /* ======================================================================
+ * FMKS geometry (src/geometry/kerr_fmks.hpp, src/geometry/kerr_schild_spherical.hpp,
+ * src/grmhd/grid_mapping.hpp): x = (t, x1 = ln r, x2, phi) with
+ * theta(x1, x2) blending the MKS and the polynomial jet maps.
+ * ==================================================================== */
+/* fmks_power / FMKSGridMapping::power (kerr_fmks.hpp:37-47, grid_mapping.hpp:45-55). */
+static double fmks_power(double x, double p) {
+    const int n = (int)(p + 0.5);
+    if (absv(p - (double)n) < 1e-10 && n >= 0) {
+        double y = 1.0;
+        const double ax = absv(x);
+        for (int i = 0; i < n; ++i) y *= ax;
+        if (x < 0.0 && (n % 2) == 1) y = -y;
+        return y;
+    }
+    return pow(absv(x), p);
+}
+
+/* theta_from_x2 (kerr_fmks.hpp:49-56). */
+double kpo_fmks_theta(const kpo_fmks* f, double x1, double x2) {
+    const double thG = PI_D * x2 + ((1.0 - f->hslope) / 2.0) * sin(2.0 * PI_D * x2);
+    const double y = 2.0 * x2 - 1.0;
+    const double thJ = f->poly_norm * y * (1.0 + fmks_power(y / f->poly_xt, f->poly_alpha) / (f->poly_alpha + 1.0)) +
+                       0.5 * PI_D;
+    return thG + exp(f->mks_smooth * (f->startx1 - x1)) * (thJ - thG);
+}
+
+/* dtheta_dx1, dtheta_dx2 (kerr_fmks.hpp:58-80). */
+static double fmks_dtheta_dx1(const kpo_fmks* f, double x1, double x2) {
+    const double thG = PI_D * x2 + ((1.0 - f->hslope) / 2.0) * sin(2.0 * PI_D * x2);
+    const double y = 2.0 * x2 - 1.0;
+    const double thJ = f->poly_norm * y * (1.0 + fmks_power(y / f->poly_xt, f->poly_alpha) / (f->poly_alpha + 1.0)) +
+                       0.5 * PI_D;
+    return -f->mks_smooth * exp(f->mks_smooth * (f->startx1 - x1)) * (thJ - thG);
+}
+
+static double fmks_dtheta_dx2(const kpo_fmks* f, double x1, double x2) {
+    const double y = 2.0 * x2 - 1.0;
+    const double expfac = exp(f->mks_smooth * (f->startx1 - x1));
+    const double dthG = PI_D + (1.0 - f->hslope) * PI_D * cos(2.0 * PI_D * x2);
+    const double dthJ = 2.0 * f->poly_norm * (1.0 + fmks_power(y / f->poly_xt, f->poly_alpha) / (f->poly_alpha + 1.0)) +
+                        (2.0 * f->poly_alpha * f->poly_norm * y * fmks_power(y / f->poly_xt, f->poly_alpha - 1.0)) /
+                            ((f->poly_alpha + 1.0) * f->poly_xt);
+    return dthG + expfac * (dthJ - dthG);
+}
+
+/* native_x2_from_theta: 64 bisections (kerr_fmks.hpp:107-117). */
+double kpo_fmks_x2_from_theta(const kpo_fmks* f, double x1, double theta) {
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (kpo_fmks_theta(f, x1, mid) < theta)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+/* Kerr-Schild spherical gcov at (r, theta) (kerr_schild_spherical.hpp:48-66). */
+static void ks_gcov_rth(double mass, double spin, double r, double sth, double cth, double g[16]) {
+    const double s2 = sth * sth;
+    const double a2 = spin * spin;
+    const double rho2 = r * r + a2 * cth * cth;
+    for (int i = 0; i < 16; ++i) g[i] = 0.0;
+    const double f = 2.0 * mass * r / rho2;
+    g[0] = -1.0 + f;
+    g[1] = f;
+    g[3] = -spin * f * s2;
+    g[4] = g[1];
+    g[5] = 1.0 + f;
+    g[7] = -spin * s2 * (1.0 + f);
+    g[10] = rho2;
+    g[12] = g[3];
+    g[13] = g[7];
+    g[15] = s2 * (rho2 + a2 * s2 * (1.0 + f));
+}
+
+/* KerrFMKSMetric::gcov_matrix: J^T g_KS J with J = dx_KS/dX (kerr_fmks.hpp:125-157). */
+void kpo_fmks_gcov(double mass, double spin, const kpo_fmks* f, const double x[4], double g[16]) {
+    const double r = exp(x[1]);
+    const double th = kpo_fmks_theta(f, x[1], x[2]);
+    double gks[16], jac[16];
+    ks_gcov_rth(mass, spin, r, sin(th), cos(th), gks);
+    for (int i = 0; i < 16; ++i) jac[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    jac[1 * 4 + 1] = r;
+    jac[2 * 4 + 1] = fmks_dtheta_dx1(f, x[1], x[2]);
+    jac[2 * 4 + 2] = fmks_dtheta_dx2(f, x[1], x[2]);
+    for (int mu = 0; mu < 4; ++mu)
+        for (int nu = 0; nu < 4; ++nu) {
+            double sum = 0.0;
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) sum += gks[a * 4 + b] * jac[a * 4 + mu] * jac[b * 4 + nu];
+            g[mu * 4 + nu] = sum;
+        }
+}
+
+/* invert_4x4: Gauss-Jordan with a tiny-pivot floor (kerr_fmks.hpp:159-182). */
+static void invert_4x4(const double* a, double* inv) {
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            m[i][j] = a[i * 4 + j];
+            m[i][j + 4] = i == j ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        double pivot = m[col][col];
+        if (absv(pivot) < TINY) pivot = pivot < 0.0 ? -TINY : TINY;
+        const double ip = 1.0 / pivot;
+        for (int j = 0; j < 8; ++j) m[col][j] *= ip;
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            const double factor = m[row][col];
+            for (int j = 0; j < 8; ++j) m[row][j] -= factor * m[col][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) inv[i * 4 + j] = m[i][j + 4];
+}
+
+void kpo_fmks_gcon(double mass, double spin, const kpo_fmks* f, const double x[4], double g[16]) {
+    double cov[16];
+    kpo_fmks_gcov(mass, spin, f, x, cov);
+    invert_4x4(cov, g);
+}
+
+/* Spherical-KS connection work (kerr_schild_spherical.hpp:208-268):
+ * gcon and the r- and theta-derivatives of gcov. */
+typedef struct sks_work {
+    double gcon[16], dgr[16], dgth[16];
+} sks_work;
+
+static void sks_build(double mass, double spin, double r_in, double th, sks_work* c) {
+    const double r = maxv(r_in, TINY);
+    const double s = sin(th), cc = cos(th);
+    const double a = spin, a2 = a * a;
+    const double s2 = s * s;
+    const double safe_s2 = maxv(s2, TINY);
+    const double rho2 = r * r + a2 * cc * cc;
+    const double rho4 = rho2 * rho2;
+    const double f = 2.0 * mass * r / rho2;
+    const double delta = r * r - 2.0 * mass * r + a2;
+    for (int i = 0; i < 16; ++i) c->gcon[i] = c->dgr[i] = c->dgth[i] = 0.0;
+    c->gcon[0] = -(1.0 + f);
+    c->gcon[1] = c->gcon[4] = f;
+    c->gcon[5] = delta / rho2;
+    c->gcon[7] = c->gcon[13] = a / rho2;
+    c->gcon[10] = 1.0 / rho2;
+    c->gcon[15] = 1.0 / (rho2 * safe_s2);
+    const double drho_dr = 2.0 * r;
+    const double drho_dth = -2.0 * a2 * s * cc;
+    const double ds2_dth = 2.0 * s * cc;
+    const double df_dr = 2.0 * mass * (rho2 - r * drho_dr) / rho4;
+    const double df_dth = -2.0 * mass * r * drho_dth / rho4;
+#define SYM(M, mu, nu, v) do { (M)[(mu) * 4 + (nu)] = (v); (M)[(nu) * 4 + (mu)] = (v); } while (0)
+    SYM(c->dgr, 0, 0, df_dr);
+    SYM(c->dgth, 0, 0, df_dth);
+    SYM(c->dgr, 0, 1, df_dr);
+    SYM(c->dgth, 0, 1, df_dth);
+    SYM(c->dgr, 0, 3, -a * s2 * df_dr);
+    SYM(c->dgth, 0, 3, -a * (s2 * df_dth + ds2_dth * f));
+    SYM(c->dgr, 1, 1, df_dr);
+    SYM(c->dgth, 1, 1, df_dth);
+    SYM(c->dgr, 1, 3, -a * s2 * df_dr);
+    SYM(c->dgth, 1, 3, -a * (ds2_dth * (1.0 + f) + s2 * df_dth));
+    SYM(c->dgr, 2, 2, drho_dr);
+    SYM(c->dgth, 2, 2, drho_dth);
+    {
+        const double bracket = rho2 + a2 * s2 * (1.0 + f);
+        const double dbr_dr = drho_dr + a2 * s2 * df_dr;
+        const double dbr_dth = drho_dth + a2 * (ds2_dth * (1.0 + f) + s2 * df_dth);
+        SYM(c->dgr, 3, 3, s2 * dbr_dr);
+        SYM(c->dgth, 3, 3, ds2_dth * bracket + s2 * dbr_dth);
+    }
+#undef SYM
+}
+
+/* spherical_ks_connection_contract_fixed_p (kerr_schild_spherical.hpp:318-356):
+ * Gamma^mu_ab p^a q^b with the p row products precomputed. */
+static void sks_contract(const sks_work* c, const double* p, const double* q, double* out) {
+    double rp_r[4], rp_t[4], rq_r[4], rq_t[4];
+    for (int nu = 0; nu < 4; ++nu) {
+        double ar = 0.0, at = 0.0, br = 0.0, bt = 0.0;
+        for (int b = 0; b < 4; ++b) {
+            ar += c->dgr[nu * 4 + b] * p[b];
+            at += c->dgth[nu * 4 + b] * p[b];
+            br += c->dgr[nu * 4 + b] * q[b];
+            bt += c->dgth[nu * 4 + b] * q[b];
+        }
+        rp_r[nu] = ar;
+        rp_t[nu] = at;
+        rq_r[nu] = br;
+        rq_t[nu] = bt;
+    }
+    double pD1q = 0.0, pD2q = 0.0;
+    for (int nu = 0; nu < 4; ++nu) {
+        pD1q += p[nu] * rq_r[nu];
+        pD2q += p[nu] * rq_t[nu];
+    }
+    double t[4];
+    for (int nu = 0; nu < 4; ++nu) {
+        double term = p[1] * rq_r[nu] + q[1] * rp_r[nu] + p[2] * rq_t[nu] + q[2] * rp_t[nu];
+        if (nu == 1)
+            term -= pD1q;
+        else if (nu == 2)
+            term -= pD2q;
+        t[nu] = term;
+    }
+    for (int mu = 0; mu < 4; ++mu) {
+        double comp = 0.0;
+        for (int nu = 0; nu < 4; ++nu) comp += c->gcon[mu * 4 + nu] * t[nu];
+        out[mu] = 0.5 * comp;
+    }
+}
+
+/* FMKSConnectionWork (kerr_fmks.hpp:336-402): r, theta and the first and
+ * second derivatives of theta(x1, x2), with the spherical-KS work at (r, theta). */
+typedef struct fmks_work {
+    double r, th, dth1, dth2, d2th11, d2th12, d2th22;
+    sks_work sph;
+} fmks_work;
+
+static void fmks_build(double mass, double spin, const kpo_fmks* f, const double* X, fmks_work* c) {
+    const double x1 = X[1], x2 = X[2];
+    const double y = 2.0 * x2 - 1.0;
+    const double z = y / f->poly_xt;
+    const double expfac = exp(f->mks_smooth * (f->startx1 - x1));
+    double za, zam1, zam2;
+    if (absv(f->poly_alpha - 14.0) < 1e-12) {
+        const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+        zam2 = z8 * z4;
+        zam1 = zam2 * z;
+        za = zam2 * z2;
+    } else {
+        za = fmks_power(z, f->poly_alpha);
+        zam1 = fmks_power(z, f->poly_alpha - 1.0);
+        zam2 = fmks_power(z, f->poly_alpha - 2.0);
+    }
+    const double thG = PI_D * x2 + ((1.0 - f->hslope) / 2.0) * sin(2.0 * PI_D * x2);
+    const double thJ = f->poly_norm * y * (1.0 + za / (f->poly_alpha + 1.0)) + 0.5 * PI_D;
+    const double dthG = PI_D + (1.0 - f->hslope) * PI_D * cos(2.0 * PI_D * x2);
+    const double dthJ = 2.0 * f->poly_norm * (1.0 + za / (f->poly_alpha + 1.0)) +
+                        (2.0 * f->poly_alpha * f->poly_norm * y * zam1) / ((f->poly_alpha + 1.0) * f->poly_xt);
+    const double d2thG = -2.0 * (1.0 - f->hslope) * PI_D * PI_D * sin(2.0 * PI_D * x2);
+    const double d2thJ = 4.0 * f->poly_norm * f->poly_alpha / (f->poly_alpha + 1.0) *
+                         (2.0 * zam1 / f->poly_xt + y * (f->poly_alpha - 1.0) * zam2 / (f->poly_xt * f->poly_xt));
+    c->r = exp(x1);
+    c->th = thG + expfac * (thJ - thG);
+    c->dth1 = -f->mks_smooth * expfac * (thJ - thG);
+    c->dth2 = dthG + expfac * (dthJ - dthG);
+    c->d2th11 = -f->mks_smooth * c->dth1;
+    c->d2th12 = -f->mks_smooth * expfac * (dthJ - dthG);
+    c->d2th22 = d2thG + expfac * (d2thJ - d2thG);
+    sks_build(mass, spin, c->r, c->th, &c->sph);
+}
+
+/* fmks_to_spherical_vector (kerr_fmks.hpp:404-413). */
+static void fmks_to_sph(const fmks_work* c, const double* v, double* out) {
+    out[0] = v[0];
+    out[1] = c->r * v[1];
+    out[2] = c->dth1 * v[1] + c->dth2 * v[2];
+    out[3] = v[3];
+}
+
+/* fmks_from_spherical_connection_contract (kerr_fmks.hpp:415-432). */
+static void fmks_from_sph(const fmks_work* c, double* hs, const double* p, const double* q, double* out) {
+    hs[1] += c->r * p[1] * q[1];
+    hs[2] += c->d2th11 * p[1] * q[1] + c->d2th12 * (p[1] * q[2] + p[2] * q[1]) + c->d2th22 * p[2] * q[2];
+    out[0] = hs[0];
+    out[1] = hs[1] / c->r;
+    out[2] = (hs[2] - c->dth1 * hs[1] / c->r) / c->dth2;
+    out[3] = hs[3];
+}
+
+/* transport_rhs for KerrFMKSMetric (kerr_fmks.hpp:455-479). */
+void kpo_fmks_transport_rhs(double mass, double spin, const kpo_fmks* f, const kpo_state* s, kpo_state* d) {
+    fmks_work c;
+    fmks_build(mass, spin, f, s->v, &c);
+    const double* k = s->v + 4;
+    double ks[4], e1s[4], e2s[4];
+    fmks_to_sph(&c, k, ks);
+    fmks_to_sph(&c, s->v + 8, e1s);
+    fmks_to_sph(&c, s->v + 12, e2s);
+    double hs[4], out[4];
+    for (int mu = 0; mu < 4; ++mu) d->v[mu] = k[mu];
+    sks_contract(&c.sph, ks, ks, hs);
+    fmks_from_sph(&c, hs, k, k, out);
+    for (int mu = 0; mu < 4; ++mu) d->v[4 + mu] = -out[mu];
+    sks_contract(&c.sph, ks, e1s, hs);
+    fmks_from_sph(&c, hs, k, s->v + 8, out);
+    for (int mu = 0; mu < 4; ++mu) d->v[8 + mu] = -out[mu];
+    sks_contract(&c.sph, ks, e2s, hs);
+    fmks_from_sph(&c, hs, k, s->v + 12, out);
+    for (int mu = 0; mu < 4; ++mu) d->v[12 + mu] = -out[mu];
+}
+
+/* Generic transport RHS from the FMKS metric by central differences of gcov
+ * and the Christoffel formula: the check on the specialised contraction. */
+void kpo_fmks_transport_rhs_fd(double mass, double spin, const kpo_fmks* f, const kpo_state* s, double h,
+                               kpo_state* d) {
+    double gcon[16], dg[64];
+    kpo_fmks_gcon(mass, spin, f, s->v, gcon);
+    for (int a = 0; a < 4; ++a) {
+        double xp[4], xm[4], gp[16], gm[16];
+        for (int i = 0; i < 4; ++i) xp[i] = xm[i] = s->v[i];
+        xp[a] += h;
+        xm[a] -= h;
+        kpo_fmks_gcov(mass, spin, f, xp, gp);
+        kpo_fmks_gcov(mass, spin, f, xm, gm);
+        for (int i = 0; i < 16; ++i) dg[i * 4 + a] = (gp[i] - gm[i]) / (2.0 * h);
+    }
+    double gam[64];
+    for (int mu = 0; mu < 4; ++mu)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                double sum = 0.0;
+                for (int nu = 0; nu < 4; ++nu)
+                    sum += gcon[mu * 4 + nu] *
+                           (dg[(nu * 4 + a) * 4 + b] + dg[(nu * 4 + b) * 4 + a] - dg[(a * 4 + b) * 4 + nu]);
+                gam[(mu * 4 + a) * 4 + b] = 0.5 * sum;
+            }
+    const double* k = s->v + 4;
+    for (int mu = 0; mu < 4; ++mu) {
+        d->v[mu] = k[mu];
+        double ck = 0.0, c1 = 0.0, c2 = 0.0;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                const double gm = gam[(mu * 4 + a) * 4 + b];
+                ck += gm * k[a] * k[b];
+                c1 += gm * k[a] * s->v[8 + b];
+                c2 += gm * k[a] * s->v[12 + b];
+            }
+        d->v[4 + mu] = -ck;
+        d->v[8 + mu] = -c1;
+        d->v[12 + mu] = -c2;
+    }
+}
+
+/* ======================================================================
  * Integrators (src/geodesic/rk4.hpp)
  * ==================================================================== */
 typedef struct metric_ctx {
     double mass, spin;
     int64_t* rhs_counter;
+    int fmks;               /* 0: Boyer-Lindquist, 1: FMKS */
+    const kpo_fmks* fm;
 } metric_ctx;
 
 static inline void rhs(const metric_ctx* m, const kpo_state* s, kpo_state* d) {
     if (m->rhs_counter) ++*m->rhs_counter;
-    kpo_bl_transport_rhs(m->mass, m->spin, s, d);
+    if (m->fmks)
+        kpo_fmks_transport_rhs(m->mass, m->spin, m->fm, s, d);
+    else
+        kpo_bl_transport_rhs(m->mass, m->spin, s, d);
+}
+
+/* metric_domain_radius (metric_domain.hpp:159-172): r = x1 (BL) or exp(x1) (FMKS). */
+static inline double radius_of(const metric_ctx* m, const double* x) { return m->fmks ? exp(x[1]) : x[1]; }
+
+static void metric_gcov(const metric_ctx* m, const double* x, double* g) {
+    if (m->fmks)
+        kpo_fmks_gcov(m->mass, m->spin, m->fm, x, g);
+    else
+        kpo_bl_gcov(m->mass, m->spin, x, g);
 }
 
 /* axpy(y, a, x) = y + x * a (src/geodesic/state.hpp:39-44). */
@@ -301,7 +654,7 @@ static void rk4(const metric_ctx* m, const kpo_state* s, double h, kpo_state* ou
 }
 
 void kpo_bl_rk4_step(double mass, double spin, const kpo_state* s, double h, kpo_state* out) {
-    metric_ctx m = {mass, spin, NULL};
+    metric_ctx m = {mass, spin, NULL, 0, NULL};
     rk4(&m, s, h, out);
 }
 
@@ -383,7 +736,7 @@ int32_t kpo_bl_adaptive_rk4_step(double mass, double spin, const kpo_state* s,
                                  double min_step, double max_step, int32_t max_attempts,
                                  kpo_state* state, kpo_state* mid_state,
                                  double* used_h, double* next_h, double* error) {
-    metric_ctx m = {mass, spin, NULL};
+    metric_ctx m = {mass, spin, NULL, 0, NULL};
     adaptive_control c = {tolerance, min_step, max_step, max_attempts};
     adaptive_result r;
     adaptive_rk4(&m, s, requested_h, c, &r);
@@ -527,7 +880,7 @@ void kpo_pixel_offsets(int32_t pixel, const kp_camera_params* cam, double* sx, d
 
 /* initialize_pinhole_camera_ray (camera.hpp:484-513) with
  * normalize_spacelike (:304-313) and orthogonalize_spacelike (:315-325). */
-static void camera_ray_from_tetrad(const kp_pass_a_params* p, const double* tetrad,
+static void camera_ray_from_tetrad(const kp_pass_a_params* p, const double* xcam, const double* tetrad,
                                    const double* gcov, int32_t pixel, kpo_state* out) {
     const double* e0 = tetrad;
     const double* e1 = tetrad + 4;
@@ -542,10 +895,7 @@ static void camera_ray_from_tetrad(const kp_pass_a_params* p, const double* tetr
     const double kt3 = 1.0 / spatial_norm;
     double sd[4];
     for (int i = 0; i < 4; ++i) sd[i] = e1[i] * kt1 + e2[i] * kt2 + e3[i] * kt3;
-    out->v[0] = 0.0;
-    out->v[1] = p->camera.radius;
-    out->v[2] = p->camera.inclination;
-    out->v[3] = 0.0;
+    for (int i = 0; i < 4; ++i) out->v[i] = xcam[i];
     for (int i = 0; i < 4; ++i) out->v[4 + i] = e0[i] * kt0 + sd[i];
     double a[4], b[4];
     for (int i = 0; i < 4; ++i) a[i] = e1[i] - sd[i] * kt1;
@@ -580,15 +930,71 @@ void kpo_bl_camera_ray(const kp_pass_a_params* p, int32_t pixel, kpo_state* out)
     const double x[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
     kpo_bl_camera_tetrad(p, tetrad);
     kpo_bl_gcov(p->mass, p->spin, x, gcov);
-    camera_ray_from_tetrad(p, tetrad, gcov, pixel, out);
+    camera_ray_from_tetrad(p, x, tetrad, gcov, pixel, out);
+}
+
+/* Camera position and ipole tetrad in FMKS (camera.hpp:303-315 and 415-453 with
+ * the FMKS branches: position (0, ln R, x2(ln R, inclination), 0), normal
+ * observer, centring covector (1, 1, 0, 0), trial vectors (0,0,1,0), (1,1,1,1)). */
+void kpo_fmks_camera_tetrad(const kp_pass_a_params* p, const kpo_fmks* f, double xcam[4], double tetrad[16]) {
+    const double x1 = log(maxv(p->camera.radius, TINY));
+    xcam[0] = 0.0;
+    xcam[1] = x1;
+    xcam[2] = kpo_fmks_x2_from_theta(f, x1, p->camera.inclination);
+    xcam[3] = 0.0;
+    double gcov[16], gcon[16];
+    kpo_fmks_gcov(p->mass, p->spin, f, xcam, gcov);
+    kpo_fmks_gcon(p->mass, p->spin, f, xcam, gcon);
+    double* e0 = tetrad;
+    double* e1 = tetrad + 4;
+    double* e2 = tetrad + 8;
+    double* e3 = tetrad + 12;
+    /* normal_observer_velocity (camera.hpp:141-152) */
+    for (int mu = 0; mu < 4; ++mu) e0[mu] = -gcon[mu * 4 + 0];
+    {
+        const double n2 = dot_g(gcov, e0, e0);
+        if (n2 < 0.0) {
+            const double inv = 1.0 / sqrt(-n2);
+            for (int mu = 0; mu < 4; ++mu) e0[mu] = e0[mu] * inv;
+        }
+    }
+    const double center[4] = {1.0, 1.0, 0.0, 0.0};
+    raise_cov(gcon, center, e3);
+    e2[0] = 0.0; e2[1] = 0.0; e2[2] = 1.0; e2[3] = 0.0;
+    e1[0] = 1.0; e1[1] = 1.0; e1[2] = 1.0; e1[3] = 1.0;
+    normalize_basis(gcov, e0);
+    project_out(gcov, e3, e0);
+    project_out(gcov, e3, e0);
+    normalize_basis(gcov, e3);
+    project_out(gcov, e2, e0);
+    project_out(gcov, e2, e3);
+    project_out(gcov, e2, e0);
+    project_out(gcov, e2, e3);
+    normalize_basis(gcov, e2);
+    project_out(gcov, e1, e0);
+    project_out(gcov, e1, e2);
+    project_out(gcov, e1, e3);
+    project_out(gcov, e1, e0);
+    project_out(gcov, e1, e2);
+    project_out(gcov, e1, e3);
+    normalize_basis(gcov, e1);
+    if (tetrad_orientation(e0, e1, e2, e3) < 0.0)
+        for (int i = 0; i < 4; ++i) e1[i] = e1[i] * -1.0;
+}
+
+void kpo_fmks_camera_ray(const kp_pass_a_params* p, const kpo_fmks* f, int32_t pixel, kpo_state* out) {
+    double x[4], tetrad[16], gcov[16];
+    kpo_fmks_camera_tetrad(p, f, x, tetrad);
+    kpo_fmks_gcov(p->mass, p->spin, f, x, gcov);
+    camera_ray_from_tetrad(p, x, tetrad, gcov, pixel, out);
 }
 
 /* ======================================================================
  * Frame diagnostics (src/geodesic/state.hpp:44-76)
  * ==================================================================== */
-static double max_frame_error(double mass, double spin, const kpo_state* s) {
+static double max_frame_error(const metric_ctx* m, const kpo_state* s) {
     double g[16];
-    kpo_bl_gcov(mass, spin, s->v, g);
+    metric_gcov(m, s->v, g);
     const double* k = s->v + 4;
     const double* e1 = s->v + 8;
     const double* e2 = s->v + 12;
@@ -1312,21 +1718,21 @@ static void surface_path_state(const metric_ctx* m, const kpo_state* first, doub
     rk4(m, first, h, out);
 }
 
-/* locate_metric_surface_crossing for a radial surface (pass_a_core.hpp:94-197).
- * For BL the inner surface value and the domain radius are both x[1]. */
+/* locate_metric_surface_crossing for a radial surface (pass_a_core.hpp:94-197);
+ * the surface value is the domain radius minus the surface radius. */
 static void locate_crossing(const metric_ctx* m, const kpo_state* first, const kpo_state* second,
                             double full_h, double surface, int composed, kpo_state* out_state,
                             double* out_h) {
     *out_state = *first;
     *out_h = 0.0;
     double lo = 0.0, hi = 1.0;
-    double flo = first->v[1] - surface;
+    double flo = radius_of(m, first->v) - surface;
     if (flo == 0.0) return;
     if (!finite_state(second)) return;
     kpo_state hi_state;
     surface_path_state(m, first, full_h, composed, &hi_state);
     if (!finite_state(&hi_state)) return;
-    double fhi = hi_state.v[1] - surface;
+    double fhi = radius_of(m, hi_state.v) - surface;
     if (!(fhi == fhi) || absv(fhi) >= LARGE) return;
     if (fhi == 0.0) {
         *out_state = hi_state;
@@ -1351,7 +1757,7 @@ static void locate_crossing(const metric_ctx* m, const kpo_state* first, const k
             use_secant = 0;
             continue;
         }
-        const double ft = ts.v[1] - surface;
+        const double ft = radius_of(m, ts.v) - surface;
         if (!(ft == ft) || absv(ft) >= LARGE) {
             hi = trial;
             use_secant = 0;
@@ -1386,7 +1792,7 @@ static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const
     int entered = 0;
     int steps_inside = 0;
     for (int n = 0; n < p->max_steps; ++n) {
-        const double r_bl = state.v[1];
+        const double r_bl = radius_of(m, state.v);
         if (!entered) {
             if (r_bl <= p->outer_radius) {
                 entered = 1;
@@ -1405,7 +1811,7 @@ static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const
         const kpo_state old = state;
         const double eff_min = absv(p->min_step);
         const double old_r = r_bl;
-        const double old_inner = old.v[1] - p->inner_radius;
+        const double old_inner = r_bl - p->inner_radius;
         double used_h = h_current;
         kpo_state next;
         if (p->adaptive) {
@@ -1428,8 +1834,8 @@ static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const
         } else {
             rk4(m, &state, used_h, &next);
         }
-        const double next_r = next.v[1];
-        const double next_inner = next.v[1] - p->inner_radius;
+        const double next_r = radius_of(m, next.v);
+        const double next_inner = next_r - p->inner_radius;
         if (!entered && next_r <= p->outer_radius) {
             entered = 1;
             steps_inside = 0;
@@ -1473,7 +1879,7 @@ static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const
 
 int32_t kpo_bl_pass_a_endpoint(const kp_pass_a_params* p, int32_t pixel, kpo_state* endpoint,
                                int32_t* steps, int32_t* reason) {
-    metric_ctx m = {p->mass, p->spin, NULL};
+    metric_ctx m = {p->mass, p->spin, NULL, 0, NULL};
     kpo_state init;
     kpo_bl_camera_ray(p, pixel, &init);
     return pass_a_endpoint(p, &m, &init, endpoint, steps, reason);
@@ -1506,6 +1912,337 @@ static void sample_coefficients(const kp_pass_a_params* p, const kp_riaf_model* 
     if (!p->emission_selection.faraday_rotation) c->rV = 0.0;
 }
 
+/* ======================================================================
+ * GRMHD grid model on FMKS data (src/model/grmhd_grid.hpp) with derived
+ * scalars precomputed as the iharm loader does (src/grmhd/iharm_loader.cpp).
+ * Cells are kp_grmhd_cell records [n1][n2][n3] in host memory here.
+ * ==================================================================== */
+static kpo_fmks grmhd_map(const kp_grmhd_model* g) {
+    kpo_fmks f = {g->startx1, g->hslope, g->mks_smooth, g->poly_alpha, g->poly_xt, g->poly_norm};
+    return f;
+}
+
+/* length_unit_cgs, rho_unit_cgs, b_unit_cgs, dlambda_scale (grmhd_grid.hpp:140-160). */
+static double grmhd_length_unit(const kp_grmhd_model* g) {
+    return 6.67430e-8 * g->mbh_solar * 1.98847e33 / (2.99792458e10 * 2.99792458e10);
+}
+static double grmhd_rho_unit(const kp_grmhd_model* g) {
+    const double L = grmhd_length_unit(g);
+    return g->M_unit / maxv(L * L * L, 1e-300);
+}
+static double grmhd_b_unit(const kp_grmhd_model* g) {
+    return 2.99792458e10 * sqrt(4.0 * PI_D * grmhd_rho_unit(g));
+}
+double kpo_grmhd_dlambda_scale(const kp_grmhd_model* g) { return grmhd_length_unit(g) / maxv(g->freq_cgs, 1.0); }
+
+/* native_fluid_metric (grmhd_grid.hpp:721-766): FMKS gcov and the first row of gcon. */
+static void grmhd_native_metric(const kp_grmhd_model* g, const kpo_fmks* f, double x1, double x2, double r,
+                                double th, double gcov[16], double gcon0[4]) {
+    const double cth = cos(th), sth = sin(th);
+    const double s2 = sth * sth;
+    const double a2 = g->spin * g->spin;
+    const double rho2 = r * r + a2 * cth * cth;
+    const double fr = 2.0 * r / rho2;
+    const double dth1 = fmks_dtheta_dx1(f, x1, x2);
+    const double dth2 = fmks_dtheta_dx2(f, x1, x2);
+    for (int i = 0; i < 16; ++i) gcov[i] = 0.0;
+    const double g00 = -1.0 + fr, g01 = fr, g03 = -g->spin * fr * s2, g11 = 1.0 + fr;
+    const double g13 = -g->spin * s2 * (1.0 + fr), g22 = rho2, g33 = s2 * (rho2 + a2 * s2 * (1.0 + fr));
+    gcov[0] = g00;
+    gcov[1] = gcov[4] = g01 * r;
+    gcov[3] = gcov[12] = g03;
+    gcov[5] = g11 * r * r + g22 * dth1 * dth1;
+    gcov[6] = gcov[9] = g22 * dth1 * dth2;
+    gcov[7] = gcov[13] = g13 * r;
+    gcov[10] = g22 * dth2 * dth2;
+    gcov[15] = g33;
+    const double inv_r = 1.0 / maxv(r, 1e-300);
+    const double inv_dth2 = 1.0 / maxv(dth2, 1e-300);
+    gcon0[0] = -(1.0 + fr);
+    gcon0[1] = fr * inv_r;
+    gcon0[2] = -fr * dth1 * inv_r * inv_dth2;
+    gcon0[3] = 0.0;
+}
+
+/* Fluid four-velocity and magnetic four-vector from native primitives
+ * (grmhd_grid.hpp:980-1014, iharm_loader.cpp:453-487).  Returns b^2 (floored). */
+static double grmhd_four_vectors(const double gcov[16], const double gcon0[4], double U1, double U2, double U3,
+                                 double B1, double B2, double B3, double ucon[4], double bcon[4]) {
+    const double v[4] = {0.0, U1, U2, U3};
+    double spatial = 0.0;
+    for (int i = 1; i < 4; ++i)
+        for (int j = 1; j < 4; ++j) spatial += gcov[i * 4 + j] * v[i] * v[j];
+    const double vfac = sqrt(maxv(-1.0 / gcon0[0] * (1.0 + absv(spatial)), 0.0));
+    ucon[0] = -vfac * gcon0[0];
+    for (int i = 1; i < 4; ++i) ucon[i] = v[i] - vfac * gcon0[i];
+    double ucov[4];
+    for (int mu = 0; mu < 4; ++mu) {
+        double sum = 0.0;
+        for (int nu = 0; nu < 4; ++nu) sum += gcov[mu * 4 + nu] * ucon[nu];
+        ucov[mu] = sum;
+    }
+    const double B[4] = {0.0, B1, B2, B3};
+    double udotB = 0.0;
+    for (int i = 1; i < 4; ++i) udotB += ucov[i] * B[i];
+    bcon[0] = udotB;
+    for (int i = 1; i < 4; ++i) bcon[i] = (B[i] + ucon[i] * udotB) / maxv(ucon[0], 1e-300);
+    double bsq = 0.0;
+    for (int mu = 0; mu < 4; ++mu) {
+        double bc = 0.0;
+        for (int nu = 0; nu < 4; ++nu) bc += gcov[mu * 4 + nu] * bcon[nu];
+        bsq += bcon[mu] * bc;
+    }
+    return maxv(absv(bsq), 1e-40);
+}
+
+/* KPOLARISIharmDerivedScalars (iharm_loader.cpp:420-505): per cell centre. */
+void kpo_grmhd_derive_scalars(const kp_grmhd_model* g, kp_grmhd_cell* cells) {
+    const double mp = 1.67262171e-24, me = 9.1093826e-28;
+    const double mp_me = mp / me;
+    const double game = 4.0 / 3.0, gamp = 5.0 / 3.0;
+    const kpo_fmks f = grmhd_map(g);
+    const double rho_unit = grmhd_rho_unit(g), b_unit = grmhd_b_unit(g);
+    const int n1 = g->n1, n2 = g->n2, n3 = g->n3;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j) {
+            const double x1 = g->startx1 + ((double)i + 0.5) * g->dx1;
+            const double x2 = g->startx2 + ((double)j + 0.5) * g->dx2;
+            const double r = exp(x1);
+            const double th = kpo_fmks_theta(&f, x1, x2);
+            double gcov[16], gcon0[4];
+            grmhd_native_metric(g, &f, x1, x2, r, th, gcov, gcon0);
+            for (int k = 0; k < n3; ++k) {
+                kp_grmhd_cell* c = &cells[((size_t)i * n2 + j) * n3 + k];
+                const double rho = c->prim[0], uu = c->prim[1];
+                double ucon[4], bcon[4];
+                const double bsq = grmhd_four_vectors(gcov, gcon0, c->prim[2], c->prim[3], c->prim[4], c->prim[5],
+                                                      c->prim[6], c->prim[7], ucon, bcon);
+                const double beta = (g->gam - 1.0) * uu / (0.5 * bsq);
+                const double sigma = bsq / maxv(rho, 1e-300);
+                const double betasq = beta * beta / maxv(g->beta_crit * g->beta_crit, 1e-40);
+                const double trat = (g->trat_large * betasq + g->trat_small) / (1.0 + betasq);
+                const double thetae_unit = mp_me * (game - 1.0) * (gamp - 1.0) / ((gamp - 1.0) + (game - 1.0) * trat);
+                c->derived[0] = rho * rho_unit / (mp + me) * g->ne_factor;
+                c->derived[1] = thetae_unit * uu / maxv(rho, 1e-300);
+                c->derived[2] = sqrt(bsq) * b_unit;
+                c->derived[3] = sigma;
+                c->derived[4] = beta;
+            }
+        }
+}
+
+/* native_indices_from_native_coords (grmhd_grid.hpp:271-296). */
+static int grmhd_indices(const kp_grmhd_model* g, double x1, double x2, double phi, int* i0, int* j0, int* k0,
+                         double* di, double* dj, double* dk) {
+    if (!(x1 >= g->startx1 && x1 < g->startx1 + (double)g->n1 * g->dx1 && x2 >= g->startx2 &&
+          x2 < g->startx2 + (double)g->n2 * g->dx2))
+        return 0;
+    const double two_pi = 6.283185307179586476925286766559005768;
+    phi = phi - floor(phi / two_pi) * two_pi;
+    const double ii = (x1 - g->startx1) / g->dx1 - 0.5;
+    const double jj = (x2 - g->startx2) / g->dx2 - 0.5;
+    const double kk = (phi - g->startx3) / g->dx3 - 0.5;
+    *i0 = (int)floor(ii);
+    *j0 = (int)floor(jj);
+    *k0 = (int)floor(kk);
+    if (*i0 < -1) *i0 = -1;
+    if (*j0 < -1) *j0 = -1;
+    if (*k0 < -1) *k0 = -1;
+    if (*i0 >= g->n1) *i0 = g->n1 - 1;
+    if (*j0 >= g->n2) *j0 = g->n2 - 1;
+    if (*k0 >= g->n3) *k0 = g->n3 - 1;
+    *di = clampv(ii - (double)*i0, 0.0, 1.0);
+    *dj = clampv(jj - (double)*j0, 0.0, 1.0);
+    *dk = clampv(kk - (double)*k0, 0.0, 1.0);
+    return 1;
+}
+
+/* map_ghost_index (grmhd_grid.hpp:309-321): clamped radius, polar reflection
+ * across the axis (k + n3/2), periodic phi. */
+static size_t grmhd_cell_index(const kp_grmhd_model* g, int i, int j, int k) {
+    if (i < 0) i = 0;
+    if (i >= g->n1) i = g->n1 - 1;
+    if (j < 0) {
+        j = 0;
+        k += g->n3 / 2;
+    } else if (j >= g->n2) {
+        j = g->n2 - 1;
+        k += g->n3 / 2;
+    }
+    while (k < 0) k += g->n3;
+    while (k >= g->n3) k -= g->n3;
+    return ((size_t)i * (size_t)g->n2 + (size_t)j) * (size_t)g->n3 + (size_t)k;
+}
+
+/* make_interp_stencil (grmhd_grid.hpp:379-408). */
+static void grmhd_stencil(const kp_grmhd_model* g, int i0, int j0, int k0, double di, double dj, double dk,
+                          size_t cell[8], double w[8]) {
+    const double wi0 = 1.0 - di, wi1 = di, wj0 = 1.0 - dj, wj1 = dj, wk0 = 1.0 - dk, wk1 = dk;
+    cell[0] = grmhd_cell_index(g, i0, j0, k0);
+    cell[1] = grmhd_cell_index(g, i0 + 1, j0, k0);
+    cell[2] = grmhd_cell_index(g, i0, j0 + 1, k0);
+    cell[3] = grmhd_cell_index(g, i0 + 1, j0 + 1, k0);
+    cell[4] = grmhd_cell_index(g, i0, j0, k0 + 1);
+    cell[5] = grmhd_cell_index(g, i0 + 1, j0, k0 + 1);
+    cell[6] = grmhd_cell_index(g, i0, j0 + 1, k0 + 1);
+    cell[7] = grmhd_cell_index(g, i0 + 1, j0 + 1, k0 + 1);
+    w[0] = wi0 * wj0 * wk0;
+    w[1] = wi1 * wj0 * wk0;
+    w[2] = wi0 * wj1 * wk0;
+    w[3] = wi1 * wj1 * wk0;
+    w[4] = wi0 * wj0 * wk1;
+    w[5] = wi1 * wj0 * wk1;
+    w[6] = wi0 * wj1 * wk1;
+    w[7] = wi1 * wj1 * wk1;
+}
+
+/* sigma_smooth_factor (grmhd_grid.hpp:495-503). */
+static double sigma_smooth_factor(const kp_grmhd_model* g, double sigma) {
+    double above = g->sigma_cut;
+    if (g->sigma_cut_high > 0.0) above = g->sigma_cut_high;
+    if (sigma < g->sigma_cut) return 1.0;
+    if (sigma >= above) return 0.0;
+    const double dsig = maxv(above - g->sigma_cut, 1e-300);
+    return cos(PI_D / 2.0 / dsig * (sigma - g->sigma_cut));
+}
+
+/* Trilinear sample of primitive v (interp_prim_stencil, grmhd_grid.hpp:410-420). */
+double kpo_grmhd_sample_prim(const kp_grmhd_model* g, const kp_grmhd_cell* cells, int v, double x1, double x2,
+                             double phi) {
+    int i0, j0, k0;
+    double di, dj, dk;
+    if (!grmhd_indices(g, x1, x2, phi, &i0, &j0, &k0, &di, &dj, &dk)) return 0.0;
+    size_t cell[8];
+    double w[8];
+    grmhd_stencil(g, i0, j0, k0, di, dj, dk, cell, w);
+    double out = 0.0;
+    for (int n = 0; n < 8; ++n) out += w[n] * (double)cells[cell[n]].prim[v];
+    return out;
+}
+
+/* GRMHDGridRadiationModel::coefficients for FMKS data and the FMKS metric
+ * (grmhd_grid.hpp:883-1080 fluid_state, 1091-1240 coefficients). */
+void kpo_grmhd_coefficients(const kp_grmhd_model* g, const kpo_state* s, const kpo_perturbation* pert,
+                            int faraday_rotation, kpo_coeffs* out) {
+    memset(out, 0, sizeof(*out));
+    const kp_grmhd_cell* cells = (const kp_grmhd_cell*)g->cells;
+    const kpo_fmks f = grmhd_map(g);
+    const double x1 = s->v[1], x2 = s->v[2], phi = s->v[3];
+    const double r = exp(x1);
+    const double th = kpo_fmks_theta(&f, x1, x2);
+    if (!(r >= g->r_in && r < g->r_out)) return;
+    int i0, j0, k0;
+    double di, dj, dk;
+    if (!grmhd_indices(g, x1, x2, phi, &i0, &j0, &k0, &di, &dj, &dk)) return;
+    size_t cell[8];
+    double w[8];
+    grmhd_stencil(g, i0, j0, k0, di, dj, dk, cell, w);
+    double prim[8], der[5];
+    for (int v = 0; v < 8; ++v) {
+        double acc = 0.0;
+        for (int n = 0; n < 8; ++n) acc += w[n] * (double)cells[cell[n]].prim[v];
+        prim[v] = acc;
+    }
+    if (!(prim[0] > 0.0 && prim[1] > 0.0)) return;
+    double gcov[16], gcon0[4];
+    grmhd_native_metric(g, &f, x1, x2, r, th, gcov, gcon0);
+    double ucon[4], bcon[4];
+    const double bsq = grmhd_four_vectors(gcov, gcon0, prim[2], prim[3], prim[4], prim[5], prim[6], prim[7], ucon,
+                                          bcon);
+    const double bnorm_geom = sqrt(bsq);
+    for (int v = 0; v < 5; ++v) {
+        double acc = 0.0;
+        for (int n = 0; n < 8; ++n) acc += w[n] * cells[cell[n]].derived[v];
+        der[v] = acc;
+    }
+    double ne = der[0], thetae = der[1], b_cgs = der[2];
+    const double sigma = der[3];
+    ne *= sigma_smooth_factor(g, sigma);
+    /* PlasmaPerturbation::apply (plasma_perturbation.hpp:17-32) */
+    if (pert && pert->parameter != 0 && pert->log_scale != 0.0) {
+        const double fq = exp(pert->log_scale);
+        if (pert->parameter == 1) {
+            ne *= fq;
+            b_cgs *= exp(pert->log_scale * 0.5);
+        } else if (pert->parameter == 2) {
+            thetae *= fq;
+        } else if (pert->parameter == 3) {
+            b_cgs *= fq;
+        }
+    }
+    const double* k = s->v + 4;
+    const double* e1 = s->v + 8;
+    const double* e2 = s->v + 12;
+    double ucov[4];
+    for (int mu = 0; mu < 4; ++mu) {
+        double sum = 0.0;
+        for (int nu = 0; nu < 4; ++nu) sum += gcov[mu * 4 + nu] * ucon[nu];
+        ucov[mu] = sum;
+    }
+    double uk = 0.0;
+    for (int mu = 0; mu < 4; ++mu) uk += ucov[mu] * k[mu];
+    const double nu_scale = maxv(absv(-uk), 1e-8);
+    const double nu = maxv(g->freq_cgs * nu_scale, 1.0);
+    const double bnorm = maxv(bnorm_geom, 1e-150);
+    const double inv_bnorm = 1.0 / bnorm;
+    double kdotb = 0.0, ub = 0.0, ue1 = 0.0, ue2 = 0.0, ke1 = 0.0, ke2 = 0.0, e1b = 0.0, e2b = 0.0;
+    for (int mu = 0; mu < 4; ++mu) {
+        double bcov = 0.0, kcov = 0.0;
+        for (int n = 0; n < 4; ++n) {
+            bcov += gcov[mu * 4 + n] * bcon[n];
+            kcov += gcov[mu * 4 + n] * k[n];
+        }
+        kdotb += k[mu] * bcov;
+        ub += ucov[mu] * bcon[mu];
+        ue1 += ucov[mu] * e1[mu];
+        ue2 += ucov[mu] * e2[mu];
+        ke1 += kcov * e1[mu];
+        ke2 += kcov * e2[mu];
+        e1b += bcov * e1[mu];
+        e2b += bcov * e2[mu];
+    }
+    const double cos_theta = clampv(kdotb / maxv(nu_scale * bnorm, 1e-30), -1.0, 1.0);
+    const double sin_theta = maxv(sqrt(maxv(0.0, 1.0 - cos_theta * cos_theta)), 1e-4);
+    kpo_coeffs mc;
+    kpo_thermal_coefficients(g->emission_type, nu, ne, thetae, b_cgs, 0.0, sin_theta, cos_theta, g->max_pol_frac,
+                             1e-4, g->emission_scale, g->absorption_scale, g->faraday_scale, &mc);
+    const double ubunit = ub * inv_bnorm;
+    const double kbunit = kdotb * inv_bnorm;
+    double b1 = e1b * inv_bnorm;
+    double b2 = e2b * inv_bnorm;
+    if (absv(uk) > 1e-300) {
+        const double minus_inv_uk = -1.0 / uk;
+        const double eKb = minus_inv_uk * kbunit - ubunit;
+        b1 += ue1 * ubunit - (minus_inv_uk * ke1 - ue1) * eKb;
+        b2 += ue2 * ubunit - (minus_inv_uk * ke2 - ue2) * eKb;
+    }
+    const double bproj2 = b1 * b1 + b2 * b2;
+    out->jI = mc.jI;
+    out->jV = mc.jV;
+    out->aI = mc.aI;
+    out->aV = mc.aV;
+    out->rV = faraday_rotation ? mc.rV : 0.0;
+    if (bproj2 > 1e-30) {
+        const double cos2chi = (b2 * b2 - b1 * b1) / bproj2;
+        const double sin2chi = -2.0 * b1 * b2 / bproj2;
+        out->jQ = mc.jQ * cos2chi;
+        out->jU = mc.jQ * sin2chi;
+        out->aQ = mc.aQ * cos2chi;
+        out->aU = mc.aQ * sin2chi;
+        out->rQ = mc.rQ * cos2chi;
+        out->rU = mc.rQ * sin2chi;
+    }
+}
+
+/* The radiation model of one image: the analytic RIAF or a GRMHD grid. */
+typedef struct model_ctx {
+    const kp_riaf_model* riaf;
+    const kp_grmhd_model* grmhd;
+} model_ctx;
+
 /* The frequency lists of one pass B: output frequencies and the frequencies
  * whose coefficients drive the substep control.  A single-frequency image is
  * the list {freq_cgs} for both (pass_b.hpp:515-722 is the nfreq = 1 case of
@@ -1520,18 +2257,27 @@ typedef struct freq_plan {
 } freq_plan;
 
 /* Coefficients of the model copy with freq_cgs = f, and its dlambda_scale. */
-static double coeffs_at_frequency(const kp_pass_a_params* p, const kp_riaf_model* model, double f,
+static double coeffs_at_frequency(const kp_pass_a_params* p, const model_ctx* mc, double f,
                                   const kpo_state* sample, int inside, const kpo_perturbation* pert,
                                   kpo_coeffs* c, kpo_stats* stats) {
-    kp_riaf_model mf = *model;
-    mf.freq_cgs = f;
     memset(c, 0, sizeof(*c));
+    if (mc->grmhd) {
+        kp_grmhd_model gf = *mc->grmhd;
+        gf.freq_cgs = f;
+        if (inside) {
+            if (stats) stats->coefficient_evals++;
+            kpo_grmhd_coefficients(&gf, sample, pert, p->emission_selection.faraday_rotation, c);
+        }
+        return kpo_grmhd_dlambda_scale(&gf);
+    }
+    kp_riaf_model mf = *mc->riaf;
+    mf.freq_cgs = f;
     if (inside) sample_coefficients(p, &mf, sample, pert, c, stats);
     return riaf_dlambda_scale(&mf);
 }
 
 /* trace_pass_b_segment_model_endpoint_pixel_metric (pass_b.hpp:515-722). */
-static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
+static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
                    const kpo_perturbation* pert, const freq_plan* fp, const metric_ctx* m,
                    const kpo_state* camera, const kpo_state* endpoint, int pass_a_steps,
                    int pass_a_reason, int valid, pixel_result* out, kpo_stats* stats) {
@@ -1544,8 +2290,10 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
     if (!valid) {
         out->closure_x = spatial_distance(state.v, camera->v);
         out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
-        out->final_null = kpo_bl_dot(p->mass, p->spin, state.v, state.v + 4, state.v + 4);
-        out->frame_error = max_frame_error(p->mass, p->spin, &state);
+        double g[16];
+        metric_gcov(m, state.v, g);
+        out->final_null = dot_g(g, state.v + 4, state.v + 4);
+        out->frame_error = max_frame_error(m, &state);
         return;
     }
     const adaptive_control tcontrol = {p->adaptive_tolerance, p->min_step, p->max_step, 10};
@@ -1560,7 +2308,7 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
         double h = h_current;
         const kpo_state old = state;
         const double eff_min = absv(p->min_step);
-        const double start_r = old.v[1];
+        const double start_r = radius_of(m, old.v);
         if (start_r <= p->outer_radius && cap > 0.0) h = minv(h, cap);
         kpo_state next, sample;
         if (p->adaptive) {
@@ -1584,8 +2332,8 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
             rk4(m, &old, h, &next);
             sample = next;
         }
-        const double s0 = old.v[1] - p->camera.radius;
-        const double s1 = next.v[1] - p->camera.radius;
+        const double s0 = radius_of(m, old.v) - p->camera.radius;
+        const double s1 = radius_of(m, next.v) - p->camera.radius;
         const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1
                             : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
         if (crosses) {
@@ -1617,8 +2365,9 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
                 }
             }
         }
-        const double old_r = old.v[1], sample_r = sample.v[1], next_r = next.v[1];
-        const int inside = sample.v[1] - p->inner_radius > 0.0 && sample_r < p->outer_radius;
+        const double old_r = radius_of(m, old.v), sample_r = radius_of(m, sample.v),
+                     next_r = radius_of(m, next.v);
+        const int inside = sample_r - p->inner_radius > 0.0 && sample_r < p->outer_radius;
         const double local_max_rs =
             (p->max_radiation_step > 0.0 &&
              (old_r <= p->outer_radius || sample_r <= p->outer_radius || next_r <= p->outer_radius))
@@ -1674,7 +2423,7 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
     }
     /* screen_overlap (pass_b.hpp:87-98) at the final position. */
     double g[16];
-    kpo_bl_gcov(p->mass, p->spin, state.v, g);
+    metric_gcov(m, state.v, g);
     out->r11 = dot_g(g, camera->v + 8, state.v + 8);
     out->r12 = dot_g(g, camera->v + 8, state.v + 12);
     out->r21 = dot_g(g, camera->v + 12, state.v + 8);
@@ -1685,7 +2434,7 @@ static void pass_b(const kp_pass_a_params* p, const kp_riaf_model* model,
     out->closure_x = spatial_distance(state.v, camera->v);
     out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
     out->final_null = dot_g(g, state.v + 4, state.v + 4);
-    out->frame_error = max_frame_error(p->mass, p->spin, &state);
+    out->frame_error = max_frame_error(m, &state);
     out->steps = steps;
     if (reached) {
         reason = KP_REASON_REACHED_CAMERA;
@@ -1728,14 +2477,22 @@ static void store_pixel(kp_image_data* o, int p, int nout, int nfreq, const pixe
  * i (compact) or at the pixel index.  Split pass A / pass B per pixel
  * (src/image/split_driver.hpp:41-104); OpenMP static schedule as the
  * reference's RangePolicy. */
-static kp_status run_pixels(const kp_pass_a_params* p, const kp_riaf_model* model,
+static kp_status run_pixels(const kp_pass_a_params* p, const model_ctx* model,
                             const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
                             int count, int compact, int nout, int nthreads, kp_image_data* out,
                             kpo_stats* stats) {
     double tetrad[16], gcam[16];
-    const double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
-    kpo_bl_camera_tetrad(p, tetrad);
-    kpo_bl_gcov(p->mass, p->spin, xcam, gcam);
+    double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
+    const int fmks = p->coordinate_system == KP_COORD_FMKS;
+    const kpo_fmks fm = {p->fmks_startx1, p->fmks_hslope, p->fmks_mks_smooth, p->fmks_poly_alpha,
+                         p->fmks_poly_xt, p->fmks_poly_norm};
+    if (fmks) {
+        kpo_fmks_camera_tetrad(p, &fm, xcam, tetrad);
+        kpo_fmks_gcov(p->mass, p->spin, &fm, xcam, gcam);
+    } else {
+        kpo_bl_camera_tetrad(p, tetrad);
+        kpo_bl_gcov(p->mass, p->spin, xcam, gcam);
+    }
     int64_t rhs_total = 0, coeff_total = 0, transfer_total = 0, a_total = 0, b_total = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -1744,10 +2501,10 @@ static kp_status run_pixels(const kp_pass_a_params* p, const kp_riaf_model* mode
     for (int i = 0; i < count; ++i) {
         const int pix = first + i * stride;
         int64_t rhs_count = 0;
-        metric_ctx m = {p->mass, p->spin, &rhs_count};
+        metric_ctx m = {p->mass, p->spin, &rhs_count, fmks, &fm};
         kpo_stats local = {0, 0, 0, 0, 0};
         kpo_state camera, endpoint;
-        camera_ray_from_tetrad(p, tetrad, gcam, pix, &camera);
+        camera_ray_from_tetrad(p, xcam, tetrad, gcam, pix, &camera);
         int32_t a_steps = 0, a_reason = 0;
         const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
         pixel_result r;
@@ -1787,7 +2544,8 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
     const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
     const double f0 = model->freq_cgs;
     const freq_plan fp = {&f0, 1, &f0, 1, 1};
-    return run_pixels(p, model, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
+    const model_ctx mc = {model, NULL};
+    return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
                       out, stats);
 }
 
@@ -1805,7 +2563,8 @@ kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* p, const kp_riaf_mo
         return KP_INVALID_ARGUMENT;
     const double f0 = model->freq_cgs;
     const freq_plan fp = {&f0, 1, &f0, 1, 1};
-    return run_pixels(p, model, pert, &fp, offset, stride, count, 1, count, nthreads, out, stats);
+    const model_ctx mc = {model, NULL};
+    return run_pixels(p, &mc, pert, &fp, offset, stride, count, 1, count, nthreads, out, stats);
 }
 
 /* Multi-frequency image on shared geodesics: Stokes outputs [nfreq][npix]
@@ -1828,6 +2587,38 @@ kp_status kpo_run_riaf_multifrequency_image(const kp_pass_a_params* p, const kp_
     const int npix = p->camera.nx * p->camera.ny;
     const int begin = pixel_begin < 0 ? 0 : pixel_begin;
     const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
-    return run_pixels(p, model, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
+    const model_ctx mc = {model, NULL};
+    return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
                       out, stats);
+}
+
+/* GRMHD image on FMKS geodesics (src/image/grmhd.cpp:132-173 FMKS backend),
+ * one or more frequencies; g->cells are HOST kp_grmhd_cell records with the
+ * derived scalars filled (kpo_grmhd_derive_scalars). */
+kp_status kpo_run_grmhd_image(const kp_pass_a_params* p, const kp_grmhd_model* g, const kpo_perturbation* pert,
+                              const double* freqs, int32_t nfreq, const double* ctrl, int32_t nctrl,
+                              int32_t pixel_begin, int32_t pixel_end, int32_t nthreads, kp_image_data* out,
+                              kpo_stats* stats) {
+    if (!p || !g || !out || !g->cells) return KP_INVALID_ARGUMENT;
+    if (p->coordinate_system != KP_COORD_FMKS || p->camera.model != KP_CAMERA_PINHOLE) return KP_UNSUPPORTED;
+    const double f0 = g->freq_cgs;
+    if (!freqs) {
+        freqs = &f0;
+        nfreq = 1;
+    }
+    if (!ctrl) {
+        ctrl = freqs;
+        nctrl = nfreq;
+    }
+    if (nfreq < 1 || nfreq > KPO_MAX_FREQUENCIES || nctrl < 1 || nctrl > KPO_MAX_FREQUENCIES)
+        return KP_INVALID_ARGUMENT;
+    int same = nctrl == nfreq;
+    for (int f = 0; same && f < nfreq; ++f) same = freqs[f] == ctrl[f];
+    const freq_plan fp = {freqs, nfreq, ctrl, nctrl, same};
+    const int npix = p->camera.nx * p->camera.ny;
+    const int begin = pixel_begin < 0 ? 0 : pixel_begin;
+    const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
+    const model_ctx mc = {NULL, g};
+    return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads, out,
+                      stats);
 }

@@ -56,6 +56,20 @@ void kpo_bl_transport_rhs_generic(double mass, double spin, const kpo_state* s, 
 double kpo_bl_dot(double mass, double spin, const double x[4],
                   const double u[4], const double v[4]);
 
+/* FMKS coordinate map parameters (src/geometry/kerr_fmks.hpp:11-30). */
+typedef struct kpo_fmks {
+    double startx1, hslope, mks_smooth, poly_alpha, poly_xt, poly_norm;
+} kpo_fmks;
+
+double kpo_fmks_theta(const kpo_fmks* f, double x1, double x2);
+double kpo_fmks_x2_from_theta(const kpo_fmks* f, double x1, double theta);
+void kpo_fmks_gcov(double mass, double spin, const kpo_fmks* f, const double x[4], double g[16]);
+void kpo_fmks_gcon(double mass, double spin, const kpo_fmks* f, const double x[4], double g[16]);
+void kpo_fmks_transport_rhs(double mass, double spin, const kpo_fmks* f, const kpo_state* s, kpo_state* d);
+/* Christoffel contraction from central differences of gcov with step h. */
+void kpo_fmks_transport_rhs_fd(double mass, double spin, const kpo_fmks* f, const kpo_state* s, double h,
+                               kpo_state* d);
+
 /* ---- integrators ----------------------------------------------------- */
 void kpo_bl_rk4_step(double mass, double spin, const kpo_state* s, double h, kpo_state* out);
 /* Returns accepted flag; fills state, mid_state, used_h, next_h, error. */
@@ -120,6 +134,23 @@ kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* params, const kp_ri
                                      const kpo_perturbation* pert, int32_t offset, int32_t stride,
                                      int32_t count, int32_t nthreads, kp_image_data* out,
                                      kpo_stats* stats);
+
+/* ---- FMKS camera and the GRMHD grid model ------------------------------ */
+void kpo_fmks_camera_tetrad(const kp_pass_a_params* params, const kpo_fmks* f, double xcam[4],
+                            double tetrad[16]);
+void kpo_fmks_camera_ray(const kp_pass_a_params* params, const kpo_fmks* f, int32_t pixel, kpo_state* out);
+/* Derived scalars of every cell from its primitives (host cells). */
+void kpo_grmhd_derive_scalars(const kp_grmhd_model* model, kp_grmhd_cell* cells);
+double kpo_grmhd_sample_prim(const kp_grmhd_model* model, const kp_grmhd_cell* cells, int v, double x1,
+                             double x2, double phi);
+double kpo_grmhd_dlambda_scale(const kp_grmhd_model* model);
+void kpo_grmhd_coefficients(const kp_grmhd_model* model, const kpo_state* s, const kpo_perturbation* pert,
+                            int faraday_rotation, kpo_coeffs* out);
+/* GRMHD image (FMKS); freqs/ctrl NULL: the model's freq_cgs.  model->cells: host. */
+kp_status kpo_run_grmhd_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                              const kpo_perturbation* pert, const double* freqs, int32_t nfreq,
+                              const double* ctrl, int32_t nctrl, int32_t pixel_begin, int32_t pixel_end,
+                              int32_t nthreads, kp_image_data* out, kpo_stats* stats);
 
 #ifdef __cplusplus
 }

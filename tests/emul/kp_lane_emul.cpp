@@ -10,16 +10,17 @@
 template <bool MIXED, int MODE>
 static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, double fp64_radius,
                           const FreqLists& fl, kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps,
-                          const double* log_scales = nullptr, double* grad = nullptr) {
+                          const double* log_scales = nullptr, double* grad = nullptr,
+                          const kp_grmhd_model* gm = nullptr) {
     Params P;
-    kp_status st = prepare(pa, m, P);
+    kp_status st = MODE == M_GRMHD ? prepare_grmhd(pa, gm, P) : prepare(pa, m, P);
     if (st != KP_OK) return st;
     if (MIXED) {
         prepare_mixed(P);
         P.fp64_radius = fp64_radius;
     }
-    if (MODE == M_MF) {
-        st = prepare_frequencies(*m, fl, P);
+    if (lds_stokes<MODE>()) {
+        st = prepare_frequencies(MODE == M_GRMHD ? gm->mbh_solar : m->mbh_solar, fl, P);
         if (st != KP_OK) return st;
     }
     if (MODE == M_TANGENT) {
@@ -40,10 +41,10 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
     int64_t total = 0, guard_total = 0;
     for (int pix = 0; pix < np; ++pix) {
         uint32_t rc = 0, sa = 0, c1 = 0, c2 = 0, sb = 0, guard = 0;
-        Geo<double> G{P.mass, P.spin, &rc};
+        const auto G = GeomT<geom_of<MODE>()>::geo(P, &rc);
         LaneA A;
         lane_a_init(P, pix, A);
-        while (!pass_a_tick<MIXED>(P, G, A, sa, guard)) {
+        while (!pass_a_tick<MIXED, geom_of<MODE>()>(P, G, A, sa, guard)) {
         }
         lane_a_store(P, A);
         LaneB B;
@@ -96,4 +97,42 @@ extern "C" void kp_emul_sincos(double x, double* s, double* c) { kpd::sincos_bou
 extern "C" kp_status kp_emul_run_riaf_image_gradient(const kp_pass_a_params* pa, const kp_riaf_model* m,
                                                      const double* log_scales, kp_image_data* out, double* grad) {
     return emul_run<false, M_TANGENT>(pa, m, 0.0, FreqLists{}, out, nullptr, nullptr, log_scales, grad);
+}
+
+// GRMHD lanes on FMKS geodesics; model->cells are HOST records here.
+extern "C" kp_status kp_emul_run_grmhd_image(const kp_pass_a_params* pa, const kp_grmhd_model* gm,
+                                             const double* freqs, int32_t nfreq, const double* ctrl, int32_t nctrl,
+                                             kp_image_data* out, int64_t* rhs_evals) {
+    FreqLists fl;
+    fl.nfreq = nfreq;
+    fl.freqs = freqs;
+    fl.nctrl = nctrl;
+    fl.ctrl = ctrl;
+    return emul_run<false, M_GRMHD>(pa, nullptr, 0.0, fl, out, rhs_evals, nullptr, nullptr, nullptr, gm);
+}
+
+// The device derived-scalar formulas run on host cells.
+extern "C" kp_status kp_emul_grmhd_derive_scalars(const kp_grmhd_model* gm, kp_grmhd_cell* cells) {
+    kp_pass_a_params p;
+    kp_pass_a_params_default(&p);
+    p.coordinate_system = KP_COORD_FMKS;
+    p.camera.nx = p.camera.ny = 1;
+    p.camera.radius = 100.0;
+    kp_grmhd_model g = *gm;
+    g.cells = cells;
+    Params P;
+    const kp_status st = prepare_grmhd(&p, &g, P);
+    if (st != KP_OK) return st;
+    for (int i = 0; i < g.n1; ++i)
+        for (int j = 0; j < g.n2; ++j)
+            for (int k = 0; k < g.n3; ++k) grmhd_derive_cell(P.grmhd, i, j, cells[((size_t)i * g.n2 + j) * g.n3 + k]);
+    return KP_OK;
+}
+
+// FMKS transport RHS of the device code, for its parity check.
+extern "C" void kp_emul_fmks_rhs(double mass, double spin, const double fm[6], const double* s, double* d) {
+    const FmksMap f{fm[0], fm[1], fm[2], fm[3], fm[4], fm[5]};
+    St<double> y;
+    for (int i = 0; i < 16; ++i) y.v[i] = s[i];
+    fmks_rhs_apply(mass, spin, f, y, [&](int i, double v) { d[i] = v; });
 }
