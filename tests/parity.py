@@ -129,11 +129,19 @@ def mixed_errors(got, ref):
     }
 
 
-def assert_mixed(got, ref, label):
+def assert_mixed(got, ref, label, pol="relative"):
+    """Mixed-precision image bars.  pol="relative" holds the image-integrated
+    LP and CP fractions to MIXED_POL_FRAC of their FP64 values (full-size
+    images); pol="absolute" holds them to MIXED_POL_FRAC in the fraction
+    itself, for coarse images whose net polarisation is a near-cancelling sum
+    over a few dozen pixels, where one photon-ring pixel moves the relative
+    error of a 0.4 % LP fraction by a percent."""
     e = mixed_errors(got, ref)
     assert e["total_i_rel"] <= MIXED_TOTAL_I, (label, e)
-    assert abs(e["lp"][1] - e["lp"][0]) <= MIXED_POL_FRAC * abs(e["lp"][0]) + MIXED_POL_FLOOR, (label, e)
-    assert abs(e["cp"][1] - e["cp"][0]) <= MIXED_POL_FRAC * abs(e["cp"][0]) + MIXED_POL_FLOOR, (label, e)
+    for key in ("lp", "cp"):
+        f0, f1 = e[key]
+        bar = MIXED_POL_FRAC * abs(f0) + MIXED_POL_FLOOR if pol == "relative" else MIXED_POL_FRAC
+        assert abs(f1 - f0) <= bar, (label, key, e)
     assert e["pixel_max_rel_peak"] <= MIXED_PIXEL, (label, e)
     assert e["reason_mismatch"] <= MIXED_REASONS * e["pixels"], (label, e)
     for n in STOKES:

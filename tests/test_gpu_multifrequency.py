@@ -66,8 +66,18 @@ def test_gpu_multifrequency_separate_control_and_single_list():
     one, _ = _run(p, m, [m.freq_cgs])
     plain, _ = image.run_riaf_image(p, m)
     plain = {k: v.cpu().numpy() for k, v in plain.items()}
+    # one output frequency runs the multi-frequency kernel, whose coefficient
+    # code is split differently from the single-frequency kernel's, so the
+    # compiler may fuse multiply-adds differently: identical step sequences
+    # and termination, Stokes equal to rounding
     for k in plain:
-        assert np.array_equal(one[k], plain[k]), k
+        if k.startswith("image_"):
+            peak = max(float(np.max(np.abs(plain["image_i"]))), 1e-300)
+            assert np.max(np.abs(one[k] - plain[k])) <= 1e-12 * peak, k
+        elif k in ("reason", "steps", "pass_a_steps"):
+            assert np.array_equal(one[k], plain[k]), k
+        else:
+            assert np.allclose(one[k], plain[k], rtol=1e-12, atol=1e-12), k
 
 
 def test_gpu_multifrequency_mixed():
