@@ -37,7 +37,12 @@ typedef enum kp_status {
     KP_INVALID_ARGUMENT = 1,
     KP_UNSUPPORTED = 2,
     KP_HIP_ERROR = 3,
-    KP_OUT_OF_MEMORY = 4
+    KP_OUT_OF_MEMORY = 4,
+    /* slow light, decoupled stepping: a radiating sample lies outside the
+     * snapshot times (the reference's "radiation sample outside supplied
+     * physical time support", src/image/slow_light_batch_driver.hpp:286-289);
+     * the image is written, its waiting rays with reason 6 */
+    KP_TIME_SUPPORT = 5
 } kp_status;
 
 /* Per-pixel termination codes, same values and meanings as
@@ -246,7 +251,12 @@ typedef struct kp_run_options {
     int32_t block_size;       /* rays per workgroup; 0 = tuned default */
     int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread */
     int32_t waves_per_cu;     /* persistent grid size; 0 = occupancy-derived */
-    int32_t reserved0;
+    int32_t guard_mode;       /* mixed, persistent split kernels: 0 = staged
+                                 launches (FP32-geodesic and FP64-geodesic
+                                 kernels; a ray moves to the FP64 kernel when it
+                                 reaches r < fp64_radius and back at
+                                 r >= 1.25 fp64_radius); 1 = one kernel whose
+                                 lanes choose per attempt */
     double fp64_radius;       /* mixed: attempts starting at r < fp64_radius run in FP64 */
     void* stream;             /* hipStream_t, NULL = default stream */
 } kp_run_options;
@@ -407,6 +417,50 @@ kp_status kp_run_riaf_batch(const kp_batch_job* jobs, int32_t njobs, const kp_ru
 kp_status kp_run_grmhd_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
                              int32_t radiation_substeps, const kp_run_options* options,
                              kp_image_data* out, kp_image_stats* stats);
+
+/* Slow light: a time sequence of GRMHD snapshots on one grid, sampled at
+ * retarded time t = observation_time + x^0 (x^0 = 0 at the camera,
+ * docs/slow_light.md:9-15).  Mirrors the arguments of
+ * run_slow_light_image_metric (src/image/slow_light_driver.hpp:1940-1954) and
+ * the residency options of run_slow_light_batch (slow_light_batch_driver.hpp),
+ * with every snapshot resident in device memory. */
+typedef enum kp_slow_light_step_mode {
+    /* kpolaris::SlowLightStepMode (src/model/time_interpolation.hpp:9-12) */
+    KP_SLOW_LIGHT_DECOUPLED = 0,  /* steps never cut at snapshot times; the default */
+    KP_SLOW_LIGHT_SNAPSHOT = 1,   /* pair-by-pair windows, steps cut at each snapshot time */
+    KP_SLOW_LIGHT_BLOCK = 2       /* steps cut at the last snapshot time only */
+} kp_slow_light_step_mode;
+
+typedef enum kp_slow_light_interpolation {
+    /* kpolaris::SlowLightInterpolation (src/model/time_interpolation.hpp:19) */
+    KP_SLOW_LIGHT_COEFFICIENTS = 0,
+    KP_SLOW_LIGHT_FLUID = 1       /* the default */
+} kp_slow_light_interpolation;
+
+typedef struct kp_slow_light {
+    int32_t nsnap;                /* snapshots, at least 2 */
+    int32_t step_mode;            /* kp_slow_light_step_mode */
+    int32_t interpolation;        /* kp_slow_light_interpolation */
+    int32_t reserved0;
+    double observation_time;      /* coordinate arrival time at the camera */
+    const double* times;          /* [nsnap] strictly increasing, HOST */
+    const void* const* cells;     /* [nsnap] DEVICE kp_grmhd_cell arrays (HOST array of pointers) */
+} kp_slow_light;
+
+/* Slow-light GRMHD image (nfreq frequencies on shared geodesics, as
+ * kp_run_grmhd_multifrequency_image).  `model` gives the grid, units and
+ * emission settings (its cells pointer is not used).  Replaces
+ * run_slow_light_image_metric (src/image/slow_light_driver.hpp:1940-2028:
+ * pass A endpoints, the window kernel over every snapshot pair, the vacuum
+ * finish of unfinished rays) and, with decoupled stepping, the
+ * all-resident case of run_slow_light_batch (slow_light_batch_driver.hpp).
+ * Returns KP_TIME_SUPPORT (image written) when decoupled stepping needs a
+ * time outside [times[0], times[nsnap-1]]. */
+kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                        const kp_slow_light* sequence, const double* frequencies,
+                                        int32_t nfreq, const kp_run_options* options, kp_image_data* out,
+                                        kp_image_stats* stats);
+void kp_slow_light_default(kp_slow_light* sequence);
 
 /* Multi-frequency GRMHD image on shared geodesics; replaces
  * run_grmhd_multifrequency_image (src/image/grmhd.hpp:31-38).  Arrays and

@@ -14,6 +14,7 @@ KP_INVALID_ARGUMENT = 1
 KP_UNSUPPORTED = 2
 KP_HIP_ERROR = 3
 KP_OUT_OF_MEMORY = 4
+KP_TIME_SUPPORT = 5
 
 STATUS_NAMES = {
     KP_OK: "KP_OK",
@@ -21,7 +22,15 @@ STATUS_NAMES = {
     KP_UNSUPPORTED: "KP_UNSUPPORTED",
     KP_HIP_ERROR: "KP_HIP_ERROR",
     KP_OUT_OF_MEMORY: "KP_OUT_OF_MEMORY",
+    KP_TIME_SUPPORT: "KP_TIME_SUPPORT",
 }
+
+# SlowLightStepMode / SlowLightInterpolation (src/model/time_interpolation.hpp:9-25)
+SLOW_LIGHT_DECOUPLED = 0
+SLOW_LIGHT_SNAPSHOT = 1
+SLOW_LIGHT_BLOCK = 2
+SLOW_LIGHT_COEFFICIENTS = 0
+SLOW_LIGHT_FLUID = 1
 
 # TerminationReason (src/common/types.hpp:37-48)
 REASON_NONE = 0
@@ -157,6 +166,15 @@ class GRMHDModel(C.Structure):
     ]
 
 
+class SlowLight(C.Structure):
+    _fields_ = [
+        ("nsnap", C.c_int32), ("step_mode", C.c_int32), ("interpolation", C.c_int32), ("reserved0", C.c_int32),
+        ("observation_time", C.c_double),
+        ("times", C.POINTER(C.c_double)),
+        ("cells", C.POINTER(C.c_void_p)),
+    ]
+
+
 class BatchJob(C.Structure):
     _fields_ = [
         ("params", PassAParams), ("model", RIAFModel),
@@ -169,7 +187,7 @@ class RunOptions(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("split_transport", C.c_int32),
         ("block_size", C.c_int32), ("persistent", C.c_int32),
-        ("waves_per_cu", C.c_int32), ("reserved0", C.c_int32),
+        ("waves_per_cu", C.c_int32), ("guard_mode", C.c_int32),
         ("fp64_radius", C.c_double),
         ("stream", C.c_void_p),
     ]

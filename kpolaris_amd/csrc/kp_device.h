@@ -47,6 +47,16 @@ KP_DEV float krcp(float x) {
 }
 KP_DEV float kdiv(float a, float b) { return a * krcp(b); }
 
+// Counter increment shared by the kernels and their host build (the CPU
+// emulation of the lane logic runs one lane at a time).
+KP_DEV void count_add(unsigned long long* p, unsigned long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAdd(p, v);
+#else
+    *p += v;
+#endif
+}
+
 // Sine and cosine for the bounded arguments of the transport equations
 // (|x| < 1e5; the Boyer-Lindquist polar angle stays within a few pi).
 // Cody-Waite reduction by pi/2 with a three-part split of pi/2 (the first two

@@ -175,38 +175,46 @@ KP_DEV double grmhd_sigma_factor(const GrmhdDev& G, double sigma) {
     return cos(PI_D / 2.0 / dsig * (sigma - G.sigma_cut));
 }
 
-// The frequency-independent part of GRMHDGridRadiationModel::coefficients at
-// one FMKS sample state (grmhd_grid.hpp:883-1080 and 1091-1150, 1210-1240),
-// in the RiafLocal form shared with the per-frequency evaluation.
-KP_DEV void grmhd_local(const GrmhdDev& G, const St<double>& s, RiafLocal<double>& L) {
-    L.valid = 0;
-    L.aligned = 0;
-    L.has_chi = 0;
-    const double x1 = s.v[1], x2 = s.v[2], phi = s.v[3];
-    const double r = exp(x1);
-    const double th = fmks_theta(G.map, x1, x2).th;
-    if (!(r >= G.r_in && r < G.r_out)) return;
-    int i0, j0, k0;
-    double di, dj, dk;
-    if (!grmhd_indices(G, x1, x2, phi, i0, j0, k0, di, dj, dk)) return;
+// Trilinear samples of the 8 primitives and 5 derived scalars of one cell
+// array (interp_prim_stencil / interp_derived_stencil, grmhd_grid.hpp:379-430;
+// the reference's stencil order n = di-bit + 2 dj-bit + 4 dk-bit).
+KP_DEV void grmhd_sample(const GrmhdDev& G, const kp_grmhd_cell* cells, int i0, int j0, int k0, double di,
+                         double dj, double dk, double prim[8], double der[5]) {
     const double wi[2] = {1.0 - di, di}, wj[2] = {1.0 - dj, dj}, wk[2] = {1.0 - dk, dk};
-    double prim[8], der[5];
 #pragma unroll
     for (int v = 0; v < 8; ++v) prim[v] = 0.0;
 #pragma unroll
     for (int v = 0; v < 5; ++v) der[v] = 0.0;
-    // the reference's stencil order: n = di-bit + 2 dj-bit + 4 dk-bit
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         const int bi = n & 1, bj = (n >> 1) & 1, bk = (n >> 2) & 1;
         const double w = wi[bi] * wj[bj] * wk[bk];
-        const kp_grmhd_cell& c = G.cells[grmhd_cell(G, i0 + bi, j0 + bj, k0 + bk)];
+        const kp_grmhd_cell& c = cells[grmhd_cell(G, i0 + bi, j0 + bj, k0 + bk)];
 #pragma unroll
         for (int v = 0; v < 8; ++v) prim[v] += w * double(c.prim[v]);
 #pragma unroll
         for (int v = 0; v < 5; ++v) der[v] += w * c.derived[v];
     }
+}
+
+// interpolate_time_value (src/model/time_interpolation.hpp:27-32).
+KP_DEV double time_value(double a, double b, double f) {
+    if (f <= 0.0 || a == b) return a;
+    if (f >= 1.0) return b;
+    return (1.0 - f) * a + f * b;
+}
+
+// The rest of the frequency-independent part of
+// GRMHDGridRadiationModel::coefficients from sampled primitives and derived
+// scalars (grmhd_grid.hpp:953-1080 and 1091-1150, 1210-1240), in the
+// RiafLocal form shared with the per-frequency evaluation.
+KP_DEV void grmhd_local_from(const GrmhdDev& G, const St<double>& s, double r, double th, const double prim[8],
+                             const double der[5], RiafLocal<double>& L) {
+    L.valid = 0;
+    L.aligned = 0;
+    L.has_chi = 0;
     if (!(prim[0] > 0.0 && prim[1] > 0.0)) return;
+    const double x1 = s.v[1], x2 = s.v[2];
     Metric4<double> g;
     double gcon0[4];
     grmhd_native_metric(G, x1, x2, r, th, g, gcon0);
@@ -260,6 +268,59 @@ KP_DEV void grmhd_local(const GrmhdDev& G, const St<double>& s, RiafLocal<double
         L.cos2chi = (b2 * b2 - b1 * b1) / bp2;
         L.sin2chi = -2.0 * b1 * b2 / bp2;
     }
+}
+
+// The frequency-independent part of GRMHDGridRadiationModel::coefficients at
+// one FMKS sample state (grmhd_grid.hpp:883-1080 and 1091-1150, 1210-1240).
+KP_DEV void grmhd_local(const GrmhdDev& G, const St<double>& s, RiafLocal<double>& L) {
+    L.valid = 0;
+    L.aligned = 0;
+    L.has_chi = 0;
+    const double x1 = s.v[1], x2 = s.v[2], phi = s.v[3];
+    const double r = exp(x1);
+    const double th = fmks_theta(G.map, x1, x2).th;
+    if (!(r >= G.r_in && r < G.r_out)) return;
+    int i0, j0, k0;
+    double di, dj, dk;
+    if (!grmhd_indices(G, x1, x2, phi, i0, j0, k0, di, dj, dk)) return;
+    double prim[8], der[5];
+    grmhd_sample(G, G.cells, i0, j0, k0, di, dj, dk, prim, der);
+    grmhd_local_from(G, s, r, th, prim, der, L);
+}
+
+// Fluid time interpolation between two snapshots of one grid: both are
+// sampled at the state, every primitive and derived scalar is interpolated
+// in time (fraction f), then the four-vectors are rebuilt
+// (GRMHDGridRadiationModel::fluid_state with time_upper, grmhd_grid.hpp:940-962;
+// SlowLightTemporalModel::coefficients, slow_light_driver.hpp:166-173).
+// The upper snapshot is not read at f <= 0, nor the lower at f >= 1.
+KP_DEV void grmhd_local_time(const GrmhdDev& G, const kp_grmhd_cell* lo, const kp_grmhd_cell* hi, double f,
+                             const St<double>& s, RiafLocal<double>& L) {
+    L.valid = 0;
+    L.aligned = 0;
+    L.has_chi = 0;
+    const double x1 = s.v[1], x2 = s.v[2], phi = s.v[3];
+    const double r = exp(x1);
+    const double th = fmks_theta(G.map, x1, x2).th;
+    if (!(r >= G.r_in && r < G.r_out)) return;
+    int i0, j0, k0;
+    double di, dj, dk;
+    if (!grmhd_indices(G, x1, x2, phi, i0, j0, k0, di, dj, dk)) return;
+    double prim[8], der[5];
+    if (f >= 1.0) {
+        grmhd_sample(G, hi, i0, j0, k0, di, dj, dk, prim, der);
+    } else {
+        grmhd_sample(G, lo, i0, j0, k0, di, dj, dk, prim, der);
+        if (f > 0.0) {
+            double p1[8], d1[5];
+            grmhd_sample(G, hi, i0, j0, k0, di, dj, dk, p1, d1);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) prim[v] = time_value(prim[v], p1[v], f);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) der[v] = time_value(der[v], d1[v], f);
+        }
+    }
+    grmhd_local_from(G, s, r, th, prim, der, L);
 }
 
 // Coefficients at observing frequency freq from the local plasma state

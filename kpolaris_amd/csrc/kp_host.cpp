@@ -471,6 +471,15 @@ extern "C" void kp_grmhd_model_default(kp_grmhd_model* m) {
     m->cells = nullptr;
 }
 
+// SlowLightResidentWindows / run_slow_light_batch defaults: decoupled
+// stepping and fluid interpolation (src/model/time_interpolation.hpp:13,23).
+extern "C" void kp_slow_light_default(kp_slow_light* q) {
+    if (!q) return;
+    std::memset(q, 0, sizeof(*q));
+    q->step_mode = KP_SLOW_LIGHT_DECOUPLED;
+    q->interpolation = KP_SLOW_LIGHT_FLUID;
+}
+
 extern "C" void kp_run_options_default(kp_run_options* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));

@@ -42,6 +42,10 @@ using namespace kpd;
 #ifndef KP_WAVES_PER_EU
 #define KP_WAVES_PER_EU 1
 #endif
+// The same for the FP32-geodesic kernels of a staged mixed-precision image.
+#ifndef KP_WAVES_PER_EU_F32
+#define KP_WAVES_PER_EU_F32 1
+#endif
 // 1: each lane's state and k1 scratch live in LDS ([32][blockDim] doubles per
 // block) instead of registers, trading LDS traffic for register headroom.
 #ifndef KP_LANE_LDS
@@ -62,20 +66,30 @@ enum : int {
     R_INNER = 2,
     R_ESCAPED = 3,
     R_MAX_STEPS = 4,
+    R_SLOW_TIME = 6,
     R_UNDERFLOW = 7,
     R_TIME = 8,
 };
 
-// Pass B variants: one frequency, several frequencies on shared geodesics, or
+// Pass B variants: one frequency, several frequencies on shared geodesics,
 // one frequency with the tangent (plasma-parameter derivatives) of its Stokes
-// vector carried alongside.
-enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3 };
+// vector carried alongside, a GRMHD grid, or a GRMHD snapshot sequence
+// sampled at retarded time (slow light).
+enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4 };
 
 // Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
 // (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
 enum Geom : int { G_BL = 0, G_FMKS = 1 };
-template <int MODE> constexpr int geom_of() { return MODE == M_GRMHD ? G_FMKS : G_BL; }
-template <int MODE> constexpr bool lds_stokes() { return MODE == M_MF || MODE == M_GRMHD; }
+// Precision of the geodesic attempts (the radiation precision is separate).
+enum GeoPrec : int { GP_64 = 0, GP_GUARD = 1, GP_32 = 2 };
+// Parking rule of a staged mixed-precision launch (Params::park).
+enum Park : int { PARK_NONE = 0, PARK_INSIDE = 1, PARK_OUTSIDE = 2 };
+constexpr int KP_NQUEUE = 32;  // queue and list counters in the workspace
+// A staged FP64 launch hands a ray back to FP32 at this multiple of the
+// guard radius (hysteresis: no ray alternates stages on every step).
+constexpr double KP_BACK_RADIUS_FACTOR = 1.25;
+template <int MODE> constexpr int geom_of() { return (MODE == M_GRMHD || MODE == M_SLOW) ? G_FMKS : G_BL; }
+template <int MODE> constexpr bool lds_stokes() { return MODE == M_MF || MODE == M_GRMHD || MODE == M_SLOW; }
 
 enum Counter : int {
     C_RHS_A = 0,
@@ -85,6 +99,7 @@ enum Counter : int {
     C_STEPS_A = 4,
     C_STEPS_B = 5,
     C_GUARD = 6,
+    C_WAIT = 7,  // slow light: rays that needed a time outside the snapshots
     C_COUNT = 8
 };
 
@@ -123,6 +138,30 @@ struct Params {
     int32_t* ep_reason;
     uint32_t* queue;
     unsigned long long* counters;
+    // persistent launches: the queue counter this launch fetches with and its
+    // work source (list_in null: every pixel; else the count_in pixels a
+    // previous stage parked).  A staged mixed-precision image (stage >= 0)
+    // parks a lane between steps when its radius crosses the rule's radius
+    // (PARK_INSIDE: r < fp64_radius; PARK_OUTSIDE: r >= back_radius), saving
+    // its state to ep and pk_*, and appends its pixel to list_out.
+    int32_t stage, park;
+    double back_radius;
+    uint32_t* fetch;
+    const uint32_t* list_in;
+    const uint32_t* count_in;
+    uint32_t* list_out;
+    uint32_t* count_out;
+    double* pk_d;     // pass A: [npix] h_current; pass B: [2][npix] h_current, cap
+    int32_t* pk_i;    // pass A: [4][npix] n, steps, steps_inside, entered; pass B: [2][npix] steps, reason
+    double* pk_st;    // pass B: [4 nfreq][npix] Stokes
+    // slow light: snapshot cell arrays and times (device), the stepping and
+    // interpolation rules (kp_slow_light), the observation time, and per pixel
+    // the time a waiting ray needed (NaN for the others)
+    const kp_grmhd_cell* const* snaps;
+    const double* snap_t;
+    int32_t nsnap, slow_mode, slow_interp, reserved_slow;
+    double t_obs;
+    double* req_t;
     // outputs
     kp_image_data out;
 };
@@ -336,34 +375,34 @@ KP_DEV void locate_crossing(const GG& G, const St<double>& first, const St<doubl
     }
 }
 
-// One integration attempt from the lane state: FP64, or with FP32 derivatives
-// outside the guard radius when MIXED.  guard counts the FP64 attempts a
-// mixed lane takes.
-template <bool MIXED, class GG>
+// One integration attempt from the lane state.  GP_64: FP64 derivatives.
+// GP_32: FP32 derivatives and stage sums, increments added to the FP64
+// state.  GP_GUARD: per lane, FP64 inside the guard radius and FP32 outside
+// (the lanes of a wave may take both paths).  guard counts the FP64 attempts
+// of a mixed-precision image: those of GP_GUARD lanes inside the radius and
+// every attempt of a staged FP64 launch (P.stage >= 0).
+template <int GP, class GG>
 KP_DEV double attempt(const Params& P, const GG& G, const LaneVec& s, LaneVec& k1, double h,
                       St<double>& half, St<double>& next, uint32_t& guard) {
-    if (MIXED) {
-        if (!(s[1] < P.fp64_radius)) return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count);
-        ++guard;
-    }
+    if (GP == GP_32 || (GP == GP_GUARD && !(s[1] < P.fp64_radius)))
+        return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count);
+    if (GP == GP_GUARD || P.stage >= 0) ++guard;
     return doubling_attempt(G, s, h, k1, half, next);
 }
 
-template <bool MIXED, class GG>
+template <int GP, class GG>
 KP_DEV void step_rk4(const Params& P, const GG& G, const LaneVec& s, double h, St<double>& next,
                      uint32_t& guard) {
-    if (MIXED) {
-        if (!(s[1] < P.fp64_radius)) {
-            rk4_mixed(P.mass_f, P.spin_f, s, h, next, G.rhs_count);
-            return;
-        }
-        ++guard;
+    if (GP == GP_32 || (GP == GP_GUARD && !(s[1] < P.fp64_radius))) {
+        rk4_mixed(P.mass_f, P.spin_f, s, h, next, G.rhs_count);
+        return;
     }
+    if (GP == GP_GUARD || P.stage >= 0) ++guard;
     rk4(G, s, h, next);
 }
 
 // One tick of pass A; returns true when the ray has terminated (L.reason set).
-template <bool MIXED, int GEOM>
+template <int GP, int GEOM>
 KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G, LaneA& L, uint32_t& steps_acc,
                         uint32_t& guard) {
     using GT = GeomT<GEOM>;
@@ -404,7 +443,7 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
     if (P.adaptive) {
         // adaptive_rk4_step (src/geodesic/rk4.hpp:109-161), one attempt per tick
         St<double> half;
-        const double err = attempt<MIXED>(P, G, L.s, L.k1, L.h_try, half, next, guard);
+        const double err = attempt<GP>(P, G, L.s, L.k1, L.h_try, half, next, guard);
         const double tol = maxv(P.tol, 8.0 * eps<double>());
         const double req_sign = L.h_current < 0.0 ? -1.0 : 1.0;
         if (err <= tol) {
@@ -436,7 +475,7 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
         }
     } else {
         used_h = L.h_current;
-        step_rk4<MIXED>(P, G, L.s, used_h, next, guard);
+        step_rk4<GP>(P, G, L.s, used_h, next, guard);
     }
     const double old_r = GT::radius(L.s[1]);
     const double next_r = GT::radius(next.v[1]);
@@ -475,6 +514,55 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
     return false;
 }
 
+// Parking (staged mixed precision): between steps a lane's state moves to
+// the endpoint buffer and the park arrays, and a later stage resumes it.
+KP_DEV bool park_due(const Params& P, double r) {
+    return (P.park == PARK_INSIDE && r < P.fp64_radius) || (P.park == PARK_OUTSIDE && r >= P.back_radius);
+}
+
+// Appends the pixels of the lanes that park, one atomic per wave; called by
+// every lane of the wave.
+KP_DEV void append_parked(const Params& P, bool park, int pixel) {
+    const unsigned long long mask = __ballot(park);
+    if (mask == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(P.count_out, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (park) P.list_out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)pixel;
+}
+
+KP_DEV uint32_t work_count(const Params& P) { return P.list_in ? *P.count_in : (uint32_t)P.npix; }
+
+KP_DEV void lane_a_park(const Params& P, const LaneA& L) {
+    const size_t np = (size_t)P.npix, p = (size_t)L.pixel;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + p] = L.s[c];
+    P.pk_d[p] = L.h_current;
+    P.pk_i[p] = L.n;
+    P.pk_i[np + p] = L.steps;
+    P.pk_i[2 * np + p] = L.steps_inside;
+    P.pk_i[3 * np + p] = L.entered;
+}
+
+KP_DEV void lane_a_resume(const Params& P, int pixel, LaneA& L) {
+    const size_t np = (size_t)P.npix, p = (size_t)pixel;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) vset(L.s, c, P.ep[(size_t)c * np + p]);
+    L.h_current = P.pk_d[p];
+    L.n = P.pk_i[p];
+    L.steps = P.pk_i[np + p];
+    L.steps_inside = P.pk_i[2 * np + p];
+    L.entered = P.pk_i[3 * np + p];
+    L.h_try = 0.0;
+    L.call_min = 0.0;
+    L.att = 0;
+    L.in_call = 0;
+    L.pixel = pixel;
+    L.reason = R_MAX_STEPS;
+}
+
 KP_DEV void lane_a_store(const Params& P, const LaneA& L) {
     const size_t np = (size_t)P.npix;
 #pragma unroll
@@ -494,28 +582,40 @@ KP_DEV void lane_lds(LdsVec<double>& s, LdsVec<double>& k1) {
 }
 KP_DEV void lane_lds(St<double>&, St<double>&) {}
 
-template <bool PERSISTENT, bool MIXED, int GEOM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_a_kernel(Params P) {
+template <bool PERSISTENT, int GP, int GEOM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32 ? KP_WAVES_PER_EU_F32 : KP_WAVES_PER_EU))) void pass_a_kernel(Params P) {
     uint32_t rhs_count = 0, steps_a = 0, guard = 0;
     const auto G = GeomT<GEOM>::geo(P, &rhs_count);
     LaneA L;
     lane_lds(L.s, L.k1);
     bool has = false;
     if (PERSISTENT) {
+        const uint32_t limit = work_count(P);
         bool exhausted = false;
         while (true) {
-            int pix = 0;
+            int idx = 0;
             const bool need = !has && !exhausted;
             if (__ballot(need)) {
-                if (fetch_pixel(&P.queue[0], need, (uint32_t)P.npix, pix)) {
-                    lane_a_init(P, pix, L);
+                if (fetch_pixel(P.fetch, need, limit, idx)) {
+                    if (P.list_in)
+                        lane_a_resume(P, (int)P.list_in[idx], L);
+                    else
+                        lane_a_init(P, idx, L);
                     has = true;
                 } else if (need) {
                     exhausted = true;
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_a_tick<MIXED, GEOM>(P, G, L, steps_a, guard)) {
+            if (P.park != PARK_NONE) {
+                const bool park = has && !L.in_call && park_due(P, GeomT<GEOM>::radius(L.s[1]));
+                if (park) {
+                    lane_a_park(P, L);
+                    has = false;
+                }
+                append_parked(P, park, L.pixel);
+            }
+            if (has && pass_a_tick<GP, GEOM>(P, G, L, steps_a, guard)) {
                 lane_a_store(P, L);
                 has = false;
             }
@@ -524,7 +624,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
         if (pix < P.npix) {
             lane_a_init(P, pix, L);
-            while (!pass_a_tick<MIXED, GEOM>(P, G, L, steps_a, guard)) {
+            while (!pass_a_tick<GP, GEOM>(P, G, L, steps_a, guard)) {
             }
             lane_a_store(P, L);
         }
@@ -535,7 +635,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
 // ---------------------------------------------------------------------------
 // Pass B: trace_pass_b_segment_model_endpoint_pixel_metric (src/geodesic/pass_b.hpp:515-722).
 // ---------------------------------------------------------------------------
-enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2 };
+enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2, PH_TIME = 3 };
 
 struct LaneB {
     LaneVec s, k1;  // the lane's state and its k1 scratch
@@ -543,7 +643,9 @@ struct LaneB {
     LdsVec<double> mf;  // multi-frequency: Stokes component 4 f + c of this lane, in LDS
     Stokes<D3> sd;      // tangent mode: Stokes vector with its three parameter derivatives
     double h_current, cap, h, h_try, call_min, call_max;
+    double req;     // slow light: the time a waiting ray needed (NaN otherwise)
     int steps, reason, att, phase, crosses, reached, pixel, pass_a_steps;
+    int win, vac;   // slow light: snapshot window; vacuum finish (1: entering, 2: stepping)
 };
 
 // transform_to_observer_basis (src/frame/stokes_rotation.hpp:39-57).
@@ -579,7 +681,7 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
         o = observer_basis(in, r11, r12, r21, r22);
         if (L.reached)
             reason = R_CAMERA;
-        else if (reason != R_UNDERFLOW && reason != R_TIME)
+        else if (reason != R_UNDERFLOW && reason != R_TIME && reason != R_SLOW_TIME)
             reason = R_MAX_STEPS;
     }
     const kp_image_data& O = P.out;
@@ -643,6 +745,10 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
     if (O.pass_a_steps) O.pass_a_steps[p] = L.pass_a_steps;
     if (O.steps) O.steps[p] = valid ? L.steps : 0;
     if (O.reason) O.reason[p] = reason;
+    if constexpr (MODE == M_SLOW) {
+        P.req_t[p] = L.req;
+        if (L.req == L.req) count_add(&P.counters[C_WAIT], 1ull);
+    }
 }
 
 // Loads a pass A endpoint; returns false (after writing the pixel's outputs)
@@ -669,11 +775,60 @@ template <int MODE> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L
     L.phase = PH_IDLE;
     L.crosses = 0;
     L.reached = 0;
+    L.win = 0;
+    L.vac = 0;
+    L.req = __builtin_nan("");
     if (!P.ep_valid[pixel]) {
         lane_b_finish<false, MODE>(P, L, 0);
         return false;
     }
     return true;
+}
+
+template <int MODE> KP_DEV void lane_b_park(const Params& P, const LaneB& L) {
+    const size_t np = (size_t)P.npix, p = (size_t)L.pixel;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + p] = L.s[c];
+    P.pk_d[p] = L.h_current;
+    P.pk_d[np + p] = L.cap;
+    P.pk_i[p] = L.steps;
+    P.pk_i[np + p] = L.reason;
+    if (lds_stokes<MODE>()) {
+        for (int j = 0; j < 4 * P.nfreq; ++j) P.pk_st[(size_t)j * np + p] = L.mf[j];
+    } else {
+        P.pk_st[p] = L.st.I;
+        P.pk_st[np + p] = L.st.Q;
+        P.pk_st[2 * np + p] = L.st.U;
+        P.pk_st[3 * np + p] = L.st.V;
+    }
+}
+
+template <int MODE> KP_DEV void lane_b_resume(const Params& P, int pixel, LaneB& L) {
+    const size_t np = (size_t)P.npix, p = (size_t)pixel;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) vset(L.s, c, P.ep[(size_t)c * np + p]);
+    L.pixel = pixel;
+    L.pass_a_steps = P.ep_steps[p];
+    L.h_current = P.pk_d[p];
+    L.cap = P.pk_d[np + p];
+    L.steps = P.pk_i[p];
+    L.reason = P.pk_i[np + p];
+    if (lds_stokes<MODE>()) {
+        for (int j = 0; j < 4 * P.nfreq; ++j) L.mf.set(j, P.pk_st[(size_t)j * np + p]);
+    } else {
+        L.st = {P.pk_st[p], P.pk_st[np + p], P.pk_st[2 * np + p], P.pk_st[3 * np + p]};
+    }
+    L.h = 0.0;
+    L.h_try = 0.0;
+    L.call_min = 0.0;
+    L.call_max = 0.0;
+    L.att = 0;
+    L.phase = PH_IDLE;
+    L.crosses = 0;
+    L.reached = 0;
+    L.win = 0;
+    L.vac = 0;
+    L.req = __builtin_nan("");
 }
 
 KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
@@ -841,6 +996,69 @@ template <> struct Radiation<false, M_GRMHD> {
     }
 };
 
+// Slow light: the plasma state of snapshot pair (k, k+1) at time fraction f.
+// Fluid interpolation mixes the sampled fluid before the reconstruction (lo
+// holds the result); coefficient interpolation evaluates both snapshots (lo,
+// hi) and mixes each frequency's coefficients (slow_light_driver.hpp:166-193).
+template <> struct Radiation<false, M_SLOW> {
+    RiafLocal<double> lo, hi;
+    double f;
+    int k;
+    KP_DEV void zero() {
+        lo.valid = 0;
+        hi.valid = 0;
+        f = 0.0;
+        k = 0;
+    }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        if (P.slow_interp == KP_SLOW_LIGHT_FLUID) {
+            grmhd_local_time(P.grmhd, P.snaps[k], P.snaps[k + 1], f, sample, lo);
+        } else {
+            grmhd_local_time(P.grmhd, P.snaps[k], P.snaps[k], 0.0, sample, lo);
+            grmhd_local_time(P.grmhd, P.snaps[k + 1], P.snaps[k + 1], 0.0, sample, hi);
+        }
+        return 0;
+    }
+    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
+        grmhd_coeffs_at(P.grmhd, lo, freq, c);
+        if (P.slow_interp != KP_SLOW_LIGHT_FLUID) {
+            Coeffs<double> b;
+            grmhd_coeffs_at(P.grmhd, hi, freq, b);
+            const double w = 1.0 - f;
+            c.jI = w * c.jI + f * b.jI;
+            c.jQ = w * c.jQ + f * b.jQ;
+            c.jU = w * c.jU + f * b.jU;
+            c.jV = w * c.jV + f * b.jV;
+            c.aI = w * c.aI + f * b.aI;
+            c.aQ = w * c.aQ + f * b.aQ;
+            c.aU = w * c.aU + f * b.aU;
+            c.aV = w * c.aV + f * b.aV;
+            c.rQ = w * c.rQ + f * b.rQ;
+            c.rU = w * c.rU + f * b.rU;
+            c.rV = w * c.rV + f * b.rV;
+        }
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        req = 1;
+        pred = 1;
+        for (int q = 0; q < P.nctrl; ++q) {
+            Coeffs<double> c;
+            at(P, P.ctrl[q], c);
+            req = maxv(req, substep_count(c, h, P.dls_c[q], lmrs, P.max_rd, P.max_ad, P.max_fd));
+            pred = maxv(pred, substep_count(c, h_next, P.dls_c[q], lmrs, P.max_rd, P.max_ad, P.max_fd));
+        }
+        return (lo.valid || hi.valid) ? (uint32_t)P.nctrl : 0u;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        for (int q = 0; q < P.nfreq; ++q) {
+            Coeffs<double> c;
+            at(P, P.freqs[q], c);
+            mf_strang(P, L, q, c, absv(h) * P.dls_f[q]);
+        }
+        return (lo.valid || hi.valid) ? (uint32_t)P.nfreq : 0u;
+    }
+};
+
 // Tangent mode: the plasma scalars carry their derivatives with respect to
 // the log-scale parameters of PlasmaPerturbation (plasma_perturbation.hpp:12-33):
 // q_density scales n_e and B^2 together (d ln n_e = dq, d ln B = dq / 2),
@@ -904,23 +1122,74 @@ template <> struct Radiation<false, M_TANGENT> {
     }
 };
 
+// SlowLightTemporalModel::interpolation_fraction (slow_light_driver.hpp:62-69).
+KP_DEV double slow_fraction(double t_lo, double t_hi, double t_obs, double x0) {
+    const double denom = t_hi - t_lo;
+    double f = 0.0;
+    if (absv(denom) > 1e-300) f = (t_obs + x0 - t_lo) / denom;
+    return minv(1.0, maxv(0.0, f));
+}
+
+// Slow light: the end of a ray's current window -- the next snapshot pair
+// (snapshot stepping) or, after the last, the vacuum finish (vac = 1).
+KP_DEV void slow_window_end(const Params& P, LaneB& L) {
+    if (P.slow_mode == KP_SLOW_LIGHT_SNAPSHOT && L.win + 2 < P.nsnap)
+        L.win += 1;
+    else
+        L.vac = 1;
+}
+// The time that ends the current window (snapshot: t_win+1; block: the last).
+KP_DEV double slow_upper(const Params& P, const LaneB& L) {
+    return P.slow_mode == KP_SLOW_LIGHT_SNAPSHOT ? P.snap_t[L.win + 1] : P.snap_t[P.nsnap - 1];
+}
+// SlowLightResidentWindows::sample_window (slow_light_driver.hpp:1000-1012).
+KP_DEV int slow_pair(const Params& P, double t) {
+    int lo = 0, hi = P.nsnap - 1;
+    while (lo + 1 < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (t < P.snap_t[mid])
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return lo;
+}
+
 // One tick of pass B; returns true when the ray has terminated.
-template <bool MIXED, int MODE>
+// Slow light (MODE == M_SLOW) follows the window kernel of
+// launch_slow_light_rays_metric (slow_light_driver.hpp:1083-1483) across all
+// windows of the ray in turn, then advance_slow_light_vacuum_to_camera
+// (:1572-1671) for a ray still unfinished after the last window (vac).
+template <bool MIXED, int GP, int MODE>
 KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::GeoType& G, LaneB& L,
                         uint32_t& coeff_acc, uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     using GT = GeomT<geom_of<MODE>()>;
+    constexpr bool SLOW = MODE == M_SLOW;
     const double eff_min = absv(P.min_step);
     if (L.phase == PH_IDLE) {
         if (L.reached || L.steps >= P.max_steps) return true;
+        if constexpr (SLOW) {
+            if (!L.vac && P.slow_mode != KP_SLOW_LIGHT_DECOUPLED) {
+                while (!L.vac && P.t_obs + L.s[0] >= slow_upper(P, L)) slow_window_end(P, L);
+            }
+            if (L.vac == 1) {
+                const double tol_r = 1e-8 * maxv(1.0, absv(P.outer));
+                if (GT::radius(L.s[1]) < P.outer - tol_r) {
+                    L.reason = R_SLOW_TIME;
+                    return true;
+                }
+                L.vac = 2;
+            }
+        }
         double h = L.h_current;
-        if (GT::radius(L.s[1]) <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
+        if (!(SLOW && L.vac) && GT::radius(L.s[1]) <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
         L.h = h;
         if (P.adaptive) start_call(L, h, eff_min, P.max_step);
         L.phase = PH_MAIN;
     }
     St<double> next, sample;
     if (P.adaptive) {
-        const double err = attempt<MIXED>(P, G, L.s, L.k1, L.h_try, sample, next, guard);
+        const double err = attempt<GP>(P, G, L.s, L.k1, L.h_try, sample, next, guard);
         const double tol = maxv(P.tol, 8.0 * eps<double>());
         if (err <= tol) {
             double factor = 2.0;
@@ -940,50 +1209,143 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
                 const bool main = L.phase == PH_MAIN;
                 L.phase = PH_IDLE;
                 if (main && L.h_current <= eff_min * 1.0001) {
-                    L.reason = R_UNDERFLOW;
+                    // the slow-light window and vacuum loops end such a ray as max_steps
+                    L.reason = SLOW ? R_MAX_STEPS : R_UNDERFLOW;
                     return true;
                 }
             }
             return false;
         }
     } else {
-        step_rk4<MIXED>(P, G, L.s, L.h, next, guard);
+        step_rk4<GP>(P, G, L.s, L.h, next, guard);
         st_copy(sample, next);
     }
+    const bool truncating = SLOW && !L.vac && P.slow_mode != KP_SLOW_LIGHT_DECOUPLED;
     if (L.phase == PH_MAIN) {
         // camera_surface_value/crossed/crossing_fraction for the pinhole sphere
         // (pass_b.hpp:54-85, 605-637)
         const double s0 = GT::radius(L.s[1]) - P.radius;
         const double s1 = GT::radius(next.v[1]) - P.radius;
-        const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
-        L.crosses = crosses;
+        int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+        double frac = 2.0;
         if (crosses) {
             const double denom = s0 - s1;
-            const double frac = absv(denom) <= 1e-300 ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
-            const double h_cross = L.h * frac;
-            if (absv(h_cross) <= 1e-14) {
-                L.h = 0.0;
-                st_load(next, L.s);
-                st_load(sample, L.s);
-            } else if (frac < 0.999999999999) {
-                L.h = h_cross;
-                if (P.adaptive) {
-                    const double cmin = minv(eff_min, absv(h_cross));
-                    start_call(L, h_cross, cmin, maxv(cmin, absv(h_cross)));
-                    L.phase = PH_CROSS;
-                    return false;
+            frac = absv(denom) <= 1e-300 ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
+        }
+        if (truncating) {
+            // a step that crosses the window's end time first is cut there
+            // (slow_light_driver.hpp:1198-1247)
+            const double old_t = P.t_obs + L.s[0];
+            const double next_t = P.t_obs + next.v[0];
+            const double upper = slow_upper(P, L);
+            if (next_t > upper && next_t > old_t) {
+                const double tf = minv(1.0, maxv(0.0, (upper - old_t) / (next_t - old_t)));
+                if (!crosses || tf < frac) {
+                    L.crosses = 0;
+                    const double h_b = L.h * tf;
+                    if (absv(h_b) <= 1e-14) {
+                        slow_window_end(P, L);
+                        L.phase = PH_IDLE;
+                        return false;
+                    }
+                    L.h = h_b;
+                    if (P.adaptive) {
+                        const double cmin = minv(eff_min, absv(h_b));
+                        start_call(L, h_b, cmin, maxv(cmin, absv(h_b)));
+                        L.phase = PH_TIME;
+                        return false;
+                    }
+                    step_rk4<GP>(P, G, L.s, L.h, next, guard);
+                    st_copy(sample, next);
+                    crosses = 0;
+                    L.phase = PH_TIME;
                 }
-                step_rk4<MIXED>(P, G, L.s, L.h, next, guard);
-                st_copy(sample, next);
+            }
+        }
+        if (L.phase == PH_MAIN) {
+            L.crosses = crosses;
+            if (crosses) {
+                const double h_cross = L.h * frac;
+                if (absv(h_cross) <= 1e-14) {
+                    L.h = 0.0;
+                    st_load(next, L.s);
+                    st_load(sample, L.s);
+                } else if (frac < 0.999999999999) {
+                    L.h = h_cross;
+                    if (P.adaptive) {
+                        const double cmin = minv(eff_min, absv(h_cross));
+                        start_call(L, h_cross, cmin, maxv(cmin, absv(h_cross)));
+                        L.phase = PH_CROSS;
+                        return false;
+                    }
+                    step_rk4<GP>(P, G, L.s, L.h, next, guard);
+                    st_copy(sample, next);
+                }
+            }
+        }
+    }
+    if constexpr (SLOW) {
+        if (L.phase == PH_TIME) {
+            // two half steps that round back to the old time end the window,
+            // or the ray when the gap is wider than rounding (:1227-1245)
+            L.crosses = 0;
+            if (next.v[0] == L.s[0]) {
+                const double old_t = P.t_obs + L.s[0];
+                const double upper = slow_upper(P, L);
+                const double scale = maxv(absv(L.s[0]), maxv(absv(upper), absv(old_t - L.s[0])));
+                const double rounding = 0.5 * 8.0 * eps<double>() * scale;
+                L.phase = PH_IDLE;
+                if (absv(upper - old_t) > rounding) {
+                    L.reason = R_UNDERFLOW;
+                    return true;
+                }
+                slow_window_end(P, L);
+                return false;
             }
         }
     }
     L.phase = PH_IDLE;
     const double h = L.h;
     const double old_r = GT::radius(L.s[1]), sample_r = GT::radius(sample.v[1]), next_r = GT::radius(next.v[1]);
+    if constexpr (SLOW) {
+        if (L.vac) {
+            // outward vacuum only (:1647-1660)
+            const double tol_r = 1e-8 * maxv(1.0, absv(P.outer));
+            if (!(next_r == next_r) || absv(next_r) > 1e300 || !(sample_r == sample_r) || absv(sample_r) > 1e300) {
+                L.reason = R_MAX_STEPS;
+                return true;
+            }
+            if (next_r < old_r - tol_r || sample_r < P.outer - tol_r || next_r < P.outer - tol_r) {
+                L.reason = R_SLOW_TIME;
+                return true;
+            }
+            st_store(L.s, next);
+            L.steps += 1;
+            steps_acc += 1;
+            if (L.crosses) L.reached = 1;
+            return false;
+        }
+    }
     Radiation<MIXED, MODE> rad;
     rad.zero();
-    if (sample_r - P.inner > 0.0 && sample_r < P.outer) coeff_acc += rad.eval(P, sample);
+    if constexpr (SLOW) {
+        const int radiating = sample_r >= P.inner && sample_r < P.outer;
+        const double ct = P.t_obs + sample.v[0];
+        if (P.slow_mode == KP_SLOW_LIGHT_DECOUPLED && radiating && !(ct >= P.snap_t[0] && ct <= P.snap_t[P.nsnap - 1])) {
+            // the time is not in the sequence: with every snapshot resident no
+            // later block brings it (slow_light_driver.hpp:1303-1309,
+            // slow_light_batch_driver.hpp:282-290)
+            L.req = ct;
+            L.reason = R_SLOW_TIME;
+            return true;
+        }
+        const int k = P.slow_mode == KP_SLOW_LIGHT_SNAPSHOT ? L.win : slow_pair(P, ct);
+        rad.k = k;
+        rad.f = slow_fraction(P.snap_t[k], P.snap_t[k + 1], P.t_obs, sample.v[0]);
+        if (radiating) coeff_acc += rad.eval(P, sample);
+    } else {
+        if (sample_r - P.inner > 0.0 && sample_r < P.outer) coeff_acc += rad.eval(P, sample);
+    }
     const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
                             ? P.max_rs : 0.0;
     int required, pred;
@@ -1008,8 +1370,8 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
     return false;
 }
 
-template <bool PERSISTENT, bool MIXED, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
+template <bool PERSISTENT, bool MIXED, int GP, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32 ? KP_WAVES_PER_EU_F32 : KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
     uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
     const auto G = GeomT<geom_of<MODE>()>::geo(P, &rhs_count);
     LaneB L;
@@ -1020,21 +1382,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
     }
     bool has = false;
     if (PERSISTENT) {
+        const uint32_t limit = work_count(P);
         bool exhausted = false;
         while (true) {
             // refill until every lane holds a radiating ray or the queue is dry
             while (true) {
                 const bool need = !has && !exhausted;
                 if (!__ballot(need)) break;
-                int pix = 0;
-                if (fetch_pixel(&P.queue[1], need, (uint32_t)P.npix, pix)) {
-                    has = lane_b_init<MODE>(P, pix, L);
+                int idx = 0;
+                if (fetch_pixel(P.fetch, need, limit, idx)) {
+                    if (MIXED && P.list_in) {
+                        lane_b_resume<MODE>(P, (int)P.list_in[idx], L);
+                        has = true;
+                    } else {
+                        has = lane_b_init<MODE>(P, idx, L);
+                    }
                 } else if (need) {
                     exhausted = true;
                 }
             }
             if (!__ballot(has)) break;
-            if (has && pass_b_tick<MIXED, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+            if (MIXED && P.park != PARK_NONE) {
+                using GT = GeomT<geom_of<MODE>()>;
+                const bool park = has && L.phase == PH_IDLE && park_due(P, GT::radius(L.s[1]));
+                if (park) {
+                    lane_b_park<MODE>(P, L);
+                    has = false;
+                }
+                append_parked(P, park, L.pixel);
+            }
+            if (has && pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
                 lane_b_finish<MIXED, MODE>(P, L, 1);
                 has = false;
             }
@@ -1042,7 +1419,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
     } else {
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
         if (pix < P.npix && lane_b_init<MODE>(P, pix, L)) {
-            while (!pass_b_tick<MIXED, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+            while (!pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
             }
             lane_b_finish<MIXED, MODE>(P, L, 1);
         }
@@ -1061,6 +1438,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
 // ---------------------------------------------------------------------------
 template <bool MIXED, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void unsplit_kernel(Params P) {
+    constexpr int GP = MIXED ? GP_GUARD : GP_64;
     uint32_t rhs_a = 0, rhs_b = 0, coeff = 0, transfer = 0, steps_a = 0, steps_b = 0, guard = 0;
     while (true) {
         int pix = 0;
@@ -1073,7 +1451,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
                 LaneA A;
                 lane_lds(A.s, A.k1);
                 lane_a_init(P, pix, A);
-                while (!pass_a_tick<MIXED, GEOM>(P, G, A, steps_a, guard)) {
+                while (!pass_a_tick<GP, GEOM>(P, G, A, steps_a, guard)) {
                 }
                 lane_a_store(P, A);
             }
@@ -1085,7 +1463,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
                 B.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
             }
             if (lane_b_init<MODE>(P, pix, B)) {
-                while (!pass_b_tick<MIXED, MODE>(P, G, B, coeff, transfer, steps_b, guard)) {
+                while (!pass_b_tick<MIXED, GP, MODE>(P, G, B, coeff, transfer, steps_b, guard)) {
                 }
                 lane_b_finish<MIXED, MODE>(P, B, 1);
             }
@@ -1148,7 +1526,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         if (!__ballot(has)) break;
         if (has) {
             const Geo<double> G{P->mass, P->spin, &rhs_count};
-            if (pass_a_tick<MIXED, G_BL>(*P, G, L, steps_a, guard)) {
+            if (pass_a_tick<MIXED ? GP_GUARD : GP_64, G_BL>(*P, G, L, steps_a, guard)) {
                 lane_a_store(*P, L);
                 has = false;
             }
@@ -1180,7 +1558,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         if (!__ballot(has)) break;
         if (has) {
             const Geo<double> G{P->mass, P->spin, &rhs_count};
-            if (pass_b_tick<MIXED, M_SINGLE>(*P, G, L, coeff, transfer, steps_b, guard)) {
+            if (pass_b_tick<MIXED, MIXED ? GP_GUARD : GP_64, M_SINGLE>(*P, G, L, coeff, transfer, steps_b, guard)) {
                 lane_b_finish<MIXED, M_SINGLE>(*P, L, 1);
                 has = false;
             }
@@ -1200,9 +1578,9 @@ __global__ __launch_bounds__(256) void derive_scalars_kernel(GrmhdDev G, kp_grmh
     grmhd_derive_cell(G, i, j, cells[c]);
 }
 
-__global__ void zero_kernel(uint32_t* q, unsigned long long* c) {
+__global__ void zero_kernel(uint32_t* q, int nq, unsigned long long* c) {
     const int t = threadIdx.x;
-    if (t < 2) q[t] = 0u;
+    if (t < nq) q[t] = 0u;
     if (t < C_COUNT) c[t] = 0ull;
 }
 
@@ -1246,34 +1624,48 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
             return kp_internal::fail(KP_HIP_ERROR, std::string(#call ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-struct Occupancy {
-    int cus = 0;
-    int blocks_a = 0, blocks_b = 0, blocks_u = 0;
-};
-
-template <bool PERSISTENT> const void* pass_a_fn(bool mixed, int geom = G_BL) {
-    if (geom == G_FMKS) return (const void*)pass_a_kernel<PERSISTENT, false, G_FMKS>;
-    return mixed ? (const void*)pass_a_kernel<PERSISTENT, true, G_BL> : (const void*)pass_a_kernel<PERSISTENT, false, G_BL>;
+template <bool PERSISTENT> const void* pass_a_fn(int gp, int geom = G_BL) {
+    if (geom == G_FMKS) return (const void*)pass_a_kernel<PERSISTENT, GP_64, G_FMKS>;
+    if (gp == GP_GUARD) return (const void*)pass_a_kernel<PERSISTENT, GP_GUARD, G_BL>;
+    if constexpr (PERSISTENT) {
+        if (gp == GP_32) return (const void*)pass_a_kernel<true, GP_32, G_BL>;
+    }
+    return (const void*)pass_a_kernel<PERSISTENT, GP_64, G_BL>;
 }
-template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int mode) {
-    if (mode == M_TANGENT) return (const void*)pass_b_kernel<PERSISTENT, false, M_TANGENT>;
-    if (mode == M_GRMHD) return (const void*)pass_b_kernel<PERSISTENT, false, M_GRMHD>;
-    if (mode == M_MF)
-        return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, M_MF>
-                     : (const void*)pass_b_kernel<PERSISTENT, false, M_MF>;
-    return mixed ? (const void*)pass_b_kernel<PERSISTENT, true, M_SINGLE>
-                 : (const void*)pass_b_kernel<PERSISTENT, false, M_SINGLE>;
+
+// mixed: FP32 radiation (nu^3-scaled units); gp: geodesic precision.  The
+// staged variants (GP_32, and GP_64 with FP32 radiation) exist as persistent
+// kernels only.
+template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int gp, int mode) {
+    if (mode == M_TANGENT) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_TANGENT>;
+    if (mode == M_GRMHD) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_GRMHD>;
+    if (mode == M_SLOW) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_SLOW>;
+    const bool mf = mode == M_MF;
+    if (!mixed)
+        return mf ? (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_MF>
+                  : (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_SINGLE>;
+    if constexpr (PERSISTENT) {
+        if (gp == GP_32)
+            return mf ? (const void*)pass_b_kernel<true, true, GP_32, M_MF>
+                      : (const void*)pass_b_kernel<true, true, GP_32, M_SINGLE>;
+        if (gp == GP_64)
+            return mf ? (const void*)pass_b_kernel<true, true, GP_64, M_MF>
+                      : (const void*)pass_b_kernel<true, true, GP_64, M_SINGLE>;
+    }
+    return mf ? (const void*)pass_b_kernel<PERSISTENT, true, GP_GUARD, M_MF>
+              : (const void*)pass_b_kernel<PERSISTENT, true, GP_GUARD, M_SINGLE>;
 }
 
 const void* unsplit_fn(bool mixed, int mode) {
     if (mode == M_TANGENT) return (const void*)unsplit_kernel<false, M_TANGENT>;
     if (mode == M_GRMHD) return (const void*)unsplit_kernel<false, M_GRMHD>;
+    if (mode == M_SLOW) return (const void*)unsplit_kernel<false, M_SLOW>;
     if (mode == M_MF)
         return mixed ? (const void*)unsplit_kernel<true, M_MF> : (const void*)unsplit_kernel<false, M_MF>;
     return mixed ? (const void*)unsplit_kernel<true, M_SINGLE> : (const void*)unsplit_kernel<false, M_SINGLE>;
 }
 
-int geom_of_mode(int mode) { return mode == M_GRMHD ? G_FMKS : G_BL; }
+int geom_of_mode(int mode) { return (mode == M_GRMHD || mode == M_SLOW) ? G_FMKS : G_BL; }
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
 // plus 4 doubles per lane and output frequency for multi-frequency Stokes.
@@ -1281,34 +1673,43 @@ size_t lds_bytes(int block, int nfreq_mf) {
     return (size_t(KP_LANE_LDS ? 32 : 0) + size_t(4 * nfreq_mf)) * size_t(block) * sizeof(double);
 }
 
+struct OccKey {
+    int device;
+    const void* fn;
+    int block;
+    size_t lds;
+};
 std::mutex g_occ_mutex;
-std::vector<std::pair<long long, Occupancy>> g_occ;  // key: (((device * 4096 + block) * 2 + mixed) * 4 + mode) * 32 + nfreq_mf
+std::vector<std::pair<OccKey, int>> g_occ;
 
-// mode: pass B variant (Mode); nfreq_mf: output frequencies of a
-// multi-frequency pass B, 0 otherwise.
-kp_status occupancy_for(int device, int block, bool mixed, int mode, int nfreq_mf, Occupancy& occ) {
+// Resident blocks per CU of one kernel at this block size and LDS (cached).
+kp_status blocks_per_cu(int device, const void* fn, int block, size_t lds, int& out) {
     std::lock_guard<std::mutex> lock(g_occ_mutex);
-    const long long key = ((((long long)device * 4096 + block) * 2 + (mixed ? 1 : 0)) * 4 + mode) * 32 + nfreq_mf;
     for (auto& e : g_occ)
-        if (e.first == key) {
-            occ = e.second;
+        if (e.first.device == device && e.first.fn == fn && e.first.block == block && e.first.lds == lds) {
+            out = e.second;
             return KP_OK;
         }
-    Occupancy o;
-    hipDeviceProp_t prop;
-    KP_HIP(hipGetDeviceProperties(&prop, device));
-    o.cus = prop.multiProcessorCount;
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_a, pass_a_fn<true>(mixed, geom_of_mode(mode)), block,
-                                                        lds_bytes(block, 0)));
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_b, pass_b_fn<true>(mixed, mode), block,
-                                                        lds_bytes(block, nfreq_mf)));
-    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocks_u, unsplit_fn(mixed, mode), block,
-                                                        lds_bytes(block, nfreq_mf)));
-    o.blocks_a = std::max(o.blocks_a, 1);
-    o.blocks_b = std::max(o.blocks_b, 1);
-    o.blocks_u = std::max(o.blocks_u, 1);
-    g_occ.push_back({key, o});
-    occ = o;
+    int n = 0;
+    KP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, lds));
+    n = std::max(n, 1);
+    g_occ.push_back({OccKey{device, fn, block, lds}, n});
+    out = n;
+    return KP_OK;
+}
+
+std::mutex g_cu_mutex;
+std::vector<int> g_cus;
+
+kp_status cu_count(int device, int& out) {
+    std::lock_guard<std::mutex> lock(g_cu_mutex);
+    if ((int)g_cus.size() <= device) g_cus.resize(device + 1, 0);
+    if (g_cus[device] == 0) {
+        hipDeviceProp_t prop;
+        KP_HIP(hipGetDeviceProperties(&prop, device));
+        g_cus[device] = prop.multiProcessorCount;
+    }
+    out = g_cus[device];
     return KP_OK;
 }
 
@@ -1332,6 +1733,8 @@ static kp_status prepare_common(const kp_pass_a_params* p, Params& P) {
     if (npix > (1ll << 30)) return kp_internal::fail(KP_INVALID_ARGUMENT, "image too large");
     if (p->max_steps < 0) return kp_internal::fail(KP_INVALID_ARGUMENT, "max_steps must be non-negative");
     std::memset(&P, 0, sizeof(P));
+    P.stage = -1;
+    P.park = PARK_NONE;
     P.nx = p->camera.nx;
     P.ny = p->camera.ny;
     P.npix = (int32_t)npix;
@@ -1518,7 +1921,7 @@ struct Tangent {
 static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl_in,
                            const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
                            int perturbation, double log_scale, const Tangent* tangent = nullptr,
-                           const kp_grmhd_model* grmhd = nullptr) {
+                           const kp_grmhd_model* grmhd = nullptr, const kp_slow_light* slow = nullptr) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -1555,7 +1958,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         st = prepare_frequencies(grmhd ? grmhd->mbh_solar : model->mbh_solar, fl, P);
         if (st != KP_OK) return st;
     }
-    int mode = grmhd ? M_GRMHD : (mf ? M_MF : M_SINGLE);
+    int mode = grmhd ? (slow ? M_SLOW : M_GRMHD) : (mf ? M_MF : M_SINGLE);
     if (tangent) {
         if (mixed || mf)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "parameter gradients are built for single-frequency FP64");
@@ -1576,13 +1979,30 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     int device = 0;
     KP_HIP(hipGetDevice(&device));
     const size_t np = (size_t)P.npix;
+    const bool unsplit = opt.split_transport == 0;
+    // Staged mixed precision (persistent split kernels): an FP32-geodesic
+    // launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
+    // continues them and parks those back that reach r >= back_radius, then
+    // FP32 again and a final FP64 launch; no wave mixes the two paths.
+    const bool staged = mixed && P.persistent && !unsplit && opt.guard_mode == 0 && P.fp64_radius > 0.0 &&
+                        (mode == M_SINGLE || mode == M_MF);
+    const int nst = mf ? P.nfreq : 1;
     const size_t off_ep = 0;
     const size_t off_valid = align_up(off_ep + 16 * np * sizeof(double));
     const size_t off_steps = align_up(off_valid + np * sizeof(int32_t));
     const size_t off_reason = align_up(off_steps + np * sizeof(int32_t));
     const size_t off_queue = align_up(off_reason + np * sizeof(int32_t));
-    const size_t off_counters = align_up(off_queue + 2 * sizeof(uint32_t));
-    const size_t total = align_up(off_counters + C_COUNT * sizeof(unsigned long long));
+    const size_t off_counters = align_up(off_queue + KP_NQUEUE * sizeof(uint32_t));
+    size_t total = align_up(off_counters + C_COUNT * sizeof(unsigned long long));
+    const size_t off_lists = total;
+    const size_t off_pkd = align_up(off_lists + (staged ? 2 * np * sizeof(uint32_t) : 0));
+    const size_t off_pki = align_up(off_pkd + (staged ? 2 * np * sizeof(double) : 0));
+    const size_t off_pkst = align_up(off_pki + (staged ? 4 * np * sizeof(int32_t) : 0));
+    const size_t off_slow_t = align_up(off_pkst + (staged ? 4 * (size_t)nst * np * sizeof(double) : 0));
+    const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
+    const size_t off_slow_c = align_up(off_slow_t + nsnap * sizeof(double));
+    const size_t off_req = align_up(off_slow_c + nsnap * sizeof(void*));
+    total = align_up(off_req + (slow ? np * sizeof(double) : 0));
     void* ws = nullptr;
     st = workspace_for(device, total, &ws);
     if (st != KP_OK) return st;
@@ -1593,46 +2013,112 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     P.ep_reason = reinterpret_cast<int32_t*>(base + off_reason);
     P.queue = reinterpret_cast<uint32_t*>(base + off_queue);
     P.counters = reinterpret_cast<unsigned long long*>(base + off_counters);
+    uint32_t* lists = staged ? reinterpret_cast<uint32_t*>(base + off_lists) : nullptr;
+    if (staged) {
+        P.pk_d = reinterpret_cast<double*>(base + off_pkd);
+        P.pk_i = reinterpret_cast<int32_t*>(base + off_pki);
+        P.pk_st = reinterpret_cast<double*>(base + off_pkst);
+        P.back_radius = KP_BACK_RADIUS_FACTOR * P.fp64_radius;
+    }
 
     hipStream_t stream = static_cast<hipStream_t>(opt.stream);
+    if (slow) {
+        P.snap_t = reinterpret_cast<const double*>(base + off_slow_t);
+        P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(base + off_slow_c);
+        P.req_t = reinterpret_cast<double*>(base + off_req);
+        P.nsnap = slow->nsnap;
+        P.slow_mode = slow->step_mode;
+        P.slow_interp = slow->interpolation;
+        P.t_obs = slow->observation_time;
+        KP_HIP(hipMemcpyAsync(base + off_slow_t, slow->times, nsnap * sizeof(double), hipMemcpyHostToDevice, stream));
+        KP_HIP(hipMemcpyAsync(base + off_slow_c, slow->cells, nsnap * sizeof(void*), hipMemcpyHostToDevice, stream));
+    }
     const int block = opt.block_size > 0 ? opt.block_size : 128;
     if (block % 64 != 0 || block > 256)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
-    Occupancy occ;
-    st = occupancy_for(device, block, mixed, mode, mf ? P.nfreq : 0, occ);
+    int cus = 0;
+    st = cu_count(device, cus);
     if (st != KP_OK) return st;
     const size_t smem_a = lds_bytes(block, 0);
     const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0);
     const int nblocks_flat = (int)((np + block - 1) / block);
-    int grid_a = nblocks_flat, grid_b = nblocks_flat, grid_u = nblocks_flat;
-    if (P.persistent) {
-        const int waves_per_block = block / 64;
-        auto per_cu = [&](int occ_blocks) {
-            return opt.waves_per_cu > 0 ? std::max(1, opt.waves_per_cu / waves_per_block) : occ_blocks;
-        };
-        grid_a = std::max(1, std::min(nblocks_flat, occ.cus * per_cu(occ.blocks_a)));
-        grid_b = std::max(1, std::min(nblocks_flat, occ.cus * per_cu(occ.blocks_b)));
-        grid_u = std::max(1, std::min(nblocks_flat, occ.cus * per_cu(occ.blocks_u)));
-    }
-    const bool unsplit = opt.split_transport == 0;
+    // grid of one launch: one block per pixel block, or (persistent) the
+    // resident blocks of the kernel on every CU
+    auto grid_of = [&](const void* fn, size_t smem, int& grid) -> kp_status {
+        grid = nblocks_flat;
+        if (!P.persistent) return KP_OK;
+        int per_cu = 0;
+        const kp_status s2 = blocks_per_cu(device, fn, block, smem, per_cu);
+        if (s2 != KP_OK) return s2;
+        if (opt.waves_per_cu > 0) per_cu = std::max(1, opt.waves_per_cu / (block / 64));
+        grid = std::max(1, std::min(nblocks_flat, cus * per_cu));
+        return KP_OK;
+    };
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (stats) {
         for (int i = 0; i < 3; ++i) KP_HIP(hipEventCreate(&ev[i]));
     }
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, P.queue, P.counters);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, P.queue, KP_NQUEUE, P.counters);
     KP_HIP(hipGetLastError());
     if (stats) KP_HIP(hipEventRecord(ev[0], stream));
-    void* args[] = {&P};
-    if (P.npix > 0 && unsplit)
-        KP_HIP(hipLaunchKernel(unsplit_fn(mixed, mode), dim3(grid_u), dim3(block), args, smem_b, stream));
+    auto launch = [&](const void* fn, size_t smem, Params& Q) -> kp_status {
+        int grid = 0;
+        const kp_status s2 = grid_of(fn, smem, grid);
+        if (s2 != KP_OK) return s2;
+        void* args[] = {&Q};
+        KP_HIP(hipLaunchKernel(fn, dim3(grid), dim3(block), args, smem, stream));
+        return KP_OK;
+    };
+    // the four stages of one pass: FP32, FP64, FP32, FP64 (final, no parking);
+    // queue slots: fetch counters at first_slot + k, list counts at first_slot + 4 + k
+    auto run_staged = [&](bool pass_b, int first_slot) -> kp_status {
+        for (int k = 0; k < 4; ++k) {
+            Params Q = P;
+            const bool fp32 = (k % 2) == 0;
+            Q.stage = k;
+            Q.park = k == 3 ? PARK_NONE : (fp32 ? PARK_INSIDE : PARK_OUTSIDE);
+            Q.fetch = P.queue + first_slot + k;
+            Q.list_in = k == 0 ? nullptr : lists + (size_t)((k - 1) % 2) * np;
+            Q.count_in = k == 0 ? nullptr : P.queue + first_slot + 4 + (k - 1);
+            Q.list_out = lists + (size_t)(k % 2) * np;
+            Q.count_out = P.queue + first_slot + 4 + k;
+            const int gp = fp32 ? GP_32 : GP_64;
+            const void* fn = pass_b ? pass_b_fn<true>(true, gp, mode) : pass_a_fn<true>(gp, G_BL);
+            const kp_status s2 = launch(fn, pass_b ? smem_b : smem_a, Q);
+            if (s2 != KP_OK) return s2;
+        }
+        return KP_OK;
+    };
+    const int gp_plain = mixed ? GP_GUARD : GP_64;
+    if (P.npix > 0 && unsplit) {
+        Params Q = P;
+        Q.fetch = P.queue;
+        st = launch(unsplit_fn(mixed, mode), smem_b, Q);
+        if (st != KP_OK) return st;
+    }
     if (P.npix > 0 && !unsplit) {
-        const void* fn = P.persistent ? pass_a_fn<true>(mixed, geom_of_mode(mode)) : pass_a_fn<false>(mixed, geom_of_mode(mode));
-        KP_HIP(hipLaunchKernel(fn, dim3(grid_a), dim3(block), args, smem_a, stream));
+        if (staged) {
+            st = run_staged(false, 2);
+        } else {
+            Params Q = P;
+            Q.fetch = P.queue;
+            const void* fn = P.persistent ? pass_a_fn<true>(gp_plain, geom_of_mode(mode))
+                                          : pass_a_fn<false>(gp_plain, geom_of_mode(mode));
+            st = launch(fn, smem_a, Q);
+        }
+        if (st != KP_OK) return st;
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
     if (P.npix > 0 && !unsplit) {
-        const void* fn = P.persistent ? pass_b_fn<true>(mixed, mode) : pass_b_fn<false>(mixed, mode);
-        KP_HIP(hipLaunchKernel(fn, dim3(grid_b), dim3(block), args, smem_b, stream));
+        if (staged) {
+            st = run_staged(true, 12);
+        } else {
+            Params Q = P;
+            Q.fetch = P.queue + 1;
+            const void* fn = P.persistent ? pass_b_fn<true>(mixed, gp_plain, mode) : pass_b_fn<false>(mixed, gp_plain, mode);
+            st = launch(fn, smem_b, Q);
+        }
+        if (st != KP_OK) return st;
     }
     if (stats) {
         KP_HIP(hipEventRecord(ev[2], stream));
@@ -1658,6 +2144,29 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         stats->pass_b_steps = (int64_t)cnt[C_STEPS_B];
         stats->fp64_guard_steps = (int64_t)cnt[C_GUARD];
         for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
+    }
+    if (slow && slow->step_mode == KP_SLOW_LIGHT_DECOUPLED && P.npix > 0) {
+        // decoupled stepping: a ray that needed a time outside the sequence
+        // makes the run an error, as the reference's block scheduler does
+        unsigned long long waiting = 0;
+        KP_HIP(hipMemcpyAsync(&waiting, P.counters + C_WAIT, sizeof(waiting), hipMemcpyDeviceToHost, stream));
+        KP_HIP(hipStreamSynchronize(stream));
+        if (waiting) {
+            std::vector<double> req(np);
+            KP_HIP(hipMemcpy(req.data(), P.req_t, np * sizeof(double), hipMemcpyDeviceToHost));
+            double lo = 1e300, hi = -1e300;
+            for (double t : req)
+                if (t == t) {
+                    lo = std::min(lo, t);
+                    hi = std::max(hi, t);
+                }
+            char msg[256];
+            std::snprintf(msg, sizeof(msg),
+                          "decoupled slow-light radiation sample outside supplied physical time support: %llu rays "
+                          "requested %.17g..%.17g, available %.17g..%.17g",
+                          waiting, lo, hi, slow->times[0], slow->times[slow->nsnap - 1]);
+            return kp_internal::fail(KP_TIME_SUPPORT, msg);
+        }
     }
     return KP_OK;
 }
@@ -1688,7 +2197,8 @@ static void batch_release(kp_batch* b) {
 
 static kp_status batch_build_graph(kp_batch* b) {
     KP_HIP(hipGraphCreate(&b->graph, 0));
-    void* zargs[] = {&b->args.queue, &b->args.counters};
+    int nq = 2;
+    void* zargs[] = {&b->args.queue, &nq, &b->args.counters};
     void* bargs[] = {&b->args};
     hipKernelNodeParams zp{};
     zp.func = (void*)zero_kernel;
@@ -1950,6 +2460,38 @@ extern "C" kp_status kp_run_grmhd_multifrequency_image(const kp_pass_a_params* p
     fl.nctrl = step_control_nfreq;
     fl.ctrl = step_control_frequencies;
     return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, model);
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                    const kp_slow_light* sequence, const double* frequencies,
+                                                    int32_t nfreq, const kp_run_options* options,
+                                                    kp_image_data* out, kp_image_stats* stats) {
+    kp_internal::clear_error();
+    if (!model || !sequence) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model or sequence");
+    const kp_slow_light& q = *sequence;
+    if (q.nsnap < 2) return kp_internal::fail(KP_INVALID_ARGUMENT, "slow light requires at least two snapshots");
+    if (!q.times || !q.cells) return kp_internal::fail(KP_INVALID_ARGUMENT, "null snapshot times or cells");
+    if (q.step_mode != KP_SLOW_LIGHT_DECOUPLED && q.step_mode != KP_SLOW_LIGHT_SNAPSHOT &&
+        q.step_mode != KP_SLOW_LIGHT_BLOCK)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light step mode");
+    if (q.interpolation != KP_SLOW_LIGHT_FLUID && q.interpolation != KP_SLOW_LIGHT_COEFFICIENTS)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light interpolation");
+    if (!std::isfinite(q.observation_time)) return kp_internal::fail(KP_INVALID_ARGUMENT, "observation time must be finite");
+    for (int k = 0; k < q.nsnap; ++k) {
+        if (!q.cells[k]) return kp_internal::fail(KP_INVALID_ARGUMENT, "null snapshot cells");
+        if (!std::isfinite(q.times[k]) || (k > 0 && !(q.times[k] > q.times[k - 1])))
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "snapshot times must be finite and strictly increasing");
+    }
+    if (nfreq < 0 || (nfreq > 0 && !frequencies))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "frequency list and count disagree");
+    kp_grmhd_model m = *model;
+    m.cells = q.cells[0];
+    FreqLists fl;
+    if (nfreq > 0) {
+        fl.nfreq = fl.nctrl = nfreq;
+        fl.freqs = fl.ctrl = frequencies;
+    }
+    return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, sequence);
 }
 
 extern "C" kp_status kp_grmhd_derive_scalars(const kp_grmhd_model* model, void* stream) {

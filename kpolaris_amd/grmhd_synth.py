@@ -182,6 +182,40 @@ def make_snapshot(n1=288, n2=128, n3=128, *, spin=0.9375, r_in=1.2, r_out=1000.0
     return cells, model
 
 
+def advect(cells: np.ndarray, model, t: float) -> np.ndarray:
+    """The snapshot at time t of a flow rotating rigidly on each radial shell
+    with the prograde Keplerian angular velocity Omega(r) = 1 / (r^1.5 + a):
+    every primitive at azimuth phi is the t = 0 value at phi - Omega t
+    (periodic linear interpolation in the azimuthal index).  Derived scalars
+    are left zero (recompute them on the device or with the oracle)."""
+    n1, n2, n3 = model.n1, model.n2, model.n3
+    prim = cells["prim"].reshape(n1, n2, n3, 8)
+    r = np.exp(model.startx1 + (np.arange(n1) + 0.5) * model.dx1)
+    omega = 1.0 / (r ** 1.5 + model.spin)
+    shift = omega * t / model.dx3           # cells moved in +phi, per radial index
+    k = np.arange(n3)
+    out = np.empty_like(prim)
+    for i in range(n1):
+        s0 = math.floor(shift[i])
+        w = shift[i] - s0
+        a = prim[i][:, (k - s0) % n3]
+        b = prim[i][:, (k - s0 - 1) % n3]
+        out[i] = ((1.0 - w) * a.astype(np.float64) + w * b.astype(np.float64)).astype(np.float32)
+    res = np.zeros(cells.shape, dtype=CELL_DTYPE)
+    res["prim"] = out.reshape(-1, 8)
+    return res
+
+
+def make_sequence(nsnap=32, dt=5.0, t0=0.0, n1=192, n2=96, n3=96, **kw):
+    """A slow-light snapshot sequence: the seeded torus of make_snapshot at
+    t0, advected by Keplerian rotation to t0 + k dt, k < nsnap.  Returns
+    (list of cell arrays, times, model)."""
+    base, model = make_snapshot(n1, n2, n3, **kw)
+    times = t0 + dt * np.arange(nsnap)
+    cells = [base if k == 0 else advect(base, model, dt * k) for k in range(nsnap)]
+    return cells, times, model
+
+
 def _bsq_native(r, th, dth1, dth2, a, U1, U2, U3, B1, B2, B3):
     """b^2 at cell centres (the derived-scalar formulas, vectorised)."""
     cth, sth = np.cos(th), np.sin(th)
@@ -233,7 +267,7 @@ def grmhd_model(n1, n2, n3, *, spin, gam, startx1, dx1, dx2, dx3, r_in, r_out, h
 
 
 def pass_a_params(model, *, nx=512, ny=512, inclination_deg=163.0, xspan=10.0, camera_radius=1000.0,
-                  adaptive=1, tolerance=1e-6, step=0.1, max_steps=200000):
+                  adaptive=1, tolerance=1e-6, step=0.1, max_steps=200000, outer_radius=None):
     """Camera and integrator parameters of a GRMHD image on the model's FMKS
     coordinates; inner/outer radii per the reference tool
     (tools/kpolaris_model_image.cpp:468-475, 5120-5125)."""
@@ -251,7 +285,7 @@ def pass_a_params(model, *, nx=512, ny=512, inclination_deg=163.0, xspan=10.0, c
     p.mass, p.spin = 1.0, model.spin
     horizon = 1.0 + math.sqrt(max(0.0, 1.0 - model.spin ** 2))
     p.inner_radius = max(model.r_in, 1.05 * horizon)
-    p.outer_radius = min(model.r_out, 0.9 * camera_radius)
+    p.outer_radius = min(model.r_out, 0.9 * camera_radius) if outer_radius is None else outer_radius
     p.step, p.max_steps, p.adaptive = step, max_steps, adaptive
     p.adaptive_tolerance, p.min_step, p.max_step = tolerance, 1e-10, 5.0
     p.max_radiation_step, p.max_radiation_depth = 1.0, 1.0

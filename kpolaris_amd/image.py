@@ -94,6 +94,11 @@ def library():
     L.kp_batch_destroy.restype = None
     L.kp_run_riaf_batch.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.RunOptions), P(abi.ImageStats)]
     L.kp_run_riaf_batch.restype = C.c_int32
+    L.kp_run_grmhd_slow_light_image.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
+                                                P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData),
+                                                P(abi.ImageStats)]
+    L.kp_run_grmhd_slow_light_image.restype = C.c_int32
+    L.kp_slow_light_default.argtypes = [P(abi.SlowLight)]
     if L.kp_abi_version() != abi.ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
                            f"{abi.ABI_VERSION}: rebuild the library")
@@ -162,7 +167,7 @@ def pack_image(image: dict, frequency: float, qu_sign: int, scale: float):
 
 def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
                 waves_per_cu: int = 0, stream=None, fp64_radius: float | None = None,
-                split_transport: int = 1) -> abi.RunOptions:
+                split_transport: int = 1, guard_mode: int = 0) -> abi.RunOptions:
     o = abi.RunOptions()
     library().kp_run_options_default(C.byref(o))
     o.precision = precision
@@ -172,6 +177,7 @@ def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, bl
     if fp64_radius is not None:
         o.fp64_radius = fp64_radius
     o.split_transport = split_transport
+    o.guard_mode = guard_mode
     o.stream = stream
     return o
 
@@ -202,7 +208,7 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
                    precision: int = abi.PRECISION_FP64, persistent: bool = True,
                    block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
                    fields=abi.IMAGE_FIELDS, with_stats: bool = True, fp64_radius: float | None = None,
-                   split_transport: int = 1):
+                   split_transport: int = 1, guard_mode: int = 0):
     """Run pass A + pass B on the current HIP device; returns (dict of device
     tensors, stats dict or None).  Work is queued on torch's current stream."""
     import torch
@@ -212,7 +218,7 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
     opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius,
-                       split_transport)
+                       split_transport, guard_mode)
     st = abi.ImageStats()
     _check(library().kp_run_riaf_image(C.byref(params), C.byref(model), radiation_substeps,
                                        C.byref(opts), C.byref(img),
@@ -243,7 +249,8 @@ def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel,
                                   precision: int = abi.PRECISION_FP64, persistent: bool = True,
                                   block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
                                   fields=abi.IMAGE_FIELDS, with_stats: bool = True,
-                                  fp64_radius: float | None = None, split_transport: int = 1):
+                                  fp64_radius: float | None = None, split_transport: int = 1,
+                                  guard_mode: int = 0):
     """Multi-frequency image on shared geodesics on the current HIP device.
     Stokes tensors are [nfreq * npix] (frequency-major); step control uses
     step_control_frequencies (default: the output frequencies)."""
@@ -257,7 +264,7 @@ def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel,
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
     opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius,
-                       split_transport)
+                       split_transport, guard_mode)
     st = abi.ImageStats()
     dp = C.POINTER(C.c_double)
     _check(library().kp_run_riaf_multifrequency_image(
@@ -383,5 +390,42 @@ def run_grmhd_image(params: abi.PassAParams, grid: "GRMHDGrid", frequencies=None
     dp = C.POINTER(C.c_double)
     _check(library().kp_run_grmhd_multifrequency_image(
         C.byref(params), C.byref(grid.model), 1, f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp), len(c),
+        C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
+    return out, (st.as_dict() if with_stats else None)
+
+
+def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observation_time: float, frequencies=None,
+                               *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
+                               interpolation: int = abi.SLOW_LIGHT_FLUID, persistent: bool = True,
+                               block_size: int = 0, waves_per_cu: int = 0, split_transport: int = 1,
+                               out: dict | None = None, fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+    """Slow-light GRMHD image over a snapshot sequence (grids: GRMHDGrid per
+    time, all on one grid geometry; times strictly increasing) sampled at
+    retarded time observation_time + x^0.  Raises KPError(KP_TIME_SUPPORT)
+    when decoupled stepping needs a time outside the sequence (the image is
+    still written into `out`)."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    model = grids[0].model
+    f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    ptrs = (C.c_void_p * len(grids))(*[g.model.cells for g in grids])
+    seq = abi.SlowLight()
+    library().kp_slow_light_default(C.byref(seq))
+    seq.nsnap = len(grids)
+    seq.step_mode = step_mode
+    seq.interpolation = interpolation
+    seq.observation_time = observation_time
+    seq.times = t.ctypes.data_as(C.POINTER(C.c_double))
+    seq.cells = C.cast(ptrs, C.POINTER(C.c_void_p))
+    if out is None:
+        out = alloc_device_image(npix, fields=fields, nfreq=len(f))
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, persistent, block_size, waves_per_cu, C.c_void_p(stream), None,
+                       split_transport)
+    st = abi.ImageStats()
+    _check(library().kp_run_grmhd_slow_light_image(
+        C.byref(params), C.byref(model), C.byref(seq), f.ctypes.data_as(C.POINTER(C.c_double)), len(f),
         C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)

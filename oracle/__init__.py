@@ -115,6 +115,19 @@ def lib():
     L.kpo_grmhd_coefficients.argtypes = [C.POINTER(abi.GRMHDModel), C.POINTER(State), C.POINTER(Perturbation),
                                          i32, C.POINTER(Coeffs)]
     L.kpo_grmhd_coefficients.restype = None
+    L.kpo_interpolate_time_value.argtypes = [d, d, d]
+    L.kpo_interpolate_time_value.restype = d
+    L.kpo_slow_light_fraction.argtypes = [d, d, d, d]
+    L.kpo_slow_light_fraction.restype = d
+    L.kpo_slow_light_pair.argtypes = [dp, i32, d]
+    L.kpo_slow_light_pair.restype = i32
+    L.kpo_slow_light_coefficients.argtypes = [C.POINTER(abi.GRMHDModel), C.POINTER(abi.SlowLight), i32,
+                                              C.POINTER(State), C.POINTER(Perturbation), i32, C.POINTER(Coeffs)]
+    L.kpo_slow_light_coefficients.restype = None
+    L.kpo_run_grmhd_slow_light_image.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.GRMHDModel),
+                                                 C.POINTER(abi.SlowLight), dp, i32, i32, i32, i32,
+                                                 C.POINTER(abi.ImageData), C.POINTER(Stats), dp]
+    L.kpo_run_grmhd_slow_light_image.restype = i32
     _lib = L
     return L
 
@@ -247,3 +260,46 @@ def run_grmhd_image(params: abi.PassAParams, model: abi.GRMHDModel, cells: np.nd
     if rc != 0:
         raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
     return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}
+
+
+def slow_light_sequence(cells_list, times, observation_time, step_mode=abi.SLOW_LIGHT_DECOUPLED,
+                        interpolation=abi.SLOW_LIGHT_FLUID):
+    """kp_slow_light over HOST snapshot arrays; returns (struct, keep-alive refs)."""
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    ptrs = (C.c_void_p * len(cells_list))(*[c.ctypes.data for c in cells_list])
+    seq = abi.SlowLight()
+    seq.nsnap = len(cells_list)
+    seq.step_mode = step_mode
+    seq.interpolation = interpolation
+    seq.observation_time = observation_time
+    seq.times = t.ctypes.data_as(C.POINTER(C.c_double))
+    seq.cells = C.cast(ptrs, C.POINTER(C.c_void_p))
+    return seq, (t, ptrs, cells_list)
+
+
+def run_grmhd_slow_light_image(params: abi.PassAParams, model: abi.GRMHDModel, cells_list, times,
+                               observation_time, frequencies=None, step_mode=abi.SLOW_LIGHT_DECOUPLED,
+                               interpolation=abi.SLOW_LIGHT_FLUID, pixel_begin: int = 0, pixel_end: int = -1,
+                               nthreads: int = 0):
+    """Oracle slow-light GRMHD image; returns (arrays, stats, waiting) where
+    waiting = (count, min, max) of the times decoupled stepping needed outside
+    the sequence (count 0: none; the oracle status is then KP_OK)."""
+    L = lib()
+    seq, keep = slow_light_sequence(cells_list, times, observation_time, step_mode, interpolation)
+    npix = params.camera.nx * params.camera.ny
+    f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    arrays = alloc_image(npix)
+    for n in ("image_i", "image_q", "image_u", "image_v"):
+        arrays[n] = np.zeros(len(f) * npix, dtype=np.float64)
+    img = image_struct(arrays)
+    st = Stats()
+    waiting = np.zeros(3)
+    if nthreads <= 0:
+        nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
+    dp = C.POINTER(C.c_double)
+    rc = L.kpo_run_grmhd_slow_light_image(C.byref(params), C.byref(model), C.byref(seq), f.ctypes.data_as(dp),
+                                          len(f), pixel_begin, pixel_end, nthreads, C.byref(img), C.byref(st),
+                                          waiting.ctypes.data_as(dp))
+    if rc not in (0, abi.KP_TIME_SUPPORT):
+        raise RuntimeError(f"oracle failed: {abi.STATUS_NAMES.get(rc, rc)}")
+    return arrays, {n: getattr(st, n) for n, _ in Stats._fields_}, tuple(waiting)

@@ -1900,6 +1900,8 @@ typedef struct pixel_result {
     double r11, r12, r21, r22;
     double closure_x, closure_k, final_null, frame_error;
     int pass_a_steps, steps, reason;
+    int waiting;           /* slow light, decoupled: the ray needs a time outside the sequence */
+    double requested_time; /* that time */
 } pixel_result;
 
 /* Evaluate the plasma coefficients on the sample state, with the
@@ -2123,29 +2125,41 @@ double kpo_grmhd_sample_prim(const kp_grmhd_model* g, const kp_grmhd_cell* cells
     return out;
 }
 
-/* GRMHDGridRadiationModel::coefficients for FMKS data and the FMKS metric
- * (grmhd_grid.hpp:883-1080 fluid_state, 1091-1240 coefficients). */
-void kpo_grmhd_coefficients(const kp_grmhd_model* g, const kpo_state* s, const kpo_perturbation* pert,
-                            int faraday_rotation, kpo_coeffs* out) {
-    memset(out, 0, sizeof(*out));
-    const kp_grmhd_cell* cells = (const kp_grmhd_cell*)g->cells;
-    const kpo_fmks f = grmhd_map(g);
-    const double x1 = s->v[1], x2 = s->v[2], phi = s->v[3];
-    const double r = exp(x1);
-    const double th = kpo_fmks_theta(&f, x1, x2);
-    if (!(r >= g->r_in && r < g->r_out)) return;
+/* Trilinear samples of the 8 primitives and 5 derived scalars of one cell
+ * array at FMKS (x1, x2, phi): interp_prim_stencil / interp_derived_stencil
+ * (grmhd_grid.hpp:379-430).  Returns 0 outside the grid. */
+static int grmhd_sample_cells(const kp_grmhd_model* g, const kp_grmhd_cell* cells, double x1, double x2,
+                              double phi, double prim[8], double der[5]) {
     int i0, j0, k0;
     double di, dj, dk;
-    if (!grmhd_indices(g, x1, x2, phi, &i0, &j0, &k0, &di, &dj, &dk)) return;
+    if (!grmhd_indices(g, x1, x2, phi, &i0, &j0, &k0, &di, &dj, &dk)) return 0;
     size_t cell[8];
     double w[8];
     grmhd_stencil(g, i0, j0, k0, di, dj, dk, cell, w);
-    double prim[8], der[5];
     for (int v = 0; v < 8; ++v) {
         double acc = 0.0;
         for (int n = 0; n < 8; ++n) acc += w[n] * (double)cells[cell[n]].prim[v];
         prim[v] = acc;
     }
+    for (int v = 0; v < 5; ++v) {
+        double acc = 0.0;
+        for (int n = 0; n < 8; ++n) acc += w[n] * cells[cell[n]].derived[v];
+        der[v] = acc;
+    }
+    return 1;
+}
+
+/* The coefficients at an FMKS sample state from the sampled primitives and
+ * derived scalars: the rest of fluid_state (four-vector reconstruction,
+ * grmhd_grid.hpp:953-1080) and coefficients (grmhd_grid.hpp:1091-1240). */
+static void grmhd_coefficients_from(const kp_grmhd_model* g, const kpo_state* s, const double prim[8],
+                                    const double der[5], const kpo_perturbation* pert, int faraday_rotation,
+                                    kpo_coeffs* out) {
+    memset(out, 0, sizeof(*out));
+    const kpo_fmks f = grmhd_map(g);
+    const double x1 = s->v[1], x2 = s->v[2];
+    const double r = exp(x1);
+    const double th = kpo_fmks_theta(&f, x1, x2);
     if (!(prim[0] > 0.0 && prim[1] > 0.0)) return;
     double gcov[16], gcon0[4];
     grmhd_native_metric(g, &f, x1, x2, r, th, gcov, gcon0);
@@ -2153,11 +2167,6 @@ void kpo_grmhd_coefficients(const kp_grmhd_model* g, const kpo_state* s, const k
     const double bsq = grmhd_four_vectors(gcov, gcon0, prim[2], prim[3], prim[4], prim[5], prim[6], prim[7], ucon,
                                           bcon);
     const double bnorm_geom = sqrt(bsq);
-    for (int v = 0; v < 5; ++v) {
-        double acc = 0.0;
-        for (int n = 0; n < 8; ++n) acc += w[n] * cells[cell[n]].derived[v];
-        der[v] = acc;
-    }
     double ne = der[0], thetae = der[1], b_cgs = der[2];
     const double sigma = der[3];
     ne *= sigma_smooth_factor(g, sigma);
@@ -2237,10 +2246,103 @@ void kpo_grmhd_coefficients(const kp_grmhd_model* g, const kpo_state* s, const k
     }
 }
 
-/* The radiation model of one image: the analytic RIAF or a GRMHD grid. */
+/* GRMHDGridRadiationModel::coefficients for FMKS data and the FMKS metric
+ * (grmhd_grid.hpp:883-1080 fluid_state, 1091-1240 coefficients). */
+void kpo_grmhd_coefficients(const kp_grmhd_model* g, const kpo_state* s, const kpo_perturbation* pert,
+                            int faraday_rotation, kpo_coeffs* out) {
+    memset(out, 0, sizeof(*out));
+    const double x1 = s->v[1], x2 = s->v[2], phi = s->v[3];
+    const double r = exp(x1);
+    if (!(r >= g->r_in && r < g->r_out)) return;
+    double prim[8], der[5];
+    if (!grmhd_sample_cells(g, (const kp_grmhd_cell*)g->cells, x1, x2, phi, prim, der)) return;
+    grmhd_coefficients_from(g, s, prim, der, pert, faraday_rotation, out);
+}
+
+/* interpolate_time_value (src/model/time_interpolation.hpp:27-32): exact at
+ * the endpoints and for equal values. */
+double kpo_interpolate_time_value(double a, double b, double fraction) {
+    if (fraction <= 0.0 || a == b) return a;
+    if (fraction >= 1.0) return b;
+    return (1.0 - fraction) * a + fraction * b;
+}
+
+/* SlowLightTemporalModel::interpolation_fraction (slow_light_driver.hpp:62-69). */
+double kpo_slow_light_fraction(double t_lo, double t_hi, double t_obs, double x0) {
+    const double denom = t_hi - t_lo;
+    double f = 0.0;
+    if (absv(denom) > TINY) f = (t_obs + x0 - t_lo) / denom;
+    return minv(1.0, maxv(0.0, f));
+}
+
+/* SlowLightResidentWindows::sample_window over the whole sequence
+ * (slow_light_driver.hpp:1000-1012): the pair (k, k+1) with t_k <= t < t_k+1,
+ * found by bisection; t at or past the last time gives the last pair. */
+int32_t kpo_slow_light_pair(const double* times, int32_t nsnap, double t) {
+    int lo = 0, hi = nsnap - 1;
+    while (lo + 1 < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (t < times[mid])
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return lo;
+}
+
+/* SlowLightTemporalModel::coefficients (slow_light_driver.hpp:166-193) for
+ * the snapshot pair (k, k+1) of a GRMHD sequence at model frequency
+ * g->freq_cgs.  Fluid interpolation samples both snapshots at the state,
+ * interpolates every primitive and derived scalar in time and then
+ * reconstructs (grmhd_grid.hpp:940-962); coefficient interpolation mixes the
+ * two snapshots' coefficients linearly. */
+void kpo_slow_light_coefficients(const kp_grmhd_model* g, const kp_slow_light* seq, int32_t k,
+                                 const kpo_state* s, const kpo_perturbation* pert, int faraday_rotation,
+                                 kpo_coeffs* out) {
+    memset(out, 0, sizeof(*out));
+    const double f = kpo_slow_light_fraction(seq->times[k], seq->times[k + 1], seq->observation_time, s->v[0]);
+    const kp_grmhd_cell* lo = (const kp_grmhd_cell*)seq->cells[k];
+    const kp_grmhd_cell* hi = (const kp_grmhd_cell*)seq->cells[k + 1];
+    const double x1 = s->v[1], x2 = s->v[2], phi = s->v[3];
+    const double r = exp(x1);
+    if (seq->interpolation == KP_SLOW_LIGHT_FLUID) {
+        if (!(r >= g->r_in && r < g->r_out)) return;
+        double p0[8], d0[5], p1[8], d1[5];
+        if (!grmhd_sample_cells(g, lo, x1, x2, phi, p0, d0)) return;
+        if (f > 0.0) {
+            if (!grmhd_sample_cells(g, hi, x1, x2, phi, p1, d1)) return;
+            for (int v = 0; v < 8; ++v) p0[v] = kpo_interpolate_time_value(p0[v], p1[v], f);
+            for (int v = 0; v < 5; ++v) d0[v] = kpo_interpolate_time_value(d0[v], d1[v], f);
+        }
+        grmhd_coefficients_from(g, s, p0, d0, pert, faraday_rotation, out);
+        return;
+    }
+    kp_grmhd_model ga = *g, gb = *g;
+    ga.cells = lo;
+    gb.cells = hi;
+    kpo_coeffs a, b;
+    kpo_grmhd_coefficients(&ga, s, pert, faraday_rotation, &a);
+    kpo_grmhd_coefficients(&gb, s, pert, faraday_rotation, &b);
+    const double w = 1.0 - f;
+    out->jI = w * a.jI + f * b.jI;
+    out->jQ = w * a.jQ + f * b.jQ;
+    out->jU = w * a.jU + f * b.jU;
+    out->jV = w * a.jV + f * b.jV;
+    out->aI = w * a.aI + f * b.aI;
+    out->aQ = w * a.aQ + f * b.aQ;
+    out->aU = w * a.aU + f * b.aU;
+    out->aV = w * a.aV + f * b.aV;
+    out->rQ = w * a.rQ + f * b.rQ;
+    out->rU = w * a.rU + f * b.rU;
+    out->rV = w * a.rV + f * b.rV;
+}
+
+/* The radiation model of one image: the analytic RIAF, a GRMHD grid, or a
+ * GRMHD snapshot sequence sampled at retarded time (slow light). */
 typedef struct model_ctx {
     const kp_riaf_model* riaf;
     const kp_grmhd_model* grmhd;
+    const kp_slow_light* slow;
 } model_ctx;
 
 /* The frequency lists of one pass B: output frequencies and the frequencies
@@ -2445,6 +2547,317 @@ static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
     out->reason = reason;
 }
 
+/* Pixel outputs at the final state (the window kernel's finalisation,
+ * slow_light_driver.hpp:1444-1480, and finalize_slow_light_unfinished_metric,
+ * :1725-1766): screen overlap with the camera ray, observer-basis Stokes,
+ * closure and frame diagnostics. */
+static void slow_finish(const metric_ctx* m, const kpo_state* camera, const kpo_state* state,
+                        const kpo_stokes* stokes, int nfreq, pixel_result* out) {
+    double g[16];
+    metric_gcov(m, state->v, g);
+    out->r11 = dot_g(g, camera->v + 8, state->v + 8);
+    out->r12 = dot_g(g, camera->v + 8, state->v + 12);
+    out->r21 = dot_g(g, camera->v + 12, state->v + 8);
+    out->r22 = dot_g(g, camera->v + 12, state->v + 12);
+    for (int f = 0; f < nfreq; ++f)
+        kpo_transform_to_observer_basis(&stokes[f], out->r11, out->r12, out->r21, out->r22, &out->observed[f]);
+    out->closure_x = spatial_distance(state->v, camera->v);
+    out->closure_k = max_abs_diff4(state->v + 4, camera->v + 4);
+    out->final_null = dot_g(g, state->v + 4, state->v + 4);
+    out->frame_error = max_frame_error(m, state);
+}
+
+/* advance_slow_light_vacuum_to_camera (slow_light_driver.hpp:1572-1671): a
+ * ray whose fluid time ran out may finish only through outward vacuum. */
+static int slow_vacuum_to_camera(const kp_pass_a_params* p, const metric_ctx* m, kpo_state* state,
+                                 double* h_current, int* steps) {
+    const double tol_r = 1e-8 * maxv(1.0, absv(p->outer_radius));
+    if (radius_of(m, state->v) < p->outer_radius - tol_r) return KP_REASON_SLOW_LIGHT_TIME_EXHAUSTED;
+    const adaptive_control control = {p->adaptive_tolerance, p->min_step, p->max_step, 10};
+    while (*steps < p->max_steps) {
+        double h = *h_current;
+        const kpo_state old = *state;
+        kpo_state next, sample;
+        if (p->adaptive) {
+            adaptive_result pr;
+            adaptive_rk4(m, &old, h, control, &pr);
+            if (!pr.accepted) {
+                *h_current = absv(pr.next_h);
+                if (*h_current <= p->min_step * 1.0001) return KP_REASON_MAX_STEPS;
+                continue;
+            }
+            h = absv(pr.used_h);
+            *h_current = absv(pr.next_h);
+            next = pr.state;
+            sample = pr.mid_state;
+        } else {
+            rk4(m, &old, h, &next);
+            sample = next;
+        }
+        const double s0 = radius_of(m, old.v) - p->camera.radius;
+        const double s1 = radius_of(m, next.v) - p->camera.radius;
+        const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+        if (crosses) {
+            const double denom = s0 - s1;
+            const double frac = absv(denom) <= TINY ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
+            const double h_cross = h * frac;
+            if (absv(h_cross) <= 1e-14) {
+                h = 0.0;
+                next = old;
+                sample = old;
+            } else if (frac < 0.999999999999) {
+                h = h_cross;
+                if (p->adaptive) {
+                    adaptive_control cc = control;
+                    cc.min_step = minv(cc.min_step, absv(h));
+                    cc.max_step = maxv(cc.min_step, absv(h));
+                    adaptive_result cr;
+                    adaptive_rk4(m, &old, h, cc, &cr);
+                    if (!cr.accepted) {
+                        *h_current = absv(cr.next_h);
+                        continue;
+                    }
+                    next = cr.state;
+                    sample = cr.mid_state;
+                } else {
+                    rk4(m, &old, h, &next);
+                    sample = next;
+                }
+            }
+        }
+        const double old_r = radius_of(m, old.v), next_r = radius_of(m, next.v), sample_r = radius_of(m, sample.v);
+        if (!isfinite(next_r) || !isfinite(sample_r)) return KP_REASON_MAX_STEPS;
+        if (next_r < old_r - tol_r || sample_r < p->outer_radius - tol_r || next_r < p->outer_radius - tol_r)
+            return KP_REASON_SLOW_LIGHT_TIME_EXHAUSTED;
+        *state = next;
+        *steps += 1;
+        if (crosses) return KP_REASON_REACHED_CAMERA;
+    }
+    return KP_REASON_MAX_STEPS;
+}
+
+/* One pixel of a slow-light image: the window kernel
+ * (launch_slow_light_rays_metric, slow_light_driver.hpp:1083-1483) applied to
+ * every window in turn -- each ray's windows are independent of every other
+ * ray's, so one pass over them per ray equals the per-window launches -- then
+ * the vacuum finish of a ray still unfinished after the last window
+ * (finalize_slow_light_unfinished_metric, :1674-1769).  Windows: snapshot
+ * stepping runs one window per snapshot pair (k, k+1), cuts steps at t_k+1 and
+ * samples that pair (SlowLightSingleWindow, :966-981); block stepping runs one
+ * window cut at the last time and samples the pair enclosing each sample's
+ * time; decoupled stepping cuts nothing and a radiating sample outside the
+ * sequence leaves the ray waiting (SlowLightResidentWindows, :983-1027). */
+static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, const kp_slow_light* seq,
+                        const freq_plan* fp, const metric_ctx* m, const kpo_state* camera,
+                        const kpo_state* endpoint, int pass_a_steps, int pass_a_reason, int valid,
+                        pixel_result* out, kpo_stats* stats) {
+    memset(out, 0, sizeof(*out));
+    out->r11 = 1.0;
+    out->r22 = 1.0;
+    out->pass_a_steps = pass_a_steps;
+    out->reason = pass_a_reason;
+    kpo_state state = *endpoint;
+    if (!valid) {
+        /* initialize_slow_light_states_metric (:362-458) */
+        out->closure_x = spatial_distance(state.v, camera->v);
+        out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
+        double gg[16];
+        metric_gcov(m, state.v, gg);
+        out->final_null = dot_g(gg, state.v + 4, state.v + 4);
+        out->frame_error = max_frame_error(m, &state);
+        return;
+    }
+    const int mode = seq->step_mode;
+    const int decoupled = mode == KP_SLOW_LIGHT_DECOUPLED;
+    const double* T = seq->times;
+    const int n = seq->nsnap;
+    const double t_obs = seq->observation_time;
+    const adaptive_control control = {p->adaptive_tolerance, p->min_step, p->max_step, 10};
+    kpo_stokes stokes[KPO_MAX_FREQUENCIES];
+    memset(stokes, 0, sizeof(stokes));
+    double h_current = p->step;
+    double cap = p->max_radiation_step > 0.0 ? p->max_radiation_step : p->max_step;
+    int steps = 0, reached = 0, active = 1, reason = pass_a_reason;
+    const int nwin = mode == KP_SLOW_LIGHT_SNAPSHOT ? n - 1 : 1;
+    for (int w = 0; w < nwin; ++w) {
+        if (reached || !active) break;
+        const double upper = mode == KP_SLOW_LIGHT_SNAPSHOT ? T[w + 1] : T[n - 1];
+        while (!reached && steps < p->max_steps) {
+            const double retry_h = h_current;
+            double h = h_current;
+            const kpo_state old = state;
+            const double old_t = t_obs + old.v[0];
+            if (!decoupled && old_t >= upper) break;
+            const double start_r = radius_of(m, old.v);
+            if (start_r <= p->outer_radius && cap > 0.0) h = minv(h, cap);
+            kpo_state next, sample;
+            if (p->adaptive) {
+                adaptive_result pr;
+                adaptive_rk4(m, &old, h, control, &pr);
+                if (!pr.accepted) {
+                    h_current = absv(pr.next_h);
+                    if (h_current <= p->min_step * 1.0001) {
+                        reason = KP_REASON_MAX_STEPS;
+                        active = 0;
+                        break;
+                    }
+                    continue;
+                }
+                h = absv(pr.used_h);
+                h_current = absv(pr.next_h);
+                next = pr.state;
+                sample = pr.mid_state;
+            } else {
+                rk4(m, &old, h, &next);
+                sample = next;
+            }
+            const double s0 = radius_of(m, old.v) - p->camera.radius;
+            const double s1 = radius_of(m, next.v) - p->camera.radius;
+            int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+            double camera_frac = 2.0;
+            if (crosses) {
+                const double denom = s0 - s1;
+                camera_frac = absv(denom) <= TINY ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
+            }
+            const double next_t = t_obs + next.v[0];
+            const int crosses_time = !decoupled && next_t > upper && next_t > old_t;
+            double time_frac = 2.0;
+            if (crosses_time) time_frac = minv(1.0, maxv(0.0, (upper - old_t) / (next_t - old_t)));
+            if (crosses_time && (!crosses || time_frac < camera_frac)) {
+                crosses = 0;
+                const double h_b = h * time_frac;
+                if (absv(h_b) <= 1e-14) break;
+                h = h_b;
+                if (p->adaptive) {
+                    adaptive_control bc = control;
+                    bc.min_step = minv(bc.min_step, absv(h));
+                    bc.max_step = maxv(bc.min_step, absv(h));
+                    adaptive_result br;
+                    adaptive_rk4(m, &old, h, bc, &br);
+                    if (!br.accepted) {
+                        h_current = absv(br.next_h);
+                        continue;
+                    }
+                    h = absv(br.used_h);
+                    next = br.state;
+                    sample = br.mid_state;
+                } else {
+                    rk4(m, &old, h, &next);
+                    sample = next;
+                }
+                if (next.v[0] == old.v[0]) {
+                    const double time_scale = maxv(absv(old.v[0]), maxv(absv(upper), absv(old_t - old.v[0])));
+                    const double rounding = 0.5 * 8.0 * 2.220446049250313080847263336181640625e-16 * time_scale;
+                    if (absv(upper - old_t) > rounding) {
+                        reason = KP_REASON_ADAPTIVE_STEP_UNDERFLOW;
+                        active = 0;
+                    }
+                    break;
+                }
+            } else if (crosses) {
+                const double h_cross = h * camera_frac;
+                if (absv(h_cross) <= 1e-14) {
+                    h = 0.0;
+                    next = old;
+                    sample = old;
+                } else if (camera_frac < 0.999999999999) {
+                    h = h_cross;
+                    if (p->adaptive) {
+                        adaptive_control cc = control;
+                        cc.min_step = minv(cc.min_step, absv(h));
+                        cc.max_step = maxv(cc.min_step, absv(h));
+                        adaptive_result cr;
+                        adaptive_rk4(m, &old, h, cc, &cr);
+                        if (!cr.accepted) {
+                            h_current = absv(cr.next_h);
+                            continue;
+                        }
+                        h = absv(cr.used_h);
+                        next = cr.state;
+                        sample = cr.mid_state;
+                    } else {
+                        rk4(m, &old, h, &next);
+                        sample = next;
+                    }
+                }
+            }
+            const double old_r = radius_of(m, old.v), sample_r = radius_of(m, sample.v),
+                         next_r = radius_of(m, next.v);
+            const int radiating = sample_r >= p->inner_radius && sample_r < p->outer_radius;
+            const double ct = t_obs + sample.v[0];
+            if (decoupled && radiating && !(ct >= T[0] && ct <= T[n - 1])) {
+                out->requested_time = ct;
+                out->waiting = 1;
+                h_current = retry_h;
+                break;
+            }
+            const int pair = mode == KP_SLOW_LIGHT_SNAPSHOT ? w : kpo_slow_light_pair(T, n, ct);
+            const double local_max_rs =
+                (p->max_radiation_step > 0.0 &&
+                 (old_r <= p->outer_radius || sample_r <= p->outer_radius || next_r <= p->outer_radius))
+                    ? p->max_radiation_step
+                    : 0.0;
+            kpo_coeffs cc[KPO_MAX_FREQUENCIES];
+            double dls[KPO_MAX_FREQUENCIES];
+            int required = 1, predictive = 1;
+            for (int f = 0; f < fp->nfreq; ++f) {
+                kp_grmhd_model gf = *g;
+                gf.freq_cgs = fp->freqs[f];
+                dls[f] = kpo_grmhd_dlambda_scale(&gf);
+                memset(&cc[f], 0, sizeof(cc[f]));
+                if (radiating) {
+                    if (stats) stats->coefficient_evals++;
+                    kpo_slow_light_coefficients(&gf, seq, pair, &sample, NULL,
+                                                p->emission_selection.faraday_rotation, &cc[f]);
+                }
+                const int a = kpo_radiation_substep_count(&cc[f], h, dls[f], 1, local_max_rs, p->max_radiation_depth,
+                                                          p->max_absorption_depth, p->max_faraday_depth);
+                required = a > required ? a : required;
+                const int b = kpo_radiation_substep_count(&cc[f], h_current, dls[f], 1, local_max_rs,
+                                                          p->max_radiation_depth, p->max_absorption_depth,
+                                                          p->max_faraday_depth);
+                predictive = b > predictive ? b : predictive;
+            }
+            if (required > 1 && h > p->min_step * 1.0001) {
+                h_current = maxv(p->min_step, h / (double)required);
+                cap = h_current;
+                continue;
+            }
+            cap = predictive > 1 ? maxv(p->min_step, h_current / (double)predictive) : p->max_step;
+            if (h > 0.0) {
+                if (stats) stats->transfer_steps++;
+                for (int f = 0; f < fp->nfreq; ++f) kpo_semi_analytic_step(&stokes[f], &cc[f], absv(h) * dls[f]);
+            }
+            state = next;
+            steps += 1;
+            if (crosses) reached = 1;
+        }
+        if (out->waiting) break;
+        if (reached || steps >= p->max_steps || !active) {
+            slow_finish(m, camera, &state, stokes, fp->nfreq, out);
+            out->steps = steps;
+            if (reached)
+                out->reason = KP_REASON_REACHED_CAMERA;
+            else if (reason != KP_REASON_ADAPTIVE_STEP_UNDERFLOW)
+                out->reason = KP_REASON_MAX_STEPS;
+            else
+                out->reason = reason;
+            return;
+        }
+    }
+    if (out->waiting) {
+        /* all snapshots are resident: no later block supplies the time */
+        slow_finish(m, camera, &state, stokes, fp->nfreq, out);
+        out->steps = steps;
+        out->reason = KP_REASON_SLOW_LIGHT_TIME_EXHAUSTED;
+        return;
+    }
+    const int vreason = slow_vacuum_to_camera(p, m, &state, &h_current, &steps);
+    slow_finish(m, camera, &state, stokes, fp->nfreq, out);
+    out->steps = steps;
+    out->reason = vreason;
+}
+
 /* Stokes arrays are [nfreq][nout] (frequency-major, src/image/multifrequency.hpp). */
 static void store_pixel(kp_image_data* o, int p, int nout, int nfreq, const pixel_result* r) {
 #define PUT(field, val) do { if (o->field) o->field[p] = (val); } while (0)
@@ -2477,10 +2890,10 @@ static void store_pixel(kp_image_data* o, int p, int nout, int nfreq, const pixe
  * i (compact) or at the pixel index.  Split pass A / pass B per pixel
  * (src/image/split_driver.hpp:41-104); OpenMP static schedule as the
  * reference's RangePolicy. */
-static kp_status run_pixels(const kp_pass_a_params* p, const model_ctx* model,
-                            const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
-                            int count, int compact, int nout, int nthreads, kp_image_data* out,
-                            kpo_stats* stats) {
+static kp_status run_pixels_w(const kp_pass_a_params* p, const model_ctx* model,
+                              const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
+                              int count, int compact, int nout, int nthreads, kp_image_data* out,
+                              kpo_stats* stats, double* waiting) {
     double tetrad[16], gcam[16];
     double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
     const int fmks = p->coordinate_system == KP_COORD_FMKS;
@@ -2508,7 +2921,23 @@ static kp_status run_pixels(const kp_pass_a_params* p, const model_ctx* model,
         int32_t a_steps = 0, a_reason = 0;
         const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
         pixel_result r;
-        pass_b(p, model, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
+        if (model->slow)
+            pass_b_slow(p, model->grmhd, model->slow, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r,
+                        &local);
+        else
+            pass_b(p, model, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
+        if (r.waiting) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            {
+                if (waiting) {
+                    waiting[0] += 1.0;
+                    waiting[1] = minv(waiting[1], r.requested_time);
+                    waiting[2] = maxv(waiting[2], r.requested_time);
+                }
+            }
+        }
         store_pixel(out, compact ? i : pix, nout, fp->nfreq, &r);
         rhs_total += rhs_count;
         coeff_total += local.coefficient_evals;
@@ -2524,6 +2953,13 @@ static kp_status run_pixels(const kp_pass_a_params* p, const model_ctx* model,
         stats->pass_b_steps = b_total;
     }
     return KP_OK;
+}
+
+static kp_status run_pixels(const kp_pass_a_params* p, const model_ctx* model,
+                            const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
+                            int count, int compact, int nout, int nthreads, kp_image_data* out,
+                            kpo_stats* stats) {
+    return run_pixels_w(p, model, pert, fp, first, stride, count, compact, nout, nthreads, out, stats, NULL);
 }
 
 static kp_status check_params(const kp_pass_a_params* p, const kp_riaf_model* model, const void* out) {
@@ -2544,7 +2980,7 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
     const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
     const double f0 = model->freq_cgs;
     const freq_plan fp = {&f0, 1, &f0, 1, 1};
-    const model_ctx mc = {model, NULL};
+    const model_ctx mc = {model, NULL, NULL};
     return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
                       out, stats);
 }
@@ -2563,7 +2999,7 @@ kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* p, const kp_riaf_mo
         return KP_INVALID_ARGUMENT;
     const double f0 = model->freq_cgs;
     const freq_plan fp = {&f0, 1, &f0, 1, 1};
-    const model_ctx mc = {model, NULL};
+    const model_ctx mc = {model, NULL, NULL};
     return run_pixels(p, &mc, pert, &fp, offset, stride, count, 1, count, nthreads, out, stats);
 }
 
@@ -2587,7 +3023,7 @@ kp_status kpo_run_riaf_multifrequency_image(const kp_pass_a_params* p, const kp_
     const int npix = p->camera.nx * p->camera.ny;
     const int begin = pixel_begin < 0 ? 0 : pixel_begin;
     const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
-    const model_ctx mc = {model, NULL};
+    const model_ctx mc = {model, NULL, NULL};
     return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
                       out, stats);
 }
@@ -2618,7 +3054,37 @@ kp_status kpo_run_grmhd_image(const kp_pass_a_params* p, const kp_grmhd_model* g
     const int npix = p->camera.nx * p->camera.ny;
     const int begin = pixel_begin < 0 ? 0 : pixel_begin;
     const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
-    const model_ctx mc = {NULL, g};
+    const model_ctx mc = {NULL, g, NULL};
     return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads, out,
                       stats);
+}
+
+kp_status kpo_run_grmhd_slow_light_image(const kp_pass_a_params* p, const kp_grmhd_model* g,
+                                         const kp_slow_light* seq, const double* freqs, int32_t nfreq,
+                                         int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
+                                         kp_image_data* out, kpo_stats* stats, double waiting[3]) {
+    if (!p || !g || !seq || !out || !seq->times || !seq->cells || seq->nsnap < 2) return KP_INVALID_ARGUMENT;
+    if (p->coordinate_system != KP_COORD_FMKS || p->camera.model != KP_CAMERA_PINHOLE) return KP_UNSUPPORTED;
+    for (int k = 0; k + 1 < seq->nsnap; ++k)
+        if (!(seq->times[k + 1] > seq->times[k])) return KP_INVALID_ARGUMENT;
+    const double f0 = g->freq_cgs;
+    if (!freqs) {
+        freqs = &f0;
+        nfreq = 1;
+    }
+    if (nfreq < 1 || nfreq > KPO_MAX_FREQUENCIES) return KP_INVALID_ARGUMENT;
+    const freq_plan fp = {freqs, nfreq, freqs, nfreq, 1};
+    const int npix = p->camera.nx * p->camera.ny;
+    const int begin = pixel_begin < 0 ? 0 : pixel_begin;
+    const int end = (pixel_end < 0 || pixel_end > npix) ? npix : pixel_end;
+    const model_ctx mc = {NULL, g, seq};
+    if (waiting) {
+        waiting[0] = 0.0;
+        waiting[1] = LARGE;
+        waiting[2] = -LARGE;
+    }
+    const kp_status st = run_pixels_w(p, &mc, NULL, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix,
+                                      nthreads, out, stats, waiting);
+    if (st != KP_OK) return st;
+    return (waiting && waiting[0] > 0.0) ? KP_TIME_SUPPORT : KP_OK;
 }

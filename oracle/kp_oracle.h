@@ -152,6 +152,21 @@ kp_status kpo_run_grmhd_image(const kp_pass_a_params* params, const kp_grmhd_mod
                               const double* ctrl, int32_t nctrl, int32_t pixel_begin, int32_t pixel_end,
                               int32_t nthreads, kp_image_data* out, kpo_stats* stats);
 
+/* ---- Slow light -------------------------------------------------------- */
+double kpo_interpolate_time_value(double a, double b, double fraction);
+double kpo_slow_light_fraction(double t_lo, double t_hi, double t_obs, double x0);
+int32_t kpo_slow_light_pair(const double* times, int32_t nsnap, double t);
+void kpo_slow_light_coefficients(const kp_grmhd_model* model, const kp_slow_light* sequence, int32_t pair,
+                                 const kpo_state* s, const kpo_perturbation* pert, int faraday_rotation,
+                                 kpo_coeffs* out);
+/* Slow-light GRMHD image; sequence->cells are HOST arrays.  waiting (may be
+ * NULL) receives {count, min, max} of the times decoupled stepping needed
+ * outside the sequence; KP_TIME_SUPPORT when the count is nonzero. */
+kp_status kpo_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                         const kp_slow_light* sequence, const double* freqs, int32_t nfreq,
+                                         int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
+                                         kp_image_data* out, kpo_stats* stats, double waiting[3]);
+
 #ifdef __cplusplus
 }
 #endif

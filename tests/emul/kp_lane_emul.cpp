@@ -11,17 +11,31 @@ template <bool MIXED, int MODE>
 static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, double fp64_radius,
                           const FreqLists& fl, kp_image_data* out, int64_t* rhs_evals, int64_t* guard_steps,
                           const double* log_scales = nullptr, double* grad = nullptr,
-                          const kp_grmhd_model* gm = nullptr) {
+                          const kp_grmhd_model* gm = nullptr, const kp_slow_light* slow = nullptr,
+                          double* req_t = nullptr, unsigned long long* waiting = nullptr) {
+    constexpr int GP = MIXED ? GP_GUARD : GP_64;
+    constexpr bool FM = MODE == M_GRMHD || MODE == M_SLOW;
     Params P;
-    kp_status st = MODE == M_GRMHD ? prepare_grmhd(pa, gm, P) : prepare(pa, m, P);
+    kp_status st = FM ? prepare_grmhd(pa, gm, P) : prepare(pa, m, P);
     if (st != KP_OK) return st;
     if (MIXED) {
         prepare_mixed(P);
         P.fp64_radius = fp64_radius;
     }
     if (lds_stokes<MODE>()) {
-        st = prepare_frequencies(MODE == M_GRMHD ? gm->mbh_solar : m->mbh_solar, fl, P);
+        st = prepare_frequencies(FM ? gm->mbh_solar : m->mbh_solar, fl, P);
         if (st != KP_OK) return st;
+    }
+    unsigned long long counters[C_COUNT] = {};
+    P.counters = counters;
+    if (MODE == M_SLOW) {
+        P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(slow->cells);
+        P.snap_t = slow->times;
+        P.nsnap = slow->nsnap;
+        P.slow_mode = slow->step_mode;
+        P.slow_interp = slow->interpolation;
+        P.t_obs = slow->observation_time;
+        P.req_t = req_t;
     }
     if (MODE == M_TANGENT) {
         P.riaf.ne_factor = std::exp(log_scales[0]);
@@ -44,13 +58,13 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         const auto G = GeomT<geom_of<MODE>()>::geo(P, &rc);
         LaneA A;
         lane_a_init(P, pix, A);
-        while (!pass_a_tick<MIXED, geom_of<MODE>()>(P, G, A, sa, guard)) {
+        while (!pass_a_tick<GP, geom_of<MODE>()>(P, G, A, sa, guard)) {
         }
         lane_a_store(P, A);
         LaneB B;
         B.mf = LdsVec<double>{mfbuf, 1};
         if (lane_b_init<MODE>(P, pix, B)) {
-            while (!pass_b_tick<MIXED, MODE>(P, G, B, c1, c2, sb, guard)) {
+            while (!pass_b_tick<MIXED, GP, MODE>(P, G, B, c1, c2, sb, guard)) {
             }
             lane_b_finish<MIXED, MODE>(P, B, 1);
         }
@@ -59,6 +73,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
     }
     if (rhs_evals) *rhs_evals = total;
     if (guard_steps) *guard_steps = guard_total;
+    if (waiting) *waiting = counters[C_WAIT];
     return KP_OK;
 }
 
@@ -135,4 +150,19 @@ extern "C" void kp_emul_fmks_rhs(double mass, double spin, const double fm[6], c
     St<double> y;
     for (int i = 0; i < 16; ++i) y.v[i] = s[i];
     fmks_rhs_apply(mass, spin, f, y, [&](int i, double v) { d[i] = v; });
+}
+
+// Slow-light lanes over HOST snapshot records; req_t [npix] receives the
+// time each waiting ray needed (NaN for the others), waiting their count.
+extern "C" kp_status kp_emul_run_grmhd_slow_light_image(const kp_pass_a_params* pa, const kp_grmhd_model* gm,
+                                                        const kp_slow_light* seq, const double* freqs,
+                                                        int32_t nfreq, kp_image_data* out, double* req_t,
+                                                        unsigned long long* waiting) {
+    FreqLists fl;
+    fl.nfreq = fl.nctrl = nfreq;
+    fl.freqs = fl.ctrl = freqs;
+    kp_grmhd_model g = *gm;
+    g.cells = seq->cells[0];
+    return emul_run<false, M_SLOW>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, nullptr, nullptr, &g, seq, req_t,
+                                   waiting);
 }

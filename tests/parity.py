@@ -149,6 +149,60 @@ def assert_mixed(got, ref, label, pol="relative"):
     return e
 
 
+def sensitivity(run, p, ref):
+    """Per-pixel ill-conditioning of an oracle image from a runner
+    run(params) -> image: the pixel moves by more than SENSITIVITY of its
+    value (floor LOOSE_FLOOR of the frequency's peak) when the ray slope moves
+    by PERTURBATION in x or y, or its step counts change."""
+    n = ref["reason"].size
+    nfreq = ref["image_i"].size // max(n, 1)
+    ill = np.zeros(n, dtype=bool)
+    for dx, dy in ((PERTURBATION, 0.0), (0.0, PERTURBATION)):
+        q = copy.deepcopy(p)
+        q.camera.x_offset += dx
+        q.camera.y_offset += dy
+        r2 = run(q)
+        ill |= (r2["steps"] != ref["steps"]) | (r2["pass_a_steps"] != ref["pass_a_steps"])
+        for f in range(nfreq):
+            sl = slice(f * n, (f + 1) * n)
+            peak = float(np.max(np.abs(ref["image_i"][sl]))) if n else 0.0
+            for name in STOKES:
+                d = np.abs(r2[name][sl] - ref[name][sl])
+                ill |= d > SENSITIVITY * (np.abs(ref[name][sl]) + LOOSE_FLOOR * peak)
+    return ill
+
+
+def assert_parity_grid(got, ref, nfreq, label, max_step_diff=None, ill=None):
+    """Parity of a GRMHD (grid or snapshot-sequence) image, [nfreq][npix]
+    Stokes: identical termination reasons; on rays whose step counts agree,
+    the per-pixel bar (1e-10 relative + 1e-8 of the frequency's peak I); rays
+    whose step sequence moved (a chaotic near-photon-ring ray, or a slow-light
+    ray approaching a snapshot time by ever smaller cut steps), and the
+    ill-conditioned rays `ill` of the oracle's own sensitivity, within 1e-3
+    (+1e-6 of peak).  max_step_diff bounds |steps - steps_ref| when given."""
+    assert np.array_equal(got["reason"], ref["reason"]), (
+        f"{label}: reasons differ at {np.flatnonzero(got['reason'] != ref['reason'])[:10]}")
+    moved = (got["steps"] != ref["steps"]) | (got["pass_a_steps"] != ref["pass_a_steps"])
+    if ill is not None:
+        moved = moved | ill
+    if max_step_diff is not None:
+        assert np.max(np.abs(got["steps"] - ref["steps"]), initial=0) <= max_step_diff, label
+    n = ref["reason"].size
+    worst = 0.0
+    for f in range(nfreq):
+        sl = slice(f * n, (f + 1) * n)
+        peak = float(np.max(np.abs(ref["image_i"][sl]))) if n else 0.0
+        for name in STOKES:
+            err = np.abs(got[name][sl] - ref[name][sl])
+            bad = err > REL * np.abs(ref[name][sl]) + PEAK_FLOOR * peak
+            assert not (bad & ~moved).any(), (label, f, name, np.flatnonzero(bad & ~moved)[:8])
+            loose = err > LOOSE_REL * np.abs(ref[name][sl]) + LOOSE_FLOOR * peak
+            assert not (loose & moved).any(), (label, f, name, np.flatnonzero(loose & moved)[:8])
+            if peak > 0.0:
+                worst = max(worst, float(np.max(err[~moved], initial=0.0)) / peak)
+    return {"pixels": n, "moved": int(moved.sum()), "max_err_over_peak_unmoved": worst}
+
+
 def split_frequencies(img, nfreq):
     """Per-frequency views of a multi-frequency image ([nfreq][npix] Stokes,
     per-pixel diagnostics shared)."""

@@ -1,0 +1,273 @@
+"""Slow light on the CPU: the oracle's retarded-time sampling pinned by the
+reference's own checks (fluid time interpolation: exact endpoints, the
+mid-time state equals a snapshot built from mid values,
+tests/test_fluid_time_interpolation.cpp:75-101) and by limits that fix the
+algorithm (identical snapshots reproduce the static image; bisection over
+the snapshot times against a brute-force search), then the kernel's lane code
+(tests/emul) against the oracle in every stepping and interpolation mode."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from kpolaris_amd import abi, grmhd_synth as gs
+from tests import parity
+
+P = C.POINTER
+STOKES = ("image_i", "image_q", "image_u", "image_v")
+# grid, camera and sequence of the lane tests: radiating samples of the
+# 12x12 image at camera radius 200 with outer radius 30 lie at fluid times
+# 275 + x^0 in [about 5, 95]
+T_OBS = 275.0
+
+
+@pytest.fixture(scope="module")
+def seq():
+    cells, times, model = gs.make_sequence(nsnap=6, dt=20.0, t0=0.0, n1=48, n2=32, n3=32)
+    for c in cells:
+        oracle.grmhd_derive_scalars(model, c)
+    p = gs.pass_a_params(model, nx=12, ny=12, camera_radius=200.0, outer_radius=30.0)
+    return cells, times, model, p
+
+
+@pytest.fixture(scope="module")
+def emul():
+    from tests.emul import build as emul_build
+    if not emul_build.LIB.exists():
+        emul_build.build()
+    L = C.CDLL(str(emul_build.LIB))
+    L.kp_emul_run_grmhd_slow_light_image.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
+                                                     P(C.c_double), C.c_int32, P(abi.ImageData), P(C.c_double),
+                                                     P(C.c_ulonglong)]
+    L.kp_emul_run_grmhd_slow_light_image.restype = C.c_int32
+    return L
+
+
+def _emul_run(L, p, model, cells, times, t_obs, mode, interp, freqs=None):
+    sq, keep = oracle.slow_light_sequence(cells, times, t_obs, mode, interp)
+    npix = p.camera.nx * p.camera.ny
+    f = np.array([model.freq_cgs] if freqs is None else freqs, dtype=np.float64)
+    arrays = oracle.alloc_image(npix)
+    for n in STOKES:
+        arrays[n] = np.zeros(len(f) * npix)
+    img = oracle.image_struct(arrays)
+    req = np.zeros(npix)
+    w = C.c_ulonglong()
+    assert L.kp_emul_run_grmhd_slow_light_image(C.byref(p), C.byref(model), C.byref(sq), f.ctypes.data_as(P(C.c_double)),
+                                                len(f), C.byref(img), req.ctypes.data_as(P(C.c_double)),
+                                                C.byref(w)) == 0
+    return arrays, req, w.value
+
+
+def test_interpolate_time_value_and_fraction():
+    """interpolate_time_value (time_interpolation.hpp:27-32): the endpoint
+    values come back bit for bit, equal values stay exact, the interior is
+    the linear interpolant; the fraction (slow_light_driver.hpp:62-69) is
+    clamped to [0, 1] and 0 for a zero-length interval."""
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        a, b = rng.normal(size=2) * 10.0 ** rng.integers(-30, 30, size=2)
+        assert L.kpo_interpolate_time_value(a, b, 0.0) == a
+        assert L.kpo_interpolate_time_value(a, b, -0.3) == a
+        assert L.kpo_interpolate_time_value(a, b, 1.0) == b
+        assert L.kpo_interpolate_time_value(a, b, 1.7) == b
+        assert L.kpo_interpolate_time_value(a, a, 0.37) == a
+        f = float(rng.uniform(0.01, 0.99))
+        v = L.kpo_interpolate_time_value(a, b, f)
+        assert abs(v - (a + f * (b - a))) <= 4e-16 * max(abs(a), abs(b))
+    assert L.kpo_slow_light_fraction(10.0, 20.0, 100.0, -85.0) == 0.5
+    assert L.kpo_slow_light_fraction(10.0, 20.0, 100.0, -95.0) == 0.0
+    assert L.kpo_slow_light_fraction(10.0, 20.0, 100.0, -75.0) == 1.0
+    assert L.kpo_slow_light_fraction(10.0, 10.0, 100.0, -85.0) == 0.0
+
+
+def test_snapshot_pair_bisection_matches_brute_force():
+    """sample_window (slow_light_driver.hpp:1000-1012) on nonuniform times:
+    the largest k < nsnap - 1 with t_k <= t (times before the first give 0)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(9)
+    for n in (2, 3, 7, 32):
+        times = np.cumsum(rng.uniform(0.1, 5.0, size=n))
+        tp = times.ctypes.data_as(P(C.c_double))
+        for t in np.concatenate([times, rng.uniform(times[0] - 3, times[-1] + 3, size=300)]):
+            brute = int(np.clip(np.searchsorted(times, t, side="right") - 1, 0, n - 2))
+            assert L.kpo_slow_light_pair(tp, n, float(t)) == brute, (n, t)
+
+
+def _state_at(model, rng):
+    """A transport state inside the torus with a random photon direction and basis."""
+    s = np.zeros(16)
+    s[1] = np.log(10.0 + rng.uniform(-2.0, 2.0))
+    s[2] = 0.5 + rng.uniform(-0.05, 0.05)
+    s[3] = rng.uniform(0.0, 2.0 * np.pi)
+    s[4:8] = [1.0, *(rng.normal(size=3) * 0.05)]
+    s[8:16] = rng.normal(size=8) * 0.1
+    return s
+
+
+def test_fluid_time_interpolation_mid_state_and_endpoints(seq):
+    """The reference's fluid time-interpolation checks
+    (tests/test_fluid_time_interpolation.cpp:75-101) on the oracle: with
+    snapshot B = 3 A in the primitives (so the float32 mid-time values 2 A are
+    exact) and C carrying the mid values of every primitive and derived
+    scalar, the coefficients at time fraction 1/2 equal C's static
+    coefficients; at the endpoints they are A's and B's bit for bit."""
+    cells, times, model, p = seq
+    L = oracle.lib()
+    A = cells[0].copy()
+    # 20-bit significands: 3 A and the mean 2 A are then exact in float32
+    A["prim"] = (A["prim"].view(np.uint32) & np.uint32(0xFFFFFFF0)).view(np.float32)
+    oracle.grmhd_derive_scalars(model, A)
+    B = A.copy()
+    B["prim"] = A["prim"] * np.float32(3.0)
+    oracle.grmhd_derive_scalars(model, B)
+    Cm = A.copy()
+    Cm["prim"] = A["prim"] * np.float32(2.0)
+    Cm["derived"] = 0.5 * (A["derived"] + B["derived"])
+    assert np.array_equal(Cm["prim"].astype(np.float64), 0.5 * (A["prim"].astype(np.float64) + B["prim"].astype(np.float64)))
+    rng = np.random.default_rng(3)
+    faraday = 1
+    nchecked = 0
+    for interp in (abi.SLOW_LIGHT_FLUID, abi.SLOW_LIGHT_COEFFICIENTS):
+        sq, keep = oracle.slow_light_sequence([A, B], [100.0, 140.0], 300.0, abi.SLOW_LIGHT_DECOUPLED, interp)
+        for _ in range(40):
+            s = _state_at(model, rng)
+            st = oracle.state(s)
+            out = {}
+            for name, x0 in (("lo", -200.0), ("mid", -180.0), ("hi", -160.0)):
+                st.v[0] = x0
+                c = oracle.Coeffs()
+                L.kpo_slow_light_coefficients(C.byref(model), C.byref(sq), 0, C.byref(st), None, faraday, C.byref(c))
+                out[name] = c.as_array()
+            ref = {}
+            for name, cc in (("A", A), ("B", B), ("C", Cm)):
+                m = abi.GRMHDModel.from_buffer_copy(model)
+                m.cells = C.c_void_p(cc.ctypes.data)
+                c = oracle.Coeffs()
+                L.kpo_grmhd_coefficients(C.byref(m), C.byref(st), None, faraday, C.byref(c))
+                ref[name] = c.as_array()
+            assert np.array_equal(out["lo"], ref["A"]), interp
+            assert np.array_equal(out["hi"], ref["B"]), interp
+            if interp == abi.SLOW_LIGHT_FLUID and np.any(ref["C"] != 0.0):
+                scale = np.max(np.abs(ref["C"]))
+                assert np.max(np.abs(out["mid"] - ref["C"])) <= 1e-10 * scale
+                nchecked += 1
+            if interp == abi.SLOW_LIGHT_COEFFICIENTS:
+                # the coefficient mode is the linear mix of the endpoint coefficients
+                scale = max(np.max(np.abs(ref["A"])), np.max(np.abs(ref["B"])), 1e-300)
+                assert np.max(np.abs(out["mid"] - 0.5 * (ref["A"] + ref["B"]))) <= 1e-14 * scale
+    assert nchecked >= 10
+
+
+def test_fluid_time_interpolation_four_velocity_normalisation(seq):
+    """u.u = -1 of the four-velocity rebuilt from time-interpolated velocity
+    primitives (test_fluid_time_interpolation.cpp:92-101): the oracle's fluid
+    reconstruction of a mid-time state is the static reconstruction of the mid
+    primitives, whose normalisation the GRMHD pins check; here the mid state
+    must not lose the coefficients' structure (nonzero emission where both
+    endpoint snapshots emit)."""
+    cells, times, model, p = seq
+    L = oracle.lib()
+    sq, keep = oracle.slow_light_sequence(cells[:2], times[:2], 300.0, abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_FLUID)
+    rng = np.random.default_rng(11)
+    emitting = 0
+    for _ in range(40):
+        st = oracle.state(_state_at(model, rng))
+        vals = []
+        for x0 in (-300.0, -290.0, -280.0):
+            st.v[0] = x0
+            c = oracle.Coeffs()
+            L.kpo_slow_light_coefficients(C.byref(model), C.byref(sq), 0, C.byref(st), None, 1, C.byref(c))
+            vals.append(c.jI)
+        if vals[0] > 0.0 and vals[2] > 0.0:
+            assert vals[1] > 0.0
+            assert min(vals[0], vals[2]) * 0.2 < vals[1] < max(vals[0], vals[2]) * 5.0
+            emitting += 1
+    assert emitting >= 10
+
+
+def test_identical_snapshots_reproduce_the_static_image(seq):
+    """A sequence of one repeated snapshot is the static model: decoupled and
+    block stepping with fluid interpolation give the static GRMHD image bit
+    for bit (interpolate_time_value returns equal values exactly and no step is
+    cut); snapshot stepping cuts steps at the interior snapshot times, which
+    moves the image by the integration tolerance only."""
+    cells, times, model, p = seq
+    same = [cells[0]] * 3
+    ref, _ = oracle.run_grmhd_image(p, abi.GRMHDModel.from_buffer_copy(model), cells[0])
+    peak = float(np.max(ref["image_i"]))
+    for mode in (abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_BLOCK, abi.SLOW_LIGHT_SNAPSHOT):
+        got, _, w = oracle.run_grmhd_slow_light_image(p, model, same, [-2000.0, 0.0, 2000.0], 0.0, step_mode=mode)
+        assert w[0] == 0
+        assert np.array_equal(got["reason"], ref["reason"]), mode
+        if mode == abi.SLOW_LIGHT_SNAPSHOT:
+            for k in STOKES:
+                assert np.max(np.abs(got[k] - ref[k])) <= 1e-8 * peak, (mode, k)
+        else:
+            for k in (*STOKES, "steps"):
+                assert np.array_equal(got[k], ref[k]), (mode, k)
+
+
+def test_decoupled_time_support_error_and_snapshot_vacuum_finish(seq):
+    """Decoupled stepping with radiating samples before the first snapshot
+    leaves those rays waiting (reason 6) and reports the requested range
+    (slow_light_batch_driver.hpp:282-290); snapshot stepping with the sequence
+    ending before the rays leave the emitting region ends them as
+    slow_light_time_exhausted after the last window (advance_slow_light_vacuum_to_camera,
+    slow_light_driver.hpp:1580-1584)."""
+    cells, times, model, p = seq
+    got, _, w = oracle.run_grmhd_slow_light_image(p, model, cells, times, 230.0)
+    assert w[0] > 0 and w[1] < times[0]
+    assert np.all(got["reason"][got["reason"] != abi.REASON_REACHED_CAMERA] == abi.REASON_SLOW_LIGHT_TIME_EXHAUSTED)
+    assert np.sum(got["reason"] == abi.REASON_SLOW_LIGHT_TIME_EXHAUSTED) == w[0]
+    late, _, w2 = oracle.run_grmhd_slow_light_image(p, model, cells[:3], times[:3], T_OBS,
+                                                    step_mode=abi.SLOW_LIGHT_SNAPSHOT)
+    assert w2[0] == 0
+    assert np.sum(late["reason"] == abi.REASON_SLOW_LIGHT_TIME_EXHAUSTED) > 0
+    ok, _, w3 = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS)
+    assert w3[0] == 0 and np.all(ok["reason"] == abi.REASON_REACHED_CAMERA)
+
+
+@pytest.mark.parametrize("mode", [abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_SNAPSHOT, abi.SLOW_LIGHT_BLOCK])
+@pytest.mark.parametrize("interp", [abi.SLOW_LIGHT_FLUID, abi.SLOW_LIGHT_COEFFICIENTS])
+def test_emulated_slow_light_lanes_match_oracle(seq, emul, mode, interp):
+    """The kernel's slow-light lanes (window cuts, pair bisection, vacuum
+    finish) against the oracle: identical termination reasons, step counts
+    within one (a ray that approaches a snapshot time by ever smaller cut steps
+    stops when the remaining step is below 1e-14; where that happens depends on
+    the last bits of x^0), Stokes at the parity bar."""
+    cells, times, model, p = seq
+    got, req, w = _emul_run(emul, p, model, cells, times, T_OBS, mode, interp)
+    ref, _, wo = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS, step_mode=mode,
+                                                  interpolation=interp)
+    assert w == wo[0] == 0
+    assert np.all(np.isnan(req))
+    ill = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
+        q, model, cells, times, T_OBS, step_mode=mode, interpolation=interp)[0], p, ref)
+    print(parity.assert_parity_grid(got, ref, 1, f"emulated slow light mode {mode} interp {interp}",
+                                    max_step_diff=1, ill=ill), "ill", int(ill.sum()), flush=True)
+
+
+def test_emulated_slow_light_waiting_rays_and_frequencies(seq, emul):
+    """Waiting rays carry the time they needed; two frequencies on shared
+    geodesics match the oracle."""
+    cells, times, model, p = seq
+    got, req, w = _emul_run(emul, p, model, cells, times, 230.0, abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_FLUID)
+    ref, _, wo = oracle.run_grmhd_slow_light_image(p, model, cells, times, 230.0)
+    assert w == wo[0] > 0
+    waiting = ~np.isnan(req)
+    assert waiting.sum() == w
+    assert np.all(got["reason"][waiting] == abi.REASON_SLOW_LIGHT_TIME_EXHAUSTED)
+    assert np.isclose(req[waiting].min(), wo[1], rtol=0, atol=1e-9)
+    assert np.isclose(req[waiting].max(), wo[2], rtol=0, atol=1e-9)
+    freqs = [86e9, 230e9]
+    got2, _, _ = _emul_run(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_FLUID, freqs)
+    ref2, _, _ = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS, frequencies=freqs)
+    ill = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
+        q, model, cells, times, T_OBS, frequencies=freqs)[0], p, ref2)
+    print(parity.assert_parity_grid(got2, ref2, 2, "emulated slow light 2 freqs", max_step_diff=1, ill=ill),
+          flush=True)
