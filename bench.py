@@ -203,6 +203,97 @@ def bench_grmhd(args) -> int:
     return 0
 
 
+def bench_slow(args) -> int:
+    """Slow light: 32 snapshots of a seeded 192x96x96 FMKS torus (the GRMHD
+    workload's torus at this resolution, advected by Keplerian rotation,
+    dt = 5 GM/c^3), all resident, sampled at retarded time with decoupled
+    stepping and fluid interpolation for a 256x256 image at 230 GHz; FP64.
+    The radiating region is r < 40 M, the observation time is chosen by the
+    time probe so that the radiating samples sit in the middle of the
+    sequence.  Reports the grid-sample traffic of pass B (two snapshots x 8
+    cell records x 72 bytes per radiating step) against HBM bandwidth."""
+    import numpy as np
+    import torch
+    from kpolaris_amd import grmhd_synth as gs, image
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    cells, times, m = gs.make_sequence(nsnap=32, dt=5.0, t0=0.0, n1=192, n2=96, n3=96)
+    gen_s = time.perf_counter() - t0
+    grids = [image.GRMHDGrid(m, c) for c in cells]
+    del cells
+    torch.cuda.synchronize()
+    p = gs.pass_a_params(m, nx=256, ny=256, outer_radius=40.0)
+    pr = image.slow_light_time_probe(p, m)
+    t_obs = 0.5 * (times[0] + times[-1]) - 0.5 * (pr["min_x0"] + pr["max_x0"])
+    npix = p.camera.nx * p.camera.ny
+    out = None
+    for _ in range(args.warmup):
+        out, _ = image.run_grmhd_slow_light_image(p, grids, times, t_obs, out=out)
+    torch.cuda.synchronize()
+    per_step = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st = image.run_grmhd_slow_light_image(p, grids, times, t_obs, out=out)
+        per_step.append(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = statistics.median(s["kernel_ms"] for s in per_step)
+    st = per_step[-1]
+    b_ms = statistics.median(s["pass_b_ms"] for s in per_step)
+    # one radiating step samples two snapshots (8 records of 72 bytes each);
+    # the coefficient counter counts one control and one output evaluation
+    sample_bytes = st["coefficient_evals"] / 2 * 2 * 8 * 72
+    gbs = sample_bytes / (b_ms * 1e-3) / 1e9
+    reasons = np.bincount(out["reason"].cpu().numpy(), minlength=10)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_slow(p, m, grids, times, t_obs, args.cpu_seconds)
+    print(json.dumps({
+        "metric": "rays/s on 1 MI355X, slow-light GRMHD image (32 resident snapshots), FP64",
+        "value": args.steps * npix / wall, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Fishbone-Moncrief torus, Keplerian advection)",
+        "config": {"workload": "slow light: 32 FMKS snapshots 192x96x96 (dt 5 GM/c^3), 256x256 pinhole, "
+                               "230 GHz, i=163 deg, r_out 40, decoupled stepping, fluid interpolation, "
+                               "adaptive step-doubling RK4 rtol 1e-6",
+                   "rays_per_step": npix, "sequence_generation_s": gen_s, "observation_time": t_obs,
+                   "probe_x0": [pr["min_x0"], pr["max_x0"]], "kernel_ms_median": kms,
+                   "pass_a_ms": st["pass_a_ms"], "pass_b_ms": b_ms,
+                   "steps_per_s": (st["pass_a_steps"] + st["pass_b_steps"]) / (kms * 1e-3),
+                   "coefficient_evals": st["coefficient_evals"],
+                   "reasons": {str(i): int(c) for i, c in enumerate(reasons) if c}},
+        "roofline": {"bound": "hbm", "kernel": "pass_b (slow light)", "achieved": gbs, "peak": 8000.0,
+                     "unit": "GB/s", "frac": gbs / 8000.0, "traffic": None,
+                     "algorithmic_bytes_per_launch": sample_bytes, "launch_ms": b_ms},
+        "cpu_baseline": cpu,
+    }), flush=True)
+    return 0
+
+
+def cpu_baseline_slow(p, m, grids, times, t_obs, target_s: float) -> dict:
+    """The oracle on a band of image rows through the centre of the
+    slow-light image (where the rays cross the torus), all host threads; the
+    band grows until it takes about a quarter of target_s."""
+    import os
+    import oracle
+    host = [g.host_cells() for g in grids]
+    nx, ny = p.camera.nx, p.camera.ny
+    threads = os.cpu_count() or 1
+    rows = 1
+    while True:
+        begin = (ny // 2 - rows // 2) * nx
+        end = min(ny * nx, begin + rows * nx)
+        t0 = time.perf_counter()
+        oracle.run_grmhd_slow_light_image(p, m, host, times, t_obs, pixel_begin=begin, pixel_end=end, nthreads=threads)
+        dt = time.perf_counter() - t0
+        if dt > target_s / 4 or rows >= ny:
+            break
+        rows = min(ny, rows * 4)
+    return {"value": (end - begin) / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} central rows ({end - begin} pixels) of the 256x256 slow-light image on {threads} threads"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,7 +303,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="both (default): the metric's mixed run plus the FP64 run it is compared with")
-    ap.add_argument("--workload", choices=["riaf", "grmhd"], default="riaf",
+    ap.add_argument("--workload", choices=["riaf", "grmhd", "slow"], default="riaf",
                     help="riaf (default): the headline metric; grmhd: the 512x512, 8-frequency synthetic "
                          "GRMHD snapshot on a 288x128x128 FMKS grid (FP64)")
     args = ap.parse_args()
@@ -222,6 +313,8 @@ def main() -> int:
         return 2
     if args.workload == "grmhd":
         return bench_grmhd(args)
+    if args.workload == "slow":
+        return bench_slow(args)
     import torch
     from kpolaris_amd import abi, image
     assert torch.cuda.is_available(), "bench.py needs a HIP device"

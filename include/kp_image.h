@@ -462,6 +462,21 @@ kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp
                                         kp_image_stats* stats);
 void kp_slow_light_default(kp_slow_light* sequence);
 
+/* Time support of a slow-light image: pass A endpoints, then geometric
+ * stepping to the camera without radiation, recording the x^0 range of the
+ * samples inside [inner_radius, outer_radius); the fluid times needed are
+ * observation_time + [min_x0, max_x0].  Replaces
+ * run_slow_light_time_probe_metric (src/image/slow_light_driver.hpp:632-780;
+ * the tool's slow_light_time_probe=1).  model->cells is not read. */
+typedef struct kp_slow_light_probe {
+    double min_x0, max_x0;        /* 0, 0 when no pixel radiates */
+    int64_t radiating_pixels;     /* pixels with at least one radiating sample */
+    int64_t returned_pixels;      /* pixels whose ray reached the camera */
+} kp_slow_light_probe;
+
+kp_status kp_slow_light_time_probe(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                   const kp_run_options* options, kp_slow_light_probe* out);
+
 /* Multi-frequency GRMHD image on shared geodesics; replaces
  * run_grmhd_multifrequency_image (src/image/grmhd.hpp:31-38).  Arrays and
  * frequency lists as for kp_run_riaf_multifrequency_image. */

@@ -175,6 +175,11 @@ class SlowLight(C.Structure):
     ]
 
 
+class SlowLightProbe(C.Structure):
+    _fields_ = [("min_x0", C.c_double), ("max_x0", C.c_double),
+                ("radiating_pixels", C.c_int64), ("returned_pixels", C.c_int64)]
+
+
 class BatchJob(C.Structure):
     _fields_ = [
         ("params", PassAParams), ("model", RIAFModel),

@@ -75,7 +75,7 @@ enum : int {
 // one frequency with the tangent (plasma-parameter derivatives) of its Stokes
 // vector carried alongside, a GRMHD grid, or a GRMHD snapshot sequence
 // sampled at retarded time (slow light).
-enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4 };
+enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4, M_PROBE = 5 };
 
 // Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
 // (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
@@ -88,7 +88,9 @@ constexpr int KP_NQUEUE = 32;  // queue and list counters in the workspace
 // A staged FP64 launch hands a ray back to FP32 at this multiple of the
 // guard radius (hysteresis: no ray alternates stages on every step).
 constexpr double KP_BACK_RADIUS_FACTOR = 1.25;
-template <int MODE> constexpr int geom_of() { return (MODE == M_GRMHD || MODE == M_SLOW) ? G_FMKS : G_BL; }
+template <int MODE> constexpr int geom_of() {
+    return (MODE == M_GRMHD || MODE == M_SLOW || MODE == M_PROBE) ? G_FMKS : G_BL;
+}
 template <int MODE> constexpr bool lds_stokes() { return MODE == M_MF || MODE == M_GRMHD || MODE == M_SLOW; }
 
 enum Counter : int {
@@ -162,6 +164,9 @@ struct Params {
     int32_t nsnap, slow_mode, slow_interp, reserved_slow;
     double t_obs;
     double* req_t;
+    // time probe: per pixel the smallest and largest x^0 of its radiating
+    // samples ([2][npix], NaN when it has none)
+    double* probe;
     // outputs
     kp_image_data out;
 };
@@ -644,6 +649,7 @@ struct LaneB {
     Stokes<D3> sd;      // tangent mode: Stokes vector with its three parameter derivatives
     double h_current, cap, h, h_try, call_min, call_max;
     double req;     // slow light: the time a waiting ray needed (NaN otherwise)
+    double x0lo, x0hi;  // time probe: x^0 range of the radiating samples
     int steps, reason, att, phase, crosses, reached, pixel, pass_a_steps;
     int win, vac;   // slow light: snapshot window; vacuum finish (1: entering, 2: stepping)
 };
@@ -749,6 +755,10 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
         P.req_t[p] = L.req;
         if (L.req == L.req) count_add(&P.counters[C_WAIT], 1ull);
     }
+    if constexpr (MODE == M_PROBE) {
+        P.probe[p] = L.x0lo;
+        P.probe[(size_t)P.npix + p] = L.x0hi;
+    }
 }
 
 // Loads a pass A endpoint; returns false (after writing the pixel's outputs)
@@ -778,6 +788,9 @@ template <int MODE> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L
     L.win = 0;
     L.vac = 0;
     L.req = __builtin_nan("");
+    L.x0lo = __builtin_nan("");
+    L.x0hi = __builtin_nan("");
+    if (MODE == M_PROBE) L.cap = 0.0;
     if (!P.ep_valid[pixel]) {
         lane_b_finish<false, MODE>(P, L, 0);
         return false;
@@ -829,6 +842,8 @@ template <int MODE> KP_DEV void lane_b_resume(const Params& P, int pixel, LaneB&
     L.win = 0;
     L.vac = 0;
     L.req = __builtin_nan("");
+    L.x0lo = __builtin_nan("");
+    L.x0hi = __builtin_nan("");
 }
 
 KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
@@ -1057,6 +1072,17 @@ template <> struct Radiation<false, M_SLOW> {
         }
         return (lo.valid || hi.valid) ? (uint32_t)P.nfreq : 0u;
     }
+};
+
+// Time probe: geometric stepping only (no coefficients, no substep control).
+template <> struct Radiation<false, M_PROBE> {
+    KP_DEV void zero() {}
+    KP_DEV uint32_t eval(const Params&, const St<double>&) { return 0; }
+    KP_DEV uint32_t control(const Params&, double, double, double, int& req, int& pred) const {
+        req = pred = 1;
+        return 0;
+    }
+    KP_DEV uint32_t transfer(const Params&, LaneB&, double) const { return 0; }
 };
 
 // Tangent mode: the plasma scalars carry their derivatives with respect to
@@ -1325,6 +1351,19 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
             if (L.crosses) L.reached = 1;
             return false;
         }
+    }
+    if constexpr (MODE == M_PROBE) {
+        // run_slow_light_time_probe_metric (slow_light_driver.hpp:712-721)
+        if (h > 0.0 && sample_r >= P.inner && sample_r < P.outer) {
+            const double x0 = sample.v[0];
+            L.x0lo = L.x0lo == L.x0lo ? minv(L.x0lo, x0) : x0;
+            L.x0hi = L.x0hi == L.x0hi ? maxv(L.x0hi, x0) : x0;
+        }
+        st_store(L.s, next);
+        L.steps += 1;
+        steps_acc += 1;
+        if (L.crosses) L.reached = 1;
+        return false;
     }
     Radiation<MIXED, MODE> rad;
     rad.zero();
@@ -1640,6 +1679,7 @@ template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int gp, int mode) {
     if (mode == M_TANGENT) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_TANGENT>;
     if (mode == M_GRMHD) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_GRMHD>;
     if (mode == M_SLOW) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_SLOW>;
+    if (mode == M_PROBE) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_PROBE>;
     const bool mf = mode == M_MF;
     if (!mixed)
         return mf ? (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_MF>
@@ -1660,12 +1700,13 @@ const void* unsplit_fn(bool mixed, int mode) {
     if (mode == M_TANGENT) return (const void*)unsplit_kernel<false, M_TANGENT>;
     if (mode == M_GRMHD) return (const void*)unsplit_kernel<false, M_GRMHD>;
     if (mode == M_SLOW) return (const void*)unsplit_kernel<false, M_SLOW>;
+    if (mode == M_PROBE) return (const void*)unsplit_kernel<false, M_PROBE>;
     if (mode == M_MF)
         return mixed ? (const void*)unsplit_kernel<true, M_MF> : (const void*)unsplit_kernel<false, M_MF>;
     return mixed ? (const void*)unsplit_kernel<true, M_SINGLE> : (const void*)unsplit_kernel<false, M_SINGLE>;
 }
 
-int geom_of_mode(int mode) { return (mode == M_GRMHD || mode == M_SLOW) ? G_FMKS : G_BL; }
+int geom_of_mode(int mode) { return (mode == M_GRMHD || mode == M_SLOW || mode == M_PROBE) ? G_FMKS : G_BL; }
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
 // plus 4 doubles per lane and output frequency for multi-frequency Stokes.
@@ -1791,13 +1832,14 @@ static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Para
 }
 
 // GRMHD grid image on FMKS geodesics: camera in FMKS, model constants.
-static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* g, Params& P) {
+static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* g, Params& P,
+                               bool need_cells = true) {
     if (!p || !g) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
     if (p->coordinate_system != KP_COORD_FMKS)
         return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built in FMKS coordinates");
     if (g->emission_type != KP_EMISSION_THERMAL_PANDYA && g->emission_type != KP_EMISSION_THERMAL_DEXTER)
         return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
-    if (g->n1 < 1 || g->n2 < 1 || g->n3 < 2 || !g->cells)
+    if (g->n1 < 1 || g->n2 < 1 || g->n3 < 2 || (need_cells && !g->cells))
         return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid dimensions and cells are required");
     if (!(g->dx1 > 0.0) || !(g->dx2 > 0.0) || !(g->dx3 > 0.0))
         return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid spacings must be positive");
@@ -1921,7 +1963,8 @@ struct Tangent {
 static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* model, const FreqLists& fl_in,
                            const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
                            int perturbation, double log_scale, const Tangent* tangent = nullptr,
-                           const kp_grmhd_model* grmhd = nullptr, const kp_slow_light* slow = nullptr) {
+                           const kp_grmhd_model* grmhd = nullptr, const kp_slow_light* slow = nullptr,
+                           kp_slow_light_probe* probe = nullptr) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -1936,7 +1979,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (grmhd) {
         if (mixed || tangent || perturbation)
             return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built for FP64 without parameter gradients");
-        st = prepare_grmhd(params, grmhd, P);
+        st = prepare_grmhd(params, grmhd, P, probe == nullptr);
         if (st != KP_OK) return st;
         if (fl.nfreq == 0) {
             fl.nfreq = fl.nctrl = 1;
@@ -1958,7 +2001,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         st = prepare_frequencies(grmhd ? grmhd->mbh_solar : model->mbh_solar, fl, P);
         if (st != KP_OK) return st;
     }
-    int mode = grmhd ? (slow ? M_SLOW : M_GRMHD) : (mf ? M_MF : M_SINGLE);
+    int mode = grmhd ? (slow ? M_SLOW : (probe ? M_PROBE : M_GRMHD)) : (mf ? M_MF : M_SINGLE);
     if (tangent) {
         if (mixed || mf)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "parameter gradients are built for single-frequency FP64");
@@ -2002,7 +2045,9 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
     const size_t off_slow_c = align_up(off_slow_t + nsnap * sizeof(double));
     const size_t off_req = align_up(off_slow_c + nsnap * sizeof(void*));
-    total = align_up(off_req + (slow ? np * sizeof(double) : 0));
+    const size_t off_probe = align_up(off_req + (slow ? np * sizeof(double) : 0));
+    const size_t off_preason = align_up(off_probe + (probe ? 2 * np * sizeof(double) : 0));
+    total = align_up(off_preason + (probe ? np * sizeof(int32_t) : 0));
     void* ws = nullptr;
     st = workspace_for(device, total, &ws);
     if (st != KP_OK) return st;
@@ -2022,6 +2067,11 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     }
 
     hipStream_t stream = static_cast<hipStream_t>(opt.stream);
+    if (probe) {
+        P.probe = reinterpret_cast<double*>(base + off_probe);
+        std::memset(&P.out, 0, sizeof(P.out));
+        P.out.reason = reinterpret_cast<int32_t*>(base + off_preason);
+    }
     if (slow) {
         P.snap_t = reinterpret_cast<const double*>(base + off_slow_t);
         P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(base + off_slow_c);
@@ -2144,6 +2194,33 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         stats->pass_b_steps = (int64_t)cnt[C_STEPS_B];
         stats->fp64_guard_steps = (int64_t)cnt[C_GUARD];
         for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
+    }
+    if (probe) {
+        // SlowLightTimeProbeHostResult reduction (slow_light_driver.hpp:733-778)
+        std::vector<double> x0(2 * np);
+        std::vector<int32_t> rs(np);
+        KP_HIP(hipStreamSynchronize(stream));
+        if (np) {
+            KP_HIP(hipMemcpy(x0.data(), P.probe, 2 * np * sizeof(double), hipMemcpyDeviceToHost));
+            KP_HIP(hipMemcpy(rs.data(), P.out.reason, np * sizeof(int32_t), hipMemcpyDeviceToHost));
+        }
+        probe->min_x0 = 0.0;
+        probe->max_x0 = 0.0;
+        probe->radiating_pixels = 0;
+        probe->returned_pixels = 0;
+        double lo = 1e300, hi = -1e300;
+        for (size_t i = 0; i < np; ++i) {
+            if (x0[i] == x0[i]) {
+                probe->radiating_pixels += 1;
+                lo = std::min(lo, x0[i]);
+                hi = std::max(hi, x0[np + i]);
+            }
+            if (rs[i] == R_CAMERA) probe->returned_pixels += 1;
+        }
+        if (probe->radiating_pixels) {
+            probe->min_x0 = lo;
+            probe->max_x0 = hi;
+        }
     }
     if (slow && slow->step_mode == KP_SLOW_LIGHT_DECOUPLED && P.npix > 0) {
         // decoupled stepping: a ray that needed a time outside the sequence
@@ -2492,6 +2569,15 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
         fl.freqs = fl.ctrl = frequencies;
     }
     return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, sequence);
+}
+
+extern "C" kp_status kp_slow_light_time_probe(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                              const kp_run_options* options, kp_slow_light_probe* out) {
+    kp_internal::clear_error();
+    if (!model || !out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model or probe result");
+    kp_image_data dummy;
+    std::memset(&dummy, 0, sizeof(dummy));
+    return run_image(params, nullptr, FreqLists{}, options, &dummy, nullptr, 0, 0.0, nullptr, model, nullptr, out);
 }
 
 extern "C" kp_status kp_grmhd_derive_scalars(const kp_grmhd_model* model, void* stream) {

@@ -99,6 +99,9 @@ def library():
                                                 P(abi.ImageStats)]
     L.kp_run_grmhd_slow_light_image.restype = C.c_int32
     L.kp_slow_light_default.argtypes = [P(abi.SlowLight)]
+    L.kp_slow_light_time_probe.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.RunOptions),
+                                           P(abi.SlowLightProbe)]
+    L.kp_slow_light_time_probe.restype = C.c_int32
     if L.kp_abi_version() != abi.ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
                            f"{abi.ABI_VERSION}: rebuild the library")
@@ -429,3 +432,15 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observatio
         C.byref(params), C.byref(model), C.byref(seq), f.ctypes.data_as(C.POINTER(C.c_double)), len(f),
         C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)
+
+
+def slow_light_time_probe(params: abi.PassAParams, model: abi.GRMHDModel, *, persistent: bool = True) -> dict:
+    """x^0 range of the radiating samples of an image (the fluid times a
+    slow-light image needs are observation_time + [min_x0, max_x0]); runs
+    synchronously on the current device."""
+    import torch
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, persistent, 0, 0, C.c_void_p(stream))
+    out = abi.SlowLightProbe()
+    _check(library().kp_slow_light_time_probe(C.byref(params), C.byref(model), C.byref(opts), C.byref(out)))
+    return {n: getattr(out, n) for n, _ in abi.SlowLightProbe._fields_}

@@ -124,3 +124,20 @@ def test_gpu_slow_light_rejects_bad_sequences(seq):
     with pytest.raises(image.KPError) as e:
         image.run_grmhd_slow_light_image(p, grids[:2], times[:2], T_OBS, step_mode=7)
     assert e.value.status == abi.KP_INVALID_ARGUMENT
+
+
+def test_gpu_time_probe_brackets_the_radiating_times(seq):
+    """The probe's x^0 range (geometric stepping, no radiation cap) brackets
+    the fluid times the image needs: a sequence spanning it with a margin of
+    1 M leaves no ray waiting, one that starts a third of the way in does."""
+    from kpolaris_amd import abi, image
+    grids, host, times, model, p = seq
+    print("case: slow light time probe", flush=True)
+    pr = image.slow_light_time_probe(p, model)
+    assert pr["radiating_pixels"] > 0 and pr["returned_pixels"] > 0
+    lo, hi = T_OBS + pr["min_x0"], T_OBS + pr["max_x0"]
+    assert hi > lo
+    _gpu(grids, np.linspace(lo - 1.0, hi + 1.0, len(grids)), p, T_OBS)
+    with pytest.raises(image.KPError) as e:
+        _gpu(grids, np.linspace(lo + (hi - lo) / 3.0, hi + 1.0, len(grids)), p, T_OBS)
+    assert e.value.status == abi.KP_TIME_SUPPORT
