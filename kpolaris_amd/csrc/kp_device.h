@@ -443,8 +443,10 @@ template <class T> KP_DEV T err_norm(const St<T>& a, const St<T>& b) {
 // the full step's last stage is compared with the two-half-step state as it is
 // formed, so the full-step state is never held.  s (read only) and the k1
 // scratch K may live in registers or LDS.  Returns the error norm.
+// full = false computes only the two half steps (the composed path of the
+// surface-crossing refinement) and returns 0.
 template <class G, class T, class S, class K>
-KP_DEV T doubling_attempt(const G& g, const S& s, T h, K& k1, St<T>& half, St<T>& two_half) {
+KP_DEV T doubling_attempt(const G& g, const S& s, T h, K& k1, St<T>& half, St<T>& two_half, bool full = true) {
     St<T> acc;
     ++*g.rhs_count;
     g.apply(s, [&](int i, T f) { vset(k1, i, f); });
@@ -465,6 +467,7 @@ KP_DEV T doubling_attempt(const G& g, const S& s, T h, K& k1, St<T>& half, St<T>
     stage_mid(g, two_half, half, acc, q2);
     stage_mid(g, two_half, half, acc, h2);
     stage_last(g, two_half, half, acc, h2 / T(6));
+    if (!full) return T(0);
     // full step from s with k1, compared on the fly
     St<T> y;
     const T hh = T(0.5) * h;

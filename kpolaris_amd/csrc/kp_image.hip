@@ -296,6 +296,16 @@ struct LaneA {
     LaneVec s, k1;  // the lane's state and its k1 scratch
     double h_current, h_try, call_min;
     int n, steps, steps_inside, entered, att, in_call, pixel, reason;
+    // surface-crossing refinement, one trial per tick (refine: 0 off,
+    // 1 evaluating the full step, 2 bisecting); best_ep: the best state so
+    // far is in the endpoint buffer, not the lane state
+    double lo, hi, flo, fhi, trial, full_h, surface;
+    int refine, iter, use_secant, crossing, best_ep;
+    // staged mixed precision: the FP32 kernel hands a step that crosses a
+    // surface to the FP64 kernel (want64: park now, h_req: the step's
+    // requested size); hold: the FP64 kernel keeps it until it commits a step
+    double h_req;
+    int want64, hold;
 };
 
 KP_DEV void lane_a_init(const Params& P, int pixel, LaneA& L) {
@@ -313,97 +323,104 @@ KP_DEV void lane_a_init(const Params& P, int pixel, LaneA& L) {
     L.in_call = 0;
     L.pixel = pixel;
     L.reason = R_MAX_STEPS;
-}
-
-// locate_metric_surface_crossing for a radial surface (src/geodesic/pass_a_core.hpp:94-197).
-template <int GEOM, class GG>
-KP_DEV void locate_crossing(const GG& G, const St<double>& first, const St<double>& second, double full_h,
-                            double surface, int composed, St<double>& out) {
-    using GT = GeomT<GEOM>;
-    st_copy(out, first);
-    double lo = 0.0, hi = 1.0;
-    double flo = GT::radius(first.v[1]) - surface;
-    if (flo == 0.0 || !st_finite(second)) return;
-    St<double> t;
-    auto path = [&](double h, St<double>& o) {
-        if (composed) {
-            St<double> half;
-            rk4(G, first, h * 0.5, half);
-            rk4(G, half, h * 0.5, o);
-        } else {
-            rk4(G, first, h, o);
-        }
-    };
-    path(full_h, t);
-    if (!st_finite(t)) return;
-    double fhi = GT::radius(t.v[1]) - surface;
-    if (!(fhi == fhi) || absv(fhi) >= 1e300) return;
-    if (fhi == 0.0) {
-        st_copy(out, t);
-        return;
-    }
-    if ((flo > 0.0) == (fhi > 0.0)) return;
-    int use_secant = 1;
-    for (int iter = 0; iter < 12; ++iter) {
-        const double width = hi - lo;
-        double trial = (lo + hi) * 0.5;
-        const double denom = flo - fhi;
-        if (use_secant && absv(denom) > 1e-300) {
-            const double secant = lo + width * flo / denom;
-            const double guard = width * 0.1;
-            if (secant > lo + guard && secant < hi - guard) trial = secant;
-        }
-        path(full_h * trial, t);
-        if (!st_finite(t)) {
-            hi = trial;
-            use_secant = 0;
-            continue;
-        }
-        const double ft = GT::radius(t.v[1]) - surface;
-        if (!(ft == ft) || absv(ft) >= 1e300) {
-            hi = trial;
-            use_secant = 0;
-            continue;
-        }
-        if (ft == 0.0) {
-            st_copy(out, t);
-            return;
-        }
-        if ((flo > 0.0) == (ft > 0.0)) {
-            lo = trial;
-            flo = ft;
-            st_copy(out, t);
-        } else {
-            hi = trial;
-            fhi = ft;
-        }
-    }
+    L.refine = 0;
+    L.best_ep = 0;
+    L.want64 = 0;
+    L.hold = 0;
 }
 
 // One integration attempt from the lane state.  GP_64: FP64 derivatives.
 // GP_32: FP32 derivatives and stage sums, increments added to the FP64
 // state.  GP_GUARD: per lane, FP64 inside the guard radius and FP32 outside
-// (the lanes of a wave may take both paths).  guard counts the FP64 attempts
-// of a mixed-precision image: those of GP_GUARD lanes inside the radius and
-// every attempt of a staged FP64 launch (P.stage >= 0).
+// (the lanes of a wave may take both paths).  force64: FP64 whatever GP says
+// (the surface-crossing refinement of a mixed-precision lane).  guard counts
+// the FP64 integration attempts of a mixed-precision image: those of
+// GP_GUARD lanes inside the radius and every attempt of a staged FP64 launch
+// (P.stage >= 0), not the refinement trials.
 template <int GP, class GG>
 KP_DEV double attempt(const Params& P, const GG& G, const LaneVec& s, LaneVec& k1, double h,
-                      St<double>& half, St<double>& next, uint32_t& guard) {
-    if (GP == GP_32 || (GP == GP_GUARD && !(s[1] < P.fp64_radius)))
-        return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count);
-    if (GP == GP_GUARD || P.stage >= 0) ++guard;
-    return doubling_attempt(G, s, h, k1, half, next);
+                      St<double>& half, St<double>& next, uint32_t& guard, bool full = true,
+                      bool force64 = false) {
+    if (GP == GP_32 || (GP == GP_GUARD && !force64 && !(s[1] < P.fp64_radius)))
+        return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count, full);
+    if (!force64 && (GP == GP_GUARD || P.stage >= 0)) ++guard;
+    return doubling_attempt(G, s, h, k1, half, next, full);
 }
 
 template <int GP, class GG>
 KP_DEV void step_rk4(const Params& P, const GG& G, const LaneVec& s, double h, St<double>& next,
-                     uint32_t& guard) {
-    if (GP == GP_32 || (GP == GP_GUARD && !(s[1] < P.fp64_radius))) {
+                     uint32_t& guard, bool force64 = false) {
+    if (GP == GP_32 || (GP == GP_GUARD && !force64 && !(s[1] < P.fp64_radius))) {
         rk4_mixed(P.mass_f, P.spin_f, s, h, next, G.rhs_count);
         return;
     }
-    if (GP == GP_GUARD || P.stage >= 0) ++guard;
+    if (!force64 && (GP == GP_GUARD || P.stage >= 0)) ++guard;
     rk4(G, s, h, next);
+}
+
+// Surface-crossing refinement as part of the tick: the lane keeps the step's
+// start state (first) and evaluates one trial path per tick -- the composed
+// path of the adaptive step (its two half steps) or one RK4 step, both
+// through the same call as every other lane's attempt, so a refining lane
+// never runs alone in its wave.  The trial sequence and the bookkeeping are
+// locate_metric_surface_crossing's (src/geodesic/pass_a_core.hpp:94-197):
+// the full step first, then up to 12 secant-or-bisection trials, keeping
+// the last state on the starting side.  Returns true when it has finished.
+template <int GEOM>
+KP_DEV bool refine_update(const Params& P, LaneA& L, const St<double>& t) {
+    using GT = GeomT<GEOM>;
+    auto keep = [&](const St<double>& v) {
+        const size_t np = (size_t)P.npix;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = v.v[c];
+        L.best_ep = 1;
+    };
+    if (L.refine == 1) {
+        if (!st_finite(t)) return true;
+        const double fhi = GT::radius(t.v[1]) - L.surface;
+        if (!(fhi == fhi) || absv(fhi) >= 1e300) return true;
+        if (fhi == 0.0) {
+            keep(t);
+            return true;
+        }
+        if ((L.flo > 0.0) == (fhi > 0.0)) return true;
+        L.fhi = fhi;
+        L.refine = 2;
+        L.iter = 0;
+        L.use_secant = 1;
+    } else {
+        bool ok = st_finite(t);
+        double ft = 0.0;
+        if (ok) {
+            ft = GT::radius(t.v[1]) - L.surface;
+            ok = (ft == ft) && absv(ft) < 1e300;
+        }
+        if (!ok) {
+            L.hi = L.trial;
+            L.use_secant = 0;
+        } else if (ft == 0.0) {
+            keep(t);
+            return true;
+        } else if ((L.flo > 0.0) == (ft > 0.0)) {
+            L.lo = L.trial;
+            L.flo = ft;
+            keep(t);
+        } else {
+            L.hi = L.trial;
+            L.fhi = ft;
+        }
+        if (++L.iter >= 12) return true;
+    }
+    const double width = L.hi - L.lo;
+    double trial = (L.lo + L.hi) * 0.5;
+    const double denom = L.flo - L.fhi;
+    if (L.use_secant && absv(denom) > 1e-300) {
+        const double secant = L.lo + width * L.flo / denom;
+        const double gw = width * 0.1;
+        if (secant > L.lo + gw && secant < L.hi - gw) trial = secant;
+    }
+    L.trial = trial;
+    return false;
 }
 
 // One tick of pass A; returns true when the ray has terminated (L.reason set).
@@ -412,7 +429,7 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
                         uint32_t& guard) {
     using GT = GeomT<GEOM>;
     const double eff_min = absv(P.min_step);
-    if (!L.in_call) {
+    if (!L.refine && !L.in_call) {
         if (L.n >= P.max_steps) {
             L.reason = R_MAX_STEPS;
             return true;
@@ -433,6 +450,7 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
                 return true;
             }
         }
+        L.h_req = L.h_current;
         if (P.adaptive) {
             const double req = L.h_current;
             L.call_min = eff_min;
@@ -443,12 +461,28 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
             L.in_call = 1;
         }
     }
+    // one integration call for every lane: a refinement trial path, an
+    // adaptive attempt or a fixed RK4 step
+    const double h_call = L.refine ? L.full_h * L.trial : (P.adaptive ? L.h_try : L.h_current);
     St<double> next;
+    double err = 0.0;
+    if (P.adaptive) {
+        St<double> half;
+        err = attempt<GP>(P, G, L.s, L.k1, h_call, half, next, guard, !L.refine, L.refine);
+    } else {
+        step_rk4<GP>(P, G, L.s, h_call, next, guard, L.refine);
+    }
+    if (L.refine) {
+        if (!refine_update<GEOM>(P, L, next)) return false;
+        L.refine = 0;
+        L.reason = L.crossing;
+        L.steps += 1;
+        steps_acc += 1;
+        return true;
+    }
     double used_h;
     if (P.adaptive) {
         // adaptive_rk4_step (src/geodesic/rk4.hpp:109-161), one attempt per tick
-        St<double> half;
-        const double err = attempt<GP>(P, G, L.s, L.k1, L.h_try, half, next, guard);
         const double tol = maxv(P.tol, 8.0 * eps<double>());
         const double req_sign = L.h_current < 0.0 ? -1.0 : 1.0;
         if (err <= tol) {
@@ -480,7 +514,6 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
         }
     } else {
         used_h = L.h_current;
-        step_rk4<GP>(P, G, L.s, used_h, next, guard);
     }
     const double old_r = GT::radius(L.s[1]);
     const double next_r = GT::radius(next.v[1]);
@@ -502,21 +535,49 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
     else if (L.entered && L.steps_inside > 0 && old_r < P.outer && next_r >= P.outer)
         crossing = R_ESCAPED;
     if (crossing != R_NONE) {
-        St<double> first, ev;
-        st_load(first, L.s);
-        locate_crossing<GEOM>(G, first, next, used_h, crossing == R_INNER ? P.inner : P.outer, P.adaptive, ev);
-        st_store(L.s, ev);
-        L.reason = crossing;
-        L.steps += 1;
-        steps_acc += 1;
-        return true;
+        if (GP == GP_32 && P.park == PARK_INSIDE) {
+            // the crossing is located in FP64: the FP64 kernel retakes this step
+            L.h_current = L.h_req;
+            L.in_call = 0;
+            L.want64 = 1;
+            return false;
+        }
+        // start the refinement from the step's start state (kept in L.s)
+        L.crossing = crossing;
+        L.surface = crossing == R_INNER ? P.inner : P.outer;
+        L.full_h = used_h;
+        L.lo = 0.0;
+        L.hi = 1.0;
+        L.flo = old_r - L.surface;
+        if (L.flo == 0.0 || !st_finite(next)) {
+            L.reason = crossing;
+            L.steps += 1;
+            steps_acc += 1;
+            return true;
+        }
+        L.refine = 1;
+        L.trial = 1.0;
+        return false;
     }
     st_store(L.s, next);
     L.steps += 1;
     steps_acc += 1;
     if (L.entered) L.steps_inside += 1;
     L.n += 1;
+    L.hold = 0;
     return false;
+}
+
+KP_DEV void lane_a_store(const Params& P, const LaneA& L) {
+    const size_t np = (size_t)P.npix;
+    if (!L.best_ep) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = L.s[c];
+    }
+    const int valid = L.entered && (L.reason == R_INNER || L.reason == R_ESCAPED);
+    P.ep_valid[L.pixel] = valid;
+    P.ep_steps[L.pixel] = L.steps;
+    P.ep_reason[L.pixel] = L.reason;
 }
 
 // Parking (staged mixed precision): between steps a lane's state moves to
@@ -549,6 +610,7 @@ KP_DEV void lane_a_park(const Params& P, const LaneA& L) {
     P.pk_i[np + p] = L.steps;
     P.pk_i[2 * np + p] = L.steps_inside;
     P.pk_i[3 * np + p] = L.entered;
+    P.pk_i[4 * np + p] = L.want64;
 }
 
 KP_DEV void lane_a_resume(const Params& P, int pixel, LaneA& L) {
@@ -566,16 +628,10 @@ KP_DEV void lane_a_resume(const Params& P, int pixel, LaneA& L) {
     L.in_call = 0;
     L.pixel = pixel;
     L.reason = R_MAX_STEPS;
-}
-
-KP_DEV void lane_a_store(const Params& P, const LaneA& L) {
-    const size_t np = (size_t)P.npix;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = L.s[c];
-    const int valid = L.entered && (L.reason == R_INNER || L.reason == R_ESCAPED);
-    P.ep_valid[L.pixel] = valid;
-    P.ep_steps[L.pixel] = L.steps;
-    P.ep_reason[L.pixel] = L.reason;
+    L.refine = 0;
+    L.best_ep = 0;
+    L.want64 = 0;
+    L.hold = P.pk_i[4 * np + p];
 }
 
 // Per-lane LDS slots: [32][blockDim] doubles, components 0-15 the state and
@@ -613,7 +669,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
             }
             if (!__ballot(has)) break;
             if (P.park != PARK_NONE) {
-                const bool park = has && !L.in_call && park_due(P, GeomT<GEOM>::radius(L.s[1]));
+                const bool park = has && !L.in_call && !L.refine &&
+                                  (L.want64 || (!L.hold && park_due(P, GeomT<GEOM>::radius(L.s[1]))));
                 if (park) {
                     lane_a_park(P, L);
                     has = false;
@@ -1993,8 +2050,12 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (mixed) {
         if (!(opt.fp64_radius >= 0.0))
             return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
+        if (opt.guard_mode < 0 || opt.guard_mode > 2)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "guard_mode must be 0, 1 or 2");
         prepare_mixed(P);
         P.fp64_radius = opt.fp64_radius;
+        // every geodesic attempt of guard_mode 2 is FP64 and counts as a guard step
+        if (opt.guard_mode == 2) P.stage = 0;
     }
     const bool mf = fl.nfreq > 0;
     if (mf) {
@@ -2040,7 +2101,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const size_t off_lists = total;
     const size_t off_pkd = align_up(off_lists + (staged ? 2 * np * sizeof(uint32_t) : 0));
     const size_t off_pki = align_up(off_pkd + (staged ? 2 * np * sizeof(double) : 0));
-    const size_t off_pkst = align_up(off_pki + (staged ? 4 * np * sizeof(int32_t) : 0));
+    const size_t off_pkst = align_up(off_pki + (staged ? 5 * np * sizeof(int32_t) : 0));
     const size_t off_slow_t = align_up(off_pkst + (staged ? 4 * (size_t)nst * np * sizeof(double) : 0));
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
     const size_t off_slow_c = align_up(off_slow_t + nsnap * sizeof(double));
@@ -2139,7 +2200,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         }
         return KP_OK;
     };
-    const int gp_plain = mixed ? GP_GUARD : GP_64;
+    // guard_mode 2: FP64 geodesics with the FP32 radiation path
+    const int gp_plain = mixed ? (opt.guard_mode == 2 ? GP_64 : GP_GUARD) : GP_64;
     if (P.npix > 0 && unsplit) {
         Params Q = P;
         Q.fetch = P.queue;

@@ -35,7 +35,7 @@ KP_DEV void stage_mid_f(float m, float a, St<float>& y, const St<float>& base, S
 // s: FP64 base (registers or LDS); half, two_half: FP64 outputs.
 template <class S>
 KP_DEV double doubling_attempt_mixed(float m, float a, const S& s, double h, St<double>& half,
-                                     St<double>& two_half, uint32_t* cnt) {
+                                     St<double>& two_half, uint32_t* cnt, bool full = true) {
     St<float> s32, k1, acc, y;
 #pragma unroll
     for (int i = 0; i < 16; ++i) s32.v[i] = float(s[i]);
@@ -68,6 +68,7 @@ KP_DEV double doubling_attempt_mixed(float m, float a, const S& s, double h, St<
     stage_mid_f(m, a, y, s32, acc, h2, cnt);
     ++*cnt;
     bl_rhs_apply(m, a, y, [&](int i, float f) { two_half.v[i] = half.v[i] + double((acc.v[i] + f) * c6); });
+    if (!full) return 0.0;
     // full step from s, compared with two_half in FP64
 #pragma unroll
     for (int i = 0; i < 16; ++i) s32.v[i] = float(s[i]);

@@ -137,6 +137,20 @@ def test_gpu_mixed_fixed_step_rk4():
     assert e["pixel_max_rel_peak"] <= parity.MIXED_PIXEL, e
 
 
+@pytest.mark.parametrize("guard_mode", [1, 2])
+def test_gpu_mixed_guard_modes(guard_mode):
+    """The one-kernel per-lane guard (1) and FP64 geodesics with the FP32
+    radiation path (2) against the FP64 oracle at the mixed bars."""
+    import oracle
+    from kpolaris_amd import abi
+    print(f"case: mixed guard_mode {guard_mode} 40x40", flush=True)
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=40, ny=40).resolve_riaf()
+    got, st = _gpu_image(p, m, precision=abi.PRECISION_MIXED, guard_mode=guard_mode)
+    ref, _ = oracle.run_riaf_image(p, m)
+    print(parity.assert_mixed(got, ref, f"mixed guard_mode {guard_mode}"), st["fp64_guard_steps"], flush=True)
+    assert st["fp64_guard_steps"] > 0
+
+
 def test_gpu_mixed_edge_shapes_and_repeat():
     """1x1 and ragged 7x3 images; a repeated run gives identical bits."""
     from kpolaris_amd import abi
