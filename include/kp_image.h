@@ -88,11 +88,12 @@ typedef enum kp_emission_type {
 typedef enum kp_precision {
     /* FP64 throughout: the parity kernel. */
     KP_PRECISION_FP64 = 0,
-    /* FP64 ray state; FP32 transport derivatives, RK stage sums, plasma
-     * coefficients and Stokes transfer (in nu_obs^3-scaled units).  FP64 is
-     * kept for integration attempts inside kp_run_options.fp64_radius, the
-     * step-error estimate, surface-crossing refinement and Faraday rotation
-     * angles above one radian. */
+    /* FP64 ray state; FP32 plasma coefficients and Stokes transfer (in
+     * nu_obs^3-scaled units), Faraday rotation angles above one radian in
+     * FP64.  Geodesic stepping per kp_run_options.guard_mode: FP64 (default),
+     * or FP32 derivatives and RK stage sums with FP64 attempts inside
+     * fp64_radius, FP64 step-error comparison and surface-crossing
+     * refinement. */
     KP_PRECISION_MIXED = 1
 } kp_precision;
 
@@ -251,12 +252,16 @@ typedef struct kp_run_options {
     int32_t block_size;       /* rays per workgroup; 0 = tuned default */
     int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread */
     int32_t waves_per_cu;     /* persistent grid size; 0 = occupancy-derived */
-    int32_t guard_mode;       /* mixed, persistent split kernels: 0 = staged
-                                 launches (FP32-geodesic and FP64-geodesic
-                                 kernels; a ray moves to the FP64 kernel when it
-                                 reaches r < fp64_radius and back at
-                                 r >= 1.25 fp64_radius); 1 = one kernel whose
-                                 lanes choose per attempt */
+    int32_t guard_mode;       /* mixed: geodesic precision.  0 = FP64 geodesics,
+                                 FP32 coefficients and transfer (default; FP32
+                                 arithmetic issues at the FP64 rate on this
+                                 hardware without packed FP32, so FP32 stepping
+                                 buys nothing);  1 = FP32 stepping with FP64
+                                 attempts inside fp64_radius, chosen per lane in
+                                 one kernel;  2 = the same, staged: FP32-geodesic
+                                 and FP64-geodesic kernels, rays moved between
+                                 them at fp64_radius and 1.25 fp64_radius
+                                 (persistent split launches; others use 1) */
     double fp64_radius;       /* mixed: attempts starting at r < fp64_radius run in FP64 */
     void* stream;             /* hipStream_t, NULL = default stream */
 } kp_run_options;

@@ -127,6 +127,11 @@ struct Params {
     // frequency lists with their dl/dlambda, mixed-precision Planck constants
     // 2 h nu^3 / c^2 and 1 / nu^3
     int32_t nfreq, nctrl;
+    // single_pass: the step-control list is the output list and it fits the
+    // LDS budget of a second Stokes buffer, so each frequency's coefficients
+    // are evaluated once per step (control, then a tentative transfer into
+    // the second buffer, kept if the step is accepted)
+    int32_t single_pass, reserved_mf;
     double freqs[KP_MAX_FREQUENCIES], ctrl[KP_MAX_FREQUENCIES];
     double dls_f[KP_MAX_FREQUENCIES], dls_c[KP_MAX_FREQUENCIES];
     double inv_nu3_f[KP_MAX_FREQUENCIES];
@@ -703,6 +708,7 @@ struct LaneB {
     LaneVec s, k1;  // the lane's state and its k1 scratch
     Stokes<double> st;
     LdsVec<double> mf;  // multi-frequency: Stokes component 4 f + c of this lane, in LDS
+    LdsVec<double> mf2; // single-pass multi-frequency: the tentative Stokes of a step
     Stokes<D3> sd;      // tangent mode: Stokes vector with its three parameter derivatives
     double h_current, cap, h, h_try, call_min, call_max;
     double req;     // slow light: the time a waiting ray needed (NaN otherwise)
@@ -986,6 +992,8 @@ template <> struct Radiation<true, M_SINGLE> {
 template <> struct Radiation<false, M_MF> {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
+    KP_DEV bool valid() const { return loc.valid; }
+    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const { riaf_coeffs_at(P.riaf, loc, freq, c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         riaf_local(P.riaf, sample, loc);
         return 0;
@@ -1043,6 +1051,8 @@ template <> struct Radiation<true, M_MF> {
 template <> struct Radiation<false, M_GRMHD> {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
+    KP_DEV bool valid() const { return loc.valid; }
+    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const { grmhd_coeffs_at(P.grmhd, loc, freq, c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         grmhd_local(P.grmhd, sample, loc);
         return 0;
@@ -1091,6 +1101,7 @@ template <> struct Radiation<false, M_SLOW> {
         }
         return 0;
     }
+    KP_DEV bool valid() const { return lo.valid || hi.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
         grmhd_coeffs_at(P.grmhd, lo, freq, c);
         if (P.slow_interp != KP_SLOW_LIGHT_FLUID) {
@@ -1445,6 +1456,53 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
     const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
                             ? P.max_rs : 0.0;
     int required, pred;
+    constexpr bool SHARED = !MIXED && (MODE == M_MF || MODE == M_GRMHD || MODE == M_SLOW);
+    if constexpr (SHARED) {
+        if (P.single_pass) {
+            // the control frequencies are the output frequencies: one
+            // coefficient evaluation per frequency serves the substep control
+            // (pass_b.hpp:1416-1460) and a tentative transfer of that
+            // frequency's Stokes vector into the second buffer
+            required = 1;
+            pred = 1;
+            const bool valid = rad.valid();
+            for (int f = 0; f < P.nfreq; ++f) {
+                Coeffs<double> c;
+                rad.at(P, P.freqs[f], c);
+                required = maxv(required, substep_count(c, h, P.dls_f[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+                pred = maxv(pred, substep_count(c, L.h_current, P.dls_f[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+                if (h > 0.0) {
+                    Stokes<double> st = {L.mf[4 * f], L.mf[4 * f + 1], L.mf[4 * f + 2], L.mf[4 * f + 3]};
+                    strang_step(st, c, absv(h) * P.dls_f[f]);
+                    L.mf2.set(4 * f, st.I);
+                    L.mf2.set(4 * f + 1, st.Q);
+                    L.mf2.set(4 * f + 2, st.U);
+                    L.mf2.set(4 * f + 3, st.V);
+                }
+            }
+            if (valid) coeff_acc += (uint32_t)P.nfreq;
+            if (required > 1 && h > eff_min * 1.0001) {
+                L.h_current = maxv(eff_min, h / double(required));
+                L.cap = L.h_current;
+                return false;
+            }
+            if (lmrs > 0.0 || P.max_rd > 0.0 || P.max_ad > 0.0 || P.max_fd > 0.0)
+                L.cap = pred > 1 ? maxv(eff_min, L.h_current / double(pred)) : P.max_step;
+            else
+                L.cap = P.max_step;
+            if (h > 0.0) {
+                const LdsVec<double> t = L.mf;
+                L.mf = L.mf2;
+                L.mf2 = t;
+                transfer_acc += 1;
+            }
+            st_store(L.s, next);
+            L.steps += 1;
+            steps_acc += 1;
+            if (L.crosses) L.reached = 1;
+            return false;
+        }
+    }
     coeff_acc += rad.control(P, h, L.h_current, lmrs, required, pred);
     if (required > 1 && h > eff_min * 1.0001) {
         L.h_current = maxv(eff_min, h / double(required));
@@ -1475,6 +1533,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
     if (lds_stokes<MODE>()) {
         extern __shared__ double kp_lds[];
         L.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
+        L.mf2 = LdsVec<double>{kp_lds + ((KP_LANE_LDS ? 32 : 0) + 4 * P.nfreq) * blockDim.x + threadIdx.x,
+                               (int)blockDim.x};
     }
     bool has = false;
     if (PERSISTENT) {
@@ -1557,6 +1617,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
             if (lds_stokes<MODE>()) {
                 extern __shared__ double kp_lds[];
                 B.mf = LdsVec<double>{kp_lds + (KP_LANE_LDS ? 32 : 0) * blockDim.x + threadIdx.x, (int)blockDim.x};
+                B.mf2 = LdsVec<double>{kp_lds + ((KP_LANE_LDS ? 32 : 0) + 4 * P.nfreq) * blockDim.x + threadIdx.x,
+                                       (int)blockDim.x};
             }
             if (lane_b_init<MODE>(P, pix, B)) {
                 while (!pass_b_tick<MIXED, GP, MODE>(P, G, B, coeff, transfer, steps_b, guard)) {
@@ -1767,8 +1829,8 @@ int geom_of_mode(int mode) { return (mode == M_GRMHD || mode == M_SLOW || mode =
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
 // plus 4 doubles per lane and output frequency for multi-frequency Stokes.
-size_t lds_bytes(int block, int nfreq_mf) {
-    return (size_t(KP_LANE_LDS ? 32 : 0) + size_t(4 * nfreq_mf)) * size_t(block) * sizeof(double);
+size_t lds_bytes(int block, int nfreq_mf, int stokes_buffers = 1) {
+    return (size_t(KP_LANE_LDS ? 32 : 0) + size_t(4 * nfreq_mf * stokes_buffers)) * size_t(block) * sizeof(double);
 }
 
 struct OccKey {
@@ -1990,6 +2052,15 @@ static kp_status prepare_frequencies(double mbh_solar, const FreqLists& fl, Para
                                  "step-control frequencies: 1 to KP_MAX_FREQUENCIES values required");
     P.nfreq = fl.nfreq;
     P.nctrl = fl.nctrl;
+    // same control and output lists, at most 8 frequencies: two Stokes
+    // buffers of 4 nfreq doubles per lane stay within the LDS of the
+    // resident waves
+    P.single_pass = 0;
+    if (fl.nfreq == fl.nctrl && fl.nfreq <= 8) {
+        bool same = true;
+        for (int f = 0; f < fl.nfreq; ++f) same = same && fl.freqs[f] == fl.ctrl[f];
+        P.single_pass = same ? 1 : 0;
+    }
     const double cl = KP_CL;
     const double l_unit = 6.67430e-8 * mbh_solar * 1.98847e33 / (cl * cl);
     auto fill = [&](const double* src, int n, double* f, double* dls, float* bconst, double* inv3) {
@@ -2054,8 +2125,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             return kp_internal::fail(KP_INVALID_ARGUMENT, "guard_mode must be 0, 1 or 2");
         prepare_mixed(P);
         P.fp64_radius = opt.fp64_radius;
-        // every geodesic attempt of guard_mode 2 is FP64 and counts as a guard step
-        if (opt.guard_mode == 2) P.stage = 0;
+        // every geodesic attempt of guard_mode 0 is FP64 and counts as a guard step
+        if (opt.guard_mode == 0) P.stage = 0;
     }
     const bool mf = fl.nfreq > 0;
     if (mf) {
@@ -2084,11 +2155,11 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     KP_HIP(hipGetDevice(&device));
     const size_t np = (size_t)P.npix;
     const bool unsplit = opt.split_transport == 0;
-    // Staged mixed precision (persistent split kernels): an FP32-geodesic
-    // launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
+    // Staged mixed precision (guard_mode 2, persistent split kernels): an
+    // FP32-geodesic launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
     // continues them and parks those back that reach r >= back_radius, then
     // FP32 again and a final FP64 launch; no wave mixes the two paths.
-    const bool staged = mixed && P.persistent && !unsplit && opt.guard_mode == 0 && P.fp64_radius > 0.0 &&
+    const bool staged = mixed && P.persistent && !unsplit && opt.guard_mode == 2 && P.fp64_radius > 0.0 &&
                         (mode == M_SINGLE || mode == M_MF);
     const int nst = mf ? P.nfreq : 1;
     const size_t off_ep = 0;
@@ -2151,7 +2222,10 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     st = cu_count(device, cus);
     if (st != KP_OK) return st;
     const size_t smem_a = lds_bytes(block, 0);
-    const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0);
+    // the single-pass frequency loop serves the FP64 multi-frequency, GRMHD
+    // and slow-light lanes
+    if (mixed || !(mode == M_MF || mode == M_GRMHD || mode == M_SLOW)) P.single_pass = 0;
+    const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0, P.single_pass ? 2 : 1);
     const int nblocks_flat = (int)((np + block - 1) / block);
     // grid of one launch: one block per pixel block, or (persistent) the
     // resident blocks of the kernel on every CU
@@ -2200,8 +2274,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         }
         return KP_OK;
     };
-    // guard_mode 2: FP64 geodesics with the FP32 radiation path
-    const int gp_plain = mixed ? (opt.guard_mode == 2 ? GP_64 : GP_GUARD) : GP_64;
+    // guard_mode 0: FP64 geodesics with the FP32 radiation path
+    const int gp_plain = mixed ? (opt.guard_mode == 0 ? GP_64 : GP_GUARD) : GP_64;
     if (P.npix > 0 && unsplit) {
         Params Q = P;
         Q.fetch = P.queue;

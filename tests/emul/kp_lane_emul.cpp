@@ -43,7 +43,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         P.riaf.b_factor = std::exp(0.5 * log_scales[0]) * std::exp(log_scales[2]);
         P.grad = grad;
     }
-    double mfbuf[4 * KP_MAX_FREQUENCIES];
+    double mfbuf[4 * KP_MAX_FREQUENCIES], mfbuf2[4 * KP_MAX_FREQUENCIES];
     const int np = P.npix;
     std::vector<double> ep(16 * (size_t)np);
     std::vector<int32_t> valid(np), steps(np), reason(np);
@@ -63,6 +63,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         lane_a_store(P, A);
         LaneB B;
         B.mf = LdsVec<double>{mfbuf, 1};
+        B.mf2 = LdsVec<double>{mfbuf2, 1};
         if (lane_b_init<MODE>(P, pix, B)) {
             while (!pass_b_tick<MIXED, GP, MODE>(P, G, B, c1, c2, sb, guard)) {
             }

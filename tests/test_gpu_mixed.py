@@ -93,11 +93,11 @@ def test_gpu_mixed_vs_oracle_adaptive(persistent):
 
 
 def test_gpu_mixed_dexter_guarded_and_unguarded():
-    """Adaptive RK4 (rtol 1e-6), Dexter fit, Faraday rotation off.  With the
-    default FP64 guard radius the image meets the mixed bars; with the guard
-    off (fp64_radius 0, every attempt FP32-derived) no attempt runs in FP64
-    and total I stays within the bar, the polarisation fractions within
-    ten times it."""
+    """Adaptive RK4 (rtol 1e-6), Dexter fit, Faraday rotation off.  The default
+    mixed kernel meets the mixed bars; FP32 stepping with the guard off
+    (guard_mode 1, fp64_radius 0: every attempt FP32-derived) runs no attempt
+    in FP64 and keeps total I within the bar, the polarisation fractions
+    within ten times it."""
     import oracle
     from kpolaris_amd import abi
     print("case: mixed dexter adaptive 16x12 guarded/unguarded", flush=True)
@@ -108,7 +108,7 @@ def test_gpu_mixed_dexter_guarded_and_unguarded():
     got, st = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
     assert st["fp64_guard_steps"] > 0
     print(parity.assert_mixed(got, ref, "mixed dexter c2", pol="absolute"), flush=True)
-    bare, st0 = _gpu_image(p, m, precision=abi.PRECISION_MIXED, fp64_radius=0.0)
+    bare, st0 = _gpu_image(p, m, precision=abi.PRECISION_MIXED, fp64_radius=0.0, guard_mode=1)
     assert st0["fp64_guard_steps"] == 0
     e = parity.mixed_errors(bare, ref)
     print("unguarded", e, flush=True)
@@ -139,8 +139,8 @@ def test_gpu_mixed_fixed_step_rk4():
 
 @pytest.mark.parametrize("guard_mode", [1, 2])
 def test_gpu_mixed_guard_modes(guard_mode):
-    """The one-kernel per-lane guard (1) and FP64 geodesics with the FP32
-    radiation path (2) against the FP64 oracle at the mixed bars."""
+    """FP32 stepping with the per-lane FP64 guard in one kernel (1) and staged
+    FP32/FP64 geodesic kernels (2) against the FP64 oracle at the mixed bars."""
     import oracle
     from kpolaris_amd import abi
     print(f"case: mixed guard_mode {guard_mode} 40x40", flush=True)
