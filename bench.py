@@ -193,9 +193,10 @@ def bench_grmhd(args) -> int:
                    "kernel_ms_median": kms, "pass_a_ms": st["pass_a_ms"], "pass_b_ms": st["pass_b_ms"],
                    "steps_per_s": (st["pass_a_steps"] + st["pass_b_steps"]) / (kms * 1e-3),
                    "coefficient_evals": st["coefficient_evals"],
-                   # one grid sample (8 cell records, 576 bytes) per radiating step; the
-                   # coefficient counter counts 8 control + 8 output frequencies per step
-                   "grid_sample_bytes_per_s": st["coefficient_evals"] / 16 * 576 / (st["pass_b_ms"] * 1e-3)},
+                   # one grid sample (8 cell records, 576 bytes) per radiating step; with
+                   # the step-control list equal to the output list the coefficient counter
+                   # counts the 8 frequencies once per step
+                   "grid_sample_bytes_per_s": st["coefficient_evals"] / 8 * 576 / (st["pass_b_ms"] * 1e-3)},
         "roofline": {"bound": "hbm", "kernel": "derive_scalars", "achieved": derive_gbs, "peak": 8000.0,
                      "unit": "GB/s", "frac": derive_gbs / 8000.0, "traffic": None,
                      "algorithmic_bytes_per_launch": ncell * 72, "launch_ms": derive_ms},
@@ -241,8 +242,9 @@ def bench_slow(args) -> int:
     st = per_step[-1]
     b_ms = statistics.median(s["pass_b_ms"] for s in per_step)
     # one radiating step samples two snapshots (8 records of 72 bytes each);
-    # the coefficient counter counts one control and one output evaluation
-    sample_bytes = st["coefficient_evals"] / 2 * 2 * 8 * 72
+    # the coefficient counter counts one evaluation per step (one frequency,
+    # single-pass control and transfer)
+    sample_bytes = st["coefficient_evals"] * 2 * 8 * 72
     gbs = sample_bytes / (b_ms * 1e-3) / 1e9
     reasons = np.bincount(out["reason"].cpu().numpy(), minlength=10)
     cpu = None

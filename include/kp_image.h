@@ -467,6 +467,23 @@ kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp
                                         kp_image_stats* stats);
 void kp_slow_light_default(kp_slow_light* sequence);
 
+/* Slow-light image and its derivatives along the fixed geodesics with
+ * respect to (q_density, R_high, q_magnetic): q_density scales n_e by e^q and
+ * B by e^(q/2), q_magnetic scales B by e^q (PlasmaPerturbation,
+ * src/radiation/plasma_perturbation.hpp:12-33), R_high is the model's
+ * trat_large entering Theta_e through the derived scalars
+ * (src/grmhd/iharm_loader.cpp:494-504).  log_scales = {q_density,
+ * q_magnetic} is the evaluation point; grad is a DEVICE [3][4][npix] array
+ * (parameter-major, then I, Q, U, V).  One frequency (model->freq_cgs).
+ * The reference computes these responses by finite differences and
+ * perturbation reruns (src/diagnostics/response.hpp:118-191, its slow-light
+ * form slow_light_driver.hpp:1376); here they are exact forward-mode
+ * tangents of the same transfer. */
+kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                 const kp_slow_light* sequence, const double* log_scales,
+                                                 const kp_run_options* options, kp_image_data* out,
+                                                 double* grad, kp_image_stats* stats);
+
 /* Time support of a slow-light image: pass A endpoints, then geometric
  * stepping to the camera without radiation, recording the x^0 range of the
  * samples inside [inner_radius, outer_radius); the fluid times needed are

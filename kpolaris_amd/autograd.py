@@ -41,3 +41,40 @@ class RiafStokes(torch.autograd.Function):
 def riaf_stokes(log_scales: torch.Tensor, params, model) -> torch.Tensor:
     """Differentiable Stokes image [4, npix] at plasma log-scales (3,)."""
     return RiafStokes.apply(log_scales, params, model)
+
+
+class SlowLightStokes(torch.autograd.Function):
+    """Slow-light Stokes image [4, npix] at (q_density, R_high, q_magnetic).
+
+    R_high enters through the snapshots' derived scalars: when it differs from
+    the grids' current value the derived scalars of every snapshot are
+    recomputed on the device (kp_grmhd_derive_scalars) before the image."""
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, params, grids, times, observation_time, step_mode, interpolation):
+        q_density, r_high, q_magnetic = [float(v) for v in theta.detach().cpu().reshape(3)]
+        for g in grids:
+            if g.model.trat_large != r_high:
+                g.model.trat_large = r_high
+                g.derive()
+        out, grad, _ = image.run_grmhd_slow_light_image_gradient(
+            params, grids, times, observation_time, (q_density, q_magnetic), step_mode=step_mode,
+            interpolation=interpolation, with_stats=False)
+        stokes = torch.stack([out["image_i"], out["image_q"], out["image_u"], out["image_v"]])
+        ctx.save_for_backward(grad)
+        ctx.q_device = theta.device
+        ctx.q_dtype = theta.dtype
+        return stokes
+
+    @staticmethod
+    def backward(ctx, grad_stokes: torch.Tensor):
+        (jac,) = ctx.saved_tensors
+        g = (jac * grad_stokes.to(jac.dtype).unsqueeze(0)).sum(dim=(1, 2))
+        return g.to(device=ctx.q_device, dtype=ctx.q_dtype), None, None, None, None, None, None
+
+
+def slow_light_stokes(theta: torch.Tensor, params, grids, times, observation_time: float,
+                      step_mode: int = 0, interpolation: int = 1) -> torch.Tensor:
+    """Differentiable slow-light Stokes image [4, npix] at
+    theta = (q_density, R_high, q_magnetic) (3,)."""
+    return SlowLightStokes.apply(theta, params, grids, times, observation_time, step_mode, interpolation)

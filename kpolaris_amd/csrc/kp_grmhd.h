@@ -197,6 +197,41 @@ KP_DEV void grmhd_sample(const GrmhdDev& G, const kp_grmhd_cell* cells, int i0, 
     }
 }
 
+// d Theta_e / d R_high of one cell's derived scalars: the iharm loader sets
+// Theta_e = (m_p/m_e)(game-1)(gamp-1) / D * u/rho with
+// D = (gamp-1) + (game-1) trat and trat = (R_high b2 + R_low) / (1 + b2),
+// b2 = beta^2 / beta_crit^2 (src/grmhd/iharm_loader.cpp:494-504), so
+// dTheta_e/dR_high = -Theta_e (game-1) b2 / ((1 + b2) D).
+KP_DEV double grmhd_dthetae_drhigh(const GrmhdDev& G, double thetae, double beta) {
+    const double game = 4.0 / 3.0, gamp = 5.0 / 3.0;
+    const double b2 = beta * beta / maxv(G.beta_crit * G.beta_crit, 1e-40);
+    const double trat = (G.trat_large * b2 + G.trat_small) / (1.0 + b2);
+    const double D = (gamp - 1.0) + (game - 1.0) * trat;
+    return -thetae * (game - 1.0) * (b2 / (1.0 + b2)) / D;
+}
+
+// grmhd_sample plus the trilinear sample of dTheta_e/dR_high.
+KP_DEV void grmhd_sample_dth(const GrmhdDev& G, const kp_grmhd_cell* cells, int i0, int j0, int k0, double di,
+                             double dj, double dk, double prim[8], double der[5], double& dth) {
+    const double wi[2] = {1.0 - di, di}, wj[2] = {1.0 - dj, dj}, wk[2] = {1.0 - dk, dk};
+#pragma unroll
+    for (int v = 0; v < 8; ++v) prim[v] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) der[v] = 0.0;
+    dth = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int bi = n & 1, bj = (n >> 1) & 1, bk = (n >> 2) & 1;
+        const double w = wi[bi] * wj[bj] * wk[bk];
+        const kp_grmhd_cell& c = cells[grmhd_cell(G, i0 + bi, j0 + bj, k0 + bk)];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) prim[v] += w * double(c.prim[v]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) der[v] += w * c.derived[v];
+        dth += w * grmhd_dthetae_drhigh(G, c.derived[1], c.derived[4]);
+    }
+}
+
 // interpolate_time_value (src/model/time_interpolation.hpp:27-32).
 KP_DEV double time_value(double a, double b, double f) {
     if (f <= 0.0 || a == b) return a;
@@ -314,6 +349,40 @@ KP_DEV void grmhd_local_time(const GrmhdDev& G, const kp_grmhd_cell* lo, const k
         if (f > 0.0) {
             double p1[8], d1[5];
             grmhd_sample(G, hi, i0, j0, k0, di, dj, dk, p1, d1);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) prim[v] = time_value(prim[v], p1[v], f);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) der[v] = time_value(der[v], d1[v], f);
+        }
+    }
+    grmhd_local_from(G, s, r, th, prim, der, L);
+}
+
+// grmhd_local_time with the sampled dTheta_e/dR_high (time-interpolated
+// like Theta_e itself: the lower value's derivative where the values agree
+// or f <= 0, the upper one's at f >= 1).
+KP_DEV void grmhd_local_time_dth(const GrmhdDev& G, const kp_grmhd_cell* lo, const kp_grmhd_cell* hi, double f,
+                                 const St<double>& s, RiafLocal<double>& L, double& dth) {
+    L.valid = 0;
+    L.aligned = 0;
+    L.has_chi = 0;
+    dth = 0.0;
+    const double x1 = s.v[1], x2 = s.v[2], phi = s.v[3];
+    const double r = exp(x1);
+    const double th = fmks_theta(G.map, x1, x2).th;
+    if (!(r >= G.r_in && r < G.r_out)) return;
+    int i0, j0, k0;
+    double di, dj, dk;
+    if (!grmhd_indices(G, x1, x2, phi, i0, j0, k0, di, dj, dk)) return;
+    double prim[8], der[5];
+    if (f >= 1.0) {
+        grmhd_sample_dth(G, hi, i0, j0, k0, di, dj, dk, prim, der, dth);
+    } else {
+        grmhd_sample_dth(G, lo, i0, j0, k0, di, dj, dk, prim, der, dth);
+        if (f > 0.0) {
+            double p1[8], d1[5], dth1;
+            grmhd_sample_dth(G, hi, i0, j0, k0, di, dj, dk, p1, d1, dth1);
+            if (der[1] != d1[1]) dth = (1.0 - f) * dth + f * dth1;
 #pragma unroll
             for (int v = 0; v < 8; ++v) prim[v] = time_value(prim[v], p1[v], f);
 #pragma unroll

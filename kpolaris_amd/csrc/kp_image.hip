@@ -75,7 +75,7 @@ enum : int {
 // one frequency with the tangent (plasma-parameter derivatives) of its Stokes
 // vector carried alongside, a GRMHD grid, or a GRMHD snapshot sequence
 // sampled at retarded time (slow light).
-enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4, M_PROBE = 5 };
+enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4, M_PROBE = 5, M_STAN = 6 };
 
 // Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
 // (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
@@ -89,7 +89,7 @@ constexpr int KP_NQUEUE = 32;  // queue and list counters in the workspace
 // guard radius (hysteresis: no ray alternates stages on every step).
 constexpr double KP_BACK_RADIUS_FACTOR = 1.25;
 template <int MODE> constexpr int geom_of() {
-    return (MODE == M_GRMHD || MODE == M_SLOW || MODE == M_PROBE) ? G_FMKS : G_BL;
+    return (MODE == M_GRMHD || MODE == M_SLOW || MODE == M_PROBE || MODE == M_STAN) ? G_FMKS : G_BL;
 }
 template <int MODE> constexpr bool lds_stokes() { return MODE == M_MF || MODE == M_GRMHD || MODE == M_SLOW; }
 
@@ -169,6 +169,9 @@ struct Params {
     int32_t nsnap, slow_mode, slow_interp, reserved_slow;
     double t_obs;
     double* req_t;
+    // slow-light gradients: the evaluation point's density and field factors
+    // (n_e e^q_density, B e^(q_density/2 + q_magnetic))
+    double tq_ne, tq_b;
     // time probe: per pixel the smallest and largest x^0 of its radiating
     // samples ([2][npix], NaN when it has none)
     double* probe;
@@ -770,7 +773,7 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
             if (O.image_u) O.image_u[q] = of.U;
             if (O.image_v) O.image_v[q] = of.V;
         }
-    } else if (MODE == M_TANGENT) {
+    } else if (MODE == M_TANGENT || MODE == M_STAN) {
         Stokes<double> ov = {0.0, 0.0, 0.0, 0.0};
         if (valid) ov = observer_basis({L.sd.I.v, L.sd.Q.v, L.sd.U.v, L.sd.V.v}, r11, r12, r21, r22);
         if (O.image_i) O.image_i[p] = ov.I;
@@ -814,7 +817,7 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
     if (O.pass_a_steps) O.pass_a_steps[p] = L.pass_a_steps;
     if (O.steps) O.steps[p] = valid ? L.steps : 0;
     if (O.reason) O.reason[p] = reason;
-    if constexpr (MODE == M_SLOW) {
+    if constexpr (MODE == M_SLOW || MODE == M_STAN) {
         P.req_t[p] = L.req;
         if (L.req == L.req) count_add(&P.counters[C_WAIT], 1ull);
     }
@@ -836,7 +839,7 @@ template <int MODE> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L
     L.st = {0.0, 0.0, 0.0, 0.0};
     if (lds_stokes<MODE>())
         for (int j = 0; j < 4 * P.nfreq; ++j) L.mf.set(j, 0.0);
-    if (MODE == M_TANGENT) L.sd = {D3(0.0), D3(0.0), D3(0.0), D3(0.0)};
+    if (MODE == M_TANGENT || MODE == M_STAN) L.sd = {D3(0.0), D3(0.0), D3(0.0), D3(0.0)};
     L.h_current = P.step;
     L.cap = P.max_rs > 0.0 ? P.max_rs : P.max_step;
     L.h = 0.0;
@@ -1216,6 +1219,93 @@ template <> struct Radiation<false, M_TANGENT> {
     }
 };
 
+// Slow-light gradients: the slow-light sampling with dTheta_e/dR_high, the
+// Dexter/Pandya coefficients in Dual<3> arithmetic with respect to
+// (q_density, R_high, q_magnetic) -- n_e e^q and B e^(q/2) for the density
+// family, Theta_e(R_high) through the derived scalars, B e^q for the field
+// (plasma_perturbation.hpp:17-32) -- and the Strang step on the dual Stokes
+// vector.  The substep control uses the values, so the geodesic and the step
+// sequence are the FP64 slow-light kernel's.
+KP_DEV void grmhd_tangent_coeffs(const GrmhdDev& G, const RiafLocal<double>& L, double dth, double ne_f,
+                                 double b_f, double freq, Coeffs<D3>& c) {
+    coeffs_zero(c);
+    if (!L.valid) return;
+    const double nev = L.ne * ne_f, bv = L.b * b_f;
+    D3 ne(nev), th(L.thetae), b(bv);
+    ne.d[0] = nev;
+    b.d[0] = 0.5 * bv;
+    th.d[1] = dth;
+    b.d[2] = bv;
+    const D3 nu(maxv(freq * L.nu_scale, 1.0));
+    Coeffs<D3> mc;
+    thermal_coeffs(G.fit, nu, ne, th, b, D3(L.sin_theta), D3(L.cos_theta), D3(G.max_pol_frac), D3(1e-4),
+                   D3(G.emission_scale), D3(G.absorption_scale), D3(G.faraday_scale), mc);
+    c.jI = mc.jI;
+    c.jV = mc.jV;
+    c.aI = mc.aI;
+    c.aV = mc.aV;
+    if (G.faraday_rotation) c.rV = mc.rV;
+    if (L.has_chi) {
+        const D3 c2(L.cos2chi), s2(L.sin2chi);
+        c.jQ = mc.jQ * c2;
+        c.jU = mc.jQ * s2;
+        c.aQ = mc.aQ * c2;
+        c.aU = mc.aQ * s2;
+        c.rQ = mc.rQ * c2;
+        c.rU = mc.rQ * s2;
+    }
+}
+
+template <> struct Radiation<false, M_STAN> {
+    Coeffs<D3> c;
+    double f;
+    int k;
+    KP_DEV void zero() {
+        coeffs_zero(c);
+        f = 0.0;
+        k = 0;
+    }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        RiafLocal<double> lo;
+        double dlo;
+        if (P.slow_interp == KP_SLOW_LIGHT_FLUID) {
+            grmhd_local_time_dth(P.grmhd, P.snaps[k], P.snaps[k + 1], f, sample, lo, dlo);
+            grmhd_tangent_coeffs(P.grmhd, lo, dlo, P.tq_ne, P.tq_b, P.freqs[0], c);
+        } else {
+            RiafLocal<double> hi;
+            double dhi;
+            grmhd_local_time_dth(P.grmhd, P.snaps[k], P.snaps[k], 0.0, sample, lo, dlo);
+            grmhd_local_time_dth(P.grmhd, P.snaps[k + 1], P.snaps[k + 1], 0.0, sample, hi, dhi);
+            Coeffs<D3> a, b;
+            grmhd_tangent_coeffs(P.grmhd, lo, dlo, P.tq_ne, P.tq_b, P.freqs[0], a);
+            grmhd_tangent_coeffs(P.grmhd, hi, dhi, P.tq_ne, P.tq_b, P.freqs[0], b);
+            const D3 w(1.0 - f), ff(f);
+            c.jI = w * a.jI + ff * b.jI;
+            c.jQ = w * a.jQ + ff * b.jQ;
+            c.jU = w * a.jU + ff * b.jU;
+            c.jV = w * a.jV + ff * b.jV;
+            c.aI = w * a.aI + ff * b.aI;
+            c.aQ = w * a.aQ + ff * b.aQ;
+            c.aU = w * a.aU + ff * b.aU;
+            c.aV = w * a.aV + ff * b.aV;
+            c.rQ = w * a.rQ + ff * b.rQ;
+            c.rU = w * a.rU + ff * b.rU;
+            c.rV = w * a.rV + ff * b.rV;
+        }
+        return 1;
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        const Coeffs<double> v = value_of(c);
+        req = substep_count(v, h, P.dls_f[0], lmrs, P.max_rd, P.max_ad, P.max_fd);
+        pred = substep_count(v, h_next, P.dls_f[0], lmrs, P.max_rd, P.max_ad, P.max_fd);
+        return 0;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        strang_step(L.sd, c, D3(absv(h) * P.dls_f[0]));
+        return 0;
+    }
+};
+
 // SlowLightTemporalModel::interpolation_fraction (slow_light_driver.hpp:62-69).
 KP_DEV double slow_fraction(double t_lo, double t_hi, double t_obs, double x0) {
     const double denom = t_hi - t_lo;
@@ -1258,7 +1348,7 @@ template <bool MIXED, int GP, int MODE>
 KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::GeoType& G, LaneB& L,
                         uint32_t& coeff_acc, uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     using GT = GeomT<geom_of<MODE>()>;
-    constexpr bool SLOW = MODE == M_SLOW;
+    constexpr bool SLOW = MODE == M_SLOW || MODE == M_STAN;
     const double eff_min = absv(P.min_step);
     if (L.phase == PH_IDLE) {
         if (L.reached || L.steps >= P.max_steps) return true;
@@ -1796,6 +1886,7 @@ template <bool PERSISTENT> const void* pass_a_fn(int gp, int geom = G_BL) {
 // kernels only.
 template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int gp, int mode) {
     if (mode == M_TANGENT) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_TANGENT>;
+    if (mode == M_STAN) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_STAN>;
     if (mode == M_GRMHD) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_GRMHD>;
     if (mode == M_SLOW) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_SLOW>;
     if (mode == M_PROBE) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_PROBE>;
@@ -1817,6 +1908,7 @@ template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int gp, int mode) {
 
 const void* unsplit_fn(bool mixed, int mode) {
     if (mode == M_TANGENT) return (const void*)unsplit_kernel<false, M_TANGENT>;
+    if (mode == M_STAN) return (const void*)unsplit_kernel<false, M_STAN>;
     if (mode == M_GRMHD) return (const void*)unsplit_kernel<false, M_GRMHD>;
     if (mode == M_SLOW) return (const void*)unsplit_kernel<false, M_SLOW>;
     if (mode == M_PROBE) return (const void*)unsplit_kernel<false, M_PROBE>;
@@ -1825,7 +1917,9 @@ const void* unsplit_fn(bool mixed, int mode) {
     return mixed ? (const void*)unsplit_kernel<true, M_SINGLE> : (const void*)unsplit_kernel<false, M_SINGLE>;
 }
 
-int geom_of_mode(int mode) { return (mode == M_GRMHD || mode == M_SLOW || mode == M_PROBE) ? G_FMKS : G_BL; }
+int geom_of_mode(int mode) {
+    return (mode == M_GRMHD || mode == M_SLOW || mode == M_PROBE || mode == M_STAN) ? G_FMKS : G_BL;
+}
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
 // plus 4 doubles per lane and output frequency for multi-frequency Stokes.
@@ -2105,8 +2199,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     FreqLists fl = fl_in;
     kp_status st;
     if (grmhd) {
-        if (mixed || tangent || perturbation)
-            return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built for FP64 without parameter gradients");
+        if (mixed || perturbation || (tangent && !slow))
+            return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built for FP64; gradients for slow-light sequences");
         st = prepare_grmhd(params, grmhd, P, probe == nullptr);
         if (st != KP_OK) return st;
         if (fl.nfreq == 0) {
@@ -2134,7 +2228,20 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         if (st != KP_OK) return st;
     }
     int mode = grmhd ? (slow ? M_SLOW : (probe ? M_PROBE : M_GRMHD)) : (mf ? M_MF : M_SINGLE);
-    if (tangent) {
+    if (tangent && slow) {
+        if (mixed || fl.nfreq != 1)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light gradients are built for single-frequency FP64");
+        if (!tangent->log_scales || !tangent->grad)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "null log_scales or gradient array");
+        const double qd = tangent->log_scales[0], qm = tangent->log_scales[1];
+        if (!std::isfinite(qd) || !std::isfinite(qm))
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "log_scales must be finite");
+        P.tq_ne = std::exp(qd);
+        P.tq_b = std::exp(0.5 * qd) * std::exp(qm);
+        P.grad = tangent->grad;
+        P.single_pass = 0;
+        mode = M_STAN;
+    } else if (tangent) {
         if (mixed || mf)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "parameter gradients are built for single-frequency FP64");
         if (!tangent->log_scales || !tangent->grad)
@@ -2225,7 +2332,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     // the single-pass frequency loop serves the FP64 multi-frequency, GRMHD
     // and slow-light lanes
     if (mixed || !(mode == M_MF || mode == M_GRMHD || mode == M_SLOW)) P.single_pass = 0;
-    const size_t smem_b = lds_bytes(block, mf ? P.nfreq : 0, P.single_pass ? 2 : 1);
+    const size_t smem_b = mode == M_STAN ? lds_bytes(block, 0) : lds_bytes(block, mf ? P.nfreq : 0, P.single_pass ? 2 : 1);
     const int nblocks_flat = (int)((np + block - 1) / block);
     // grid of one launch: one block per pixel block, or (persistent) the
     // resident blocks of the kernel on every CU
@@ -2705,6 +2812,34 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
         fl.freqs = fl.ctrl = frequencies;
     }
     return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, sequence);
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_params* params,
+                                                             const kp_grmhd_model* model,
+                                                             const kp_slow_light* sequence,
+                                                             const double* log_scales,
+                                                             const kp_run_options* options, kp_image_data* out,
+                                                             double* grad, kp_image_stats* stats) {
+    kp_internal::clear_error();
+    if (!model || !sequence || !sequence->cells || !sequence->times || sequence->nsnap < 2)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD model and a sequence of at least two snapshots required");
+    for (int k = 0; k < sequence->nsnap; ++k) {
+        if (!sequence->cells[k]) return kp_internal::fail(KP_INVALID_ARGUMENT, "null snapshot cells");
+        if (!std::isfinite(sequence->times[k]) || (k > 0 && !(sequence->times[k] > sequence->times[k - 1])))
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "snapshot times must be finite and strictly increasing");
+    }
+    if (sequence->step_mode < KP_SLOW_LIGHT_DECOUPLED || sequence->step_mode > KP_SLOW_LIGHT_BLOCK ||
+        (sequence->interpolation != KP_SLOW_LIGHT_FLUID && sequence->interpolation != KP_SLOW_LIGHT_COEFFICIENTS))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light step mode or interpolation");
+    kp_grmhd_model m = *model;
+    m.cells = sequence->cells[0];
+    Tangent t;
+    t.log_scales = log_scales;
+    t.grad = grad;
+    FreqLists fl;
+    fl.nfreq = fl.nctrl = 1;
+    fl.freqs = fl.ctrl = &model->freq_cgs;
+    return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, &t, &m, sequence);
 }
 
 extern "C" kp_status kp_slow_light_time_probe(const kp_pass_a_params* params, const kp_grmhd_model* model,

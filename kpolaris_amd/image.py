@@ -101,6 +101,10 @@ def library():
     L.kp_slow_light_default.argtypes = [P(abi.SlowLight)]
     L.kp_slow_light_time_probe.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.RunOptions),
                                            P(abi.SlowLightProbe)]
+    L.kp_run_grmhd_slow_light_image_gradient.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
+                                                         P(C.c_double), P(abi.RunOptions), P(abi.ImageData),
+                                                         P(C.c_double), P(abi.ImageStats)]
+    L.kp_run_grmhd_slow_light_image_gradient.restype = C.c_int32
     L.kp_slow_light_time_probe.restype = C.c_int32
     if L.kp_abi_version() != abi.ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
@@ -395,6 +399,47 @@ def run_grmhd_image(params: abi.PassAParams, grid: "GRMHDGrid", frequencies=None
         C.byref(params), C.byref(grid.model), 1, f.ctypes.data_as(dp), len(f), c.ctypes.data_as(dp), len(c),
         C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)
+
+
+def _sequence(grids, times, observation_time, step_mode, interpolation):
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    ptrs = (C.c_void_p * len(grids))(*[g.model.cells for g in grids])
+    seq = abi.SlowLight()
+    library().kp_slow_light_default(C.byref(seq))
+    seq.nsnap = len(grids)
+    seq.step_mode = step_mode
+    seq.interpolation = interpolation
+    seq.observation_time = observation_time
+    seq.times = t.ctypes.data_as(C.POINTER(C.c_double))
+    seq.cells = C.cast(ptrs, C.POINTER(C.c_void_p))
+    return seq, (t, ptrs)
+
+
+def run_grmhd_slow_light_image_gradient(params: abi.PassAParams, grids, times, observation_time: float,
+                                        log_scales=(0.0, 0.0), *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
+                                        interpolation: int = abi.SLOW_LIGHT_FLUID, persistent: bool = True,
+                                        split_transport: int = 1, out: dict | None = None,
+                                        fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+    """Slow-light image at one frequency (the model's) and its derivatives
+    along the fixed geodesics with respect to (q_density, R_high,
+    q_magnetic); returns (outputs, grad [3, 4, npix] device tensor, stats).
+    log_scales = (q_density, q_magnetic) is the evaluation point."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    model = grids[0].model
+    seq, keep = _sequence(grids, times, observation_time, step_mode, interpolation)
+    if out is None:
+        out = alloc_device_image(npix, fields=fields)
+    img = image_struct_from_tensors(out)
+    grad = torch.empty((3, 4, npix), dtype=torch.float64, device="cuda")
+    q = (C.c_double * 2)(*log_scales)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, persistent, 0, 0, C.c_void_p(stream), None, split_transport)
+    st = abi.ImageStats()
+    _check(library().kp_run_grmhd_slow_light_image_gradient(
+        C.byref(params), C.byref(model), C.byref(seq), q, C.byref(opts), C.byref(img),
+        C.cast(C.c_void_p(grad.data_ptr()), C.POINTER(C.c_double)), C.byref(st) if with_stats else None))
+    return out, grad, (st.as_dict() if with_stats else None)
 
 
 def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observation_time: float, frequencies=None,

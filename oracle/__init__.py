@@ -125,8 +125,8 @@ def lib():
                                               C.POINTER(State), C.POINTER(Perturbation), i32, C.POINTER(Coeffs)]
     L.kpo_slow_light_coefficients.restype = None
     L.kpo_run_grmhd_slow_light_image.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.GRMHDModel),
-                                                 C.POINTER(abi.SlowLight), dp, i32, i32, i32, i32,
-                                                 C.POINTER(abi.ImageData), C.POINTER(Stats), dp]
+                                                 C.POINTER(abi.SlowLight), C.POINTER(Perturbation), dp, i32,
+                                                 i32, i32, i32, C.POINTER(abi.ImageData), C.POINTER(Stats), dp]
     L.kpo_run_grmhd_slow_light_image.restype = i32
     _lib = L
     return L
@@ -280,7 +280,7 @@ def slow_light_sequence(cells_list, times, observation_time, step_mode=abi.SLOW_
 def run_grmhd_slow_light_image(params: abi.PassAParams, model: abi.GRMHDModel, cells_list, times,
                                observation_time, frequencies=None, step_mode=abi.SLOW_LIGHT_DECOUPLED,
                                interpolation=abi.SLOW_LIGHT_FLUID, pixel_begin: int = 0, pixel_end: int = -1,
-                               nthreads: int = 0):
+                               nthreads: int = 0, perturbation=None):
     """Oracle slow-light GRMHD image; returns (arrays, stats, waiting) where
     waiting = (count, min, max) of the times decoupled stepping needed outside
     the sequence (count 0: none; the oracle status is then KP_OK)."""
@@ -297,7 +297,11 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, model: abi.GRMHDModel, c
     if nthreads <= 0:
         nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
     dp = C.POINTER(C.c_double)
-    rc = L.kpo_run_grmhd_slow_light_image(C.byref(params), C.byref(model), C.byref(seq), f.ctypes.data_as(dp),
+    pert = None
+    if perturbation is not None:
+        pert = Perturbation(perturbation[0], 0, perturbation[1])
+    rc = L.kpo_run_grmhd_slow_light_image(C.byref(params), C.byref(model), C.byref(seq),
+                                          C.byref(pert) if pert is not None else None, f.ctypes.data_as(dp),
                                           len(f), pixel_begin, pixel_end, nthreads, C.byref(img), C.byref(st),
                                           waiting.ctypes.data_as(dp))
     if rc not in (0, abi.KP_TIME_SUPPORT):

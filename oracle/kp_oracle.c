@@ -2648,7 +2648,8 @@ static int slow_vacuum_to_camera(const kp_pass_a_params* p, const metric_ctx* m,
  * time; decoupled stepping cuts nothing and a radiating sample outside the
  * sequence leaves the ray waiting (SlowLightResidentWindows, :983-1027). */
 static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, const kp_slow_light* seq,
-                        const freq_plan* fp, const metric_ctx* m, const kpo_state* camera,
+                        const kpo_perturbation* pert, const freq_plan* fp, const metric_ctx* m,
+                        const kpo_state* camera,
                         const kpo_state* endpoint, int pass_a_steps, int pass_a_reason, int valid,
                         pixel_result* out, kpo_stats* stats) {
     memset(out, 0, sizeof(*out));
@@ -2807,7 +2808,7 @@ static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, cons
                 memset(&cc[f], 0, sizeof(cc[f]));
                 if (radiating) {
                     if (stats) stats->coefficient_evals++;
-                    kpo_slow_light_coefficients(&gf, seq, pair, &sample, NULL,
+                    kpo_slow_light_coefficients(&gf, seq, pair, &sample, pert,
                                                 p->emission_selection.faraday_rotation, &cc[f]);
                 }
                 const int a = kpo_radiation_substep_count(&cc[f], h, dls[f], 1, local_max_rs, p->max_radiation_depth,
@@ -2922,8 +2923,8 @@ static kp_status run_pixels_w(const kp_pass_a_params* p, const model_ctx* model,
         const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
         pixel_result r;
         if (model->slow)
-            pass_b_slow(p, model->grmhd, model->slow, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r,
-                        &local);
+            pass_b_slow(p, model->grmhd, model->slow, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid,
+                        &r, &local);
         else
             pass_b(p, model, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
         if (r.waiting) {
@@ -3060,7 +3061,8 @@ kp_status kpo_run_grmhd_image(const kp_pass_a_params* p, const kp_grmhd_model* g
 }
 
 kp_status kpo_run_grmhd_slow_light_image(const kp_pass_a_params* p, const kp_grmhd_model* g,
-                                         const kp_slow_light* seq, const double* freqs, int32_t nfreq,
+                                         const kp_slow_light* seq, const kpo_perturbation* pert,
+                                         const double* freqs, int32_t nfreq,
                                          int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
                                          kp_image_data* out, kpo_stats* stats, double waiting[3]) {
     if (!p || !g || !seq || !out || !seq->times || !seq->cells || seq->nsnap < 2) return KP_INVALID_ARGUMENT;
@@ -3083,7 +3085,7 @@ kp_status kpo_run_grmhd_slow_light_image(const kp_pass_a_params* p, const kp_grm
         waiting[1] = LARGE;
         waiting[2] = -LARGE;
     }
-    const kp_status st = run_pixels_w(p, &mc, NULL, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix,
+    const kp_status st = run_pixels_w(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix,
                                       nthreads, out, stats, waiting);
     if (st != KP_OK) return st;
     return (waiting && waiting[0] > 0.0) ? KP_TIME_SUPPORT : KP_OK;

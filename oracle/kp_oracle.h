@@ -163,7 +163,8 @@ void kpo_slow_light_coefficients(const kp_grmhd_model* model, const kp_slow_ligh
  * NULL) receives {count, min, max} of the times decoupled stepping needed
  * outside the sequence; KP_TIME_SUPPORT when the count is nonzero. */
 kp_status kpo_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
-                                         const kp_slow_light* sequence, const double* freqs, int32_t nfreq,
+                                         const kp_slow_light* sequence, const kpo_perturbation* pert,
+                                         const double* freqs, int32_t nfreq,
                                          int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
                                          kp_image_data* out, kpo_stats* stats, double waiting[3]);
 

@@ -14,7 +14,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
                           const kp_grmhd_model* gm = nullptr, const kp_slow_light* slow = nullptr,
                           double* req_t = nullptr, unsigned long long* waiting = nullptr) {
     constexpr int GP = MIXED ? GP_GUARD : GP_64;
-    constexpr bool FM = MODE == M_GRMHD || MODE == M_SLOW;
+    constexpr bool FM = MODE == M_GRMHD || MODE == M_SLOW || MODE == M_STAN;
     Params P;
     kp_status st = FM ? prepare_grmhd(pa, gm, P) : prepare(pa, m, P);
     if (st != KP_OK) return st;
@@ -22,13 +22,13 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         prepare_mixed(P);
         P.fp64_radius = fp64_radius;
     }
-    if (lds_stokes<MODE>()) {
+    if (lds_stokes<MODE>() || MODE == M_STAN) {
         st = prepare_frequencies(FM ? gm->mbh_solar : m->mbh_solar, fl, P);
         if (st != KP_OK) return st;
     }
     unsigned long long counters[C_COUNT] = {};
     P.counters = counters;
-    if (MODE == M_SLOW) {
+    if (MODE == M_SLOW || MODE == M_STAN) {
         P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(slow->cells);
         P.snap_t = slow->times;
         P.nsnap = slow->nsnap;
@@ -36,6 +36,12 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         P.slow_interp = slow->interpolation;
         P.t_obs = slow->observation_time;
         P.req_t = req_t;
+    }
+    if (MODE == M_STAN) {
+        P.tq_ne = std::exp(log_scales[0]);
+        P.tq_b = std::exp(0.5 * log_scales[0]) * std::exp(log_scales[1]);
+        P.grad = grad;
+        P.single_pass = 0;
     }
     if (MODE == M_TANGENT) {
         P.riaf.ne_factor = std::exp(log_scales[0]);
@@ -166,4 +172,17 @@ extern "C" kp_status kp_emul_run_grmhd_slow_light_image(const kp_pass_a_params* 
     g.cells = seq->cells[0];
     return emul_run<false, M_SLOW>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, nullptr, nullptr, &g, seq, req_t,
                                    waiting);
+}
+
+// Slow-light gradient lanes (one frequency, the model's); grad is [3][4][npix].
+extern "C" kp_status kp_emul_run_grmhd_slow_light_gradient(const kp_pass_a_params* pa, const kp_grmhd_model* gm,
+                                                           const kp_slow_light* seq, const double* log_scales,
+                                                           kp_image_data* out, double* grad, double* req_t) {
+    FreqLists fl;
+    fl.nfreq = fl.nctrl = 1;
+    fl.freqs = fl.ctrl = &gm->freq_cgs;
+    kp_grmhd_model g = *gm;
+    g.cells = seq->cells[0];
+    return emul_run<false, M_STAN>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, log_scales, grad, &g, seq, req_t,
+                                   nullptr);
 }

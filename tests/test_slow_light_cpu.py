@@ -271,3 +271,73 @@ def test_emulated_slow_light_waiting_rays_and_frequencies(seq, emul):
         q, model, cells, times, T_OBS, frequencies=freqs)[0], p, ref2)
     print(parity.assert_parity_grid(got2, ref2, 2, "emulated slow light 2 freqs", max_step_diff=1, ill=ill),
           flush=True)
+
+
+def _slow_fd(p, model, cells, times, kind, h=1e-4):
+    """Central differences of the oracle's slow-light image: kind 0 the density
+    family and 2 the field scale (log-scale h, PlasmaPerturbation), kind 1
+    R_high (relative step h, the derived scalars recomputed per snapshot)."""
+    if kind == 1:
+        res = []
+        for sgn in (1.0, -1.0):
+            m2 = abi.GRMHDModel.from_buffer_copy(model)
+            m2.trat_large = model.trat_large * (1.0 + sgn * h)
+            cs = []
+            for c in cells:
+                c2 = c.copy()
+                oracle.grmhd_derive_scalars(m2, c2)
+                cs.append(c2)
+            res.append(oracle.run_grmhd_slow_light_image(p, m2, cs, times, T_OBS)[0])
+        step = 2.0 * h * model.trat_large
+    else:
+        param = 1 if kind == 0 else 3
+        res = [oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS, perturbation=(param, s * h))[0]
+               for s in (1.0, -1.0)]
+        step = 2.0 * h
+    same = (res[0]["steps"] == res[1]["steps"]) & (res[0]["pass_a_steps"] == res[1]["pass_a_steps"])
+    return {n: (res[0][n] - res[1][n]) / step for n in STOKES}, same
+
+
+def test_emulated_slow_light_gradient_against_central_differences(seq, emul):
+    """The kernel's slow-light tangent lanes: the value equals the slow-light
+    image, and d(I,Q,U,V)/d(q_density, R_high, q_magnetic) along the fixed
+    geodesics matches central differences of the oracle (h = 1e-4) on the rays
+    whose step sequence the perturbation leaves unchanged."""
+    cells, times, model, p = seq
+    L = emul
+    L.kp_emul_run_grmhd_slow_light_gradient.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
+                                                        P(C.c_double), P(abi.ImageData), P(C.c_double),
+                                                        P(C.c_double)]
+    L.kp_emul_run_grmhd_slow_light_gradient.restype = C.c_int32
+    sq, keep = oracle.slow_light_sequence(cells, times, T_OBS)
+    npix = p.camera.nx * p.camera.ny
+    arrays = oracle.alloc_image(npix)
+    img = oracle.image_struct(arrays)
+    grad = np.zeros((3, 4, npix))
+    req = np.zeros(npix)
+    q = np.zeros(2)
+    assert L.kp_emul_run_grmhd_slow_light_gradient(C.byref(p), C.byref(model), C.byref(sq), q.ctypes.data_as(P(C.c_double)),
+                                                   C.byref(img), grad.ctypes.data_as(P(C.c_double)),
+                                                   req.ctypes.data_as(P(C.c_double))) == 0
+    ref, _, _ = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS)
+    ill = parity.sensitivity(lambda qq: oracle.run_grmhd_slow_light_image(qq, model, cells, times, T_OBS)[0], p, ref)
+    print(parity.assert_parity_grid(arrays, ref, 1, "emulated slow-light tangent value", max_step_diff=1, ill=ill),
+          flush=True)
+    # central differences at h = 1e-4 carry an O(h^2) truncation error (up to
+    # 3e-5 of the largest derivative on strongly Faraday-rotated rays); the
+    # Richardson combination (4 D(h/2) - D(h)) / 3 removes it
+    for kind in range(3):
+        fd, same = _slow_fd(p, model, cells, times, kind, 1e-4)
+        fd2, same2 = _slow_fd(p, model, cells, times, kind, 5e-5)
+        ok = same & same2 & ~ill
+        assert ok.sum() >= 30
+        for s, name in enumerate(STOKES):
+            scale = np.abs(fd[name][ok]).max()
+            assert scale > 0.0
+            err = np.abs(grad[kind, s][ok] - fd[name][ok]).max() / scale
+            rich = (4.0 * fd2[name] - fd[name]) / 3.0
+            err_r = np.abs(grad[kind, s][ok] - rich[ok]).max() / scale
+            print(f"slow-light d/d{('q_density', 'R_high', 'q_magnetic')[kind]} {name}: max rel err {err:.2e}, "
+                  f"against Richardson {err_r:.2e}", flush=True)
+            assert err < 1e-4, (kind, name, err)
+            assert err_r < 1e-7, (kind, name, err_r)
