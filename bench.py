@@ -32,6 +32,23 @@ MIXED_TOTAL_I = 1e-4      # mixed precision: relative error of total Stokes I
 MIXED_POL = 1e-3          # mixed precision: relative error of the LP and CP fractions
 
 
+PROFILE = ROOT / "profiles" / "riaf_c2_1024_pass_kernels.md"
+
+
+def profiled_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of a kernel of this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines ``traffic_bytes <kernel>
+    <bytes>``, counters scaled by a copy-kernel calibration).  Counters cannot
+    be read inside a timed run, so the figure is the profiled one."""
+    if not PROFILE.exists():
+        return None
+    for line in PROFILE.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def opcounts() -> dict:
     from kpolaris_amd import build_lib
     return json.loads(build_lib.OPCOUNT.read_text())
@@ -369,7 +386,8 @@ def main() -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": r["achieved_tflops"] / peak,
-            "traffic": None,
+            "traffic": profiled_traffic(r["dominant"] + ("" if r["dominant"] == "pass_a" else "_" + head)),
+            "traffic_source": "profiles/riaf_c2_1024_pass_kernels.md",
             "algorithmic_flops_per_launch": r["dominant_flops"],
             "launch_ms": r["dominant_ms"],
         },
