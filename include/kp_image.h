@@ -216,7 +216,8 @@ typedef struct kp_image_data {
  */
 typedef struct kp_grmhd_model {
     int32_t n1, n2, n3;
-    int32_t emission_type;      /* 4: thermal Dexter (default), 1: thermal Pandya */
+    int32_t emission_type;      /* kp_emission_type: 4 thermal Dexter (default), 1 thermal
+                                   Pandya, 2 kappa, 3 power law */
     double spin, gam;
     double startx1, startx2, startx3, dx1, dx2, dx3;
     double r_in, r_out;
@@ -226,6 +227,17 @@ typedef struct kp_grmhd_model {
     double sigma_cut, sigma_cut_high;
     double ne_factor;
     double emission_scale, absorption_scale, faraday_scale, max_pol_frac;
+    /* nonthermal distributions (grmhd_grid.hpp:79-89, :1176-1205): kappa,
+     * or with variable_kappa != 0 the kappa of sigma and beta, floored at
+     * variable_kappa_min; blended toward thermal from
+     * min(variable_kappa_interp_start, variable_kappa_max) to
+     * variable_kappa_max.  Power law: p, eta (fraction of the magnetic
+     * energy density), Lorentz-factor range. */
+    double nonthermal_kappa;
+    int32_t variable_kappa;
+    int32_t reserved_nt;
+    double variable_kappa_min, variable_kappa_interp_start, variable_kappa_max;
+    double powerlaw_p, powerlaw_eta, powerlaw_gamma_min, powerlaw_gamma_max, powerlaw_gamma_cutoff;
     const void* cells;          /* kp_grmhd_cell[n1 * n2 * n3], device */
 } kp_grmhd_model;
 

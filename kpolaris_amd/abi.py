@@ -162,6 +162,11 @@ class GRMHDModel(C.Structure):
         ("ne_factor", C.c_double),
         ("emission_scale", C.c_double), ("absorption_scale", C.c_double),
         ("faraday_scale", C.c_double), ("max_pol_frac", C.c_double),
+        ("nonthermal_kappa", C.c_double), ("variable_kappa", C.c_int32), ("reserved_nt", C.c_int32),
+        ("variable_kappa_min", C.c_double), ("variable_kappa_interp_start", C.c_double),
+        ("variable_kappa_max", C.c_double), ("powerlaw_p", C.c_double), ("powerlaw_eta", C.c_double),
+        ("powerlaw_gamma_min", C.c_double), ("powerlaw_gamma_max", C.c_double),
+        ("powerlaw_gamma_cutoff", C.c_double),
         ("cells", C.c_void_p),
     ]
 
@@ -295,8 +300,7 @@ def run_options_defaults() -> RunOptions:
 
 
 def grmhd_defaults() -> GRMHDModel:
-    """GRMHDGridRadiationModel field defaults (src/model/grmhd_grid.hpp:53-80),
-    with trat_large = 40 (R_high) as the C3 snapshot uses."""
+    """GRMHDGridRadiationModel field defaults (src/model/grmhd_grid.hpp:53-89)."""
     m = GRMHDModel()
     m.emission_type = EMISSION_THERMAL_DEXTER
     m.spin, m.gam = 0.5, 4.0 / 3.0
@@ -309,5 +313,9 @@ def grmhd_defaults() -> GRMHDModel:
     m.ne_factor = 1.0
     m.emission_scale = m.absorption_scale = m.faraday_scale = 1.0
     m.max_pol_frac = 0.99
+    m.nonthermal_kappa, m.variable_kappa, m.variable_kappa_min = 3.5, 0, 3.1
+    m.variable_kappa_interp_start, m.variable_kappa_max = 1e20, 7.0
+    m.powerlaw_p, m.powerlaw_eta = 3.25, 0.02
+    m.powerlaw_gamma_min, m.powerlaw_gamma_max, m.powerlaw_gamma_cutoff = 1e2, 1e5, 1e10
     m.cells = None
     return m

@@ -30,6 +30,8 @@ struct GrmhdDev {
     double freq_cgs, length_unit, rho_unit, b_unit, ne_factor;
     double trat_small, trat_large, beta_crit;
     int faraday_rotation;
+    int variable_kappa;          // nonthermal kappa from sigma and beta per sample
+    double kappa, kappa_min;     // fixed kappa (floored); the floor of every kappa
     const kp_grmhd_cell* cells;
 };
 
@@ -284,6 +286,9 @@ KP_DEV void grmhd_local_from(const GrmhdDev& G, const St<double>& s, double r, d
     L.ne = ne;
     L.thetae = der[1];
     L.b = der[2];
+    // an invalid sigma or beta leaves the NonthermalSynchrotronParams default 3.5
+    // (nonthermal_synchrotron.hpp:34, :496-498), floored like any kappa
+    L.kappa = G.variable_kappa ? variable_kappa(der[3], der[4], 3.5, G.kappa_min) : G.kappa;
     L.nu_scale = nu_scale;
     L.cos_theta = cos_theta;
     L.sin_theta = maxv(sqrt(maxv(0.0, 1.0 - cos_theta * cos_theta)), 1e-4);
@@ -393,14 +398,26 @@ KP_DEV void grmhd_local_time_dth(const GrmhdDev& G, const kp_grmhd_cell* lo, con
 }
 
 // Coefficients at observing frequency freq from the local plasma state
-// (grmhd_grid.hpp:1150-1240).
-KP_DEV void grmhd_coeffs_at(const GrmhdDev& G, const RiafLocal<double>& L, double freq, Coeffs<double>& c) {
+// (grmhd_grid.hpp:1150-1240); N carries the nonthermal distribution
+// (grmhd_grid.hpp:1176-1205).
+KP_DEV void grmhd_coeffs_at(const GrmhdDev& G, const RiafLocal<double>& L, double freq, Coeffs<double>& c,
+                            const NtDev* N = nullptr) {
     coeffs_zero(c);
     if (!L.valid) return;
     const double nu = maxv(freq * L.nu_scale, 1.0);
     Coeffs<double> mc;
-    thermal_coeffs(G.fit, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, G.max_pol_frac, 1e-4,
-                   G.emission_scale, G.absorption_scale, G.faraday_scale, mc);
+    if (N && (G.fit == KP_EMISSION_KAPPA || G.fit == KP_EMISSION_POWER_LAW)) {
+        if (G.fit == KP_EMISSION_KAPPA && G.variable_kappa) {
+            KappaConst K;
+            kappa_constants(L.kappa, K);
+            nonthermal_coeffs(*N, K, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, mc);
+        } else {
+            nonthermal_coeffs(*N, N->kc, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, mc);
+        }
+    } else {
+        thermal_coeffs(G.fit, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, G.max_pol_frac, 1e-4,
+                       G.emission_scale, G.absorption_scale, G.faraday_scale, mc);
+    }
     c.jI = mc.jI;
     c.jV = mc.jV;
     c.aI = mc.aI;

@@ -351,7 +351,9 @@ void riaf_constants(const kp_pass_a_params& p, const kp_riaf_model& m, int pertu
     }
     o.mass = p.mass;
     o.spin = p.spin;
-    o.fit = m.emission_type == KP_EMISSION_THERMAL_DEXTER ? 4 : 1;
+    // 4 Dexter, 2 kappa, 3 power law (FP64 only); anything else the Pandya fit
+    o.fit = (m.emission_type >= KP_EMISSION_KAPPA && m.emission_type <= KP_EMISSION_THERMAL_DEXTER)
+                ? m.emission_type : KP_EMISSION_THERMAL_PANDYA;
     o.faraday_rotation = p.emission_selection.faraday_rotation;
 }
 
@@ -442,7 +444,7 @@ extern "C" void kp_riaf_model_default(kp_riaf_model* m) {
     m->min_radius = 1e-6;
 }
 
-// GRMHDGridRadiationModel field defaults (src/model/grmhd_grid.hpp:53-80).
+// GRMHDGridRadiationModel field defaults (src/model/grmhd_grid.hpp:53-89).
 extern "C" void kp_grmhd_model_default(kp_grmhd_model* m) {
     if (!m) return;
     std::memset(m, 0, sizeof(*m));
@@ -468,6 +470,16 @@ extern "C" void kp_grmhd_model_default(kp_grmhd_model* m) {
     m->ne_factor = 1.0;
     m->emission_scale = m->absorption_scale = m->faraday_scale = 1.0;
     m->max_pol_frac = 0.99;
+    m->nonthermal_kappa = 3.5;
+    m->variable_kappa = 0;
+    m->variable_kappa_min = 3.1;
+    m->variable_kappa_interp_start = 1e20;
+    m->variable_kappa_max = 7.0;
+    m->powerlaw_p = 3.25;
+    m->powerlaw_eta = 0.02;
+    m->powerlaw_gamma_min = 1e2;
+    m->powerlaw_gamma_max = 1e5;
+    m->powerlaw_gamma_cutoff = 1e10;
     m->cells = nullptr;
 }
 

@@ -119,6 +119,8 @@ struct Params {
     double mass, spin, inner, outer, step, tol, min_step, max_step;
     double max_rs, max_rd, max_ad, max_fd;
     RiafDev<double> riaf;
+    // nonthermal distributions (kappa, power law) of either model
+    NtDev nt;
     // mixed precision: FP32 model constants, FP64 guard radius, 1 / nu_obs^3
     RiafMixed rm;
     float mass_f, spin_f;
@@ -956,7 +958,7 @@ template <> struct Radiation<false, M_SINGLE> {
     Coeffs<double> c;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
-        riaf_coefficients(P.riaf, sample, c);
+        riaf_coefficients(P.riaf, sample, c, &P.nt);
         return 1;
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
@@ -996,7 +998,9 @@ template <> struct Radiation<false, M_MF> {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
-    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const { riaf_coeffs_at(P.riaf, loc, freq, c); }
+    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
+        riaf_coeffs_at(P.riaf, loc, freq, c, &P.nt);
+    }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         riaf_local(P.riaf, sample, loc);
         return 0;
@@ -1006,7 +1010,7 @@ template <> struct Radiation<false, M_MF> {
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<double> c;
-            riaf_coeffs_at(P.riaf, loc, P.ctrl[f], c);
+            riaf_coeffs_at(P.riaf, loc, P.ctrl[f], c, &P.nt);
             req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
         }
@@ -1015,7 +1019,7 @@ template <> struct Radiation<false, M_MF> {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<double> c;
-            riaf_coeffs_at(P.riaf, loc, P.freqs[f], c);
+            riaf_coeffs_at(P.riaf, loc, P.freqs[f], c, &P.nt);
             mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
@@ -1055,7 +1059,9 @@ template <> struct Radiation<false, M_GRMHD> {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
-    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const { grmhd_coeffs_at(P.grmhd, loc, freq, c); }
+    KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
+        grmhd_coeffs_at(P.grmhd, loc, freq, c, &P.nt);
+    }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         grmhd_local(P.grmhd, sample, loc);
         return 0;
@@ -1065,7 +1071,7 @@ template <> struct Radiation<false, M_GRMHD> {
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<double> c;
-            grmhd_coeffs_at(P.grmhd, loc, P.ctrl[f], c);
+            grmhd_coeffs_at(P.grmhd, loc, P.ctrl[f], c, &P.nt);
             req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
         }
@@ -1074,7 +1080,7 @@ template <> struct Radiation<false, M_GRMHD> {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<double> c;
-            grmhd_coeffs_at(P.grmhd, loc, P.freqs[f], c);
+            grmhd_coeffs_at(P.grmhd, loc, P.freqs[f], c, &P.nt);
             mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
@@ -1106,10 +1112,10 @@ template <> struct Radiation<false, M_SLOW> {
     }
     KP_DEV bool valid() const { return lo.valid || hi.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
-        grmhd_coeffs_at(P.grmhd, lo, freq, c);
+        grmhd_coeffs_at(P.grmhd, lo, freq, c, &P.nt);
         if (P.slow_interp != KP_SLOW_LIGHT_FLUID) {
             Coeffs<double> b;
-            grmhd_coeffs_at(P.grmhd, hi, freq, b);
+            grmhd_coeffs_at(P.grmhd, hi, freq, b, &P.nt);
             const double w = 1.0 - f;
             c.jI = w * c.jI + f * b.jI;
             c.jQ = w * c.jQ + f * b.jQ;
@@ -2017,14 +2023,44 @@ static kp_status prepare_common(const kp_pass_a_params* p, Params& P) {
     return KP_OK;
 }
 
+static bool emission_known(int t) {
+    return t == KP_EMISSION_THERMAL_PANDYA || t == KP_EMISSION_THERMAL_DEXTER || t == KP_EMISSION_KAPPA ||
+           t == KP_EMISSION_POWER_LAW;
+}
+
+static bool emission_nonthermal(int t) { return t == KP_EMISSION_KAPPA || t == KP_EMISSION_POWER_LAW; }
+
+// NonthermalSynchrotronParams as the models assemble them (riaf.hpp:497-513,
+// grmhd_grid.hpp:1176-1197), with the kappa- and slope-only constants of the
+// fits evaluated once here (host libm) for a fixed kappa.
+static void fill_nonthermal(NtDev& N, int dist, double kappa, double kappa_min, int variable, double interp_begin,
+                            double interp_end, double max_pol, double min_sin, double es, double as, double fs,
+                            double p, double eta, double gmin, double gmax) {
+    std::memset(&N, 0, sizeof(N));
+    N.dist = dist;
+    N.variable_kappa = variable;
+    N.kappa_min = kappa_min;
+    N.kappa_width = -1.0;
+    N.interp_begin = interp_begin;
+    N.interp_end = interp_end;
+    N.max_pol_em = N.max_pol_abs = max_pol;
+    N.min_sin = min_sin;
+    N.es = es;
+    N.as = as;
+    N.fs = fs;
+    N.p = p;
+    kappa_constants(kappa, N.kc);
+    power_law_constants(p, eta, gmin, gmax, N.pl);
+}
+
 static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Params& P) {
     if (!p || !m) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
     if (p->coordinate_system != KP_COORD_BOYER_LINDQUIST)
         return kp_internal::fail(KP_UNSUPPORTED, "the RIAF image is built only in Boyer-Lindquist coordinates");
     if (p->camera.model != KP_CAMERA_PINHOLE)
         return kp_internal::fail(KP_UNSUPPORTED, "only the pinhole camera is defined in Boyer-Lindquist coordinates");
-    if (m->emission_type != KP_EMISSION_THERMAL_PANDYA && m->emission_type != KP_EMISSION_THERMAL_DEXTER)
-        return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
+    if (!emission_known(m->emission_type))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown emission_type");
     kp_status st = prepare_common(p, P);
     if (st != KP_OK) return st;
     double g5[5];
@@ -2041,6 +2077,10 @@ static kp_status prepare(const kp_pass_a_params* p, const kp_riaf_model* m, Para
     b.g33 = g5[4];
     P.gcam = metric4_of(b);
     kp_internal::riaf_constants(*p, *m, 0, 0.0, P.riaf);
+    fill_nonthermal(P.nt, m->emission_type, m->nonthermal_kappa, 0.0, 0, m->variable_kappa_interp_start,
+                    m->variable_kappa_max, m->max_pol_frac, m->min_sin_theta, m->emission_scale,
+                    m->absorption_scale, m->faraday_scale, m->powerlaw_p, m->powerlaw_eta, m->powerlaw_gamma_min,
+                    m->powerlaw_gamma_max);
     return KP_OK;
 }
 
@@ -2050,8 +2090,8 @@ static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* 
     if (!p || !g) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
     if (p->coordinate_system != KP_COORD_FMKS)
         return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built in FMKS coordinates");
-    if (g->emission_type != KP_EMISSION_THERMAL_PANDYA && g->emission_type != KP_EMISSION_THERMAL_DEXTER)
-        return kp_internal::fail(KP_UNSUPPORTED, "only the thermal Pandya and Dexter fits are implemented");
+    if (!emission_known(g->emission_type))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown emission_type");
     if (g->n1 < 1 || g->n2 < 1 || g->n3 < 2 || (need_cells && !g->cells))
         return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid dimensions and cells are required");
     if (!(g->dx1 > 0.0) || !(g->dx2 > 0.0) || !(g->dx3 > 0.0))
@@ -2100,6 +2140,15 @@ static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* 
     G.beta_crit = g->beta_crit;
     G.faraday_rotation = p->emission_selection.faraday_rotation ? 1 : 0;
     G.cells = static_cast<const kp_grmhd_cell*>(g->cells);
+    // kappa floored at variable_kappa_min, fixed or from sigma and beta
+    // (grmhd_grid.hpp:1185-1192); the blend starts at min(start, max)
+    G.variable_kappa = g->variable_kappa ? 1 : 0;
+    G.kappa_min = g->variable_kappa_min;
+    G.kappa = g->nonthermal_kappa < g->variable_kappa_min ? g->variable_kappa_min : g->nonthermal_kappa;
+    fill_nonthermal(P.nt, g->emission_type, G.kappa, g->variable_kappa_min, G.variable_kappa,
+                    std::min(g->variable_kappa_interp_start, g->variable_kappa_max), g->variable_kappa_max,
+                    g->max_pol_frac, 1e-4, g->emission_scale, g->absorption_scale, g->faraday_scale, g->powerlaw_p,
+                    g->powerlaw_eta, g->powerlaw_gamma_min, g->powerlaw_gamma_max);
     return KP_OK;
 }
 
@@ -2212,6 +2261,9 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         if (st != KP_OK) return st;
         if (perturbation) kp_internal::riaf_constants(*params, *model, perturbation, log_scale, P.riaf);
     }
+    if (emission_nonthermal(P.nt.dist) && (mixed || tangent))
+        return kp_internal::fail(KP_UNSUPPORTED,
+                                 "the kappa and power-law distributions are evaluated in FP64 without tangents");
     if (mixed) {
         if (!(opt.fp64_radius >= 0.0))
             return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
@@ -2568,6 +2620,8 @@ extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, co
             return kp_internal::fail(KP_INVALID_ARGUMENT, "batch radiation_substeps must be positive");
         kp_status st = prepare(&jobs[j].params, &jobs[j].model, table[j]);
         if (st != KP_OK) return st;
+        if (mixed && emission_nonthermal(table[j].nt.dist))
+            return kp_internal::fail(KP_UNSUPPORTED, "the kappa and power-law distributions are evaluated in FP64");
         if (mixed) {
             prepare_mixed(table[j]);
             table[j].fp64_radius = opt.fp64_radius;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "kp_device.h"
+#include "kp_nonthermal.h"
 
 namespace kpd {
 
@@ -156,6 +157,7 @@ template <class T> struct RiafLocal {
     int aligned;  // k exactly along b: only rho_V survives (riaf.hpp:505-512)
     int has_chi;  // the field has a screen projection
     T ne, thetae, b, nu_scale, cos_theta, sin_theta, cos2chi, sin2chi;
+    T kappa;      // GRMHD nonthermal kappa at the sample (from sigma and beta when variable)
 };
 
 // riaf.hpp:410-500 and 529-547 without the frequency.
@@ -221,9 +223,11 @@ template <class T> KP_DEV void riaf_local(const RiafDev<T>& M, const St<T>& s, R
     }
 }
 
-// riaf.hpp:500-549 at observing frequency freq.
+// riaf.hpp:500-549 at observing frequency freq; N carries the nonthermal
+// distribution (kappa, power law: riaf.hpp:497-522) in FP64.
 template <class T>
-KP_DEV void riaf_coeffs_at(const RiafDev<T>& M, const RiafLocal<T>& L, T freq, Coeffs<T>& c) {
+KP_DEV void riaf_coeffs_at(const RiafDev<T>& M, const RiafLocal<T>& L, T freq, Coeffs<T>& c,
+                           const NtDev* N = nullptr) {
     coeffs_zero(c);
     if (!L.valid) return;
     const T nu = maxv(freq * L.nu_scale, T(1));
@@ -234,8 +238,17 @@ KP_DEV void riaf_coeffs_at(const RiafDev<T>& M, const RiafLocal<T>& L, T freq, C
         return;
     }
     Coeffs<T> mc;
-    thermal_coeffs(M.fit, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, M.max_pol_frac, M.min_sin_theta,
-                   M.emission_scale, M.absorption_scale, M.faraday_scale, mc);
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (N && (M.fit == KP_EMISSION_KAPPA || M.fit == KP_EMISSION_POWER_LAW)) {
+            nonthermal_coeffs(*N, N->kc, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, mc);
+        } else {
+            thermal_coeffs(M.fit, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, M.max_pol_frac,
+                           M.min_sin_theta, M.emission_scale, M.absorption_scale, M.faraday_scale, mc);
+        }
+    } else {
+        thermal_coeffs(M.fit, nu, L.ne, L.thetae, L.b, L.sin_theta, L.cos_theta, M.max_pol_frac, M.min_sin_theta,
+                       M.emission_scale, M.absorption_scale, M.faraday_scale, mc);
+    }
     c.jI = mc.jI;
     c.jV = mc.jV;
     c.aI = mc.aI;
@@ -252,10 +265,11 @@ KP_DEV void riaf_coeffs_at(const RiafDev<T>& M, const RiafLocal<T>& L, T freq, C
 }
 
 // RIAFAnalyticRadiationModel::coefficients for the BL metric (riaf.hpp:410-549).
-template <class T> KP_DEV void riaf_coefficients(const RiafDev<T>& M, const St<T>& s, Coeffs<T>& c) {
+template <class T>
+KP_DEV void riaf_coefficients(const RiafDev<T>& M, const St<T>& s, Coeffs<T>& c, const NtDev* N = nullptr) {
     RiafLocal<T> L;
     riaf_local(M, s, L);
-    riaf_coeffs_at(M, L, M.freq_cgs, c);
+    riaf_coeffs_at(M, L, M.freq_cgs, c, N);
 }
 
 }  // namespace kpd

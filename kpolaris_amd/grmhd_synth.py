@@ -99,9 +99,14 @@ def gaussian_field(shape, corr: float, rng) -> np.ndarray:
 def make_snapshot(n1=288, n2=128, n3=128, *, spin=0.9375, r_in=1.2, r_out=1000.0, r_fm_in=6.0,
                   r_fm_max=12.0, gam=4.0 / 3.0, kappa=1e-3, sigma=0.3, corr=4.0, beta_min=100.0,
                   seed=20261202, hslope=0.3, mks_smooth=0.5, poly_alpha=14.0, poly_xt=0.82,
-                  freq_cgs=230e9, mbh_solar=6.5e9, M_unit=1e28, r_high=40.0, r_low=1.0, u_heat=25.0):
+                  freq_cgs=230e9, mbh_solar=6.5e9, M_unit=1e28, r_high=40.0, r_low=1.0, u_heat=25.0,
+                  background_field=0.0):
     """Returns (cells: structured array [n1*n2*n3] of CELL_DTYPE with derived
-    scalars zero, model: abi.GRMHDModel without the device pointer)."""
+    scalars zero, model: abi.GRMHDModel without the device pointer).
+    background_field > 0 adds a uniform toroidal B^3 of that fraction of the
+    largest |B^1| of the loops, so no cell is field-free (the loops vanish
+    outside the torus, where the field direction, and with it the emission
+    angle, is undefined)."""
     rng = np.random.default_rng(seed)
     np.seterr(invalid="ignore", divide="ignore")  # inside the horizon the torus formulas are masked out
     startx1 = math.log(r_in)
@@ -176,6 +181,8 @@ def make_snapshot(n1=288, n2=128, n3=128, *, spin=0.9375, r_in=1.2, r_out=1000.0
     scale = math.sqrt(bmin / beta_min)
     B1 *= scale
     B2 *= scale
+    if background_field > 0.0:
+        B3 = B3 + background_field * float(np.abs(B1).max())
     cells = np.zeros(n1 * n2 * n3, dtype=CELL_DTYPE)
     prim = np.stack([rho, uu, U1, U2, U3, B1, B2, B3], axis=-1).reshape(-1, 8)
     cells["prim"] = prim.astype(np.float32)

@@ -32,6 +32,34 @@ class Coeffs(C.Structure):
         return np.array([getattr(self, n) for n, _ in self._fields_])
 
 
+class Nonthermal(C.Structure):
+    """kpo_nonthermal: NonthermalSynchrotronParams
+    (src/radiation/nonthermal_synchrotron.hpp:17-36), reference defaults."""
+    _fields_ = [("distribution", C.c_int32), ("reserved0", C.c_int32)] + [(n, C.c_double) for n in (
+        "max_pol_frac_emission", "max_pol_frac_absorption", "min_sin_theta", "emission_scale", "absorption_scale",
+        "faraday_scale", "power_law_p", "power_law_eta", "power_law_gamma_min", "power_law_gamma_max",
+        "power_law_gamma_cutoff", "kappa", "kappa_width", "kappa_interp_begin", "kappa_interp_end")]
+
+    def __init__(self, distribution: int = 3, **kw):
+        super().__init__()
+        self.distribution = distribution
+        self.max_pol_frac_emission = self.max_pol_frac_absorption = 0.99
+        self.min_sin_theta = 1e-4
+        self.emission_scale = self.absorption_scale = self.faraday_scale = 1.0
+        self.power_law_p, self.power_law_eta = 3.25, 0.02
+        self.power_law_gamma_min, self.power_law_gamma_max, self.power_law_gamma_cutoff = 1e2, 1e5, 1e10
+        self.kappa, self.kappa_width, self.kappa_interp_begin, self.kappa_interp_end = 3.5, -1.0, 1e20, 7.0
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+def nonthermal_coefficients(params: "Nonthermal", nu: float, ne: float, thetae: float, b: float,
+                            sin_theta: float, cos_theta: float) -> "Coeffs":
+    c = Coeffs()
+    lib().kpo_nonthermal_coefficients(C.byref(params), nu, ne, thetae, b, sin_theta, cos_theta, C.byref(c))
+    return c
+
+
 class Stokes(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("I", "Q", "U", "V")]
 
@@ -78,6 +106,11 @@ def lib():
     L.kpo_bl_camera_ray.argtypes = [C.POINTER(abi.PassAParams), i32, sp]
     L.kpo_bl_camera_tetrad.argtypes = [C.POINTER(abi.PassAParams), dp]
     L.kpo_thermal_coefficients.argtypes = [i32] + [d] * 12 + [C.POINTER(Coeffs)]
+    L.kpo_nonthermal_coefficients.argtypes = [C.POINTER(Nonthermal)] + [d] * 6 + [C.POINTER(Coeffs)]
+    L.kpo_hyp2f1.argtypes = [d] * 4
+    L.kpo_hyp2f1.restype = d
+    L.kpo_variable_kappa.argtypes = [d] * 3
+    L.kpo_variable_kappa.restype = d
     L.kpo_symphony_rho_v.argtypes = [d] * 6
     L.kpo_symphony_rho_v.restype = d
     L.kpo_bessel_k012.argtypes = [d, dp, dp, dp]

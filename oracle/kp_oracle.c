@@ -1321,6 +1321,330 @@ void kpo_thermal_coefficients(int32_t fit, double nu_in, double ne, double theta
 }
 
 /* ======================================================================
+ * Nonthermal synchrotron: kappa and power-law distributions
+ * (src/radiation/nonthermal_synchrotron.hpp).  The fits are Pandya et al.
+ * (2016) for j and alpha and Marszewski et al. (2021) for the kappa rho;
+ * their coefficients are the published ones.  The header carries its own
+ * CGS constants (:106-110, :125-127), which differ from the thermal fits'
+ * in the last digits, so they are kept separately here.
+ * ==================================================================== */
+#define NT_EE 4.80320680e-10
+#define NT_ME 9.1093826e-28
+
+static double nt_nonzero(double x) {
+    if (absv(x) < TINY) return x >= 0.0 ? TINY : -TINY;
+    return x;
+}
+
+/* hyp2f1_series (nonthermal_synchrotron.hpp:55-73): the Gauss series,
+ * at most 200 terms, stopped at |term| < 1e-13 |sum|. */
+static double nt_hyp2f1_series(double a, double b, double c, double z) {
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n <= 200; ++n) {
+        const double m = (double)n - 1.0;
+        term *= (a + m) * (b + m) * z / nt_nonzero((c + m) * (double)n);
+        sum += term;
+        if (absv(term) < absv(sum) * 1e-13) break;
+    }
+    return sum;
+}
+
+/* stable_hyp2f1 (:83-96): the series inside (-1, 1); outside, the 1/(1-z)
+ * transformation (Abramowitz & Stegun 15.3.8) with two series in 1/(1-z). */
+double kpo_hyp2f1(double a, double b, double c, double z) {
+    if (z > -1.0 && z < 1.0) return nt_hyp2f1_series(a, b, c, z);
+    const double w = 1.0 - z;
+    const double zp = 1.0 / w;
+    const double gc = tgamma(c);
+    const double t1 = pow(w, -a) * gc * tgamma(b - a) / nt_nonzero(tgamma(b) * tgamma(c - a)) *
+                      nt_hyp2f1_series(a, c - b, a - b + 1.0, zp);
+    const double t2 = pow(w, -b) * gc * tgamma(a - b) / nt_nonzero(tgamma(a) * tgamma(c - b)) *
+                      nt_hyp2f1_series(b, c - a, b - a + 1.0, zp);
+    return t1 + t2;
+}
+
+/* The common shape of the kappa fits (:238-250 and their siblings):
+ * pref * N_low * X^low_power * (1 + X^(x * high_power) * (N_low / N_high)^x)^(-1/x). */
+static double nt_blend(double pref, double n_low, double n_high, double X, double low_power,
+                       double high_power, double x) {
+    return pref * n_low * pow(X, low_power) * pow(1.0 + pow(X, x * high_power) * pow(n_low / n_high, x), -1.0 / x);
+}
+
+typedef struct nt_local {
+    double nu, ne, b, sin_raw, cos_theta;  /* state as given (nu unclamped) */
+    double nuc, sint, w, X;                /* derived */
+} nt_local;
+
+/* nu_c = e B / (2 pi m_e c) (:101-108). */
+static double nt_nu_c(double b) { return NT_EE * b / (2.0 * PI_D * NT_ME * CL); }
+
+/* ---- power law (:112-215) ---- */
+static double pl_density(double b, const kpo_nonthermal* p) {
+    if (!(p->power_law_p > 2.0) || !(p->power_law_gamma_min >= 1.0)) return 0.0;
+    const double u_nth = p->power_law_eta * b * b / 2.0;
+    return u_nth * (p->power_law_p - 2.0) / (p->power_law_p - 1.0) /
+           maxv(NT_ME * CL * CL * p->power_law_gamma_min, TINY);
+}
+
+static double pl_gamma_span(const kpo_nonthermal* p) {
+    return pow(p->power_law_gamma_min, 1.0 - p->power_law_p) - pow(p->power_law_gamma_max, 1.0 - p->power_law_p);
+}
+
+static double pl_j_i(const nt_local* s, const kpo_nonthermal* p, double n) {
+    const double pp = p->power_law_p;
+    if (!(s->nu > 0.0) || !(s->nuc > 0.0) || !(n > 0.0) || !(pp > 1.0)) return 0.0;
+    const double pref = n * NT_EE * NT_EE * s->nuc / CL;
+    const double t1 = pow(3.0, pp / 2.0) * (pp - 1.0) * s->sint;
+    const double t2 = 2.0 * (pp + 1.0) * pl_gamma_span(p);
+    const double t3 = tgamma((3.0 * pp - 1.0) / 12.0) * tgamma((3.0 * pp + 19.0) / 12.0);
+    const double t4 = pow(s->nu / maxv(s->nuc * s->sint, TINY), -(pp - 1.0) / 2.0);
+    return pref * t1 / maxv(t2, TINY) * t3 * t4;
+}
+
+static double pl_j_v(const nt_local* s, const kpo_nonthermal* p, double jI) {
+    double cost = s->cos_theta;
+    if (absv(cost) < 1e-12) cost = cost >= 0.0 ? 1e-12 : -1e-12;
+    const double t1 = -(171.0 / 250.0) * pow(p->power_law_p, 49.0 / 100.0);
+    const double t2 = cost / s->sint * pow(s->nu / maxv(3.0 * s->nuc * s->sint, TINY), -0.5);
+    return -t1 * t2 * jI;
+}
+
+static double pl_a_i(const nt_local* s, const kpo_nonthermal* p, double n) {
+    const double pp = p->power_law_p;
+    if (!(s->nu > 0.0) || !(s->nuc > 0.0) || !(n > 0.0) || !(pp > 1.0)) return 0.0;
+    const double pref = n * NT_EE * NT_EE / maxv(s->nu * NT_ME * CL, TINY);
+    const double t1 = pow(3.0, (pp + 1.0) / 2.0) * (pp - 1.0);
+    const double t2 = 4.0 * pl_gamma_span(p);
+    const double t3 = tgamma((3.0 * pp + 2.0) / 12.0) * tgamma((3.0 * pp + 22.0) / 12.0);
+    const double t4 = pow(s->nu / maxv(s->nuc * s->sint, TINY), -(pp + 2.0) / 2.0);
+    return pref * t1 / maxv(t2, TINY) * t3 * t4;
+}
+
+static double pl_a_v(const nt_local* s, const kpo_nonthermal* p, double aI) {
+    const double t5 = -pow((71.0 / 100.0) * p->power_law_p + 22.0 / 625.0, 197.0 / 500.0);
+    const double t6 = pow(maxv((31.0 / 10.0) * pow(s->sint, -48.0 / 25.0) - 31.0 / 10.0, 0.0), 64.0 / 125.0);
+    const double t7 = pow(s->nu / maxv(s->nuc * s->sint, TINY), -0.5);
+    const double sign = s->cos_theta >= 0.0 ? 1.0 : -1.0;
+    return -aI * t5 * t6 * t7 * sign;
+}
+
+/* ---- kappa (:217-489) ---- */
+/* Emission normalisations of the I fit (:226-231): low- and high-frequency limits. */
+static double kappa_jn_low(double k) {
+    return 4.0 * PI_D * tgamma(k - 4.0 / 3.0) / (pow(3.0, 7.0 / 3.0) * tgamma(k - 2.0));
+}
+static double kappa_jn_high(double k) {
+    return 0.25 * pow(3.0, (k - 1.0) / 2.0) * (k - 2.0) * (k - 1.0) * tgamma(k / 4.0 - 1.0 / 3.0) *
+           tgamma(k / 4.0 + 4.0 / 3.0);
+}
+/* Absorption normalisations of the I fit (:327-336), without hyp2f1 in the low one. */
+static double kappa_an_low(double k, double w) {
+    return pow(3.0, 1.0 / 6.0) * 10.0 / 41.0 * pow(2.0 * PI_D, 2.0) / pow(w * k, 16.0 / 3.0 - k) *
+           (k - 2.0) * (k - 1.0) * k / (3.0 * k - 1.0) * tgamma(5.0 / 3.0);
+}
+static double kappa_an_high(double k, double w) {
+    return 2.0 * pow(PI_D, 2.5) / 3.0 * (k - 2.0) * (k - 1.0) * k / pow(w * k, 5.0) *
+           (2.0 * tgamma(2.0 + k / 2.0) / (2.0 + k) - 1.0);
+}
+
+static double kappa_j(const nt_local* s, double k, int stokes) {
+    if (!(k > 2.0) || !(s->w > 0.0) || !(s->X > 0.0)) return 0.0;
+    const double pref = s->ne * NT_EE * NT_EE * s->nuc * s->sint / CL;
+    const double lo = kappa_jn_low(k), hi = kappa_jn_high(k);
+    const double high_power = (3.0 * k - 4.0) / 6.0;
+    if (stokes == 0) return nt_blend(pref, lo, hi, s->X, 1.0 / 3.0, high_power, 3.0 * pow(k, -1.5));
+    if (stokes == 1)
+        return nt_blend(pref, -0.5 * lo, -(pow(4.0 / 5.0, 2.0) + k / 50.0) * hi, s->X, 1.0 / 3.0, high_power,
+                        3.7 * pow(k, -1.6));
+    const double angle1 = maxv(pow(s->sint, -12.0 / 5.0) - 1.0, 0.0);
+    const double angle2 = maxv(pow(s->sint, -5.0 / 2.0) - 1.0, 0.0);
+    const double n_low = -pow(3.0 / 4.0, 2.0) * pow(angle1, 12.0 / 25.0) * pow(k, -66.0 / 125.0) / s->w *
+                         pow(s->X, -7.0 / 20.0) * lo;
+    const double n_high = -pow(7.0 / 8.0, 2.0) * pow(angle2, 11.0 / 25.0) * pow(k, -11.0 / 25.0) / s->w *
+                          pow(s->X, -0.5) * hi;
+    const double v = nt_blend(pref, n_low, n_high, s->X, 1.0 / 3.0, high_power, 3.0 * pow(k, -1.5));
+    return -v * (s->cos_theta >= 0.0 ? 1.0 : -1.0);
+}
+
+static double kappa_a(const nt_local* s, double k, int stokes) {
+    const double pref = s->ne * NT_EE / maxv(s->b * s->sint, TINY);
+    const double hyp = kpo_hyp2f1(k - 1.0 / 3.0, k + 1.0, k + 2.0 / 3.0, -k * s->w);
+    if (!(k > 2.0) || !(s->w > 0.0) || !(s->X > 0.0) || !(absv(hyp) > TINY)) return 0.0;
+    const double lo = kappa_an_low(k, s->w) * hyp, hi = kappa_an_high(k, s->w);
+    const double high_power = (3.0 * k - 1.0) / 6.0;
+    if (stokes == 0) {
+        const double x = pow(-7.0 / 4.0 + 8.0 * k / 5.0, -43.0 / 50.0);
+        return nt_blend(pref, lo, hi * (pow(3.0 / k, 19.0 / 4.0) + 3.0 / 5.0) + TINY, s->X, -5.0 / 3.0,
+                        high_power, x);
+    }
+    if (stokes == 1) {
+        const double x = 7.0 / 5.0 * pow(k, -23.0 / 20.0);
+        return nt_blend(pref, -25.0 / 48.0 * lo, -(pow(21.0, 2.0) * pow(k, -144.0 / 25.0) + 11.0 / 20.0) * hi + TINY,
+                        s->X, -5.0 / 3.0, high_power, x);
+    }
+    const double angle1 = maxv(pow(s->sint, -114.0 / 50.0) - 1.0, 0.0);
+    const double angle2 = maxv(pow(s->sint, -41.0 / 20.0) - 1.0, 0.0);
+    const double n_low = -77.0 / (100.0 * s->w) * pow(angle1, 223.0 / 500.0) * pow(s->X, -7.0 / 20.0) *
+                         pow(k, -7.0 / 10.0) * lo;
+    const double nfac = 13.0 * 13.0 * pow(k, -8.0) + 13.0 / 2500.0 * k - 263.0 / 5000.0 + 47.0 / (200.0 * k);
+    const double n_high = -143.0 / 10.0 * pow(s->w, -116.0 / 125.0) * pow(angle2, 0.5) * nfac * pow(s->X, -0.5) *
+                          hi + TINY;
+    const double x = 61.0 / 50.0 * pow(k, -142.0 / 125.0) + 7.0 / 1000.0;
+    const double v = nt_blend(pref, n_low, n_high, s->X, -5.0 / 3.0, high_power, x);
+    return -v * (s->cos_theta >= 0.0 ? 1.0 : -1.0);
+}
+
+/* kappa_rho_Q_at / kappa_rho_V_at (:393-460): the fast rho fits at the
+ * anchors kappa = 3.5, 4, 4.5, 5 (Marszewski et al. 2021, table 2). */
+static double kappa_rho_q_at(const nt_local* s, int anchor) {
+    if (!(s->X > 0.0)) return 0.0;
+    const double w = s->w, X = s->X, sw = sqrt(w);
+    const double pref = -(s->ne * NT_EE * NT_EE * s->nuc * s->nuc * s->sint * s->sint) /
+                        maxv(NT_ME * CL * s->nu * s->nu * s->nu, TINY);
+    static const double wa[4] = {17.0, 46.0 / 3.0, 14.0, 25.0 / 2.0};
+    static const double wb[4] = {3.0, 5.0 / 3.0, 13.0 / 8.0, 1.0};
+    static const double wc[4] = {7.0, 17.0 / 3.0, 9.0 / 2.0, 5.0};
+    static const double fd[4] = {30.0, 18.0, 12.0, 8.0};
+    const double wterm = wa[anchor] * w - wb[anchor] * sw + wc[anchor] * sw * exp(-5.0 * w);
+    double wave;
+    switch (anchor) {
+        case 0: wave = sin(X / 10.0) * exp(-1.5 * pow(X, 0.471)); break;
+        case 1: wave = sin(X / 6.0) * exp(-7.0 * sqrt(X) / 4.0); break;
+        case 2: wave = sin(X / 4.0) * exp(-2.0 * pow(X, 0.525)); break;
+        default: wave = sin(3.0 * X / 8.0) * exp(-9.0 * pow(X, 0.541) / 4.0); break;
+    }
+    const double fx = 1.0 - exp(-pow(X, 0.84) / fd[anchor]) - wave;
+    return pref * fx * wterm;
+}
+
+static double kappa_rho_v_at(const nt_local* s, int anchor) {
+    if (!(s->X > 0.0) || !(s->w > 0.0)) return 0.0;
+    const double w = s->w;
+    const double pref = 2.0 * s->ne * NT_EE * NT_EE * s->nuc * s->cos_theta / maxv(NT_ME * CL * s->nu * s->nu, TINY);
+    double k0 = 0.0, k1 = 0.0, k2 = 0.0;
+    kpo_bessel_k012(1.0 / w, &k0, &k1, &k2);
+    const double bessel = k0 / maxv(k2 + TINY, TINY);
+    static const double gc[4] = {0.447, 0.391, 0.348, 0.313};
+    double wterm;
+    switch (anchor) {
+        case 0: wterm = (w * w + 2.0 * w + 1.0) / (25.0 / 8.0 * w * w + 4.0 * w + 1.0); break;
+        case 1: wterm = (w * w + 54.0 * w + 50.0) / (30.0 / 11.0 * w * w + 134.0 * w + 50.0); break;
+        case 2: wterm = (w * w + 43.0 * w + 38.0) / (7.0 / 3.0 * w * w + 185.0 / 2.0 * w + 38.0); break;
+        default: wterm = (w + 13.0 / 14.0) / (2.0 * w + 13.0 / 14.0); break;
+    }
+    const double gx = 1.0 - 0.17 * log(1.0 + gc[anchor] * pow(s->X, -0.5));
+    return pref * bessel * gx * wterm;
+}
+
+/* interpolate_kappa_rho_{Q,V} (:462-489): linear in kappa between the
+ * anchors, zero outside [3.5, 5] where the fast fit is not defined. */
+static double kappa_rho(const nt_local* s, double k, int stokes) {
+    if (k < 3.5 || k > 5.0) return 0.0;
+    double (*at)(const nt_local*, int) = stokes == 1 ? kappa_rho_q_at : kappa_rho_v_at;
+    if (k >= 5.0) return at(s, 3);
+    const int seg = k < 4.0 ? 0 : (k < 4.5 ? 1 : 2);
+    const double k_lo = 3.5 + 0.5 * seg, k_hi = k_lo + 0.5;
+    return ((k_hi - k) * at(s, seg) + (k - k_lo) * at(s, seg + 1)) / 0.5;
+}
+
+/* polarised-fraction cap of emission or absorption (:540-545, :557-562, :595-606) */
+static void nt_cap(double i, double* q, double* v, double frac) {
+    const double pol = sqrt(*q * *q + *v * *v);
+    if (i > 0.0 && pol > 0.0 && i < pol / maxv(frac, 1e-30)) {
+        const double s = i / pol * frac;
+        *q *= s;
+        *v *= s;
+    }
+}
+
+/* variable_kappa_from_sigma_beta (:493-502). */
+double kpo_variable_kappa(double sigma, double beta, double kappa_default) {
+    if (!(sigma > 0.0) || !(beta >= 0.0)) return kappa_default;
+    const double k = 2.8 + 0.7 / sqrt(sigma) + 3.7 * pow(sigma, -0.19) * tanh(23.4 * pow(sigma, 0.26) * beta);
+    return k < 3.1 ? 3.1 : k;
+}
+
+/* nonthermal_synchrotron_magnetic_basis_coefficients (:504-618): invariant
+ * coefficients (j / nu^2, alpha nu, rho nu) in the magnetic screen basis. */
+void kpo_nonthermal_coefficients(const kpo_nonthermal* p, double nu_in, double ne, double thetae, double b,
+                                 double sin_theta, double cos_theta, kpo_coeffs* c) {
+    memset(c, 0, sizeof(*c));
+    if (!(nu_in > 0.0) || !(b > 0.0) || !(ne > 0.0)) return;
+    const double nu = maxv(nu_in, 1.0);
+    const double inu2 = 1.0 / maxv(nu * nu, TINY);
+    nt_local s;
+    s.nu = nu_in;
+    s.ne = ne;
+    s.b = b;
+    s.sin_raw = sin_theta;
+    s.cos_theta = cos_theta;
+    s.nuc = nt_nu_c(b);
+    s.sint = maxv(sin_theta, p->min_sin_theta);
+    if (p->distribution == KP_EMISSION_POWER_LAW) {
+        const double n = pl_density(b, p);
+        const double jI = pl_j_i(&s, p, n);
+        const double jQ_phys = -(p->power_law_p + 1.0) / (p->power_law_p + 7.0 / 3.0) * jI;
+        double ji = jI * inu2;
+        double jq = -jQ_phys * inu2;
+        double jv = pl_j_v(&s, p, jI) * inu2;
+        nt_cap(ji, &jq, &jv, p->max_pol_frac_emission);
+        const double aI = pl_a_i(&s, p, n);
+        const double qfac = -pow((17.0 / 500.0) * p->power_law_p - 43.0 / 1250.0, 43.0 / 500.0);
+        double ai = aI * nu;
+        double aq = -(aI * qfac) * nu;
+        double av = pl_a_v(&s, p, aI) * nu;
+        nt_cap(ai, &aq, &av, p->max_pol_frac_absorption);
+        c->jI = p->emission_scale * ji;
+        c->jQ = p->emission_scale * jq;
+        c->jV = p->emission_scale * jv;
+        c->aI = p->absorption_scale * ai;
+        c->aQ = p->absorption_scale * aq;
+        c->aV = p->absorption_scale * av;
+        return;  /* the fast fits carry no power-law rotativity */
+    }
+    if (p->distribution != KP_EMISSION_KAPPA) return;
+    const double k = p->kappa;
+    /* kappa_width <= 0 selects the width that keeps the mean energy of the
+     * thermal distribution at Theta_e (:522-524, :217-222) */
+    s.w = p->kappa_width > 0.0 ? p->kappa_width : (k - 3.0) / maxv(k, TINY) * thetae;
+    s.X = nu_in / maxv(s.w * s.w * k * k * s.nuc * s.sint, TINY);
+    kpo_coeffs kc;
+    kc.jI = kappa_j(&s, k, 0) * inu2;
+    kc.jQ = -kappa_j(&s, k, 1) * inu2;
+    kc.jV = kappa_j(&s, k, 3) * inu2;
+    kc.aI = kappa_a(&s, k, 0) * nu;
+    kc.aQ = -kappa_a(&s, k, 1) * nu;
+    kc.aV = kappa_a(&s, k, 3) * nu;
+    kc.rQ = kappa_rho(&s, k, 1) * nu;
+    kc.rV = kappa_rho(&s, k, 3) * nu;
+    double mix = 0.0;
+    if (k >= p->kappa_interp_begin && k < p->kappa_interp_end)
+        mix = (k - p->kappa_interp_begin) / maxv(p->kappa_interp_end - p->kappa_interp_begin, TINY);
+    else if (k >= p->kappa_interp_end)
+        mix = 1.0;
+    /* the thermal Pandya fit it blends toward, with unit scales (:525-540) */
+    kpo_coeffs th;
+    kpo_thermal_coefficients(KP_EMISSION_THERMAL_PANDYA, nu_in, ne, thetae, b, 0.0, sin_theta, cos_theta,
+                             p->max_pol_frac_emission, p->min_sin_theta, 1.0, 1.0, 1.0, &th);
+    double ji = (1.0 - mix) * kc.jI + mix * th.jI;
+    double jq = (1.0 - mix) * kc.jQ + mix * th.jQ;
+    double jv = (1.0 - mix) * kc.jV + mix * th.jV;
+    double ai = (1.0 - mix) * kc.aI + mix * th.aI;
+    double aq = (1.0 - mix) * kc.aQ + mix * th.aQ;
+    double av = (1.0 - mix) * kc.aV + mix * th.aV;
+    nt_cap(ji, &jq, &jv, p->max_pol_frac_emission);
+    nt_cap(ai, &aq, &av, p->max_pol_frac_absorption);
+    c->jI = p->emission_scale * ji;
+    c->jQ = p->emission_scale * jq;
+    c->jV = p->emission_scale * jv;
+    c->aI = p->absorption_scale * ai;
+    c->aQ = p->absorption_scale * aq;
+    c->aV = p->absorption_scale * av;
+    c->rQ = p->faraday_scale * kc.rQ;
+    c->rV = p->faraday_scale * kc.rV;
+}
+
+/* ======================================================================
  * RIAF model (src/model/riaf.hpp), BL metric branches
  * ==================================================================== */
 /* density_profile_bl (riaf.hpp:54-60). */
@@ -1546,12 +1870,34 @@ void kpo_riaf_coefficients(const kp_pass_a_params* p, const kp_riaf_model* m,
         c->rV = m->faraday_scale * kpo_symphony_rho_v(nu, ne, thetae, b, theta, m->min_sin_theta) * nu;
         return;
     }
-    const int fit = m->emission_type == KP_EMISSION_THERMAL_DEXTER ? KP_EMISSION_THERMAL_DEXTER
-                                                                    : KP_EMISSION_THERMAL_PANDYA;
     kpo_coeffs mc;
-    kpo_thermal_coefficients(fit, nu, ne, thetae, b, theta, sin_theta, cos_theta, m->max_pol_frac,
-                             m->min_sin_theta, m->emission_scale, m->absorption_scale,
-                             m->faraday_scale, &mc);
+    if (m->emission_type == KP_EMISSION_KAPPA || m->emission_type == KP_EMISSION_POWER_LAW) {
+        /* riaf.hpp:497-522: fixed kappa, the width from Theta_e */
+        kpo_nonthermal nt;
+        memset(&nt, 0, sizeof(nt));
+        nt.distribution = m->emission_type;
+        nt.max_pol_frac_emission = nt.max_pol_frac_absorption = m->max_pol_frac;
+        nt.min_sin_theta = m->min_sin_theta;
+        nt.emission_scale = m->emission_scale;
+        nt.absorption_scale = m->absorption_scale;
+        nt.faraday_scale = m->faraday_scale;
+        nt.kappa = m->nonthermal_kappa;
+        nt.kappa_width = -1.0;
+        nt.kappa_interp_begin = m->variable_kappa_interp_start;
+        nt.kappa_interp_end = m->variable_kappa_max;
+        nt.power_law_p = m->powerlaw_p;
+        nt.power_law_eta = m->powerlaw_eta;
+        nt.power_law_gamma_min = m->powerlaw_gamma_min;
+        nt.power_law_gamma_max = m->powerlaw_gamma_max;
+        nt.power_law_gamma_cutoff = m->powerlaw_gamma_cutoff;
+        kpo_nonthermal_coefficients(&nt, nu, ne, thetae, b, sin_theta, cos_theta, &mc);
+    } else {
+        const int fit = m->emission_type == KP_EMISSION_THERMAL_DEXTER ? KP_EMISSION_THERMAL_DEXTER
+                                                                        : KP_EMISSION_THERMAL_PANDYA;
+        kpo_thermal_coefficients(fit, nu, ne, thetae, b, theta, sin_theta, cos_theta, m->max_pol_frac,
+                                 m->min_sin_theta, m->emission_scale, m->absorption_scale,
+                                 m->faraday_scale, &mc);
+    }
     const double b1 = screen_inner(g, ucon, k, s->v + 8, bcon);
     const double b2 = screen_inner(g, ucon, k, s->v + 12, bcon);
     const double bproj2 = b1 * b1 + b2 * b2;
@@ -2168,7 +2514,7 @@ static void grmhd_coefficients_from(const kp_grmhd_model* g, const kpo_state* s,
                                           bcon);
     const double bnorm_geom = sqrt(bsq);
     double ne = der[0], thetae = der[1], b_cgs = der[2];
-    const double sigma = der[3];
+    double sigma = der[3], beta = der[4];
     ne *= sigma_smooth_factor(g, sigma);
     /* PlasmaPerturbation::apply (plasma_perturbation.hpp:17-32) */
     if (pert && pert->parameter != 0 && pert->log_scale != 0.0) {
@@ -2180,6 +2526,8 @@ static void grmhd_coefficients_from(const kp_grmhd_model* g, const kpo_state* s,
             thetae *= fq;
         } else if (pert->parameter == 3) {
             b_cgs *= fq;
+            sigma *= fq * fq;
+            beta /= fq * fq;
         }
     }
     const double* k = s->v + 4;
@@ -2216,8 +2564,35 @@ static void grmhd_coefficients_from(const kp_grmhd_model* g, const kpo_state* s,
     const double cos_theta = clampv(kdotb / maxv(nu_scale * bnorm, 1e-30), -1.0, 1.0);
     const double sin_theta = maxv(sqrt(maxv(0.0, 1.0 - cos_theta * cos_theta)), 1e-4);
     kpo_coeffs mc;
-    kpo_thermal_coefficients(g->emission_type, nu, ne, thetae, b_cgs, 0.0, sin_theta, cos_theta, g->max_pol_frac,
-                             1e-4, g->emission_scale, g->absorption_scale, g->faraday_scale, &mc);
+    if (g->emission_type == KP_EMISSION_KAPPA || g->emission_type == KP_EMISSION_POWER_LAW) {
+        /* grmhd_grid.hpp:1176-1205 */
+        kpo_nonthermal nt;
+        memset(&nt, 0, sizeof(nt));
+        nt.distribution = g->emission_type;
+        nt.max_pol_frac_emission = nt.max_pol_frac_absorption = g->max_pol_frac;
+        nt.min_sin_theta = 1e-4;
+        nt.emission_scale = g->emission_scale;
+        nt.absorption_scale = g->absorption_scale;
+        nt.faraday_scale = g->faraday_scale;
+        /* variable_kappa_from_sigma_beta falls back to the params' kappa, still
+         * the NonthermalSynchrotronParams default 3.5 there (:34, :496-498) */
+        nt.kappa = g->variable_kappa ? kpo_variable_kappa(sigma, beta, 3.5) : g->nonthermal_kappa;
+        if (nt.kappa < g->variable_kappa_min) nt.kappa = g->variable_kappa_min;
+        nt.kappa_width = -1.0;
+        nt.kappa_interp_begin = g->variable_kappa_interp_start < g->variable_kappa_max
+                                    ? g->variable_kappa_interp_start : g->variable_kappa_max;
+        nt.kappa_interp_end = g->variable_kappa_max;
+        nt.power_law_p = g->powerlaw_p;
+        nt.power_law_eta = g->powerlaw_eta;
+        nt.power_law_gamma_min = g->powerlaw_gamma_min;
+        nt.power_law_gamma_max = g->powerlaw_gamma_max;
+        nt.power_law_gamma_cutoff = g->powerlaw_gamma_cutoff;
+        kpo_nonthermal_coefficients(&nt, nu, ne, thetae, b_cgs, sin_theta, cos_theta, &mc);
+    } else {
+        kpo_thermal_coefficients(g->emission_type, nu, ne, thetae, b_cgs, 0.0, sin_theta, cos_theta,
+                                 g->max_pol_frac, 1e-4, g->emission_scale, g->absorption_scale, g->faraday_scale,
+                                 &mc);
+    }
     const double ubunit = ub * inv_bnorm;
     const double kbunit = kdotb * inv_bnorm;
     double b1 = e1b * inv_bnorm;

@@ -92,6 +92,20 @@ void kpo_thermal_coefficients(int32_t fit, double nu, double ne, double thetae,
                               double min_sin_theta, double emission_scale,
                               double absorption_scale, double faraday_scale,
                               kpo_coeffs* out);
+/* NonthermalSynchrotronParams (src/radiation/nonthermal_synchrotron.hpp:17-36);
+ * distribution 2 = kappa, 3 = power law. */
+typedef struct kpo_nonthermal {
+    int32_t distribution;
+    int32_t reserved0;
+    double max_pol_frac_emission, max_pol_frac_absorption, min_sin_theta;
+    double emission_scale, absorption_scale, faraday_scale;
+    double power_law_p, power_law_eta, power_law_gamma_min, power_law_gamma_max, power_law_gamma_cutoff;
+    double kappa, kappa_width, kappa_interp_begin, kappa_interp_end;
+} kpo_nonthermal;
+void kpo_nonthermal_coefficients(const kpo_nonthermal* params, double nu, double ne, double thetae,
+                                 double b_cgs, double sin_theta, double cos_theta, kpo_coeffs* out);
+double kpo_hyp2f1(double a, double b, double c, double z);
+double kpo_variable_kappa(double sigma, double beta, double kappa_default);
 double kpo_symphony_rho_v(double nu, double ne, double thetae, double b_cgs,
                           double theta, double min_sin_theta);
 void kpo_bessel_k012(double x, double* k0, double* k1, double* k2);
