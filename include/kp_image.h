@@ -496,6 +496,19 @@ kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_params* params,
                                                  const kp_run_options* options, kp_image_data* out,
                                                  double* grad, kp_image_stats* stats);
 
+/* Static GRMHD image and its derivatives along the fixed geodesics with
+ * respect to (q_density, R_high, q_magnetic), as
+ * kp_run_grmhd_slow_light_image_gradient defines them, for one snapshot:
+ * the plasma responses of a GRMHD image (PlasmaPerturbation,
+ * src/radiation/plasma_perturbation.hpp:12-33; response runs,
+ * src/diagnostics/response.hpp:118-191).  The snapshot is imaged as the
+ * two-snapshot sequence of itself, whose fluid interpolation returns the
+ * snapshot's values unchanged (src/model/time_interpolation.hpp:27-32).
+ * log_scales = {q_density, q_magnetic}; grad is a DEVICE [3][4][npix] array. */
+kp_status kp_run_grmhd_image_gradient(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                      const double* log_scales, const kp_run_options* options,
+                                      kp_image_data* out, double* grad, kp_image_stats* stats);
+
 /* Time support of a slow-light image: pass A endpoints, then geometric
  * stepping to the camera without radiation, recording the x^0 range of the
  * samples inside [inner_radius, outer_radius); the fluid times needed are

@@ -78,3 +78,33 @@ def slow_light_stokes(theta: torch.Tensor, params, grids, times, observation_tim
     """Differentiable slow-light Stokes image [4, npix] at
     theta = (q_density, R_high, q_magnetic) (3,)."""
     return SlowLightStokes.apply(theta, params, grids, times, observation_time, step_mode, interpolation)
+
+
+class GrmhdStokes(torch.autograd.Function):
+    """Static GRMHD Stokes image [4, npix] at (q_density, R_high, q_magnetic);
+    R_high re-derives the snapshot's scalars when it changes."""
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, params, grid):
+        q_density, r_high, q_magnetic = [float(v) for v in theta.detach().cpu().reshape(3)]
+        if grid.model.trat_large != r_high:
+            grid.model.trat_large = r_high
+            grid.derive()
+        out, grad, _ = image.run_grmhd_image_gradient(params, grid, (q_density, q_magnetic), with_stats=False)
+        stokes = torch.stack([out["image_i"], out["image_q"], out["image_u"], out["image_v"]])
+        ctx.save_for_backward(grad)
+        ctx.q_device = theta.device
+        ctx.q_dtype = theta.dtype
+        return stokes
+
+    @staticmethod
+    def backward(ctx, grad_stokes: torch.Tensor):
+        (jac,) = ctx.saved_tensors
+        g = (jac * grad_stokes.to(jac.dtype).unsqueeze(0)).sum(dim=(1, 2))
+        return g.to(device=ctx.q_device, dtype=ctx.q_dtype), None, None
+
+
+def grmhd_stokes(theta: torch.Tensor, params, grid) -> torch.Tensor:
+    """Differentiable static GRMHD Stokes image [4, npix] at
+    theta = (q_density, R_high, q_magnetic) (3,)."""
+    return GrmhdStokes.apply(theta, params, grid)

@@ -75,7 +75,18 @@ enum : int {
 // one frequency with the tangent (plasma-parameter derivatives) of its Stokes
 // vector carried alongside, a GRMHD grid, or a GRMHD snapshot sequence
 // sampled at retarded time (slow light).
-enum Mode : int { M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4, M_PROBE = 5, M_STAN = 6 };
+enum Mode : int {
+    M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4, M_PROBE = 5, M_STAN = 6,
+    // the kappa / power-law forms of M_MF, M_GRMHD and M_SLOW: kernels of their
+    // own, so the thermal kernels carry no call to the nonthermal routine
+    M_MF_NT = 7, M_GRMHD_NT = 8, M_SLOW_NT = 9
+};
+
+// The mode whose lane logic a mode runs (a nonthermal mode runs its thermal one).
+constexpr int base_mode(int m) {
+    return m == M_MF_NT ? M_MF : (m == M_GRMHD_NT ? M_GRMHD : (m == M_SLOW_NT ? M_SLOW : m));
+}
+constexpr bool nt_mode(int m) { return m == M_MF_NT || m == M_GRMHD_NT || m == M_SLOW_NT; }
 
 // Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
 // (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
@@ -89,9 +100,12 @@ constexpr int KP_NQUEUE = 32;  // queue and list counters in the workspace
 // guard radius (hysteresis: no ray alternates stages on every step).
 constexpr double KP_BACK_RADIUS_FACTOR = 1.25;
 template <int MODE> constexpr int geom_of() {
-    return (MODE == M_GRMHD || MODE == M_SLOW || MODE == M_PROBE || MODE == M_STAN) ? G_FMKS : G_BL;
+    constexpr int B = base_mode(MODE);
+    return (B == M_GRMHD || B == M_SLOW || B == M_PROBE || B == M_STAN) ? G_FMKS : G_BL;
 }
-template <int MODE> constexpr bool lds_stokes() { return MODE == M_MF || MODE == M_GRMHD || MODE == M_SLOW; }
+template <int MODE> constexpr bool lds_stokes() {
+    return base_mode(MODE) == M_MF || base_mode(MODE) == M_GRMHD || base_mode(MODE) == M_SLOW;
+}
 
 enum Counter : int {
     C_RHS_A = 0,
@@ -819,7 +833,7 @@ template <bool SCALED, int MODE> KP_DEV void lane_b_finish(const Params& P, cons
     if (O.pass_a_steps) O.pass_a_steps[p] = L.pass_a_steps;
     if (O.steps) O.steps[p] = valid ? L.steps : 0;
     if (O.reason) O.reason[p] = reason;
-    if constexpr (MODE == M_SLOW || MODE == M_STAN) {
+    if constexpr (base_mode(MODE) == M_SLOW || MODE == M_STAN) {
         P.req_t[p] = L.req;
         if (L.req == L.req) count_add(&P.counters[C_WAIT], 1ull);
     }
@@ -958,7 +972,7 @@ template <> struct Radiation<false, M_SINGLE> {
     Coeffs<double> c;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
-        riaf_coefficients(P.riaf, sample, c, &P.nt);
+        riaf_coefficients(P.riaf, sample, c);
         return 1;
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
@@ -994,12 +1008,14 @@ template <> struct Radiation<true, M_SINGLE> {
     }
 };
 
-template <> struct Radiation<false, M_MF> {
+// Thermal (NT false) and kappa / power-law (NT true) coefficient evaluation;
+// with NT false the nonthermal branch is compiled out.
+template <bool NT> struct RadiationMF {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
-        riaf_coeffs_at(P.riaf, loc, freq, c, &P.nt);
+        riaf_coeffs_at(P.riaf, loc, freq, c, NT ? &P.nt : nullptr);
     }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         riaf_local(P.riaf, sample, loc);
@@ -1010,7 +1026,7 @@ template <> struct Radiation<false, M_MF> {
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<double> c;
-            riaf_coeffs_at(P.riaf, loc, P.ctrl[f], c, &P.nt);
+            riaf_coeffs_at(P.riaf, loc, P.ctrl[f], c, NT ? &P.nt : nullptr);
             req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
         }
@@ -1019,12 +1035,14 @@ template <> struct Radiation<false, M_MF> {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<double> c;
-            riaf_coeffs_at(P.riaf, loc, P.freqs[f], c, &P.nt);
+            riaf_coeffs_at(P.riaf, loc, P.freqs[f], c, NT ? &P.nt : nullptr);
             mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
     }
 };
+template <> struct Radiation<false, M_MF> : RadiationMF<false> {};
+template <> struct Radiation<false, M_MF_NT> : RadiationMF<true> {};
 
 template <> struct Radiation<true, M_MF> {
     RiafLocal<float> loc;
@@ -1055,12 +1073,12 @@ template <> struct Radiation<true, M_MF> {
     }
 };
 
-template <> struct Radiation<false, M_GRMHD> {
+template <bool NT> struct RadiationGRMHD {
     RiafLocal<double> loc;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
-        grmhd_coeffs_at(P.grmhd, loc, freq, c, &P.nt);
+        grmhd_coeffs_at(P.grmhd, loc, freq, c, NT ? &P.nt : nullptr);
     }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         grmhd_local(P.grmhd, sample, loc);
@@ -1071,7 +1089,7 @@ template <> struct Radiation<false, M_GRMHD> {
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<double> c;
-            grmhd_coeffs_at(P.grmhd, loc, P.ctrl[f], c, &P.nt);
+            grmhd_coeffs_at(P.grmhd, loc, P.ctrl[f], c, NT ? &P.nt : nullptr);
             req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
         }
@@ -1080,18 +1098,20 @@ template <> struct Radiation<false, M_GRMHD> {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<double> c;
-            grmhd_coeffs_at(P.grmhd, loc, P.freqs[f], c, &P.nt);
+            grmhd_coeffs_at(P.grmhd, loc, P.freqs[f], c, NT ? &P.nt : nullptr);
             mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
     }
 };
+template <> struct Radiation<false, M_GRMHD> : RadiationGRMHD<false> {};
+template <> struct Radiation<false, M_GRMHD_NT> : RadiationGRMHD<true> {};
 
 // Slow light: the plasma state of snapshot pair (k, k+1) at time fraction f.
 // Fluid interpolation mixes the sampled fluid before the reconstruction (lo
 // holds the result); coefficient interpolation evaluates both snapshots (lo,
 // hi) and mixes each frequency's coefficients (slow_light_driver.hpp:166-193).
-template <> struct Radiation<false, M_SLOW> {
+template <bool NT> struct RadiationSlow {
     RiafLocal<double> lo, hi;
     double f;
     int k;
@@ -1112,10 +1132,10 @@ template <> struct Radiation<false, M_SLOW> {
     }
     KP_DEV bool valid() const { return lo.valid || hi.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
-        grmhd_coeffs_at(P.grmhd, lo, freq, c, &P.nt);
+        grmhd_coeffs_at(P.grmhd, lo, freq, c, NT ? &P.nt : nullptr);
         if (P.slow_interp != KP_SLOW_LIGHT_FLUID) {
             Coeffs<double> b;
-            grmhd_coeffs_at(P.grmhd, hi, freq, b, &P.nt);
+            grmhd_coeffs_at(P.grmhd, hi, freq, b, NT ? &P.nt : nullptr);
             const double w = 1.0 - f;
             c.jI = w * c.jI + f * b.jI;
             c.jQ = w * c.jQ + f * b.jQ;
@@ -1150,6 +1170,8 @@ template <> struct Radiation<false, M_SLOW> {
         return (lo.valid || hi.valid) ? (uint32_t)P.nfreq : 0u;
     }
 };
+template <> struct Radiation<false, M_SLOW> : RadiationSlow<false> {};
+template <> struct Radiation<false, M_SLOW_NT> : RadiationSlow<true> {};
 
 // Time probe: geometric stepping only (no coefficients, no substep control).
 template <> struct Radiation<false, M_PROBE> {
@@ -1354,7 +1376,7 @@ template <bool MIXED, int GP, int MODE>
 KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::GeoType& G, LaneB& L,
                         uint32_t& coeff_acc, uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     using GT = GeomT<geom_of<MODE>()>;
-    constexpr bool SLOW = MODE == M_SLOW || MODE == M_STAN;
+    constexpr bool SLOW = base_mode(MODE) == M_SLOW || MODE == M_STAN;
     const double eff_min = absv(P.min_step);
     if (L.phase == PH_IDLE) {
         if (L.reached || L.steps >= P.max_steps) return true;
@@ -1552,7 +1574,7 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
     const double lmrs = (P.max_rs > 0.0 && (old_r <= P.outer || sample_r <= P.outer || next_r <= P.outer))
                             ? P.max_rs : 0.0;
     int required, pred;
-    constexpr bool SHARED = !MIXED && (MODE == M_MF || MODE == M_GRMHD || MODE == M_SLOW);
+    constexpr bool SHARED = !MIXED && (base_mode(MODE) == M_MF || base_mode(MODE) == M_GRMHD || base_mode(MODE) == M_SLOW);
     if constexpr (SHARED) {
         if (P.single_pass) {
             // the control frequencies are the output frequencies: one
@@ -1895,6 +1917,9 @@ template <bool PERSISTENT> const void* pass_b_fn(bool mixed, int gp, int mode) {
     if (mode == M_STAN) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_STAN>;
     if (mode == M_GRMHD) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_GRMHD>;
     if (mode == M_SLOW) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_SLOW>;
+    if (mode == M_MF_NT) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_MF_NT>;
+    if (mode == M_GRMHD_NT) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_GRMHD_NT>;
+    if (mode == M_SLOW_NT) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_SLOW_NT>;
     if (mode == M_PROBE) return (const void*)pass_b_kernel<PERSISTENT, false, GP_64, M_PROBE>;
     const bool mf = mode == M_MF;
     if (!mixed)
@@ -1917,6 +1942,9 @@ const void* unsplit_fn(bool mixed, int mode) {
     if (mode == M_STAN) return (const void*)unsplit_kernel<false, M_STAN>;
     if (mode == M_GRMHD) return (const void*)unsplit_kernel<false, M_GRMHD>;
     if (mode == M_SLOW) return (const void*)unsplit_kernel<false, M_SLOW>;
+    if (mode == M_MF_NT) return (const void*)unsplit_kernel<false, M_MF_NT>;
+    if (mode == M_GRMHD_NT) return (const void*)unsplit_kernel<false, M_GRMHD_NT>;
+    if (mode == M_SLOW_NT) return (const void*)unsplit_kernel<false, M_SLOW_NT>;
     if (mode == M_PROBE) return (const void*)unsplit_kernel<false, M_PROBE>;
     if (mode == M_MF)
         return mixed ? (const void*)unsplit_kernel<true, M_MF> : (const void*)unsplit_kernel<false, M_MF>;
@@ -1924,7 +1952,8 @@ const void* unsplit_fn(bool mixed, int mode) {
 }
 
 int geom_of_mode(int mode) {
-    return (mode == M_GRMHD || mode == M_SLOW || mode == M_PROBE || mode == M_STAN) ? G_FMKS : G_BL;
+    const int b = base_mode(mode);
+    return (b == M_GRMHD || b == M_SLOW || b == M_PROBE || b == M_STAN) ? G_FMKS : G_BL;
 }
 
 // LDS bytes per block: 32 doubles per lane when the lane state lives in LDS,
@@ -2274,12 +2303,19 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         // every geodesic attempt of guard_mode 0 is FP64 and counts as a guard step
         if (opt.guard_mode == 0) P.stage = 0;
     }
+    const bool nonthermal = emission_nonthermal(P.nt.dist);
+    if (nonthermal && !grmhd && fl.nfreq == 0) {
+        // kappa / power law run in the frequency-list kernels
+        fl.nfreq = fl.nctrl = 1;
+        fl.freqs = fl.ctrl = &model->freq_cgs;
+    }
     const bool mf = fl.nfreq > 0;
     if (mf) {
         st = prepare_frequencies(grmhd ? grmhd->mbh_solar : model->mbh_solar, fl, P);
         if (st != KP_OK) return st;
     }
     int mode = grmhd ? (slow ? M_SLOW : (probe ? M_PROBE : M_GRMHD)) : (mf ? M_MF : M_SINGLE);
+    if (nonthermal) mode = mode == M_SLOW ? M_SLOW_NT : (mode == M_GRMHD ? M_GRMHD_NT : (mode == M_MF ? M_MF_NT : mode));
     if (tangent && slow) {
         if (mixed || fl.nfreq != 1)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light gradients are built for single-frequency FP64");
@@ -2383,7 +2419,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const size_t smem_a = lds_bytes(block, 0);
     // the single-pass frequency loop serves the FP64 multi-frequency, GRMHD
     // and slow-light lanes
-    if (mixed || !(mode == M_MF || mode == M_GRMHD || mode == M_SLOW)) P.single_pass = 0;
+    if (mixed || !(base_mode(mode) == M_MF || base_mode(mode) == M_GRMHD || base_mode(mode) == M_SLOW))
+        P.single_pass = 0;
     const size_t smem_b = mode == M_STAN ? lds_bytes(block, 0) : lds_bytes(block, mf ? P.nfreq : 0, P.single_pass ? 2 : 1);
     const int nblocks_flat = (int)((np + block - 1) / block);
     // grid of one launch: one block per pixel block, or (persistent) the
@@ -2620,8 +2657,9 @@ extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, co
             return kp_internal::fail(KP_INVALID_ARGUMENT, "batch radiation_substeps must be positive");
         kp_status st = prepare(&jobs[j].params, &jobs[j].model, table[j]);
         if (st != KP_OK) return st;
-        if (mixed && emission_nonthermal(table[j].nt.dist))
-            return kp_internal::fail(KP_UNSUPPORTED, "the kappa and power-law distributions are evaluated in FP64");
+        if (emission_nonthermal(table[j].nt.dist))
+            return kp_internal::fail(KP_UNSUPPORTED, "batches carry the thermal fits; image a kappa or power-law "
+                                                     "model with kp_run_riaf_(multifrequency_)image");
         if (mixed) {
             prepare_mixed(table[j]);
             table[j].fp64_radius = opt.fp64_radius;
@@ -2894,6 +2932,24 @@ extern "C" kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_para
     fl.nfreq = fl.nctrl = 1;
     fl.freqs = fl.ctrl = &model->freq_cgs;
     return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, &t, &m, sequence);
+}
+
+extern "C" kp_status kp_run_grmhd_image_gradient(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                 const double* log_scales, const kp_run_options* options,
+                                                 kp_image_data* out, double* grad, kp_image_stats* stats) {
+    kp_internal::clear_error();
+    if (!model || !model->cells) return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD model with cells required");
+    // the snapshot twice, at times no sample reaches: every sample's time
+    // fraction lies strictly inside the pair and interpolates equal values
+    const void* cells[2] = {model->cells, model->cells};
+    const double times[2] = {-1e15, 1e15};
+    kp_slow_light seq;
+    kp_slow_light_default(&seq);
+    seq.nsnap = 2;
+    seq.observation_time = 0.0;
+    seq.times = times;
+    seq.cells = cells;
+    return kp_run_grmhd_slow_light_image_gradient(params, model, &seq, log_scales, options, out, grad, stats);
 }
 
 extern "C" kp_status kp_slow_light_time_probe(const kp_pass_a_params* params, const kp_grmhd_model* model,

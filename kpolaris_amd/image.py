@@ -105,6 +105,10 @@ def library():
                                                          P(C.c_double), P(abi.RunOptions), P(abi.ImageData),
                                                          P(C.c_double), P(abi.ImageStats)]
     L.kp_run_grmhd_slow_light_image_gradient.restype = C.c_int32
+    L.kp_run_grmhd_image_gradient.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(C.c_double),
+                                              P(abi.RunOptions), P(abi.ImageData), P(C.c_double),
+                                              P(abi.ImageStats)]
+    L.kp_run_grmhd_image_gradient.restype = C.c_int32
     L.kp_slow_light_time_probe.restype = C.c_int32
     if L.kp_abi_version() != abi.ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
@@ -438,6 +442,28 @@ def run_grmhd_slow_light_image_gradient(params: abi.PassAParams, grids, times, o
     st = abi.ImageStats()
     _check(library().kp_run_grmhd_slow_light_image_gradient(
         C.byref(params), C.byref(model), C.byref(seq), q, C.byref(opts), C.byref(img),
+        C.cast(C.c_void_p(grad.data_ptr()), C.POINTER(C.c_double)), C.byref(st) if with_stats else None))
+    return out, grad, (st.as_dict() if with_stats else None)
+
+
+def run_grmhd_image_gradient(params: abi.PassAParams, grid: "GRMHDGrid", log_scales=(0.0, 0.0), *,
+                             persistent: bool = True, split_transport: int = 1, out: dict | None = None,
+                             fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+    """Static GRMHD image at one frequency (the model's) and its derivatives
+    along the fixed geodesics with respect to (q_density, R_high,
+    q_magnetic); returns (outputs, grad [3, 4, npix] device tensor, stats)."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    if out is None:
+        out = alloc_device_image(npix, fields=fields)
+    img = image_struct_from_tensors(out)
+    grad = torch.empty((3, 4, npix), dtype=torch.float64, device="cuda")
+    q = (C.c_double * 2)(*log_scales)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, persistent, 0, 0, C.c_void_p(stream), None, split_transport)
+    st = abi.ImageStats()
+    _check(library().kp_run_grmhd_image_gradient(
+        C.byref(params), C.byref(grid.model), q, C.byref(opts), C.byref(img),
         C.cast(C.c_void_p(grad.data_ptr()), C.POINTER(C.c_double)), C.byref(st) if with_stats else None))
     return out, grad, (st.as_dict() if with_stats else None)
 

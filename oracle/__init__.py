@@ -271,8 +271,11 @@ def grmhd_derive_scalars(model: abi.GRMHDModel, cells: np.ndarray) -> None:
 
 
 def run_grmhd_image(params: abi.PassAParams, model: abi.GRMHDModel, cells: np.ndarray, frequencies=None,
-                    step_control_frequencies=None, pixel_begin: int = 0, pixel_end: int = -1, nthreads: int = 0):
-    """Oracle GRMHD image on FMKS geodesics; Stokes arrays [nfreq * npix]."""
+                    step_control_frequencies=None, pixel_begin: int = 0, pixel_end: int = -1, nthreads: int = 0,
+                    perturbation=None):
+    """Oracle GRMHD image on FMKS geodesics; Stokes arrays [nfreq * npix].
+    perturbation = (parameter, log_scale): PlasmaPerturbation (1 density,
+    2 temperature, 3 magnetic)."""
     L = lib()
     m = abi.GRMHDModel.from_buffer_copy(model)
     m.cells = C.c_void_p(cells.ctypes.data)
@@ -287,7 +290,9 @@ def run_grmhd_image(params: abi.PassAParams, model: abi.GRMHDModel, cells: np.nd
     if nthreads <= 0:
         nthreads = int(os.environ.get("KP_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
     dp = C.POINTER(C.c_double)
-    rc = L.kpo_run_grmhd_image(C.byref(params), C.byref(m), None, f.ctypes.data_as(dp), len(f),
+    pert = Perturbation(perturbation[0], 0, perturbation[1]) if perturbation is not None else None
+    rc = L.kpo_run_grmhd_image(C.byref(params), C.byref(m), C.byref(pert) if pert is not None else None,
+                               f.ctypes.data_as(dp), len(f),
                                c.ctypes.data_as(dp), len(c), pixel_begin, pixel_end, nthreads, C.byref(img),
                                C.byref(st))
     if rc != 0:

@@ -14,7 +14,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
                           const kp_grmhd_model* gm = nullptr, const kp_slow_light* slow = nullptr,
                           double* req_t = nullptr, unsigned long long* waiting = nullptr) {
     constexpr int GP = MIXED ? GP_GUARD : GP_64;
-    constexpr bool FM = MODE == M_GRMHD || MODE == M_SLOW || MODE == M_STAN;
+    constexpr bool FM = base_mode(MODE) == M_GRMHD || base_mode(MODE) == M_SLOW || MODE == M_STAN;
     Params P;
     kp_status st = FM ? prepare_grmhd(pa, gm, P) : prepare(pa, m, P);
     if (st != KP_OK) return st;
@@ -28,7 +28,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
     }
     unsigned long long counters[C_COUNT] = {};
     P.counters = counters;
-    if (MODE == M_SLOW || MODE == M_STAN) {
+    if (base_mode(MODE) == M_SLOW || MODE == M_STAN) {
         P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(slow->cells);
         P.snap_t = slow->times;
         P.nsnap = slow->nsnap;
@@ -84,8 +84,17 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
     return KP_OK;
 }
 
+static bool nonthermal(int emission_type) { return emission_nonthermal(emission_type); }
+
 extern "C" kp_status kp_emul_run_riaf_image(const kp_pass_a_params* pa, const kp_riaf_model* m,
                                             kp_image_data* out, int64_t* rhs_evals) {
+    if (m && nonthermal(m->emission_type)) {
+        // kappa / power law run in the frequency-list lanes (as the library does)
+        FreqLists fl;
+        fl.nfreq = fl.nctrl = 1;
+        fl.freqs = fl.ctrl = &m->freq_cgs;
+        return emul_run<false, M_MF_NT>(pa, m, 0.0, fl, out, rhs_evals, nullptr);
+    }
     return emul_run<false, false>(pa, m, 0.0, FreqLists{}, out, rhs_evals, nullptr);
 }
 
@@ -109,6 +118,7 @@ extern "C" kp_status kp_emul_run_riaf_multifrequency_image(const kp_pass_a_param
     fl.nctrl = nctrl;
     fl.ctrl = ctrl;
     if (mixed) return emul_run<true, true>(pa, m, fp64_radius, fl, out, rhs_evals, nullptr);
+    if (m && nonthermal(m->emission_type)) return emul_run<false, M_MF_NT>(pa, m, 0.0, fl, out, rhs_evals, nullptr);
     return emul_run<false, true>(pa, m, 0.0, fl, out, rhs_evals, nullptr);
 }
 
@@ -130,6 +140,8 @@ extern "C" kp_status kp_emul_run_grmhd_image(const kp_pass_a_params* pa, const k
     fl.freqs = freqs;
     fl.nctrl = nctrl;
     fl.ctrl = ctrl;
+    if (nonthermal(gm->emission_type))
+        return emul_run<false, M_GRMHD_NT>(pa, nullptr, 0.0, fl, out, rhs_evals, nullptr, nullptr, nullptr, gm);
     return emul_run<false, M_GRMHD>(pa, nullptr, 0.0, fl, out, rhs_evals, nullptr, nullptr, nullptr, gm);
 }
 
@@ -170,6 +182,9 @@ extern "C" kp_status kp_emul_run_grmhd_slow_light_image(const kp_pass_a_params* 
     fl.freqs = fl.ctrl = freqs;
     kp_grmhd_model g = *gm;
     g.cells = seq->cells[0];
+    if (nonthermal(gm->emission_type))
+        return emul_run<false, M_SLOW_NT>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, nullptr, nullptr, &g, seq,
+                                          req_t, waiting);
     return emul_run<false, M_SLOW>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, nullptr, nullptr, &g, seq, req_t,
                                    waiting);
 }

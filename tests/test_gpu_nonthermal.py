@@ -102,8 +102,8 @@ def test_gpu_grmhd_nonthermal_parity(dist, variable):
 
 
 def test_gpu_nonthermal_rejections():
-    """Mixed precision and the tangent (gradient) entry points carry only the
-    thermal fits: they return KP_UNSUPPORTED for kappa or power law."""
+    """Mixed precision, the tangent (gradient) entry points and batches carry
+    only the thermal fits: they return KP_UNSUPPORTED for kappa or power law."""
     from kpolaris_amd import abi, image
     p, m = _riaf(2, nx=4, ny=4)
     with pytest.raises(image.KPError) as e:
@@ -111,6 +111,9 @@ def test_gpu_nonthermal_rejections():
     assert e.value.status == abi.KP_UNSUPPORTED
     with pytest.raises(image.KPError) as e:
         image.run_riaf_image_gradient(p, m, (0.0, 0.0, 0.0))
+    assert e.value.status == abi.KP_UNSUPPORTED
+    with pytest.raises(image.KPError) as e:
+        image.Batch([(p, m)])
     assert e.value.status == abi.KP_UNSUPPORTED
     m.emission_type = 9
     with pytest.raises(image.KPError) as e:
