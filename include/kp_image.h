@@ -509,6 +509,27 @@ kp_status kp_run_grmhd_image_gradient(const kp_pass_a_params* params, const kp_g
                                       const double* log_scales, const kp_run_options* options,
                                       kp_image_data* out, double* grad, kp_image_stats* stats);
 
+/* Launch-configuration tuning (no reference counterpart: Kokkos chooses the
+ * reference's launch shapes).  Times the image kernels of a reduced copy of
+ * the RIAF image (camera nx / sample_divisor by ny / sample_divisor over the
+ * same field of view, outputs discarded) for block_size in {64, 128, 256}
+ * and waves_per_cu in {0, 2, 8}, each configuration twice, and writes the
+ * fastest block_size and waves_per_cu into *options (its other fields,
+ * precision included, select the kernels that are timed).  frequencies may
+ * be NULL (nfreq 0): the single-frequency image.  The results of an image do
+ * not depend on these two fields. */
+typedef struct kp_launch_tuning {
+    int32_t ncandidates;     /* configurations timed */
+    int32_t block_size;      /* the chosen configuration */
+    int32_t waves_per_cu;
+    int32_t sample_pixels;   /* pixels of the reduced image */
+    double best_ms;          /* kernel time of the reduced image, chosen configuration */
+    double default_ms;       /* the same with block_size = waves_per_cu = 0 */
+} kp_launch_tuning;
+kp_status kp_tune_riaf_launch(const kp_pass_a_params* params, const kp_riaf_model* model, const double* frequencies,
+                              int32_t nfreq, int32_t sample_divisor, kp_run_options* options,
+                              kp_launch_tuning* report);
+
 /* Time support of a slow-light image: pass A endpoints, then geometric
  * stepping to the camera without radiation, recording the x^0 range of the
  * samples inside [inner_radius, outer_radius); the fluid times needed are

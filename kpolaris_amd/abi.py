@@ -180,6 +180,11 @@ class SlowLight(C.Structure):
     ]
 
 
+class LaunchTuning(C.Structure):
+    _fields_ = [("ncandidates", C.c_int32), ("block_size", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("sample_pixels", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
 class SlowLightProbe(C.Structure):
     _fields_ = [("min_x0", C.c_double), ("max_x0", C.c_double),
                 ("radiating_pixels", C.c_int64), ("returned_pixels", C.c_int64)]

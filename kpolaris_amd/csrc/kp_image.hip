@@ -2952,6 +2952,57 @@ extern "C" kp_status kp_run_grmhd_image_gradient(const kp_pass_a_params* params,
     return kp_run_grmhd_slow_light_image_gradient(params, model, &seq, log_scales, options, out, grad, stats);
 }
 
+extern "C" kp_status kp_tune_riaf_launch(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                         const double* frequencies, int32_t nfreq, int32_t sample_divisor,
+                                         kp_run_options* options, kp_launch_tuning* report) {
+    kp_internal::clear_error();
+    if (!params || !model || !options) return kp_internal::fail(KP_INVALID_ARGUMENT, "null parameters");
+    if (sample_divisor < 1) return kp_internal::fail(KP_INVALID_ARGUMENT, "sample_divisor must be positive");
+    if (nfreq < 0 || (nfreq > 0 && !frequencies))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "frequencies required when nfreq > 0");
+    kp_pass_a_params p = *params;
+    p.camera.nx = std::max(1, p.camera.nx / sample_divisor);
+    p.camera.ny = std::max(1, p.camera.ny / sample_divisor);
+    kp_image_data out;
+    std::memset(&out, 0, sizeof(out));  // every output skipped: only the kernels run
+    FreqLists fl;
+    if (nfreq > 0) {
+        fl.nfreq = fl.nctrl = nfreq;
+        fl.freqs = fl.ctrl = frequencies;
+    }
+    static const int blocks[3] = {64, 128, 256};
+    static const int waves[3] = {0, 2, 8};
+    kp_launch_tuning r;
+    std::memset(&r, 0, sizeof(r));
+    r.sample_pixels = p.camera.nx * p.camera.ny;
+    r.best_ms = -1.0;
+    for (int b = 0; b < 3; ++b) {
+        for (int w = 0; w < 3; ++w) {
+            kp_run_options o = *options;
+            o.block_size = blocks[b];
+            o.waves_per_cu = waves[w];
+            kp_image_stats st;
+            double ms = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {  // the first run loads the kernels
+                const kp_status s2 = run_image(&p, model, fl, &o, &out, &st, 0, 0.0);
+                if (s2 != KP_OK) return s2;
+                ms = st.kernel_ms;
+            }
+            ++r.ncandidates;
+            if (r.best_ms < 0.0 || ms < r.best_ms) {
+                r.best_ms = ms;
+                r.block_size = blocks[b];
+                r.waves_per_cu = waves[w];
+            }
+            if (b == 1 && w == 0) r.default_ms = ms;  // block 128 and the occupancy grid are the defaults
+        }
+    }
+    options->block_size = r.block_size;
+    options->waves_per_cu = r.waves_per_cu;
+    if (report) *report = r;
+    return KP_OK;
+}
+
 extern "C" kp_status kp_slow_light_time_probe(const kp_pass_a_params* params, const kp_grmhd_model* model,
                                               const kp_run_options* options, kp_slow_light_probe* out) {
     kp_internal::clear_error();

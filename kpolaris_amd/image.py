@@ -110,6 +110,9 @@ def library():
                                               P(abi.ImageStats)]
     L.kp_run_grmhd_image_gradient.restype = C.c_int32
     L.kp_slow_light_time_probe.restype = C.c_int32
+    L.kp_tune_riaf_launch.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), P(C.c_double), C.c_int32, C.c_int32,
+                                      P(abi.RunOptions), P(abi.LaunchTuning)]
+    L.kp_tune_riaf_launch.restype = C.c_int32
     if L.kp_abi_version() != abi.ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.kp_abi_version()}, this binding expects "
                            f"{abi.ABI_VERSION}: rebuild the library")
@@ -444,6 +447,23 @@ def run_grmhd_slow_light_image_gradient(params: abi.PassAParams, grids, times, o
         C.byref(params), C.byref(model), C.byref(seq), q, C.byref(opts), C.byref(img),
         C.cast(C.c_void_p(grad.data_ptr()), C.POINTER(C.c_double)), C.byref(st) if with_stats else None))
     return out, grad, (st.as_dict() if with_stats else None)
+
+
+def tune_riaf_launch(params: abi.PassAParams, model: abi.RIAFModel, frequencies=None, *,
+                     precision: int = abi.PRECISION_FP64, persistent: bool = True, sample_divisor: int = 4,
+                     split_transport: int = 1, guard_mode: int = 0) -> dict:
+    """Time the image kernels over launch shapes on a reduced copy of the
+    image (kp_tune_riaf_launch); returns the chosen block_size and
+    waves_per_cu with the timings.  Pass the result to run_riaf_image."""
+    import torch
+    f = None if frequencies is None else np.ascontiguousarray(frequencies, dtype=np.float64)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, persistent, 0, 0, C.c_void_p(stream), None, split_transport, guard_mode)
+    rep = abi.LaunchTuning()
+    _check(library().kp_tune_riaf_launch(
+        C.byref(params), C.byref(model), None if f is None else f.ctypes.data_as(C.POINTER(C.c_double)),
+        0 if f is None else len(f), sample_divisor, C.byref(opts), C.byref(rep)))
+    return {n: getattr(rep, n) for n, _ in abi.LaunchTuning._fields_}
 
 
 def run_grmhd_image_gradient(params: abi.PassAParams, grid: "GRMHDGrid", log_scales=(0.0, 0.0), *,
