@@ -33,16 +33,18 @@ MIXED_POL = 1e-3          # mixed precision: relative error of the LP and CP fra
 
 
 PROFILE = ROOT / "profiles" / "riaf_c2_1024_pass_kernels.md"
+PROFILE_GRMHD = ROOT / "profiles" / "grmhd_512_8freq_kernels.md"
+PROFILE_SLOW = ROOT / "profiles" / "slow_light_256_32snap_kernels.md"
 
 
-def profiled_traffic(kernel: str) -> float | None:
+def profiled_traffic(kernel: str, profile: Path = PROFILE) -> float | None:
     """HBM bytes per launch of a kernel of this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines ``traffic_bytes <kernel>
     <bytes>``, counters scaled by a copy-kernel calibration).  Counters cannot
     be read inside a timed run, so the figure is the profiled one."""
-    if not PROFILE.exists():
+    if not profile.exists():
         return None
-    for line in PROFILE.read_text().splitlines():
+    for line in profile.read_text().splitlines():
         f = line.split()
         if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
             return float(f[2])
@@ -215,7 +217,9 @@ def bench_grmhd(args) -> int:
                    # counts the 8 frequencies once per step
                    "grid_sample_bytes_per_s": st["coefficient_evals"] / 8 * 576 / (st["pass_b_ms"] * 1e-3)},
         "roofline": {"bound": "hbm", "kernel": "derive_scalars", "achieved": derive_gbs, "peak": 8000.0,
-                     "unit": "GB/s", "frac": derive_gbs / 8000.0, "traffic": None,
+                     "unit": "GB/s", "frac": derive_gbs / 8000.0,
+                     "traffic": profiled_traffic("derive_scalars", PROFILE_GRMHD),
+                     "traffic_source": "profiles/grmhd_512_8freq_kernels.md",
                      "algorithmic_bytes_per_launch": ncell * 72, "launch_ms": derive_ms},
     }), flush=True)
     return 0
@@ -283,7 +287,9 @@ def bench_slow(args) -> int:
                    "coefficient_evals": st["coefficient_evals"],
                    "reasons": {str(i): int(c) for i, c in enumerate(reasons) if c}},
         "roofline": {"bound": "hbm", "kernel": "pass_b (slow light)", "achieved": gbs, "peak": 8000.0,
-                     "unit": "GB/s", "frac": gbs / 8000.0, "traffic": None,
+                     "unit": "GB/s", "frac": gbs / 8000.0,
+                     "traffic": profiled_traffic("pass_b", PROFILE_SLOW),
+                     "traffic_source": "profiles/slow_light_256_32snap_kernels.md",
                      "algorithmic_bytes_per_launch": sample_bytes, "launch_ms": b_ms},
         "cpu_baseline": cpu,
     }), flush=True)
