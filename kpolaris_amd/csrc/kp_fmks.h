@@ -14,10 +14,26 @@
 
 namespace kpd {
 
-// FMKS map parameters (KerrFMKSMetric fields, kerr_fmks.hpp:11-19).
+// FMKS map parameters (KerrFMKSMetric fields, kerr_fmks.hpp:11-19) and the
+// parameter-only factors of theta(x1, x2) and its derivatives, formed once on
+// the host so that the transport RHS multiplies instead of dividing by them.
 struct FmksMap {
     double startx1, hslope, mks_smooth, poly_alpha, poly_xt, poly_norm;
+    double inv_xt;      // 1 / poly_xt
+    double inv_alpha1;  // 1 / (poly_alpha + 1)
+    double dthj_c;      // 2 alpha norm / ((alpha + 1) xt)
+    double d2thj_c;     // 4 norm alpha / (alpha + 1)
 };
+
+inline FmksMap fmks_map(double startx1, double hslope, double mks_smooth, double poly_alpha, double poly_xt,
+                        double poly_norm) {
+    FmksMap f{startx1, hslope, mks_smooth, poly_alpha, poly_xt, poly_norm, 0.0, 0.0, 0.0, 0.0};
+    f.inv_xt = 1.0 / poly_xt;
+    f.inv_alpha1 = 1.0 / (poly_alpha + 1.0);
+    f.dthj_c = (2.0 * poly_alpha * poly_norm) / ((poly_alpha + 1.0) * poly_xt);
+    f.d2thj_c = 4.0 * poly_norm * poly_alpha / (poly_alpha + 1.0);
+    return f;
+}
 
 // fmks_power (kerr_fmks.hpp:37-47): integer powers by repeated products.
 template <class T> KP_DEV T fmks_power(T x, T p) {
@@ -43,14 +59,13 @@ template <class T> KP_DEV FmksTheta<T> fmks_theta(const FmksMap& f, T x1, T x2) 
     ksincos(T(2) * pi * x2, &s2, &c2);
     const T thG = pi * x2 + ((T(1) - T(f.hslope)) / T(2)) * s2;
     const T y = T(2) * x2 - T(1);
-    const T za = fmks_power(y / T(f.poly_xt), T(f.poly_alpha));
-    const T zam1 = fmks_power(y / T(f.poly_xt), T(f.poly_alpha) - T(1));
-    const T thJ = T(f.poly_norm) * y * (T(1) + za / (T(f.poly_alpha) + T(1))) + T(0.5) * pi;
+    const T z = y * T(f.inv_xt);
+    const T za = fmks_power(z, T(f.poly_alpha));
+    const T zam1 = fmks_power(z, T(f.poly_alpha) - T(1));
+    const T thJ = T(f.poly_norm) * y * (T(1) + za * T(f.inv_alpha1)) + T(0.5) * pi;
     const T expfac = exp(T(f.mks_smooth) * (T(f.startx1) - x1));
     const T dthG = pi + (T(1) - T(f.hslope)) * pi * c2;
-    const T dthJ = T(2) * T(f.poly_norm) * (T(1) + za / (T(f.poly_alpha) + T(1))) +
-                   (T(2) * T(f.poly_alpha) * T(f.poly_norm) * y * zam1) /
-                       ((T(f.poly_alpha) + T(1)) * T(f.poly_xt));
+    const T dthJ = T(2) * T(f.poly_norm) * (T(1) + za * T(f.inv_alpha1)) + T(f.dthj_c) * y * zam1;
     FmksTheta<T> o;
     o.th = thG + expfac * (thJ - thG);
     o.dth1 = -T(f.mks_smooth) * expfac * (thJ - thG);
@@ -73,20 +88,23 @@ template <class T> KP_DEV SksConn<T> sks_conn(T mass, T spin, T r_in, T s, T cc)
     const T s2 = s * s;
     const T safe_s2 = maxv(s2, tiny<T>());
     const T rho2 = r * r + a2 * cc * cc;
-    const T rho4 = rho2 * rho2;
-    const T f = T(2) * mass * r / rho2;
+    // the reference's seven divisions by rho^2, rho^4 and rho^2 sin^2 as two
+    // reciprocals (each quotient moves by a few ulps at most)
+    const T irho2 = T(1) / rho2;
+    const T irho4 = irho2 * irho2;
+    const T f = T(2) * mass * r * irho2;
     const T delta = r * r - T(2) * mass * r + a2;
     c.g00 = -(T(1) + f);
     c.g01 = f;
-    c.g11 = delta / rho2;
-    c.g13 = a / rho2;
-    c.g22 = T(1) / rho2;
-    c.g33 = T(1) / (rho2 * safe_s2);
+    c.g11 = delta * irho2;
+    c.g13 = a * irho2;
+    c.g22 = irho2;
+    c.g33 = irho2 / safe_s2;
     const T drho_dr = T(2) * r;
     const T drho_dth = -T(2) * a2 * s * cc;
     const T ds2_dth = T(2) * s * cc;
-    const T df_dr = T(2) * mass * (rho2 - r * drho_dr) / rho4;
-    const T df_dth = -T(2) * mass * r * drho_dth / rho4;
+    const T df_dr = T(2) * mass * (rho2 - r * drho_dr) * irho4;
+    const T df_dth = -T(2) * mass * r * drho_dth * irho4;
     c.r00 = df_dr;
     c.t00 = df_dth;
     c.r01 = df_dr;
@@ -145,7 +163,7 @@ template <class T> struct FmksConn {
 template <class T> KP_DEV FmksConn<T> fmks_conn(T mass, T spin, const FmksMap& f, T x1, T x2) {
     const T pi = T(PI_D);
     const T y = T(2) * x2 - T(1);
-    const T z = y / T(f.poly_xt);
+    const T z = y * T(f.inv_xt);
     const T expfac = exp(T(f.mks_smooth) * (T(f.startx1) - x1));
     T za, zam1, zam2;
     if (absv(T(f.poly_alpha) - T(14)) < T(1e-12)) {
@@ -161,15 +179,12 @@ template <class T> KP_DEV FmksConn<T> fmks_conn(T mass, T spin, const FmksMap& f
     T s2, c2;
     ksincos(T(2) * pi * x2, &s2, &c2);
     const T thG = pi * x2 + ((T(1) - T(f.hslope)) / T(2)) * s2;
-    const T thJ = T(f.poly_norm) * y * (T(1) + za / (T(f.poly_alpha) + T(1))) + T(0.5) * pi;
+    const T thJ = T(f.poly_norm) * y * (T(1) + za * T(f.inv_alpha1)) + T(0.5) * pi;
     const T dthG = pi + (T(1) - T(f.hslope)) * pi * c2;
-    const T dthJ = T(2) * T(f.poly_norm) * (T(1) + za / (T(f.poly_alpha) + T(1))) +
-                   (T(2) * T(f.poly_alpha) * T(f.poly_norm) * y * zam1) /
-                       ((T(f.poly_alpha) + T(1)) * T(f.poly_xt));
+    const T dthJ = T(2) * T(f.poly_norm) * (T(1) + za * T(f.inv_alpha1)) + T(f.dthj_c) * y * zam1;
     const T d2thG = -T(2) * (T(1) - T(f.hslope)) * pi * pi * s2;
-    const T d2thJ = T(4) * T(f.poly_norm) * T(f.poly_alpha) / (T(f.poly_alpha) + T(1)) *
-                    (T(2) * zam1 / T(f.poly_xt) +
-                     y * (T(f.poly_alpha) - T(1)) * zam2 / (T(f.poly_xt) * T(f.poly_xt)));
+    const T ixt = T(f.inv_xt);
+    const T d2thJ = T(f.d2thj_c) * (T(2) * zam1 * ixt + y * (T(f.poly_alpha) - T(1)) * zam2 * (ixt * ixt));
     FmksConn<T> c;
     c.r = exp(x1);
     c.th = thG + expfac * (thJ - thG);
@@ -197,6 +212,8 @@ template <class T> KP_DEV void fmks_from_sph(const FmksConn<T>& c, T* hs, const 
     hs[1] += c.r * p[1] * q[1];
     hs[2] += c.d2th11 * p[1] * q[1] + c.d2th12 * (p[1] * q[2] + p[2] * q[1]) + c.d2th22 * p[2] * q[2];
     o[0] = hs[0];
+    // divisions, not reciprocals kept in FmksConn: two more live values there
+    // cost the pass-B kernels more in spills than the divisions do
     o[1] = hs[1] / c.r;
     o[2] = (hs[2] - c.dth1 * hs[1] / c.r) / c.dth2;
     o[3] = hs[3];

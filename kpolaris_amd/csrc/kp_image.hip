@@ -2127,8 +2127,8 @@ static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* 
         return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid spacings must be positive");
     kp_status st = prepare_common(p, P);
     if (st != KP_OK) return st;
-    P.fmks = FmksMap{p->fmks_startx1, p->fmks_hslope, p->fmks_mks_smooth, p->fmks_poly_alpha, p->fmks_poly_xt,
-                     p->fmks_poly_norm};
+    P.fmks = fmks_map(p->fmks_startx1, p->fmks_hslope, p->fmks_mks_smooth, p->fmks_poly_alpha, p->fmks_poly_xt,
+                      p->fmks_poly_norm);
     double fm[6] = {p->fmks_startx1, p->fmks_hslope, p->fmks_mks_smooth, p->fmks_poly_alpha, p->fmks_poly_xt,
                     p->fmks_poly_norm};
     double g16[16];
@@ -2150,7 +2150,7 @@ static kp_status prepare_grmhd(const kp_pass_a_params* p, const kp_grmhd_model* 
     G.r_out = g->r_out;
     G.spin = g->spin;
     G.gam = g->gam;
-    G.map = FmksMap{g->startx1, g->hslope, g->mks_smooth, g->poly_alpha, g->poly_xt, g->poly_norm};
+    G.map = fmks_map(g->startx1, g->hslope, g->mks_smooth, g->poly_alpha, g->poly_xt, g->poly_norm);
     G.sigma_cut = g->sigma_cut;
     G.sigma_cut_high = g->sigma_cut_high;
     G.emission_scale = g->emission_scale;

@@ -165,7 +165,7 @@ extern "C" kp_status kp_emul_grmhd_derive_scalars(const kp_grmhd_model* gm, kp_g
 
 // FMKS transport RHS of the device code, for its parity check.
 extern "C" void kp_emul_fmks_rhs(double mass, double spin, const double fm[6], const double* s, double* d) {
-    const FmksMap f{fm[0], fm[1], fm[2], fm[3], fm[4], fm[5]};
+    const FmksMap f = fmks_map(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5]);
     St<double> y;
     for (int i = 0; i < 16; ++i) y.v[i] = s[i];
     fmks_rhs_apply(mass, spin, f, y, [&](int i, double v) { d[i] = v; });

@@ -179,14 +179,20 @@ def assert_parity_grid(got, ref, nfreq, label, max_step_diff=None, ill=None):
     whose step sequence moved (a chaotic near-photon-ring ray, or a slow-light
     ray approaching a snapshot time by ever smaller cut steps), and the
     ill-conditioned rays `ill` of the oracle's own sensitivity, within 1e-3
-    (+1e-6 of peak).  max_step_diff bounds |steps - steps_ref| when given."""
+    (+1e-6 of peak).  max_step_diff bounds |steps - steps_ref| on the rays
+    that are not ill-conditioned when given."""
     assert np.array_equal(got["reason"], ref["reason"]), (
         f"{label}: reasons differ at {np.flatnonzero(got['reason'] != ref['reason'])[:10]}")
     moved = (got["steps"] != ref["steps"]) | (got["pass_a_steps"] != ref["pass_a_steps"])
     if ill is not None:
         moved = moved | ill
     if max_step_diff is not None:
-        assert np.max(np.abs(got["steps"] - ref["steps"]), initial=0) <= max_step_diff, label
+        # ill-conditioned rays change their step counts under rounding-level
+        # perturbations of the oracle itself, so the bound holds on the others
+        d = np.abs(got["steps"] - ref["steps"])
+        if ill is not None:
+            d = np.where(ill, 0, d)
+        assert np.max(d, initial=0) <= max_step_diff, (label, np.flatnonzero(d > max_step_diff)[:8])
     n = ref["reason"].size
     worst = 0.0
     for f in range(nfreq):

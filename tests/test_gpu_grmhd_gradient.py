@@ -79,7 +79,13 @@ def test_gpu_grmhd_gradient_value_and_oracle_differences(snap):
                   flush=True)
             # plain central differences carry an O(h^2) truncation (up to ~1e-4 of the
             # largest derivative); the Richardson value removes it
-            assert err < 1e-3 and err_r < 1e-7, (kind, name, err, err_r)
+            assert err < 1e-3 and err_r < 1e-6, (kind, name, err, err_r)
+            # most pixels agree to rounding; a rounding-level difference can flip one
+            # accept/reject decision along a ray without changing its step count, and
+            # the derivative of the discretised transfer then moves by O(rtol)
+            per_pixel = np.abs(g[kind, s][ok] - rich[ok]) / scale
+            assert np.percentile(per_pixel, 95) < 1e-8, (kind, name, np.percentile(per_pixel, 95))
+
 
 
 def test_gpu_grmhd_autograd_contraction(snap):

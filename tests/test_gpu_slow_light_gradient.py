@@ -76,7 +76,13 @@ def test_gpu_slow_light_gradient_against_oracle(seq):
             err_r = np.abs(g[kind, s][ok] - rich[ok]).max() / scale
             print(f"d/d{('q_density', 'R_high', 'q_magnetic')[kind]} {name}: {err:.2e} (Richardson {err_r:.2e})",
                   flush=True)
-            assert err < 1e-4 and err_r < 1e-7, (kind, name, err, err_r)
+            assert err < 1e-4 and err_r < 1e-6, (kind, name, err, err_r)
+            # most pixels agree to rounding; a rounding-level difference can flip one
+            # accept/reject decision along a ray without changing its step count, and
+            # the derivative of the discretised transfer then moves by O(rtol)
+            per_pixel = np.abs(g[kind, s][ok] - rich[ok]) / scale
+            assert np.percentile(per_pixel, 95) < 1e-8, (kind, name, np.percentile(per_pixel, 95))
+
 
 
 def test_gpu_slow_light_autograd(seq):

@@ -340,4 +340,10 @@ def test_emulated_slow_light_gradient_against_central_differences(seq, emul):
             print(f"slow-light d/d{('q_density', 'R_high', 'q_magnetic')[kind]} {name}: max rel err {err:.2e}, "
                   f"against Richardson {err_r:.2e}", flush=True)
             assert err < 1e-4, (kind, name, err)
-            assert err_r < 1e-7, (kind, name, err_r)
+            assert err_r < 1e-6, (kind, name, err_r)
+            # most pixels agree to rounding; a rounding-level difference can flip one
+            # accept/reject decision along a ray without changing its step count, and
+            # the derivative of the discretised transfer then moves by O(rtol)
+            per_pixel = np.abs(grad[kind, s][ok] - rich[ok]) / scale
+            assert np.percentile(per_pixel, 95) < 1e-8, (kind, name, np.percentile(per_pixel, 95))
+
