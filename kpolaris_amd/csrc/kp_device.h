@@ -733,37 +733,34 @@ template <class T> KP_DEV T bnu_inv(T nu, T thetae) {
 template <class T>
 KP_DEV void rho_qv(T nu, T ne, T thetae, T b, T sin_theta, T cos_theta, T& rhoQ, T& rhoV) {
     const T pi = T(PI_D);
-    // constant quotients folded into single constants and the repeated
-    // division by 2 pi nu taken as one reciprocal (each factor moves by an ulp)
-    const T omega0 = b * T(KP_EE / (KP_ME * KP_CL));
-    const T wp2 = ne * T(4.0 * PI_D * KP_EE * KP_EE / KP_ME);
+    const T omega0 = T(KP_EE) * b / (T(KP_ME) * T(KP_CL));
+    const T wp2 = T(4) * pi * ne * T(KP_EE) * T(KP_EE) / T(KP_ME);
     const T two_pi_nu = T(2) * pi * nu;
     const T safe = maxv(two_pi_nu, tiny<T>());
-    const T isafe = T(1) / safe;
-    const T x = maxv(thetae * sqrt(maxv(T(0), sqrt(T(2)) * sin_theta * (T(1e3) * omega0 * isafe))), T(1e-12));
+    const T x = maxv(thetae * sqrt(maxv(T(0), sqrt(T(2)) * sin_theta * (T(1e3) * omega0 / safe))), T(1e-12));
     T k0, k1, k2;
     bessel_k012(T(1) / thetae, k0, k1, k2);
-    const T exp_x_47 = exp(-x * T(1.0 / 47.2));
+    const T exp_x_47 = exp(-x / T(47.2));
     const T extraterm = (T(0.011) * exp_x_47 - T(369.69342869090815) * pow(x + T(1e-16), T(-8) / T(3))) *
-                        (T(0.5) + T(0.5) * tanh((log(x) - T(4.787491742782046)) * T(1.0 / 0.1)));
-    const T jffunc = T(2.011) * exp(-pow(x, T(1.035)) * T(1.0 / 4.7)) -
-                     cos(x * T(0.5)) * exp(-pow(x, T(1.2)) * T(1.0 / 2.73)) - T(0.011) * exp_x_47 + extraterm;
+                        (T(0.5) + T(0.5) * tanh((log(x) - T(4.787491742782046)) / T(0.1)));
+    const T jffunc = T(2.011) * exp(-pow(x, T(1.035)) / T(4.7)) -
+                     cos(x / T(2)) * exp(-pow(x, T(1.2)) / T(2.73)) - T(0.011) * exp_x_47 + extraterm;
     const T k1_ratio = k2 > T(0) ? k1 / k2 : T(1);
     T e11 = jffunc * wp2 * omega0 * omega0;
-    e11 *= isafe;
-    e11 *= isafe;
-    e11 *= isafe;
-    e11 *= isafe;
+    e11 /= safe;
+    e11 /= safe;
+    e11 /= safe;
+    e11 /= safe;
     e11 *= (k1_ratio + T(6) * thetae) * sin_theta * sin_theta;
-    rhoQ = nu * T(PI_D / KP_CL) * e11;
+    rhoQ = pi * nu / T(KP_CL) * e11;
     const T shgm = T(1) - T(0.11) * log(T(1) + T(0.035) * x);
     const T k0_ratio = k2 > T(0) ? k0 / k2 : T(1);
     T e12 = wp2 * omega0;
-    e12 *= isafe;
-    e12 *= isafe;
-    e12 *= isafe;
+    e12 /= safe;
+    e12 /= safe;
+    e12 /= safe;
     e12 *= k0_ratio * shgm * cos_theta;
-    rhoV = nu * T(2.0 * PI_D / KP_CL) * e12;
+    rhoV = T(2) * pi * nu / T(KP_CL) * e12;
 }
 
 // thermal_synchrotron_magnetic_basis_coefficients_fit (thermal_synchrotron.hpp:512-616);
@@ -779,12 +776,11 @@ KP_DEV void thermal_coeffs(int fit, T nu_in, T ne, T thetae, T b, T sin_theta_in
     const T sin_theta = maxv(sin_theta_in, min_sin_theta);
     T jI, jQ, jV;
     if (fit == 4) {
-        const T nus = b * sin_theta * T(3.0 * KP_EE / (4.0 * PI_D * KP_ME * KP_CL)) * thetae * thetae + T(1);
+        const T nus = T(3) * T(KP_EE) * b * sin_theta / (T(4) * pi * T(KP_ME) * T(KP_CL)) * thetae * thetae + T(1);
         const T x = maxv(nu / maxv(nus, tiny<T>()), tiny<T>());
-        const T x13 = cbrt(x);
-        const T xm13 = T(1) / x13;
-        const T exp_shape = exp(-T(1.8899) * x13);
-        const T pref = ne * nu * T(KP_EE * KP_EE / (2.0 * 1.7320508075688772 * KP_CL)) / (thetae * thetae);
+        const T xm13 = T(1) / cbrt(x);
+        const T exp_shape = exp(-T(1.8899) / xm13);
+        const T pref = ne * T(KP_EE) * T(KP_EE) * nu / (T(2) * sqrt(T(3)) * T(KP_CL) * thetae * thetae);
         jI = pref * (T(2.5651) * (T(1) + T(1.92) * xm13 + T(0.9977) * xm13 * xm13) * exp_shape);
         jQ = pref * (T(2.5651) * (T(1) + T(0.93193) * xm13 + T(0.499873) * xm13 * xm13) * exp_shape);
         const T iv = (T(1.81384) / x + T(3.42319) * xm13 * xm13 + T(0.0292545) / sqrt(x) +
@@ -792,13 +788,13 @@ KP_DEV void thermal_coeffs(int fit, T nu_in, T ne, T thetae, T b, T sin_theta_in
         T cft = cos_theta;
         if (absv(cft) < T(1e-10)) cft = cft >= T(0) ? T(1e-10) : T(-1e-10);
         const T tan_theta = sin_theta / cft;
-        jV = ne * nu * T(2.0 * KP_EE * KP_EE / (3.0 * 1.7320508075688772 * KP_CL)) /
-             (tan_theta * thetae * thetae * thetae) * iv;
+        jV = T(2) * ne * T(KP_EE) * T(KP_EE) * nu /
+             (tan_theta * T(3) * sqrt(T(3)) * T(KP_CL) * thetae * thetae * thetae) * iv;
     } else {
-        const T nu_c = b * T(KP_EE / (2.0 * PI_D * KP_ME * KP_CL));
+        const T nu_c = T(KP_EE) * b / (T(2) * pi * T(KP_ME) * T(KP_CL));
         const T nu_s = T(2) / T(9) * nu_c * sin_theta * thetae * thetae;
         const T x = maxv(nu / maxv(nu_s, tiny<T>()), tiny<T>());
-        const T pref = ne * nu_c * T(KP_EE * KP_EE / KP_CL);
+        const T pref = ne * T(KP_EE) * T(KP_EE) * nu_c / T(KP_CL);
         const T term1 = sqrt(T(2)) * pi / T(27) * sin_theta;
         const T x13 = cbrt(x);
         const T sx = sqrt(x), sx13 = sqrt(x13);
