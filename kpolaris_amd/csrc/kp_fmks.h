@@ -213,9 +213,11 @@ template <class T> KP_DEV void fmks_from_sph(const FmksConn<T>& c, T* hs, const 
     hs[2] += c.d2th11 * p[1] * q[1] + c.d2th12 * (p[1] * q[2] + p[2] * q[1]) + c.d2th22 * p[2] * q[2];
     o[0] = hs[0];
     // divisions, not reciprocals kept in FmksConn: two more live values there
-    // cost the pass-B kernels more in spills than the divisions do
+    // cost the pass-B kernels more in spills than the divisions do.  The theta
+    // row reuses the quotient hs[1] / r (the reference forms (dth1 hs[1]) / r:
+    // the two differ by an ulp), one FP64 division less per contraction.
     o[1] = hs[1] / c.r;
-    o[2] = (hs[2] - c.dth1 * hs[1] / c.r) / c.dth2;
+    o[2] = (hs[2] - c.dth1 * o[1]) / c.dth2;
     o[3] = hs[3];
 }
 
