@@ -589,23 +589,36 @@ template <class T> KP_DEV void strang_step(Stokes<T>& s, const Coeffs<T>& c, T d
     faraday_rotate(s, c, T(0.5) * dl);
 }
 
-// radiation_substep_count (src/geodesic/pass_b.hpp:126-179).
+// radiation_substep_count (src/geodesic/pass_b.hpp:126-179).  The count n
+// is a power of two, so x / n is formed as x * (1/n) with 1/n = 2^-k held
+// exactly: both are the correctly rounded value of the same real number (the
+// same bits), and the doubling loops run without a division.
 template <class T>
 KP_DEV int substep_count(const Coeffs<T>& c, T h, T dls, T max_rs, T max_rd, T max_ad, T max_fd) {
     int n = 1;
+    T inv = T(1);
     const T ah = absv(h);
     if (max_rs > T(0))
-        while (ah / T(n) > max_rs && n < 4096) n *= 2;
+        while (ah * inv > max_rs && n < 4096) {
+            n *= 2;
+            inv *= T(0.5);
+        }
     const T dl = ah * absv(dls);
     const T alim = max_ad >= T(0) ? max_ad : max_rd;
     if (alim > T(0)) {
         const T depth = (absv(c.aI) + sqrt(sq(c.aQ) + sq(c.aU) + sq(c.aV))) * dl;
-        while (depth / T(n) > alim && n < 4096) n *= 2;
+        while (depth * inv > alim && n < 4096) {
+            n *= 2;
+            inv *= T(0.5);
+        }
     }
     const T flim = max_fd >= T(0) ? max_fd : max_rd;
     if (flim > T(0)) {
         const T depth = sqrt(sq(c.rQ) + sq(c.rU) + sq(c.rV)) * dl;
-        while (depth / T(n) > flim && n < 4096) n *= 2;
+        while (depth * inv > flim && n < 4096) {
+            n *= 2;
+            inv *= T(0.5);
+        }
     }
     return n;
 }
