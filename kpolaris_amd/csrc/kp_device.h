@@ -46,6 +46,8 @@ KP_DEV float krcp(float x) {
 #endif
 }
 KP_DEV float kdiv(float a, float b) { return a * krcp(b); }
+// other number types (the forward-mode duals): the exact quotient
+template <class T> KP_DEV T krcp(const T& x) { return T(1) / x; }
 
 // Counter increment shared by the kernels and their host build (the CPU
 // emulation of the lane logic runs one lane at a time).
@@ -545,7 +547,7 @@ template <class T> KP_DEV void faraday_rotate(Stokes<T>& s, const Coeffs<T>& c, 
         s.V = pz + (c.rQ * py - c.rU * px) * dl;
         return;
     }
-    const T inv = T(1) / rn;
+    const T inv = krcp(rn);
     const T nx = c.rQ * inv, ny = c.rU * inv, nz = c.rV * inv;
     T st, ct;
     lsincos(theta, &st, &ct);

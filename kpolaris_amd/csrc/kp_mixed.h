@@ -137,23 +137,23 @@ KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float freq, float bconst, 
     const float sin_theta = fmaxf(sin_theta_in, M.min_sin_theta);
     float jI, jQ, jV;  // physical emissivities (cgs), ~1e-15..1e-30 or 0 on underflow
     if (M.fit == 4) {
-        const float nus = 3.0f * ee * b * sin_theta / (4.0f * pi * me * cl) * thetae * thetae + 1.0f;
-        const float x = fmaxf(nu / fmaxf(nus, 1e-30f), 1e-30f);
-        const float xm13 = 1.0f / cbrtf(x);
-        const float exp_shape = expf(-1.8899f / xm13);
-        const float pref = ne * ee * ee * nu / (2.0f * sqrtf(3.0f) * cl * thetae * thetae);
+        const float nus = kdiv(3.0f * ee * b * sin_theta, 4.0f * pi * me * cl) * thetae * thetae + 1.0f;
+        const float x = fmaxf(kdiv(nu, fmaxf(nus, 1e-30f)), 1e-30f);
+        const float xm13 = krcp(cbrtf(x));
+        const float exp_shape = expf(kdiv(-1.8899f, xm13));
+        const float pref = kdiv(ne * ee * ee * nu, 2.0f * sqrtf(3.0f) * cl * thetae * thetae);
         jI = pref * (2.5651f * (1.0f + 1.92f * xm13 + 0.9977f * xm13 * xm13) * exp_shape);
         jQ = pref * (2.5651f * (1.0f + 0.93193f * xm13 + 0.499873f * xm13 * xm13) * exp_shape);
-        const float iv = (1.81384f / x + 3.42319f * xm13 * xm13 + 0.0292545f / sqrtf(x) + 2.03773f * xm13) * exp_shape;
+        const float iv = (kdiv(1.81384f, x) + 3.42319f * xm13 * xm13 + kdiv(0.0292545f, sqrtf(x)) + 2.03773f * xm13) * exp_shape;
         float cft = cos_theta;
         if (fabsf(cft) < 1e-10f) cft = cft >= 0.0f ? 1e-10f : -1e-10f;
-        const float tan_theta = sin_theta / cft;
-        jV = 2.0f * ne * ee * ee * nu / (tan_theta * 3.0f * sqrtf(3.0f) * cl * thetae * thetae * thetae) * iv;
+        const float tan_theta = kdiv(sin_theta, cft);
+        jV = kdiv(2.0f * ne * ee * ee * nu, tan_theta * 3.0f * sqrtf(3.0f) * cl * thetae * thetae * thetae) * iv;
     } else {
-        const float nu_c = ee * b / (2.0f * pi * me * cl);
+        const float nu_c = ee * b * (1.0f / (2.0f * pi * me * cl));
         const float nu_s = 2.0f / 9.0f * nu_c * sin_theta * thetae * thetae;
-        const float x = fmaxf(nu / fmaxf(nu_s, 1e-30f), 1e-30f);
-        const float pref = ne * ee * ee * nu_c / cl;
+        const float x = fmaxf(kdiv(nu, fmaxf(nu_s, 1e-30f)), 1e-30f);
+        const float pref = ne * ee * ee * nu_c * (1.0f / cl);
         const float term1 = sqrtf(2.0f) * pi / 27.0f * sin_theta;
         const float x13 = cbrtf(x);
         const float sx = sqrtf(x), sx13 = sqrtf(x13);
@@ -161,23 +161,24 @@ KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float freq, float bconst, 
         const float shapeI = sx + 1.8877486253633869933f * sx13;
         jI = pref * term1 * (shapeI * shapeI) * ex13;
         const float t2425 = powf(thetae, 24.0f / 25.0f);
-        const float term2 = (7.0f * t2425 + 35.0f) / (10.0f * t2425 + 75.0f);
+        const float term2 = kdiv(7.0f * t2425 + 35.0f, 10.0f * t2425 + 75.0f);
         const float shapeQ = sx + term2 * 1.8877486253633869933f * sx13;
         jQ = pref * term1 * (shapeQ * shapeQ) * ex13;
         const float sin_shift = sin_theta * 0.4356824462767121f - cos_theta * 0.9001004421765051f;
-        const float v1 = (37.0f - 87.0f * sin_shift) / (100.0f * (thetae + 1.0f));
-        const float v2 = powf(1.0f + (powf(thetae, 0.6f) / 25.0f + 0.7f) * powf(x, 0.36f), 5.0f / 3.0f);
+        const float v1 = kdiv(37.0f - 87.0f * sin_shift, 100.0f * (thetae + 1.0f));
+        const float v2 = powf(1.0f + (powf(thetae, 0.6f) * 0.04f + 0.7f) * powf(x, 0.36f), 5.0f / 3.0f);
         jV = pref * v1 * v2 * ex13;
     }
     // rescaled emissivities and the polarisation cap
-    const float jscale = R.L / (g * g);
+    const float jscale = kdiv(R.L, g * g);
     const float jIh = jI * jscale;
     float jQh = jQ * jscale;
     float jVh = jV * jscale;
     const float jps = fmaxf(fabsf(jQh), fabsf(jVh));
-    const float jP = jps > 0.0f ? jps * sqrtf(sq(jQh / jps) + sq(jVh / jps)) : 0.0f;
-    if (jIh > 0.0f && jP > 0.0f && jIh < jP / fmaxf(M.max_pol_frac, 1e-30f)) {
-        const float scale = jIh / jP * M.max_pol_frac;
+    const float ijps = krcp(jps);
+    const float jP = jps > 0.0f ? jps * sqrtf(sq(jQh * ijps) + sq(jVh * ijps)) : 0.0f;
+    if (jIh > 0.0f && jP > 0.0f && jIh < kdiv(jP, fmaxf(M.max_pol_frac, 1e-30f))) {
+        const float scale = kdiv(jIh, jP) * M.max_pol_frac;
         jQh *= scale;
         jVh *= scale;
     }
@@ -185,38 +186,40 @@ KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float freq, float bconst, 
     // the double-precision path does once B_nu underflows.
     float aI = 0.0f, aQ = 0.0f, aV = 0.0f;
     {
-        const float x = R.x0 * nu / thetae;
-        const float sp = bconst / expm1f(x);
+        const float x = kdiv(R.x0 * nu, thetae);
+        const float sp = kdiv(bconst, expm1f(x));
         if (sp > 0.0f) {
-            aI = jIh / sp;
-            aQ = jQh / sp;
-            aV = jVh / sp;
+            const float isp = krcp(sp);
+            aI = jIh * isp;
+            aQ = jQh * isp;
+            aV = jVh * isp;
         }
     }
     // Faraday rotation and conversion (symphony_rho_QV, thermal_synchrotron.hpp:461-506)
     float rhoQ, rhoV;
     {
-        const float omega0 = ee * b / (me * cl);
+        const float omega0 = ee * b * (1.0f / (me * cl));
         const float two_pi_nu = 2.0f * pi * nu;
         const float safe = fmaxf(two_pi_nu, 1e-30f);
+        const float isafe = krcp(safe);
         const float wp2 = 4.0f * pi * ne * (ee * ee / me);
-        const float xx = fmaxf(thetae * sqrtf(fmaxf(0.0f, sqrtf(2.0f) * sin_theta * (1e3f * omega0 / safe))), 1e-12f);
+        const float xx = fmaxf(thetae * sqrtf(fmaxf(0.0f, sqrtf(2.0f) * sin_theta * (1e3f * omega0 * isafe))), 1e-12f);
         float k0, k1, k2;
-        bessel_k012(1.0f / thetae, k0, k1, k2);
-        const float exp_x_47 = expf(-xx / 47.2f);
+        bessel_k012(krcp(thetae), k0, k1, k2);
+        const float exp_x_47 = expf(-xx * (1.0f / 47.2f));
         const float extraterm = (0.011f * exp_x_47 - 369.69342869090815f * powf(xx + 1e-16f, -8.0f / 3.0f)) *
-                                (0.5f + 0.5f * tanhf((logf(xx) - 4.787491742782046f) / 0.1f));
-        const float jffunc = 2.011f * expf(-powf(xx, 1.035f) / 4.7f) -
-                             cosf(xx / 2.0f) * expf(-powf(xx, 1.2f) / 2.73f) - 0.011f * exp_x_47 + extraterm;
-        const float k1_ratio = k2 > 0.0f ? k1 / k2 : 1.0f;
-        // eps11m22 = jffunc wp2 omega0^2 / (2 pi nu)^4, kept in range by dividing early
-        const float w = omega0 / safe;
-        const float e11 = jffunc * (wp2 / safe) * w * (w / safe) * (k1_ratio + 6.0f * thetae) * sin_theta * sin_theta;
-        rhoQ = pi * nu / cl * e11;
+                                (0.5f + 0.5f * tanhf((logf(xx) - 4.787491742782046f) * 10.0f));
+        const float jffunc = 2.011f * expf(-powf(xx, 1.035f) * (1.0f / 4.7f)) -
+                             cosf(xx / 2.0f) * expf(-powf(xx, 1.2f) * (1.0f / 2.73f)) - 0.011f * exp_x_47 + extraterm;
+        const float k1_ratio = k2 > 0.0f ? kdiv(k1, k2) : 1.0f;
+        // eps11m22 = jffunc wp2 omega0^2 / (2 pi nu)^4, kept in range by scaling by 1/(2 pi nu) early
+        const float w = omega0 * isafe;
+        const float e11 = jffunc * (wp2 * isafe) * w * (w * isafe) * (k1_ratio + 6.0f * thetae) * sin_theta * sin_theta;
+        rhoQ = pi * nu * (1.0f / cl) * e11;
         const float shgm = 1.0f - 0.11f * logf(1.0f + 0.035f * xx);
-        const float k0_ratio = k2 > 0.0f ? k0 / k2 : 1.0f;
-        const float e12 = (wp2 / safe) * w * (1.0f / safe) * k0_ratio * shgm * cos_theta;
-        rhoV = 2.0f * pi * nu / cl * e12;
+        const float k0_ratio = k2 > 0.0f ? kdiv(k0, k2) : 1.0f;
+        const float e12 = (wp2 * isafe) * w * isafe * k0_ratio * shgm * cos_theta;
+        rhoV = 2.0f * pi * nu * (1.0f / cl) * e12;
     }
     const float rscale = g * R.L;  // rho_inv * dl/dlambda = rho_phys * g * L
     c.jI = M.emission_scale * jIh;
@@ -248,13 +251,13 @@ KP_DEV void riaf_local_scaled(const RiafMixed& R, const St<double>& sd, RiafLoca
     const float zc = r * cs;
     const float rc = fmaxf(r * sn, M.min_radius);
     const float hd = fmaxf(M.disk_h, 1e-6f);
-    const float ne_norm = M.nth0 * expf(-(zc * zc) / (2.0f * rc * rc * hd * hd)) * powf(r, M.pow_nth);
+    const float ne_norm = M.nth0 * expf(kdiv(-(zc * zc), 2.0f * rc * rc * hd * hd)) * powf(r, M.pow_nth);
     if (!(ne_norm > 0.0f)) return;
     float ne = ne_norm * M.ne_unit;
     float thetae = fmaxf(M.Te0 * powf(r, M.pow_T) * M.thetae_unit, M.min_thetae);
     // 8 pi 0.1 m_p c^2 / 6 folded into one constant (in range for FP32)
     const float bk = float(8.0 * PI_D * 0.1 * 1.67262192369e-24 * KP_CL * KP_CL / 6.0);
-    float b = fmaxf(sqrtf(fmaxf(bk * ne / r, 0.0f)), 1e-12f);
+    float b = fmaxf(sqrtf(fmaxf(kdiv(bk * ne, r), 0.0f)), 1e-12f);
     ne *= M.ne_factor;
     thetae *= M.thetae_factor;
     b *= M.b_factor;
@@ -269,7 +272,7 @@ KP_DEV void riaf_local_scaled(const RiafMixed& R, const St<double>& sd, RiafLoca
     riaf_b_unit(g, u, bc);
     glower(g, bc, bcov);
     const float kdotb = k[0] * bcov[0] + k[1] * bcov[1] + k[2] * bcov[2] + k[3] * bcov[3];
-    const float cos_theta = clampv(kdotb / fmaxf(gscale, 1e-30f), -1.0f, 1.0f);
+    const float cos_theta = clampv(kdiv(kdotb, fmaxf(gscale, 1e-30f)), -1.0f, 1.0f);
     L.valid = 1;
     L.ne = ne;
     L.thetae = thetae;
@@ -284,7 +287,7 @@ KP_DEV void riaf_local_scaled(const RiafMixed& R, const St<double>& sd, RiafLoca
     const float b2 = screen_inner(g, u, k, s.v + 12, bc, ukc);
     const float bp2 = b1 * b1 + b2 * b2;
     if (bp2 > 1e-30f) {
-        const float ib = 1.0f / bp2;
+        const float ib = krcp(bp2);
         L.has_chi = 1;
         L.cos2chi = (b2 * b2 - b1 * b1) * ib;
         L.sin2chi = -2.0f * b1 * b2 * ib;
@@ -339,7 +342,7 @@ KP_DEV float att32(float kappa, float dl) {
 KP_DEV float src32(float kappa, float dl) {
     const float x = kappa * dl;
     if (fabsf(x) < 1e-4f) return dl * (1.0f - x * 0.5f + x * x * (1.0f / 6.0f));
-    return -expm1f(-x) / kappa;
+    return kdiv(-expm1f(-x), kappa);
 }
 
 KP_DEV void faraday32(Stokes<float>& s, const Coeffs<float>& c, float dl) {
@@ -375,7 +378,7 @@ KP_DEV void absorb_emit32(Stokes<float>& s, const Coeffs<float>& c, float dl) {
         s.V = ei * s.V + fi * c.jV;
         return;
     }
-    const float inv = 1.0f / an;
+    const float inv = krcp(an);
     const float nx = c.aQ * inv, ny = c.aU * inv, nz = c.aV * inv;
     const float ppar = nx * px + ny * py + nz * pz;
     const float jpar = nx * c.jQ + ny * c.jU + nz * c.jV;
