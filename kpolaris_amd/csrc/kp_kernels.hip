@@ -1,0 +1,181 @@
+// kp_kernels.hip -- instantiation of the imaging kernels, one family per part.
+//
+// The build compiles this file once per part (-DKP_PART=0..KP_NPARTS-1, in
+// parallel); each part instantiates its kernels and defines the selectors
+// that hand their addresses to the host code (kp_kernels.h, namespace
+// kpsel).  Every kernel is instantiated in exactly one part.
+#include "kp_kernels.h"
+
+#ifndef KP_PART
+#error "compile kp_kernels.hip with -DKP_PART=<part>"
+#endif
+
+namespace kpsel {
+
+#if KP_PART == 0
+// Pass A (every geometry and geodesic precision), the batched pass A and the
+// small utility kernels.
+const void* pass_a(bool persistent, int gp, int geom) {
+    if (geom == G_FMKS)
+        return persistent ? (const void*)pass_a_kernel<true, GP_64, G_FMKS>
+                          : (const void*)pass_a_kernel<false, GP_64, G_FMKS>;
+    if (gp == GP_GUARD)
+        return persistent ? (const void*)pass_a_kernel<true, GP_GUARD, G_BL>
+                          : (const void*)pass_a_kernel<false, GP_GUARD, G_BL>;
+    if (gp == GP_32) return persistent ? (const void*)pass_a_kernel<true, GP_32, G_BL> : nullptr;
+    return persistent ? (const void*)pass_a_kernel<true, GP_64, G_BL> : (const void*)pass_a_kernel<false, GP_64, G_BL>;
+}
+
+const void* pass_a_batch(int gp) {
+    return gp == GP_GUARD ? (const void*)pass_a_batch_kernel<GP_GUARD> : (const void*)pass_a_batch_kernel<GP_64>;
+}
+
+const void* zero_counters() { return (const void*)zero_kernel; }
+const void* derive_scalars() { return (const void*)derive_scalars_kernel; }
+#endif
+
+#if KP_PART == 1
+// Single-frequency Boyer-Lindquist pass B: FP64, and mixed with every
+// geodesic precision (the staged GP_32 / GP_64 kernels are persistent only).
+const void* pass_b_single(bool persistent, bool mixed, int gp) {
+    if (!mixed)
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_SINGLE>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_SINGLE>;
+    if (gp == GP_32) return persistent ? (const void*)pass_b_kernel<true, true, GP_32, M_SINGLE> : nullptr;
+    if (gp == GP_64)
+        return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_SINGLE>
+                          : (const void*)pass_b_kernel<false, true, GP_64, M_SINGLE>;
+    return persistent ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_SINGLE>
+                      : (const void*)pass_b_kernel<false, true, GP_GUARD, M_SINGLE>;
+}
+#endif
+
+#if KP_PART == 2
+// Multi-frequency Boyer-Lindquist pass B (thermal fits).
+const void* pass_b_mf(bool persistent, bool mixed, int gp) {
+    if (!mixed)
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_MF>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_MF>;
+    if (gp == GP_32) return persistent ? (const void*)pass_b_kernel<true, true, GP_32, M_MF> : nullptr;
+    if (gp == GP_64)
+        return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_MF>
+                          : (const void*)pass_b_kernel<false, true, GP_64, M_MF>;
+    return persistent ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_MF>
+                      : (const void*)pass_b_kernel<false, true, GP_GUARD, M_MF>;
+}
+#endif
+
+#if KP_PART == 3
+// GRMHD pass B on FMKS geodesics: thermal, nonthermal, and the slow-light
+// time probe.
+const void* pass_b_fmks(bool persistent, int mode) {
+    switch (mode) {
+    case M_GRMHD:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_GRMHD>;
+    case M_GRMHD_NT:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD_NT>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_GRMHD_NT>;
+    case M_PROBE:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_PROBE>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_PROBE>;
+    default:
+        return nullptr;
+    }
+}
+#endif
+
+#if KP_PART == 4
+// Slow-light pass B: thermal and nonthermal (part 4), and the tangent of the
+// slow-light sequence (part 8).
+const void* pass_b_slow(bool persistent, int mode) {
+    switch (mode) {
+    case M_SLOW:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_SLOW>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_SLOW>;
+    case M_SLOW_NT:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_NT>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_SLOW_NT>;
+    case M_STAN:
+        return pass_b_stan(persistent);
+    default:
+        return nullptr;
+    }
+}
+#endif
+
+#if KP_PART == 8
+const void* pass_b_stan(bool persistent) {
+    return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_STAN>
+                      : (const void*)pass_b_kernel<false, false, GP_64, M_STAN>;
+}
+#endif
+
+#if KP_PART == 5
+// RIAF tangent and nonthermal multi-frequency pass B, and the batched pass B.
+const void* pass_b_other(bool persistent, int mode) {
+    switch (mode) {
+    case M_TANGENT:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_TANGENT>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_TANGENT>;
+    case M_MF_NT:
+        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_MF_NT>
+                          : (const void*)pass_b_kernel<false, false, GP_64, M_MF_NT>;
+    default:
+        return nullptr;
+    }
+}
+
+const void* pass_b_batch(bool mixed, int gp) {
+    if (!mixed) return (const void*)pass_b_batch_kernel<false, GP_64>;
+    return gp == GP_GUARD ? (const void*)pass_b_batch_kernel<true, GP_GUARD>
+                          : (const void*)pass_b_batch_kernel<true, GP_64>;
+}
+#endif
+
+#if KP_PART == 6
+// Unsplit kernels on Boyer-Lindquist geodesics.
+const void* unsplit_bl(bool mixed, int gp, int mode) {
+    switch (mode) {
+    case M_SINGLE:
+        if (!mixed) return (const void*)unsplit_kernel<false, GP_64, M_SINGLE>;
+        return gp == GP_GUARD ? (const void*)unsplit_kernel<true, GP_GUARD, M_SINGLE>
+                              : (const void*)unsplit_kernel<true, GP_64, M_SINGLE>;
+    case M_MF:
+        if (!mixed) return (const void*)unsplit_kernel<false, GP_64, M_MF>;
+        return gp == GP_GUARD ? (const void*)unsplit_kernel<true, GP_GUARD, M_MF>
+                              : (const void*)unsplit_kernel<true, GP_64, M_MF>;
+    case M_TANGENT:
+        return mixed ? nullptr : (const void*)unsplit_kernel<false, GP_64, M_TANGENT>;
+    case M_MF_NT:
+        return mixed ? nullptr : (const void*)unsplit_kernel<false, GP_64, M_MF_NT>;
+    default:
+        return nullptr;
+    }
+}
+#endif
+
+#if KP_PART == 7
+// Unsplit kernels on FMKS geodesics: static grids here, slow light in part 9.
+const void* unsplit_fmks(int mode) {
+    switch (mode) {
+    case M_GRMHD: return (const void*)unsplit_kernel<false, GP_64, M_GRMHD>;
+    case M_GRMHD_NT: return (const void*)unsplit_kernel<false, GP_64, M_GRMHD_NT>;
+    case M_PROBE: return (const void*)unsplit_kernel<false, GP_64, M_PROBE>;
+    default: return unsplit_slow(mode);
+    }
+}
+#endif
+
+#if KP_PART == 9
+const void* unsplit_slow(int mode) {
+    switch (mode) {
+    case M_SLOW: return (const void*)unsplit_kernel<false, GP_64, M_SLOW>;
+    case M_SLOW_NT: return (const void*)unsplit_kernel<false, GP_64, M_SLOW_NT>;
+    case M_STAN: return (const void*)unsplit_kernel<false, GP_64, M_STAN>;
+    default: return nullptr;
+    }
+}
+#endif
+
+}  // namespace kpsel

@@ -13,6 +13,7 @@ is no fallback path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 from pathlib import Path
 
@@ -21,7 +22,9 @@ import numpy as np
 from . import abi
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "_lib" / "libkpolaris_amd.so"
+# KPOLARIS_AMD_LIB selects another build of the library (a tuning build);
+# the default is the in-tree build.
+LIB_PATH = Path(os.environ.get("KPOLARIS_AMD_LIB") or (PKG / "_lib" / "libkpolaris_amd.so"))
 HEADER = PKG.parent / "include" / "kp_image.h"
 
 _lib = None

@@ -2382,6 +2382,22 @@ void kpo_grmhd_derive_scalars(const kp_grmhd_model* g, kp_grmhd_cell* cells) {
         }
 }
 
+/* u^mu and b^mu at the centre of cell (i, j, k) from its primitives, as the
+ * derived-scalar pass forms them (a test hook of grmhd_four_vectors). */
+void kpo_grmhd_cell_four_vectors(const kp_grmhd_model* g, const kp_grmhd_cell* cells, int32_t i, int32_t j,
+                                 int32_t k, double ucon[4], double bcon[4]) {
+    const kpo_fmks f = grmhd_map(g);
+    const double x1 = g->startx1 + ((double)i + 0.5) * g->dx1;
+    const double x2 = g->startx2 + ((double)j + 0.5) * g->dx2;
+    const double r = exp(x1);
+    const double th = kpo_fmks_theta(&f, x1, x2);
+    double gcov[16], gcon0[4];
+    grmhd_native_metric(g, &f, x1, x2, r, th, gcov, gcon0);
+    const kp_grmhd_cell* c = &cells[((size_t)i * g->n2 + j) * g->n3 + k];
+    (void)grmhd_four_vectors(gcov, gcon0, c->prim[2], c->prim[3], c->prim[4], c->prim[5], c->prim[6], c->prim[7],
+                             ucon, bcon);
+}
+
 /* native_indices_from_native_coords (grmhd_grid.hpp:271-296). */
 static int grmhd_indices(const kp_grmhd_model* g, double x1, double x2, double phi, int* i0, int* j0, int* k0,
                          double* di, double* dj, double* dk) {

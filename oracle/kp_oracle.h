@@ -153,6 +153,9 @@ kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* params, const kp_ri
 void kpo_fmks_camera_tetrad(const kp_pass_a_params* params, const kpo_fmks* f, double xcam[4],
                             double tetrad[16]);
 void kpo_fmks_camera_ray(const kp_pass_a_params* params, const kpo_fmks* f, int32_t pixel, kpo_state* out);
+/* u^mu and b^mu at the centre of one cell from its primitives (host cells). */
+void kpo_grmhd_cell_four_vectors(const kp_grmhd_model* model, const kp_grmhd_cell* cells, int32_t i, int32_t j,
+                                 int32_t k, double ucon[4], double bcon[4]);
 /* Derived scalars of every cell from its primitives (host cells). */
 void kpo_grmhd_derive_scalars(const kp_grmhd_model* model, kp_grmhd_cell* cells);
 double kpo_grmhd_sample_prim(const kp_grmhd_model* model, const kp_grmhd_cell* cells, int v, double x1,

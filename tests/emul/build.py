@@ -13,10 +13,14 @@ LIB = HERE / "_build" / "libkp_lane_emul.so"
 
 
 def build() -> Path:
+    """Host-only build (no device code) of the lanes and the host layer."""
     from kpolaris_amd.build_lib import HIPCC_FLAGS, hipcc
     LIB.parent.mkdir(parents=True, exist_ok=True)
     srcs = [HERE / "kp_lane_emul.cpp", ROOT / "kpolaris_amd" / "csrc" / "kp_host.cpp"]
-    cmd = [hipcc(), *HIPCC_FLAGS, "-x", "hip", "-shared", *map(str, srcs), "-o", str(LIB)]
+    flags = [f for f in HIPCC_FLAGS if not f.startswith("--offload-arch")]
+    # -mfma: the host lowers the fused multiply-adds the device forms
+    cmd = [hipcc(), *flags, "-mfma", "--offload-host-only", "-x", "hip", "-shared", *map(str, srcs), "-o",
+           str(LIB)]
     subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL)
     return LIB
 

@@ -1,9 +1,10 @@
 // CPU build of the imaging kernel's per-lane state machines (test harness).
 //
-// Includes the product translation unit and drives its pass A / pass B lane
-// functions one pixel at a time on the host, so the kernel's control logic can
-// be checked against the oracle without a GPU.  Never part of the product.
-#include "../../kpolaris_amd/csrc/kp_image.hip"
+// Includes the product's kernel header and parameter preparation, built for
+// the host only, and drives the pass A / pass B lane functions one pixel at a
+// time, so the kernel's control logic can be checked against the oracle
+// without a GPU.  Never part of the product.
+#include "../../kpolaris_amd/csrc/kp_prepare.h"
 
 #include <vector>
 

@@ -6,12 +6,26 @@ Bar (per pixel, on I/Q/U/V): |x - oracle| <= 1e-10 |oracle| + 1e-8 peak(I).
 Some rays wind around the hole or return to the camera sphere far from the
 camera point; they amplify last-bit differences chaotically, so no two
 implementations with different rounding agree on them to 1e-10 (the oracle
-disagrees with itself).  Such a pixel is identified by the oracle alone: it is
-re-run with the ray slope moved by 1e-15 (about 1e-13 relative) in x and in
-y, and if any Stokes component moves by more than 1e-11 of its value (floor
-1e-6 peak) the pixel is ill-conditioned.  Pixels beyond the bar must be
-ill-conditioned, and must still agree to 1e-3 (floor 1e-6 peak); termination
-reasons must agree everywhere and step counts on well-conditioned pixels.
+disagrees with itself).  Such a pixel is identified by the oracle alone: the
+oracle is re-run with the camera offset moved by +-1e-15 in x and in y (four
+runs; about 1e-13 of the ray slope), and the per-pixel spread of those runs
+around the unperturbed oracle is the pixel's measured sensitivity.
+
+* A pixel is ill-conditioned when some Stokes component moves by more than
+  1e-11 of its value (floor 1e-6 of the frequency's peak) or its step counts
+  change in one of the perturbed runs.  Each test bounds the fraction of
+  ill-conditioned pixels it accepts and reports it.
+* Every pixel must meet |x - oracle| <= 1e-10 |oracle| + 1e-8 peak
+  + AMPLIFICATION x spread: the bar widens by a fixed multiple of the
+  oracle's own measured sensitivity, not to a fixed loose tolerance.  A pixel
+  whose value matches one of the perturbed oracle runs to the tight bar (the
+  kernel took the branch the oracle takes under a 1e-15 change) passes too.
+* Termination reasons agree on every pixel; step counts agree on every
+  well-conditioned pixel.  The one documented exception is slow light, where
+  a ray that approaches a snapshot time by ever smaller cut steps (the
+  reference stops when the remaining step is below 1e-14) may take one step
+  more or less depending on the last bits of x^0; its Stokes values must
+  still meet the bar (time_cut_steps=1).
 """
 from __future__ import annotations
 
@@ -25,78 +39,162 @@ REL = 1e-10
 PEAK_FLOOR = 1e-8
 SENSITIVITY = 1e-11
 PERTURBATION = 1e-15
-LOOSE_REL = 1e-3
-LOOSE_FLOOR = 1e-6
+SENS_FLOOR = 1e-6        # floor (of the peak) of the ill-conditioning test
+AMPLIFICATION = 10.0     # the bar's multiple of the oracle's measured spread
+# default bound on the ill-conditioned share of an image: the test images are
+# coarse samplings of fields of view dominated by the photon ring, where 15-30 %
+# of the rays wind around the hole
+MAX_ILL = 0.35
+# bound for the synthetic GRMHD tori, whose cold dense gas reaches Faraday
+# depths of 1e5-1e7 rad: the oracle's own 1e-13 path change moves the
+# polarisation angle, hence Q and U, by 1e-9 to 1e-6 on most rays (I stays
+# well-conditioned there; each component is held to its own spread)
+FARADAY_MAX_ILL = 0.65
+# share of pixels on which a Stokes component's derivative must be checkable
+MIN_CHECKED = 0.35
 STOKES = ("image_i", "image_q", "image_u", "image_v")
+OFFSETS = ((PERTURBATION, 0.0), (-PERTURBATION, 0.0), (0.0, PERTURBATION), (0.0, -PERTURBATION))
+
+
+def conditioning(run, p, ref):
+    """Perturbed oracle runs of an image: run(params) -> image dict.  Returns
+    {"spread": {name: max |r_k - ref| per element}, "dsteps": the largest
+    change of the pass-B step count, "moved": pass-A steps or reason changed
+    in some run (per pixel), "samples": the perturbed images}."""
+    spread = {n: np.zeros_like(ref[n]) for n in STOKES}
+    moved = np.zeros(ref["reason"].shape, dtype=bool)
+    dsteps = np.zeros(ref["reason"].shape, dtype=np.int64)
+    samples = []
+    for dx, dy in OFFSETS:
+        q = copy.deepcopy(p)
+        q.camera.x_offset += dx
+        q.camera.y_offset += dy
+        r2 = run(q)
+        samples.append({k: r2[k] for k in (*STOKES, "steps", "pass_a_steps", "reason")})
+        for n in STOKES:
+            spread[n] = np.maximum(spread[n], np.abs(r2[n] - ref[n]))
+        dsteps = np.maximum(dsteps, np.abs(r2["steps"].astype(np.int64) - ref["steps"]))
+        moved |= (r2["pass_a_steps"] != ref["pass_a_steps"]) | (r2["reason"] != ref["reason"])
+    return {"spread": spread, "dsteps": dsteps, "moved": moved, "samples": samples}
 
 
 def oracle_with_sensitivity(p, m, offset=None, stride=None, count=None):
-    """Oracle image plus per-field max |change| under the two perturbations."""
+    """Oracle RIAF image with its conditioning under "_cond"."""
     def run(pp):
         if offset is None:
             return oracle.run_riaf_image(pp, m)[0]
         return oracle.run_riaf_image_strided(pp, m, offset, stride, count)[0]
 
     ref = run(p)
-    sens = {n: np.zeros_like(ref[n]) for n in STOKES}
-    moved_steps = np.zeros(ref["reason"].shape, dtype=bool)
-    for dx, dy in ((PERTURBATION, 0.0), (0.0, PERTURBATION)):
-        q = copy.deepcopy(p)
-        q.camera.x_offset += dx
-        q.camera.y_offset += dy
-        r2 = run(q)
-        for n in STOKES:
-            sens[n] = np.maximum(sens[n], np.abs(r2[n] - ref[n]))
-        moved_steps |= (r2["steps"] != ref["steps"]) | (r2["pass_a_steps"] != ref["pass_a_steps"])
-    ref["_sens"] = sens
-    ref["_moved_steps"] = moved_steps
+    ref["_cond"] = conditioning(run, p, ref)
     return ref
 
 
-def ill_conditioned(ref, peak):
-    s = np.zeros(ref["reason"].shape)
-    for n in STOKES:
-        s = np.maximum(s, ref["_sens"][n] / (np.abs(ref[n]) + LOOSE_FLOOR * peak + 1e-300))
-    return (s > SENSITIVITY) | ref["_moved_steps"]
+def sensitivity(run, p, ref):
+    """Conditioning of an oracle image (any model) from a runner."""
+    return conditioning(run, p, ref)
 
 
-def assert_parity(got, ref, label, peak=None, diagnostics=True):
-    """got/ref: dicts of per-pixel arrays; ref from oracle_with_sensitivity."""
+def _peaks(ref, nfreq):
+    n = ref["reason"].size
+    return [float(np.max(np.abs(ref["image_i"][f * n:(f + 1) * n]))) if n else 0.0 for f in range(nfreq)]
+
+
+def ill_conditioned(ref, cond, nfreq=1, time_cut_steps=0):
+    """Per pixel: the oracle's own perturbation moves it beyond SENSITIVITY
+    of a Stokes value (floor SENS_FLOOR of the peak) or changes its step
+    counts (beyond the time-cut step jitter that slow light allows)."""
+    n = ref["reason"].size
+    ill = cond["moved"] | (cond["dsteps"] > time_cut_steps)
+    for f, peak in enumerate(_peaks(ref, nfreq)):
+        sl = slice(f * n, (f + 1) * n)
+        for name in STOKES:
+            ill |= cond["spread"][name][sl] > SENSITIVITY * (np.abs(ref[name][sl]) + SENS_FLOOR * peak)
+    return ill
+
+
+def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, diagnostics=True):
+    """The parity rule of this module for a [nfreq][npix]-Stokes image.
+    Returns a summary (printed by the tests): pixels, ill-conditioned count,
+    pixels that needed the widened bar, the worst error over the tight bar."""
+    n = ref["reason"].size
     assert np.array_equal(got["reason"], ref["reason"]), (
         f"{label}: reasons differ at {np.flatnonzero(got['reason'] != ref['reason'])[:10]}")
-    if peak is None:
-        peak = float(np.max(np.abs(ref["image_i"]))) if ref["image_i"].size else 0.0
-    ill = ill_conditioned(ref, peak)
-    beyond = np.zeros(ill.shape, dtype=bool)
-    for n in STOKES:
-        err = np.abs(got[n] - ref[n])
-        beyond |= err > REL * np.abs(ref[n]) + PEAK_FLOOR * peak
-        loose = err > LOOSE_REL * np.abs(ref[n]) + LOOSE_FLOOR * peak
-        assert not loose.any(), (
-            f"{label} {n}: {int(loose.sum())} pixels beyond even the 1e-3 bound at "
-            f"{np.flatnonzero(loose)[:5]}")
-    bad = beyond & ~ill
-    assert not bad.any(), (
-        f"{label}: {int(bad.sum())} well-conditioned pixels beyond 1e-10 at {np.flatnonzero(bad)[:8]}")
-    steps_bad = ((got["steps"] != ref["steps"]) | (got["pass_a_steps"] != ref["pass_a_steps"])) & ~ill
+    ill = ill_conditioned(ref, cond, nfreq, time_cut_steps)
+    frac = float(ill.mean()) if n else 0.0
+    assert frac <= max_ill, f"{label}: {int(ill.sum())}/{n} ill-conditioned pixels exceed the bound {max_ill}"
+    widened = np.zeros(n, dtype=bool)
+    worst_tight = 0.0
+    for f, peak in enumerate(_peaks(ref, nfreq)):
+        sl = slice(f * n, (f + 1) * n)
+        for name in STOKES:
+            g, r = got[name][sl], ref[name][sl]
+            err = np.abs(g - r)
+            tight = REL * np.abs(r) + PEAK_FLOOR * peak
+            # well-conditioned pixels: the tight bar alone
+            bar = tight + np.where(ill, AMPLIFICATION * cond["spread"][name][sl], 0.0)
+            ok = err <= bar
+            if not ok.all():
+                # an ill-conditioned pixel on the branch a perturbed oracle run takes
+                for s in cond["samples"]:
+                    sv = s[name][sl]
+                    ok |= ill & (np.abs(g - sv) <= REL * np.abs(sv) + PEAK_FLOOR * peak)
+            assert ok.all(), (
+                f"{label} f{f} {name}: {int((~ok).sum())} pixels beyond the conditioned bar at "
+                f"{np.flatnonzero(~ok)[:6]}, err/tight {(err / tight)[~ok][:6]}, ill {ill[~ok][:6]}")
+            widened |= err > tight
+            worst_tight = max(worst_tight, float(np.max(err / tight, initial=0.0)))
+    dsteps = np.abs(got["steps"] - ref["steps"])
+    da = got["pass_a_steps"] != ref["pass_a_steps"]
+    steps_bad = ((dsteps > time_cut_steps) | da) & ~ill
     assert not steps_bad.any(), (
         f"{label}: step counts differ on well-conditioned pixels {np.flatnonzero(steps_bad)[:8]}")
     if diagnostics:
-        # end-state diagnostics are compared on well-conditioned rays that
-        # reached the camera; rays stopped by max_steps close to the horizon
-        # (where the Boyer-Lindquist k diverges) have no well-defined end state
-        wc = ~ill & (ref["reason"] == 1)
-        for n in ("closure_x", "closure_k", "final_null", "frame_error", "det_r", "overlap_r11",
-                  "overlap_r12", "overlap_r21", "overlap_r22", "basis_rotation_angle"):
-            np.testing.assert_allclose(got[n][wc], ref[n][wc], rtol=1e-4, atol=1e-8,
-                                       err_msg=f"{label} {n}")
-    return {"pixels": int(ill.size), "ill_conditioned": int(ill.sum()),
-            "beyond_1e-10": int(beyond.sum())}
+        # end-state diagnostics on well-conditioned rays that reached the
+        # camera (rays stopped by max_steps near the horizon, where the
+        # Boyer-Lindquist k diverges, have no well-defined end state)
+        wc = ~ill & (ref["reason"] == 1) & (dsteps == 0)
+        for name in ("closure_x", "closure_k", "final_null", "frame_error", "det_r", "overlap_r11",
+                     "overlap_r12", "overlap_r21", "overlap_r22", "basis_rotation_angle"):
+            if name in got and name in ref:
+                np.testing.assert_allclose(got[name][wc], ref[name][wc], rtol=1e-4, atol=1e-8,
+                                           err_msg=f"{label} {name}")
+    print(f"parity {label}: ill-conditioned {int(ill.sum())}/{n} (bound {max_ill}), needing the widened bar "
+          f"{int(widened.sum())}, worst error / tight bar {worst_tight:.3g}", flush=True)
+    return {"pixels": n, "ill_conditioned": int(ill.sum()), "ill_fraction": round(frac, 4),
+            "widened": int(widened.sum()), "max_err_over_tight_bar": worst_tight,
+            "step_count_differences": int(((dsteps > 0) | da).sum())}
+
+
+def assert_parity(got, ref, label, peak=None, diagnostics=True, max_ill=MAX_ILL):
+    """Single-frequency RIAF image against oracle_with_sensitivity's result."""
+    del peak  # the peak is the oracle image's own
+    return check(got, ref, ref["_cond"], label, 1, max_ill=max_ill, diagnostics=diagnostics)
+
+
+def assert_parity_grid(got, ref, nfreq, label, cond, max_ill=MAX_ILL, time_cut_steps=0):
+    """GRMHD, slow-light or multi-frequency image ([nfreq][npix] Stokes)
+    against the oracle with the conditioning `cond` of the same runner."""
+    return check(got, ref, cond, label, nfreq, max_ill=max_ill, time_cut_steps=time_cut_steps,
+                 diagnostics=False)
+
+
+def split_frequencies(img, nfreq):
+    """Per-frequency views of a multi-frequency image ([nfreq][npix] Stokes,
+    per-pixel diagnostics shared)."""
+    npix = img["reason"].size
+    out = []
+    for f in range(nfreq):
+        d = {k: v for k, v in img.items() if not k.startswith("_") and k not in STOKES}
+        for n in STOKES:
+            d[n] = img[n][f * npix:(f + 1) * npix]
+        out.append(d)
+    return out
 
 
 # ---------------------------------------------------------------------------
-# Mixed precision (FP32 derivatives, coefficients and transfer; FP64 state) is
-# not a parity path: it is held to image-level error bars against an FP64
+# Mixed precision (FP32 coefficients and transfer; FP64 geodesics by default)
+# is not a parity path: it is held to image-level error bars against an FP64
 # image of the same inputs.
 # ---------------------------------------------------------------------------
 MIXED_TOTAL_I = 1e-4     # |sum I - sum I_ref| / sum I_ref
@@ -149,91 +247,25 @@ def assert_mixed(got, ref, label, pol="relative"):
     return e
 
 
-def sensitivity(run, p, ref):
-    """Per-pixel ill-conditioning of an oracle image from a runner
-    run(params) -> image: the pixel moves by more than SENSITIVITY of its
-    value (floor LOOSE_FLOOR of the frequency's peak) when the ray slope moves
-    by PERTURBATION in x or y, or its step counts change."""
-    n = ref["reason"].size
-    nfreq = ref["image_i"].size // max(n, 1)
-    ill = np.zeros(n, dtype=bool)
-    for dx, dy in ((PERTURBATION, 0.0), (0.0, PERTURBATION)):
-        q = copy.deepcopy(p)
-        q.camera.x_offset += dx
-        q.camera.y_offset += dy
-        r2 = run(q)
-        ill |= (r2["steps"] != ref["steps"]) | (r2["pass_a_steps"] != ref["pass_a_steps"])
-        for f in range(nfreq):
-            sl = slice(f * n, (f + 1) * n)
-            peak = float(np.max(np.abs(ref["image_i"][sl]))) if n else 0.0
-            for name in STOKES:
-                d = np.abs(r2[name][sl] - ref[name][sl])
-                ill |= d > SENSITIVITY * (np.abs(ref[name][sl]) + LOOSE_FLOOR * peak)
-    return ill
-
-
-def assert_parity_grid(got, ref, nfreq, label, max_step_diff=None, ill=None):
-    """Parity of a GRMHD (grid or snapshot-sequence) image, [nfreq][npix]
-    Stokes: identical termination reasons; on rays whose step counts agree,
-    the per-pixel bar (1e-10 relative + 1e-8 of the frequency's peak I); rays
-    whose step sequence moved (a chaotic near-photon-ring ray, or a slow-light
-    ray approaching a snapshot time by ever smaller cut steps), and the
-    ill-conditioned rays `ill` of the oracle's own sensitivity, within 1e-3
-    (+1e-6 of peak).  max_step_diff bounds |steps - steps_ref| on the rays
-    that are not ill-conditioned when given."""
-    assert np.array_equal(got["reason"], ref["reason"]), (
-        f"{label}: reasons differ at {np.flatnonzero(got['reason'] != ref['reason'])[:10]}")
-    moved = (got["steps"] != ref["steps"]) | (got["pass_a_steps"] != ref["pass_a_steps"])
-    if ill is not None:
-        moved = moved | ill
-    if max_step_diff is not None:
-        # ill-conditioned rays change their step counts under rounding-level
-        # perturbations of the oracle itself, so the bound holds on the others
-        d = np.abs(got["steps"] - ref["steps"])
-        if ill is not None:
-            d = np.where(ill, 0, d)
-        assert np.max(d, initial=0) <= max_step_diff, (label, np.flatnonzero(d > max_step_diff)[:8])
-    n = ref["reason"].size
-    worst = 0.0
-    for f in range(nfreq):
-        sl = slice(f * n, (f + 1) * n)
-        peak = float(np.max(np.abs(ref["image_i"][sl]))) if n else 0.0
-        for name in STOKES:
-            err = np.abs(got[name][sl] - ref[name][sl])
-            bad = err > REL * np.abs(ref[name][sl]) + PEAK_FLOOR * peak
-            assert not (bad & ~moved).any(), (label, f, name, np.flatnonzero(bad & ~moved)[:8])
-            loose = err > LOOSE_REL * np.abs(ref[name][sl]) + LOOSE_FLOOR * peak
-            assert not (loose & moved).any(), (label, f, name, np.flatnonzero(loose & moved)[:8])
-            if peak > 0.0:
-                worst = max(worst, float(np.max(err[~moved], initial=0.0)) / peak)
-    return {"pixels": n, "moved": int(moved.sum()), "max_err_over_peak_unmoved": worst}
-
-
-def split_frequencies(img, nfreq):
-    """Per-frequency views of a multi-frequency image ([nfreq][npix] Stokes,
-    per-pixel diagnostics shared)."""
-    npix = img["reason"].size
-    out = []
-    for f in range(nfreq):
-        d = {k: v for k, v in img.items() if not k.startswith("_") and k not in STOKES}
-        for n in STOKES:
-            d[n] = img[n][f * npix:(f + 1) * npix]
-        out.append(d)
-    return out
-
-
-def assert_parity_multifrequency(got, ref_mf, nfreq, ref_single, label):
-    """Each frequency slice of got against the oracle's multi-frequency image at
-    the parity bar, with ill-conditioned pixels taken from ref_single (an
-    oracle_with_sensitivity run of the same geodesics: the conditioning of a
-    ray is a property of its geodesic)."""
+def assert_parity_multifrequency(got, ref_mf, nfreq, ref_single, label, max_ill=MAX_ILL):
+    """Each frequency slice of got against the oracle's multi-frequency image,
+    with the conditioning of ref_single (an oracle_with_sensitivity run of the
+    same geodesics: a ray's conditioning is a property of its geodesic, and
+    each frequency's spread is bounded by the single-frequency one scaled to
+    that frequency's values)."""
     g = split_frequencies(got, nfreq)
     r = split_frequencies(ref_mf, nfreq)
+    single = ref_single["_cond"]
     res = []
     for f in range(nfreq):
         rf = dict(r[f])
-        rf["_sens"] = ref_single["_sens"]
-        rf["_moved_steps"] = ref_single["_moved_steps"]
-        res.append(assert_parity(g[f], rf, f"{label} f{f}",
-                                 peak=float(np.max(np.abs(rf["image_i"]))), diagnostics=(f == 0)))
+        # the single-frequency spread, relative to its own values, carried to
+        # this frequency's values
+        scale = {}
+        for n in STOKES:
+            rel = single["spread"][n] / (np.abs(ref_single[n]) + SENS_FLOOR * float(
+                np.max(np.abs(ref_single["image_i"]))) + 1e-300)
+            scale[n] = rel * (np.abs(rf[n]) + SENS_FLOOR * float(np.max(np.abs(rf["image_i"]))))
+        cond = {"spread": scale, "moved": single["moved"], "dsteps": single["dsteps"], "samples": []}
+        res.append(check(g[f], rf, cond, f"{label} f{f}", 1, max_ill=max_ill, diagnostics=(f == 0)))
     return res

@@ -52,25 +52,16 @@ def test_gpu_grmhd_image_parity(snap, persistent):
     torch.cuda.synchronize()
     got = {k: v.cpu().numpy() for k, v in out.items()}
     ref, ost = oracle.run_grmhd_image(p, m, host, F)
-    assert np.array_equal(got["reason"], ref["reason"])
-    assert st["rhs_evals"] == ost["rhs_evals"] or not np.array_equal(got["steps"], ref["steps"])
-    n = 144
-    for f in range(3):
-        sl = slice(f * n, (f + 1) * n)
-        peak = np.abs(ref["image_i"][sl]).max()
-        assert peak > 0.0
-        for name in parity.STOKES:
-            err = np.abs(got[name][sl] - ref[name][sl])
-            bad = err > 1e-10 * np.abs(ref[name][sl]) + 1e-8 * peak
-            # rays whose step sequence differs (chaotic near-photon-ring rays) get the loose bar
-            moved = (got["steps"] != ref["steps"]) | (got["pass_a_steps"] != ref["pass_a_steps"])
-            assert not (bad & ~moved).any(), (f, name, float(err.max() / peak))
-            assert np.all(err[moved] <= 1e-3 * np.abs(ref[name][sl][moved]) + 1e-6 * peak), (f, name)
-        print(f, "max err / peak", float(max(np.abs(got[nm][sl] - ref[nm][sl]).max() for nm in parity.STOKES) / peak),
-              flush=True)
+    cond = parity.sensitivity(lambda q: oracle.run_grmhd_image(q, m, host, F)[0], p, ref)
+    print(parity.assert_parity_grid(got, ref, 3, f"gpu GRMHD 3 freqs persistent={persistent}", cond,
+                                    max_ill=parity.FARADAY_MAX_ILL), flush=True)
+    if np.array_equal(got["steps"], ref["steps"]) and np.array_equal(got["pass_a_steps"], ref["pass_a_steps"]):
+        assert st["rhs_evals"] == ost["rhs_evals"]
 
 
 def test_gpu_grmhd_unsplit_and_single_frequency(snap):
+    """The unsplit kernel runs the split kernels' lane code (one
+    contraction setting for every instantiation): identical images."""
     import torch
     from kpolaris_amd import grmhd_synth as gs, image
     print("case: GRMHD unsplit vs split 10x10", flush=True)
@@ -81,8 +72,7 @@ def test_gpu_grmhd_unsplit_and_single_frequency(snap):
     b, _ = image.run_grmhd_image(p, grid, split_transport=0)
     torch.cuda.synchronize()
     for k in a:
-        np.testing.assert_allclose(b[k].cpu().numpy(), a[k].cpu().numpy(), rtol=1e-9, atol=1e-12 *
-                                   float(a["image_i"].abs().max()) if k.startswith("image") else 1e-9, err_msg=k)
+        assert np.array_equal(b[k].cpu().numpy(), a[k].cpu().numpy(), equal_nan=True), k
 
 
 def test_gpu_grmhd_rejects_wrong_coordinates(snap):

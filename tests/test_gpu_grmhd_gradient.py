@@ -62,15 +62,20 @@ def test_gpu_grmhd_gradient_value_and_oracle_differences(snap):
         assert np.max(np.abs(got[k] - ref_static[k])) <= 1e-12 * peak, k
     g = grad.cpu().numpy()
     ref, _ = oracle.run_grmhd_image(p, m, host)
-    ill = parity.sensitivity(lambda q: oracle.run_grmhd_image(q, m, host)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 1, "gpu static GRMHD tangent value", ill=ill), flush=True)
-    moved = ill | (got["steps"] != ref["steps"])
+    cond = parity.sensitivity(lambda q: oracle.run_grmhd_image(q, m, host)[0], p, ref)
+    print(parity.assert_parity_grid(got, ref, 1, "gpu static GRMHD tangent value", cond,
+                                    max_ill=parity.FARADAY_MAX_ILL), flush=True)
+    # the derivatives are compared on the pixels that the oracle's own
+    # sensitivity leaves well-conditioned (every component) and whose step
+    # counts equal the oracle's
+    same_steps = got["steps"] == ref["steps"]
     for kind in range(3):
         fd, same = _fd(p, m, host, kind, 1e-4)
         fd2, same2 = _fd(p, m, host, kind, 5e-5)
-        ok = same & same2 & ~moved
-        assert ok.sum() >= 30
         for s, name in enumerate(STOKES):
+            ok = same & same2 & same_steps & ~parity.ill_conditioned(ref, cond, 1, 0)
+            print(f"gradient check of {name} on {int(ok.sum())}/{ok.size} pixels", flush=True)
+            assert ok.sum() >= parity.MIN_CHECKED * ok.size
             scale = np.abs(fd[name][ok]).max()
             rich = (4.0 * fd2[name] - fd[name]) / 3.0
             err = np.abs(g[kind, s][ok] - fd[name][ok]).max() / scale
@@ -79,7 +84,7 @@ def test_gpu_grmhd_gradient_value_and_oracle_differences(snap):
                   flush=True)
             # plain central differences carry an O(h^2) truncation (up to ~1e-4 of the
             # largest derivative); the Richardson value removes it
-            assert err < 1e-3 and err_r < 1e-6, (kind, name, err, err_r)
+            assert err < 1e-3 and err_r < 1e-7, (kind, name, err, err_r)
             # most pixels agree to rounding; a rounding-level difference can flip one
             # accept/reject decision along a ray without changing its step count, and
             # the derivative of the discretised transfer then moves by O(rtol)

@@ -66,8 +66,8 @@ def test_gpu_riaf_nonthermal_multifrequency_and_unsplit():
     torch.cuda.synchronize()
     got = {k: v.cpu().numpy() for k, v in out.items()}
     ref, _ = oracle.run_riaf_multifrequency_image(p, m, F)
-    ill = parity.sensitivity(lambda q: oracle.run_riaf_multifrequency_image(q, m, F)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 3, "riaf kappa mf", ill=ill), flush=True)
+    cond = parity.sensitivity(lambda q: oracle.run_riaf_multifrequency_image(q, m, F)[0], p, ref)
+    print(parity.assert_parity_grid(got, ref, 3, "riaf kappa mf", cond), flush=True)
     out2, _ = image.run_riaf_image(p, m, split_transport=0)
     torch.cuda.synchronize()
     got2 = {k: v.cpu().numpy() for k, v in out2.items()}
@@ -96,8 +96,8 @@ def test_gpu_grmhd_nonthermal_parity(dist, variable):
     torch.cuda.synchronize()
     got = {k: v.cpu().numpy() for k, v in out.items()}
     ref, _ = oracle.run_grmhd_image(p, m, host, F)
-    ill = parity.sensitivity(lambda q: oracle.run_grmhd_image(q, m, host, F)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 2, f"grmhd emission {dist}/{variable}", ill=ill), flush=True)
+    cond = parity.sensitivity(lambda q: oracle.run_grmhd_image(q, m, host, F)[0], p, ref)
+    print(parity.assert_parity_grid(got, ref, 2, f"grmhd emission {dist}/{variable}", cond), flush=True)
     assert np.max(ref["image_i"]) > 0.0
 
 

@@ -201,8 +201,8 @@ def test_gpu_unsupported_inputs_fail_loudly():
 def test_gpu_unsplit_transport():
     """split_transport = 0 (one kernel, pass A then pass B per lane) against the
     oracle at the parity bar (pass_b.hpp:768-1114 computes what
-    split_driver.hpp:41-104 does), and its mixed-precision form against the
-    split mixed kernels at the mixed bars."""
+    split_driver.hpp:41-104 does), and bit for bit against the split kernels,
+    FP64 and mixed."""
     from kpolaris_amd import abi
     print("case: unsplit 32x32 fp64 and mixed", flush=True)
     p, m, _, _ = _options("riaf_m87_c2.par", nx=32, ny=32).resolve_riaf()
@@ -210,6 +210,13 @@ def test_gpu_unsplit_transport():
     ref = parity.oracle_with_sensitivity(p, m)
     print(parity.assert_parity(got, ref, "unsplit"), flush=True)
     assert st["pass_b_ms"] == 0.0 and st["pass_a_ms"] > 0.0
-    a, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED)
-    b, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED, split_transport=0)
-    parity.assert_mixed(b, a, "unsplit mixed")
+    # the unsplit kernel runs the split kernels' lane code with the same
+    # geodesic precision: bit-identical images in FP64 and in every mixed mode
+    for guard in (0, 1):
+        a, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED, guard_mode=guard)
+        b, _ = _gpu_image(p, m, precision=abi.PRECISION_MIXED, guard_mode=guard, split_transport=0)
+        for k in a:
+            assert np.array_equal(a[k], b[k], equal_nan=True), (guard, k)
+    a, _ = _gpu_image(p, m)
+    for k in a:
+        assert np.array_equal(a[k], got[k], equal_nan=True), k

@@ -61,22 +61,27 @@ def test_gpu_slow_light_gradient_against_oracle(seq):
     got = {k: v.cpu().numpy() for k, v in out.items()}
     g = grad.cpu().numpy()
     ref, _, _ = oracle.run_grmhd_slow_light_image(p, model, host, times, T_OBS)
-    ill = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(q, model, host, times, T_OBS)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 1, "gpu slow-light tangent value", max_step_diff=1, ill=ill), flush=True)
-    moved = ill | (got["steps"] != ref["steps"])
+    cond = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(q, model, host, times, T_OBS)[0], p, ref)
+    print(parity.assert_parity_grid(got, ref, 1, "gpu slow-light tangent value", cond, time_cut_steps=1,
+                                    max_ill=parity.FARADAY_MAX_ILL), flush=True)
+    # the derivatives are compared on the pixels that the oracle's own
+    # sensitivity leaves well-conditioned (every component) and whose step
+    # counts equal the oracle's
+    same_steps = got["steps"] == ref["steps"]
     for kind in range(3):
         fd, same = _fd(p, model, host, times, kind, 1e-4)
         fd2, same2 = _fd(p, model, host, times, kind, 5e-5)
-        ok = same & same2 & ~moved
-        assert ok.sum() >= 30
         for s, name in enumerate(STOKES):
+            ok = same & same2 & same_steps & ~parity.ill_conditioned(ref, cond, 1, 1)
+            print(f"gradient check of {name} on {int(ok.sum())}/{ok.size} pixels", flush=True)
+            assert ok.sum() >= parity.MIN_CHECKED * ok.size
             scale = np.abs(fd[name][ok]).max()
             rich = (4.0 * fd2[name] - fd[name]) / 3.0
             err = np.abs(g[kind, s][ok] - fd[name][ok]).max() / scale
             err_r = np.abs(g[kind, s][ok] - rich[ok]).max() / scale
             print(f"d/d{('q_density', 'R_high', 'q_magnetic')[kind]} {name}: {err:.2e} (Richardson {err_r:.2e})",
                   flush=True)
-            assert err < 1e-4 and err_r < 1e-6, (kind, name, err, err_r)
+            assert err < 1e-4 and err_r < 1e-7, (kind, name, err, err_r)
             # most pixels agree to rounding; a rounding-level difference can flip one
             # accept/reject decision along a ray without changing its step count, and
             # the derivative of the discretised transfer then moves by O(rtol)

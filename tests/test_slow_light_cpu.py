@@ -246,10 +246,10 @@ def test_emulated_slow_light_lanes_match_oracle(seq, emul, mode, interp):
                                                   interpolation=interp)
     assert w == wo[0] == 0
     assert np.all(np.isnan(req))
-    ill = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
+    cond = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
         q, model, cells, times, T_OBS, step_mode=mode, interpolation=interp)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 1, f"emulated slow light mode {mode} interp {interp}",
-                                    max_step_diff=1, ill=ill), "ill", int(ill.sum()), flush=True)
+    print(parity.assert_parity_grid(got, ref, 1, f"emulated slow light mode {mode} interp {interp}", cond,
+                                    time_cut_steps=1, max_ill=parity.FARADAY_MAX_ILL), flush=True)
 
 
 def test_emulated_slow_light_waiting_rays_and_frequencies(seq, emul):
@@ -267,9 +267,10 @@ def test_emulated_slow_light_waiting_rays_and_frequencies(seq, emul):
     freqs = [86e9, 230e9]
     got2, _, _ = _emul_run(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_FLUID, freqs)
     ref2, _, _ = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS, frequencies=freqs)
-    ill = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
+    cond = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
         q, model, cells, times, T_OBS, frequencies=freqs)[0], p, ref2)
-    print(parity.assert_parity_grid(got2, ref2, 2, "emulated slow light 2 freqs", max_step_diff=1, ill=ill),
+    print(parity.assert_parity_grid(got2, ref2, 2, "emulated slow light 2 freqs", cond, time_cut_steps=1,
+                                    max_ill=parity.FARADAY_MAX_ILL),
           flush=True)
 
 
@@ -320,8 +321,9 @@ def test_emulated_slow_light_gradient_against_central_differences(seq, emul):
                                                    C.byref(img), grad.ctypes.data_as(P(C.c_double)),
                                                    req.ctypes.data_as(P(C.c_double))) == 0
     ref, _, _ = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS)
-    ill = parity.sensitivity(lambda qq: oracle.run_grmhd_slow_light_image(qq, model, cells, times, T_OBS)[0], p, ref)
-    print(parity.assert_parity_grid(arrays, ref, 1, "emulated slow-light tangent value", max_step_diff=1, ill=ill),
+    cond = parity.sensitivity(lambda qq: oracle.run_grmhd_slow_light_image(qq, model, cells, times, T_OBS)[0], p, ref)
+    print(parity.assert_parity_grid(arrays, ref, 1, "emulated slow-light tangent value", cond, time_cut_steps=1,
+                                    max_ill=parity.FARADAY_MAX_ILL),
           flush=True)
     # central differences at h = 1e-4 carry an O(h^2) truncation error (up to
     # 3e-5 of the largest derivative on strongly Faraday-rotated rays); the
@@ -329,9 +331,10 @@ def test_emulated_slow_light_gradient_against_central_differences(seq, emul):
     for kind in range(3):
         fd, same = _slow_fd(p, model, cells, times, kind, 1e-4)
         fd2, same2 = _slow_fd(p, model, cells, times, kind, 5e-5)
-        ok = same & same2 & ~ill
-        assert ok.sum() >= 30
         for s, name in enumerate(STOKES):
+            ok = same & same2 & (arrays["steps"] == ref["steps"]) & ~parity.ill_conditioned(ref, cond, 1, 1)
+            print(f"gradient check of {name} on {int(ok.sum())}/{ok.size} pixels", flush=True)
+            assert ok.sum() >= parity.MIN_CHECKED * ok.size
             scale = np.abs(fd[name][ok]).max()
             assert scale > 0.0
             err = np.abs(grad[kind, s][ok] - fd[name][ok]).max() / scale
@@ -340,7 +343,7 @@ def test_emulated_slow_light_gradient_against_central_differences(seq, emul):
             print(f"slow-light d/d{('q_density', 'R_high', 'q_magnetic')[kind]} {name}: max rel err {err:.2e}, "
                   f"against Richardson {err_r:.2e}", flush=True)
             assert err < 1e-4, (kind, name, err)
-            assert err_r < 1e-6, (kind, name, err_r)
+            assert err_r < 1e-7, (kind, name, err_r)
             # most pixels agree to rounding; a rounding-level difference can flip one
             # accept/reject decision along a ray without changing its step count, and
             # the derivative of the discretised transfer then moves by O(rtol)
