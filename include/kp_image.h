@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define KP_ABI_VERSION 2
+#define KP_ABI_VERSION 3
 
 /* Largest frequency list of one shared-geodesic image (the reference's
  * KPOLARIS_MAX_FREQUENCIES build option, CMakeLists.txt:14). */
@@ -276,7 +276,24 @@ typedef struct kp_run_options {
                                  (persistent split launches; others use 1) */
     double fp64_radius;       /* mixed: attempts starting at r < fp64_radius run in FP64 */
     void* stream;             /* hipStream_t, NULL = default stream */
+    /* Device workspace of the run (pass-A endpoints, queue counters, staged
+     * and slow-light tables), owned by the caller, at least the size
+     * kp_image_workspace_size reports, and not used by other work while the
+     * run is in flight on `stream`.  NULL: the library keeps one workspace
+     * per (device, stream), grown on demand (it synchronises that stream
+     * and reallocates when a larger image arrives), so runs on different
+     * streams never share buffers. */
+    void* workspace;
+    uint64_t workspace_bytes;
 } kp_run_options;
+
+/* Device workspace bytes of an image of npix pixels with nfreq output
+ * frequencies (0 or 1: one), nsnap slow-light snapshots (0: none) and these
+ * options (precision, guard_mode and split_transport select the staged
+ * mixed-precision buffers).  The kernels never allocate: a caller that
+ * passes options->workspace of this size makes the run allocation-free. */
+kp_status kp_image_workspace_size(int64_t npix, int32_t nfreq, int32_t nsnap, const kp_run_options* options,
+                                  uint64_t* bytes);
 
 /* Work counters and device timings of one run.  The unit counts are what the
  * roofline's algorithmic FLOPs are priced on. */
@@ -423,6 +440,35 @@ void kp_batch_destroy(kp_batch* batch);
 /* create + launch + wait + destroy; stats->kernel_ms is the graph's device time. */
 kp_status kp_run_riaf_batch(const kp_batch_job* jobs, int32_t njobs, const kp_run_options* options,
                             kp_image_stats* stats);
+
+/* ImageBatchOptions (src/image/batch_driver.hpp:23-27): at most
+ * max_in_flight jobs per window, and a window's device outputs
+ * (KP_BATCH_BYTES_PER_PIXEL per pixel) within max_device_output_bytes. */
+typedef struct kp_batch_limits {
+    int64_t max_in_flight;
+    int64_t max_device_output_bytes;
+} kp_batch_limits;
+/* BatchImageDeviceData::bytes_per_pixel: 15 doubles and 3 int32 per pixel. */
+#define KP_BATCH_BYTES_PER_PIXEL (15 * 8 + 3 * 4)
+void kp_batch_limits_default(kp_batch_limits* limits); /* 4 jobs, 256 MiB */
+/* ImageBatchTiming (src/image/batch_driver.hpp:29-35). */
+typedef struct kp_batch_timing {
+    double allocate_seconds;
+    double kernel_seconds;
+    double copy_seconds;
+    int64_t windows;
+    int64_t peak_device_output_bytes;
+} kp_batch_timing;
+/* run_image_batch_metric with host outputs (batch_driver.hpp:126-187): each
+ * job's `out` holds HOST pointers (null fields are skipped).  Every job is
+ * validated before any work: limits must be positive, each job's nx * ny a
+ * positive int-sized product whose outputs fit the budget, radiation_substeps
+ * positive (KP_INVALID_ARGUMENT otherwise).  Jobs then run in windows, in
+ * order: a window takes jobs while both limits hold, allocates their device
+ * outputs, runs them as one batch graph, copies back and frees.  njobs == 0
+ * is a valid empty batch.  Synchronous on options->stream. */
+kp_status kp_run_riaf_batch_host(const kp_batch_job* jobs, int32_t njobs, const kp_batch_limits* limits,
+                                 const kp_run_options* options, kp_batch_timing* timing);
 
 /*
  * GRMHD grid image on FMKS geodesics (params->coordinate_system must be

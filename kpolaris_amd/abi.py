@@ -64,7 +64,7 @@ PRECISION_MIXED = 1
 FP64_RADIUS_DEFAULT = 2.5
 
 # include/kp_image.h KP_ABI_VERSION this mirror matches
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)
@@ -198,6 +198,23 @@ class BatchJob(C.Structure):
     ]
 
 
+class BatchLimits(C.Structure):
+    """kp_batch_limits (ImageBatchOptions, src/image/batch_driver.hpp:23-27)."""
+    _fields_ = [("max_in_flight", C.c_int64), ("max_device_output_bytes", C.c_int64)]
+
+
+class BatchTiming(C.Structure):
+    """kp_batch_timing (ImageBatchTiming, src/image/batch_driver.hpp:29-35)."""
+    _fields_ = [("allocate_seconds", C.c_double), ("kernel_seconds", C.c_double), ("copy_seconds", C.c_double),
+                ("windows", C.c_int64), ("peak_device_output_bytes", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+BATCH_BYTES_PER_PIXEL = 15 * 8 + 3 * 4
+
+
 class RunOptions(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("split_transport", C.c_int32),
@@ -205,6 +222,7 @@ class RunOptions(C.Structure):
         ("waves_per_cu", C.c_int32), ("guard_mode", C.c_int32),
         ("fp64_radius", C.c_double),
         ("stream", C.c_void_p),
+        ("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64),
     ]
 
 

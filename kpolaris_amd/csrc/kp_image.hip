@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,7 +24,11 @@ namespace {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Library-kept workspaces, one per (device, stream): runs on different
+// streams never share buffers, runs on one stream are ordered by it.
 struct DeviceWorkspace {
+    int device = -1;
+    void* stream = nullptr;
     void* ptr = nullptr;
     size_t bytes = 0;
 };
@@ -30,27 +36,71 @@ struct DeviceWorkspace {
 std::mutex g_ws_mutex;
 std::vector<DeviceWorkspace> g_ws;
 
-kp_status workspace_for(int device, size_t bytes, void** out) {
+kp_status workspace_for(int device, void* stream, size_t bytes, void** out) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if ((int)g_ws.size() <= device) g_ws.resize(device + 1);
-    DeviceWorkspace& w = g_ws[device];
-    if (w.bytes < bytes) {
-        if (w.ptr) {
-            if (hipDeviceSynchronize() != hipSuccess) return kp_internal::fail(KP_HIP_ERROR, "hipDeviceSynchronize failed");
-            (void)hipFree(w.ptr);
-            w.ptr = nullptr;
-            w.bytes = 0;
+    DeviceWorkspace* w = nullptr;
+    for (auto& e : g_ws)
+        if (e.device == device && e.stream == stream) w = &e;
+    if (!w) {
+        g_ws.push_back(DeviceWorkspace{device, stream, nullptr, 0});
+        w = &g_ws.back();
+    }
+    if (w->bytes < bytes) {
+        if (w->ptr) {
+            // the previous run on this stream may still use the old buffer
+            if (hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+                return kp_internal::fail(KP_HIP_ERROR, "hipStreamSynchronize failed");
+            (void)hipFree(w->ptr);
+            w->ptr = nullptr;
+            w->bytes = 0;
         }
-        if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+        if (hipMalloc(&w->ptr, bytes) != hipSuccess) {
             return kp_internal::fail(KP_OUT_OF_MEMORY, "hipMalloc of the image workspace failed");
         }
-        w.bytes = bytes;
+        w->bytes = bytes;
     }
-    *out = w.ptr;
+    *out = w->ptr;
     return KP_OK;
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Workspace layout of one image: pass-A endpoints [16][npix] doubles and
+// valid/steps/reason [npix] int32, the queue and counter block, then the
+// staged mixed-precision lists and parking buffers, the slow-light snapshot
+// tables and the time-probe outputs when the run needs them.
+struct WsLayout {
+    size_t ep, valid, steps, reason, queue, counters, lists, pkd, pki, pkst, slow_t, slow_c, req, probe, preason;
+    size_t total;
+};
+
+WsLayout workspace_layout(size_t np, size_t nst, bool staged, size_t nsnap, bool probe) {
+    WsLayout L;
+    L.ep = 0;
+    L.valid = align_up(L.ep + 16 * np * sizeof(double));
+    L.steps = align_up(L.valid + np * sizeof(int32_t));
+    L.reason = align_up(L.steps + np * sizeof(int32_t));
+    L.queue = align_up(L.reason + np * sizeof(int32_t));
+    L.counters = align_up(L.queue + KP_NQUEUE * sizeof(uint32_t));
+    L.lists = align_up(L.counters + C_COUNT * sizeof(unsigned long long));
+    L.pkd = align_up(L.lists + (staged ? 2 * np * sizeof(uint32_t) : 0));
+    L.pki = align_up(L.pkd + (staged ? 2 * np * sizeof(double) : 0));
+    L.pkst = align_up(L.pki + (staged ? 5 * np * sizeof(int32_t) : 0));
+    L.slow_t = align_up(L.pkst + (staged ? 4 * nst * np * sizeof(double) : 0));
+    L.slow_c = align_up(L.slow_t + nsnap * sizeof(double));
+    L.req = align_up(L.slow_c + nsnap * sizeof(void*));
+    L.probe = align_up(L.req + (nsnap ? np * sizeof(double) : 0));
+    L.preason = align_up(L.probe + (probe ? 2 * np * sizeof(double) : 0));
+    L.total = align_up(L.preason + (probe ? np * sizeof(int32_t) : 0));
+    return L;
+}
+
+// Staged mixed precision: guard_mode 2 on persistent split launches of the
+// single- and multi-frequency RIAF images.
+bool staged_run(const kp_run_options& opt, bool riaf) {
+    return opt.precision == KP_PRECISION_MIXED && opt.persistent && opt.split_transport != 0 &&
+           opt.guard_mode == 2 && opt.fp64_radius > 0.0 && riaf;
+}
 
 #define KP_HIP(call)                                                                   \
     do {                                                                               \
@@ -234,61 +284,54 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     // FP32-geodesic launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
     // continues them and parks those back that reach r >= back_radius, then
     // FP32 again and a final FP64 launch; no wave mixes the two paths.
-    const bool staged = mixed && P.persistent && !unsplit && opt.guard_mode == 2 && P.fp64_radius > 0.0 &&
-                        (mode == M_SINGLE || mode == M_MF);
+    const bool staged = staged_run(opt, mode == M_SINGLE || mode == M_MF);
     const int nst = mf ? P.nfreq : 1;
-    const size_t off_ep = 0;
-    const size_t off_valid = align_up(off_ep + 16 * np * sizeof(double));
-    const size_t off_steps = align_up(off_valid + np * sizeof(int32_t));
-    const size_t off_reason = align_up(off_steps + np * sizeof(int32_t));
-    const size_t off_queue = align_up(off_reason + np * sizeof(int32_t));
-    const size_t off_counters = align_up(off_queue + KP_NQUEUE * sizeof(uint32_t));
-    size_t total = align_up(off_counters + C_COUNT * sizeof(unsigned long long));
-    const size_t off_lists = total;
-    const size_t off_pkd = align_up(off_lists + (staged ? 2 * np * sizeof(uint32_t) : 0));
-    const size_t off_pki = align_up(off_pkd + (staged ? 2 * np * sizeof(double) : 0));
-    const size_t off_pkst = align_up(off_pki + (staged ? 5 * np * sizeof(int32_t) : 0));
-    const size_t off_slow_t = align_up(off_pkst + (staged ? 4 * (size_t)nst * np * sizeof(double) : 0));
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
-    const size_t off_slow_c = align_up(off_slow_t + nsnap * sizeof(double));
-    const size_t off_req = align_up(off_slow_c + nsnap * sizeof(void*));
-    const size_t off_probe = align_up(off_req + (slow ? np * sizeof(double) : 0));
-    const size_t off_preason = align_up(off_probe + (probe ? 2 * np * sizeof(double) : 0));
-    total = align_up(off_preason + (probe ? np * sizeof(int32_t) : 0));
+    const WsLayout W = workspace_layout(np, (size_t)nst, staged, nsnap, probe != nullptr);
     void* ws = nullptr;
-    st = workspace_for(device, total, &ws);
-    if (st != KP_OK) return st;
+    if (opt.workspace) {
+        if (opt.workspace_bytes < W.total) {
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "workspace of %llu bytes is smaller than the %llu bytes this run needs",
+                          (unsigned long long)opt.workspace_bytes, (unsigned long long)W.total);
+            return kp_internal::fail(KP_INVALID_ARGUMENT, msg);
+        }
+        ws = opt.workspace;
+    } else {
+        st = workspace_for(device, opt.stream, W.total, &ws);
+        if (st != KP_OK) return st;
+    }
     char* base = static_cast<char*>(ws);
-    P.ep = reinterpret_cast<double*>(base + off_ep);
-    P.ep_valid = reinterpret_cast<int32_t*>(base + off_valid);
-    P.ep_steps = reinterpret_cast<int32_t*>(base + off_steps);
-    P.ep_reason = reinterpret_cast<int32_t*>(base + off_reason);
-    P.queue = reinterpret_cast<uint32_t*>(base + off_queue);
-    P.counters = reinterpret_cast<unsigned long long*>(base + off_counters);
-    uint32_t* lists = staged ? reinterpret_cast<uint32_t*>(base + off_lists) : nullptr;
+    P.ep = reinterpret_cast<double*>(base + W.ep);
+    P.ep_valid = reinterpret_cast<int32_t*>(base + W.valid);
+    P.ep_steps = reinterpret_cast<int32_t*>(base + W.steps);
+    P.ep_reason = reinterpret_cast<int32_t*>(base + W.reason);
+    P.queue = reinterpret_cast<uint32_t*>(base + W.queue);
+    P.counters = reinterpret_cast<unsigned long long*>(base + W.counters);
+    uint32_t* lists = staged ? reinterpret_cast<uint32_t*>(base + W.lists) : nullptr;
     if (staged) {
-        P.pk_d = reinterpret_cast<double*>(base + off_pkd);
-        P.pk_i = reinterpret_cast<int32_t*>(base + off_pki);
-        P.pk_st = reinterpret_cast<double*>(base + off_pkst);
+        P.pk_d = reinterpret_cast<double*>(base + W.pkd);
+        P.pk_i = reinterpret_cast<int32_t*>(base + W.pki);
+        P.pk_st = reinterpret_cast<double*>(base + W.pkst);
         P.back_radius = KP_BACK_RADIUS_FACTOR * P.fp64_radius;
     }
 
     hipStream_t stream = static_cast<hipStream_t>(opt.stream);
     if (probe) {
-        P.probe = reinterpret_cast<double*>(base + off_probe);
+        P.probe = reinterpret_cast<double*>(base + W.probe);
         std::memset(&P.out, 0, sizeof(P.out));
-        P.out.reason = reinterpret_cast<int32_t*>(base + off_preason);
+        P.out.reason = reinterpret_cast<int32_t*>(base + W.preason);
     }
     if (slow) {
-        P.snap_t = reinterpret_cast<const double*>(base + off_slow_t);
-        P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(base + off_slow_c);
-        P.req_t = reinterpret_cast<double*>(base + off_req);
+        P.snap_t = reinterpret_cast<const double*>(base + W.slow_t);
+        P.snaps = reinterpret_cast<const kp_grmhd_cell* const*>(base + W.slow_c);
+        P.req_t = reinterpret_cast<double*>(base + W.req);
         P.nsnap = slow->nsnap;
         P.slow_mode = slow->step_mode;
         P.slow_interp = slow->interpolation;
         P.t_obs = slow->observation_time;
-        KP_HIP(hipMemcpyAsync(base + off_slow_t, slow->times, nsnap * sizeof(double), hipMemcpyHostToDevice, stream));
-        KP_HIP(hipMemcpyAsync(base + off_slow_c, slow->cells, nsnap * sizeof(void*), hipMemcpyHostToDevice, stream));
+        KP_HIP(hipMemcpyAsync(base + W.slow_t, slow->times, nsnap * sizeof(double), hipMemcpyHostToDevice, stream));
+        KP_HIP(hipMemcpyAsync(base + W.slow_c, slow->cells, nsnap * sizeof(void*), hipMemcpyHostToDevice, stream));
     }
     const int block = opt.block_size > 0 ? opt.block_size : 128;
     if (block % 64 != 0 || block > 256)
@@ -708,6 +751,136 @@ extern "C" kp_status kp_run_riaf_batch(const kp_batch_job* jobs, int32_t njobs, 
     if (e1) (void)hipEventDestroy(e1);
     kp_batch_destroy(b);
     return st;
+}
+
+extern "C" kp_status kp_image_workspace_size(int64_t npix, int32_t nfreq, int32_t nsnap,
+                                             const kp_run_options* options, uint64_t* bytes) {
+    kp_internal::clear_error();
+    if (!bytes) return kp_internal::fail(KP_INVALID_ARGUMENT, "null result");
+    if (npix < 0 || npix > (int64_t(1) << 30) || nfreq < 0 || nfreq > KP_MAX_FREQUENCIES || nsnap < 0)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "npix, nfreq or nsnap out of range");
+    kp_run_options opt;
+    kp_run_options_default(&opt);
+    if (options) opt = *options;
+    const WsLayout W = workspace_layout((size_t)npix, (size_t)std::max(nfreq, 1), staged_run(opt, nsnap == 0),
+                                        (size_t)nsnap, false);
+    *bytes = (uint64_t)W.total;
+    return KP_OK;
+}
+
+extern "C" void kp_batch_limits_default(kp_batch_limits* limits) {
+    if (!limits) return;
+    limits->max_in_flight = 4;
+    limits->max_device_output_bytes = (int64_t)256 * 1024 * 1024;
+}
+
+// The 15 real and 3 integer per-pixel output fields of an image, in the
+// order of kp_image_data.
+static double** real_field(kp_image_data& d, int i) {
+    double** f[15] = {&d.image_i, &d.image_q, &d.image_u, &d.image_v, &d.closure_x, &d.closure_k, &d.final_null,
+                      &d.frame_error, &d.det_r, &d.overlap_r11, &d.overlap_r12, &d.overlap_r21, &d.overlap_r22,
+                      &d.basis_identity_error, &d.basis_rotation_angle};
+    return f[i];
+}
+static int32_t** int_field(kp_image_data& d, int i) {
+    int32_t** f[3] = {&d.pass_a_steps, &d.steps, &d.reason};
+    return f[i];
+}
+
+extern "C" kp_status kp_run_riaf_batch_host(const kp_batch_job* jobs, int32_t njobs, const kp_batch_limits* limits,
+                                            const kp_run_options* options, kp_batch_timing* timing) {
+    using clock = std::chrono::steady_clock;
+    kp_internal::clear_error();
+    kp_batch_limits lim;
+    kp_batch_limits_default(&lim);
+    if (limits) lim = *limits;
+    if (lim.max_in_flight <= 0 || lim.max_device_output_bytes <= 0)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "batch limits must be positive");
+    if (njobs < 0 || (njobs > 0 && !jobs)) return kp_internal::fail(KP_INVALID_ARGUMENT, "invalid job list");
+    if (timing) std::memset(timing, 0, sizeof(*timing));
+    // validate every job before launching any work (batch_driver.hpp:137-146)
+    const size_t budget = (size_t)lim.max_device_output_bytes;
+    std::vector<size_t> pixels((size_t)njobs);
+    for (int j = 0; j < njobs; ++j) {
+        const int nx = jobs[j].params.camera.nx, ny = jobs[j].params.camera.ny;
+        if (nx <= 0 || ny <= 0 || nx > INT_MAX / ny)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "batch image dimensions must have a positive int-sized product");
+        pixels[j] = (size_t)nx * (size_t)ny;
+        if (pixels[j] > budget / KP_BATCH_BYTES_PER_PIXEL)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "one batch image exceeds the device output budget");
+        if (jobs[j].radiation_substeps < 1)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "batch radiation_substeps must be positive");
+    }
+    if (njobs == 0) return KP_OK;
+    kp_run_options opt;
+    kp_run_options_default(&opt);
+    if (options) opt = *options;
+    hipStream_t stream = static_cast<hipStream_t>(opt.stream);
+    for (int begin = 0; begin < njobs;) {
+        auto t0 = clock::now();
+        int count = 0;
+        size_t bytes = 0;
+        while (count < lim.max_in_flight && begin + count < njobs) {
+            const size_t next = pixels[begin + count] * KP_BATCH_BYTES_PER_PIXEL;
+            if (next > budget - bytes) break;
+            bytes += next;
+            ++count;
+        }
+        void* dmem = nullptr;
+        if (hipMalloc(&dmem, bytes) != hipSuccess)
+            return kp_internal::fail(KP_OUT_OF_MEMORY, "hipMalloc of the batch window outputs failed");
+        std::vector<kp_batch_job> w(jobs + begin, jobs + begin + count);
+        char* cur = static_cast<char*>(dmem);
+        for (int i = 0; i < count; ++i) {
+            const size_t np = pixels[begin + i];
+            kp_image_data dev;
+            std::memset(&dev, 0, sizeof(dev));
+            for (int f = 0; f < 15; ++f) {
+                *real_field(dev, f) = reinterpret_cast<double*>(cur);
+                cur += np * sizeof(double);
+            }
+            for (int f = 0; f < 3; ++f) {
+                *int_field(dev, f) = reinterpret_cast<int32_t*>(cur);
+                cur += np * sizeof(int32_t);
+            }
+            w[i].out = dev;
+        }
+        auto t1 = clock::now();
+        if (timing) {
+            timing->allocate_seconds += std::chrono::duration<double>(t1 - t0).count();
+            timing->windows += 1;
+            timing->peak_device_output_bytes = std::max<int64_t>(timing->peak_device_output_bytes, (int64_t)bytes);
+        }
+        kp_batch* b = nullptr;
+        kp_status st = kp_batch_create(w.data(), count, &opt, &b);
+        if (st == KP_OK) st = kp_batch_launch(b, stream);
+        if (st == KP_OK && hipStreamSynchronize(stream) != hipSuccess)
+            st = kp_internal::fail(KP_HIP_ERROR, "batch window did not complete");
+        auto t2 = clock::now();
+        for (int i = 0; i < count && st == KP_OK; ++i) {
+            kp_image_data host = jobs[begin + i].out;
+            const size_t np = pixels[begin + i];
+            for (int f = 0; f < 15 && st == KP_OK; ++f)
+                if (*real_field(host, f) &&
+                    hipMemcpy(*real_field(host, f), *real_field(w[i].out, f), np * sizeof(double),
+                              hipMemcpyDeviceToHost) != hipSuccess)
+                    st = kp_internal::fail(KP_HIP_ERROR, "device to host copy failed");
+            for (int f = 0; f < 3 && st == KP_OK; ++f)
+                if (*int_field(host, f) &&
+                    hipMemcpy(*int_field(host, f), *int_field(w[i].out, f), np * sizeof(int32_t),
+                              hipMemcpyDeviceToHost) != hipSuccess)
+                    st = kp_internal::fail(KP_HIP_ERROR, "device to host copy failed");
+        }
+        if (b) kp_batch_destroy(b);
+        (void)hipFree(dmem);
+        if (st != KP_OK) return st;
+        if (timing) {
+            timing->kernel_seconds += std::chrono::duration<double>(t2 - t1).count();
+            timing->copy_seconds += std::chrono::duration<double>(clock::now() - t2).count();
+        }
+        begin += count;
+    }
+    return KP_OK;
 }
 
 extern "C" int32_t kp_device_count(void) {

@@ -3,14 +3,12 @@
 #pragma once
 
 #include "../../include/kp_image.h"
+#include "kp_errors.h"
 #include "kp_riaf.h"
 
 #include <string>
 
 namespace kp_internal {
-
-void clear_error();
-kp_status fail(kp_status status, const std::string& message);
 
 // ipole camera tetrad for the BL pinhole camera at (r, theta, phi=0):
 // tetrad[0..15] = e0,e1,e2,e3 and gcov5 = (g00, g03, g11, g22, g33) there.

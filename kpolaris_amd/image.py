@@ -97,6 +97,13 @@ def library():
     L.kp_batch_destroy.restype = None
     L.kp_run_riaf_batch.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.RunOptions), P(abi.ImageStats)]
     L.kp_run_riaf_batch.restype = C.c_int32
+    L.kp_image_workspace_size.argtypes = [C.c_int64, C.c_int32, C.c_int32, P(abi.RunOptions), P(C.c_uint64)]
+    L.kp_image_workspace_size.restype = C.c_int32
+    L.kp_batch_limits_default.argtypes = [P(abi.BatchLimits)]
+    L.kp_batch_limits_default.restype = None
+    L.kp_run_riaf_batch_host.argtypes = [P(abi.BatchJob), C.c_int32, P(abi.BatchLimits), P(abi.RunOptions),
+                                         P(abi.BatchTiming)]
+    L.kp_run_riaf_batch_host.restype = C.c_int32
     L.kp_run_grmhd_slow_light_image.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
                                                 P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData),
                                                 P(abi.ImageStats)]
@@ -184,7 +191,10 @@ def pack_image(image: dict, frequency: float, qu_sign: int, scale: float):
 
 def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
                 waves_per_cu: int = 0, stream=None, fp64_radius: float | None = None,
-                split_transport: int = 1, guard_mode: int = 0) -> abi.RunOptions:
+                split_transport: int = 1, guard_mode: int = 0, workspace=None) -> abi.RunOptions:
+    """workspace: a caller-owned device tensor (any dtype) used as the run's
+    workspace (kp_run_options.workspace); None lets the library keep one per
+    stream."""
     o = abi.RunOptions()
     library().kp_run_options_default(C.byref(o))
     o.precision = precision
@@ -196,7 +206,17 @@ def run_options(precision: int = abi.PRECISION_FP64, persistent: bool = True, bl
     o.split_transport = split_transport
     o.guard_mode = guard_mode
     o.stream = stream
+    if workspace is not None:
+        o.workspace = C.c_void_p(workspace.data_ptr())
+        o.workspace_bytes = workspace.numel() * workspace.element_size()
     return o
+
+
+def workspace_size(npix: int, nfreq: int = 1, nsnap: int = 0, **run_kw) -> int:
+    """kp_image_workspace_size: device bytes of a run's workspace."""
+    b = C.c_uint64()
+    _check(library().kp_image_workspace_size(npix, nfreq, nsnap, C.byref(run_options(**run_kw)), C.byref(b)))
+    return int(b.value)
 
 
 def alloc_device_image(npix: int, device="cuda", fields=abi.IMAGE_FIELDS, nfreq: int = 1):
@@ -225,9 +245,10 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
                    precision: int = abi.PRECISION_FP64, persistent: bool = True,
                    block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
                    fields=abi.IMAGE_FIELDS, with_stats: bool = True, fp64_radius: float | None = None,
-                   split_transport: int = 1, guard_mode: int = 0):
+                   split_transport: int = 1, guard_mode: int = 0, workspace=None):
     """Run pass A + pass B on the current HIP device; returns (dict of device
-    tensors, stats dict or None).  Work is queued on torch's current stream."""
+    tensors, stats dict or None).  Work is queued on torch's current stream;
+    workspace: optional caller-owned device tensor (see run_options)."""
     import torch
     npix = params.camera.nx * params.camera.ny
     if out is None:
@@ -235,7 +256,7 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
     opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius,
-                       split_transport, guard_mode)
+                       split_transport, guard_mode, workspace)
     st = abi.ImageStats()
     _check(library().kp_run_riaf_image(C.byref(params), C.byref(model), radiation_substeps,
                                        C.byref(opts), C.byref(img),
@@ -290,6 +311,42 @@ def run_riaf_multifrequency_image(params: abi.PassAParams, model: abi.RIAFModel,
     return out, (st.as_dict() if with_stats else None)
 
 
+def run_riaf_batch_host(jobs, *, max_in_flight: int | None = None, max_device_output_bytes: int | None = None,
+                        radiation_substeps=None, precision: int = abi.PRECISION_FP64, block_size: int = 0,
+                        waves_per_cu: int = 0, fp64_radius: float | None = None, guard_mode: int = 0):
+    """run_image_batch_metric (src/image/batch_driver.hpp:126-187) with host
+    outputs: jobs run in windows bounded by max_in_flight and the device
+    output budget.  Returns (list of host image dicts, timing dict).
+    radiation_substeps: one value per job (default 1)."""
+    import torch
+    n = len(jobs)
+    arr = (abi.BatchJob * max(n, 1))()
+    outputs = []
+    for j, (p, m) in enumerate(jobs):
+        npix = max(p.camera.nx, 0) * max(p.camera.ny, 0) if p.camera.nx < 2 ** 20 and p.camera.ny < 2 ** 20 else 0
+        a = {k: np.zeros(npix, dtype=np.int32 if k in abi.IMAGE_INT_FIELDS else np.float64) for k in abi.IMAGE_FIELDS}
+        outputs.append(a)
+        arr[j].params = p
+        arr[j].model = m
+        arr[j].radiation_substeps = 1 if radiation_substeps is None else int(radiation_substeps[j])
+        img = abi.ImageData()
+        for k, v in a.items():
+            ct = C.c_int32 if k in abi.IMAGE_INT_FIELDS else C.c_double
+            setattr(img, k, v.ctypes.data_as(C.POINTER(ct)) if v.size else None)
+        arr[j].out = img
+    lim = abi.BatchLimits()
+    library().kp_batch_limits_default(C.byref(lim))
+    if max_in_flight is not None:
+        lim.max_in_flight = max_in_flight
+    if max_device_output_bytes is not None:
+        lim.max_device_output_bytes = max_device_output_bytes
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, True, block_size, waves_per_cu, C.c_void_p(stream), fp64_radius, 1, guard_mode)
+    t = abi.BatchTiming()
+    _check(library().kp_run_riaf_batch_host(arr, n, C.byref(lim), C.byref(opts), C.byref(t)))
+    return outputs, t.as_dict()
+
+
 class Batch:
     """Many RIAF images in one persistent two-pass HIP graph (kp_batch_*).
 
@@ -297,7 +354,8 @@ class Batch:
     device tensors (``self.outputs[j]``) and stay owned by this object."""
 
     def __init__(self, jobs, *, precision: int = abi.PRECISION_FP64, block_size: int = 0,
-                 waves_per_cu: int = 0, fp64_radius: float | None = None, fields=abi.IMAGE_FIELDS):
+                 waves_per_cu: int = 0, fp64_radius: float | None = None, fields=abi.IMAGE_FIELDS,
+                 guard_mode: int = 0):
         self._h = None
         n = len(jobs)
         arr = (abi.BatchJob * n)()
@@ -309,7 +367,7 @@ class Batch:
             arr[j].model = m
             arr[j].radiation_substeps = 1
             arr[j].out = image_struct_from_tensors(out)
-        opts = run_options(precision, True, block_size, waves_per_cu, None, fp64_radius)
+        opts = run_options(precision, True, block_size, waves_per_cu, None, fp64_radius, 1, guard_mode)
         h = C.c_void_p()
         _check(library().kp_batch_create(arr, n, C.byref(opts), C.byref(h)))
         self._h = h

@@ -11,10 +11,10 @@ oracle is re-run with the camera offset moved by +-1e-15 in x and in y (four
 runs; about 1e-13 of the ray slope), and the per-pixel spread of those runs
 around the unperturbed oracle is the pixel's measured sensitivity.
 
-* A pixel is ill-conditioned when some Stokes component moves by more than
-  1e-11 of its value (floor 1e-6 of the frequency's peak) or its step counts
-  change in one of the perturbed runs.  Each test bounds the fraction of
-  ill-conditioned pixels it accepts and reports it.
+* A pixel is ill-conditioned when its step counts or reason change in one of
+  the perturbed runs, or AMPLIFICATION times the spread of some Stokes
+  component exceeds that component's tight bar.  Each test bounds the
+  fraction of ill-conditioned pixels it accepts and reports it.
 * Every pixel must meet |x - oracle| <= 1e-10 |oracle| + 1e-8 peak
   + AMPLIFICATION x spread: the bar widens by a fixed multiple of the
   oracle's own measured sensitivity, not to a fixed loose tolerance.  A pixel
@@ -41,23 +41,19 @@ SENSITIVITY = 1e-11
 PERTURBATION = 1e-15
 SENS_FLOOR = 1e-6        # floor (of the peak) of the ill-conditioning test
 AMPLIFICATION = 10.0     # the bar's multiple of the oracle's measured spread
-# default bound on the ill-conditioned share of an image: the test images are
-# coarse samplings of fields of view dominated by the photon ring, where 15-30 %
-# of the rays wind around the hole
-MAX_ILL = 0.35
-# bound for the synthetic GRMHD tori, whose cold dense gas reaches Faraday
-# depths of 1e5-1e7 rad: the oracle's own 1e-13 path change moves the
-# polarisation angle, hence Q and U, by 1e-9 to 1e-6 on most rays (I stays
-# well-conditioned there; each component is held to its own spread)
-FARADAY_MAX_ILL = 0.65
+# bound on the ill-conditioned share of an image (measured on the GPU test
+# images: at most 5 %, on ragged images of a few dozen photon-ring pixels)
+MAX_ILL = 0.1
 # share of pixels on which a Stokes component's derivative must be checkable
 MIN_CHECKED = 0.35
 STOKES = ("image_i", "image_q", "image_u", "image_v")
 OFFSETS = ((PERTURBATION, 0.0), (-PERTURBATION, 0.0), (0.0, PERTURBATION), (0.0, -PERTURBATION))
 
 
-def conditioning(run, p, ref):
-    """Perturbed oracle runs of an image: run(params) -> image dict.  Returns
+def conditioning(run, p, ref, extra=()):
+    """Perturbed oracle runs of an image: run(params) -> image dict, plus the
+    zero-argument runners in `extra` (other rounding-level perturbations of
+    the same image, e.g. time_shifts).  Returns
     {"spread": {name: max |r_k - ref| per element}, "dsteps": the largest
     change of the pass-B step count, "moved": pass-A steps or reason changed
     in some run (per pixel), "samples": the perturbed images}."""
@@ -65,11 +61,16 @@ def conditioning(run, p, ref):
     moved = np.zeros(ref["reason"].shape, dtype=bool)
     dsteps = np.zeros(ref["reason"].shape, dtype=np.int64)
     samples = []
-    for dx, dy in OFFSETS:
-        q = copy.deepcopy(p)
-        q.camera.x_offset += dx
-        q.camera.y_offset += dy
-        r2 = run(q)
+    def runs():
+        for dx, dy in OFFSETS:
+            q = copy.deepcopy(p)
+            q.camera.x_offset += dx
+            q.camera.y_offset += dy
+            yield run(q)
+        for e in extra:
+            yield e()
+
+    for r2 in runs():
         samples.append({k: r2[k] for k in (*STOKES, "steps", "pass_a_steps", "reason")})
         for n in STOKES:
             spread[n] = np.maximum(spread[n], np.abs(r2[n] - ref[n]))
@@ -90,9 +91,18 @@ def oracle_with_sensitivity(p, m, offset=None, stride=None, count=None):
     return ref
 
 
-def sensitivity(run, p, ref):
+def sensitivity(run, p, ref, extra=()):
     """Conditioning of an oracle image (any model) from a runner."""
-    return conditioning(run, p, ref)
+    return conditioning(run, p, ref, extra)
+
+
+def time_shifts(run_at, t_obs, rel=1e-13):
+    """Slow light: the image at observation times t_obs (1 +- rel).  Where a
+    ray meets a snapshot time, the reference cuts its step to land on it and
+    stops once the remaining step is below 1e-14, so the number and size of
+    those cut steps depend on the last bits of x^0; shifting t_obs by 1e-13
+    exposes the rays whose result depends on that."""
+    return [lambda: run_at(t_obs * (1.0 + rel)), lambda: run_at(t_obs * (1.0 - rel))]
 
 
 def _peaks(ref, nfreq):
@@ -100,32 +110,51 @@ def _peaks(ref, nfreq):
     return [float(np.max(np.abs(ref["image_i"][f * n:(f + 1) * n]))) if n else 0.0 for f in range(nfreq)]
 
 
-def ill_conditioned(ref, cond, nfreq=1, time_cut_steps=0):
-    """Per pixel: the oracle's own perturbation moves it beyond SENSITIVITY
-    of a Stokes value (floor SENS_FLOOR of the peak) or changes its step
-    counts (beyond the time-cut step jitter that slow light allows)."""
+def ill_conditioned(ref, cond, nfreq=1, time_cut_steps=0, peaks=None):
+    """Per pixel: the oracle's own perturbation changes its step counts
+    (beyond the time-cut jitter slow light allows) or its reason, or moves a
+    Stokes component by more than the tight bar / AMPLIFICATION, i.e. the
+    conditioned bar of that component is wider than the tight one."""
     n = ref["reason"].size
     ill = cond["moved"] | (cond["dsteps"] > time_cut_steps)
-    for f, peak in enumerate(_peaks(ref, nfreq)):
+    for f, peak in enumerate(peaks if peaks is not None else _peaks(ref, nfreq)):
         sl = slice(f * n, (f + 1) * n)
         for name in STOKES:
-            ill |= cond["spread"][name][sl] > SENSITIVITY * (np.abs(ref[name][sl]) + SENS_FLOOR * peak)
+            tight = REL * np.abs(ref[name][sl]) + PEAK_FLOOR * peak
+            ill |= AMPLIFICATION * cond["spread"][name][sl] > tight
     return ill
 
 
-def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, diagnostics=True):
+def smooth_for_derivatives(ref, cond, time_cut_steps=0):
+    """Single-frequency pixels on which a derivative can be compared with
+    difference quotients of the oracle: no step or reason change under the
+    oracle's own perturbations, and no Stokes component moved by more than
+    SENSITIVITY of its value (floor SENS_FLOOR of the peak), which excludes
+    the rays whose transfer amplifies perturbations (large Faraday depth,
+    photon-ring winding)."""
+    peak = float(np.max(np.abs(ref["image_i"]))) if ref["image_i"].size else 0.0
+    ok = ~cond["moved"] & (cond["dsteps"] <= time_cut_steps)
+    for name in STOKES:
+        ok &= cond["spread"][name] <= SENSITIVITY * (np.abs(ref[name]) + SENS_FLOOR * peak)
+    return ok
+
+
+def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, diagnostics=True, peak=None):
     """The parity rule of this module for a [nfreq][npix]-Stokes image.
-    Returns a summary (printed by the tests): pixels, ill-conditioned count,
-    pixels that needed the widened bar, the worst error over the tight bar."""
+    peak: the image's peak I for a sample of its pixels (default: the
+    sample's own, per frequency).  Returns a summary (printed by the tests):
+    pixels, ill-conditioned count, pixels that needed the widened bar, the
+    worst error over the tight bar."""
     n = ref["reason"].size
     assert np.array_equal(got["reason"], ref["reason"]), (
         f"{label}: reasons differ at {np.flatnonzero(got['reason'] != ref['reason'])[:10]}")
-    ill = ill_conditioned(ref, cond, nfreq, time_cut_steps)
+    peaks = _peaks(ref, nfreq) if peak is None else [peak] * nfreq
+    ill = ill_conditioned(ref, cond, nfreq, time_cut_steps, peaks)
     frac = float(ill.mean()) if n else 0.0
     assert frac <= max_ill, f"{label}: {int(ill.sum())}/{n} ill-conditioned pixels exceed the bound {max_ill}"
     widened = np.zeros(n, dtype=bool)
     worst_tight = 0.0
-    for f, peak in enumerate(_peaks(ref, nfreq)):
+    for f, peak in enumerate(peaks):
         sl = slice(f * n, (f + 1) * n)
         for name in STOKES:
             g, r = got[name][sl], ref[name][sl]
@@ -167,9 +196,9 @@ def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, dia
 
 
 def assert_parity(got, ref, label, peak=None, diagnostics=True, max_ill=MAX_ILL):
-    """Single-frequency RIAF image against oracle_with_sensitivity's result."""
-    del peak  # the peak is the oracle image's own
-    return check(got, ref, ref["_cond"], label, 1, max_ill=max_ill, diagnostics=diagnostics)
+    """Single-frequency RIAF image against oracle_with_sensitivity's result;
+    peak: the full image's peak I when got/ref are a sample of its pixels."""
+    return check(got, ref, ref["_cond"], label, 1, max_ill=max_ill, diagnostics=diagnostics, peak=peak)
 
 
 def assert_parity_grid(got, ref, nfreq, label, cond, max_ill=MAX_ILL, time_cut_steps=0):

@@ -109,3 +109,11 @@ def test_batch_job_draws_are_seeded_and_in_range():
     assert p.camera.nx == a[0]["nx"] and m.freq_cgs == a[0]["freq"]
     rh = 1.0 + np.sqrt(1.0 - a[0]["spin"] ** 2)
     assert p.inner_radius == pytest.approx(1.05 * rh)
+
+
+def test_plain_c_caller_compiles_and_links():
+    """A C99 program that includes only include/kp_image.h builds and links
+    against the library (the ABI is plain C)."""
+    from tests.c_caller import build as cb
+    exe = cb.build()
+    assert exe.exists()

@@ -53,10 +53,12 @@ def test_gpu_gradient_against_oracle_central_differences():
     ref = parity.oracle_with_sensitivity(p, m)
     print(parity.assert_parity(got, ref, "gradient value"), flush=True)
     g = grad.cpu().numpy()
-    ill = parity.ill_conditioned(ref, ref["_cond"])
+    smooth = parity.smooth_for_derivatives(ref, ref["_cond"])
     for k in range(3):
         fd, same = _fd_oracle(p, m, k + 1)
-        ok = same & ~ill
+        ok = same & smooth
+        print(f"gradient check on {int(ok.sum())}/{ok.size} pixels", flush=True)
+        assert ok.sum() >= parity.MIN_CHECKED * ok.size
         for s, name in enumerate(parity.STOKES):
             scale = np.abs(fd[name]).max()
             err = np.abs(g[k, s][ok] - fd[name][ok]).max() / scale

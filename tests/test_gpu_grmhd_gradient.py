@@ -63,8 +63,7 @@ def test_gpu_grmhd_gradient_value_and_oracle_differences(snap):
     g = grad.cpu().numpy()
     ref, _ = oracle.run_grmhd_image(p, m, host)
     cond = parity.sensitivity(lambda q: oracle.run_grmhd_image(q, m, host)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 1, "gpu static GRMHD tangent value", cond,
-                                    max_ill=parity.FARADAY_MAX_ILL), flush=True)
+    print(parity.assert_parity_grid(got, ref, 1, "gpu static GRMHD tangent value", cond), flush=True)
     # the derivatives are compared on the pixels that the oracle's own
     # sensitivity leaves well-conditioned (every component) and whose step
     # counts equal the oracle's
@@ -73,7 +72,7 @@ def test_gpu_grmhd_gradient_value_and_oracle_differences(snap):
         fd, same = _fd(p, m, host, kind, 1e-4)
         fd2, same2 = _fd(p, m, host, kind, 5e-5)
         for s, name in enumerate(STOKES):
-            ok = same & same2 & same_steps & ~parity.ill_conditioned(ref, cond, 1, 0)
+            ok = same & same2 & same_steps & parity.smooth_for_derivatives(ref, cond, 0)
             print(f"gradient check of {name} on {int(ok.sum())}/{ok.size} pixels", flush=True)
             assert ok.sum() >= parity.MIN_CHECKED * ok.size
             scale = np.abs(fd[name][ok]).max()

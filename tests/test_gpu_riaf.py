@@ -131,9 +131,11 @@ def test_gpu_full_size_c2_spot_parity_and_invariants():
     assert np.all(unclosed[over]), "|P| > I only on rays that did not return to the camera"
     worst = over[np.argsort(-(lp[over] / np.maximum(gpu["image_i"][over], 1e-300)))][:4]
     for pix in worst:
+        # chosen because they wind near the photon ring: ill-conditioned by
+        # construction, so only the conditioned bar bounds them
         ref1 = parity.oracle_with_sensitivity(p, m, int(pix), 1, 1)
         parity.assert_parity({k: v[pix:pix + 1] for k, v in gpu.items()}, ref1, f"pixel{pix}",
-                             peak=peak, diagnostics=False)
+                             peak=peak, diagnostics=False, max_ill=1.0)
     closed = ok & ~unclosed
     assert closed.mean() > 0.99
     # the transported screen basis keeps its orientation (det R ~ 1 up to the

@@ -50,10 +50,10 @@ def test_gpu_slow_light_parity(seq, mode, interp):
     ref, ost, w = oracle.run_grmhd_slow_light_image(p, model, host, times, T_OBS, step_mode=mode,
                                                     interpolation=interp)
     assert w[0] == 0
-    cond = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
-        q, model, host, times, T_OBS, step_mode=mode, interpolation=interp)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 1, f"gpu slow light {mode}/{interp}", cond, time_cut_steps=1,
-                                    max_ill=parity.FARADAY_MAX_ILL),
+    run = lambda q, t=T_OBS: oracle.run_grmhd_slow_light_image(  # noqa: E731
+        q, model, host, times, t, step_mode=mode, interpolation=interp)[0]
+    cond = parity.sensitivity(run, p, ref, parity.time_shifts(lambda t: run(p, t), T_OBS))
+    print(parity.assert_parity_grid(got, ref, 1, f"gpu slow light {mode}/{interp}", cond, time_cut_steps=1),
           st["pass_b_ms"], flush=True)
     assert np.max(ref["image_i"]) > 0.0
 
@@ -67,10 +67,9 @@ def test_gpu_slow_light_launch_modes_and_frequencies(seq, persistent, split):
     print(f"case: slow light 2 freqs persistent={persistent} split={split}", flush=True)
     got, _ = _gpu(grids, times, p, T_OBS, frequencies=F, persistent=persistent, split_transport=split)
     ref, _, w = oracle.run_grmhd_slow_light_image(p, model, host, times, T_OBS, frequencies=F)
-    cond = parity.sensitivity(lambda q: oracle.run_grmhd_slow_light_image(
-        q, model, host, times, T_OBS, frequencies=F)[0], p, ref)
-    print(parity.assert_parity_grid(got, ref, 2, "gpu slow light 2 freqs", cond, time_cut_steps=1,
-                                    max_ill=parity.FARADAY_MAX_ILL), flush=True)
+    run = lambda q, t=T_OBS: oracle.run_grmhd_slow_light_image(q, model, host, times, t, frequencies=F)[0]  # noqa: E731
+    cond = parity.sensitivity(run, p, ref, parity.time_shifts(lambda t: run(p, t), T_OBS))
+    print(parity.assert_parity_grid(got, ref, 2, "gpu slow light 2 freqs", cond, time_cut_steps=1), flush=True)
 
 
 def test_gpu_slow_light_time_support_error(seq):
