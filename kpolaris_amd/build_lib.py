@@ -103,7 +103,28 @@ def build(verbose: bool = False, resource_usage: bool = False, defines: dict | N
         Path(o).unlink(missing_ok=True)
     if lib == LIB:
         build_opcount(verbose)
+        build_io(verbose)
     return lib
+
+
+IO_LIB = OUT_DIR / "libkpolaris_io.so"
+HDF5_PREFIX = Path(os.environ.get("KP_HDF5_PREFIX", "/opt/conda"))
+
+
+def build_io(verbose: bool = False) -> Path | None:
+    """HDF5 input/output library (kp_io.cpp, host only, HDF5 C API), linked
+    against the imaging library.  Skipped with a message when HDF5's headers
+    are absent."""
+    if not (HDF5_PREFIX / "include" / "hdf5.h").exists():
+        print(f"kpolaris_amd: no HDF5 under {HDF5_PREFIX}; libkpolaris_io.so not built", flush=True)
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{HDF5_PREFIX / 'include'}",
+           str(CSRC / "kp_io.cpp"), "-o", str(IO_LIB), f"-L{HDF5_PREFIX / 'lib'}", "-lhdf5",
+           f"-Wl,-rpath,{HDF5_PREFIX / 'lib'}", f"-L{OUT_DIR}", "-lkpolaris_amd", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return IO_LIB
 
 
 def build_opcount(verbose: bool = False) -> Path:
