@@ -175,20 +175,24 @@ KP_DEV void thermal_coeffs_scaled(const RiafMixed& R, float freq, float bconst, 
     float jQh = jQ * jscale;
     float jVh = jV * jscale;
     const float jps = fmaxf(fabsf(jQh), fabsf(jVh));
-    const float ijps = krcp(jps);
-    const float jP = jps > 0.0f ? jps * sqrtf(sq(jQh * ijps) + sq(jVh * ijps)) : 0.0f;
+    // a subnormal jps has no finite reciprocal: the polarised emissivity has
+    // underflowed and is left uncapped
+    const bool jps_normal = jps >= 1.17549435e-38f;
+    const float ijps = jps_normal ? krcp(jps) : 0.0f;
+    const float jP = jps_normal ? jps * sqrtf(sq(jQh * ijps) + sq(jVh * ijps)) : 0.0f;
     if (jIh > 0.0f && jP > 0.0f && jIh < kdiv(jP, fmaxf(M.max_pol_frac, 1e-30f))) {
         const float scale = kdiv(jIh, jP) * M.max_pol_frac;
         jQh *= scale;
         jVh *= scale;
     }
-    // Kirchhoff: a_h = j_h / S'.  S' = 0 (expm1 overflowed) leaves a_h = 0, as
-    // the double-precision path does once B_nu underflows.
+    // Kirchhoff: a_h = j_h / S'.  S' below FLT_MIN (expm1 overflowed, or a
+    // subnormal S' whose reciprocal is infinite) leaves a_h = 0, as the
+    // double-precision path does once B_nu underflows.
     float aI = 0.0f, aQ = 0.0f, aV = 0.0f;
     {
         const float x = kdiv(R.x0 * nu, thetae);
         const float sp = kdiv(bconst, expm1f(x));
-        if (sp > 0.0f) {
+        if (sp >= 1.17549435e-38f) {
             const float isp = krcp(sp);
             aI = jIh * isp;
             aQ = jQh * isp;

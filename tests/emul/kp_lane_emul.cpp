@@ -202,3 +202,20 @@ extern "C" kp_status kp_emul_run_grmhd_slow_light_gradient(const kp_pass_a_param
     return emul_run<false, M_STAN>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, log_scales, grad, &g, seq, req_t,
                                    nullptr);
 }
+
+// The FP32 rescaled thermal coefficients of the mixed-precision lanes at one
+// plasma state (model constants from kp_riaf_model m); out: jI jQ jU jV aI aQ
+// aU aV rQ rU rV.
+extern "C" kp_status kp_emul_thermal_coeffs_scaled(const kp_pass_a_params* pa, const kp_riaf_model* m, float nu,
+                                                    float g, float ne, float thetae, float b, float sin_theta,
+                                                    float cos_theta, float* out) {
+    Params P;
+    const kp_status st = prepare(pa, m, P);
+    if (st != KP_OK) return st;
+    prepare_mixed(P);
+    Coeffs<float> c;
+    thermal_coeffs_scaled(P.rm, nu, P.rm.bconst, g, ne, thetae, b, sin_theta, cos_theta, c);
+    const float v[11] = {c.jI, c.jQ, c.jU, c.jV, c.aI, c.aQ, c.aU, c.aV, c.rQ, c.rU, c.rV};
+    for (int i = 0; i < 11; ++i) out[i] = v[i];
+    return KP_OK;
+}

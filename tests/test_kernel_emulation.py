@@ -231,3 +231,29 @@ def test_emulated_tangent_matches_oracle_central_differences(emul):
         for s, name in enumerate(parity.STOKES):
             scale = np.abs(fd[name]).max()
             np.testing.assert_allclose(g[k, s], fd[name], rtol=0, atol=1e-6 * scale, err_msg=f"q{k} {name}")
+
+
+def test_emulated_mixed_coefficients_stay_finite_at_underflow(emul):
+    """The FP32 rescaled source function S' = bconst / expm1(x) of the mixed
+    lanes is subnormal for x between ln(bconst / FLT_MIN) and 88.7 (a very
+    cold sample below the default temperature floor): the absorption and the
+    polarisation cap must stay finite there (S' is treated as underflowed, as
+    the FP64 path treats B_nu), not multiply zero emissivity by an infinite
+    reciprocal."""
+    import math
+    L = emul
+    L.kp_emul_thermal_coeffs_scaled.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel)] + \
+        [C.c_float] * 7 + [C.POINTER(C.c_float)]
+    L.kp_emul_thermal_coeffs_scaled.restype = C.c_int32
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=4, ny=4)
+    nu = m.freq_cgs
+    h, me, c = 6.62607015e-27, 9.1093837015e-28, 2.99792458e10
+    bconst = 2 * h * nu ** 3 / c ** 2
+    for x in (62.0, 70.0, 80.0, 88.0, 95.0):
+        assert x > math.log(bconst / 1.17549435e-38) or x > 88.0
+        thetae = h * nu / (me * c * c * x)
+        out = (C.c_float * 11)()
+        assert L.kp_emul_thermal_coeffs_scaled(C.byref(p), C.byref(m), nu, 1.0, 1e6, thetae, 10.0, 0.8, 0.6,
+                                               out) == 0
+        vals = list(out)
+        assert all(math.isfinite(v) for v in vals), (x, vals)
