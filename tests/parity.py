@@ -141,14 +141,17 @@ def smooth_for_derivatives(ref, cond, time_cut_steps=0):
 
 def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, diagnostics=True, peak=None):
     """The parity rule of this module for a [nfreq][npix]-Stokes image.
-    peak: the image's peak I for a sample of its pixels (default: the
-    sample's own, per frequency).  Returns a summary (printed by the tests):
+    peak: the image's peak I for a sample of its pixels, one value or one
+    per frequency (default: the sample's own, per frequency).  Returns a summary (printed by the tests):
     pixels, ill-conditioned count, pixels that needed the widened bar, the
     worst error over the tight bar."""
     n = ref["reason"].size
     assert np.array_equal(got["reason"], ref["reason"]), (
         f"{label}: reasons differ at {np.flatnonzero(got['reason'] != ref['reason'])[:10]}")
-    peaks = _peaks(ref, nfreq) if peak is None else [peak] * nfreq
+    if peak is None:
+        peaks = _peaks(ref, nfreq)
+    else:
+        peaks = list(peak) if np.ndim(peak) else [float(peak)] * nfreq
     ill = ill_conditioned(ref, cond, nfreq, time_cut_steps, peaks)
     frac = float(ill.mean()) if n else 0.0
     assert frac <= max_ill, f"{label}: {int(ill.sum())}/{n} ill-conditioned pixels exceed the bound {max_ill}"
