@@ -337,6 +337,28 @@ kp_status kp_run_riaf_image(const kp_pass_a_params* params,
                             kp_image_data* out,
                             kp_image_stats* stats);
 
+/* Allocation-free variant of kp_run_riaf_image.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The pass-A endpoints and queue counters of a single-frequency RIAF image of
+ * at most KP_NOALLOC_MAX_PIXELS pixels live in a static device array of the
+ * library (140 bytes per pixel, about 37 MB, held on every GPU the library
+ * runs on until the process ends); the caller passes inputs and device
+ * outputs only.  Every call on one GPU shares that array and it keeps the
+ * last call's contents (the queue and counters are cleared by a kernel at the
+ * start of each call): calls that overlap on one GPU (on two streams, or
+ * from two threads) race and corrupt each other; calls ordered on one stream
+ * do not.  Larger images, staged mixed precision (guard_mode 2) and a caller
+ * workspace are rejected (KP_INVALID_ARGUMENT / KP_UNSUPPORTED).  No stats.
+ * Stream-ordered on options->stream; the results equal kp_run_riaf_image's
+ * bit for bit (same kernels). */
+#define KP_NOALLOC_MAX_PIXELS (512 * 512)
+kp_status kp_run_riaf_image_noalloc(const kp_pass_a_params* params,
+                                    const kp_riaf_model* model,
+                                    int32_t radiation_substeps,
+                                    const kp_run_options* options,
+                                    kp_image_data* out);
+
 /* Same computation with HOST output pointers; synchronous. */
 kp_status kp_run_riaf_image_host(const kp_pass_a_params* params,
                                  const kp_riaf_model* model,

@@ -213,6 +213,7 @@ class BatchTiming(C.Structure):
 
 
 BATCH_BYTES_PER_PIXEL = 15 * 8 + 3 * 4
+NOALLOC_MAX_PIXELS = 512 * 512  # KP_NOALLOC_MAX_PIXELS
 
 
 class RunOptions(C.Structure):

@@ -63,7 +63,7 @@ kp_status workspace_for(int device, void* stream, size_t bytes, void** out) {
     return KP_OK;
 }
 
-size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Workspace layout of one image: pass-A endpoints [16][npix] doubles and
 // valid/steps/reason [npix] int32, the queue and counter block, then the
@@ -74,8 +74,8 @@ struct WsLayout {
     size_t total;
 };
 
-WsLayout workspace_layout(size_t np, size_t nst, bool staged, size_t nsnap, bool probe) {
-    WsLayout L;
+constexpr WsLayout workspace_layout(size_t np, size_t nst, bool staged, size_t nsnap, bool probe) {
+    WsLayout L{};
     L.ep = 0;
     L.valid = align_up(L.ep + 16 * np * sizeof(double));
     L.steps = align_up(L.valid + np * sizeof(int32_t));
@@ -93,6 +93,37 @@ WsLayout workspace_layout(size_t np, size_t nst, bool staged, size_t nsnap, bool
     L.preason = align_up(L.probe + (probe ? 2 * np * sizeof(double) : 0));
     L.total = align_up(L.preason + (probe ? np * sizeof(int32_t) : 0));
     return L;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free images (kp_run_riaf_image_noalloc): the workspace of a
+// single-frequency image of at most KP_NOALLOC_MAX_PIXELS pixels lives in this
+// static device array, one copy per GPU, held until the process ends.  Host
+// code never uses the array's name as an address: the address of each
+// device's copy comes from hipGetSymbolAddress (scratch_for).
+// ---------------------------------------------------------------------------
+constexpr size_t KP_NOALLOC_BYTES = workspace_layout(KP_NOALLOC_MAX_PIXELS, 1, false, 0, false).total;
+static __device__ __align__(256) unsigned char scratch[KP_NOALLOC_BYTES];
+
+constexpr int KP_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch[KP_MAX_DEVICES] = {};
+
+// The address of the current device's copy of `scratch` (device `device`),
+// looked up once per device.
+kp_status scratch_for(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= KP_MAX_DEVICES)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "device index beyond the allocation-free table");
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return kp_internal::fail(KP_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = ptr;
+    }
+    *out = g_scratch[device];
+    return KP_OK;
 }
 
 // Staged mixed precision: guard_mode 2 on persistent split launches of the
@@ -194,7 +225,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
                            const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
                            int perturbation, double log_scale, const Tangent* tangent = nullptr,
                            const kp_grmhd_model* grmhd = nullptr, const kp_slow_light* slow = nullptr,
-                           kp_slow_light_probe* probe = nullptr) {
+                           kp_slow_light_probe* probe = nullptr, bool noalloc = false) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -289,7 +320,21 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
     const WsLayout W = workspace_layout(np, (size_t)nst, staged, nsnap, probe != nullptr);
     void* ws = nullptr;
-    if (opt.workspace) {
+    if (noalloc) {
+        // the static scratch array: no staged lists, no timing events
+        if (staged)
+            return kp_internal::fail(KP_UNSUPPORTED, "the allocation-free entry point has no staged (guard_mode 2) path");
+        if (stats || grmhd || slow || probe || tangent || fl.nfreq > 1)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "the allocation-free entry point images one RIAF frequency");
+        if (np > KP_NOALLOC_MAX_PIXELS || W.total > KP_NOALLOC_BYTES) {
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "image of %llu pixels exceeds the allocation-free maximum of %d pixels",
+                          (unsigned long long)np, (int)KP_NOALLOC_MAX_PIXELS);
+            return kp_internal::fail(KP_INVALID_ARGUMENT, msg);
+        }
+        st = scratch_for(device, &ws);
+        if (st != KP_OK) return st;
+    } else if (opt.workspace) {
         if (opt.workspace_bytes < W.total) {
             char msg[160];
             std::snprintf(msg, sizeof(msg), "workspace of %llu bytes is smaller than the %llu bytes this run needs",
@@ -894,6 +939,16 @@ extern "C" kp_status kp_run_riaf_image(const kp_pass_a_params* params, const kp_
                                        kp_image_data* out, kp_image_stats* stats) {
     (void)radiation_substeps;
     return run_image(params, model, FreqLists{}, options, out, stats, 0, 0.0);
+}
+
+extern "C" kp_status kp_run_riaf_image_noalloc(const kp_pass_a_params* params, const kp_riaf_model* model,
+                                               int32_t radiation_substeps, const kp_run_options* options,
+                                               kp_image_data* out) {
+    (void)radiation_substeps;
+    if (options && options->workspace)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "the allocation-free entry point uses its own scratch array");
+    return run_image(params, model, FreqLists{}, options, out, nullptr, 0, 0.0, nullptr, nullptr, nullptr, nullptr,
+                     true);
 }
 
 extern "C" kp_status kp_run_riaf_multifrequency_image(const kp_pass_a_params* params, const kp_riaf_model* model,

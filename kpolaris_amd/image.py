@@ -54,6 +54,9 @@ def library():
         fn.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(abi.RunOptions),
                        P(abi.ImageData), P(abi.ImageStats)]
         fn.restype = C.c_int32
+    L.kp_run_riaf_image_noalloc.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(abi.RunOptions),
+                                            P(abi.ImageData)]
+    L.kp_run_riaf_image_noalloc.restype = C.c_int32
     for fn in (L.kp_run_riaf_multifrequency_image, L.kp_run_riaf_multifrequency_image_host):
         fn.argtypes = [P(abi.PassAParams), P(abi.RIAFModel), C.c_int32, P(C.c_double), C.c_int32,
                        P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData), P(abi.ImageStats)]
@@ -262,6 +265,22 @@ def run_riaf_image(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_s
                                        C.byref(opts), C.byref(img),
                                        C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)
+
+
+def run_riaf_image_noalloc(params: abi.PassAParams, model: abi.RIAFModel, out: dict, *,
+                           precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
+                           fp64_radius: float | None = None, split_transport: int = 1, guard_mode: int = 0):
+    """kp_run_riaf_image_noalloc into the caller's device tensors `out` (see
+    alloc_device_image) on torch's current stream; the library allocates
+    nothing (its workspace is a static device array of at most
+    abi.NOALLOC_MAX_PIXELS pixels)."""
+    import torch
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, persistent, block_size, 0, C.c_void_p(stream), fp64_radius, split_transport,
+                       guard_mode)
+    _check(library().kp_run_riaf_image_noalloc(C.byref(params), C.byref(model), 1, C.byref(opts), C.byref(img)))
+    return out
 
 
 def run_riaf_image_host(params: abi.PassAParams, model: abi.RIAFModel, *, radiation_substeps: int = 1,
