@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define KP_ABI_VERSION 3
+#define KP_ABI_VERSION 4
 
 /* Largest frequency list of one shared-geodesic image (the reference's
  * KPOLARIS_MAX_FREQUENCIES build option, CMakeLists.txt:14). */
@@ -507,8 +507,9 @@ kp_status kp_run_grmhd_image(const kp_pass_a_params* params, const kp_grmhd_mode
  * retarded time t = observation_time + x^0 (x^0 = 0 at the camera,
  * docs/slow_light.md:9-15).  Mirrors the arguments of
  * run_slow_light_image_metric (src/image/slow_light_driver.hpp:1940-1954) and
- * the residency options of run_slow_light_batch (slow_light_batch_driver.hpp),
- * with every snapshot resident in device memory. */
+ * the residency options of run_slow_light_batch (slow_light_batch_driver.hpp:
+ * 13-24): pass B runs over blocks of windows_per_block snapshot pairs, as the
+ * reference's block scheduler does, or over the whole sequence at once. */
 typedef enum kp_slow_light_step_mode {
     /* kpolaris::SlowLightStepMode (src/model/time_interpolation.hpp:9-12) */
     KP_SLOW_LIGHT_DECOUPLED = 0,  /* steps never cut at snapshot times; the default */
@@ -526,7 +527,13 @@ typedef struct kp_slow_light {
     int32_t nsnap;                /* snapshots, at least 2 */
     int32_t step_mode;            /* kp_slow_light_step_mode */
     int32_t interpolation;        /* kp_slow_light_interpolation */
-    int32_t reserved0;
+    int32_t windows_per_block;    /* SlowLightBatchOptions::windows_per_block
+                                   * (slow_light_batch_driver.hpp:20): snapshot
+                                   * pairs per pass-B block, 0 = one block over
+                                   * the whole sequence.  Block stepping cuts
+                                   * steps at each block's last time, so its
+                                   * image depends on this value; decoupled and
+                                   * snapshot images do not. */
     double observation_time;      /* coordinate arrival time at the camera */
     const double* times;          /* [nsnap] strictly increasing, HOST */
     const void* const* cells;     /* [nsnap] DEVICE kp_grmhd_cell arrays (HOST array of pointers) */
@@ -546,6 +553,64 @@ kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp
                                         int32_t nfreq, const kp_run_options* options, kp_image_data* out,
                                         kp_image_stats* stats);
 void kp_slow_light_default(kp_slow_light* sequence);
+
+/* Streamed slow light: a sequence read snapshot by snapshot from host storage
+ * instead of resident in device memory, for sequences larger than the device
+ * holds.  Replaces run_slow_light_batch_metric_impl's block loop for one
+ * observation time (src/image/slow_light_batch_driver.hpp:105-340) with its
+ * host read-ahead (SlowLightHostStream, src/image/slow_light_prefetch.hpp:
+ * 15-88): a library thread calls `load` for snapshots in time order, up to
+ * prefetch_snapshots ahead, into pinned host buffers; a copy stream uploads
+ * each into a device ring of 2 windows_per_block + halo + 1 snapshot slots
+ * (halo = 4 for decoupled stepping, decoupled_cache_halo_snapshots,
+ * src/model/time_interpolation.hpp:11; 0 otherwise) while pass B runs the
+ * current block.  A decoupled ray that needs an earlier time than its block
+ * holds makes the scheduler rewind to that snapshot (:317-330).  The image
+ * equals kp_run_grmhd_slow_light_image's over the same snapshots with the
+ * same windows_per_block.
+ *
+ * `load` fills n1*n2*n3 kp_grmhd_cell records of snapshot `index` at
+ * `cells` (host memory); with derive_scalars = 1 it fills the primitives
+ * only and the library computes the derived scalars on the device
+ * (kp_grmhd_derive_scalars).  It runs on the read-ahead thread (prefetch =
+ * 1) or the calling thread (prefetch = 0), one call at a time, and returns
+ * KP_OK or an error that ends the run.  kp_iharm_load_snapshot (kp_io.h) is
+ * such a loader for iHARM dump files. */
+typedef kp_status (*kp_snapshot_loader)(void* user, int32_t index, kp_grmhd_cell* cells, uint64_t ncells);
+
+typedef struct kp_slow_light_stream {
+    int32_t nsnap;                /* snapshots, at least 2 */
+    int32_t step_mode;            /* kp_slow_light_step_mode */
+    int32_t interpolation;        /* kp_slow_light_interpolation */
+    int32_t windows_per_block;    /* snapshot pairs per block, at least 1 */
+    int32_t prefetch;             /* 1: read ahead on a library thread (default) */
+    int32_t prefetch_snapshots;   /* read-ahead queue length, 0 = windows_per_block */
+    int32_t derive_scalars;       /* 1: loader fills primitives, derived on the device */
+    int32_t reserved0;
+    double observation_time;
+    const double* times;          /* [nsnap] strictly increasing, HOST */
+    kp_snapshot_loader load;
+    void* user;                   /* passed to load */
+} kp_slow_light_stream;
+
+/* What the block scheduler did (the reference's timing lines
+ * slow_light_block_*, slow_light_prefetch_*, slow_light_cache_rewind). */
+typedef struct kp_slow_light_schedule {
+    int32_t blocks;               /* pass-B launches */
+    int32_t rewinds;              /* decoupled rewinds to an earlier snapshot */
+    int32_t loads;                /* loader calls */
+    int32_t slots;                /* device snapshot slots */
+    int64_t slot_bytes;           /* device bytes of the slots */
+    double load_ms;               /* time inside the loader */
+    double wait_ms;               /* host time waiting for a snapshot to be read */
+    double pass_b_ms;             /* device time of the pass-B launches */
+} kp_slow_light_schedule;
+
+void kp_slow_light_stream_default(kp_slow_light_stream* sequence);
+kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                         const kp_slow_light_stream* sequence, const double* frequencies,
+                                         int32_t nfreq, const kp_run_options* options, kp_image_data* out,
+                                         kp_image_stats* stats, kp_slow_light_schedule* schedule);
 
 /* Slow-light image and its derivatives along the fixed geodesics with
  * respect to (q_density, R_high, q_magnetic): q_density scales n_e by e^q and

@@ -42,6 +42,21 @@ kp_status kp_iharm_write(const char* path, const kp_iharm_header* header, const 
 /* Copies the header's grid and geometry into `model` (other fields kept). */
 void kp_iharm_apply_header(const kp_iharm_header* header, kp_grmhd_model* model);
 
+/* A time sequence of iHARM dumps streamed by kp_run_grmhd_slow_light_stream:
+ * kp_iharm_load_snapshot is its kp_snapshot_loader, with a kp_iharm_sequence
+ * as `user` and derive_scalars = 1.  Every dump must have the header's grid
+ * and coordinate mapping (fluid time interpolation needs "a common coordinate
+ * mapping/domain", src/image/slow_light_driver.hpp:31-42); the loader reads
+ * each dump as kp_iharm_read_cells does (src/grmhd/iharm_loader.cpp:262-396).
+ * HDF5 calls happen on the scheduler's read-ahead thread, one at a time. */
+typedef struct kp_iharm_sequence {
+    int32_t nsnap;
+    int32_t reserved0;
+    const char* const* paths;   /* [nsnap] dump files in time order */
+    kp_iharm_header header;     /* the grid of every dump (kp_iharm_read_header of the first) */
+} kp_iharm_sequence;
+kp_status kp_iharm_load_snapshot(void* user, int32_t index, kp_grmhd_cell* cells, uint64_t ncells);
+
 /* ---- kpolaris_image v1 ------------------------------------------------------ */
 typedef struct kp_image_file_meta {
     const char* model;           /* "riaf", "iharm", ... */

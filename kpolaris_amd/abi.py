@@ -64,7 +64,7 @@ PRECISION_MIXED = 1
 FP64_RADIUS_DEFAULT = 2.5
 
 # include/kp_image.h KP_ABI_VERSION this mirror matches
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)
@@ -173,11 +173,39 @@ class GRMHDModel(C.Structure):
 
 class SlowLight(C.Structure):
     _fields_ = [
-        ("nsnap", C.c_int32), ("step_mode", C.c_int32), ("interpolation", C.c_int32), ("reserved0", C.c_int32),
+        ("nsnap", C.c_int32), ("step_mode", C.c_int32), ("interpolation", C.c_int32),
+        ("windows_per_block", C.c_int32),
         ("observation_time", C.c_double),
         ("times", C.POINTER(C.c_double)),
         ("cells", C.POINTER(C.c_void_p)),
     ]
+
+
+# kp_snapshot_loader: (user, index, cells, ncells) -> kp_status
+SnapshotLoader = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64)
+
+
+class SlowLightStream(C.Structure):
+    """kp_slow_light_stream (SlowLightBatchOptions residency,
+    src/image/slow_light_batch_driver.hpp:13-24, with a host loader)."""
+    _fields_ = [
+        ("nsnap", C.c_int32), ("step_mode", C.c_int32), ("interpolation", C.c_int32),
+        ("windows_per_block", C.c_int32), ("prefetch", C.c_int32), ("prefetch_snapshots", C.c_int32),
+        ("derive_scalars", C.c_int32), ("reserved0", C.c_int32),
+        ("observation_time", C.c_double),
+        ("times", C.POINTER(C.c_double)),
+        ("load", SnapshotLoader),
+        ("user", C.c_void_p),
+    ]
+
+
+class SlowLightSchedule(C.Structure):
+    _fields_ = [("blocks", C.c_int32), ("rewinds", C.c_int32), ("loads", C.c_int32), ("slots", C.c_int32),
+                ("slot_bytes", C.c_int64), ("load_ms", C.c_double), ("wait_ms", C.c_double),
+                ("pass_b_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class LaunchTuning(C.Structure):

@@ -492,6 +492,15 @@ extern "C" void kp_slow_light_default(kp_slow_light* q) {
     q->interpolation = KP_SLOW_LIGHT_FLUID;
 }
 
+extern "C" void kp_slow_light_stream_default(kp_slow_light_stream* q) {
+    if (!q) return;
+    std::memset(q, 0, sizeof(*q));
+    q->step_mode = KP_SLOW_LIGHT_DECOUPLED;
+    q->interpolation = KP_SLOW_LIGHT_FLUID;
+    q->windows_per_block = 1;  // SlowLightBatchOptions defaults (slow_light_batch_driver.hpp:20-23)
+    q->prefetch = 1;
+}
+
 extern "C" void kp_run_options_default(kp_run_options* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));

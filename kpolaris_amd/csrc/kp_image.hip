@@ -4,6 +4,7 @@
 #include "kp_kernels.h"
 #include "kp_prepare.h"
 #include "kp_internal.h"
+#include "kp_slow_blocks.h"
 
 #include <hip/hip_runtime.h>
 
@@ -225,7 +226,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
                            const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
                            int perturbation, double log_scale, const Tangent* tangent = nullptr,
                            const kp_grmhd_model* grmhd = nullptr, const kp_slow_light* slow = nullptr,
-                           kp_slow_light_probe* probe = nullptr, bool noalloc = false) {
+                           kp_slow_light_probe* probe = nullptr, bool noalloc = false,
+                           kp_slow::Source* source = nullptr, kp_slow_light_schedule* schedule = nullptr) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -310,7 +312,12 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     int device = 0;
     KP_HIP(hipGetDevice(&device));
     const size_t np = (size_t)P.npix;
-    const bool unsplit = opt.split_transport == 0;
+    // windowed slow light: pass B over blocks of snapshots from `source`
+    // (always split and persistent: rays park between blocks)
+    const bool windowed = slow && source && base_mode(mode) == M_SLOW;
+    if (slow && source && !windowed)
+        return kp_internal::fail(KP_UNSUPPORTED, "slow-light blocks are built for images, not gradients");
+    const bool unsplit = opt.split_transport == 0 && !windowed;
     // Staged mixed precision (guard_mode 2, persistent split kernels): an
     // FP32-geodesic launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
     // continues them and parks those back that reach r >= back_radius, then
@@ -318,7 +325,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const bool staged = staged_run(opt, mode == M_SINGLE || mode == M_MF);
     const int nst = mf ? P.nfreq : 1;
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
-    const WsLayout W = workspace_layout(np, (size_t)nst, staged, nsnap, probe != nullptr);
+    const WsLayout W = workspace_layout(np, (size_t)nst, staged || slow != nullptr, nsnap, probe != nullptr);
     void* ws = nullptr;
     if (noalloc) {
         // the static scratch array: no staged lists, no timing events
@@ -353,13 +360,13 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     P.ep_reason = reinterpret_cast<int32_t*>(base + W.reason);
     P.queue = reinterpret_cast<uint32_t*>(base + W.queue);
     P.counters = reinterpret_cast<unsigned long long*>(base + W.counters);
-    uint32_t* lists = staged ? reinterpret_cast<uint32_t*>(base + W.lists) : nullptr;
-    if (staged) {
+    uint32_t* lists = (staged || windowed) ? reinterpret_cast<uint32_t*>(base + W.lists) : nullptr;
+    if (staged || windowed) {
         P.pk_d = reinterpret_cast<double*>(base + W.pkd);
         P.pk_i = reinterpret_cast<int32_t*>(base + W.pki);
         P.pk_st = reinterpret_cast<double*>(base + W.pkst);
-        P.back_radius = KP_BACK_RADIUS_FACTOR * P.fp64_radius;
     }
+    if (staged) P.back_radius = KP_BACK_RADIUS_FACTOR * P.fp64_radius;
 
     hipStream_t stream = static_cast<hipStream_t>(opt.stream);
     if (probe) {
@@ -375,8 +382,13 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         P.slow_mode = slow->step_mode;
         P.slow_interp = slow->interpolation;
         P.t_obs = slow->observation_time;
-        KP_HIP(hipMemcpyAsync(base + W.slow_t, slow->times, nsnap * sizeof(double), hipMemcpyHostToDevice, stream));
-        KP_HIP(hipMemcpyAsync(base + W.slow_c, slow->cells, nsnap * sizeof(void*), hipMemcpyHostToDevice, stream));
+        P.slow_final = 1;
+        P.sup_lo = slow->times[0];
+        P.sup_hi = slow->times[slow->nsnap - 1];
+        if (!windowed) {
+            KP_HIP(hipMemcpyAsync(base + W.slow_t, slow->times, nsnap * sizeof(double), hipMemcpyHostToDevice, stream));
+            KP_HIP(hipMemcpyAsync(base + W.slow_c, slow->cells, nsnap * sizeof(void*), hipMemcpyHostToDevice, stream));
+        }
     }
     const int block = opt.block_size > 0 ? opt.block_size : 128;
     if (block % 64 != 0 || block > 256)
@@ -470,7 +482,100 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         if (st != KP_OK) return st;
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
-    if (P.npix > 0 && !unsplit) {
+    // Windowed slow light: the block loop of run_slow_light_batch_metric_impl
+    // (slow_light_batch_driver.hpp:211-340) for one observation time.  Block
+    // b holds snapshots res_lo..stop (stop = start + windows_per_block, the
+    // halo kept from the block before it for decoupled stepping) and runs the
+    // rays the block before it parked (every pixel in the first); a
+    // decoupled ray that needs an earlier time than its block holds makes
+    // the next block start at that time's snapshot (:317-330).
+    double blocks_ms = 0.0;
+    auto run_blocks = [&]() -> kp_status {
+        const int n = slow->nsnap, end = n - 1;
+        const int wpb = std::max(1, slow->windows_per_block);
+        const bool decoupled = slow->step_mode == KP_SLOW_LIGHT_DECOUPLED;
+        struct BlockEvents {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~BlockEvents() {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } bev;
+        for (int i = 0; i < 2; ++i) KP_HIP(hipEventCreate(&bev.e[i]));
+        std::vector<const void*> ptrs;
+        std::vector<uint32_t> parked_list;
+        std::vector<double> req;
+        kp_slow::BlockPlan plan;
+        int parity = 0, blocks = 0, rewinds = 0;
+        bool first = true;
+        // every block advances some ray by a step or finishes it, so the
+        // loop ends; the bound only turns a logic error into an error
+        const long long max_blocks = (long long)(n + 1) * (long long)std::max(P.max_steps, 1) + 16;
+        for (long long iter = 0;; ++iter) {
+            if (iter > max_blocks) return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light block loop did not end");
+            const int stop = kp_slow::block_stop(plan, wpb, n);
+            const int res_lo = plan.res_lo;
+            kp_status s2 = source->acquire(res_lo, stop, stream, ptrs);
+            if (s2 != KP_OK) return s2;
+            const int cnt = stop - res_lo + 1;
+            KP_HIP(hipMemcpyAsync(base + W.slow_t, slow->times + res_lo, (size_t)cnt * sizeof(double),
+                                  hipMemcpyHostToDevice, stream));
+            KP_HIP(hipMemcpyAsync(base + W.slow_c, ptrs.data(), (size_t)cnt * sizeof(void*), hipMemcpyHostToDevice,
+                                  stream));
+            Params Q = P;
+            Q.nsnap = cnt;
+            Q.slow_final = stop == end ? 1 : 0;
+            Q.fetch = P.queue + 2;
+            Q.list_in = first ? nullptr : lists + (size_t)(1 - parity) * np;
+            Q.count_in = first ? nullptr : P.queue + 3 + (1 - parity);
+            Q.list_out = lists + (size_t)parity * np;
+            Q.count_out = P.queue + 3 + parity;
+            KP_HIP(hipMemsetAsync(Q.fetch, 0, sizeof(uint32_t), stream));
+            KP_HIP(hipMemsetAsync(Q.count_out, 0, sizeof(uint32_t), stream));
+            KP_HIP(hipEventRecord(bev.e[0], stream));
+            s2 = launch(kpsel::pass_b_slow_windowed(mode), smem_b, Q);
+            if (s2 != KP_OK) return s2;
+            KP_HIP(hipEventRecord(bev.e[1], stream));
+            ++blocks;
+            // the next block's snapshots upload while this one runs
+            if (stop < end) {
+                s2 = source->prefetch(stop + 1, stop + std::min(wpb, end - stop));
+                if (s2 != KP_OK) return s2;
+            }
+            uint32_t parked = 0;
+            KP_HIP(hipMemcpyAsync(&parked, Q.count_out, sizeof(parked), hipMemcpyDeviceToHost, stream));
+            KP_HIP(hipStreamSynchronize(stream));
+            float ms = 0.f;
+            KP_HIP(hipEventElapsedTime(&ms, bev.e[0], bev.e[1]));
+            blocks_ms += ms;
+            if (parked == 0) break;
+            if (stop == end && !decoupled)
+                return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light rays parked after the last block");
+            first = false;
+            parity ^= 1;
+            double earliest = slow->times[end];
+            if (decoupled) {
+                // slow_light_pending_range (slow_light_batch_driver.hpp:81-103)
+                parked_list.resize(parked);
+                req.resize(np);
+                KP_HIP(hipMemcpy(parked_list.data(), lists + (size_t)(1 - parity) * np, parked * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost));
+                KP_HIP(hipMemcpy(req.data(), P.req_t, np * sizeof(double), hipMemcpyDeviceToHost));
+                for (uint32_t pix : parked_list) earliest = std::min(earliest, req[pix]);
+            }
+            if (kp_slow::advance_plan(plan, stop, decoupled, earliest, slow->times, n)) ++rewinds;
+        }
+        if (schedule) {
+            schedule->blocks = blocks;
+            schedule->rewinds = rewinds;
+            schedule->pass_b_ms = blocks_ms;
+        }
+        return KP_OK;
+    };
+    if (P.npix > 0 && windowed) {
+        st = run_blocks();
+        if (st != KP_OK) return st;
+    } else if (P.npix > 0 && !unsplit) {
         if (staged) {
             st = run_staged(true, 12);
         } else {
@@ -492,6 +597,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         // unsplit: the one kernel is timed as pass A; the empty second
         // interval (a few microseconds of event overhead) is not reported
         if (unsplit) b_ms = 0.f;
+        // windowed: the block launches without the host's waits between them
+        if (windowed) b_ms = (float)blocks_ms;
         std::memset(stats, 0, sizeof(*stats));
         stats->pass_a_ms = a_ms;
         stats->pass_b_ms = b_ms;
@@ -1024,6 +1131,7 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
     }
     if (nfreq < 0 || (nfreq > 0 && !frequencies))
         return kp_internal::fail(KP_INVALID_ARGUMENT, "frequency list and count disagree");
+    if (q.windows_per_block < 0) return kp_internal::fail(KP_INVALID_ARGUMENT, "windows_per_block must be non-negative");
     kp_grmhd_model m = *model;
     m.cells = q.cells[0];
     FreqLists fl;
@@ -1031,7 +1139,81 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
         fl.nfreq = fl.nctrl = nfreq;
         fl.freqs = fl.ctrl = frequencies;
     }
-    return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, sequence);
+    // blocks shorter than the sequence run the block scheduler over the
+    // resident snapshots
+    kp_slow::Resident resident(q.cells);
+    kp_slow::Source* source = (q.windows_per_block > 0 && q.windows_per_block < q.nsnap - 1) ? &resident : nullptr;
+    return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, sequence, nullptr, false,
+                     source);
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                    const kp_slow_light_stream* sequence, const double* frequencies,
+                                                    int32_t nfreq, const kp_run_options* options, kp_image_data* out,
+                                                    kp_image_stats* stats, kp_slow_light_schedule* schedule) {
+    kp_internal::clear_error();
+    if (!model || !sequence) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model or sequence");
+    const kp_slow_light_stream& q = *sequence;
+    // the batch scheduler's argument checks (slow_light_batch_driver.hpp:133-148)
+    if (q.nsnap < 2) return kp_internal::fail(KP_INVALID_ARGUMENT, "slow light requires at least two snapshots");
+    if (!q.times || !q.load) return kp_internal::fail(KP_INVALID_ARGUMENT, "null snapshot times or loader");
+    if (q.windows_per_block < 1 || q.prefetch_snapshots < 0)
+        return kp_internal::fail(KP_INVALID_ARGUMENT,
+                                 "slow-light windows_per_block must be positive and prefetch_snapshots nonnegative");
+    if (q.step_mode != KP_SLOW_LIGHT_DECOUPLED && q.step_mode != KP_SLOW_LIGHT_SNAPSHOT &&
+        q.step_mode != KP_SLOW_LIGHT_BLOCK)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light step mode");
+    if (q.interpolation != KP_SLOW_LIGHT_FLUID && q.interpolation != KP_SLOW_LIGHT_COEFFICIENTS)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light interpolation");
+    if (!std::isfinite(q.observation_time)) return kp_internal::fail(KP_INVALID_ARGUMENT, "observation time must be finite");
+    for (int k = 0; k < q.nsnap; ++k)
+        if (!std::isfinite(q.times[k]) || (k > 0 && !(q.times[k] > q.times[k - 1])))
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "snapshot times must be finite and strictly increasing");
+    if (nfreq < 0 || (nfreq > 0 && !frequencies))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "frequency list and count disagree");
+    if (model->n1 < 1 || model->n2 < 1 || model->n3 < 1)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "GRMHD grid dimensions must be positive");
+    const size_t ncells = (size_t)model->n1 * (size_t)model->n2 * (size_t)model->n3;
+    int device = 0;
+    KP_HIP(hipGetDevice(&device));
+    const int halo = q.step_mode == KP_SLOW_LIGHT_DECOUPLED ? kp_slow::DECOUPLED_HALO : 0;
+    const long long want = 2ll * q.windows_per_block + halo + 1;
+    const int slots = (int)std::min<long long>(want, q.nsnap);
+    kp_slow::Streamed source(q, ncells, slots, device);
+    kp_grmhd_model m = *model;
+    // the grid model of the derived-scalar kernel (units, electron model)
+    kp_pass_a_params probe_params;
+    kp_pass_a_params_default(&probe_params);
+    probe_params.coordinate_system = KP_COORD_FMKS;
+    probe_params.camera.nx = probe_params.camera.ny = 1;
+    probe_params.camera.radius = 100.0;
+    Params G;
+    kp_status st = prepare_grmhd(&probe_params, &m, G, false);
+    if (st != KP_OK) return st;
+    st = source.init(q.derive_scalars ? kpsel::derive_scalars() : nullptr, G.grmhd);
+    if (st != KP_OK) return st;
+    m.cells = source.slot(0);
+    kp_slow_light seq;
+    kp_slow_light_default(&seq);
+    seq.nsnap = q.nsnap;
+    seq.step_mode = q.step_mode;
+    seq.interpolation = q.interpolation;
+    seq.windows_per_block = q.windows_per_block;
+    seq.observation_time = q.observation_time;
+    seq.times = q.times;
+    seq.cells = nullptr;
+    FreqLists fl;
+    if (nfreq > 0) {
+        fl.nfreq = fl.nctrl = nfreq;
+        fl.freqs = fl.ctrl = frequencies;
+    }
+    kp_slow_light_schedule rep;
+    std::memset(&rep, 0, sizeof(rep));
+    st = run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, &seq, nullptr, false, &source, &rep);
+    if (st != KP_OK && st != KP_TIME_SUPPORT) return st;
+    source.report(rep);
+    if (schedule) *schedule = rep;
+    return st;
 }
 
 extern "C" kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_params* params,
@@ -1051,6 +1233,11 @@ extern "C" kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_para
     if (sequence->step_mode < KP_SLOW_LIGHT_DECOUPLED || sequence->step_mode > KP_SLOW_LIGHT_BLOCK ||
         (sequence->interpolation != KP_SLOW_LIGHT_FLUID && sequence->interpolation != KP_SLOW_LIGHT_COEFFICIENTS))
         return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light step mode or interpolation");
+    // the gradient kernel runs one block; the images of decoupled and
+    // snapshot stepping do not depend on the block length, block stepping's do
+    if (sequence->step_mode == KP_SLOW_LIGHT_BLOCK && sequence->windows_per_block > 0 &&
+        sequence->windows_per_block < sequence->nsnap - 1)
+        return kp_internal::fail(KP_UNSUPPORTED, "slow-light gradients with block stepping run one block of the whole sequence");
     kp_grmhd_model m = *model;
     m.cells = sequence->cells[0];
     Tangent t;

@@ -277,6 +277,28 @@ extern "C" kp_status kp_iharm_read_cells(const char* path, const kp_iharm_header
     return KP_OK;
 }
 
+extern "C" kp_status kp_iharm_load_snapshot(void* user, int32_t index, kp_grmhd_cell* cells, uint64_t ncells) {
+    kp_internal::clear_error();
+    const kp_iharm_sequence* q = static_cast<const kp_iharm_sequence*>(user);
+    if (!q || !q->paths || !cells) return fail(KP_INVALID_ARGUMENT, "null argument");
+    if (index < 0 || index >= q->nsnap || !q->paths[index]) return fail(KP_INVALID_ARGUMENT, "snapshot index out of range");
+    const kp_iharm_header& g = q->header;
+    if (ncells != (uint64_t)g.n1 * (uint64_t)g.n2 * (uint64_t)g.n3)
+        return fail(KP_INVALID_ARGUMENT, "snapshot cell count does not match the sequence grid");
+    kp_iharm_header h;
+    kp_status st = kp_iharm_read_header(q->paths[index], &h);
+    if (st != KP_OK) return st;
+    const bool same = h.n1 == g.n1 && h.n2 == g.n2 && h.n3 == g.n3 && h.a == g.a && h.startx1 == g.startx1 &&
+                      h.startx2 == g.startx2 && h.startx3 == g.startx3 && h.dx1 == g.dx1 && h.dx2 == g.dx2 &&
+                      h.dx3 == g.dx3 && h.hslope == g.hslope && h.mks_smooth == g.mks_smooth &&
+                      h.poly_alpha == g.poly_alpha && h.poly_xt == g.poly_xt;
+    if (!same)
+        return fail(KP_INVALID_ARGUMENT,
+                    "slow_light_interpolation=fluid requires native fluid variables, a common coordinate "
+                    "mapping/domain, and matching scalar reconstruction");
+    return kp_iharm_read_cells(q->paths[index], &h, cells);
+}
+
 extern "C" kp_status kp_iharm_write(const char* path, const kp_iharm_header* h, const float* prims) {
     kp_internal::clear_error();
     if (!path || !h || !prims) return fail(KP_INVALID_ARGUMENT, "null argument");

@@ -85,14 +85,22 @@ enum Mode : int {
     M_SINGLE = 0, M_MF = 1, M_TANGENT = 2, M_GRMHD = 3, M_SLOW = 4, M_PROBE = 5, M_STAN = 6,
     // the kappa / power-law forms of M_MF, M_GRMHD and M_SLOW: kernels of their
     // own, so the thermal kernels carry no call to the nonthermal routine
-    M_MF_NT = 7, M_GRMHD_NT = 8, M_SLOW_NT = 9
+    M_MF_NT = 7, M_GRMHD_NT = 8, M_SLOW_NT = 9,
+    // windowed slow light (thermal, nonthermal): pass B over one block of the
+    // sequence, its rays parking at the block's end or while they wait for a
+    // time another block holds; instantiations of their own, so the kernels
+    // of whole sequences carry none of the parking code
+    M_SLOW_W = 10, M_SLOW_NT_W = 11
 };
 
-// The mode whose lane logic a mode runs (a nonthermal mode runs its thermal one).
+// The mode whose lane logic a mode runs (a nonthermal mode runs its thermal
+// one, a windowed slow-light mode the slow-light one).
 constexpr int base_mode(int m) {
-    return m == M_MF_NT ? M_MF : (m == M_GRMHD_NT ? M_GRMHD : (m == M_SLOW_NT ? M_SLOW : m));
+    return m == M_MF_NT ? M_MF
+                        : (m == M_GRMHD_NT ? M_GRMHD : ((m == M_SLOW_NT || m == M_SLOW_W || m == M_SLOW_NT_W) ? M_SLOW : m));
 }
-constexpr bool nt_mode(int m) { return m == M_MF_NT || m == M_GRMHD_NT || m == M_SLOW_NT; }
+constexpr bool nt_mode(int m) { return m == M_MF_NT || m == M_GRMHD_NT || m == M_SLOW_NT || m == M_SLOW_NT_W; }
+constexpr bool win_mode(int m) { return m == M_SLOW_W || m == M_SLOW_NT_W; }
 
 // Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
 // (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
@@ -183,13 +191,20 @@ struct Params {
     double* pk_d;     // pass A: [npix] h_current; pass B: [2][npix] h_current, cap
     int32_t* pk_i;    // pass A: [4][npix] n, steps, steps_inside, entered; pass B: [2][npix] steps, reason
     double* pk_st;    // pass B: [4 nfreq][npix] Stokes
-    // slow light: snapshot cell arrays and times (device), the stepping and
-    // interpolation rules (kp_slow_light), the observation time, and per pixel
-    // the time a waiting ray needed (NaN for the others)
+    // slow light: the resident snapshot cell arrays and times (device), the
+    // stepping and interpolation rules (kp_slow_light), the observation time,
+    // and per pixel the time a waiting ray needed (NaN for the others).
+    // Windowed modes (M_SLOW_W, M_SLOW_NT_W): the resident snapshots are one block of
+    // the sequence; slow_final says whether its last snapshot is the
+    // sequence's last (else a ray reaching that time parks for the next
+    // block), and [sup_lo, sup_hi] is the whole sequence's time support (a
+    // decoupled ray needing a time inside it but outside the block parks
+    // waiting, its pk_d holding the step size its current call started with).
     const kp_grmhd_cell* const* snaps;
     const double* snap_t;
-    int32_t nsnap, slow_mode, slow_interp, reserved_slow;
+    int32_t nsnap, slow_mode, slow_interp, slow_final;
     double t_obs;
+    double sup_lo, sup_hi;
     double* req_t;
     // slow-light gradients: the evaluation point's density and field factors
     // (n_e e^q_density, B e^(q_density/2 + q_magnetic))
@@ -727,7 +742,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
 // ---------------------------------------------------------------------------
 // Pass B: trace_pass_b_segment_model_endpoint_pixel_metric (src/geodesic/pass_b.hpp:515-722).
 // ---------------------------------------------------------------------------
-enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2, PH_TIME = 3 };
+// PH_PARK: a windowed slow-light ray leaves the launch (block end, or a
+// decoupled sample time in a later or earlier block) and resumes in the next.
+enum : int { PH_IDLE = 0, PH_MAIN = 1, PH_CROSS = 2, PH_TIME = 3, PH_PARK = 4 };
 
 struct LaneB {
     LaneVec s, k1;  // the lane's state and its k1 scratch
@@ -886,11 +903,18 @@ template <int MODE> KP_DEV bool lane_b_init(const Params& P, int pixel, LaneB& L
     return true;
 }
 
+// A waiting slow-light ray (req set) keeps the step size its call started
+// with, already in pk_d, and records the time it needs.
 template <int MODE> KP_DEV void lane_b_park(const Params& P, const LaneB& L) {
     const size_t np = (size_t)P.npix, p = (size_t)L.pixel;
 #pragma unroll
     for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + p] = L.s[c];
-    P.pk_d[p] = L.h_current;
+    if constexpr (win_mode(MODE)) {
+        if (!(L.req == L.req)) P.pk_d[p] = L.h_current;
+        P.req_t[p] = L.req;
+    } else {
+        P.pk_d[p] = L.h_current;
+    }
     P.pk_d[np + p] = L.cap;
     P.pk_i[p] = L.steps;
     P.pk_i[np + p] = L.reason;
@@ -1178,6 +1202,8 @@ template <bool NT> struct RadiationSlow {
 };
 template <> struct Radiation<false, M_SLOW> : RadiationSlow<false> {};
 template <> struct Radiation<false, M_SLOW_NT> : RadiationSlow<true> {};
+template <> struct Radiation<false, M_SLOW_W> : RadiationSlow<false> {};
+template <> struct Radiation<false, M_SLOW_NT_W> : RadiationSlow<true> {};
 
 // Time probe: geometric stepping only (no coefficients, no substep control).
 template <> struct Radiation<false, M_PROBE> {
@@ -1391,6 +1417,14 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
                 while (!L.vac && P.t_obs + L.s[0] >= slow_upper(P, L)) slow_window_end(P, L);
             }
             if (L.vac == 1) {
+                if constexpr (win_mode(MODE)) {
+                    if (!P.slow_final) {
+                        // the block's last window ended: the ray waits for the next block
+                        // (slow_light_driver.hpp:1115-1116, slow_light_batch_driver.hpp:300)
+                        L.phase = PH_PARK;
+                        return false;
+                    }
+                }
                 const double tol_r = 1e-8 * maxv(1.0, absv(P.outer));
                 if (GT::radius(L.s[1]) < P.outer - tol_r) {
                     L.reason = R_SLOW_TIME;
@@ -1400,6 +1434,10 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
             }
         }
         double h = L.h_current;
+        if constexpr (win_mode(MODE)) {
+            // retry_h of this call (slow_light_driver.hpp:1156, 1302)
+            if (P.slow_mode == KP_SLOW_LIGHT_DECOUPLED) P.pk_d[L.pixel] = h;
+        }
         if (!(SLOW && L.vac) && GT::radius(L.s[1]) <= P.outer && L.cap > 0.0) h = minv(h, L.cap);
         L.h = h;
         if (P.adaptive) start_call(L, h, eff_min, P.max_step);
@@ -1563,10 +1601,17 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
         const int radiating = sample_r >= P.inner && sample_r < P.outer;
         const double ct = P.t_obs + sample.v[0];
         if (P.slow_mode == KP_SLOW_LIGHT_DECOUPLED && radiating && !(ct >= P.snap_t[0] && ct <= P.snap_t[P.nsnap - 1])) {
-            // the time is not in the sequence: with every snapshot resident no
-            // later block brings it (slow_light_driver.hpp:1303-1309,
-            // slow_light_batch_driver.hpp:282-290)
+            // the time is not resident: the ray waits, unrefined, for the
+            // block that holds it (slow_light_driver.hpp:1299-1304); a time
+            // outside the whole sequence no block brings
+            // (slow_light_batch_driver.hpp:290-297)
             L.req = ct;
+            if constexpr (win_mode(MODE)) {
+                if (ct >= P.sup_lo && ct <= P.sup_hi) {
+                    L.phase = PH_PARK;
+                    return false;
+                }
+            }
             L.reason = R_SLOW_TIME;
             return true;
         }
@@ -1671,7 +1716,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
                 if (!__ballot(need)) break;
                 int idx = 0;
                 if (fetch_pixel(P.fetch, need, limit, idx)) {
-                    if (MIXED && P.list_in) {
+                    if ((MIXED || win_mode(MODE)) && P.list_in) {
                         lane_b_resume<MODE>(P, (int)P.list_in[idx], L);
                         has = true;
                     } else {
@@ -1691,7 +1736,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
                 }
                 append_parked(P, park, L.pixel);
             }
-            if (has && pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+            if constexpr (win_mode(MODE)) {
+                // windowed slow light: a ray that parks leaves the launch
+                bool park = false;
+                if (has) {
+                    if (pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+                        lane_b_finish<MIXED, MODE>(P, L, 1);
+                        has = false;
+                    } else if (L.phase == PH_PARK) {
+                        lane_b_park<MODE>(P, L);
+                        has = false;
+                        park = true;
+                    }
+                }
+                append_parked(P, park, L.pixel);
+            } else if (has && pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
                 lane_b_finish<MIXED, MODE>(P, L, 1);
                 has = false;
             }
@@ -1887,5 +1946,6 @@ const void* pass_b_batch(bool mixed, int gp);
 const void* unsplit_bl(bool mixed, int gp, int mode);
 const void* unsplit_fmks(int mode);
 const void* pass_b_stan(bool persistent);
+const void* pass_b_slow_windowed(int mode);
 const void* unsplit_slow(int mode);
 }  // namespace kpsel

@@ -167,6 +167,17 @@ const void* unsplit_fmks(int mode) {
 }
 #endif
 
+#if KP_PART == 10
+// Windowed slow light: persistent pass B over one block of the sequence.
+const void* pass_b_slow_windowed(int mode) {
+    switch (mode) {
+    case M_SLOW: return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_W>;
+    case M_SLOW_NT: return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_NT_W>;
+    default: return nullptr;
+    }
+}
+#endif
+
 #if KP_PART == 9
 const void* unsplit_slow(int mode) {
     switch (mode) {

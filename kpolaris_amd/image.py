@@ -112,6 +112,12 @@ def library():
                                                 P(abi.ImageStats)]
     L.kp_run_grmhd_slow_light_image.restype = C.c_int32
     L.kp_slow_light_default.argtypes = [P(abi.SlowLight)]
+    L.kp_slow_light_stream_default.argtypes = [P(abi.SlowLightStream)]
+    L.kp_slow_light_stream_default.restype = None
+    L.kp_run_grmhd_slow_light_stream.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLightStream),
+                                                 P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData),
+                                                 P(abi.ImageStats), P(abi.SlowLightSchedule)]
+    L.kp_run_grmhd_slow_light_stream.restype = C.c_int32
     L.kp_slow_light_time_probe.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.RunOptions),
                                            P(abi.SlowLightProbe)]
     L.kp_run_grmhd_slow_light_image_gradient.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
@@ -137,6 +143,8 @@ def header_functions() -> list[str]:
     """Names of every function the public header declares."""
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    # typedef kp_status (*kp_loader)(...) declares a pointer type, not a function
+    text = re.sub(r"typedef[^;]*\(\s*\*[^;]*;", "", text)
     return sorted(set(re.findall(r"\b(kp_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -488,7 +496,7 @@ def run_grmhd_image(params: abi.PassAParams, grid: "GRMHDGrid", frequencies=None
     return out, (st.as_dict() if with_stats else None)
 
 
-def _sequence(grids, times, observation_time, step_mode, interpolation):
+def _sequence(grids, times, observation_time, step_mode, interpolation, windows_per_block=0):
     t = np.ascontiguousarray(times, dtype=np.float64)
     ptrs = (C.c_void_p * len(grids))(*[g.model.cells for g in grids])
     seq = abi.SlowLight()
@@ -496,6 +504,7 @@ def _sequence(grids, times, observation_time, step_mode, interpolation):
     seq.nsnap = len(grids)
     seq.step_mode = step_mode
     seq.interpolation = interpolation
+    seq.windows_per_block = windows_per_block
     seq.observation_time = observation_time
     seq.times = t.ctypes.data_as(C.POINTER(C.c_double))
     seq.cells = C.cast(ptrs, C.POINTER(C.c_void_p))
@@ -572,26 +581,19 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observatio
                                *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
                                interpolation: int = abi.SLOW_LIGHT_FLUID, persistent: bool = True,
                                block_size: int = 0, waves_per_cu: int = 0, split_transport: int = 1,
-                               out: dict | None = None, fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+                               windows_per_block: int = 0, out: dict | None = None, fields=abi.IMAGE_FIELDS,
+                               with_stats: bool = True):
     """Slow-light GRMHD image over a snapshot sequence (grids: GRMHDGrid per
     time, all on one grid geometry; times strictly increasing) sampled at
-    retarded time observation_time + x^0.  Raises KPError(KP_TIME_SUPPORT)
-    when decoupled stepping needs a time outside the sequence (the image is
-    still written into `out`)."""
+    retarded time observation_time + x^0.  windows_per_block > 0 runs pass B
+    in blocks of that many snapshot pairs (the reference's batch scheduler).
+    Raises KPError(KP_TIME_SUPPORT) when decoupled stepping needs a time
+    outside the sequence (the image is still written into `out`)."""
     import torch
     npix = params.camera.nx * params.camera.ny
     model = grids[0].model
     f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
-    t = np.ascontiguousarray(times, dtype=np.float64)
-    ptrs = (C.c_void_p * len(grids))(*[g.model.cells for g in grids])
-    seq = abi.SlowLight()
-    library().kp_slow_light_default(C.byref(seq))
-    seq.nsnap = len(grids)
-    seq.step_mode = step_mode
-    seq.interpolation = interpolation
-    seq.observation_time = observation_time
-    seq.times = t.ctypes.data_as(C.POINTER(C.c_double))
-    seq.cells = C.cast(ptrs, C.POINTER(C.c_void_p))
+    seq, keep = _sequence(grids, times, observation_time, step_mode, interpolation, windows_per_block)
     if out is None:
         out = alloc_device_image(npix, fields=fields, nfreq=len(f))
     img = image_struct_from_tensors(out)
@@ -603,6 +605,65 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observatio
         C.byref(params), C.byref(model), C.byref(seq), f.ctypes.data_as(C.POINTER(C.c_double)), len(f),
         C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)
+
+
+def host_cell_loader(snapshots):
+    """A kp_snapshot_loader over host snapshots (numpy arrays of 72-byte
+    kp_grmhd_cell records, one per time); returns the ctypes callback, which
+    the caller keeps alive for the run."""
+    raws = [np.ascontiguousarray(c).view(np.uint8).reshape(-1) for c in snapshots]
+
+    def load(user, index, cells, ncells):
+        try:
+            raw = raws[index]
+            if raw.size != int(ncells) * 72:
+                return abi.KP_INVALID_ARGUMENT
+            C.memmove(cells, raw.ctypes.data, raw.size)
+            return abi.KP_OK
+        except Exception:  # an exception cannot cross the C boundary
+            return abi.KP_INVALID_ARGUMENT
+
+    return abi.SnapshotLoader(load)
+
+
+def run_grmhd_slow_light_stream(params: abi.PassAParams, model: abi.GRMHDModel, times, observation_time: float,
+                                loader, frequencies=None, *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
+                                interpolation: int = abi.SLOW_LIGHT_FLUID, windows_per_block: int = 1,
+                                prefetch: bool = True, prefetch_snapshots: int = 0, derive_scalars: bool = False,
+                                user=None, block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
+                                fields=abi.IMAGE_FIELDS, with_stats: bool = True):
+    """Slow-light GRMHD image of a sequence streamed from host storage
+    (kp_run_grmhd_slow_light_stream): `loader` is an abi.SnapshotLoader
+    (host_cell_loader, or a C loader such as kp_iharm_load_snapshot with its
+    `user` pointer).  Returns (outputs, stats, schedule)."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    q = abi.SlowLightStream()
+    library().kp_slow_light_stream_default(C.byref(q))
+    q.nsnap = len(t)
+    q.step_mode = step_mode
+    q.interpolation = interpolation
+    q.windows_per_block = windows_per_block
+    q.prefetch = 1 if prefetch else 0
+    q.prefetch_snapshots = prefetch_snapshots
+    q.derive_scalars = 1 if derive_scalars else 0
+    q.observation_time = observation_time
+    q.times = t.ctypes.data_as(C.POINTER(C.c_double))
+    q.load = loader
+    q.user = user
+    if out is None:
+        out = alloc_device_image(npix, fields=fields, nfreq=len(f))
+    img = image_struct_from_tensors(out)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(abi.PRECISION_FP64, True, block_size, waves_per_cu, C.c_void_p(stream))
+    st = abi.ImageStats()
+    rep = abi.SlowLightSchedule()
+    _check(library().kp_run_grmhd_slow_light_stream(
+        C.byref(params), C.byref(model), C.byref(q), f.ctypes.data_as(C.POINTER(C.c_double)), len(f),
+        C.byref(opts), C.byref(img), C.byref(st) if with_stats else None, C.byref(rep)))
+    return out, (st.as_dict() if with_stats else None), rep.as_dict()
 
 
 def slow_light_time_probe(params: abi.PassAParams, model: abi.GRMHDModel, *, persistent: bool = True) -> dict:

@@ -23,6 +23,12 @@ class IharmHeader(C.Structure):
                                   "mks_smooth", "poly_alpha", "poly_xt", "r_in", "r_out", "poly_norm")]
 
 
+class IharmSequence(C.Structure):
+    """kp_iharm_sequence."""
+    _fields_ = [("nsnap", C.c_int32), ("reserved0", C.c_int32), ("paths", C.POINTER(C.c_char_p)),
+                ("header", IharmHeader)]
+
+
 class ImageFileMeta(C.Structure):
     """kp_image_file_meta."""
     _fields_ = [("model", C.c_char_p), ("camera", C.c_char_p), ("coordinate", C.c_char_p), ("metric", C.c_char_p),
@@ -74,6 +80,8 @@ def library():
     L.kp_iharm_read_header.argtypes = [C.c_char_p, P(IharmHeader)]
     L.kp_iharm_read_cells.argtypes = [C.c_char_p, P(IharmHeader), C.c_void_p]
     L.kp_iharm_write.argtypes = [C.c_char_p, P(IharmHeader), C.c_void_p]
+    L.kp_iharm_load_snapshot.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]
+    L.kp_iharm_load_snapshot.restype = C.c_int32
     L.kp_iharm_apply_header.argtypes = [P(IharmHeader), P(abi.GRMHDModel)]
     L.kp_iharm_apply_header.restype = None
     L.kp_image_write_hdf5.argtypes = [C.c_char_p, P(ImageFileMeta), P(abi.PassAParams), P(abi.ImageData)]
@@ -127,6 +135,22 @@ def read_iharm(path, model: abi.GRMHDModel | None = None):
     L.kp_iharm_apply_header(C.byref(h), C.byref(m))
     m.cells = None
     return h, cells, m
+
+
+class IharmStream:
+    """The C loader kp_iharm_load_snapshot over dump files in time order, as
+    (loader, user) for image.run_grmhd_slow_light_stream (with
+    derive_scalars=True).  Keep the object alive for the run."""
+
+    def __init__(self, paths):
+        self.encoded = [str(p).encode() for p in paths]
+        self.array = (C.c_char_p * len(self.encoded))(*self.encoded)
+        self.seq = IharmSequence()
+        self.seq.nsnap = len(self.encoded)
+        self.seq.paths = C.cast(self.array, C.POINTER(C.c_char_p))
+        _check(library().kp_iharm_read_header(self.encoded[0], C.byref(self.seq.header)))
+        self.loader = abi.SnapshotLoader(C.cast(library().kp_iharm_load_snapshot, C.c_void_p).value)
+        self.user = C.cast(C.byref(self.seq), C.c_void_p)
 
 
 def write_image(path, params: abi.PassAParams, arrays: dict, frequencies, *, model_name="riaf",

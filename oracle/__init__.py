@@ -301,7 +301,7 @@ def run_grmhd_image(params: abi.PassAParams, model: abi.GRMHDModel, cells: np.nd
 
 
 def slow_light_sequence(cells_list, times, observation_time, step_mode=abi.SLOW_LIGHT_DECOUPLED,
-                        interpolation=abi.SLOW_LIGHT_FLUID):
+                        interpolation=abi.SLOW_LIGHT_FLUID, windows_per_block=0):
     """kp_slow_light over HOST snapshot arrays; returns (struct, keep-alive refs)."""
     t = np.ascontiguousarray(times, dtype=np.float64)
     ptrs = (C.c_void_p * len(cells_list))(*[c.ctypes.data for c in cells_list])
@@ -309,6 +309,7 @@ def slow_light_sequence(cells_list, times, observation_time, step_mode=abi.SLOW_
     seq.nsnap = len(cells_list)
     seq.step_mode = step_mode
     seq.interpolation = interpolation
+    seq.windows_per_block = windows_per_block
     seq.observation_time = observation_time
     seq.times = t.ctypes.data_as(C.POINTER(C.c_double))
     seq.cells = C.cast(ptrs, C.POINTER(C.c_void_p))
@@ -318,12 +319,13 @@ def slow_light_sequence(cells_list, times, observation_time, step_mode=abi.SLOW_
 def run_grmhd_slow_light_image(params: abi.PassAParams, model: abi.GRMHDModel, cells_list, times,
                                observation_time, frequencies=None, step_mode=abi.SLOW_LIGHT_DECOUPLED,
                                interpolation=abi.SLOW_LIGHT_FLUID, pixel_begin: int = 0, pixel_end: int = -1,
-                               nthreads: int = 0, perturbation=None):
+                               nthreads: int = 0, perturbation=None, windows_per_block: int = 0):
     """Oracle slow-light GRMHD image; returns (arrays, stats, waiting) where
     waiting = (count, min, max) of the times decoupled stepping needed outside
     the sequence (count 0: none; the oracle status is then KP_OK)."""
     L = lib()
-    seq, keep = slow_light_sequence(cells_list, times, observation_time, step_mode, interpolation)
+    seq, keep = slow_light_sequence(cells_list, times, observation_time, step_mode, interpolation,
+                                    windows_per_block)
     npix = params.camera.nx * params.camera.ny
     f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
     arrays = alloc_image(npix)

@@ -3037,7 +3037,14 @@ static int slow_vacuum_to_camera(const kp_pass_a_params* p, const metric_ctx* m,
  * samples that pair (SlowLightSingleWindow, :966-981); block stepping runs one
  * window cut at the last time and samples the pair enclosing each sample's
  * time; decoupled stepping cuts nothing and a radiating sample outside the
- * sequence leaves the ray waiting (SlowLightResidentWindows, :983-1027). */
+ * sequence leaves the ray waiting (SlowLightResidentWindows, :983-1027).
+ * With windows_per_block W (0 < W < nsnap - 1) the batch scheduler
+ * (slow_light_batch_driver.hpp:211-340) runs the sequence as blocks of W
+ * snapshot pairs, [bW, min((b+1)W, nsnap - 1)]: block stepping then runs one
+ * window per block, cut at the block's last time and sampling the pair of
+ * that block enclosing each sample's time; snapshot and decoupled stepping
+ * give the same image as one block (a waiting decoupled ray resumes with the
+ * same call in the block holding its time). */
 static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, const kp_slow_light* seq,
                         const kpo_perturbation* pert, const freq_plan* fp, const metric_ctx* m,
                         const kpo_state* camera,
@@ -3070,10 +3077,14 @@ static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, cons
     double h_current = p->step;
     double cap = p->max_radiation_step > 0.0 ? p->max_radiation_step : p->max_step;
     int steps = 0, reached = 0, active = 1, reason = pass_a_reason;
-    const int nwin = mode == KP_SLOW_LIGHT_SNAPSHOT ? n - 1 : 1;
+    const int wpb = seq->windows_per_block;
+    const int blocked = mode == KP_SLOW_LIGHT_BLOCK && wpb > 0 && wpb < n - 1;
+    const int nwin = mode == KP_SLOW_LIGHT_SNAPSHOT ? n - 1 : (blocked ? (n - 1 + wpb - 1) / wpb : 1);
     for (int w = 0; w < nwin; ++w) {
         if (reached || !active) break;
-        const double upper = mode == KP_SLOW_LIGHT_SNAPSHOT ? T[w + 1] : T[n - 1];
+        const int blo = blocked ? w * wpb : 0;
+        const int bhi = blocked ? (w * wpb + wpb < n - 1 ? w * wpb + wpb : n - 1) : n - 1;
+        const double upper = mode == KP_SLOW_LIGHT_SNAPSHOT ? T[w + 1] : T[bhi];
         while (!reached && steps < p->max_steps) {
             const double retry_h = h_current;
             double h = h_current;
@@ -3183,7 +3194,7 @@ static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, cons
                 h_current = retry_h;
                 break;
             }
-            const int pair = mode == KP_SLOW_LIGHT_SNAPSHOT ? w : kpo_slow_light_pair(T, n, ct);
+            const int pair = mode == KP_SLOW_LIGHT_SNAPSHOT ? w : blo + kpo_slow_light_pair(T + blo, bhi - blo + 1, ct);
             const double local_max_rs =
                 (p->max_radiation_step > 0.0 &&
                  (old_r <= p->outer_radius || sample_r <= p->outer_radius || next_r <= p->outer_radius))

@@ -5,6 +5,7 @@
 // time, so the kernel's control logic can be checked against the oracle
 // without a GPU.  Never part of the product.
 #include "../../kpolaris_amd/csrc/kp_prepare.h"
+#include "../../kpolaris_amd/csrc/kp_slow_blocks.h"
 
 #include <vector>
 
@@ -218,4 +219,128 @@ extern "C" kp_status kp_emul_thermal_coeffs_scaled(const kp_pass_a_params* pa, c
     const float v[11] = {c.jI, c.jQ, c.jU, c.jV, c.aI, c.aQ, c.aU, c.aV, c.rQ, c.rU, c.rV};
     for (int i = 0; i < 11; ++i) out[i] = v[i];
     return KP_OK;
+}
+
+// Windowed slow-light lanes: pass A for every pixel, then the library's block
+// loop (kp_slow_blocks.h plan) over HOST snapshots, each block running its
+// work list (every pixel first, then the rays the block before it parked)
+// one lane at a time through the kernel's tick, park and resume code.
+template <int MODE>
+static kp_status emul_slow_blocks(const kp_pass_a_params* pa, const kp_grmhd_model* gm, const kp_slow_light* slow,
+                                  const FreqLists& fl, kp_image_data* out, double* req_t,
+                                  unsigned long long* waiting, int32_t* nblocks, int32_t* nrewinds) {
+    kp_grmhd_model g = *gm;
+    g.cells = slow->cells[0];
+    Params P;
+    kp_status st = prepare_grmhd(pa, &g, P);
+    if (st != KP_OK) return st;
+    st = prepare_frequencies(gm->mbh_solar, fl, P);
+    if (st != KP_OK) return st;
+    unsigned long long counters[C_COUNT] = {};
+    P.counters = counters;
+    const int n = slow->nsnap, end = n - 1;
+    const kp_grmhd_cell* const* cells = reinterpret_cast<const kp_grmhd_cell* const*>(slow->cells);
+    P.slow_mode = slow->step_mode;
+    P.slow_interp = slow->interpolation;
+    P.t_obs = slow->observation_time;
+    P.sup_lo = slow->times[0];
+    P.sup_hi = slow->times[end];
+    P.req_t = req_t;
+    const int np = P.npix;
+    const int nst = lds_stokes<MODE>() ? P.nfreq : 1;
+    std::vector<double> ep(16 * (size_t)np), pkd(2 * (size_t)np), pkst(4 * (size_t)nst * np);
+    std::vector<int32_t> valid(np), steps(np), reason(np), pki(5 * (size_t)np);
+    std::vector<uint32_t> lists(2 * (size_t)np);
+    uint32_t counts[2] = {0, 0};
+    P.ep = ep.data();
+    P.ep_valid = valid.data();
+    P.ep_steps = steps.data();
+    P.ep_reason = reason.data();
+    P.pk_d = pkd.data();
+    P.pk_i = pki.data();
+    P.pk_st = pkst.data();
+    P.out = *out;
+    for (int pix = 0; pix < np; ++pix) {
+        uint32_t rc = 0, sa = 0, guard = 0;
+        const auto G = GeomT<geom_of<MODE>()>::geo(P, &rc);
+        LaneA A;
+        lane_a_init(P, pix, A);
+        while (!pass_a_tick<GP_64, geom_of<MODE>()>(P, G, A, sa, guard)) {
+        }
+        lane_a_store(P, A);
+    }
+    double mfbuf[4 * KP_MAX_FREQUENCIES], mfbuf2[4 * KP_MAX_FREQUENCIES];
+    const int wpb = std::max(1, slow->windows_per_block);
+    const bool decoupled = slow->step_mode == KP_SLOW_LIGHT_DECOUPLED;
+    kp_slow::BlockPlan plan;
+    int parity = 0, blocks = 0, rewinds = 0;
+    bool first = true;
+    for (;;) {
+        const int stop = kp_slow::block_stop(plan, wpb, n);
+        Params Q = P;
+        Q.snaps = cells + plan.res_lo;
+        Q.snap_t = slow->times + plan.res_lo;
+        Q.nsnap = stop - plan.res_lo + 1;
+        Q.slow_final = stop == end ? 1 : 0;
+        Q.list_in = first ? nullptr : lists.data() + (size_t)(1 - parity) * np;
+        Q.count_in = first ? nullptr : &counts[1 - parity];
+        Q.list_out = lists.data() + (size_t)parity * np;
+        Q.count_out = &counts[parity];
+        counts[parity] = 0;
+        const uint32_t work = first ? (uint32_t)np : counts[1 - parity];
+        for (uint32_t w = 0; w < work; ++w) {
+            uint32_t rc = 0, c1 = 0, c2 = 0, sb = 0, guard = 0;
+            const auto G = GeomT<geom_of<MODE>()>::geo(Q, &rc);
+            LaneB B;
+            B.mf = LdsVec<double>{mfbuf, 1};
+            B.mf2 = LdsVec<double>{mfbuf2, 1};
+            bool has = true;
+            if (first) {
+                has = lane_b_init<MODE>(Q, (int)w, B);
+            } else {
+                lane_b_resume<MODE>(Q, (int)Q.list_in[w], B);
+            }
+            while (has) {
+                if (pass_b_tick<false, GP_64, MODE>(Q, G, B, c1, c2, sb, guard)) {
+                    lane_b_finish<false, MODE>(Q, B, 1);
+                    break;
+                }
+                if (B.phase == PH_PARK) {
+                    lane_b_park<MODE>(Q, B);
+                    Q.list_out[counts[parity]++] = (uint32_t)B.pixel;
+                    break;
+                }
+            }
+        }
+        ++blocks;
+        const uint32_t parked = counts[parity];
+        if (parked == 0) break;
+        if (stop == end && !decoupled) return KP_INVALID_ARGUMENT;
+        if (blocks > 100000) return KP_INVALID_ARGUMENT;
+        first = false;
+        parity ^= 1;
+        double earliest = slow->times[end];
+        if (decoupled)
+            for (uint32_t i = 0; i < parked; ++i)
+                earliest = std::min(earliest, req_t[lists[(size_t)(1 - parity) * np + i]]);
+        if (kp_slow::advance_plan(plan, stop, decoupled, earliest, slow->times, n)) ++rewinds;
+    }
+    if (waiting) *waiting = counters[C_WAIT];
+    if (nblocks) *nblocks = blocks;
+    if (nrewinds) *nrewinds = rewinds;
+    return KP_OK;
+}
+
+// Slow-light lanes in blocks of seq->windows_per_block snapshot pairs.
+extern "C" kp_status kp_emul_run_grmhd_slow_light_blocks(const kp_pass_a_params* pa, const kp_grmhd_model* gm,
+                                                         const kp_slow_light* seq, const double* freqs,
+                                                         int32_t nfreq, kp_image_data* out, double* req_t,
+                                                         unsigned long long* waiting, int32_t* blocks,
+                                                         int32_t* rewinds) {
+    FreqLists fl;
+    fl.nfreq = fl.nctrl = nfreq;
+    fl.freqs = fl.ctrl = freqs;
+    if (nonthermal(gm->emission_type))
+        return emul_slow_blocks<M_SLOW_NT_W>(pa, gm, seq, fl, out, req_t, waiting, blocks, rewinds);
+    return emul_slow_blocks<M_SLOW_W>(pa, gm, seq, fl, out, req_t, waiting, blocks, rewinds);
 }

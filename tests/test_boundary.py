@@ -29,6 +29,9 @@ STRUCTS = {
     "kp_run_options": abi.RunOptions,
     "kp_image_stats": abi.ImageStats,
     "kp_batch_job": abi.BatchJob,
+    "kp_slow_light": abi.SlowLight,
+    "kp_slow_light_stream": abi.SlowLightStream,
+    "kp_slow_light_schedule": abi.SlowLightSchedule,
 }
 
 

@@ -62,6 +62,38 @@ def test_iharm_round_trip(tmp_path, kio, snap):
         assert prims.shape == (m.n1, m.n2, m.n3, 10) and cls == h5lite.H5T_FLOAT and size == 4
 
 
+def test_iharm_sequence_loader(tmp_path, kio, snap):
+    """kp_iharm_load_snapshot, the streamed slow-light scheduler's loader for
+    dump files: each index reads its own file as kp_iharm_read_cells does; a
+    dump on another grid, a wrong cell count and an index outside the
+    sequence are rejected."""
+    import ctypes as C
+    cells, m = snap
+    paths = []
+    for k in range(3):
+        c = cells.copy()
+        c["prim"][:, 0] *= 1.0 + 0.25 * k
+        paths.append(tmp_path / f"dump_{k}.h5")
+        kio.write_iharm(paths[-1], m, c)
+    stream = kio.IharmStream(paths)
+    L = kio.library()
+    n = m.n1 * m.n2 * m.n3
+    for k in (2, 0, 1):
+        got = np.zeros(n, dtype=cells.dtype)
+        assert L.kp_iharm_load_snapshot(stream.user, k, C.c_void_p(got.ctypes.data), n) == abi.KP_OK
+        _, ref, _ = kio.read_iharm(paths[k])
+        assert np.array_equal(got["prim"], ref["prim"]) and np.all(got["derived"] == 0.0)
+        assert np.array_equal(got["prim"][:, 0], (cells["prim"][:, 0] * (1.0 + 0.25 * k)).astype(np.float32))
+    buf = np.zeros(n, dtype=cells.dtype)
+    assert L.kp_iharm_load_snapshot(stream.user, 3, C.c_void_p(buf.ctypes.data), n) == abi.KP_INVALID_ARGUMENT
+    assert L.kp_iharm_load_snapshot(stream.user, 0, C.c_void_p(buf.ctypes.data), n - 1) == abi.KP_INVALID_ARGUMENT
+    m2 = abi.GRMHDModel.from_buffer_copy(m)
+    m2.dx1 = m.dx1 * 1.5
+    kio.write_iharm(paths[1], m2, cells)
+    assert L.kp_iharm_load_snapshot(stream.user, 1, C.c_void_p(buf.ctypes.data), n) == abi.KP_INVALID_ARGUMENT
+    assert b"common coordinate mapping" in image.library().kp_last_error()
+
+
 def test_iharm_dump_feeds_the_same_image(tmp_path, kio, snap):
     """Derived scalars and the oracle GRMHD image from the dump equal those
     of the in-memory snapshot (the dump keeps the FP32 primitives exactly)."""

@@ -43,6 +43,10 @@ def emul():
                                                      P(C.c_double), C.c_int32, P(abi.ImageData), P(C.c_double),
                                                      P(C.c_ulonglong)]
     L.kp_emul_run_grmhd_slow_light_image.restype = C.c_int32
+    L.kp_emul_run_grmhd_slow_light_blocks.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
+                                                      P(C.c_double), C.c_int32, P(abi.ImageData), P(C.c_double),
+                                                      P(C.c_ulonglong), P(C.c_int32), P(C.c_int32)]
+    L.kp_emul_run_grmhd_slow_light_blocks.restype = C.c_int32
     return L
 
 
@@ -188,6 +192,109 @@ def test_fluid_time_interpolation_four_velocity_normalisation(seq):
             assert min(vals[0], vals[2]) * 0.2 < vals[1] < max(vals[0], vals[2]) * 5.0
             emitting += 1
     assert emitting >= 10
+
+
+def _emul_blocks(L, p, model, cells, times, t_obs, mode, interp, windows_per_block, freqs=None):
+    sq, keep = oracle.slow_light_sequence(cells, times, t_obs, mode, interp, windows_per_block)
+    npix = p.camera.nx * p.camera.ny
+    f = np.array([model.freq_cgs] if freqs is None else freqs, dtype=np.float64)
+    arrays = oracle.alloc_image(npix)
+    for n in STOKES:
+        arrays[n] = np.zeros(len(f) * npix)
+    img = oracle.image_struct(arrays)
+    req = np.zeros(npix)
+    w = C.c_ulonglong()
+    blocks, rewinds = C.c_int32(), C.c_int32()
+    assert L.kp_emul_run_grmhd_slow_light_blocks(C.byref(p), C.byref(model), C.byref(sq),
+                                                 f.ctypes.data_as(P(C.c_double)), len(f), C.byref(img),
+                                                 req.ctypes.data_as(P(C.c_double)), C.byref(w), C.byref(blocks),
+                                                 C.byref(rewinds)) == 0
+    return arrays, req, w.value, blocks.value, rewinds.value
+
+
+def test_oracle_block_stepping_in_blocks_of_one_pair_is_snapshot_stepping(seq):
+    """Blocks of windows_per_block snapshot pairs (the batch scheduler,
+    slow_light_batch_driver.hpp:211-340): block stepping in blocks of one pair
+    cuts every step at each snapshot time and samples that pair, which is
+    snapshot stepping, so the two images agree bit for bit; blocks as long as
+    the sequence are the single block.  Two-pair blocks cut at alternate
+    snapshot times only: a different image, whose Stokes I stays within
+    1e-3 of the peak (the linear polarisation of this model moves far more
+    with the step layout; the next test holds two-pair blocks to the oracle's
+    own two-pair image instead)."""
+    cells, times, model, p = seq
+    run = lambda mode, w: oracle.run_grmhd_slow_light_image(  # noqa: E731
+        p, model, cells, times, T_OBS, step_mode=mode, windows_per_block=w)[0]
+    snap = run(abi.SLOW_LIGHT_SNAPSHOT, 0)
+    one = run(abi.SLOW_LIGHT_BLOCK, 1)
+    whole, longest = run(abi.SLOW_LIGHT_BLOCK, 0), run(abi.SLOW_LIGHT_BLOCK, len(times) - 1)
+    two = run(abi.SLOW_LIGHT_BLOCK, 2)
+    for k in (*STOKES, "steps", "reason", "pass_a_steps"):
+        assert np.array_equal(one[k], snap[k]), k
+        assert np.array_equal(longest[k], whole[k]), k
+    assert not np.array_equal(two["steps"], whole["steps"])
+    assert not np.array_equal(two["steps"], one["steps"])
+    peak = float(np.max(whole["image_i"]))
+    assert np.max(np.abs(two["image_i"] - whole["image_i"])) <= 1e-3 * peak
+
+
+@pytest.mark.parametrize("mode,wpb", [(abi.SLOW_LIGHT_DECOUPLED, 1), (abi.SLOW_LIGHT_DECOUPLED, 2),
+                                      (abi.SLOW_LIGHT_SNAPSHOT, 2), (abi.SLOW_LIGHT_SNAPSHOT, 1)])
+def test_emulated_blocks_equal_the_single_block_lanes(seq, emul, mode, wpb):
+    """The kernel's park and resume code under the library's block loop
+    (CPU build): decoupled and snapshot images do not depend on the block
+    length, so rays parked at block ends or waiting for a later block and
+    resumed in the next launch give the single-block lanes' image bit for
+    bit -- Stokes, step counts and reasons."""
+    cells, times, model, p = seq
+    ref, req0, w0 = _emul_run(emul, p, model, cells, times, T_OBS, mode, abi.SLOW_LIGHT_FLUID)
+    got, req, w, blocks, rewinds = _emul_blocks(emul, p, model, cells, times, T_OBS, mode, abi.SLOW_LIGHT_FLUID, wpb)
+    assert blocks == -(-(len(times) - 1) // wpb) and rewinds == 0
+    assert w == w0 == 0
+    for k in (*STOKES, "steps", "reason", "pass_a_steps", "closure_x", "final_null"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def test_emulated_block_stepping_in_blocks_matches_oracle(seq, emul):
+    """Block stepping in two-pair blocks: the emulated lanes against the
+    oracle's blocks at the parity bar, and in one-pair blocks bit for bit
+    equal to the emulated snapshot stepping."""
+    cells, times, model, p = seq
+    got, _, w, blocks, _ = _emul_blocks(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_BLOCK,
+                                        abi.SLOW_LIGHT_FLUID, 2)
+    assert w == 0 and blocks == 3
+    run = lambda q, t=T_OBS: oracle.run_grmhd_slow_light_image(  # noqa: E731
+        q, model, cells, times, t, step_mode=abi.SLOW_LIGHT_BLOCK, windows_per_block=2)[0]
+    ref = run(p)
+    cond = parity.sensitivity(run, p, ref, parity.time_shifts(lambda t: run(p, t), T_OBS))
+    print(parity.assert_parity_grid(got, ref, 1, "emulated block stepping, two-pair blocks", cond,
+                                    time_cut_steps=1), flush=True)
+    one, _, _, _, _ = _emul_blocks(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_BLOCK, abi.SLOW_LIGHT_FLUID, 1)
+    snap, _, _ = _emul_run(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_SNAPSHOT, abi.SLOW_LIGHT_FLUID)
+    for k in (*STOKES, "steps", "reason"):
+        assert np.array_equal(one[k], snap[k]), k
+
+
+def test_emulated_blocks_waiting_rays_outside_the_sequence(seq, emul):
+    """Decoupled rays needing a time before the first snapshot end as
+    slow_light_time_exhausted with the time they needed, in blocks as in the
+    single block; two frequencies on shared geodesics park and resume with
+    their Stokes vectors."""
+    cells, times, model, p = seq
+    ref, req0, w0 = _emul_run(emul, p, model, cells, times, 230.0, abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_FLUID)
+    got, req, w, _, _ = _emul_blocks(emul, p, model, cells, times, 230.0, abi.SLOW_LIGHT_DECOUPLED,
+                                     abi.SLOW_LIGHT_FLUID, 1)
+    assert w == w0 > 0
+    assert np.array_equal(np.isnan(req), np.isnan(req0))
+    assert np.array_equal(req[~np.isnan(req)], req0[~np.isnan(req0)])
+    for k in (*STOKES, "steps", "reason"):
+        assert np.array_equal(got[k], ref[k]), k
+    freqs = [86e9, 230e9]
+    ref2, _, _ = _emul_run(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_FLUID, freqs)
+    got2, _, _, _, _ = _emul_blocks(emul, p, model, cells, times, T_OBS, abi.SLOW_LIGHT_DECOUPLED,
+                                    abi.SLOW_LIGHT_FLUID, 2, freqs)
+    for k in (*STOKES, "steps", "reason"):
+        assert np.array_equal(got2[k], ref2[k]), k
 
 
 def test_identical_snapshots_reproduce_the_static_image(seq):
