@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define KP_ABI_VERSION 4
+#define KP_ABI_VERSION 5
 
 /* Largest frequency list of one shared-geodesic image (the reference's
  * KPOLARIS_MAX_FREQUENCIES build option, CMakeLists.txt:14). */
@@ -307,8 +307,13 @@ typedef struct kp_image_stats {
     int64_t rhs_evals;          /* transport_rhs evaluations, both passes */
     int64_t pass_a_rhs_evals;   /* transport_rhs evaluations in pass A */
     int64_t coefficient_evals;  /* plasma coefficient evaluations */
-    int64_t transfer_steps;     /* Strang transfer steps */
+    int64_t transfer_steps;     /* Strang transfer steps computed */
     int64_t fp64_guard_steps;   /* mixed precision: steps taken in FP64 */
+    int64_t vacuum_steps;       /* pass-B steps of the vacuum leg (outside the
+                                 * emitting region, moving outward): their
+                                 * transfer step is the identity and is not
+                                 * computed; transfer_steps + vacuum_steps is
+                                 * the reference's transfer-step count */
 } kp_image_stats;
 
 /* ABI version and last error message of the calling thread. */

@@ -134,6 +134,15 @@ bool staged_run(const kp_run_options& opt, bool riaf) {
            opt.guard_mode == 2 && opt.fp64_radius > 0.0 && riaf;
 }
 
+// Single-frequency Boyer-Lindquist pass B in two legs (radiating leg, then
+// vacuum leg from the emitting region's edge to the camera): persistent split
+// launches with an outer radius beyond every photon orbit (r > 4 M), where an
+// outgoing ray never turns back.
+bool vacuum_legs(const kp_run_options& opt, bool single_bl, double outer, double mass) {
+    return single_bl && opt.persistent && opt.split_transport != 0 && !staged_run(opt, true) &&
+           outer >= 5.0 * mass;
+}
+
 #define KP_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
@@ -318,6 +327,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (slow && source && !windowed)
         return kp_internal::fail(KP_UNSUPPORTED, "slow-light blocks are built for images, not gradients");
     const bool unsplit = opt.split_transport == 0 && !windowed;
+    // single-frequency images: pass B in a radiating and a vacuum leg
+    const bool legs = !noalloc && mode == M_SINGLE && vacuum_legs(opt, true, P.outer, P.mass);
     // Staged mixed precision (guard_mode 2, persistent split kernels): an
     // FP32-geodesic launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
     // continues them and parks those back that reach r >= back_radius, then
@@ -325,7 +336,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const bool staged = staged_run(opt, mode == M_SINGLE || mode == M_MF);
     const int nst = mf ? P.nfreq : 1;
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
-    const WsLayout W = workspace_layout(np, (size_t)nst, staged || slow != nullptr, nsnap, probe != nullptr);
+    const WsLayout W = workspace_layout(np, (size_t)nst, staged || slow != nullptr || legs, nsnap, probe != nullptr);
     void* ws = nullptr;
     if (noalloc) {
         // the static scratch array: no staged lists, no timing events
@@ -360,8 +371,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     P.ep_reason = reinterpret_cast<int32_t*>(base + W.reason);
     P.queue = reinterpret_cast<uint32_t*>(base + W.queue);
     P.counters = reinterpret_cast<unsigned long long*>(base + W.counters);
-    uint32_t* lists = (staged || windowed) ? reinterpret_cast<uint32_t*>(base + W.lists) : nullptr;
-    if (staged || windowed) {
+    uint32_t* lists = (staged || windowed || legs) ? reinterpret_cast<uint32_t*>(base + W.lists) : nullptr;
+    if (staged || windowed || legs) {
         P.pk_d = reinterpret_cast<double*>(base + W.pkd);
         P.pk_i = reinterpret_cast<int32_t*>(base + W.pki);
         P.pk_st = reinterpret_cast<double*>(base + W.pkst);
@@ -578,6 +589,21 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     } else if (P.npix > 0 && !unsplit) {
         if (staged) {
             st = run_staged(true, 12);
+        } else if (legs) {
+            // radiating leg: rays that leave the emitting region outward park
+            // (queue slot 2 counts them); the vacuum leg takes that list
+            Params Q = P;
+            Q.fetch = P.queue + 1;
+            Q.list_out = lists;
+            Q.count_out = P.queue + 2;
+            st = launch(kpsel::pass_b_legs(mixed, gp_plain, false), smem_b, Q);
+            if (st == KP_OK) {
+                Params V = P;
+                V.fetch = P.queue + 3;
+                V.list_in = lists;
+                V.count_in = P.queue + 2;
+                st = launch(kpsel::pass_b_legs(mixed, gp_plain, true), smem_b, V);
+            }
         } else {
             Params Q = P;
             Q.fetch = P.queue + 1;
@@ -611,6 +637,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         stats->pass_a_steps = (int64_t)cnt[C_STEPS_A];
         stats->pass_b_steps = (int64_t)cnt[C_STEPS_B];
         stats->fp64_guard_steps = (int64_t)cnt[C_GUARD];
+        stats->vacuum_steps = (int64_t)cnt[C_VACUUM];
     }
     if (probe) {
         // SlowLightTimeProbeHostResult reduction (slow_light_driver.hpp:733-778)
@@ -864,6 +891,7 @@ extern "C" kp_status kp_batch_read_stats(kp_batch* b, kp_image_stats* stats) {
     stats->pass_a_steps = (int64_t)cnt[C_STEPS_A];
     stats->pass_b_steps = (int64_t)cnt[C_STEPS_B];
     stats->fp64_guard_steps = (int64_t)cnt[C_GUARD];
+    stats->vacuum_steps = (int64_t)cnt[C_VACUUM];
     return KP_OK;
 }
 
@@ -914,8 +942,12 @@ extern "C" kp_status kp_image_workspace_size(int64_t npix, int32_t nfreq, int32_
     kp_run_options opt;
     kp_run_options_default(&opt);
     if (options) opt = *options;
-    const WsLayout W = workspace_layout((size_t)npix, (size_t)std::max(nfreq, 1), staged_run(opt, nsnap == 0),
-                                        (size_t)nsnap, false);
+    // park buffers: staged mixed precision, slow-light blocks, and the two
+    // pass-B legs of single-frequency images (whose outer radius the query
+    // does not know)
+    const bool parks = staged_run(opt, nsnap == 0) || nsnap > 0 ||
+                       (nsnap == 0 && nfreq <= 1 && vacuum_legs(opt, true, 1.0, 0.0));
+    const WsLayout W = workspace_layout((size_t)npix, (size_t)std::max(nfreq, 1), parks, (size_t)nsnap, false);
     *bytes = (uint64_t)W.total;
     return KP_OK;
 }

@@ -90,17 +90,28 @@ enum Mode : int {
     // sequence, its rays parking at the block's end or while they wait for a
     // time another block holds; instantiations of their own, so the kernels
     // of whole sequences carry none of the parking code
-    M_SLOW_W = 10, M_SLOW_NT_W = 11
+    M_SLOW_W = 10, M_SLOW_NT_W = 11,
+    // the single-frequency Boyer-Lindquist pass B in two legs: M_SINGLE_V is
+    // M_SINGLE until the ray leaves the emitting region outward, where it
+    // parks; M_VAC continues it to the camera with no radiation code at all
+    // (outside the region every coefficient is zero and the transfer step is
+    // the identity), so no wave of either kernel carries lanes of the other leg
+    M_SINGLE_V = 12, M_VAC = 13
 };
 
 // The mode whose lane logic a mode runs (a nonthermal mode runs its thermal
 // one, a windowed slow-light mode the slow-light one).
 constexpr int base_mode(int m) {
     return m == M_MF_NT ? M_MF
-                        : (m == M_GRMHD_NT ? M_GRMHD : ((m == M_SLOW_NT || m == M_SLOW_W || m == M_SLOW_NT_W) ? M_SLOW : m));
+           : (m == M_GRMHD_NT ? M_GRMHD
+                              : ((m == M_SLOW_NT || m == M_SLOW_W || m == M_SLOW_NT_W) ? M_SLOW
+                                                                                         : (m == M_SINGLE_V ? M_SINGLE : m)));
 }
 constexpr bool nt_mode(int m) { return m == M_MF_NT || m == M_GRMHD_NT || m == M_SLOW_NT || m == M_SLOW_NT_W; }
 constexpr bool win_mode(int m) { return m == M_SLOW_W || m == M_SLOW_NT_W; }
+// the radiating leg that parks at the vacuum boundary, and the vacuum leg
+constexpr bool vpark_mode(int m) { return m == M_SINGLE_V; }
+constexpr bool vac_mode(int m) { return m == M_VAC; }
 
 // Coordinates of the geodesics: Boyer-Lindquist (analytic RIAF) or FMKS
 // (GRMHD grids, whose pass B carries one Stokes vector per frequency in LDS).
@@ -130,7 +141,8 @@ enum Counter : int {
     C_STEPS_B = 5,
     C_GUARD = 6,
     C_WAIT = 7,  // slow light: rays that needed a time outside the snapshots
-    C_COUNT = 8
+    C_VACUUM = 8,  // pass-B vacuum leg: steps whose transfer is the identity
+    C_COUNT = 9
 };
 
 struct Params {
@@ -315,7 +327,8 @@ KP_DEV bool fetch_pixel(uint32_t* counter, bool need, uint32_t npix, int& pixel)
 
 // Wave-level sum of per-lane counters, one atomic per counter per wave.
 KP_DEV void flush_counters(unsigned long long* counters, int rhs_slot, uint32_t rhs, uint32_t coeff,
-                           uint32_t transfer, uint32_t steps_a, uint32_t steps_b, uint32_t guard) {
+                           uint32_t transfer, uint32_t steps_a, uint32_t steps_b, uint32_t guard,
+                           int transfer_slot = C_TRANSFER) {
     unsigned long long v[6] = {rhs, coeff, transfer, steps_a, steps_b, guard};
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -326,7 +339,7 @@ KP_DEV void flush_counters(unsigned long long* counters, int rhs_slot, uint32_t 
     if (__lane_id() == 0) {
         atomicAdd(&counters[rhs_slot], v[0]);
         atomicAdd(&counters[C_COEFF], v[1]);
-        atomicAdd(&counters[C_TRANSFER], v[2]);
+        atomicAdd(&counters[transfer_slot], v[2]);
         atomicAdd(&counters[C_STEPS_A], v[3]);
         atomicAdd(&counters[C_STEPS_B], v[4]);
         atomicAdd(&counters[C_GUARD], v[5]);
@@ -1038,6 +1051,25 @@ template <> struct Radiation<true, M_SINGLE> {
     }
 };
 
+template <> struct Radiation<false, M_SINGLE_V> : Radiation<false, M_SINGLE> {};
+template <> struct Radiation<true, M_SINGLE_V> : Radiation<true, M_SINGLE> {};
+// The vacuum leg returns before any radiation code; this empty form only
+// completes the tick's template.
+struct RadiationNone {
+    KP_DEV void zero() {}
+    KP_DEV bool valid() const { return false; }
+    KP_DEV uint32_t eval(const Params&, const St<double>&) { return 0; }
+    KP_DEV void at(const Params&, double, Coeffs<double>& c) const { coeffs_zero(c); }
+    KP_DEV uint32_t control(const Params&, double, double, double, int& req, int& pred) const {
+        req = 1;
+        pred = 1;
+        return 0;
+    }
+    KP_DEV uint32_t transfer(const Params&, LaneB&, double) const { return 0; }
+};
+template <> struct Radiation<false, M_VAC> : RadiationNone {};
+template <> struct Radiation<true, M_VAC> : RadiationNone {};
+
 // Thermal (NT false) and kappa / power-law (NT true) coefficient evaluation;
 // with NT false the nonthermal branch is compiled out.
 template <bool NT> struct RadiationMF {
@@ -1412,6 +1444,15 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
     const double eff_min = absv(P.min_step);
     if (L.phase == PH_IDLE) {
         if (L.reached || L.steps >= P.max_steps) return true;
+        if constexpr (vpark_mode(MODE)) {
+            // outside the emitting region and moving outward (k^r > 0) at
+            // r >= outer, which the host keeps beyond every photon orbit: no
+            // later sample radiates, the vacuum leg takes the ray over
+            if (GT::radius(L.s[1]) >= P.outer && L.s[5] > 0.0) {
+                L.phase = PH_PARK;
+                return false;
+            }
+        }
         if constexpr (SLOW) {
             if (!L.vac && P.slow_mode != KP_SLOW_LIGHT_DECOUPLED) {
                 while (!L.vac && P.t_obs + L.s[0] >= slow_upper(P, L)) slow_window_end(P, L);
@@ -1561,6 +1602,18 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
         }
     }
     L.phase = PH_IDLE;
+    if constexpr (vac_mode(MODE)) {
+        // no sample radiates: the substep counts are 1, the cap is max_step
+        // and the transfer step leaves the Stokes vector as it is (counted,
+        // not computed)
+        L.cap = P.max_step;
+        if (L.h > 0.0) transfer_acc += 1;
+        st_store(L.s, next);
+        L.steps += 1;
+        steps_acc += 1;
+        if (L.crosses) L.reached = 1;
+        return false;
+    }
     const double h = L.h;
     const double old_r = GT::radius(L.s[1]), sample_r = GT::radius(sample.v[1]), next_r = GT::radius(next.v[1]);
     if constexpr (SLOW) {
@@ -1716,7 +1769,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
                 if (!__ballot(need)) break;
                 int idx = 0;
                 if (fetch_pixel(P.fetch, need, limit, idx)) {
-                    if ((MIXED || win_mode(MODE)) && P.list_in) {
+                    if ((MIXED || win_mode(MODE) || vac_mode(MODE)) && P.list_in) {
                         lane_b_resume<MODE>(P, (int)P.list_in[idx], L);
                         has = true;
                     } else {
@@ -1736,8 +1789,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
                 }
                 append_parked(P, park, L.pixel);
             }
-            if constexpr (win_mode(MODE)) {
-                // windowed slow light: a ray that parks leaves the launch
+            if constexpr (win_mode(MODE) || vpark_mode(MODE)) {
+                // windowed slow light, radiating leg: a ray that parks leaves the launch
                 bool park = false;
                 if (has) {
                     if (pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
@@ -1763,7 +1816,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
             lane_b_finish<MIXED, MODE>(P, L, 1);
         }
     }
-    flush_counters(P.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard);
+    flush_counters(P.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard,
+                   vac_mode(MODE) ? C_VACUUM : C_TRANSFER);
 }
 
 // ---------------------------------------------------------------------------
@@ -1947,5 +2001,6 @@ const void* unsplit_bl(bool mixed, int gp, int mode);
 const void* unsplit_fmks(int mode);
 const void* pass_b_stan(bool persistent);
 const void* pass_b_slow_windowed(int mode);
+const void* pass_b_legs(bool mixed, int gp, bool vacuum);
 const void* unsplit_slow(int mode);
 }  // namespace kpsel

@@ -178,6 +178,23 @@ const void* pass_b_slow_windowed(int mode) {
 }
 #endif
 
+#if KP_PART == 11
+// Single-frequency Boyer-Lindquist pass B in two legs (persistent): the
+// radiating leg parks rays at the vacuum boundary, the vacuum leg ends them.
+const void* pass_b_legs(bool mixed, int gp, bool vacuum) {
+    if (!mixed)
+        return vacuum ? (const void*)pass_b_kernel<true, false, GP_64, M_VAC>
+                      : (const void*)pass_b_kernel<true, false, GP_64, M_SINGLE_V>;
+    if (gp == GP_64)
+        return vacuum ? (const void*)pass_b_kernel<true, true, GP_64, M_VAC>
+                      : (const void*)pass_b_kernel<true, true, GP_64, M_SINGLE_V>;
+    if (gp == GP_GUARD)
+        return vacuum ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_VAC>
+                      : (const void*)pass_b_kernel<true, true, GP_GUARD, M_SINGLE_V>;
+    return nullptr;
+}
+#endif
+
 #if KP_PART == 9
 const void* unsplit_slow(int mode) {
     switch (mode) {

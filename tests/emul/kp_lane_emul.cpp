@@ -344,3 +344,78 @@ extern "C" kp_status kp_emul_run_grmhd_slow_light_blocks(const kp_pass_a_params*
         return emul_slow_blocks<M_SLOW_NT_W>(pa, gm, seq, fl, out, req_t, waiting, blocks, rewinds);
     return emul_slow_blocks<M_SLOW_W>(pa, gm, seq, fl, out, req_t, waiting, blocks, rewinds);
 }
+
+// The single-frequency pass B in two legs: each pixel's ray runs the
+// radiating leg (M_SINGLE_V) until it parks at the vacuum boundary, then the
+// vacuum leg (M_VAC) from its parked state, through the kernels' park and
+// resume code.  mixed != 0 selects the mixed-precision lanes (FP64
+// geodesics); legs = 0 runs the one-leg lanes (M_SINGLE) of the same
+// precision instead.  *parked counts the rays that took the vacuum leg.
+template <bool MIXED>
+static kp_status emul_legs(const kp_pass_a_params* pa, const kp_riaf_model* m, bool legs, kp_image_data* out,
+                           int64_t* parked) {
+    constexpr int GP = GP_64;
+    Params P;
+    kp_status st = prepare(pa, m, P);
+    if (st != KP_OK) return st;
+    if (MIXED) prepare_mixed(P);
+    unsigned long long counters[C_COUNT] = {};
+    P.counters = counters;
+    const int np = P.npix;
+    std::vector<double> ep(16 * (size_t)np), pkd(2 * (size_t)np), pkst(4 * (size_t)np);
+    std::vector<int32_t> valid(np), steps(np), reason(np), pki(5 * (size_t)np);
+    P.ep = ep.data();
+    P.ep_valid = valid.data();
+    P.ep_steps = steps.data();
+    P.ep_reason = reason.data();
+    P.pk_d = pkd.data();
+    P.pk_i = pki.data();
+    P.pk_st = pkst.data();
+    P.out = *out;
+    int64_t nparked = 0;
+    for (int pix = 0; pix < np; ++pix) {
+        uint32_t rc = 0, sa = 0, c1 = 0, c2 = 0, sb = 0, guard = 0;
+        const auto G = GeomT<G_BL>::geo(P, &rc);
+        LaneA A;
+        lane_a_init(P, pix, A);
+        while (!pass_a_tick<GP, G_BL>(P, G, A, sa, guard)) {
+        }
+        lane_a_store(P, A);
+        LaneB B;
+        if (!legs) {
+            if (lane_b_init<M_SINGLE>(P, pix, B)) {
+                while (!pass_b_tick<MIXED, GP, M_SINGLE>(P, G, B, c1, c2, sb, guard)) {
+                }
+                lane_b_finish<MIXED, M_SINGLE>(P, B, 1);
+            }
+            continue;
+        }
+        if (!lane_b_init<M_SINGLE_V>(P, pix, B)) continue;
+        bool vacuum = false;
+        while (true) {
+            if (pass_b_tick<MIXED, GP, M_SINGLE_V>(P, G, B, c1, c2, sb, guard)) {
+                lane_b_finish<MIXED, M_SINGLE_V>(P, B, 1);
+                break;
+            }
+            if (B.phase == PH_PARK) {
+                lane_b_park<M_SINGLE_V>(P, B);
+                vacuum = true;
+                break;
+            }
+        }
+        if (!vacuum) continue;
+        ++nparked;
+        LaneB V;
+        lane_b_resume<M_VAC>(P, pix, V);
+        while (!pass_b_tick<MIXED, GP, M_VAC>(P, G, V, c1, c2, sb, guard)) {
+        }
+        lane_b_finish<MIXED, M_VAC>(P, V, 1);
+    }
+    if (parked) *parked = nparked;
+    return KP_OK;
+}
+
+extern "C" kp_status kp_emul_run_riaf_image_legs(const kp_pass_a_params* pa, const kp_riaf_model* m, int32_t mixed,
+                                                 int32_t legs, kp_image_data* out, int64_t* parked) {
+    return mixed ? emul_legs<true>(pa, m, legs != 0, out, parked) : emul_legs<false>(pa, m, legs != 0, out, parked);
+}

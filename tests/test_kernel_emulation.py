@@ -39,6 +39,9 @@ def emul():
     L.kp_emul_run_riaf_image_gradient.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), dp,
                                                   C.POINTER(abi.ImageData), dp]
     L.kp_emul_run_riaf_image_gradient.restype = C.c_int32
+    L.kp_emul_run_riaf_image_legs.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.c_int32,
+                                              C.c_int32, C.POINTER(abi.ImageData), C.POINTER(C.c_int64)]
+    L.kp_emul_run_riaf_image_legs.restype = C.c_int32
     return L
 
 
@@ -257,3 +260,26 @@ def test_emulated_mixed_coefficients_stay_finite_at_underflow(emul):
                                                out) == 0
         vals = list(out)
         assert all(math.isfinite(v) for v in vals), (x, vals)
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_emulated_pass_b_legs_equal_the_one_leg_lanes(emul, mixed):
+    """Pass B in two legs (the radiating lanes park where the ray leaves the
+    emitting region outward, the vacuum lanes take it to the camera with no
+    radiation code): outside the region every coefficient is zero, the substep
+    counts are 1 and the transfer step is the identity, so the image equals
+    the one-leg lanes' bit for bit, FP64 and mixed."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=24, ny=24)
+    npix = p.camera.nx * p.camera.ny
+    res = []
+    for legs in (0, 1):
+        arrays = oracle.alloc_image(npix)
+        img = oracle.image_struct(arrays)
+        parked = C.c_int64()
+        assert emul.kp_emul_run_riaf_image_legs(C.byref(p), C.byref(m), mixed, legs, C.byref(img),
+                                                C.byref(parked)) == 0
+        res.append((arrays, parked.value))
+    (one, _), (two, parked) = res
+    assert parked > 0.5 * npix
+    for k in one:
+        assert np.array_equal(one[k], two[k]), k
