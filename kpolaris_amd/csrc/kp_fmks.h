@@ -12,6 +12,8 @@
 
 #include "kp_device.h"
 
+#include <cmath>
+
 namespace kpd {
 
 // FMKS map parameters (KerrFMKSMetric fields, kerr_fmks.hpp:11-19) and the
@@ -23,29 +25,47 @@ struct FmksMap {
     double inv_alpha1;  // 1 / (poly_alpha + 1)
     double dthj_c;      // 2 alpha norm / ((alpha + 1) xt)
     double d2thj_c;     // 4 norm alpha / (alpha + 1)
+    // the powers z^(alpha - k), k = 0, 1, 2, of the jet map: exponent and
+    // the factor a negative z contributes (-1 for an odd integer exponent)
+    double pw[3], pw_neg[3];
 };
+
+// fmks_power's sign rule (kerr_fmks.hpp:37-47) for exponent p and z < 0.
+inline double fmks_power_sign(double p) {
+    const double n = std::floor(p + 0.5);
+    const bool integer = std::fabs(p - n) < 1e-10 && n >= 0.0;
+    return (integer && std::fmod(n, 2.0) != 0.0) ? -1.0 : 1.0;
+}
 
 inline FmksMap fmks_map(double startx1, double hslope, double mks_smooth, double poly_alpha, double poly_xt,
                         double poly_norm) {
-    FmksMap f{startx1, hslope, mks_smooth, poly_alpha, poly_xt, poly_norm, 0.0, 0.0, 0.0, 0.0};
+    FmksMap f{startx1, hslope, mks_smooth, poly_alpha, poly_xt, poly_norm, 0.0, 0.0, 0.0, 0.0, {}, {}};
     f.inv_xt = 1.0 / poly_xt;
     f.inv_alpha1 = 1.0 / (poly_alpha + 1.0);
     f.dthj_c = (2.0 * poly_alpha * poly_norm) / ((poly_alpha + 1.0) * poly_xt);
     f.d2thj_c = 4.0 * poly_norm * poly_alpha / (poly_alpha + 1.0);
+    for (int k = 0; k < 3; ++k) {
+        f.pw[k] = poly_alpha - double(k);
+        f.pw_neg[k] = fmks_power_sign(f.pw[k]);
+    }
     return f;
 }
 
-// fmks_power (kerr_fmks.hpp:37-47): integer powers by repeated products.
-template <class T> KP_DEV T fmks_power(T x, T p) {
-    const int n = int(p + T(0.5));
-    if (absv(p - T(n)) < T(1e-10) && n >= 0) {
-        T y = T(1);
-        const T ax = absv(x);
-        for (int i = 0; i < n; ++i) y *= ax;
-        if (x < T(0) && (n % 2) == 1) y = -y;
-        return y;
-    }
-    return pow(absv(x), p);
+// fmks_power (kerr_fmks.hpp:37-47) for the three exponents alpha, alpha - 1
+// and alpha - 2 of the jet map: |z|^p as exp(p ln|z|) with |z| floored at
+// 1e-300 (so p = 0 gives 1), times the sign factor of an odd integer power
+// when z < 0.  The reference forms integer powers by repeated products and
+// others with pow; this form agrees with both to a few ulps (|alpha ln|z|| <= 3
+// on the jet map's range) and has no branch: a branch inside the transport
+// RHS, inlined into the eleven RHS evaluations of an adaptive attempt, made the
+// FMKS kernels spill 0.9-2 KB per lane, so everything here is arithmetic and
+// selects between computed values.
+template <class T> KP_DEV void fmks_powers(const FmksMap& f, T z, T& za, T& zam1, T& zam2) {
+    const T lz = log(maxv(absv(z), T(1e-300)));
+    const bool neg = z < T(0);
+    za = exp(T(f.pw[0]) * lz) * (neg ? T(f.pw_neg[0]) : T(1));
+    zam1 = exp(T(f.pw[1]) * lz) * (neg ? T(f.pw_neg[1]) : T(1));
+    zam2 = exp(T(f.pw[2]) * lz) * (neg ? T(f.pw_neg[2]) : T(1));
 }
 
 // theta_from_x2 and its first derivatives (kerr_fmks.hpp:49-80).
@@ -60,8 +80,9 @@ template <class T> KP_DEV FmksTheta<T> fmks_theta(const FmksMap& f, T x1, T x2) 
     const T thG = pi * x2 + ((T(1) - T(f.hslope)) / T(2)) * s2;
     const T y = T(2) * x2 - T(1);
     const T z = y * T(f.inv_xt);
-    const T za = fmks_power(z, T(f.poly_alpha));
-    const T zam1 = fmks_power(z, T(f.poly_alpha) - T(1));
+    T za, zam1, zam2;
+    fmks_powers(f, z, za, zam1, zam2);
+    (void)zam2;
     const T thJ = T(f.poly_norm) * y * (T(1) + za * T(f.inv_alpha1)) + T(0.5) * pi;
     const T expfac = exp(T(f.mks_smooth) * (T(f.startx1) - x1));
     const T dthG = pi + (T(1) - T(f.hslope)) * pi * c2;
@@ -166,16 +187,7 @@ template <class T> KP_DEV FmksConn<T> fmks_conn(T mass, T spin, const FmksMap& f
     const T z = y * T(f.inv_xt);
     const T expfac = exp(T(f.mks_smooth) * (T(f.startx1) - x1));
     T za, zam1, zam2;
-    if (absv(T(f.poly_alpha) - T(14)) < T(1e-12)) {
-        const T z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
-        zam2 = z8 * z4;
-        zam1 = zam2 * z;
-        za = zam2 * z2;
-    } else {
-        za = fmks_power(z, T(f.poly_alpha));
-        zam1 = fmks_power(z, T(f.poly_alpha) - T(1));
-        zam2 = fmks_power(z, T(f.poly_alpha) - T(2));
-    }
+    fmks_powers(f, z, za, zam1, zam2);
     T s2, c2;
     ksincos(T(2) * pi * x2, &s2, &c2);
     const T thG = pi * x2 + ((T(1) - T(f.hslope)) / T(2)) * s2;
