@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define KP_ABI_VERSION 5
+#define KP_ABI_VERSION 6
 
 /* Largest frequency list of one shared-geodesic image (the reference's
  * KPOLARIS_MAX_FREQUENCIES build option, CMakeLists.txt:14). */
@@ -314,6 +314,9 @@ typedef struct kp_image_stats {
                                  * transfer step is the identity and is not
                                  * computed; transfer_steps + vacuum_steps is
                                  * the reference's transfer-step count */
+    int64_t vacuum_rhs_evals;   /* transport_rhs evaluations of the vacuum leg
+                                 * (included in rhs_evals) */
+    double pass_b_vacuum_ms;    /* the vacuum leg's share of pass_b_ms */
 } kp_image_stats;
 
 /* ABI version and last error message of the calling thread. */

@@ -64,7 +64,7 @@ PRECISION_MIXED = 1
 FP64_RADIUS_DEFAULT = 2.5
 
 # include/kp_image.h KP_ABI_VERSION this mirror matches
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)
@@ -262,6 +262,7 @@ class ImageStats(C.Structure):
         ("rhs_evals", C.c_int64), ("pass_a_rhs_evals", C.c_int64),
         ("coefficient_evals", C.c_int64),
         ("transfer_steps", C.c_int64), ("fp64_guard_steps", C.c_int64), ("vacuum_steps", C.c_int64),
+        ("vacuum_rhs_evals", C.c_int64), ("pass_b_vacuum_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -428,7 +428,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     };
     // timing events, destroyed on every return path
     struct Events {
-        hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
         ~Events() {
             for (hipEvent_t x : e)
                 if (x) (void)hipEventDestroy(x);
@@ -436,7 +436,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     } events;
     hipEvent_t* ev = events.e;
     if (stats) {
-        for (int i = 0; i < 3; ++i) KP_HIP(hipEventCreate(&ev[i]));
+        for (int i = 0; i < 4; ++i) KP_HIP(hipEventCreate(&ev[i]));
     }
     {
         int nq = KP_NQUEUE;
@@ -597,6 +597,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             Q.list_out = lists;
             Q.count_out = P.queue + 2;
             st = launch(kpsel::pass_b_legs(mixed, gp_plain, false), smem_b, Q);
+            if (st == KP_OK && stats) KP_HIP(hipEventRecord(ev[3], stream));
             if (st == KP_OK) {
                 Params V = P;
                 V.fetch = P.queue + 3;
@@ -625,12 +626,20 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         if (unsplit) b_ms = 0.f;
         // windowed: the block launches without the host's waits between them
         if (windowed) b_ms = (float)blocks_ms;
+        float v_ms = 0.f;
+        if (legs && P.npix > 0) {
+            float r_ms = 0.f;
+            KP_HIP(hipEventElapsedTime(&r_ms, ev[1], ev[3]));
+            v_ms = b_ms - r_ms;
+        }
         std::memset(stats, 0, sizeof(*stats));
         stats->pass_a_ms = a_ms;
         stats->pass_b_ms = b_ms;
+        stats->pass_b_vacuum_ms = v_ms;
         stats->kernel_ms = double(a_ms) + double(b_ms);
         stats->rays = P.npix;
-        stats->rhs_evals = (int64_t)(cnt[C_RHS_A] + cnt[C_RHS_B]);
+        stats->rhs_evals = (int64_t)(cnt[C_RHS_A] + cnt[C_RHS_B] + cnt[C_RHS_V]);
+        stats->vacuum_rhs_evals = (int64_t)cnt[C_RHS_V];
         stats->pass_a_rhs_evals = (int64_t)cnt[C_RHS_A];
         stats->coefficient_evals = (int64_t)cnt[C_COEFF];
         stats->transfer_steps = (int64_t)cnt[C_TRANSFER];

@@ -142,7 +142,8 @@ enum Counter : int {
     C_GUARD = 6,
     C_WAIT = 7,  // slow light: rays that needed a time outside the snapshots
     C_VACUUM = 8,  // pass-B vacuum leg: steps whose transfer is the identity
-    C_COUNT = 9
+    C_RHS_V = 9,   // pass-B vacuum leg: RHS evaluations
+    C_COUNT = 10
 };
 
 struct Params {
@@ -1816,7 +1817,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
             lane_b_finish<MIXED, MODE>(P, L, 1);
         }
     }
-    flush_counters(P.counters, C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard,
+    flush_counters(P.counters, vac_mode(MODE) ? C_RHS_V : C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard,
                    vac_mode(MODE) ? C_VACUUM : C_TRANSFER);
 }
 

@@ -7,7 +7,12 @@
 // literals are compile-time constants and are not counted.  Prints JSON with
 // per-unit counts: one transport_rhs, one step-doubling attempt, one RK4 step,
 // one RIAF coefficient evaluation (Pandya and Dexter, averaged over sample
-// states), one Strang transfer step and one substep-count evaluation.
+// states), one Strang transfer step and one substep-count evaluation; for
+// GRMHD grids one FMKS transport_rhs, one FMKS step-doubling attempt and one
+// thermal coefficient evaluation of a sampled plasma state at one frequency
+// (Dexter and Pandya, averaged over eight frequencies 86-345 GHz and a few
+// plasma states; the grid sample and four-vector reconstruction are not
+// counted).
 #include <cmath>
 #include <cstdio>
 
@@ -85,6 +90,7 @@ static inline void sincos(CD a, CD* s, CD* c) {
 
 #include "kp_device.h"
 #include "kp_riaf.h"
+#include "kp_fmks.h"
 
 // the kernels' bounded-argument sine/cosine pair counts as two transcendentals
 KP_DEV void ksincos(CD a, CD* s, CD* c) { sincos(a, s, c); }
@@ -163,6 +169,37 @@ int main() {
             riaf_coefficients(M, s, c);
         }
         emit(fit == 1 ? "coefficients_pandya" : "coefficients_dexter", g_ops, 6.0);
+    }
+    {
+        // FMKS geometry of the synthetic GRMHD grids (poly_alpha 14, xt 0.82)
+        const FmksMap f = fmks_map(std::log(1.2), 0.3, 0.5, 14.0, 0.82, 0.5 * M_PI / (1.0 + 1.0 / 15.0 / std::pow(0.82, 14.0)));
+        const CD mf = CD::run(1.0), af = CD::run(0.9375);
+        GeoFmks<CD> GF{mf, af, &rhs_count, f};
+        const double v[16] = {0.0, std::log(7.0), 0.45, 0.3, 1.2, -0.2, 0.01, 0.02, 0.0, 0.01, 0.0, 0.0, 0.0, 0.0, 0.05, 0.1};
+        St<CD> s, d, half, two;
+        for (int i = 0; i < 16; ++i) s.v[i] = CD::run(v[i]);
+        g_ops = OpTally();
+        fmks_rhs_apply(mf, af, f, s, [&](int i, CD x) { d.v[i] = x; });
+        emit("fmks_rhs", g_ops, 1.0);
+        g_ops = OpTally();
+        CD err = doubling_attempt(GF, s, CD::run(0.1), half, two);
+        (void)err;
+        emit("fmks_doubling_attempt", g_ops, 1.0);
+        for (int fit : {1, 4}) {
+            g_ops = OpTally();
+            int n = 0;
+            const double states[3][4] = {{1e5, 3.0, 20.0, 0.7}, {3e4, 12.0, 5.0, 0.9}, {2e6, 0.8, 60.0, 0.4}};
+            for (const auto& st : states)
+                for (int k = 0; k < 8; ++k) {
+                    const double nu = 86e9 * std::pow(345.0 / 86.0, k / 7.0);
+                    Coeffs<CD> c;
+                    thermal_coeffs(fit, CD::run(nu), CD::run(st[0]), CD::run(st[1]), CD::run(st[2]),
+                                   CD::run(st[3]), CD::run(std::sqrt(1.0 - st[3] * st[3])), CD::run(0.99),
+                                   CD(1e-4), CD::run(1.0), CD::run(1.0), CD::run(1.0), c);
+                    ++n;
+                }
+            emit(fit == 1 ? "grmhd_coefficients_pandya" : "grmhd_coefficients_dexter", g_ops, double(n));
+        }
     }
     {
         Coeffs<CD> c;
