@@ -161,7 +161,7 @@ const void* pass_b_fn(bool persistent, bool mixed, int gp, int mode) {
     switch (mode) {
     case M_SINGLE: return kpsel::pass_b_single(persistent, mixed, gp);
     case M_MF: return kpsel::pass_b_mf(persistent, mixed, gp);
-    case M_GRMHD:
+    case M_GRMHD: return mixed ? kpsel::pass_b_grmhd_mixed(persistent) : kpsel::pass_b_fmks(persistent, mode);
     case M_GRMHD_NT:
     case M_PROBE: return mixed ? nullptr : kpsel::pass_b_fmks(persistent, mode);
     case M_SLOW:
@@ -173,6 +173,7 @@ const void* pass_b_fn(bool persistent, bool mixed, int gp, int mode) {
 
 const void* unsplit_fn(bool mixed, int gp, int mode) {
     const int b = base_mode(mode);
+    if (mode == M_GRMHD && mixed) return kpsel::unsplit_grmhd_mixed();
     if (b == M_GRMHD || b == M_SLOW || b == M_PROBE || mode == M_STAN) return mixed ? nullptr : kpsel::unsplit_fmks(mode);
     return kpsel::unsplit_bl(mixed, gp, mode);
 }
@@ -249,8 +250,10 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     FreqLists fl = fl_in;
     kp_status st;
     if (grmhd) {
-        if (mixed || perturbation || (tangent && !slow))
-            return kp_internal::fail(KP_UNSUPPORTED, "GRMHD images are built for FP64; gradients for slow-light sequences");
+        if (mixed && (slow || probe))
+            return kp_internal::fail(KP_UNSUPPORTED, "slow-light sequences and time probes are built for FP64");
+        if (perturbation || (tangent && !slow))
+            return kp_internal::fail(KP_UNSUPPORTED, "GRMHD gradients are built for slow-light sequences");
         st = prepare_grmhd(params, grmhd, P, probe == nullptr);
         if (st != KP_OK) return st;
         if (fl.nfreq == 0) {
@@ -270,10 +273,14 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             return kp_internal::fail(KP_INVALID_ARGUMENT, "fp64_radius must be non-negative");
         if (opt.guard_mode < 0 || opt.guard_mode > 2)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "guard_mode must be 0, 1 or 2");
-        prepare_mixed(P);
+        if (grmhd)
+            prepare_mixed_grmhd(P, grmhd->mbh_solar);
+        else
+            prepare_mixed(P);
         P.fp64_radius = opt.fp64_radius;
-        // every geodesic attempt of guard_mode 0 is FP64 and counts as a guard step
-        if (opt.guard_mode == 0) P.stage = 0;
+        // every geodesic attempt of guard_mode 0 is FP64 and counts as a guard
+        // step; FMKS geodesics are FP64 under every guard_mode
+        if (opt.guard_mode == 0 || grmhd) P.stage = 0;
     }
     const bool nonthermal = emission_nonthermal(P.nt.dist);
     if (nonthermal && !grmhd && fl.nfreq == 0) {
@@ -409,8 +416,9 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (st != KP_OK) return st;
     const size_t smem_a = lds_bytes(block, 0);
     // the single-pass frequency loop serves the FP64 multi-frequency, GRMHD
-    // and slow-light lanes
-    if (mixed || !(base_mode(mode) == M_MF || base_mode(mode) == M_GRMHD || base_mode(mode) == M_SLOW))
+    // and slow-light lanes and the mixed-precision GRMHD lanes
+    if ((mixed && mode != M_GRMHD) ||
+        !(base_mode(mode) == M_MF || base_mode(mode) == M_GRMHD || base_mode(mode) == M_SLOW))
         P.single_pass = 0;
     const size_t smem_b = mode == M_STAN ? lds_bytes(block, 0) : lds_bytes(block, mf ? P.nfreq : 0, P.single_pass ? 2 : 1);
     const int nblocks_flat = (int)((np + block - 1) / block);
@@ -474,7 +482,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         return KP_OK;
     };
     // guard_mode 0: FP64 geodesics with the FP32 radiation path
-    const int gp_plain = mixed ? (opt.guard_mode == 0 ? GP_64 : GP_GUARD) : GP_64;
+    const int gp_plain = mixed && !grmhd ? (opt.guard_mode == 0 ? GP_64 : GP_GUARD) : GP_64;
     if (P.npix > 0 && unsplit) {
         Params Q = P;
         Q.fetch = P.queue;

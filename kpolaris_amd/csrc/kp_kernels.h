@@ -1107,12 +1107,15 @@ template <bool NT> struct RadiationMF {
 template <> struct Radiation<false, M_MF> : RadiationMF<false> {};
 template <> struct Radiation<false, M_MF_NT> : RadiationMF<true> {};
 
-template <> struct Radiation<true, M_MF> {
+// Mixed-precision frequency lists: FP32 coefficients of each frequency from
+// one FP32 local plasma state, in nu^3-scaled units per unit |h|.
+struct RadiationScaledMF {
     RiafLocal<float> loc;
     KP_DEV void zero() { loc.valid = 0; }
-    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
-        riaf_local_scaled(P.rm, sample, loc);
-        return 0;
+    KP_DEV bool valid() const { return loc.valid; }
+    // output frequency f (single pass: also the control frequency f)
+    KP_DEV void at32(const Params& P, int f, Coeffs<float>& c) const {
+        riaf_coeffs_scaled_at(P.rm, loc, float(P.freqs[f]), P.bconst_f[f], c);
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
         req = 1;
@@ -1133,6 +1136,13 @@ template <> struct Radiation<true, M_MF> {
             mf_strang32(P, L, f, c, float(absv(h)));
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
+    }
+};
+
+template <> struct Radiation<true, M_MF> : RadiationScaledMF {
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        riaf_local_scaled(P.rm, sample, loc);
+        return 0;
     }
 };
 
@@ -1169,6 +1179,31 @@ template <bool NT> struct RadiationGRMHD {
 };
 template <> struct Radiation<false, M_GRMHD> : RadiationGRMHD<false> {};
 template <> struct Radiation<false, M_GRMHD_NT> : RadiationGRMHD<true> {};
+
+// Mixed-precision GRMHD (thermal fits): the grid gather, the native metric
+// and the four-vectors stay FP64 (grmhd_local); the local plasma state is
+// narrowed to FP32 and every frequency's coefficients follow the FP32
+// thermal fit in nu^3-scaled units, with the GRMHD model's constants in
+// P.rm (the sin(theta) floor 1e-4 of grmhd_grid.hpp:1210-1240).  The state
+// is never field-aligned here: the GRMHD form has no aligned special case.
+template <> struct Radiation<true, M_GRMHD> : RadiationScaledMF {
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        RiafLocal<double> d;
+        grmhd_local(P.grmhd, sample, d);
+        loc.valid = d.valid;
+        loc.aligned = 0;
+        loc.has_chi = d.has_chi;
+        loc.ne = float(d.ne);
+        loc.thetae = float(d.thetae);
+        loc.b = float(d.b);
+        loc.nu_scale = float(d.nu_scale);
+        loc.cos_theta = float(d.cos_theta);
+        loc.sin_theta = float(d.sin_theta);
+        loc.cos2chi = float(d.cos2chi);
+        loc.sin2chi = float(d.sin2chi);
+        return 0;
+    }
+};
 
 // Slow light: the plasma state of snapshot pair (k, k+1) at time fraction f.
 // Fluid interpolation mixes the sampled fluid before the reconstruction (lo
@@ -1680,7 +1715,8 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
                             ? P.max_rs : 0.0;
     int required, pred;
     constexpr bool SHARED = !MIXED && (base_mode(MODE) == M_MF || base_mode(MODE) == M_GRMHD || base_mode(MODE) == M_SLOW);
-    if constexpr (SHARED) {
+    constexpr bool SHARED32 = MIXED && MODE == M_GRMHD;
+    if constexpr (SHARED || SHARED32) {
         if (P.single_pass) {
             // the control frequencies are the output frequencies: one
             // coefficient evaluation per frequency serves the substep control
@@ -1690,17 +1726,36 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
             pred = 1;
             const bool valid = rad.valid();
             for (int f = 0; f < P.nfreq; ++f) {
-                Coeffs<double> c;
-                rad.at(P, P.freqs[f], c);
-                required = maxv(required, substep_count(c, h, P.dls_f[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
-                pred = maxv(pred, substep_count(c, L.h_current, P.dls_f[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
-                if (h > 0.0) {
-                    Stokes<double> st = {L.mf[4 * f], L.mf[4 * f + 1], L.mf[4 * f + 2], L.mf[4 * f + 3]};
-                    strang_step(st, c, absv(h) * P.dls_f[f]);
-                    L.mf2.set(4 * f, st.I);
-                    L.mf2.set(4 * f + 1, st.Q);
-                    L.mf2.set(4 * f + 2, st.U);
-                    L.mf2.set(4 * f + 3, st.V);
+                if constexpr (SHARED32) {
+                    // FP32 coefficients per unit |h| (the scaled units of
+                    // Radiation<true, M_GRMHD>::control and transfer)
+                    Coeffs<float> c;
+                    rad.at32(P, f, c);
+                    const Coeffs<double> d = widen(c);
+                    required = maxv(required, substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
+                    pred = maxv(pred, substep_count(d, L.h_current, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
+                    if (h > 0.0) {
+                        Stokes<float> st = {float(L.mf[4 * f]), float(L.mf[4 * f + 1]), float(L.mf[4 * f + 2]),
+                                            float(L.mf[4 * f + 3])};
+                        strang_step32(st, c, float(absv(h)));
+                        L.mf2.set(4 * f, st.I);
+                        L.mf2.set(4 * f + 1, st.Q);
+                        L.mf2.set(4 * f + 2, st.U);
+                        L.mf2.set(4 * f + 3, st.V);
+                    }
+                } else {
+                    Coeffs<double> c;
+                    rad.at(P, P.freqs[f], c);
+                    required = maxv(required, substep_count(c, h, P.dls_f[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+                    pred = maxv(pred, substep_count(c, L.h_current, P.dls_f[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
+                    if (h > 0.0) {
+                        Stokes<double> st = {L.mf[4 * f], L.mf[4 * f + 1], L.mf[4 * f + 2], L.mf[4 * f + 3]};
+                        strang_step(st, c, absv(h) * P.dls_f[f]);
+                        L.mf2.set(4 * f, st.I);
+                        L.mf2.set(4 * f + 1, st.Q);
+                        L.mf2.set(4 * f + 2, st.U);
+                        L.mf2.set(4 * f + 3, st.V);
+                    }
                 }
             }
             if (valid) coeff_acc += (uint32_t)P.nfreq;
@@ -2004,4 +2059,6 @@ const void* pass_b_stan(bool persistent);
 const void* pass_b_slow_windowed(int mode);
 const void* pass_b_legs(bool mixed, int gp, bool vacuum);
 const void* unsplit_slow(int mode);
+const void* pass_b_grmhd_mixed(bool persistent);
+const void* unsplit_grmhd_mixed();
 }  // namespace kpsel

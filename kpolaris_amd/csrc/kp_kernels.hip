@@ -195,6 +195,17 @@ const void* pass_b_legs(bool mixed, int gp, bool vacuum) {
 }
 #endif
 
+#if KP_PART == 12
+// Mixed-precision GRMHD pass B (FP64 FMKS geodesics, FP32 thermal
+// coefficients), split and unsplit.
+const void* pass_b_grmhd_mixed(bool persistent) {
+    return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_GRMHD>
+                      : (const void*)pass_b_kernel<false, true, GP_64, M_GRMHD>;
+}
+
+const void* unsplit_grmhd_mixed() { return (const void*)unsplit_kernel<true, GP_64, M_GRMHD>; }
+#endif
+
 #if KP_PART == 9
 const void* unsplit_slow(int mode) {
     switch (mode) {

@@ -214,6 +214,30 @@ static void prepare_mixed(Params& P) {
     P.spin_f = float(P.spin);
 }
 
+// The constants of the mixed-precision GRMHD coefficients (thermal fits in
+// FP32, nu^3-scaled units): the GRMHD model's fit, scales and polarisation
+// cap, the sin(theta) floor 1e-4 of its coefficient evaluation
+// (grmhd_grid.hpp:1210-1240), and the length unit GM/c^2 of mbh_solar
+// (the dl/dlambda of frequency nu is that length over nu).
+static void prepare_mixed_grmhd(Params& P, double mbh_solar) {
+    const GrmhdDev& G = P.grmhd;
+    RiafDev<float>& f = P.rm.f;
+    f = RiafDev<float>{};
+    f.fit = G.fit;
+    f.faraday_rotation = G.faraday_rotation;
+    f.emission_scale = float(G.emission_scale);
+    f.absorption_scale = float(G.absorption_scale);
+    f.faraday_scale = float(G.faraday_scale);
+    f.max_pol_frac = float(G.max_pol_frac);
+    f.min_sin_theta = 1e-4f;
+    const double cl = KP_CL;
+    P.rm.L = float(6.67430e-8 * mbh_solar * 1.98847e33 / (cl * cl));
+    P.rm.bconst = 0.0f;  // per frequency: P.bconst_f, P.bconst_c
+    P.rm.x0 = float(KP_HPL / (KP_ME * KP_CL * KP_CL));
+    P.mass_f = float(P.mass);
+    P.spin_f = float(P.spin);
+}
+
 // Frequency lists of a multi-frequency image (host copies); nfreq = 0 for
 // the single-frequency image at model->freq_cgs.
 struct FreqLists {

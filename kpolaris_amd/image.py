@@ -473,11 +473,13 @@ class GRMHDGrid:
 
 
 def run_grmhd_image(params: abi.PassAParams, grid: "GRMHDGrid", frequencies=None, step_control_frequencies=None,
-                    *, persistent: bool = True, block_size: int = 0, waves_per_cu: int = 0,
-                    split_transport: int = 1, out: dict | None = None, fields=abi.IMAGE_FIELDS,
-                    with_stats: bool = True):
+                    *, precision: int = abi.PRECISION_FP64, persistent: bool = True, block_size: int = 0,
+                    waves_per_cu: int = 0, split_transport: int = 1, out: dict | None = None,
+                    fields=abi.IMAGE_FIELDS, with_stats: bool = True):
     """GRMHD image on FMKS geodesics; frequencies None = the model's freq_cgs.
-    Stokes tensors are [nfreq * npix]."""
+    Stokes tensors are [nfreq * npix].  precision PRECISION_MIXED evaluates
+    the thermal coefficients and the transfer in FP32 (geodesics and the
+    grid sampling stay FP64)."""
     import torch
     npix = params.camera.nx * params.camera.ny
     f = np.ascontiguousarray([grid.model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
@@ -486,7 +488,7 @@ def run_grmhd_image(params: abi.PassAParams, grid: "GRMHDGrid", frequencies=None
         out = alloc_device_image(npix, fields=fields, nfreq=len(f))
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = run_options(abi.PRECISION_FP64, persistent, block_size, waves_per_cu, C.c_void_p(stream), None,
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), None,
                        split_transport)
     st = abi.ImageStats()
     dp = C.POINTER(C.c_double)

@@ -15,13 +15,16 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
                           const double* log_scales = nullptr, double* grad = nullptr,
                           const kp_grmhd_model* gm = nullptr, const kp_slow_light* slow = nullptr,
                           double* req_t = nullptr, unsigned long long* waiting = nullptr) {
-    constexpr int GP = MIXED ? GP_GUARD : GP_64;
     constexpr bool FM = base_mode(MODE) == M_GRMHD || base_mode(MODE) == M_SLOW || MODE == M_STAN;
+    constexpr int GP = MIXED && !FM ? GP_GUARD : GP_64;
     Params P;
     kp_status st = FM ? prepare_grmhd(pa, gm, P) : prepare(pa, m, P);
     if (st != KP_OK) return st;
     if (MIXED) {
-        prepare_mixed(P);
+        if (FM)
+            prepare_mixed_grmhd(P, gm->mbh_solar);
+        else
+            prepare_mixed(P);
         P.fp64_radius = fp64_radius;
     }
     if (lds_stokes<MODE>() || MODE == M_STAN) {
@@ -39,6 +42,8 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         P.t_obs = slow->observation_time;
         P.req_t = req_t;
     }
+    // the library's single-pass rule: FP64 frequency lists, and mixed GRMHD
+    if (MIXED && MODE != M_GRMHD) P.single_pass = 0;
     if (MODE == M_STAN) {
         P.tq_ne = std::exp(log_scales[0]);
         P.tq_b = std::exp(0.5 * log_scales[0]) * std::exp(log_scales[1]);
@@ -145,6 +150,20 @@ extern "C" kp_status kp_emul_run_grmhd_image(const kp_pass_a_params* pa, const k
     if (nonthermal(gm->emission_type))
         return emul_run<false, M_GRMHD_NT>(pa, nullptr, 0.0, fl, out, rhs_evals, nullptr, nullptr, nullptr, gm);
     return emul_run<false, M_GRMHD>(pa, nullptr, 0.0, fl, out, rhs_evals, nullptr, nullptr, nullptr, gm);
+}
+
+// Mixed-precision GRMHD lanes (thermal fits): FP32 coefficients and
+// transfer on FP64 FMKS geodesics; model->cells are HOST records here.
+extern "C" kp_status kp_emul_run_grmhd_image_mixed(const kp_pass_a_params* pa, const kp_grmhd_model* gm,
+                                                   const double* freqs, int32_t nfreq, const double* ctrl,
+                                                   int32_t nctrl, kp_image_data* out, int64_t* rhs_evals) {
+    FreqLists fl;
+    fl.nfreq = nfreq;
+    fl.freqs = freqs;
+    fl.nctrl = nctrl;
+    fl.ctrl = ctrl;
+    if (nonthermal(gm->emission_type)) return KP_UNSUPPORTED;
+    return emul_run<true, M_GRMHD>(pa, nullptr, 0.0, fl, out, rhs_evals, nullptr, nullptr, nullptr, gm);
 }
 
 // The device derived-scalar formulas run on host cells.
