@@ -332,14 +332,16 @@ def bench_slow(args) -> int:
     """Slow light: 32 snapshots of a seeded 192x96x96 FMKS torus (the GRMHD
     workload's torus at this resolution, advected by Keplerian rotation,
     dt = 5 GM/c^3), all resident, sampled at retarded time with decoupled
-    stepping and fluid interpolation for a 256x256 image at 230 GHz; FP64.
-    The radiating region is r < 40 M, the observation time is chosen by the
-    time probe so that the radiating samples sit in the middle of the
-    sequence.  Reports the grid-sample traffic of pass B (two snapshots x 8
-    cell records x 72 bytes per radiating step) against HBM bandwidth."""
+    stepping and fluid interpolation for a 256x256 image at 230 GHz; FP64 and
+    mixed precision, the mixed rate reported when its image meets the error
+    bar against the FP64 image.  The radiating region is r < 40 M, the
+    observation time is chosen by the time probe so that the radiating
+    samples sit in the middle of the sequence.  Reports the grid-sample
+    traffic of pass B (two snapshots x 8 cell records x 72 bytes per
+    radiating step) against HBM bandwidth."""
     import numpy as np
     import torch
-    from kpolaris_amd import grmhd_synth as gs, image
+    from kpolaris_amd import abi, grmhd_synth as gs, image
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
     cells, times, m = gs.make_sequence(nsnap=32, dt=5.0, t0=0.0, n1=192, n2=96, n3=96)
@@ -351,60 +353,76 @@ def bench_slow(args) -> int:
     pr = image.slow_light_time_probe(p, m)
     t_obs = 0.5 * (times[0] + times[-1]) - 0.5 * (pr["min_x0"] + pr["max_x0"])
     npix = p.camera.nx * p.camera.ny
-    out = None
-    for _ in range(args.warmup):
-        out, _ = image.run_grmhd_slow_light_image(p, grids, times, t_obs, out=out)
-    torch.cuda.synchronize()
-    per_step = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out, st = image.run_grmhd_slow_light_image(p, grids, times, t_obs, out=out)
-        per_step.append(st)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kms = statistics.median(s["kernel_ms"] for s in per_step)
-    st = per_step[-1]
-    b_ms = statistics.median(s["pass_b_ms"] for s in per_step)
-    # one radiating step samples two snapshots (8 records of 72 bytes each);
-    # the coefficient counter counts one evaluation per step (one frequency,
-    # single-pass control and transfer)
-    sample_bytes = st["coefficient_evals"] * 2 * 8 * 72
-    gbs = sample_bytes / (b_ms * 1e-3) / 1e9
-    # pass B priced as the GRMHD pass B (FMKS attempts, coefficients, one
-    # Strang step and the substep control per step; samples not counted)
-    oc = opcounts()
-    rhs_b = st["rhs_evals"] - st["pass_a_rhs_evals"]
-    flops_b = (rhs_b * oc["fmks_doubling_attempt"]["flops"] / 11.0
-               + st["coefficient_evals"] * oc["grmhd_coefficients_dexter"]["flops"]
-               + st["transfer_steps"] * oc["transfer_step"]["flops"]
-               + 2.0 * st["pass_b_steps"] * oc["substep_count"]["flops"])
-    b_tflops = flops_b / (b_ms * 1e-3) / 1e12
-    reasons = np.bincount(out["reason"].cpu().numpy(), minlength=10)
+
+    def run(precision):
+        out = image.alloc_device_image(npix)
+        for _ in range(args.warmup):
+            image.run_grmhd_slow_light_image(p, grids, times, t_obs, precision=precision, out=out)
+        torch.cuda.synchronize()
+        per_step = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, st = image.run_grmhd_slow_light_image(p, grids, times, t_obs, precision=precision, out=out)
+            per_step.append(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = statistics.median(s["kernel_ms"] for s in per_step)
+        st = per_step[-1]
+        b_ms = statistics.median(s["pass_b_ms"] for s in per_step)
+        # one radiating step samples two snapshots (8 records of 72 bytes each);
+        # the coefficient counter counts one evaluation per step (one frequency,
+        # single-pass control and transfer)
+        sample_bytes = st["coefficient_evals"] * 2 * 8 * 72
+        gbs = sample_bytes / (b_ms * 1e-3) / 1e9
+        # pass B priced as the GRMHD pass B (one frequency)
+        flops_b, f32_b = grmhd_pass_b_work(st, 1, precision == abi.PRECISION_MIXED)
+        reasons = np.bincount(out["reason"].cpu().numpy(), minlength=10)
+        return out, {
+            "rays_per_s": args.steps * npix / wall, "ms_per_step": wall / args.steps * 1e3,
+            "kernel_ms_median": kms, "pass_a_ms": st["pass_a_ms"], "pass_b_ms_median": b_ms,
+            "steps_per_s": (st["pass_a_steps"] + st["pass_b_steps"]) / (kms * 1e-3),
+            "coefficient_evals": st["coefficient_evals"],
+            "grid_sample": {"bound": "hbm", "achieved_gbs": gbs, "frac": gbs / 8000.0,
+                            "algorithmic_bytes_per_launch": sample_bytes},
+            "reasons": {str(i): int(c) for i, c in enumerate(reasons) if c},
+            "pass_b_flops": flops_b, "pass_b_f32_flops": f32_b,
+            "pass_b_peak_tflops": vector_peak(flops_b, f32_b),
+            "pass_b_tflops": flops_b / (b_ms * 1e-3) / 1e12,
+        }
+
+    runs, images = {}, {}
+    if args.precision in ("fp64", "both"):
+        images["fp64"], runs["fp64"] = run(abi.PRECISION_FP64)
+    if args.precision in ("mixed", "both"):
+        images["mixed"], runs["mixed"] = run(abi.PRECISION_MIXED)
+    err = mixed_error(images["mixed"], images["fp64"]) if len(runs) == 2 else None
+    head = "mixed" if "mixed" in runs and (err is None or err["meets_bar"]) else "fp64"
+    r = runs[head]
+    peak = r["pass_b_peak_tflops"]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_slow(p, m, grids, times, t_obs, args.cpu_seconds)
     print(json.dumps({
-        "metric": "rays/s on 1 MI355X, slow-light GRMHD image (32 resident snapshots), FP64",
-        "value": args.steps * npix / wall, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": "rays/s on 1 MI355X, slow-light GRMHD image (32 resident snapshots)",
+        "value": r["rays_per_s"], "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded Fishbone-Moncrief torus, Keplerian advection)",
         "config": {"workload": "slow light: 32 FMKS snapshots 192x96x96 (dt 5 GM/c^3), 256x256 pinhole, "
                                "230 GHz, i=163 deg, r_out 40, decoupled stepping, fluid interpolation, "
                                "adaptive step-doubling RK4 rtol 1e-6",
                    "rays_per_step": npix, "sequence_generation_s": gen_s, "observation_time": t_obs,
-                   "probe_x0": [pr["min_x0"], pr["max_x0"]], "kernel_ms_median": kms,
-                   "pass_a_ms": st["pass_a_ms"], "pass_b_ms": b_ms,
-                   "steps_per_s": (st["pass_a_steps"] + st["pass_b_steps"]) / (kms * 1e-3),
-                   "coefficient_evals": st["coefficient_evals"],
-                   "grid_sample": {"bound": "hbm", "achieved_gbs": gbs, "frac": gbs / 8000.0,
-                                   "algorithmic_bytes_per_launch": sample_bytes},
-                   "reasons": {str(i): int(c) for i, c in enumerate(reasons) if c}},
-        "roofline": {"bound": "valu", "kernel": "pass_b", "achieved": b_tflops, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": b_tflops / FP64_PEAK_TFLOPS,
-                     "traffic": profiled_traffic("pass_b", PROFILE_SLOW),
+                   "probe_x0": [pr["min_x0"], pr["max_x0"]], "precision": head, "runs": runs,
+                   "mixed_vs_fp64": err,
+                   "speedup_mixed_over_fp64": (runs["mixed"]["rays_per_s"] / runs["fp64"]["rays_per_s"]
+                                               if len(runs) == 2 else None)},
+        "roofline": {"bound": "valu", "kernel": "pass_b", "achieved": r["pass_b_tflops"], "peak": peak,
+                     "peak_basis": "MI355X vector peaks, FP64 78.6 and FP32 157.3 TFLOP/s, weighted by the "
+                                   "kernel's FP64 and FP32 FLOPs",
+                     "unit": "TFLOP/s", "frac": r["pass_b_tflops"] / peak,
+                     "traffic": profiled_traffic("pass_b" if head == "fp64" else "pass_b_mixed", PROFILE_SLOW),
                      "traffic_source": "profiles/slow_light_256_32snap_kernels.md",
-                     "algorithmic_flops_per_launch": flops_b, "launch_ms": b_ms},
+                     "algorithmic_flops_per_launch": r["pass_b_flops"], "launch_ms": r["pass_b_ms_median"]},
         "cpu_baseline": cpu,
     }), flush=True)
     return 0

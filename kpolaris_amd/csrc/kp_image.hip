@@ -161,10 +161,10 @@ const void* pass_b_fn(bool persistent, bool mixed, int gp, int mode) {
     switch (mode) {
     case M_SINGLE: return kpsel::pass_b_single(persistent, mixed, gp);
     case M_MF: return kpsel::pass_b_mf(persistent, mixed, gp);
-    case M_GRMHD: return mixed ? kpsel::pass_b_grmhd_mixed(persistent) : kpsel::pass_b_fmks(persistent, mode);
+    case M_GRMHD: return mixed ? kpsel::pass_b_grmhd_mixed(persistent, mode) : kpsel::pass_b_fmks(persistent, mode);
     case M_GRMHD_NT:
     case M_PROBE: return mixed ? nullptr : kpsel::pass_b_fmks(persistent, mode);
-    case M_SLOW:
+    case M_SLOW: return mixed ? kpsel::pass_b_grmhd_mixed(persistent, mode) : kpsel::pass_b_slow(persistent, mode);
     case M_SLOW_NT:
     case M_STAN: return mixed ? nullptr : kpsel::pass_b_slow(persistent, mode);
     default: return mixed ? nullptr : kpsel::pass_b_other(persistent, mode);
@@ -173,7 +173,7 @@ const void* pass_b_fn(bool persistent, bool mixed, int gp, int mode) {
 
 const void* unsplit_fn(bool mixed, int gp, int mode) {
     const int b = base_mode(mode);
-    if (mode == M_GRMHD && mixed) return kpsel::unsplit_grmhd_mixed();
+    if (mixed && (mode == M_GRMHD || mode == M_SLOW)) return kpsel::unsplit_grmhd_mixed(mode);
     if (b == M_GRMHD || b == M_SLOW || b == M_PROBE || mode == M_STAN) return mixed ? nullptr : kpsel::unsplit_fmks(mode);
     return kpsel::unsplit_bl(mixed, gp, mode);
 }
@@ -250,8 +250,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     FreqLists fl = fl_in;
     kp_status st;
     if (grmhd) {
-        if (mixed && (slow || probe))
-            return kp_internal::fail(KP_UNSUPPORTED, "slow-light sequences and time probes are built for FP64");
+        if (mixed && probe) return kp_internal::fail(KP_UNSUPPORTED, "the slow-light time probe is built for FP64");
         if (perturbation || (tangent && !slow))
             return kp_internal::fail(KP_UNSUPPORTED, "GRMHD gradients are built for slow-light sequences");
         st = prepare_grmhd(params, grmhd, P, probe == nullptr);
@@ -416,8 +415,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (st != KP_OK) return st;
     const size_t smem_a = lds_bytes(block, 0);
     // the single-pass frequency loop serves the FP64 multi-frequency, GRMHD
-    // and slow-light lanes and the mixed-precision GRMHD lanes
-    if ((mixed && mode != M_GRMHD) ||
+    // and slow-light lanes and the mixed-precision GRMHD and slow-light lanes
+    if ((mixed && mode != M_GRMHD && mode != M_SLOW) ||
         !(base_mode(mode) == M_MF || base_mode(mode) == M_GRMHD || base_mode(mode) == M_SLOW))
         P.single_pass = 0;
     const size_t smem_b = mode == M_STAN ? lds_bytes(block, 0) : lds_bytes(block, mf ? P.nfreq : 0, P.single_pass ? 2 : 1);
@@ -552,7 +551,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             KP_HIP(hipMemsetAsync(Q.fetch, 0, sizeof(uint32_t), stream));
             KP_HIP(hipMemsetAsync(Q.count_out, 0, sizeof(uint32_t), stream));
             KP_HIP(hipEventRecord(bev.e[0], stream));
-            s2 = launch(kpsel::pass_b_slow_windowed(mode), smem_b, Q);
+            s2 = launch(mixed ? kpsel::pass_b_grmhd_mixed(true, M_SLOW_W) : kpsel::pass_b_slow_windowed(mode), smem_b,
+                        Q);
             if (s2 != KP_OK) return s2;
             KP_HIP(hipEventRecord(bev.e[1], stream));
             ++blocks;

@@ -1180,6 +1180,21 @@ template <bool NT> struct RadiationGRMHD {
 template <> struct Radiation<false, M_GRMHD> : RadiationGRMHD<false> {};
 template <> struct Radiation<false, M_GRMHD_NT> : RadiationGRMHD<true> {};
 
+// The FP64 local plasma state in FP32 (the GRMHD form is never field-aligned).
+KP_DEV void narrow_local(const RiafLocal<double>& d, RiafLocal<float>& o) {
+    o.valid = d.valid;
+    o.aligned = 0;
+    o.has_chi = d.has_chi;
+    o.ne = float(d.ne);
+    o.thetae = float(d.thetae);
+    o.b = float(d.b);
+    o.nu_scale = float(d.nu_scale);
+    o.cos_theta = float(d.cos_theta);
+    o.sin_theta = float(d.sin_theta);
+    o.cos2chi = float(d.cos2chi);
+    o.sin2chi = float(d.sin2chi);
+}
+
 // Mixed-precision GRMHD (thermal fits): the grid gather, the native metric
 // and the four-vectors stay FP64 (grmhd_local); the local plasma state is
 // narrowed to FP32 and every frequency's coefficients follow the FP32
@@ -1190,17 +1205,7 @@ template <> struct Radiation<true, M_GRMHD> : RadiationScaledMF {
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         RiafLocal<double> d;
         grmhd_local(P.grmhd, sample, d);
-        loc.valid = d.valid;
-        loc.aligned = 0;
-        loc.has_chi = d.has_chi;
-        loc.ne = float(d.ne);
-        loc.thetae = float(d.thetae);
-        loc.b = float(d.b);
-        loc.nu_scale = float(d.nu_scale);
-        loc.cos_theta = float(d.cos_theta);
-        loc.sin_theta = float(d.sin_theta);
-        loc.cos2chi = float(d.cos2chi);
-        loc.sin2chi = float(d.sin2chi);
+        narrow_local(d, loc);
         return 0;
     }
 };
@@ -1272,6 +1277,81 @@ template <> struct Radiation<false, M_SLOW> : RadiationSlow<false> {};
 template <> struct Radiation<false, M_SLOW_NT> : RadiationSlow<true> {};
 template <> struct Radiation<false, M_SLOW_W> : RadiationSlow<false> {};
 template <> struct Radiation<false, M_SLOW_NT_W> : RadiationSlow<true> {};
+
+// Mixed-precision slow light (thermal fits): the time-interpolated plasma
+// state is reconstructed in FP64 as above and narrowed to FP32; each
+// frequency's coefficients follow the FP32 thermal fit in nu^3-scaled units,
+// mixed in FP32 between the two snapshots under coefficient interpolation.
+struct RadiationSlowMixed {
+    RiafLocal<float> lo, hi;
+    double f;
+    int k;
+    KP_DEV void zero() {
+        lo.valid = 0;
+        hi.valid = 0;
+        f = 0.0;
+        k = 0;
+    }
+    KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
+        RiafLocal<double> d;
+        if (P.slow_interp == KP_SLOW_LIGHT_FLUID) {
+            grmhd_local_time(P.grmhd, P.snaps[k], P.snaps[k + 1], f, sample, d);
+            narrow_local(d, lo);
+        } else {
+            grmhd_local_time(P.grmhd, P.snaps[k], P.snaps[k], 0.0, sample, d);
+            narrow_local(d, lo);
+            grmhd_local_time(P.grmhd, P.snaps[k + 1], P.snaps[k + 1], 0.0, sample, d);
+            narrow_local(d, hi);
+        }
+        return 0;
+    }
+    KP_DEV bool valid() const { return lo.valid || hi.valid; }
+    KP_DEV void at_freq(const Params& P, float freq, float bconst, Coeffs<float>& c) const {
+        riaf_coeffs_scaled_at(P.rm, lo, freq, bconst, c);
+        if (P.slow_interp != KP_SLOW_LIGHT_FLUID) {
+            Coeffs<float> b;
+            riaf_coeffs_scaled_at(P.rm, hi, freq, bconst, b);
+            const float ff = float(f), w = 1.0f - ff;
+            c.jI = w * c.jI + ff * b.jI;
+            c.jQ = w * c.jQ + ff * b.jQ;
+            c.jU = w * c.jU + ff * b.jU;
+            c.jV = w * c.jV + ff * b.jV;
+            c.aI = w * c.aI + ff * b.aI;
+            c.aQ = w * c.aQ + ff * b.aQ;
+            c.aU = w * c.aU + ff * b.aU;
+            c.aV = w * c.aV + ff * b.aV;
+            c.rQ = w * c.rQ + ff * b.rQ;
+            c.rU = w * c.rU + ff * b.rU;
+            c.rV = w * c.rV + ff * b.rV;
+        }
+    }
+    // output frequency q (single pass: also the control frequency q)
+    KP_DEV void at32(const Params& P, int q, Coeffs<float>& c) const {
+        at_freq(P, float(P.freqs[q]), P.bconst_f[q], c);
+    }
+    KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
+        req = 1;
+        pred = 1;
+        for (int q = 0; q < P.nctrl; ++q) {
+            Coeffs<float> c;
+            at_freq(P, float(P.ctrl[q]), P.bconst_c[q], c);
+            const Coeffs<double> d = widen(c);
+            req = maxv(req, substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
+            pred = maxv(pred, substep_count(d, h_next, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
+        }
+        return (lo.valid || hi.valid) ? (uint32_t)P.nctrl : 0u;
+    }
+    KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
+        for (int q = 0; q < P.nfreq; ++q) {
+            Coeffs<float> c;
+            at32(P, q, c);
+            mf_strang32(P, L, q, c, float(absv(h)));
+        }
+        return (lo.valid || hi.valid) ? (uint32_t)P.nfreq : 0u;
+    }
+};
+template <> struct Radiation<true, M_SLOW> : RadiationSlowMixed {};
+template <> struct Radiation<true, M_SLOW_W> : RadiationSlowMixed {};
 
 // Time probe: geometric stepping only (no coefficients, no substep control).
 template <> struct Radiation<false, M_PROBE> {
@@ -1715,7 +1795,7 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
                             ? P.max_rs : 0.0;
     int required, pred;
     constexpr bool SHARED = !MIXED && (base_mode(MODE) == M_MF || base_mode(MODE) == M_GRMHD || base_mode(MODE) == M_SLOW);
-    constexpr bool SHARED32 = MIXED && MODE == M_GRMHD;
+    constexpr bool SHARED32 = MIXED && (MODE == M_GRMHD || MODE == M_SLOW || MODE == M_SLOW_W);
     if constexpr (SHARED || SHARED32) {
         if (P.single_pass) {
             // the control frequencies are the output frequencies: one
@@ -2059,6 +2139,8 @@ const void* pass_b_stan(bool persistent);
 const void* pass_b_slow_windowed(int mode);
 const void* pass_b_legs(bool mixed, int gp, bool vacuum);
 const void* unsplit_slow(int mode);
-const void* pass_b_grmhd_mixed(bool persistent);
-const void* unsplit_grmhd_mixed();
+const void* pass_b_grmhd_mixed(bool persistent, int mode);
+const void* unsplit_grmhd_mixed(int mode);
+const void* pass_b_slow_mixed(bool persistent, int mode);
+const void* unsplit_slow_mixed(int mode);
 }  // namespace kpsel

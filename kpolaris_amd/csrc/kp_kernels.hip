@@ -197,13 +197,34 @@ const void* pass_b_legs(bool mixed, int gp, bool vacuum) {
 
 #if KP_PART == 12
 // Mixed-precision GRMHD pass B (FP64 FMKS geodesics, FP32 thermal
-// coefficients), split and unsplit.
-const void* pass_b_grmhd_mixed(bool persistent) {
+// coefficients), split and unsplit; slow light in part 13.
+const void* pass_b_grmhd_mixed(bool persistent, int mode) {
+    if (mode != M_GRMHD) return pass_b_slow_mixed(persistent, mode);
     return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_GRMHD>
                       : (const void*)pass_b_kernel<false, true, GP_64, M_GRMHD>;
 }
 
-const void* unsplit_grmhd_mixed() { return (const void*)unsplit_kernel<true, GP_64, M_GRMHD>; }
+const void* unsplit_grmhd_mixed(int mode) {
+    return mode == M_GRMHD ? (const void*)unsplit_kernel<true, GP_64, M_GRMHD> : unsplit_slow_mixed(mode);
+}
+#endif
+
+#if KP_PART == 13
+// Mixed-precision slow light: resident sequences (split and unsplit) and the
+// windowed block launches (persistent).
+const void* pass_b_slow_mixed(bool persistent, int mode) {
+    switch (mode) {
+    case M_SLOW:
+        return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_SLOW>
+                          : (const void*)pass_b_kernel<false, true, GP_64, M_SLOW>;
+    case M_SLOW_W: return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_SLOW_W> : nullptr;
+    default: return nullptr;
+    }
+}
+
+const void* unsplit_slow_mixed(int mode) {
+    return mode == M_SLOW ? (const void*)unsplit_kernel<true, GP_64, M_SLOW> : nullptr;
+}
 #endif
 
 #if KP_PART == 9

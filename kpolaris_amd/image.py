@@ -583,12 +583,13 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observatio
                                *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
                                interpolation: int = abi.SLOW_LIGHT_FLUID, persistent: bool = True,
                                block_size: int = 0, waves_per_cu: int = 0, split_transport: int = 1,
-                               windows_per_block: int = 0, out: dict | None = None, fields=abi.IMAGE_FIELDS,
-                               with_stats: bool = True):
+                               windows_per_block: int = 0, precision: int = abi.PRECISION_FP64,
+                               out: dict | None = None, fields=abi.IMAGE_FIELDS, with_stats: bool = True):
     """Slow-light GRMHD image over a snapshot sequence (grids: GRMHDGrid per
     time, all on one grid geometry; times strictly increasing) sampled at
     retarded time observation_time + x^0.  windows_per_block > 0 runs pass B
     in blocks of that many snapshot pairs (the reference's batch scheduler).
+    precision PRECISION_MIXED: FP32 thermal coefficients and transfer.
     Raises KPError(KP_TIME_SUPPORT) when decoupled stepping needs a time
     outside the sequence (the image is still written into `out`)."""
     import torch
@@ -600,7 +601,7 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observatio
         out = alloc_device_image(npix, fields=fields, nfreq=len(f))
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = run_options(abi.PRECISION_FP64, persistent, block_size, waves_per_cu, C.c_void_p(stream), None,
+    opts = run_options(precision, persistent, block_size, waves_per_cu, C.c_void_p(stream), None,
                        split_transport)
     st = abi.ImageStats()
     _check(library().kp_run_grmhd_slow_light_image(
@@ -632,7 +633,8 @@ def run_grmhd_slow_light_stream(params: abi.PassAParams, model: abi.GRMHDModel, 
                                 loader, frequencies=None, *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
                                 interpolation: int = abi.SLOW_LIGHT_FLUID, windows_per_block: int = 1,
                                 prefetch: bool = True, prefetch_snapshots: int = 0, derive_scalars: bool = False,
-                                user=None, block_size: int = 0, waves_per_cu: int = 0, out: dict | None = None,
+                                user=None, block_size: int = 0, waves_per_cu: int = 0,
+                                precision: int = abi.PRECISION_FP64, out: dict | None = None,
                                 fields=abi.IMAGE_FIELDS, with_stats: bool = True):
     """Slow-light GRMHD image of a sequence streamed from host storage
     (kp_run_grmhd_slow_light_stream): `loader` is an abi.SnapshotLoader
@@ -659,7 +661,7 @@ def run_grmhd_slow_light_stream(params: abi.PassAParams, model: abi.GRMHDModel, 
         out = alloc_device_image(npix, fields=fields, nfreq=len(f))
     img = image_struct_from_tensors(out)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = run_options(abi.PRECISION_FP64, True, block_size, waves_per_cu, C.c_void_p(stream))
+    opts = run_options(precision, True, block_size, waves_per_cu, C.c_void_p(stream))
     st = abi.ImageStats()
     rep = abi.SlowLightSchedule()
     _check(library().kp_run_grmhd_slow_light_stream(

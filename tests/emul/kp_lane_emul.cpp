@@ -43,7 +43,7 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         P.req_t = req_t;
     }
     // the library's single-pass rule: FP64 frequency lists, and mixed GRMHD
-    if (MIXED && MODE != M_GRMHD) P.single_pass = 0;
+    if (MIXED && MODE != M_GRMHD && MODE != M_SLOW) P.single_pass = 0;
     if (MODE == M_STAN) {
         P.tq_ne = std::exp(log_scales[0]);
         P.tq_b = std::exp(0.5 * log_scales[0]) * std::exp(log_scales[1]);
@@ -208,6 +208,21 @@ extern "C" kp_status kp_emul_run_grmhd_slow_light_image(const kp_pass_a_params* 
                                           req_t, waiting);
     return emul_run<false, M_SLOW>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, nullptr, nullptr, &g, seq, req_t,
                                    waiting);
+}
+
+// Mixed-precision slow-light lanes (thermal fits) over HOST snapshot records.
+extern "C" kp_status kp_emul_run_grmhd_slow_light_image_mixed(const kp_pass_a_params* pa, const kp_grmhd_model* gm,
+                                                              const kp_slow_light* seq, const double* freqs,
+                                                              int32_t nfreq, kp_image_data* out, double* req_t,
+                                                              unsigned long long* waiting) {
+    FreqLists fl;
+    fl.nfreq = fl.nctrl = nfreq;
+    fl.freqs = fl.ctrl = freqs;
+    kp_grmhd_model g = *gm;
+    g.cells = seq->cells[0];
+    if (nonthermal(gm->emission_type)) return KP_UNSUPPORTED;
+    return emul_run<true, M_SLOW>(pa, nullptr, 0.0, fl, out, nullptr, nullptr, nullptr, nullptr, &g, seq, req_t,
+                                  waiting);
 }
 
 // Slow-light gradient lanes (one frequency, the model's); grad is [3][4][npix].

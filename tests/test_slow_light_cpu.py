@@ -457,3 +457,37 @@ def test_emulated_slow_light_gradient_against_central_differences(seq, emul):
             per_pixel = np.abs(grad[kind, s][ok] - rich[ok]) / scale
             assert np.percentile(per_pixel, 95) < 1e-8, (kind, name, np.percentile(per_pixel, 95))
 
+
+
+@pytest.mark.parametrize("mode", [abi.SLOW_LIGHT_DECOUPLED, abi.SLOW_LIGHT_SNAPSHOT])
+@pytest.mark.parametrize("interp", [abi.SLOW_LIGHT_FLUID, abi.SLOW_LIGHT_COEFFICIENTS])
+def test_emulated_mixed_slow_light_against_oracle(seq, emul, mode, interp):
+    """The mixed-precision slow-light lanes (FP32 thermal coefficients mixed
+    in FP32 between snapshots under coefficient interpolation, FP32 transfer,
+    FP64 geodesics and time logic) against the FP64 oracle at the
+    mixed-precision image bars, two frequencies on shared geodesics; the
+    coarse image's net polarisation fractions are held in absolute terms."""
+    emul.kp_emul_run_grmhd_slow_light_image_mixed.argtypes = emul.kp_emul_run_grmhd_slow_light_image.argtypes
+    emul.kp_emul_run_grmhd_slow_light_image_mixed.restype = C.c_int32
+    cells, times, model, p = seq
+    freqs = [model.freq_cgs, 345e9]
+    sq, keep = oracle.slow_light_sequence(cells, times, T_OBS, mode, interp)
+    npix = p.camera.nx * p.camera.ny
+    f = np.array(freqs, dtype=np.float64)
+    arrays = oracle.alloc_image(npix)
+    for n in STOKES:
+        arrays[n] = np.zeros(2 * npix)
+    img = oracle.image_struct(arrays)
+    req = np.zeros(npix)
+    w = C.c_ulonglong()
+    assert emul.kp_emul_run_grmhd_slow_light_image_mixed(C.byref(p), C.byref(model), C.byref(sq),
+                                                         f.ctypes.data_as(P(C.c_double)), 2, C.byref(img),
+                                                         req.ctypes.data_as(P(C.c_double)), C.byref(w)) == 0
+    ref, _, _ = oracle.run_grmhd_slow_light_image(p, model, cells, times, T_OBS, freqs, step_mode=mode,
+                                                  interpolation=interp)
+    assert np.array_equal(arrays["reason"], ref["reason"])
+    g = parity.split_frequencies(arrays, 2)
+    r = parity.split_frequencies(ref, 2)
+    for k in range(2):
+        parity.assert_mixed(g[k], r[k], f"emulated mixed slow light mode {mode} interp {interp} f{k}",
+                            pol="absolute")
