@@ -92,8 +92,9 @@ typedef enum kp_precision {
      * nu_obs^3-scaled units), Faraday rotation angles above one radian in
      * FP64.  Geodesic stepping per kp_run_options.guard_mode: FP64 (default),
      * or FP32 derivatives and RK stage sums with FP64 attempts inside
-     * fp64_radius, FP64 step-error comparison and surface-crossing
-     * refinement. */
+     * fp64_radius and at turning points of theta, FP64 step-error comparison
+     * and surface-crossing refinement.  The GRMHD and slow-light images take
+     * the thermal fits only in this precision. */
     KP_PRECISION_MIXED = 1
 } kp_precision;
 
@@ -269,12 +270,19 @@ typedef struct kp_run_options {
                                  arithmetic issues at the FP64 rate on this
                                  hardware without packed FP32, so FP32 stepping
                                  buys nothing);  1 = FP32 stepping with FP64
-                                 attempts inside fp64_radius, chosen per lane in
-                                 one kernel;  2 = the same, staged: FP32-geodesic
-                                 and FP64-geodesic kernels, rays moved between
-                                 them at fp64_radius and 1.25 fp64_radius
-                                 (persistent split launches; others use 1) */
-    double fp64_radius;       /* mixed: attempts starting at r < fp64_radius run in FP64 */
+                                 attempts inside fp64_radius and across turning
+                                 points of theta (an FP32 attempt whose
+                                 dtheta/dlambda changes sign is taken again in
+                                 FP64), chosen per lane in one kernel;  2 = FP32
+                                 stepping staged: FP32-geodesic and
+                                 FP64-geodesic kernels, rays moved between them
+                                 at fp64_radius and 1.25 fp64_radius, without
+                                 the turning-point retake (persistent split
+                                 launches; others use 1).  GRMHD and slow-light
+                                 images step FMKS geodesics in FP64 in every
+                                 mode. */
+    double fp64_radius;       /* mixed: attempts starting at r < fp64_radius run in FP64;
+                                 0 turns every FP64 guard off */
     void* stream;             /* hipStream_t, NULL = default stream */
     /* Device workspace of the run (pass-A endpoints, queue counters, staged
      * and slow-light tables), owned by the caller, at least the size

@@ -391,17 +391,26 @@ KP_DEV void lane_a_init(const Params& P, int pixel, LaneA& L) {
 // One integration attempt from the lane state.  GP_64: FP64 derivatives.
 // GP_32: FP32 derivatives and stage sums, increments added to the FP64
 // state.  GP_GUARD: per lane, FP64 inside the guard radius and FP32 outside
-// (the lanes of a wave may take both paths).  force64: FP64 whatever GP says
-// (the surface-crossing refinement of a mixed-precision lane).  guard counts
-// the FP64 integration attempts of a mixed-precision image: those of
-// GP_GUARD lanes inside the radius and every attempt of a staged FP64 launch
+// (the lanes of a wave may take both paths); an FP32 attempt across a
+// turning point of theta (k^theta = dtheta/dlambda changes sign between its
+// start and its end) is taken again in FP64 while the guard is on
+// (fp64_radius > 0).  force64: FP64 whatever GP says (the surface-crossing
+// refinement of a mixed-precision lane).  guard counts the FP64 integration
+// attempts of a mixed-precision image: those of GP_GUARD lanes inside the
+// radius or across a turning point and every attempt of a staged FP64 launch
 // (P.stage >= 0), not the refinement trials.
+KP_DEV bool theta_turns(const Params& P, double kth0, double kth1) {
+    return P.fp64_radius > 0.0 && (kth0 < 0.0) != (kth1 < 0.0);
+}
+
 template <int GP, class GG>
 KP_DEV double attempt(const Params& P, const GG& G, const LaneVec& s, LaneVec& k1, double h,
                       St<double>& half, St<double>& next, uint32_t& guard, bool full = true,
                       bool force64 = false) {
-    if (GP == GP_32 || (GP == GP_GUARD && !force64 && !(s[1] < P.fp64_radius)))
-        return doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count, full);
+    if (GP == GP_32 || (GP == GP_GUARD && !force64 && !(s[1] < P.fp64_radius))) {
+        const double err = doubling_attempt_mixed(P.mass_f, P.spin_f, s, h, half, next, G.rhs_count, full);
+        if (GP == GP_32 || !theta_turns(P, s[6], next.v[6])) return err;
+    }
     if (!force64 && (GP == GP_GUARD || P.stage >= 0)) ++guard;
     return doubling_attempt(G, s, h, k1, half, next, full);
 }
@@ -411,7 +420,7 @@ KP_DEV void step_rk4(const Params& P, const GG& G, const LaneVec& s, double h, S
                      uint32_t& guard, bool force64 = false) {
     if (GP == GP_32 || (GP == GP_GUARD && !force64 && !(s[1] < P.fp64_radius))) {
         rk4_mixed(P.mass_f, P.spin_f, s, h, next, G.rhs_count);
-        return;
+        if (GP == GP_32 || !theta_turns(P, s[6], next.v[6])) return;
     }
     if (!force64 && (GP == GP_GUARD || P.stage >= 0)) ++guard;
     rk4(G, s, h, next);

@@ -151,6 +151,29 @@ def test_gpu_mixed_guard_modes(guard_mode):
     assert st["fp64_guard_steps"] > 0
 
 
+def test_gpu_mixed_turning_point_guard():
+    """guard_mode 1 with a guard radius no ray reaches: the FP64 attempts are
+    the turning-point retakes (dtheta/dlambda changes sign across an FP32
+    attempt), a few per ray, fewer than with the default radius.  Total I and
+    the pixels meet the mixed bars against the FP64 oracle; the LP and CP
+    fractions of this 40x40 image are near-cancelling sums that any change of
+    rounding along a few photon-ring rays moves by ~1e-3 relative (CPU lane
+    emulation: 1.4e-5 with no FP64 guard, 2.6e-3 with the retakes alone,
+    1.5e-4 with both guards), so they are held in absolute terms here; the
+    default guards are held to the relative bars in test_gpu_mixed_guard_modes."""
+    import oracle
+    from kpolaris_amd import abi
+    p, m, _, _ = _options("riaf_m87_c2.par", nx=40, ny=40).resolve_riaf()
+    print("case: mixed guard_mode 1, turning-point guard only, 40x40", flush=True)
+    ref, _ = oracle.run_riaf_image(p, m)
+    got, st = _gpu_image(p, m, precision=abi.PRECISION_MIXED, guard_mode=1, fp64_radius=1e-9)
+    _, st2 = _gpu_image(p, m, precision=abi.PRECISION_MIXED, guard_mode=1)
+    print(parity.assert_mixed(got, ref, "mixed turning-point guard", pol="absolute"), st["fp64_guard_steps"],
+          st2["fp64_guard_steps"], flush=True)
+    assert 0 < st["fp64_guard_steps"] < 0.01 * st["rhs_evals"] / 11
+    assert st["fp64_guard_steps"] < st2["fp64_guard_steps"]
+
+
 def test_gpu_mixed_edge_shapes_and_repeat():
     """1x1 and ragged 7x3 images; a repeated run gives identical bits."""
     from kpolaris_amd import abi

@@ -146,12 +146,30 @@ def test_emulated_mixed_precision_dexter_fixed_step(emul):
 
 
 def test_emulated_mixed_precision_unguarded(emul):
-    """Without the FP64 guard radius every attempt is FP32-derived; still in bounds."""
+    """With fp64_radius 0 (every FP64 guard off) every attempt is FP32-derived;
+    still in bounds on this coarse image."""
     p, m, _, _ = _opts("riaf_m87_c2.par", nx=12, ny=12)
     e, _, guard = _run_emul_mixed(emul, p, m, fp64_radius=0.0)
     assert guard == 0
     r, _ = oracle.run_riaf_image(p, m)
     parity.assert_mixed(e, r, "emulated mixed unguarded")
+
+
+def test_emulated_mixed_turning_point_guard(emul):
+    """With a guard radius no ray reaches (1e-9 M) the FP64 attempts are the
+    turning-point retakes alone (an FP32 attempt across which dtheta/dlambda
+    changes sign is taken again in FP64): present, a few per ray (under 1 % of
+    the attempts), fewer than with the default radius, and the image stays
+    within the mixed bars."""
+    p, m, _, _ = _opts("riaf_m87_c2.par", nx=20, ny=20)
+    r, _ = oracle.run_riaf_image(p, m)
+    _, _, g0 = _run_emul_mixed(emul, p, m, fp64_radius=0.0)
+    turn, rhs, g1 = _run_emul_mixed(emul, p, m, fp64_radius=1e-9)
+    _, _, g2 = _run_emul_mixed(emul, p, m)
+    assert g0 == 0
+    assert 0 < g1 < 0.01 * rhs / 11
+    assert g1 < g2
+    parity.assert_mixed(turn, r, "emulated mixed, turning-point guard only")
 
 
 def _run_emul_mf(L, p, m, freqs, ctrl=None, mixed=False):
