@@ -22,7 +22,7 @@ OPCOUNT = OUT_DIR / "opcount.json"
 
 SOURCES = ["kp_image.hip", "kp_host.cpp"]
 KERNELS = "kp_kernels.hip"
-NPARTS = 14  # KP_PART values of kp_kernels.hip
+NPARTS = 17  # KP_PART values of kp_kernels.hip
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -323,7 +323,11 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         mode = M_TANGENT;
     }
     P.out = *out;
-    P.persistent = opt.persistent ? 1 : 0;
+    // an image with an equatorial wedge runs the wedge kernels: persistent
+    // split pass B (one ray per thread and unsplit launches give the same
+    // results)
+    const bool sel = P.wedge_hr > 0.0;
+    P.persistent = opt.persistent || sel ? 1 : 0;
     int device = 0;
     KP_HIP(hipGetDevice(&device));
     const size_t np = (size_t)P.npix;
@@ -332,14 +336,15 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const bool windowed = slow && source && base_mode(mode) == M_SLOW;
     if (slow && source && !windowed)
         return kp_internal::fail(KP_UNSUPPORTED, "slow-light blocks are built for images, not gradients");
-    const bool unsplit = opt.split_transport == 0 && !windowed;
+    const bool unsplit = opt.split_transport == 0 && !windowed && !sel;
     // single-frequency images: pass B in a radiating and a vacuum leg
-    const bool legs = !noalloc && mode == M_SINGLE && vacuum_legs(opt, true, P.outer, P.mass);
+    const bool legs = !noalloc && !sel && mode == M_SINGLE && vacuum_legs(opt, true, P.outer, P.mass);
     // Staged mixed precision (guard_mode 2, persistent split kernels): an
     // FP32-geodesic launch parks rays that reach r < fp64_radius, an FP64-geodesic launch
     // continues them and parks those back that reach r >= back_radius, then
     // FP32 again and a final FP64 launch; no wave mixes the two paths.
-    const bool staged = staged_run(opt, mode == M_SINGLE || mode == M_MF);
+    // (a direct_only image locates its vertical turns in FP64: guard_mode 1 kernels)
+    const bool staged = staged_run(opt, mode == M_SINGLE || mode == M_MF) && !P.direct_only && !sel;
     const int nst = mf ? P.nfreq : 1;
     const size_t nsnap = slow ? (size_t)slow->nsnap : 0;
     const WsLayout W = workspace_layout(np, (size_t)nst, staged || slow != nullptr || legs, nsnap, probe != nullptr);
@@ -551,8 +556,9 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             KP_HIP(hipMemsetAsync(Q.fetch, 0, sizeof(uint32_t), stream));
             KP_HIP(hipMemsetAsync(Q.count_out, 0, sizeof(uint32_t), stream));
             KP_HIP(hipEventRecord(bev.e[0], stream));
-            s2 = launch(mixed ? kpsel::pass_b_grmhd_mixed(true, M_SLOW_W) : kpsel::pass_b_slow_windowed(mode), smem_b,
-                        Q);
+            const void* wfn = sel ? kpsel::pass_b_sel(mixed, GP_64, mode == M_SLOW_NT ? M_SLOW_NT_W : M_SLOW_W)
+                                  : (mixed ? kpsel::pass_b_grmhd_mixed(true, M_SLOW_W) : kpsel::pass_b_slow_windowed(mode));
+            s2 = launch(wfn, smem_b, Q);
             if (s2 != KP_OK) return s2;
             KP_HIP(hipEventRecord(bev.e[1], stream));
             ++blocks;
@@ -616,7 +622,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         } else {
             Params Q = P;
             Q.fetch = P.queue + 1;
-            const void* fn = pass_b_fn(P.persistent != 0, mixed, gp_plain, mode);
+            const void* fn = sel ? kpsel::pass_b_sel(mixed, gp_plain, mode) : pass_b_fn(P.persistent != 0, mixed, gp_plain, mode);
             st = launch(fn, smem_b, Q);
         }
         if (st != KP_OK) return st;
@@ -715,6 +721,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
 struct kp_batch {
     int device = 0;
     bool mixed = false;
+    bool sel = false;  // some job has an equatorial wedge: the batch runs the wedge kernel
     int gp = GP_64;  // geodesic precision of the jobs (guard_mode 0: GP_64, 1: GP_GUARD)
     int block = 128;
     int grid_a = 1, grid_b = 1;
@@ -751,7 +758,7 @@ static kp_status batch_build_graph(kp_batch* b) {
     ap.sharedMemBytes = (unsigned)lds_bytes(b->block, 0);
     ap.kernelParams = bargs;
     hipKernelNodeParams bp = ap;
-    bp.func = const_cast<void*>(kpsel::pass_b_batch(b->mixed, b->gp));
+    bp.func = const_cast<void*>(b->sel ? kpsel::pass_b_batch_sel(b->mixed, b->gp) : kpsel::pass_b_batch(b->mixed, b->gp));
     bp.gridDim = dim3(b->grid_b);
     hipGraphNode_t nz, na, nb;
     KP_HIP(hipGraphAddKernelNode(&nz, b->graph, nullptr, 0, &zp));
@@ -788,6 +795,7 @@ extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, co
     std::vector<Params> table((size_t)njobs);
     std::vector<uint32_t> start((size_t)njobs + 1, 0);
     uint64_t total = 0;
+    bool sel = false;
     for (int j = 0; j < njobs; ++j) {
         if (jobs[j].radiation_substeps < 1)
             return kp_internal::fail(KP_INVALID_ARGUMENT, "batch radiation_substeps must be positive");
@@ -804,6 +812,7 @@ extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, co
         }
         table[j].out = jobs[j].out;
         table[j].persistent = 1;
+        sel = sel || table[j].wedge_hr > 0.0;
         start[j] = (uint32_t)total;
         total += (uint64_t)table[j].npix;
         if (total > 0x7fffffffull) return kp_internal::fail(KP_INVALID_ARGUMENT, "batch exceeds 2^31 rays");
@@ -827,6 +836,7 @@ extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, co
     kp_batch* b = new kp_batch();
     b->device = device;
     b->mixed = mixed;
+    b->sel = sel;
     b->gp = gp;
     b->block = block;
     b->njobs = njobs;
@@ -866,7 +876,7 @@ extern "C" kp_status kp_batch_create(const kp_batch_job* jobs, int32_t njobs, co
                 &ba, kpsel::pass_a_batch(b->gp), block,
                 lds_bytes(block, 0)) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &bb, kpsel::pass_b_batch(mixed, b->gp), block,
+                &bb, sel ? kpsel::pass_b_batch_sel(mixed, b->gp) : kpsel::pass_b_batch(mixed, b->gp), block,
                 lds_bytes(block, 0)) != hipSuccess) {
             st = kp_internal::fail(KP_HIP_ERROR, "occupancy query for the batch kernels failed");
         } else {

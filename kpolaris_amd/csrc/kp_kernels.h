@@ -75,6 +75,7 @@ enum : int {
     R_SLOW_TIME = 6,
     R_UNDERFLOW = 7,
     R_TIME = 8,
+    R_DIRECT = 9,
 };
 
 // Pass B variants: one frequency, several frequencies on shared geodesics,
@@ -159,6 +160,13 @@ struct Params {
     // integrator (PassAParams scalars)
     double mass, spin, inner, outer, step, tol, min_step, max_step;
     double max_rs, max_rd, max_ad, max_fd;
+    // direct_only: pass A ends at the first vertical turn of the ray
+    // (pass_a_core.hpp:240-275)
+    int32_t direct_only, wedge_samples;
+    // emission wedge: 0 < wedge_hr < 1 keeps the emissivities only where
+    // |cos theta| <= wedge_hr, resolved by wedge_samples samples across it
+    // (emission_selection.hpp:11-122); 0 otherwise
+    double wedge_hr;
     RiafDev<double> riaf;
     // nonthermal distributions (kappa, power law) of either model
     NtDev nt;
@@ -296,6 +304,21 @@ template <> struct GeomT<G_BL> {
         ksincos(x[2], &sn, &cs);
         return metric4_of(bl_gcov(P.mass, P.spin, x[1], sn, cs));
     }
+    // vertical_position_velocity (pass_a_core.hpp:202-224): the spin-axis
+    // height z = r cos(theta) and dz/ds from the position (x1, x2) and the
+    // wavevector components (k1, k2)
+    static KP_DEV void vertical(const Params&, double x1, double x2, double k1, double k2, double& z, double& v) {
+        double sn, cs;
+        ksincos(x2, &sn, &cs);
+        z = x1 * cs;
+        v = cs * k1 - x1 * sn * k2;
+    }
+    // equatorial_height_ratio (emission_selection.hpp:21-33): cos(theta)
+    static KP_DEV double height_ratio(const Params&, double, double x2) {
+        double sn, cs;
+        ksincos(x2, &sn, &cs);
+        return cs;
+    }
 };
 
 template <> struct GeomT<G_FMKS> {
@@ -304,6 +327,21 @@ template <> struct GeomT<G_FMKS> {
     static KP_DEV double radius(double x1) { return exp(x1); }
     static KP_DEV Metric4<double> metric(const Params& P, const double* x) {
         return fmks_gcov(P.mass, P.spin, P.fmks, x[1], x[2]);
+    }
+    // the FMKS theta depends on x1 and x2, and r = exp(x1)
+    static KP_DEV void vertical(const Params& P, double x1, double x2, double k1, double k2, double& z, double& v) {
+        const double r = exp(x1);
+        const FmksTheta<double> t = fmks_theta(P.fmks, x1, x2);
+        double sn, cs;
+        ksincos(t.th, &sn, &cs);
+        z = r * cs;
+        v = z * k1 - r * sn * (t.dth1 * k1 + t.dth2 * k2);
+    }
+    // the physical theta of the native (x1, x2)
+    static KP_DEV double height_ratio(const Params& P, double x1, double x2) {
+        double sn, cs;
+        ksincos(fmks_theta(P.fmks, x1, x2).th, &sn, &cs);
+        return cs;
     }
 };
 
@@ -356,8 +394,10 @@ struct LaneA {
     double h_current, h_try, call_min;
     int n, steps, steps_inside, entered, att, in_call, pixel, reason;
     // surface-crossing refinement, one trial per tick (refine: 0 off,
-    // 1 evaluating the full step, 2 bisecting); best_ep: the best state so
-    // far is in the endpoint buffer, not the lane state
+    // 1 evaluating the full step, 2 bisecting; 3 bisecting a vertical turn
+    // of a direct_only image, flo holding dz/ds at the step's start);
+    // best_ep: the best state so far is in the endpoint buffer, not the lane
+    // state
     double lo, hi, flo, fhi, trial, full_h, surface;
     int refine, iter, use_secant, crossing, best_ep;
     // staged mixed precision: the FP32 kernel hands a step that crosses a
@@ -437,11 +477,13 @@ KP_DEV void step_rk4(const Params& P, const GG& G, const LaneVec& s, double h, S
 template <int GEOM>
 KP_DEV bool refine_update(const Params& P, LaneA& L, const St<double>& t) {
     using GT = GeomT<GEOM>;
+    // the kept state's step fraction ends in L.lo
     auto keep = [&](const St<double>& v) {
         const size_t np = (size_t)P.npix;
 #pragma unroll
         for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = v.v[c];
         L.best_ep = 1;
+        L.lo = L.trial;
     };
     if (L.refine == 1) {
         if (!st_finite(t)) return true;
@@ -470,7 +512,6 @@ KP_DEV bool refine_update(const Params& P, LaneA& L, const St<double>& t) {
             keep(t);
             return true;
         } else if ((L.flo > 0.0) == (ft > 0.0)) {
-            L.lo = L.trial;
             L.flo = ft;
             keep(t);
         } else {
@@ -488,6 +529,65 @@ KP_DEV bool refine_update(const Params& P, LaneA& L, const St<double>& t) {
         if (secant > L.lo + gw && secant < L.hi - gw) trial = secant;
     }
     L.trial = trial;
+    return false;
+}
+
+// vertical_turn_bracket (pass_a_core.hpp:226-238): an away-from-midplane to
+// toward-midplane reversal on one side of the midplane, both heights above
+// 64 ulp of the radius scale.
+KP_DEV bool vertical_turn_bracket(double z0, double v0, double z1, double v1, double h, double radius_scale) {
+    const double floor = 64.0 * eps<double>() * radius_scale;
+    return absv(z0) > floor && absv(z1) > floor && ((z0 > 0.0) == (z1 > 0.0)) && z0 * v0 * h > 0.0 &&
+           z1 * v1 * h <= 0.0;
+}
+
+// truncate_at_first_vertical_turn (pass_a_core.hpp:240-275), started when the
+// step from the lane state to `last` (step h) brackets a vertical turn: the
+// lane then bisects the step, one trial path per tick on the step's own RK
+// composition (as the crossing refinement does), for the last point whose
+// dz/ds keeps the start's sign; that point is the endpoint (the start state
+// when no trial keeps it).  Returns true when the bisection has started.
+template <int GEOM>
+KP_DEV bool turn_start(const Params& P, LaneA& L, const St<double>& last, double h) {
+    using GT = GeomT<GEOM>;
+    double z0, v0, z1, v1;
+    GT::vertical(P, L.s[1], L.s[2], L.s[5], L.s[6], z0, v0);
+    GT::vertical(P, last.v[1], last.v[2], last.v[5], last.v[6], z1, v1);
+    const double scale = maxv(GT::radius(L.s[1]), GT::radius(last.v[1]));
+    if (!st_finite(last) || !vertical_turn_bracket(z0, v0, z1, v1, h, scale)) return false;
+    L.refine = 3;
+    L.full_h = h;
+    L.lo = 0.0;
+    L.hi = 1.0;
+    L.flo = v0;
+    L.iter = 0;
+    L.best_ep = 0;
+    L.trial = 0.5;
+    return true;
+}
+
+// One bisection trial t of the vertical turn; returns true when the
+// bisection has finished.
+template <int GEOM>
+KP_DEV bool turn_update(const Params& P, LaneA& L, const St<double>& t) {
+    using GT = GeomT<GEOM>;
+    double z, v;
+    GT::vertical(P, t.v[1], t.v[2], t.v[5], t.v[6], z, v);
+    if (!st_finite(t)) {
+        L.hi = L.trial;
+    } else if ((v > 0.0) == (L.flo > 0.0) && v != 0.0) {
+        L.lo = L.trial;
+        const size_t np = (size_t)P.npix;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = t.v[c];
+        L.best_ep = 1;
+    } else {
+        L.hi = L.trial;
+    }
+    if (++L.iter >= 48) return true;
+    const double mid = (L.lo + L.hi) * 0.5;
+    if (mid == L.lo || mid == L.hi) return true;
+    L.trial = mid;
     return false;
 }
 
@@ -540,8 +640,25 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
     } else {
         step_rk4<GP>(P, G, L.s, h_call, next, guard, L.refine);
     }
+    if (L.refine == 3) {
+        if (!turn_update<GEOM>(P, L, next)) return false;
+        L.refine = 0;
+        L.reason = R_DIRECT;
+        L.steps += 1;
+        steps_acc += 1;
+        return true;
+    }
     if (L.refine) {
         if (!refine_update<GEOM>(P, L, next)) return false;
+        if (P.direct_only && L.best_ep) {
+            // the refined crossing state (kept in the endpoint buffer, at
+            // step fraction lo) may itself lie past a vertical turn
+            const size_t np = (size_t)P.npix;
+            St<double> kept;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) kept.v[c] = P.ep[(size_t)c * np + L.pixel];
+            if (turn_start<GEOM>(P, L, kept, L.full_h * L.lo)) return false;
+        }
         L.refine = 0;
         L.reason = L.crossing;
         L.steps += 1;
@@ -627,6 +744,7 @@ KP_DEV bool pass_a_tick(const Params& P, const typename GeomT<GEOM>::GeoType& G,
         L.trial = 1.0;
         return false;
     }
+    if (P.direct_only && turn_start<GEOM>(P, L, next, used_h)) return false;
     st_store(L.s, next);
     L.steps += 1;
     steps_acc += 1;
@@ -642,7 +760,7 @@ KP_DEV void lane_a_store(const Params& P, const LaneA& L) {
 #pragma unroll
         for (int c = 0; c < 16; ++c) P.ep[(size_t)c * np + L.pixel] = L.s[c];
     }
-    const int valid = L.entered && (L.reason == R_INNER || L.reason == R_ESCAPED);
+    const int valid = L.reason == R_DIRECT || (L.entered && (L.reason == R_INNER || L.reason == R_ESCAPED));
     P.ep_valid[L.pixel] = valid;
     P.ep_steps[L.pixel] = L.steps;
     P.ep_reason[L.pixel] = L.reason;
@@ -999,6 +1117,14 @@ KP_DEV void start_call(LaneB& L, double req, double cmin, double cmax) {
 //    transfer of its own Stokes vector, held in LDS.
 template <bool MIXED, int MODE> struct Radiation;
 
+// Emission selection outside the equatorial wedge (apply_emission_selection,
+// emission_selection.hpp:35-42): every radiation form carries `dark`, set
+// per sample, and removes the emissivities of each coefficient set it makes
+// (absorption and Faraday terms stay).
+template <class C> KP_DEV void darken(C& c) {
+    c.jI = c.jQ = c.jU = c.jV = decltype(c.jI)(0);
+}
+
 KP_DEV void mf_strang(const Params& P, LaneB& L, int f, const Coeffs<double>& c, double dl) {
     Stokes<double> st = {L.mf[4 * f], L.mf[4 * f + 1], L.mf[4 * f + 2], L.mf[4 * f + 3]};
     strang_step(st, c, dl);
@@ -1023,9 +1149,11 @@ KP_DEV Coeffs<double> widen(const Coeffs<float>& c) {
 
 template <> struct Radiation<false, M_SINGLE> {
     Coeffs<double> c;
+    bool dark = false;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         riaf_coefficients(P.riaf, sample, c);
+        if (dark) darken(c);
         return 1;
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
@@ -1041,9 +1169,11 @@ template <> struct Radiation<false, M_SINGLE> {
 
 template <> struct Radiation<true, M_SINGLE> {
     Coeffs<float> c;
+    bool dark = false;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         riaf_coefficients_scaled(P.rm, sample, c);
+        if (dark) darken(c);
         return 1;
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
@@ -1066,6 +1196,7 @@ template <> struct Radiation<true, M_SINGLE_V> : Radiation<true, M_SINGLE> {};
 // The vacuum leg returns before any radiation code; this empty form only
 // completes the tick's template.
 struct RadiationNone {
+    bool dark = false;
     KP_DEV void zero() {}
     KP_DEV bool valid() const { return false; }
     KP_DEV uint32_t eval(const Params&, const St<double>&) { return 0; }
@@ -1084,10 +1215,12 @@ template <> struct Radiation<true, M_VAC> : RadiationNone {};
 // with NT false the nonthermal branch is compiled out.
 template <bool NT> struct RadiationMF {
     RiafLocal<double> loc;
+    bool dark = false;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
         riaf_coeffs_at(P.riaf, loc, freq, c, NT ? &P.nt : nullptr);
+        if (dark) darken(c);
     }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         riaf_local(P.riaf, sample, loc);
@@ -1098,7 +1231,7 @@ template <bool NT> struct RadiationMF {
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<double> c;
-            riaf_coeffs_at(P.riaf, loc, P.ctrl[f], c, NT ? &P.nt : nullptr);
+            at(P, P.ctrl[f], c);
             req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
         }
@@ -1107,7 +1240,7 @@ template <bool NT> struct RadiationMF {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<double> c;
-            riaf_coeffs_at(P.riaf, loc, P.freqs[f], c, NT ? &P.nt : nullptr);
+            at(P, P.freqs[f], c);
             mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
@@ -1120,18 +1253,23 @@ template <> struct Radiation<false, M_MF_NT> : RadiationMF<true> {};
 // one FP32 local plasma state, in nu^3-scaled units per unit |h|.
 struct RadiationScaledMF {
     RiafLocal<float> loc;
+    bool dark = false;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
+    KP_DEV void at_freq(const Params& P, float freq, float bconst, Coeffs<float>& c) const {
+        riaf_coeffs_scaled_at(P.rm, loc, freq, bconst, c);
+        if (dark) darken(c);
+    }
     // output frequency f (single pass: also the control frequency f)
     KP_DEV void at32(const Params& P, int f, Coeffs<float>& c) const {
-        riaf_coeffs_scaled_at(P.rm, loc, float(P.freqs[f]), P.bconst_f[f], c);
+        at_freq(P, float(P.freqs[f]), P.bconst_f[f], c);
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
         req = 1;
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<float> c;
-            riaf_coeffs_scaled_at(P.rm, loc, float(P.ctrl[f]), P.bconst_c[f], c);
+            at_freq(P, float(P.ctrl[f]), P.bconst_c[f], c);
             const Coeffs<double> d = widen(c);
             req = maxv(req, substep_count(d, h, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(d, h_next, 1.0, lmrs, P.max_rd, P.max_ad, P.max_fd));
@@ -1141,7 +1279,7 @@ struct RadiationScaledMF {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<float> c;
-            riaf_coeffs_scaled_at(P.rm, loc, float(P.freqs[f]), P.bconst_f[f], c);
+            at32(P, f, c);
             mf_strang32(P, L, f, c, float(absv(h)));
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
@@ -1157,10 +1295,12 @@ template <> struct Radiation<true, M_MF> : RadiationScaledMF {
 
 template <bool NT> struct RadiationGRMHD {
     RiafLocal<double> loc;
+    bool dark = false;
     KP_DEV void zero() { loc.valid = 0; }
     KP_DEV bool valid() const { return loc.valid; }
     KP_DEV void at(const Params& P, double freq, Coeffs<double>& c) const {
         grmhd_coeffs_at(P.grmhd, loc, freq, c, NT ? &P.nt : nullptr);
+        if (dark) darken(c);
     }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         grmhd_local(P.grmhd, sample, loc);
@@ -1171,7 +1311,7 @@ template <bool NT> struct RadiationGRMHD {
         pred = 1;
         for (int f = 0; f < P.nctrl; ++f) {
             Coeffs<double> c;
-            grmhd_coeffs_at(P.grmhd, loc, P.ctrl[f], c, NT ? &P.nt : nullptr);
+            at(P, P.ctrl[f], c);
             req = maxv(req, substep_count(c, h, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
             pred = maxv(pred, substep_count(c, h_next, P.dls_c[f], lmrs, P.max_rd, P.max_ad, P.max_fd));
         }
@@ -1180,7 +1320,7 @@ template <bool NT> struct RadiationGRMHD {
     KP_DEV uint32_t transfer(const Params& P, LaneB& L, double h) const {
         for (int f = 0; f < P.nfreq; ++f) {
             Coeffs<double> c;
-            grmhd_coeffs_at(P.grmhd, loc, P.freqs[f], c, NT ? &P.nt : nullptr);
+            at(P, P.freqs[f], c);
             mf_strang(P, L, f, c, absv(h) * P.dls_f[f]);
         }
         return loc.valid ? (uint32_t)P.nfreq : 0u;
@@ -1227,6 +1367,7 @@ template <bool NT> struct RadiationSlow {
     RiafLocal<double> lo, hi;
     double f;
     int k;
+    bool dark = false;
     KP_DEV void zero() {
         lo.valid = 0;
         hi.valid = 0;
@@ -1261,6 +1402,7 @@ template <bool NT> struct RadiationSlow {
             c.rU = w * c.rU + f * b.rU;
             c.rV = w * c.rV + f * b.rV;
         }
+        if (dark) darken(c);
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
         req = 1;
@@ -1295,6 +1437,7 @@ struct RadiationSlowMixed {
     RiafLocal<float> lo, hi;
     double f;
     int k;
+    bool dark = false;
     KP_DEV void zero() {
         lo.valid = 0;
         hi.valid = 0;
@@ -1333,6 +1476,7 @@ struct RadiationSlowMixed {
             c.rU = w * c.rU + ff * b.rU;
             c.rV = w * c.rV + ff * b.rV;
         }
+        if (dark) darken(c);
     }
     // output frequency q (single pass: also the control frequency q)
     KP_DEV void at32(const Params& P, int q, Coeffs<float>& c) const {
@@ -1364,6 +1508,7 @@ template <> struct Radiation<true, M_SLOW_W> : RadiationSlowMixed {};
 
 // Time probe: geometric stepping only (no coefficients, no substep control).
 template <> struct Radiation<false, M_PROBE> {
+    bool dark = false;
     KP_DEV void zero() {}
     KP_DEV uint32_t eval(const Params&, const St<double>&) { return 0; }
     KP_DEV uint32_t control(const Params&, double, double, double, int& req, int& pred) const {
@@ -1417,11 +1562,13 @@ KP_DEV void tangent_coeffs(const RiafDev<double>& M, const RiafLocal<double>& L,
 
 template <> struct Radiation<false, M_TANGENT> {
     Coeffs<D3> c;
+    bool dark = false;
     KP_DEV void zero() { coeffs_zero(c); }
     KP_DEV uint32_t eval(const Params& P, const St<double>& sample) {
         RiafLocal<double> L;
         riaf_local(P.riaf, sample, L);
         tangent_coeffs(P.riaf, L, c);
+        if (dark) darken(c);
         return 1;
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
@@ -1477,6 +1624,7 @@ template <> struct Radiation<false, M_STAN> {
     Coeffs<D3> c;
     double f;
     int k;
+    bool dark = false;
     KP_DEV void zero() {
         coeffs_zero(c);
         f = 0.0;
@@ -1509,6 +1657,7 @@ template <> struct Radiation<false, M_STAN> {
             c.rU = w * a.rU + ff * b.rU;
             c.rV = w * a.rV + ff * b.rV;
         }
+        if (dark) darken(c);
         return 1;
     }
     KP_DEV uint32_t control(const Params& P, double h, double h_next, double lmrs, int& req, int& pred) const {
@@ -1561,7 +1710,93 @@ KP_DEV int slow_pair(const Params& P, double t) {
 // launch_slow_light_rays_metric (slow_light_driver.hpp:1083-1483) across all
 // windows of the ray in turn, then advance_slow_light_vacuum_to_camera
 // (:1572-1671) for a ray still unfinished after the last window (vac).
-template <bool MIXED, int GP, int MODE>
+// The equatorial wedge on an accepted pass-B step (pass_b.hpp:640-655):
+// the step from the lane state to next (midpoint sample, step h > 0).  A
+// fixed-step image samples the step's midpoint (one RK4 step of h/2); the
+// step then needs wedge_samples samples across the wedge's 2 wedge_hr of
+// cos(theta) (equatorial_resolution_substeps, emission_selection.hpp:45-60:
+// the count is returned and the caller retakes the step), or is split at the
+// first wedge surface it crosses (refine_equatorial_boundary, :69-122: 48
+// bisection trials on the step's own RK composition, run in this tick; h,
+// sample, next and the camera crossing become the split step's).  Returns 1
+// when the step goes on.  The paths are FP64 in every precision.
+template <int GP, int GEOM>
+KP_DEV int wedge_step(const Params& P, const typename GeomT<GEOM>::GeoType& G, LaneB& L, double& h,
+                      St<double>& sample, St<double>& next, uint32_t& guard) {
+    using GT = GeomT<GEOM>;
+    if (!P.adaptive) step_rk4<GP>(P, G, L.s, h * 0.5, sample, guard, true);
+    const double outer = P.outer;
+    if (GT::radius(L.s[1]) > outer && GT::radius(sample.v[1]) > outer && GT::radius(next.v[1]) > outer) return 1;
+    const double hr = P.wedge_hr;
+    const double q0 = GT::height_ratio(P, L.s[1], L.s[2]);
+    const double qm = GT::height_ratio(P, sample.v[1], sample.v[2]);
+    const double q1 = GT::height_ratio(P, next.v[1], next.v[2]);
+    const double ratio = (absv(qm - q0) + absv(q1 - qm)) / (2.0 * hr / double(P.wedge_samples));
+    if (ratio > 1.000001) return int(minv(ceil(ratio), 1e6));
+    if (h == 0.0) return 1;
+    const double tol = 128.0 * eps<double>();
+    double edge = 0.0, lo = 0.0, hi = 1.0;
+    bool found = false;
+    for (int half = 0; half < 2 && !found; ++half) {
+        const double a = half ? qm : q0, b = half ? q1 : qm;
+        double earliest = 2.0;
+        for (int sign = -1; sign <= 1; sign += 2) {
+            const double surface = double(sign) * hr;
+            if (absv(q0 - surface) <= tol || absv(a - surface) <= tol) continue;
+            if ((a - surface) * (b - surface) > 0.0) continue;
+            const double fraction = (surface - a) / (b - a);
+            if (fraction < earliest) {
+                earliest = fraction;
+                edge = surface;
+                found = true;
+            }
+        }
+        if (found) {
+            lo = half ? 0.5 : 0.0;
+            hi = half ? 1.0 : 0.5;
+        }
+    }
+    if (!found) return 1;
+    // emission_selection_path_state: two RK4 half steps (adaptive) or one
+    // RK4 step; with two half steps `mid` is the first one's end
+    auto path = [&](double hh, St<double>& out, St<double>& mid) {
+        if (P.adaptive)
+            attempt<GP>(P, G, L.s, L.k1, hh, mid, out, guard, false, true);
+        else
+            step_rk4<GP>(P, G, L.s, hh, out, guard, true);
+    };
+    St<double> trial, mid;
+    double low_q = q0;
+    if (lo > 0.0) {
+        path(h * lo, trial, mid);
+        low_q = GT::height_ratio(P, trial.v[1], trial.v[2]);
+    }
+    const double low_value = low_q - edge;
+    for (int iter = 0; iter < 48; ++iter) {
+        const double m = (lo + hi) * 0.5;
+        if (m == lo || m == hi) break;
+        path(h * m, trial, mid);
+        const double value = GT::height_ratio(P, trial.v[1], trial.v[2]) - edge;
+        if ((value > 0.0) == (low_value > 0.0))
+            lo = m;
+        else
+            hi = m;
+    }
+    const double fraction = (lo + hi) * 0.5;
+    if (fraction <= tol || fraction >= 1.0 - tol) return 1;
+    h *= fraction;
+    path(h, next, sample);
+    if (!P.adaptive) step_rk4<GP>(P, G, L.s, h * 0.5, sample, guard, true);
+    const double s0 = GT::radius(L.s[1]) - P.radius;
+    const double s1 = GT::radius(next.v[1]) - P.radius;
+    L.crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+    return 1;
+}
+
+// SEL: the tick carries the equatorial wedge (wedge_step, the radiation
+// forms' dark flag); the kernels of images without a wedge are built
+// without that code.
+template <bool MIXED, int GP, int MODE, bool SEL = false>
 KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::GeoType& G, LaneB& L,
                         uint32_t& coeff_acc, uint32_t& transfer_acc, uint32_t& steps_acc, uint32_t& guard) {
     using GT = GeomT<geom_of<MODE>()>;
@@ -1739,8 +1974,9 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
         if (L.crosses) L.reached = 1;
         return false;
     }
-    const double h = L.h;
-    const double old_r = GT::radius(L.s[1]), sample_r = GT::radius(sample.v[1]), next_r = GT::radius(next.v[1]);
+    double h = L.h;
+    const double old_r = GT::radius(L.s[1]);
+    double sample_r = GT::radius(sample.v[1]), next_r = GT::radius(next.v[1]);
     if constexpr (SLOW) {
         if (L.vac) {
             // outward vacuum only (:1647-1660)
@@ -1773,8 +2009,26 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
         if (L.crosses) L.reached = 1;
         return false;
     }
+    if constexpr (SEL) {
+        if (P.wedge_hr > 0.0) {
+            const int n = wedge_step<GP, geom_of<MODE>()>(P, G, L, h, sample, next, guard);
+            if (n > 1) {
+                if (h <= eff_min * 1.0001) {
+                    L.reason = R_UNDERFLOW;
+                    return true;
+                }
+                L.h_current = maxv(eff_min, h / double(n));
+                L.cap = L.h_current;
+                return false;
+            }
+            L.h = h;
+            sample_r = GT::radius(sample.v[1]);
+            next_r = GT::radius(next.v[1]);
+        }
+    }
     Radiation<MIXED, MODE> rad;
     rad.zero();
+    if constexpr (SEL) rad.dark = P.wedge_hr > 0.0 && absv(GT::height_ratio(P, sample.v[1], sample.v[2])) > P.wedge_hr;
     if constexpr (SLOW) {
         const int radiating = sample_r >= P.inner && sample_r < P.outer;
         const double ct = P.t_obs + sample.v[0];
@@ -1891,7 +2145,7 @@ KP_DEV bool pass_b_tick(const Params& P, const typename GeomT<geom_of<MODE>()>::
     return false;
 }
 
-template <bool PERSISTENT, bool MIXED, int GP, int MODE>
+template <bool PERSISTENT, bool MIXED, int GP, int MODE, bool SEL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32 ? KP_WAVES_PER_EU_F32 : KP_WAVES_PER_EU))) void pass_b_kernel(Params P) {
     uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
     const auto G = GeomT<geom_of<MODE>()>::geo(P, &rhs_count);
@@ -1938,7 +2192,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
                 // windowed slow light, radiating leg: a ray that parks leaves the launch
                 bool park = false;
                 if (has) {
-                    if (pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+                    if (pass_b_tick<MIXED, GP, MODE, SEL>(P, G, L, coeff, transfer, steps_b, guard)) {
                         lane_b_finish<MIXED, MODE>(P, L, 1);
                         has = false;
                     } else if (L.phase == PH_PARK) {
@@ -1948,7 +2202,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
                     }
                 }
                 append_parked(P, park, L.pixel);
-            } else if (has && pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+            } else if (has && pass_b_tick<MIXED, GP, MODE, SEL>(P, G, L, coeff, transfer, steps_b, guard)) {
                 lane_b_finish<MIXED, MODE>(P, L, 1);
                 has = false;
             }
@@ -1956,7 +2210,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
     } else {
         const int pix = blockIdx.x * blockDim.x + threadIdx.x;
         if (pix < P.npix && lane_b_init<MODE>(P, pix, L)) {
-            while (!pass_b_tick<MIXED, GP, MODE>(P, G, L, coeff, transfer, steps_b, guard)) {
+            while (!pass_b_tick<MIXED, GP, MODE, SEL>(P, G, L, coeff, transfer, steps_b, guard)) {
             }
             lane_b_finish<MIXED, MODE>(P, L, 1);
         }
@@ -2074,7 +2328,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
     flush_counters(B.counters, C_RHS_A, rhs_count, 0, 0, steps_a, 0, guard);
 }
 
-template <bool MIXED, int GP>
+template <bool MIXED, int GP, bool SEL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PER_EU))) void pass_b_batch_kernel(BatchArgs B) {
     uint32_t rhs_count = 0, coeff = 0, transfer = 0, steps_b = 0, guard = 0;
     LaneB L;
@@ -2097,7 +2351,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP_WAVES_PE
         if (!__ballot(has)) break;
         if (has) {
             const Geo<double> G{P->mass, P->spin, &rhs_count};
-            if (pass_b_tick<MIXED, GP, M_SINGLE>(*P, G, L, coeff, transfer, steps_b, guard)) {
+            if (pass_b_tick<MIXED, GP, M_SINGLE, SEL>(*P, G, L, coeff, transfer, steps_b, guard)) {
                 lane_b_finish<MIXED, M_SINGLE>(*P, L, 1);
                 has = false;
             }
@@ -2152,4 +2406,10 @@ const void* pass_b_grmhd_mixed(bool persistent, int mode);
 const void* unsplit_grmhd_mixed(int mode);
 const void* pass_b_slow_mixed(bool persistent, int mode);
 const void* unsplit_slow_mixed(int mode);
+// persistent split pass B with the equatorial wedge (images with a wedge run
+// these whatever the launch options ask: the results are the same)
+const void* pass_b_sel(bool mixed, int gp, int mode);
+const void* pass_b_sel_fmks(bool mixed, int mode);
+const void* pass_b_sel_fmks2(bool mixed, int mode);
+const void* pass_b_batch_sel(bool mixed, int gp);
 }  // namespace kpsel

@@ -227,6 +227,71 @@ const void* unsplit_slow_mixed(int mode) {
 }
 #endif
 
+#if KP_PART == 14
+// Images with an equatorial wedge: persistent split pass B on
+// Boyer-Lindquist geodesics (FP64 and mixed) and the batched pass B.
+const void* pass_b_sel(bool mixed, int gp, int mode) {
+    if (!mixed) {
+        switch (mode) {
+        case M_SINGLE: return (const void*)pass_b_kernel<true, false, GP_64, M_SINGLE, true>;
+        case M_MF: return (const void*)pass_b_kernel<true, false, GP_64, M_MF, true>;
+        case M_MF_NT: return (const void*)pass_b_kernel<true, false, GP_64, M_MF_NT, true>;
+        case M_TANGENT: return (const void*)pass_b_kernel<true, false, GP_64, M_TANGENT, true>;
+        default: return pass_b_sel_fmks(false, mode);
+        }
+    }
+    switch (mode) {
+    case M_SINGLE:
+        return gp == GP_GUARD ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_SINGLE, true>
+                              : (const void*)pass_b_kernel<true, true, GP_64, M_SINGLE, true>;
+    case M_MF:
+        return gp == GP_GUARD ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_MF, true>
+                              : (const void*)pass_b_kernel<true, true, GP_64, M_MF, true>;
+    default: return pass_b_sel_fmks(true, mode);
+    }
+}
+
+const void* pass_b_batch_sel(bool mixed, int gp) {
+    if (!mixed) return (const void*)pass_b_batch_kernel<false, GP_64, true>;
+    return gp == GP_GUARD ? (const void*)pass_b_batch_kernel<true, GP_GUARD, true>
+                          : (const void*)pass_b_batch_kernel<true, GP_64, true>;
+}
+#endif
+
+#if KP_PART == 15
+// Images with an equatorial wedge on FMKS geodesics: GRMHD and resident
+// slow light in FP64 here; the windowed and mixed kernels in part 16.
+const void* pass_b_sel_fmks(bool mixed, int mode) {
+    if (mixed || mode == M_SLOW_W || mode == M_SLOW_NT_W) return pass_b_sel_fmks2(mixed, mode);
+    switch (mode) {
+    case M_GRMHD: return (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD, true>;
+    case M_GRMHD_NT: return (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD_NT, true>;
+    case M_SLOW: return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW, true>;
+    case M_SLOW_NT: return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_NT, true>;
+    case M_STAN: return (const void*)pass_b_kernel<true, false, GP_64, M_STAN, true>;
+    default: return nullptr;
+    }
+}
+#endif
+
+#if KP_PART == 16
+const void* pass_b_sel_fmks2(bool mixed, int mode) {
+    if (mixed) {
+        switch (mode) {
+        case M_GRMHD: return (const void*)pass_b_kernel<true, true, GP_64, M_GRMHD, true>;
+        case M_SLOW: return (const void*)pass_b_kernel<true, true, GP_64, M_SLOW, true>;
+        case M_SLOW_W: return (const void*)pass_b_kernel<true, true, GP_64, M_SLOW_W, true>;
+        default: return nullptr;
+        }
+    }
+    switch (mode) {
+    case M_SLOW_W: return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_W, true>;
+    case M_SLOW_NT_W: return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_NT_W, true>;
+    default: return nullptr;
+    }
+}
+#endif
+
 #if KP_PART == 9
 const void* unsplit_slow(int mode) {
     switch (mode) {

@@ -18,11 +18,15 @@
 static kp_status prepare_common(const kp_pass_a_params* p, Params& P) {
     if (p->camera.model != KP_CAMERA_PINHOLE)
         return kp_internal::fail(KP_UNSUPPORTED, "only the pinhole camera is built");
-    if (p->direct_only)
-        return kp_internal::fail(KP_UNSUPPORTED, "direct_only vertical-turn truncation is not implemented");
+    // the option rules of kpolaris_model_image_options.cpp:196-209
     const double hr = p->emission_selection.equatorial_h_over_r;
-    if (hr > 0.0 && hr < 1.0)
-        return kp_internal::fail(KP_UNSUPPORTED, "equatorial emission wedges are not implemented");
+    if (!std::isfinite(hr) || hr < 0.0 || hr > 1.0 || (hr > 0.0 && hr < 1e-6))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "equatorial_h_over_r must be 0 (disabled) or in [1e-6,1]");
+    const int32_t samples = p->emission_selection.equatorial_samples;
+    if (hr > 0.0 && hr < 1.0 && (samples < 2 || samples > 1024))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "equatorial_samples must be an integer in [2,1024]");
+    if (p->direct_only != 0 && p->direct_only != 1)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "direct_only must be 0 or 1");
     if (p->camera.nx <= 0 || p->camera.ny <= 0)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "camera nx and ny must be positive");
     const long long npix = (long long)p->camera.nx * (long long)p->camera.ny;
@@ -56,6 +60,10 @@ static kp_status prepare_common(const kp_pass_a_params* p, Params& P) {
     P.max_rd = p->max_radiation_depth;
     P.max_ad = p->max_absorption_depth;
     P.max_fd = p->max_faraday_depth;
+    P.direct_only = p->direct_only;
+    // a wedge of 0 (disabled) or 1 (the whole sphere) selects nothing
+    P.wedge_hr = hr > 0.0 && hr < 1.0 ? hr : 0.0;
+    P.wedge_samples = samples;
     return KP_OK;
 }
 

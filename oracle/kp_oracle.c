@@ -604,12 +604,20 @@ void kpo_fmks_transport_rhs_fd(double mass, double spin, const kpo_fmks* f, cons
 typedef struct metric_ctx {
     double mass, spin;
     int64_t* rhs_counter;
-    int fmks;               /* 0: Boyer-Lindquist, 1: FMKS */
+    int fmks;               /* 0: Boyer-Lindquist, 1: FMKS, 2: the flat test path below */
     const kpo_fmks* fm;
 } metric_ctx;
 
+/* Kind 2 is a manufactured test path, not a spacetime: zero connection
+ * (straight lines, constant wavevector and basis), radius 1 everywhere and
+ * the equatorial height ratio x[3] -- the LinearHeight fixture of the
+ * reference's emission-selection tests (tests/test_equatorial.cpp:14-21). */
 static inline void rhs(const metric_ctx* m, const kpo_state* s, kpo_state* d) {
     if (m->rhs_counter) ++*m->rhs_counter;
+    if (m->fmks == 2) {
+        for (int i = 0; i < 16; ++i) d->v[i] = i < 4 ? s->v[i + 4] : 0.0;
+        return;
+    }
     if (m->fmks)
         kpo_fmks_transport_rhs(m->mass, m->spin, m->fm, s, d);
     else
@@ -617,7 +625,9 @@ static inline void rhs(const metric_ctx* m, const kpo_state* s, kpo_state* d) {
 }
 
 /* metric_domain_radius (metric_domain.hpp:159-172): r = x1 (BL) or exp(x1) (FMKS). */
-static inline double radius_of(const metric_ctx* m, const double* x) { return m->fmks ? exp(x[1]) : x[1]; }
+static inline double radius_of(const metric_ctx* m, const double* x) {
+    return m->fmks == 2 ? 1.0 : (m->fmks ? exp(x[1]) : x[1]);
+}
 
 static void metric_gcov(const metric_ctx* m, const double* x, double* g) {
     if (m->fmks)
@@ -2126,8 +2136,180 @@ static void locate_crossing(const metric_ctx* m, const kpo_state* first, const k
     }
 }
 
+/* vertical_position_velocity (pass_a_core.hpp:202-224): the spin-axis height
+ * z = r cos(theta) and dz/ds along the ray; the FMKS theta depends on both x1
+ * and x2, and r = exp(x1). */
+static void vertical_pv(const metric_ctx* m, const kpo_state* s, double* z, double* dz) {
+    if (m->fmks) {
+        const double x1 = s->v[1], x2 = s->v[2];
+        const double r = exp(x1);
+        const double th = kpo_fmks_theta(m->fm, x1, x2);
+        const double dth = fmks_dtheta_dx1(m->fm, x1, x2) * s->v[5] + fmks_dtheta_dx2(m->fm, x1, x2) * s->v[6];
+        *z = r * cos(th);
+        *dz = *z * s->v[5] - r * sin(th) * dth;
+    } else {
+        *z = s->v[1] * cos(s->v[2]);
+        *dz = cos(s->v[2]) * s->v[5] - s->v[1] * sin(s->v[2]) * s->v[6];
+    }
+}
+
+/* vertical_turn_bracket (pass_a_core.hpp:226-238): an away-from-midplane to
+ * toward-midplane reversal on one side of the midplane, both heights above
+ * 64 ulp of the radius scale. */
+static int vertical_turn_bracket(double z0, double v0, double z1, double v1, double h, double radius_scale) {
+    const double floor = 64.0 * 0x1p-52 * radius_scale;
+    return absv(z0) > floor && absv(z1) > floor && ((z0 > 0.0) == (z1 > 0.0)) && z0 * v0 * h > 0.0 &&
+           z1 * v1 * h <= 0.0;
+}
+
+/* truncate_at_first_vertical_turn (pass_a_core.hpp:240-275): bisect the step
+ * on its own RK composition for the last point whose dz/ds keeps the start's
+ * sign; *last and *h become that point and its step. */
+static int truncate_vertical_turn(const metric_ctx* m, const kpo_state* first, kpo_state* last, double* h,
+                                  int composed) {
+    double z0, v0, z1, v1;
+    vertical_pv(m, first, &z0, &v0);
+    vertical_pv(m, last, &z1, &v1);
+    const double scale = maxv(radius_of(m, first->v), radius_of(m, last->v));
+    if (!finite_state(last) || !vertical_turn_bracket(z0, v0, z1, v1, *h, scale)) return 0;
+    double lo = 0.0, hi = 1.0;
+    kpo_state endpoint = *first;
+    for (int iter = 0; iter < 48; ++iter) {
+        const double mid = (lo + hi) * 0.5;
+        if (mid == lo || mid == hi) break;
+        kpo_state trial;
+        surface_path_state(m, first, *h * mid, composed, &trial);
+        double z, v;
+        vertical_pv(m, &trial, &z, &v);
+        if (!finite_state(&trial)) {
+            hi = mid;
+            continue;
+        }
+        if ((v > 0.0) == (v0 > 0.0) && v != 0.0) {
+            lo = mid;
+            endpoint = trial;
+        } else {
+            hi = mid;
+        }
+    }
+    *last = endpoint;
+    *h *= lo;
+    return 1;
+}
+
+/* ---- equatorial emission wedge (emission_selection.hpp:11-122) ---------- */
+
+/* equatorial_height_ratio (:21-33): cos of the physical polar angle (FMKS:
+ * mapped from x1, x2), x[3] on the flat test path. */
+static double height_ratio(const metric_ctx* m, const double* x) {
+    if (m->fmks == 2) return x[3];
+    if (m->fmks) return cos(kpo_fmks_theta(m->fm, x[1], x[2]));
+    return cos(x[2]);
+}
+
+static int narrow_wedge(const kp_emission_selection* sel) {
+    return sel->equatorial_h_over_r > 0.0 && sel->equatorial_h_over_r < 1.0;
+}
+
+/* apply_emission_selection (:35-42): outside the wedge only the emissivities
+ * are removed; the Faraday-rotation switch is applied where the model's
+ * coefficients are evaluated. */
+static void select_emission(const kp_emission_selection* sel, const metric_ctx* m, const kpo_state* s,
+                            kpo_coeffs* c) {
+    if (narrow_wedge(sel) && absv(height_ratio(m, s->v)) > sel->equatorial_h_over_r)
+        c->jI = c->jQ = c->jU = c->jV = 0.0;
+}
+
+/* equatorial_resolution_substeps (:45-60). */
+static int wedge_substeps(const kp_emission_selection* sel, const metric_ctx* m, const kpo_state* first,
+                          const kpo_state* middle, const kpo_state* last, double outer) {
+    if (!narrow_wedge(sel)) return 1;
+    if (radius_of(m, first->v) > outer && radius_of(m, middle->v) > outer && radius_of(m, last->v) > outer)
+        return 1;
+    const double q0 = height_ratio(m, first->v), qm = height_ratio(m, middle->v), q1 = height_ratio(m, last->v);
+    const double span = absv(qm - q0) + absv(q1 - qm);
+    const double allowed = 2.0 * sel->equatorial_h_over_r / (double)sel->equatorial_samples;
+    const double ratio = span / allowed;
+    return ratio > 1.000001 ? (int)minv(ceil(ratio), 1000000.0) : 1;
+}
+
+/* refine_equatorial_boundary (:69-122): split a resolved step at the first
+ * wedge surface it crosses (bisection on the step's own RK composition);
+ * *h, *last and *middle become the split step's. */
+static int refine_wedge(const kp_emission_selection* sel, const metric_ctx* m, const kpo_state* first,
+                        kpo_state* middle, kpo_state* last, double* h, int composed, double outer) {
+    if (!narrow_wedge(sel) || *h == 0.0) return 0;
+    if (radius_of(m, first->v) > outer && radius_of(m, middle->v) > outer && radius_of(m, last->v) > outer)
+        return 0;
+    const double tolerance = 128.0 * 0x1p-52;
+    const double hr = sel->equatorial_h_over_r;
+    const double q0 = height_ratio(m, first->v), qm = height_ratio(m, middle->v), q1 = height_ratio(m, last->v);
+    double edge = 0.0, lo = 0.0, hi = 1.0;
+    int found = 0;
+    for (int half = 0; half < 2 && !found; ++half) {
+        const double a = half ? qm : q0, b = half ? q1 : qm;
+        double earliest = 2.0;
+        for (int sign = -1; sign <= 1; sign += 2) {
+            const double surface = sign * hr;
+            if (absv(q0 - surface) <= tolerance || absv(a - surface) <= tolerance) continue;
+            if ((a - surface) * (b - surface) > 0.0) continue;
+            const double fraction = (surface - a) / (b - a);
+            if (fraction < earliest) {
+                earliest = fraction;
+                edge = surface;
+                found = 1;
+            }
+        }
+        if (found) {
+            lo = half ? 0.5 : 0.0;
+            hi = half ? 1.0 : 0.5;
+        }
+    }
+    if (!found) return 0;
+    kpo_state low;
+    surface_path_state(m, first, *h * lo, composed, &low);
+    const double low_value = height_ratio(m, low.v) - edge;
+    for (int iter = 0; iter < 48; ++iter) {
+        const double mid = (lo + hi) * 0.5;
+        if (mid == lo || mid == hi) break;
+        kpo_state trial;
+        surface_path_state(m, first, *h * mid, composed, &trial);
+        const double value = height_ratio(m, trial.v) - edge;
+        if ((value > 0.0) == (low_value > 0.0))
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const double fraction = (lo + hi) * 0.5;
+    if (fraction <= tolerance || fraction >= 1.0 - tolerance) return 0;
+    *h *= fraction;
+    surface_path_state(m, first, *h, composed, last);
+    rk4(m, first, *h * 0.5, middle);
+    return 1;
+}
+
+/* The pass-B step's wedge handling (pass_b.hpp:640-655): the fixed-step
+ * midpoint sample, then the resolution substep count (returned when above
+ * 1: the caller retakes the step), then the split at the first surface; 0
+ * when the step goes on (*crosses recomputed after a split). */
+static int wedge_step(const kp_pass_a_params* p, const metric_ctx* m, const kpo_state* old, kpo_state* sample,
+                      kpo_state* next, double* h, int* crosses) {
+    const kp_emission_selection* sel = &p->emission_selection;
+    if (!narrow_wedge(sel)) return 0;
+    if (!p->adaptive) rk4(m, old, *h * 0.5, sample);
+    const int n = wedge_substeps(sel, m, old, sample, next, p->outer_radius);
+    if (n > 1) return n;
+    if (refine_wedge(sel, m, old, sample, next, h, p->adaptive, p->outer_radius)) {
+        const double s0 = radius_of(m, old->v) - p->camera.radius;
+        const double s1 = radius_of(m, next->v) - p->camera.radius;
+        *crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+    }
+    return 0;
+}
+
 /* trace_pass_a_segment_endpoint_pixel_metric (pass_b.hpp:337-486), BL hooks
- * reduced per metric_domain.hpp:159-223 (no time domain, no step limits). */
+ * reduced per metric_domain.hpp:159-223 (no time domain, no step limits);
+ * direct_only ends the segment at the first vertical turn (:461-470). */
 static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const kpo_state* init,
                            kpo_state* state_out, int32_t* steps_out, int32_t* reason_out) {
     kpo_state state = *init;
@@ -2206,6 +2388,10 @@ static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const
             locate_crossing(m, &old, &next, used_h, surface, p->adaptive, &ev, &ev_h);
             next = ev;
             used_h = ev_h;
+        }
+        if (p->direct_only && truncate_vertical_turn(m, &old, &next, &used_h, p->adaptive))
+            crossing = KP_REASON_REACHED_DIRECT_TURN;
+        if (crossing != KP_REASON_NONE) {
             state = next;
             reason = crossing;
             steps += 1;
@@ -2215,12 +2401,63 @@ static int pass_a_endpoint(const kp_pass_a_params* p, const metric_ctx* m, const
         steps += 1;
         if (entered) steps_inside += 1;
     }
-    const int valid = entered && (reason == KP_REASON_REACHED_INNER_BOUNDARY ||
-                                  reason == KP_REASON_ESCAPED_DOMAIN);
+    const int valid = reason == KP_REASON_REACHED_DIRECT_TURN ||
+                      (entered && (reason == KP_REASON_REACHED_INNER_BOUNDARY || reason == KP_REASON_ESCAPED_DOMAIN));
     *state_out = state;
     *steps_out = steps;
     *reason_out = reason;
     return valid;
+}
+
+void kpo_vertical_pv(const kpo_fmks* fm, const kpo_state* s, double* z, double* dzds) {
+    metric_ctx m = {0.0, 0.0, NULL, fm != NULL, fm};
+    vertical_pv(&m, s, z, dzds);
+}
+
+int32_t kpo_vertical_turn_bracket(double z0, double v0, double z1, double v1, double h, double radius_scale) {
+    return vertical_turn_bracket(z0, v0, z1, v1, h, radius_scale);
+}
+
+int32_t kpo_truncate_vertical_turn(double mass, double spin, const kpo_fmks* fm, const kpo_state* first,
+                                   kpo_state* last, double* h, int32_t composed) {
+    metric_ctx m = {mass, spin, NULL, fm != NULL, fm};
+    return truncate_vertical_turn(&m, first, last, h, composed);
+}
+
+/* The wedge pieces on metric kind (0 BL, 1 FMKS with fm, 2 the flat test path). */
+int32_t kpo_wedge_substeps(const kp_emission_selection* sel, int32_t kind, const kpo_fmks* fm,
+                           const kpo_state* first, const kpo_state* middle, const kpo_state* last, double outer) {
+    metric_ctx m = {1.0, 0.0, NULL, kind, fm};
+    return wedge_substeps(sel, &m, first, middle, last, outer);
+}
+
+int32_t kpo_refine_wedge(const kp_emission_selection* sel, int32_t kind, const kpo_fmks* fm,
+                         const kpo_state* first, kpo_state* middle, kpo_state* last, double* h, int32_t composed,
+                         double outer) {
+    metric_ctx m = {1.0, 0.0, NULL, kind, fm};
+    return refine_wedge(sel, &m, first, middle, last, h, composed, outer);
+}
+
+void kpo_select_emission(const kp_emission_selection* sel, int32_t kind, const kpo_fmks* fm, const kpo_state* s,
+                         kpo_coeffs* c) {
+    metric_ctx m = {1.0, 0.0, NULL, kind, fm};
+    select_emission(sel, &m, s, c);
+}
+
+double kpo_height_ratio(int32_t kind, const kpo_fmks* fm, const kpo_state* s) {
+    metric_ctx m = {1.0, 0.0, NULL, kind, fm};
+    return height_ratio(&m, s->v);
+}
+
+void kpo_path_state(int32_t kind, double mass, double spin, const kpo_fmks* fm, const kpo_state* first, double h,
+                    int32_t composed, kpo_state* out) {
+    metric_ctx m = {mass, spin, NULL, kind, fm};
+    surface_path_state(&m, first, h, composed, out);
+}
+
+void kpo_rk4_step(double mass, double spin, const kpo_fmks* fm, const kpo_state* s, double h, kpo_state* out) {
+    metric_ctx m = {mass, spin, NULL, fm != NULL, fm};
+    rk4(&m, s, h, out);
 }
 
 int32_t kpo_bl_pass_a_endpoint(const kp_pass_a_params* p, int32_t pixel, kpo_state* endpoint,
@@ -2769,16 +3006,134 @@ static double coeffs_at_frequency(const kp_pass_a_params* p, const model_ctx* mc
     return riaf_dlambda_scale(&mf);
 }
 
-/* trace_pass_b_segment_model_endpoint_pixel_metric (pass_b.hpp:515-722). */
+/* ---- parameter responses (src/diagnostics/response.hpp:25-191) ----------
+ * One ray's channels (response_config.hpp:28-32): `bins` source tags, four
+ * mechanism derivatives per bin (channel bins + m bins + b: emission,
+ * absorption, rotation rho_V, conversion rho_Q/rho_U), four full transfer
+ * reruns at q = +h, -h, +h/2, -h/2 (channel 5 bins + v); data[4 ch + k] holds
+ * component k (I, Q, U, V) of channel ch. */
+static kpo_stokes resp_read(const double* d, int ch) {
+    kpo_stokes s = {d[4 * ch], d[4 * ch + 1], d[4 * ch + 2], d[4 * ch + 3]};
+    return s;
+}
+static void resp_write(double* d, int ch, kpo_stokes s) {
+    d[4 * ch] = s.I;
+    d[4 * ch + 1] = s.Q;
+    d[4 * ch + 2] = s.U;
+    d[4 * ch + 3] = s.V;
+}
+static kpo_stokes resp_add(kpo_stokes a, kpo_stokes b) {
+    kpo_stokes s = {a.I + b.I, a.Q + b.Q, a.U + b.U, a.V + b.V};
+    return s;
+}
+
+/* ResponseStepMatrix (:37-53): the step's homogeneous operator, its columns
+ * the unit Stokes vectors stepped without emission. */
+typedef struct resp_matrix { kpo_stokes col[4]; } resp_matrix;
+static void resp_matrix_make(resp_matrix* M, kpo_coeffs c, double dl) {
+    c.jI = c.jQ = c.jU = c.jV = 0.0;
+    memset(M, 0, sizeof(*M));
+    M->col[0].I = 1.0;
+    M->col[1].Q = 1.0;
+    M->col[2].U = 1.0;
+    M->col[3].V = 1.0;
+    for (int k = 0; k < 4; ++k) kpo_semi_analytic_step(&M->col[k], &c, dl);
+}
+static kpo_stokes resp_apply(const resp_matrix* M, kpo_stokes s) {
+    const kpo_stokes* c = M->col;
+    kpo_stokes o = {c[0].I * s.I + c[1].I * s.Q + c[2].I * s.U + c[3].I * s.V,
+                    c[0].Q * s.I + c[1].Q * s.Q + c[2].Q * s.U + c[3].Q * s.V,
+                    c[0].U * s.I + c[1].U * s.Q + c[2].U * s.U + c[3].U * s.V,
+                    c[0].V * s.I + c[1].V * s.Q + c[2].V * s.U + c[3].V * s.V};
+    return o;
+}
+
+/* response_replace_block (:55-69). */
+static kpo_coeffs resp_replace(kpo_coeffs base, const kpo_coeffs* v, int mechanism) {
+    if (mechanism == 0) {
+        base.jI = v->jI; base.jQ = v->jQ; base.jU = v->jU; base.jV = v->jV;
+    } else if (mechanism == 1) {
+        base.aI = v->aI; base.aQ = v->aQ; base.aU = v->aU; base.aV = v->aV;
+    } else if (mechanism == 2) {
+        base.rV = v->rV;
+    } else {
+        base.rQ = v->rQ; base.rU = v->rU;
+    }
+    return base;
+}
+
+/* response_scale_coefficients (:71-78). */
+void kpo_response_scale_coefficients(const kpo_coeffs* c, double q, kpo_coeffs* out) {
+    const double f = exp(q);
+    *out = *c;
+    out->jI *= f; out->jQ *= f; out->jU *= f; out->jV *= f;
+    out->aI *= f; out->aQ *= f; out->aU *= f; out->aV *= f;
+    out->rQ *= f; out->rU *= f; out->rV *= f;
+}
+
+/* response_transfer_step (:118-149): the source tags and the mechanism
+ * derivatives propagate through the step's homogeneous operator, the step's
+ * emission enters the sample's bin, each mechanism's forcing is the central
+ * difference of the step with that block of coefficients replaced by the
+ * variants at +h and -h; the reruns take the full variant steps. */
+void kpo_response_transfer_step(int32_t bins, int32_t responses, double step, double* data, int32_t bin,
+                                const kpo_stokes* incoming, const kpo_coeffs* base, const kpo_coeffs* variants,
+                                double dl) {
+    resp_matrix M;
+    resp_matrix_make(&M, *base, dl);
+    kpo_stokes emitted = {0.0, 0.0, 0.0, 0.0};
+    kpo_semi_analytic_step(&emitted, base, dl);
+    for (int k = 0; k < bins; ++k) {
+        kpo_stokes s = resp_apply(&M, resp_read(data, k));
+        if (k == bin) s = resp_add(s, emitted);
+        resp_write(data, k, s);
+    }
+    if (!responses) return;
+    for (int m = 0; m < 4; ++m) {
+        kpo_stokes plus = *incoming, minus = *incoming;
+        kpo_coeffs cp = resp_replace(*base, &variants[0], m), cm = resp_replace(*base, &variants[1], m);
+        kpo_semi_analytic_step(&plus, &cp, dl);
+        kpo_semi_analytic_step(&minus, &cm, dl);
+        const double d = 2.0 * step;
+        const kpo_stokes forcing = {(plus.I - minus.I) / d, (plus.Q - minus.Q) / d, (plus.U - minus.U) / d,
+                                    (plus.V - minus.V) / d};
+        for (int k = 0; k < bins; ++k) {
+            const int ch = bins + m * bins + k;
+            kpo_stokes s = resp_apply(&M, resp_read(data, ch));
+            if (k == bin) s = resp_add(s, forcing);
+            resp_write(data, ch, s);
+        }
+    }
+    for (int v = 0; v < 4; ++v) {
+        const int ch = 5 * bins + v;
+        kpo_stokes s = resp_read(data, ch);
+        kpo_semi_analytic_step(&s, &variants[v], dl);
+        resp_write(data, ch, s);
+    }
+}
+
+/* A ray's response request: one radial bin (every sample), the plasma
+ * family (kpo_perturbation.parameter) and the difference step h; data holds
+ * the ray's 9 channels. */
+typedef struct resp_ctx {
+    int parameter;
+    double step;
+    double data[4 * 9];
+} resp_ctx;
+
+/* trace_pass_b_segment_model_endpoint_pixel_metric (pass_b.hpp:515-722);
+ * rc (single frequency) accumulates the ray's parameter responses as the
+ * analysis driver does (analysis_driver.hpp:585-613). */
 static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
                    const kpo_perturbation* pert, const freq_plan* fp, const metric_ctx* m,
                    const kpo_state* camera, const kpo_state* endpoint, int pass_a_steps,
-                   int pass_a_reason, int valid, pixel_result* out, kpo_stats* stats) {
+                   int pass_a_reason, int valid, pixel_result* out, kpo_stats* stats, resp_ctx* rc) {
     memset(out, 0, sizeof(*out));
     out->r11 = 1.0;
     out->r22 = 1.0;
     out->pass_a_steps = pass_a_steps;
     out->reason = pass_a_reason;
+    if (rc) memset(rc->data, 0, sizeof(rc->data));
     kpo_state state = *endpoint;
     if (!valid) {
         out->closure_x = spatial_distance(state.v, camera->v);
@@ -2827,8 +3182,7 @@ static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
         }
         const double s0 = radius_of(m, old.v) - p->camera.radius;
         const double s1 = radius_of(m, next.v) - p->camera.radius;
-        const int crosses = (s0 == 0.0 || s1 == 0.0) ? 1
-                            : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
+        int crosses = (s0 == 0.0 || s1 == 0.0) ? 1 : ((s0 < 0.0 && s1 > 0.0) || (s0 > 0.0 && s1 < 0.0));
         if (crosses) {
             const double denom = s0 - s1;
             const double frac = absv(denom) <= TINY ? 1.0 : minv(1.0, maxv(0.0, s0 / denom));
@@ -2858,6 +3212,16 @@ static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
                 }
             }
         }
+        const int wedge_n = wedge_step(p, m, &old, &sample, &next, &h, &crosses);
+        if (wedge_n > 1) {
+            if (h <= eff_min * 1.0001) {
+                reason = KP_REASON_ADAPTIVE_STEP_UNDERFLOW;
+                break;
+            }
+            h_current = maxv(eff_min, h / (double)wedge_n);
+            cap = h_current;
+            continue;
+        }
         const double old_r = radius_of(m, old.v), sample_r = radius_of(m, sample.v),
                      next_r = radius_of(m, next.v);
         const int inside = sample_r - p->inner_radius > 0.0 && sample_r < p->outer_radius;
@@ -2872,6 +3236,7 @@ static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
         int required = 1;
         for (int f = 0; f < fp->nctrl; ++f) {
             dls_c[f] = coeffs_at_frequency(p, model, fp->ctrl[f], &sample, inside, pert, &cc[f], stats);
+            select_emission(&p->emission_selection, m, &sample, &cc[f]);
             const int n = kpo_radiation_substep_count(&cc[f], h, dls_c[f], 1, local_max_rs,
                                                       p->max_radiation_depth, p->max_absorption_depth,
                                                       p->max_faraday_depth);
@@ -2906,6 +3271,19 @@ static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
                     dls = dls_c[f];
                 } else {
                     dls = coeffs_at_frequency(p, model, fp->freqs[f], &sample, inside, pert, &c, stats);
+                    select_emission(&p->emission_selection, m, &sample, &c);
+                }
+                if (rc && f == 0) {
+                    /* accumulate_response_sample (response.hpp:151-182): the
+                     * coefficients at q = +h, -h, +h/2, -h/2 of the family */
+                    kpo_coeffs var[4];
+                    for (int v = 0; v < 4; ++v) {
+                        const double q = rc->step * (v % 2 == 0 ? 1.0 : -1.0) * (v < 2 ? 1.0 : 0.5);
+                        const kpo_perturbation pv = {rc->parameter, 0, q};
+                        coeffs_at_frequency(p, model, fp->freqs[0], &sample, inside, &pv, &var[v], NULL);
+                        select_emission(&p->emission_selection, m, &sample, &var[v]);
+                    }
+                    kpo_response_transfer_step(1, 1, rc->step, rc->data, 0, &stokes[0], &c, var, absv(h) * dls);
                 }
                 kpo_semi_analytic_step(&stokes[f], &c, absv(h) * dls);
             }
@@ -2924,6 +3302,15 @@ static void pass_b(const kp_pass_a_params* p, const model_ctx* model,
     for (int f = 0; f < fp->nfreq; ++f)
         kpo_transform_to_observer_basis(&stokes[f], out->r11, out->r12, out->r21, out->r22,
                                         &out->observed[f]);
+    if (rc) {
+        /* finalize_response_basis (response.hpp:184-191) */
+        for (int ch = 0; ch < 9; ++ch) {
+            const kpo_stokes in = resp_read(rc->data, ch);
+            kpo_stokes o;
+            kpo_transform_to_observer_basis(&in, out->r11, out->r12, out->r21, out->r22, &o);
+            resp_write(rc->data, ch, o);
+        }
+    }
     out->closure_x = spatial_distance(state.v, camera->v);
     out->closure_k = max_abs_diff4(state.v + 4, camera->v + 4);
     out->final_null = dot_g(g, state.v + 4, state.v + 4);
@@ -3184,6 +3571,18 @@ static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, cons
                     }
                 }
             }
+            /* the wedge (slow_light_driver.hpp:1272-1291) */
+            const int wedge_n = wedge_step(p, m, &old, &sample, &next, &h, &crosses);
+            if (wedge_n > 1) {
+                if (h <= p->min_step * 1.0001) {
+                    reason = KP_REASON_ADAPTIVE_STEP_UNDERFLOW;
+                    active = 0;
+                    break;
+                }
+                h_current = maxv(p->min_step, h / (double)wedge_n);
+                cap = h_current;
+                continue;
+            }
             const double old_r = radius_of(m, old.v), sample_r = radius_of(m, sample.v),
                          next_r = radius_of(m, next.v);
             const int radiating = sample_r >= p->inner_radius && sample_r < p->outer_radius;
@@ -3212,6 +3611,7 @@ static void pass_b_slow(const kp_pass_a_params* p, const kp_grmhd_model* g, cons
                     if (stats) stats->coefficient_evals++;
                     kpo_slow_light_coefficients(&gf, seq, pair, &sample, pert,
                                                 p->emission_selection.faraday_rotation, &cc[f]);
+                    select_emission(&p->emission_selection, m, &sample, &cc[f]);
                 }
                 const int a = kpo_radiation_substep_count(&cc[f], h, dls[f], 1, local_max_rs, p->max_radiation_depth,
                                                           p->max_absorption_depth, p->max_faraday_depth);
@@ -3296,7 +3696,8 @@ static void store_pixel(kp_image_data* o, int p, int nout, int nfreq, const pixe
 static kp_status run_pixels_w(const kp_pass_a_params* p, const model_ctx* model,
                               const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
                               int count, int compact, int nout, int nthreads, kp_image_data* out,
-                              kpo_stats* stats, double* waiting) {
+                              kpo_stats* stats, double* waiting, const resp_ctx* resp_cfg,
+                              double* resp_out) {
     double tetrad[16], gcam[16];
     double xcam[4] = {0.0, p->camera.radius, p->camera.inclination, 0.0};
     const int fmks = p->coordinate_system == KP_COORD_FMKS;
@@ -3324,11 +3725,19 @@ static kp_status run_pixels_w(const kp_pass_a_params* p, const model_ctx* model,
         int32_t a_steps = 0, a_reason = 0;
         const int valid = pass_a_endpoint(p, &m, &camera, &endpoint, &a_steps, &a_reason);
         pixel_result r;
+        resp_ctx rc;
+        if (resp_cfg) rc = *resp_cfg;
         if (model->slow)
             pass_b_slow(p, model->grmhd, model->slow, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid,
                         &r, &local);
         else
-            pass_b(p, model, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local);
+            pass_b(p, model, pert, fp, &m, &camera, &endpoint, a_steps, a_reason, valid, &r, &local,
+                   resp_cfg ? &rc : NULL);
+        if (resp_cfg && resp_out) {
+            /* the reference's layout: component k of channel ch at (4 ch + k) npix + pixel */
+            const int o = compact ? i : pix;
+            for (int q = 0; q < 36; ++q) resp_out[(size_t)q * (size_t)nout + (size_t)o] = rc.data[q];
+        }
         if (r.waiting) {
 #ifdef _OPENMP
 #pragma omp critical
@@ -3362,7 +3771,8 @@ static kp_status run_pixels(const kp_pass_a_params* p, const model_ctx* model,
                             const kpo_perturbation* pert, const freq_plan* fp, int first, int stride,
                             int count, int compact, int nout, int nthreads, kp_image_data* out,
                             kpo_stats* stats) {
-    return run_pixels_w(p, model, pert, fp, first, stride, count, compact, nout, nthreads, out, stats, NULL);
+    return run_pixels_w(p, model, pert, fp, first, stride, count, compact, nout, nthreads, out, stats, NULL, NULL,
+                        NULL);
 }
 
 static kp_status check_params(const kp_pass_a_params* p, const kp_riaf_model* model, const void* out) {
@@ -3386,6 +3796,29 @@ kp_status kpo_run_riaf_image(const kp_pass_a_params* p, const kp_riaf_model* mod
     const model_ctx mc = {model, NULL, NULL};
     return run_pixels(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix, nthreads,
                       out, stats);
+}
+
+/* The RIAF image with the parameter responses of one plasma family
+ * (parameter 1 density, 2 temperature, 3 magnetic; PlasmaPerturbation,
+ * plasma_perturbation.hpp:12-33) at difference step h, one radial bin: resp
+ * [36][npix] holds the reference's channels (source, four mechanism
+ * derivatives, reruns at +h, -h, +h/2, -h/2) component-major. */
+kp_status kpo_run_riaf_response_image(const kp_pass_a_params* p, const kp_riaf_model* model, int32_t parameter,
+                                      double step, int32_t nthreads, kp_image_data* out, double* resp,
+                                      kpo_stats* stats) {
+    const kp_status st = check_params(p, model, out);
+    if (st != KP_OK) return st;
+    if (parameter < 1 || parameter > 3 || !(step > 0.0) || !resp) return KP_INVALID_ARGUMENT;
+    const int npix = p->camera.nx * p->camera.ny;
+    const double f0 = model->freq_cgs;
+    const freq_plan fp = {&f0, 1, &f0, 1, 1};
+    const model_ctx mc = {model, NULL, NULL};
+    const kpo_perturbation none = {0, 0, 0.0};
+    resp_ctx cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.parameter = parameter;
+    cfg.step = step;
+    return run_pixels_w(p, &mc, &none, &fp, 0, 1, npix, 0, npix, nthreads, out, stats, NULL, &cfg, resp);
 }
 
 /* Strided pixel sample pixel = offset + i * stride, i < count, outputs compact
@@ -3488,7 +3921,7 @@ kp_status kpo_run_grmhd_slow_light_image(const kp_pass_a_params* p, const kp_grm
         waiting[2] = -LARGE;
     }
     const kp_status st = run_pixels_w(p, &mc, pert, &fp, begin, 1, end > begin ? end - begin : 0, 0, npix,
-                                      nthreads, out, stats, waiting);
+                                      nthreads, out, stats, waiting, NULL, NULL);
     if (st != KP_OK) return st;
     return (waiting && waiting[0] > 0.0) ? KP_TIME_SUPPORT : KP_OK;
 }

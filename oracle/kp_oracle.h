@@ -125,6 +125,30 @@ int32_t kpo_radiation_substep_count(const kpo_coeffs* c, double geometric_step,
 
 /* ---- passes and images ----------------------------------------------- */
 /* Pass A segment endpoint for one pixel (BL). Returns valid flag. */
+/* direct_only pieces (pass_a_core.hpp:202-275): the vertical position and
+ * velocity of a state (fm NULL: Boyer-Lindquist, else FMKS with this map),
+ * the vertical-turn bracket, and the truncation of one RK step (h and last
+ * updated; returns 1 when the step brackets a turn). */
+void kpo_vertical_pv(const kpo_fmks* fm, const kpo_state* s, double* z, double* dzds);
+int32_t kpo_vertical_turn_bracket(double z0, double v0, double z1, double v1, double h, double radius_scale);
+int32_t kpo_truncate_vertical_turn(double mass, double spin, const kpo_fmks* fm, const kpo_state* first,
+                                   kpo_state* last, double* h, int32_t composed);
+/* Equatorial emission wedge pieces (emission_selection.hpp:21-122) on metric
+ * kind 0 (BL), 1 (FMKS with fm) or 2 (a flat test path: zero connection,
+ * radius 1, height ratio x[3]). */
+int32_t kpo_wedge_substeps(const kp_emission_selection* sel, int32_t kind, const kpo_fmks* fm,
+                           const kpo_state* first, const kpo_state* middle, const kpo_state* last, double outer);
+int32_t kpo_refine_wedge(const kp_emission_selection* sel, int32_t kind, const kpo_fmks* fm,
+                         const kpo_state* first, kpo_state* middle, kpo_state* last, double* h, int32_t composed,
+                         double outer);
+void kpo_select_emission(const kp_emission_selection* sel, int32_t kind, const kpo_fmks* fm, const kpo_state* s,
+                         kpo_coeffs* c);
+double kpo_height_ratio(int32_t kind, const kpo_fmks* fm, const kpo_state* s);
+void kpo_path_state(int32_t kind, double mass, double spin, const kpo_fmks* fm, const kpo_state* first, double h,
+                    int32_t composed, kpo_state* out);
+/* One RK4 step of the transport ODE (fm NULL: Boyer-Lindquist). */
+void kpo_rk4_step(double mass, double spin, const kpo_fmks* fm, const kpo_state* s, double h, kpo_state* out);
+
 int32_t kpo_bl_pass_a_endpoint(const kp_pass_a_params* params, int32_t pixel,
                                kpo_state* endpoint, int32_t* steps, int32_t* reason);
 
@@ -143,6 +167,19 @@ kp_status kpo_run_riaf_multifrequency_image(const kp_pass_a_params* params, cons
                                             int32_t nfreq, const double* ctrl, int32_t nctrl,
                                             int32_t pixel_begin, int32_t pixel_end, int32_t nthreads,
                                             kp_image_data* out, kpo_stats* stats);
+
+/* Parameter responses (src/diagnostics/response.hpp:25-191): one step of
+ * the channels (bins source tags, four mechanism derivatives per bin, four
+ * reruns; data[4 ch + k]), the coefficient scaling of the "coefficients"
+ * family, and the RIAF image with one family's responses (parameter 1
+ * density, 2 temperature, 3 magnetic; one radial bin; resp [36][npix]). */
+void kpo_response_transfer_step(int32_t bins, int32_t responses, double step, double* data, int32_t bin,
+                                const kpo_stokes* incoming, const kpo_coeffs* base, const kpo_coeffs* variants,
+                                double dl);
+void kpo_response_scale_coefficients(const kpo_coeffs* c, double q, kpo_coeffs* out);
+kp_status kpo_run_riaf_response_image(const kp_pass_a_params* p, const kp_riaf_model* model, int32_t parameter,
+                                      double step, int32_t nthreads, kp_image_data* out, double* resp,
+                                      kpo_stats* stats);
 
 kp_status kpo_run_riaf_image_strided(const kp_pass_a_params* params, const kp_riaf_model* model,
                                      const kpo_perturbation* pert, int32_t offset, int32_t stride,

@@ -78,7 +78,9 @@ static kp_status emul_run(const kp_pass_a_params* pa, const kp_riaf_model* m, do
         B.mf = LdsVec<double>{mfbuf, 1};
         B.mf2 = LdsVec<double>{mfbuf2, 1};
         if (lane_b_init<MODE>(P, pix, B)) {
-            while (!pass_b_tick<MIXED, GP, MODE>(P, G, B, c1, c2, sb, guard)) {
+            // the lanes with the equatorial-wedge code (the library's wedge
+            // kernels); without a wedge it is skipped at run time
+            while (!pass_b_tick<MIXED, GP, MODE, true>(P, G, B, c1, c2, sb, guard)) {
             }
             lane_b_finish<MIXED, MODE>(P, B, 1);
         }
@@ -335,7 +337,7 @@ static kp_status emul_slow_blocks(const kp_pass_a_params* pa, const kp_grmhd_mod
                 lane_b_resume<MODE>(Q, (int)Q.list_in[w], B);
             }
             while (has) {
-                if (pass_b_tick<false, GP_64, MODE>(Q, G, B, c1, c2, sb, guard)) {
+                if (pass_b_tick<false, GP_64, MODE, true>(Q, G, B, c1, c2, sb, guard)) {
                     lane_b_finish<false, MODE>(Q, B, 1);
                     break;
                 }

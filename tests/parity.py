@@ -25,7 +25,12 @@ around the unperturbed oracle is the pixel's measured sensitivity.
   a ray that approaches a snapshot time by ever smaller cut steps (the
   reference stops when the remaining step is below 1e-14) may take one step
   more or less depending on the last bits of x^0; its Stokes values must
-  still meet the bar (time_cut_steps=1).
+  still meet the bar (time_cut_steps=1).  The other is the equatorial
+  emission wedge, whose boundary split (a 48-step bisection on the step's
+  RK path) decides on the last bits of cos(theta): the oracle's own 1e-15
+  perturbations move the step counts of about a third of a wedge image's
+  pixels, and a kernel's rounding moves a few more by up to 0.2 % of their
+  steps (WEDGE_MAX_ILL, WEDGE_STEP_REL); the Stokes bar is unchanged.
 """
 from __future__ import annotations
 
@@ -46,6 +51,10 @@ AMPLIFICATION = 10.0     # the bar's multiple of the oracle's measured spread
 MAX_ILL = 0.1
 # share of pixels on which a Stokes component's derivative must be checkable
 MIN_CHECKED = 0.35
+# equatorial-wedge images (module docstring): ill-conditioned share and the
+# relative step-count difference allowed on well-conditioned pixels
+WEDGE_MAX_ILL = 0.5
+WEDGE_STEP_REL = 2e-3
 STOKES = ("image_i", "image_q", "image_u", "image_v")
 OFFSETS = ((PERTURBATION, 0.0), (-PERTURBATION, 0.0), (0.0, PERTURBATION), (0.0, -PERTURBATION))
 
@@ -139,7 +148,8 @@ def smooth_for_derivatives(ref, cond, time_cut_steps=0):
     return ok
 
 
-def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, diagnostics=True, peak=None):
+def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, diagnostics=True, peak=None,
+          step_rel=0.0):
     """The parity rule of this module for a [nfreq][npix]-Stokes image.
     peak: the image's peak I for a sample of its pixels, one value or one
     per frequency (default: the sample's own, per frequency).  Returns a summary (printed by the tests):
@@ -178,7 +188,7 @@ def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, dia
             worst_tight = max(worst_tight, float(np.max(err / tight, initial=0.0)))
     dsteps = np.abs(got["steps"] - ref["steps"])
     da = got["pass_a_steps"] != ref["pass_a_steps"]
-    steps_bad = ((dsteps > time_cut_steps) | da) & ~ill
+    steps_bad = ((dsteps > time_cut_steps + step_rel * ref["steps"]) | da) & ~ill
     assert not steps_bad.any(), (
         f"{label}: step counts differ on well-conditioned pixels {np.flatnonzero(steps_bad)[:8]}")
     if diagnostics:
@@ -198,17 +208,18 @@ def check(got, ref, cond, label, nfreq=1, max_ill=MAX_ILL, time_cut_steps=0, dia
             "step_count_differences": int(((dsteps > 0) | da).sum())}
 
 
-def assert_parity(got, ref, label, peak=None, diagnostics=True, max_ill=MAX_ILL):
+def assert_parity(got, ref, label, peak=None, diagnostics=True, max_ill=MAX_ILL, step_rel=0.0):
     """Single-frequency RIAF image against oracle_with_sensitivity's result;
     peak: the full image's peak I when got/ref are a sample of its pixels."""
-    return check(got, ref, ref["_cond"], label, 1, max_ill=max_ill, diagnostics=diagnostics, peak=peak)
+    return check(got, ref, ref["_cond"], label, 1, max_ill=max_ill, diagnostics=diagnostics, peak=peak,
+                 step_rel=step_rel)
 
 
-def assert_parity_grid(got, ref, nfreq, label, cond, max_ill=MAX_ILL, time_cut_steps=0):
+def assert_parity_grid(got, ref, nfreq, label, cond, max_ill=MAX_ILL, time_cut_steps=0, step_rel=0.0):
     """GRMHD, slow-light or multi-frequency image ([nfreq][npix] Stokes)
     against the oracle with the conditioning `cond` of the same runner."""
     return check(got, ref, cond, label, nfreq, max_ill=max_ill, time_cut_steps=time_cut_steps,
-                 diagnostics=False)
+                 diagnostics=False, step_rel=step_rel)
 
 
 def split_frequencies(img, nfreq):

@@ -66,6 +66,43 @@ def test_gpu_gradient_against_oracle_central_differences():
             assert err < 1e-6, (k, name, err)
 
 
+def test_gpu_gradient_against_oracle_responses():
+    """The device tangents against the reference's parameter-response maps
+    restated in the oracle (src/diagnostics/response.hpp: mechanism
+    forcings by central differences of coefficient blocks at step 1e-3,
+    propagated along the same geodesic and step sequence), for the density,
+    temperature and magnetic families, on every well-conditioned pixel (an
+    ill-conditioned ray's device geodesic is not the oracle's)."""
+    import ctypes as C
+    import oracle
+    from kpolaris_amd import abi, image
+    print("case: gradient 16x16 vs oracle response maps", flush=True)
+    p, m, _, _ = _opts(nx=16, ny=16)
+    _, grad, _ = image.run_riaf_image_gradient(p, m, (0.0, 0.0, 0.0))
+    g = grad.cpu().numpy()
+    ref = parity.oracle_with_sensitivity(p, m)
+    well = ~parity.ill_conditioned(ref, ref["_cond"], 1)
+    L = oracle.lib()
+    L.kpo_run_riaf_response_image.argtypes = [C.POINTER(abi.PassAParams), C.POINTER(abi.RIAFModel), C.c_int32,
+                                              C.c_double, C.c_int32, C.POINTER(abi.ImageData),
+                                              C.POINTER(C.c_double), C.POINTER(oracle.Stats)]
+    L.kpo_run_riaf_response_image.restype = C.c_int32
+    npix = 256
+    for k in range(3):
+        arrays = oracle.alloc_image(npix)
+        resp = np.zeros(36 * npix)
+        assert L.kpo_run_riaf_response_image(C.byref(p), C.byref(m), k + 1, 1e-3, 0,
+                                             C.byref(oracle.image_struct(arrays)),
+                                             resp.ctypes.data_as(C.POINTER(C.c_double)), C.byref(oracle.Stats())) == 0
+        total = resp.reshape(9, 4, npix)[1:5].sum(axis=0)
+        scale = np.abs(total).max(axis=1, keepdims=True)
+        err = np.abs(g[k] - total)
+        ok = err <= 1e-5 * np.abs(total) + 1e-6 * scale
+        print(f"q{k} max |tangent - response| / peak on {int(well.sum())} well-conditioned pixels "
+              f"{float((err[:, well] / scale).max()):.2e}", flush=True)
+        assert ok[:, well].all(), (k, np.flatnonzero(~ok.all(axis=0) & well)[:8])
+
+
 def test_gpu_gradient_at_a_perturbed_point():
     """Value at q = (0.3, 0, 0) equals the oracle's density-perturbed image;
     derivative there equals central differences of the device image."""
