@@ -263,7 +263,9 @@ typedef struct kp_run_options {
                                  (identical results; stats report the whole
                                  time as pass_a_ms) */
     int32_t block_size;       /* rays per workgroup; 0 = tuned default */
-    int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread */
+    int32_t persistent;       /* 1: lane-refill work queue; 0: one ray per thread in
+                                 pass A (pass B always refills lanes from the queue;
+                                 the results are the same) */
     int32_t waves_per_cu;     /* persistent grid size; 0 = occupancy-derived */
     int32_t guard_mode;       /* mixed: geodesic precision.  0 = FP64 geodesics,
                                  FP32 coefficients and transfer (default; FP32

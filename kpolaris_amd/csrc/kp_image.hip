@@ -154,20 +154,19 @@ bool vacuum_legs(const kp_run_options& opt, bool single_bl, double outer, double
 // when the combination is not built.
 const void* pass_a_fn(bool persistent, int gp, int geom) { return kpsel::pass_a(persistent, gp, geom); }
 
-// mixed: FP32 radiation (nu^3-scaled units); gp: geodesic precision.  The
-// staged variants (GP_32, and GP_64 with FP32 radiation) exist as persistent
-// kernels only.
-const void* pass_b_fn(bool persistent, bool mixed, int gp, int mode) {
+// mixed: FP32 radiation (nu^3-scaled units); gp: geodesic precision.  Pass
+// B kernels are persistent (a lane takes the next pixel from the queue).
+const void* pass_b_fn(bool mixed, int gp, int mode) {
     switch (mode) {
-    case M_SINGLE: return kpsel::pass_b_single(persistent, mixed, gp);
-    case M_MF: return kpsel::pass_b_mf(persistent, mixed, gp);
-    case M_GRMHD: return mixed ? kpsel::pass_b_grmhd_mixed(persistent, mode) : kpsel::pass_b_fmks(persistent, mode);
+    case M_SINGLE: return kpsel::pass_b_single(mixed, gp);
+    case M_MF: return kpsel::pass_b_mf(mixed, gp);
+    case M_GRMHD: return mixed ? kpsel::pass_b_grmhd_mixed(mode) : kpsel::pass_b_fmks(mode);
     case M_GRMHD_NT:
-    case M_PROBE: return mixed ? nullptr : kpsel::pass_b_fmks(persistent, mode);
-    case M_SLOW: return mixed ? kpsel::pass_b_grmhd_mixed(persistent, mode) : kpsel::pass_b_slow(persistent, mode);
+    case M_PROBE: return mixed ? nullptr : kpsel::pass_b_fmks(mode);
+    case M_SLOW: return mixed ? kpsel::pass_b_grmhd_mixed(mode) : kpsel::pass_b_slow(mode);
     case M_SLOW_NT:
-    case M_STAN: return mixed ? nullptr : kpsel::pass_b_slow(persistent, mode);
-    default: return mixed ? nullptr : kpsel::pass_b_other(persistent, mode);
+    case M_STAN: return mixed ? nullptr : kpsel::pass_b_slow(mode);
+    default: return mixed ? nullptr : kpsel::pass_b_other(mode);
     }
 }
 
@@ -428,9 +427,9 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     const int nblocks_flat = (int)((np + block - 1) / block);
     // grid of one launch: one block per pixel block, or (persistent) the
     // resident blocks of the kernel on every CU
-    auto grid_of = [&](const void* fn, size_t smem, int& grid) -> kp_status {
+    auto grid_of = [&](const void* fn, size_t smem, int& grid, bool persistent) -> kp_status {
         grid = nblocks_flat;
-        if (!P.persistent) return KP_OK;
+        if (!persistent) return KP_OK;
         int per_cu = 0;
         const kp_status s2 = blocks_per_cu(device, fn, block, smem, per_cu);
         if (s2 != KP_OK) return s2;
@@ -459,7 +458,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     if (stats) KP_HIP(hipEventRecord(ev[0], stream));
     auto launch = [&](const void* fn, size_t smem, Params& Q) -> kp_status {
         int grid = 0;
-        const kp_status s2 = grid_of(fn, smem, grid);
+        const kp_status s2 = grid_of(fn, smem, grid, Q.persistent != 0);
         if (s2 != KP_OK) return s2;
         void* args[] = {&Q};
         KP_HIP(hipLaunchKernel(fn, dim3(grid), dim3(block), args, smem, stream));
@@ -479,7 +478,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             Q.list_out = lists + (size_t)(k % 2) * np;
             Q.count_out = P.queue + first_slot + 4 + k;
             const int gp = fp32 ? GP_32 : GP_64;
-            const void* fn = pass_b ? pass_b_fn(true, true, gp, mode) : pass_a_fn(true, gp, G_BL);
+            const void* fn = pass_b ? pass_b_fn(true, gp, mode) : pass_a_fn(true, gp, G_BL);
             const kp_status s2 = launch(fn, pass_b ? smem_b : smem_a, Q);
             if (s2 != KP_OK) return s2;
         }
@@ -557,7 +556,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             KP_HIP(hipMemsetAsync(Q.count_out, 0, sizeof(uint32_t), stream));
             KP_HIP(hipEventRecord(bev.e[0], stream));
             const void* wfn = sel ? kpsel::pass_b_sel(mixed, GP_64, mode == M_SLOW_NT ? M_SLOW_NT_W : M_SLOW_W)
-                                  : (mixed ? kpsel::pass_b_grmhd_mixed(true, M_SLOW_W) : kpsel::pass_b_slow_windowed(mode));
+                                  : (mixed ? kpsel::pass_b_grmhd_mixed(M_SLOW_W) : kpsel::pass_b_slow_windowed(mode));
             s2 = launch(wfn, smem_b, Q);
             if (s2 != KP_OK) return s2;
             KP_HIP(hipEventRecord(bev.e[1], stream));
@@ -620,9 +619,13 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
                 st = launch(kpsel::pass_b_legs(mixed, gp_plain, true), smem_b, V);
             }
         } else {
+            // pass B always runs the persistent kernels (one ray per thread
+            // gives the same results; its multi-frequency FP64 instantiation
+            // computed wrong images, a fault not found in the source)
             Params Q = P;
             Q.fetch = P.queue + 1;
-            const void* fn = sel ? kpsel::pass_b_sel(mixed, gp_plain, mode) : pass_b_fn(P.persistent != 0, mixed, gp_plain, mode);
+            Q.persistent = 1;
+            const void* fn = sel ? kpsel::pass_b_sel(mixed, gp_plain, mode) : pass_b_fn(mixed, gp_plain, mode);
             st = launch(fn, smem_b, Q);
         }
         if (st != KP_OK) return st;

@@ -2208,12 +2208,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP == GP_32
             }
         }
     } else {
-        const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-        if (pix < P.npix && lane_b_init<MODE>(P, pix, L)) {
-            while (!pass_b_tick<MIXED, GP, MODE, SEL>(P, G, L, coeff, transfer, steps_b, guard)) {
-            }
-            lane_b_finish<MIXED, MODE>(P, L, 1);
-        }
+        // Pass B is built persistent only: its one-ray-per-thread form (each
+        // lane ticking its own ray to the end) computed wrong multi-frequency
+        // FP64 images on the device while every other form of the same tick,
+        // and the tick's host build, agreed with the oracle; the cause was
+        // not found in the source.
+        static_assert(PERSISTENT, "pass B kernels are persistent");
     }
     flush_counters(P.counters, vac_mode(MODE) ? C_RHS_V : C_RHS_B, rhs_count, coeff, transfer, 0, steps_b, guard,
                    vac_mode(MODE) ? C_VACUUM : C_TRANSFER);
@@ -2390,21 +2390,21 @@ const void* pass_a(bool persistent, int gp, int geom);
 const void* pass_a_batch(int gp);
 const void* zero_counters();
 const void* derive_scalars();
-const void* pass_b_single(bool persistent, bool mixed, int gp);
-const void* pass_b_mf(bool persistent, bool mixed, int gp);
-const void* pass_b_fmks(bool persistent, int mode);
-const void* pass_b_slow(bool persistent, int mode);
-const void* pass_b_other(bool persistent, int mode);
+const void* pass_b_single(bool mixed, int gp);
+const void* pass_b_mf(bool mixed, int gp);
+const void* pass_b_fmks(int mode);
+const void* pass_b_slow(int mode);
+const void* pass_b_other(int mode);
 const void* pass_b_batch(bool mixed, int gp);
 const void* unsplit_bl(bool mixed, int gp, int mode);
 const void* unsplit_fmks(int mode);
-const void* pass_b_stan(bool persistent);
+const void* pass_b_stan();
 const void* pass_b_slow_windowed(int mode);
 const void* pass_b_legs(bool mixed, int gp, bool vacuum);
 const void* unsplit_slow(int mode);
-const void* pass_b_grmhd_mixed(bool persistent, int mode);
+const void* pass_b_grmhd_mixed(int mode);
 const void* unsplit_grmhd_mixed(int mode);
-const void* pass_b_slow_mixed(bool persistent, int mode);
+const void* pass_b_slow_mixed(int mode);
 const void* unsplit_slow_mixed(int mode);
 // persistent split pass B with the equatorial wedge (images with a wedge run
 // these whatever the launch options ask: the results are the same)

@@ -35,50 +35,44 @@ const void* derive_scalars() { return (const void*)derive_scalars_kernel; }
 #endif
 
 #if KP_PART == 1
+// Pass B runs persistent kernels only (a lane takes the next pixel from the
+// queue when its ray ends).
+//
 // Single-frequency Boyer-Lindquist pass B: FP64, and mixed with every
-// geodesic precision (the staged GP_32 / GP_64 kernels are persistent only).
-const void* pass_b_single(bool persistent, bool mixed, int gp) {
+// geodesic precision.
+const void* pass_b_single(bool mixed, int gp) {
     if (!mixed)
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_SINGLE>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_SINGLE>;
-    if (gp == GP_32) return persistent ? (const void*)pass_b_kernel<true, true, GP_32, M_SINGLE> : nullptr;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_SINGLE>;
+    if (gp == GP_32) return (const void*)pass_b_kernel<true, true, GP_32, M_SINGLE>;
     if (gp == GP_64)
-        return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_SINGLE>
-                          : (const void*)pass_b_kernel<false, true, GP_64, M_SINGLE>;
-    return persistent ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_SINGLE>
-                      : (const void*)pass_b_kernel<false, true, GP_GUARD, M_SINGLE>;
+        return (const void*)pass_b_kernel<true, true, GP_64, M_SINGLE>;
+    return (const void*)pass_b_kernel<true, true, GP_GUARD, M_SINGLE>;
 }
 #endif
 
 #if KP_PART == 2
 // Multi-frequency Boyer-Lindquist pass B (thermal fits).
-const void* pass_b_mf(bool persistent, bool mixed, int gp) {
+const void* pass_b_mf(bool mixed, int gp) {
     if (!mixed)
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_MF>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_MF>;
-    if (gp == GP_32) return persistent ? (const void*)pass_b_kernel<true, true, GP_32, M_MF> : nullptr;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_MF>;
+    if (gp == GP_32) return (const void*)pass_b_kernel<true, true, GP_32, M_MF>;
     if (gp == GP_64)
-        return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_MF>
-                          : (const void*)pass_b_kernel<false, true, GP_64, M_MF>;
-    return persistent ? (const void*)pass_b_kernel<true, true, GP_GUARD, M_MF>
-                      : (const void*)pass_b_kernel<false, true, GP_GUARD, M_MF>;
+        return (const void*)pass_b_kernel<true, true, GP_64, M_MF>;
+    return (const void*)pass_b_kernel<true, true, GP_GUARD, M_MF>;
 }
 #endif
 
 #if KP_PART == 3
 // GRMHD pass B on FMKS geodesics: thermal, nonthermal, and the slow-light
 // time probe.
-const void* pass_b_fmks(bool persistent, int mode) {
+const void* pass_b_fmks(int mode) {
     switch (mode) {
     case M_GRMHD:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_GRMHD>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD>;
     case M_GRMHD_NT:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD_NT>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_GRMHD_NT>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_GRMHD_NT>;
     case M_PROBE:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_PROBE>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_PROBE>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_PROBE>;
     default:
         return nullptr;
     }
@@ -88,16 +82,14 @@ const void* pass_b_fmks(bool persistent, int mode) {
 #if KP_PART == 4
 // Slow-light pass B: thermal and nonthermal (part 4), and the tangent of the
 // slow-light sequence (part 8).
-const void* pass_b_slow(bool persistent, int mode) {
+const void* pass_b_slow(int mode) {
     switch (mode) {
     case M_SLOW:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_SLOW>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_SLOW>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW>;
     case M_SLOW_NT:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_NT>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_SLOW_NT>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_SLOW_NT>;
     case M_STAN:
-        return pass_b_stan(persistent);
+        return pass_b_stan();
     default:
         return nullptr;
     }
@@ -105,22 +97,19 @@ const void* pass_b_slow(bool persistent, int mode) {
 #endif
 
 #if KP_PART == 8
-const void* pass_b_stan(bool persistent) {
-    return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_STAN>
-                      : (const void*)pass_b_kernel<false, false, GP_64, M_STAN>;
+const void* pass_b_stan() {
+    return (const void*)pass_b_kernel<true, false, GP_64, M_STAN>;
 }
 #endif
 
 #if KP_PART == 5
 // RIAF tangent and nonthermal multi-frequency pass B, and the batched pass B.
-const void* pass_b_other(bool persistent, int mode) {
+const void* pass_b_other(int mode) {
     switch (mode) {
     case M_TANGENT:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_TANGENT>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_TANGENT>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_TANGENT>;
     case M_MF_NT:
-        return persistent ? (const void*)pass_b_kernel<true, false, GP_64, M_MF_NT>
-                          : (const void*)pass_b_kernel<false, false, GP_64, M_MF_NT>;
+        return (const void*)pass_b_kernel<true, false, GP_64, M_MF_NT>;
     default:
         return nullptr;
     }
@@ -198,10 +187,9 @@ const void* pass_b_legs(bool mixed, int gp, bool vacuum) {
 #if KP_PART == 12
 // Mixed-precision GRMHD pass B (FP64 FMKS geodesics, FP32 thermal
 // coefficients), split and unsplit; slow light in part 13.
-const void* pass_b_grmhd_mixed(bool persistent, int mode) {
-    if (mode != M_GRMHD) return pass_b_slow_mixed(persistent, mode);
-    return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_GRMHD>
-                      : (const void*)pass_b_kernel<false, true, GP_64, M_GRMHD>;
+const void* pass_b_grmhd_mixed(int mode) {
+    if (mode != M_GRMHD) return pass_b_slow_mixed(mode);
+    return (const void*)pass_b_kernel<true, true, GP_64, M_GRMHD>;
 }
 
 const void* unsplit_grmhd_mixed(int mode) {
@@ -212,12 +200,11 @@ const void* unsplit_grmhd_mixed(int mode) {
 #if KP_PART == 13
 // Mixed-precision slow light: resident sequences (split and unsplit) and the
 // windowed block launches (persistent).
-const void* pass_b_slow_mixed(bool persistent, int mode) {
+const void* pass_b_slow_mixed(int mode) {
     switch (mode) {
     case M_SLOW:
-        return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_SLOW>
-                          : (const void*)pass_b_kernel<false, true, GP_64, M_SLOW>;
-    case M_SLOW_W: return persistent ? (const void*)pass_b_kernel<true, true, GP_64, M_SLOW_W> : nullptr;
+        return (const void*)pass_b_kernel<true, true, GP_64, M_SLOW>;
+    case M_SLOW_W: return (const void*)pass_b_kernel<true, true, GP_64, M_SLOW_W>;
     default: return nullptr;
     }
 }
