@@ -630,6 +630,42 @@ kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* params, const k
                                          int32_t nfreq, const kp_run_options* options, kp_image_data* out,
                                          kp_image_stats* stats, kp_slow_light_schedule* schedule);
 
+/* One observation time of a multi-observer run (SlowLightBatchJob,
+ * src/image/slow_light_batch_driver.hpp:13-17): the image at
+ * observation_time from snapshots first..last (inclusive support,
+ * 0 <= first < last < nsnap). */
+typedef struct kp_slow_light_job {
+    double observation_time;
+    int32_t first;
+    int32_t last;
+} kp_slow_light_job;
+
+/* Several observation times from one pass over a streamed sequence
+ * (run_slow_light_batch_metric_impl with several observer jobs,
+ * src/image/slow_light_batch_driver.hpp:105-340): pass A runs once (the
+ * geodesics do not depend on the time), every job starts its own rays from
+ * those endpoints, and each block of snapshots runs the pass B of every job
+ * whose support overlaps it, on the part of the block inside that support.
+ * The blocks start at the earliest job's first snapshot and end at the
+ * latest job's last; a decoupled rewind goes to the earliest time any job's
+ * waiting ray needs.  outs[j] is job j's image (nfreq planes, like `out` of
+ * kp_run_grmhd_slow_light_stream); sequence->observation_time is not read.
+ * Decoupled and snapshot stepping give each job the image of
+ * kp_run_grmhd_slow_light_stream over its own snapshots first..last; block
+ * stepping's image depends on where the blocks fall, as the reference's
+ * does, and equals that run's when the job's first snapshot is the earliest
+ * job's.  The job state (about the workspace of one image per job) is
+ * allocated by the call.  stats sums over the jobs (rays = npix * njobs);
+ * schedule->blocks counts the pass-B launches.  Errors as the reference
+ * (:133-148): no jobs, a non-finite time, first >= last or last >= nsnap ->
+ * KP_INVALID_ARGUMENT; a decoupled sample outside a job's support ->
+ * KP_TIME_SUPPORT. */
+kp_status kp_run_grmhd_slow_light_stream_jobs(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                              const kp_slow_light_stream* sequence, const kp_slow_light_job* jobs,
+                                              int32_t njobs, const double* frequencies, int32_t nfreq,
+                                              const kp_run_options* options, kp_image_data* outs,
+                                              kp_image_stats* stats, kp_slow_light_schedule* schedule);
+
 /* Slow-light image and its derivatives along the fixed geodesics with
  * respect to (q_density, R_high, q_magnetic): q_density scales n_e by e^q and
  * B by e^(q/2), q_magnetic scales B by e^q (PlasmaPerturbation,

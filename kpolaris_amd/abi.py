@@ -199,6 +199,11 @@ class SlowLightStream(C.Structure):
     ]
 
 
+class SlowLightJob(C.Structure):
+    """kp_slow_light_job (SlowLightBatchJob, src/image/slow_light_batch_driver.hpp:13-17)."""
+    _fields_ = [("observation_time", C.c_double), ("first", C.c_int32), ("last", C.c_int32)]
+
+
 class SlowLightSchedule(C.Structure):
     _fields_ = [("blocks", C.c_int32), ("rewinds", C.c_int32), ("loads", C.c_int32), ("slots", C.c_int32),
                 ("slot_bytes", C.c_int64), ("load_ms", C.c_double), ("wait_ms", C.c_double),

@@ -228,6 +228,13 @@ kp_status cu_count(int device, int& out) {
     return KP_OK;
 }
 
+// Observation times of one windowed slow-light run (outs[j] is job j's image).
+struct SlowJobs {
+    const kp_slow_light_job* jobs = nullptr;
+    int32_t njobs = 0;
+    kp_image_data* outs = nullptr;
+};
+
 }  // namespace
 
 // grmhd != NULL selects the GRMHD grid model on FMKS geodesics (model unused).
@@ -236,7 +243,8 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
                            int perturbation, double log_scale, const Tangent* tangent = nullptr,
                            const kp_grmhd_model* grmhd = nullptr, const kp_slow_light* slow = nullptr,
                            kp_slow_light_probe* probe = nullptr, bool noalloc = false,
-                           kp_slow::Source* source = nullptr, kp_slow_light_schedule* schedule = nullptr) {
+                           kp_slow::Source* source = nullptr, kp_slow_light_schedule* schedule = nullptr,
+                           const SlowJobs* jobs = nullptr) {
     kp_internal::clear_error();
     if (!out) return kp_internal::fail(KP_INVALID_ARGUMENT, "null output struct");
     kp_run_options opt;
@@ -411,6 +419,75 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             KP_HIP(hipMemcpyAsync(base + W.slow_c, slow->cells, nsnap * sizeof(void*), hipMemcpyHostToDevice, stream));
         }
     }
+    // Observation times of the windowed block loop, each with its own image
+    // and its own ray state between blocks: the pass-A endpoints it starts
+    // from (pass B parks rays there), parked rays, their lists and queue
+    // counters.  One observation time uses the workspace; several get a
+    // workspace-sized region each, and share the workspace's pass-A results.
+    struct SlowJob {
+        double t_obs = 0.0;
+        int first = 0, last = 0;  // inclusive snapshot support
+        kp_image_data out{};
+        double* ep = nullptr;
+        double* pk_d = nullptr;
+        int32_t* pk_i = nullptr;
+        double* pk_st = nullptr;
+        double* req_t = nullptr;
+        uint32_t* lists = nullptr;  // [2][npix]
+        uint32_t* queue = nullptr;  // fetch counter, list counts [2]
+        bool started = false, done = false;
+        int parity = 0;
+        uint32_t parked = 0;
+    };
+    std::vector<SlowJob> slow_jobs;
+    struct JobMemory {
+        void* p = nullptr;
+        ~JobMemory() {
+            if (p) (void)hipFree(p);
+        }
+    } job_mem;
+    if (jobs && !windowed) return kp_internal::fail(KP_INVALID_ARGUMENT, "observation-time jobs need the block loop");
+    if (windowed && !jobs) {
+        SlowJob J;
+        J.t_obs = slow->observation_time;
+        J.first = 0;
+        J.last = slow->nsnap - 1;
+        J.out = P.out;
+        J.ep = P.ep;
+        J.pk_d = P.pk_d;
+        J.pk_i = P.pk_i;
+        J.pk_st = P.pk_st;
+        J.req_t = P.req_t;
+        J.lists = lists;
+        J.queue = P.queue + 2;
+        slow_jobs.push_back(J);
+    } else if (windowed) {
+        const size_t per = W.total;
+        if (hipMalloc(&job_mem.p, per * (size_t)jobs->njobs) != hipSuccess) {
+            job_mem.p = nullptr;
+            (void)hipGetLastError();
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "hipMalloc of the state of %d observation times (%llu bytes) failed",
+                          (int)jobs->njobs, (unsigned long long)(per * (size_t)jobs->njobs));
+            return kp_internal::fail(KP_OUT_OF_MEMORY, msg);
+        }
+        for (int j = 0; j < jobs->njobs; ++j) {
+            char* b = static_cast<char*>(job_mem.p) + per * (size_t)j;
+            SlowJob J;
+            J.t_obs = jobs->jobs[j].observation_time;
+            J.first = jobs->jobs[j].first;
+            J.last = jobs->jobs[j].last;
+            J.out = jobs->outs[j];
+            J.ep = reinterpret_cast<double*>(b + W.ep);
+            J.pk_d = reinterpret_cast<double*>(b + W.pkd);
+            J.pk_i = reinterpret_cast<int32_t*>(b + W.pki);
+            J.pk_st = reinterpret_cast<double*>(b + W.pkst);
+            J.req_t = reinterpret_cast<double*>(b + W.req);
+            J.lists = reinterpret_cast<uint32_t*>(b + W.lists);
+            J.queue = reinterpret_cast<uint32_t*>(b + W.queue);
+            slow_jobs.push_back(J);
+        }
+    }
     const int block = opt.block_size > 0 ? opt.block_size : 128;
     if (block % 64 != 0 || block > 256)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "block_size must be a multiple of 64 and at most 256");
@@ -505,15 +582,23 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     }
     if (stats) KP_HIP(hipEventRecord(ev[1], stream));
     // Windowed slow light: the block loop of run_slow_light_batch_metric_impl
-    // (slow_light_batch_driver.hpp:211-340) for one observation time.  Block
-    // b holds snapshots res_lo..stop (stop = start + windows_per_block, the
-    // halo kept from the block before it for decoupled stepping) and runs the
-    // rays the block before it parked (every pixel in the first); a
-    // decoupled ray that needs an earlier time than its block holds makes
-    // the next block start at that time's snapshot (:317-330).
+    // (slow_light_batch_driver.hpp:211-340).  Block b holds snapshots
+    // res_lo..stop (stop = start + windows_per_block, the halo kept from the
+    // block before it for decoupled stepping) and runs, for every job whose
+    // support overlaps it (:237-251), the rays that job's last launch parked
+    // (every pixel at its first launch) on the snapshots of the block inside
+    // its support; a job whose launch parks nothing is done.  A decoupled
+    // ray that needs an earlier time than its block holds makes the next
+    // block start at the snapshot of the earliest such time over the jobs
+    // (:286-330).
     double blocks_ms = 0.0;
     auto run_blocks = [&]() -> kp_status {
-        const int n = slow->nsnap, end = n - 1;
+        const int n = slow->nsnap;
+        int begin = n - 1, end = 0;
+        for (const SlowJob& J : slow_jobs) {
+            begin = std::min(begin, J.first);
+            end = std::max(end, J.last);
+        }
         const int wpb = std::max(1, slow->windows_per_block);
         const bool decoupled = slow->step_mode == KP_SLOW_LIGHT_DECOUPLED;
         struct BlockEvents {
@@ -524,18 +609,30 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
             }
         } bev;
         for (int i = 0; i < 2; ++i) KP_HIP(hipEventCreate(&bev.e[i]));
+        // every job starts from the pass-A endpoints (its own copy: pass B
+        // parks rays over them)
+        for (SlowJob& J : slow_jobs)
+            if (J.ep != P.ep)
+                KP_HIP(hipMemcpyAsync(J.ep, P.ep, 16 * np * sizeof(double), hipMemcpyDeviceToDevice, stream));
         std::vector<const void*> ptrs;
         std::vector<uint32_t> parked_list;
         std::vector<double> req;
+        std::vector<SlowJob*> ran;
         kp_slow::BlockPlan plan;
-        int parity = 0, blocks = 0, rewinds = 0;
-        bool first = true;
+        plan.start = plan.res_lo = begin;
+        int launches = 0, rewinds = 0;
+        size_t remaining = slow_jobs.size();
+        const double* const times_dev = reinterpret_cast<const double*>(base + W.slow_t);
+        const kp_grmhd_cell* const* const cells_dev = reinterpret_cast<const kp_grmhd_cell* const*>(base + W.slow_c);
+        const void* wfn = sel ? kpsel::pass_b_sel(mixed, GP_64, mode == M_SLOW_NT ? M_SLOW_NT_W : M_SLOW_W)
+                              : (mixed ? kpsel::pass_b_grmhd_mixed(M_SLOW_W) : kpsel::pass_b_slow_windowed(mode));
         // every block advances some ray by a step or finishes it, so the
         // loop ends; the bound only turns a logic error into an error
-        const long long max_blocks = (long long)(n + 1) * (long long)std::max(P.max_steps, 1) + 16;
+        const long long max_blocks =
+            (long long)(n + 1) * (long long)std::max(P.max_steps, 1) * (long long)slow_jobs.size() + 16;
         for (long long iter = 0;; ++iter) {
             if (iter > max_blocks) return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light block loop did not end");
-            const int stop = kp_slow::block_stop(plan, wpb, n);
+            const int stop = kp_slow::block_stop(plan, wpb, end + 1);
             const int res_lo = plan.res_lo;
             kp_status s2 = source->acquire(res_lo, stop, stream, ptrs);
             if (s2 != KP_OK) return s2;
@@ -544,53 +641,75 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
                                   hipMemcpyHostToDevice, stream));
             KP_HIP(hipMemcpyAsync(base + W.slow_c, ptrs.data(), (size_t)cnt * sizeof(void*), hipMemcpyHostToDevice,
                                   stream));
-            Params Q = P;
-            Q.nsnap = cnt;
-            Q.slow_final = stop == end ? 1 : 0;
-            Q.fetch = P.queue + 2;
-            Q.list_in = first ? nullptr : lists + (size_t)(1 - parity) * np;
-            Q.count_in = first ? nullptr : P.queue + 3 + (1 - parity);
-            Q.list_out = lists + (size_t)parity * np;
-            Q.count_out = P.queue + 3 + parity;
-            KP_HIP(hipMemsetAsync(Q.fetch, 0, sizeof(uint32_t), stream));
-            KP_HIP(hipMemsetAsync(Q.count_out, 0, sizeof(uint32_t), stream));
             KP_HIP(hipEventRecord(bev.e[0], stream));
-            const void* wfn = sel ? kpsel::pass_b_sel(mixed, GP_64, mode == M_SLOW_NT ? M_SLOW_NT_W : M_SLOW_W)
-                                  : (mixed ? kpsel::pass_b_grmhd_mixed(M_SLOW_W) : kpsel::pass_b_slow_windowed(mode));
-            s2 = launch(wfn, smem_b, Q);
-            if (s2 != KP_OK) return s2;
+            ran.clear();
+            for (SlowJob& J : slow_jobs) {
+                if (J.done || stop <= J.first || res_lo >= J.last) continue;
+                // the block's snapshots inside the job's support: lo < hi
+                const int lo = std::max(res_lo, J.first), hi = std::min(stop, J.last);
+                Params Q = P;
+                Q.out = J.out;
+                Q.ep = J.ep;
+                Q.pk_d = J.pk_d;
+                Q.pk_i = J.pk_i;
+                Q.pk_st = J.pk_st;
+                Q.req_t = J.req_t;
+                Q.t_obs = J.t_obs;
+                Q.sup_lo = slow->times[J.first];
+                Q.sup_hi = slow->times[J.last];
+                Q.snap_t = times_dev + (lo - res_lo);
+                Q.snaps = cells_dev + (lo - res_lo);
+                Q.nsnap = hi - lo + 1;
+                Q.slow_final = hi == J.last ? 1 : 0;
+                Q.fetch = J.queue;
+                Q.list_in = J.started ? J.lists + (size_t)(1 - J.parity) * np : nullptr;
+                Q.count_in = J.started ? J.queue + 1 + (1 - J.parity) : nullptr;
+                Q.list_out = J.lists + (size_t)J.parity * np;
+                Q.count_out = J.queue + 1 + J.parity;
+                KP_HIP(hipMemsetAsync(Q.fetch, 0, sizeof(uint32_t), stream));
+                KP_HIP(hipMemsetAsync(Q.count_out, 0, sizeof(uint32_t), stream));
+                s2 = launch(wfn, smem_b, Q);
+                if (s2 != KP_OK) return s2;
+                KP_HIP(hipMemcpyAsync(&J.parked, Q.count_out, sizeof(J.parked), hipMemcpyDeviceToHost, stream));
+                ran.push_back(&J);
+                ++launches;
+            }
             KP_HIP(hipEventRecord(bev.e[1], stream));
-            ++blocks;
             // the next block's snapshots upload while this one runs
             if (stop < end) {
                 s2 = source->prefetch(stop + 1, stop + std::min(wpb, end - stop));
                 if (s2 != KP_OK) return s2;
             }
-            uint32_t parked = 0;
-            KP_HIP(hipMemcpyAsync(&parked, Q.count_out, sizeof(parked), hipMemcpyDeviceToHost, stream));
             KP_HIP(hipStreamSynchronize(stream));
             float ms = 0.f;
             KP_HIP(hipEventElapsedTime(&ms, bev.e[0], bev.e[1]));
             blocks_ms += ms;
-            if (parked == 0) break;
-            if (stop == end && !decoupled)
-                return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light rays parked after the last block");
-            first = false;
-            parity ^= 1;
             double earliest = slow->times[end];
-            if (decoupled) {
-                // slow_light_pending_range (slow_light_batch_driver.hpp:81-103)
-                parked_list.resize(parked);
-                req.resize(np);
-                KP_HIP(hipMemcpy(parked_list.data(), lists + (size_t)(1 - parity) * np, parked * sizeof(uint32_t),
-                                 hipMemcpyDeviceToHost));
-                KP_HIP(hipMemcpy(req.data(), P.req_t, np * sizeof(double), hipMemcpyDeviceToHost));
-                for (uint32_t pix : parked_list) earliest = std::min(earliest, req[pix]);
+            for (SlowJob* J : ran) {
+                J->started = true;
+                if (J->parked == 0) {
+                    J->done = true;
+                    --remaining;
+                    continue;
+                }
+                if (stop >= J->last && !decoupled)
+                    return kp_internal::fail(KP_INVALID_ARGUMENT, "slow-light rays parked after the last block");
+                J->parity ^= 1;
+                if (decoupled) {
+                    // slow_light_pending_range (slow_light_batch_driver.hpp:81-103)
+                    parked_list.resize(J->parked);
+                    req.resize(np);
+                    KP_HIP(hipMemcpy(parked_list.data(), J->lists + (size_t)(1 - J->parity) * np,
+                                     J->parked * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                    KP_HIP(hipMemcpy(req.data(), J->req_t, np * sizeof(double), hipMemcpyDeviceToHost));
+                    for (uint32_t pix : parked_list) earliest = std::min(earliest, req[pix]);
+                }
             }
-            if (kp_slow::advance_plan(plan, stop, decoupled, earliest, slow->times, n)) ++rewinds;
+            if (remaining == 0) break;
+            if (kp_slow::advance_plan(plan, stop, decoupled, earliest, slow->times, end + 1)) ++rewinds;
         }
         if (schedule) {
-            schedule->blocks = blocks;
+            schedule->blocks = launches;
             schedule->rewinds = rewinds;
             schedule->pass_b_ms = blocks_ms;
         }
@@ -654,7 +773,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         stats->pass_b_ms = b_ms;
         stats->pass_b_vacuum_ms = v_ms;
         stats->kernel_ms = double(a_ms) + double(b_ms);
-        stats->rays = P.npix;
+        stats->rays = P.npix * (int64_t)std::max<size_t>(1, slow_jobs.size());
         stats->rhs_evals = (int64_t)(cnt[C_RHS_A] + cnt[C_RHS_B] + cnt[C_RHS_V]);
         stats->vacuum_rhs_evals = (int64_t)cnt[C_RHS_V];
         stats->pass_a_rhs_evals = (int64_t)cnt[C_RHS_A];
@@ -699,20 +818,32 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
         KP_HIP(hipMemcpyAsync(&waiting, P.counters + C_WAIT, sizeof(waiting), hipMemcpyDeviceToHost, stream));
         KP_HIP(hipStreamSynchronize(stream));
         if (waiting) {
+            // the first job (observation time) with such rays names its support
             std::vector<double> req(np);
-            KP_HIP(hipMemcpy(req.data(), P.req_t, np * sizeof(double), hipMemcpyDeviceToHost));
-            double lo = 1e300, hi = -1e300;
-            for (double t : req)
-                if (t == t) {
-                    lo = std::min(lo, t);
-                    hi = std::max(hi, t);
-                }
-            char msg[256];
-            std::snprintf(msg, sizeof(msg),
-                          "decoupled slow-light radiation sample outside supplied physical time support: %llu rays "
-                          "requested %.17g..%.17g, available %.17g..%.17g",
-                          waiting, lo, hi, slow->times[0], slow->times[slow->nsnap - 1]);
-            return kp_internal::fail(KP_TIME_SUPPORT, msg);
+            const int njob = std::max<int>(1, (int)slow_jobs.size());
+            for (int j = 0; j < njob; ++j) {
+                const bool one = slow_jobs.empty();
+                KP_HIP(hipMemcpy(req.data(), one ? P.req_t : slow_jobs[j].req_t, np * sizeof(double),
+                                 hipMemcpyDeviceToHost));
+                double lo = 1e300, hi = -1e300;
+                for (double t : req)
+                    if (t == t) {
+                        lo = std::min(lo, t);
+                        hi = std::max(hi, t);
+                    }
+                if (!(lo <= hi)) continue;
+                const int first = one ? 0 : slow_jobs[j].first;
+                const int last = one ? slow->nsnap - 1 : slow_jobs[j].last;
+                char msg[300];
+                std::snprintf(msg, sizeof(msg),
+                              "decoupled slow-light radiation sample outside supplied physical time support: %llu rays "
+                              "requested %.17g..%.17g, available %.17g..%.17g (observation time %.17g)",
+                              waiting, lo, hi, slow->times[first], slow->times[last],
+                              one ? slow->observation_time : slow_jobs[j].t_obs);
+                return kp_internal::fail(KP_TIME_SUPPORT, msg);
+            }
+            return kp_internal::fail(KP_TIME_SUPPORT,
+                                     "decoupled slow-light radiation sample outside supplied physical time support");
         }
     }
     return KP_OK;
@@ -1209,10 +1340,11 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
                      source);
 }
 
-extern "C" kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* params, const kp_grmhd_model* model,
-                                                    const kp_slow_light_stream* sequence, const double* frequencies,
-                                                    int32_t nfreq, const kp_run_options* options, kp_image_data* out,
-                                                    kp_image_stats* stats, kp_slow_light_schedule* schedule) {
+// The streamed runs: one observation time (jobs null) or several.
+static kp_status run_stream(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                            const kp_slow_light_stream* sequence, const SlowJobs* jobs, const double* frequencies,
+                            int32_t nfreq, const kp_run_options* options, kp_image_data* out, kp_image_stats* stats,
+                            kp_slow_light_schedule* schedule) {
     kp_internal::clear_error();
     if (!model || !sequence) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model or sequence");
     const kp_slow_light_stream& q = *sequence;
@@ -1227,10 +1359,21 @@ extern "C" kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* para
         return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light step mode");
     if (q.interpolation != KP_SLOW_LIGHT_FLUID && q.interpolation != KP_SLOW_LIGHT_COEFFICIENTS)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light interpolation");
-    if (!std::isfinite(q.observation_time)) return kp_internal::fail(KP_INVALID_ARGUMENT, "observation time must be finite");
+    if (!jobs && !std::isfinite(q.observation_time))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "observation time must be finite");
     for (int k = 0; k < q.nsnap; ++k)
         if (!std::isfinite(q.times[k]) || (k > 0 && !(q.times[k] > q.times[k - 1])))
             return kp_internal::fail(KP_INVALID_ARGUMENT, "snapshot times must be finite and strictly increasing");
+    if (jobs) {
+        // slow_light_batch_driver.hpp:116, 144-146
+        if (jobs->njobs < 1 || !jobs->jobs || !jobs->outs)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "no slow-light observation-time jobs");
+        for (int j = 0; j < jobs->njobs; ++j) {
+            const kp_slow_light_job& J = jobs->jobs[j];
+            if (!std::isfinite(J.observation_time) || J.first < 0 || J.first >= J.last || J.last >= q.nsnap)
+                return kp_internal::fail(KP_INVALID_ARGUMENT, "invalid slow-light batch observer time or snapshot support");
+        }
+    }
     if (nfreq < 0 || (nfreq > 0 && !frequencies))
         return kp_internal::fail(KP_INVALID_ARGUMENT, "frequency list and count disagree");
     if (model->n1 < 1 || model->n2 < 1 || model->n3 < 1)
@@ -1271,11 +1414,35 @@ extern "C" kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* para
     }
     kp_slow_light_schedule rep;
     std::memset(&rep, 0, sizeof(rep));
-    st = run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, &seq, nullptr, false, &source, &rep);
+    st = run_image(params, nullptr, fl, options, jobs ? jobs->outs : out, stats, 0, 0.0, nullptr, &m, &seq, nullptr,
+                   false, &source, &rep, jobs);
     if (st != KP_OK && st != KP_TIME_SUPPORT) return st;
     source.report(rep);
     if (schedule) *schedule = rep;
     return st;
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_stream(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                    const kp_slow_light_stream* sequence, const double* frequencies,
+                                                    int32_t nfreq, const kp_run_options* options, kp_image_data* out,
+                                                    kp_image_stats* stats, kp_slow_light_schedule* schedule) {
+    return run_stream(params, model, sequence, nullptr, frequencies, nfreq, options, out, stats, schedule);
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_stream_jobs(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                         const kp_slow_light_stream* sequence,
+                                                         const kp_slow_light_job* jobs, int32_t njobs,
+                                                         const double* frequencies, int32_t nfreq,
+                                                         const kp_run_options* options, kp_image_data* outs,
+                                                         kp_image_stats* stats, kp_slow_light_schedule* schedule) {
+    kp_internal::clear_error();
+    if (njobs < 1 || !jobs || !outs)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "no slow-light observation-time jobs");
+    SlowJobs J;
+    J.jobs = jobs;
+    J.njobs = njobs;
+    J.outs = outs;
+    return run_stream(params, model, sequence, &J, frequencies, nfreq, options, nullptr, stats, schedule);
 }
 
 extern "C" kp_status kp_run_grmhd_slow_light_image_gradient(const kp_pass_a_params* params,

@@ -118,6 +118,11 @@ def library():
                                                  P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData),
                                                  P(abi.ImageStats), P(abi.SlowLightSchedule)]
     L.kp_run_grmhd_slow_light_stream.restype = C.c_int32
+    L.kp_run_grmhd_slow_light_stream_jobs.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLightStream),
+                                                      P(abi.SlowLightJob), C.c_int32, P(C.c_double), C.c_int32,
+                                                      P(abi.RunOptions), P(abi.ImageData), P(abi.ImageStats),
+                                                      P(abi.SlowLightSchedule)]
+    L.kp_run_grmhd_slow_light_stream_jobs.restype = C.c_int32
     L.kp_slow_light_time_probe.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.RunOptions),
                                            P(abi.SlowLightProbe)]
     L.kp_run_grmhd_slow_light_image_gradient.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
@@ -668,6 +673,49 @@ def run_grmhd_slow_light_stream(params: abi.PassAParams, model: abi.GRMHDModel, 
         C.byref(params), C.byref(model), C.byref(q), f.ctypes.data_as(C.POINTER(C.c_double)), len(f),
         C.byref(opts), C.byref(img), C.byref(st) if with_stats else None, C.byref(rep)))
     return out, (st.as_dict() if with_stats else None), rep.as_dict()
+
+
+def run_grmhd_slow_light_stream_jobs(params: abi.PassAParams, model: abi.GRMHDModel, times, jobs, loader,
+                                     frequencies=None, *, step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
+                                     interpolation: int = abi.SLOW_LIGHT_FLUID, windows_per_block: int = 1,
+                                     prefetch: bool = True, prefetch_snapshots: int = 0, derive_scalars: bool = False,
+                                     user=None, block_size: int = 0, waves_per_cu: int = 0,
+                                     precision: int = abi.PRECISION_FP64, fields=abi.IMAGE_FIELDS,
+                                     with_stats: bool = True):
+    """Images at several observation times from one pass over a streamed
+    sequence (kp_run_grmhd_slow_light_stream_jobs): `jobs` is a list of
+    (observation_time, first, last) with the inclusive snapshot support of
+    each.  Returns (list of outputs, one per job, stats summed over the jobs,
+    schedule)."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    q = abi.SlowLightStream()
+    library().kp_slow_light_stream_default(C.byref(q))
+    q.nsnap = len(t)
+    q.step_mode = step_mode
+    q.interpolation = interpolation
+    q.windows_per_block = windows_per_block
+    q.prefetch = 1 if prefetch else 0
+    q.prefetch_snapshots = prefetch_snapshots
+    q.derive_scalars = 1 if derive_scalars else 0
+    q.times = t.ctypes.data_as(C.POINTER(C.c_double))
+    q.load = loader
+    q.user = user
+    table = (abi.SlowLightJob * max(1, len(jobs)))()
+    for j, (tobs, first, last) in enumerate(jobs):
+        table[j] = abi.SlowLightJob(float(tobs), int(first), int(last))
+    outs = [alloc_device_image(npix, fields=fields, nfreq=len(f)) for _ in jobs]
+    imgs = (abi.ImageData * max(1, len(jobs)))(*[image_struct_from_tensors(o) for o in outs])
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, True, block_size, waves_per_cu, C.c_void_p(stream))
+    st = abi.ImageStats()
+    rep = abi.SlowLightSchedule()
+    _check(library().kp_run_grmhd_slow_light_stream_jobs(
+        C.byref(params), C.byref(model), C.byref(q), table, len(jobs), f.ctypes.data_as(C.POINTER(C.c_double)),
+        len(f), C.byref(opts), imgs, C.byref(st) if with_stats else None, C.byref(rep)))
+    return outs, (st.as_dict() if with_stats else None), rep.as_dict()
 
 
 def slow_light_time_probe(params: abi.PassAParams, model: abi.GRMHDModel, *, persistent: bool = True) -> dict:

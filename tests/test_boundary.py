@@ -32,6 +32,7 @@ STRUCTS = {
     "kp_slow_light": abi.SlowLight,
     "kp_slow_light_stream": abi.SlowLightStream,
     "kp_slow_light_schedule": abi.SlowLightSchedule,
+    "kp_slow_light_job": abi.SlowLightJob,
 }
 
 
