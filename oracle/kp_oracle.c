@@ -27,6 +27,27 @@
  *   camera                              -u.k = 1, k.k = 0, e_i.e_j = delta_ij
  *   whole RIAF image (pass A + B)       end-to-end 8x8 golden fluxes of
  *                                       tests/test_hdf5_contract.py:671-681 (rel 2e-6)
+ *   FMKS map, metric, RHS               theta map and its inverse, g g^-1 = I and the
+ *                                       KS congruence, specialised RHS == connection
+ *                                       from finite differences of the metric
+ *   GRMHD sampling, derived scalars     trilinear exactness on linear fields and
+ *                                       ghosts; n_e, B, sigma, beta, Theta_e closed
+ *                                       forms of iharm_loader.cpp:420-505
+ *   kappa / power-law coefficients      Symphony checkpoints and the reference's
+ *                                       coefficient checks; 2F1 against scipy
+ *   slow-light time interpolation       interpolation rules, pair bisection by brute
+ *                                       force, identical snapshots == static image
+ *   direct_only, equatorial wedge       tests/test_direct_only.cpp and
+ *                                       tests/test_equatorial.cpp checks (bracket truth
+ *                                       table, dz/ds closed form, Kerr turn endpoints,
+ *                                       analytic slab behind a screen)
+ *   response accumulation               closed forms of
+ *                                       tests/test_physical_response.cpp:44-90
+ *   d/dR_high of slow-light images      central differences of the pinned image over
+ *                                       R_high (no reference output exists)
+ *   Dexter and kappa fit constants      PARITY UNPINNED to their printed digits: held
+ *                                       to the fits' accuracy (~1 %) against the exact
+ *                                       integrals
  */
 #include "kp_oracle.h"
 
