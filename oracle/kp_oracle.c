@@ -16,9 +16,11 @@
  *                                       pi/2 Faraday rotation, Jones-vector oracle)
  *   thermal Pandya coefficients         Symphony fast-fit checkpoints,
  *                                       tests/test_core.cpp:2188-2237 (rel 1e-5)
- *   thermal rho_V, Dexter fit           PARITY UNPINNED by reference vectors; checked
- *                                       only by Kirchhoff's law a=j/B_nu, sign and
- *                                       theta-symmetry invariants
+ *   thermal rho_Q/rho_V, Dexter fit     no reference vectors: exact Maxwell-Juettner
+ *                                       integrals, the large-X saddle point (exponent,
+ *                                       amplitudes), high-frequency and cold-plasma
+ *                                       limits, Kirchhoff's law a = j / B_nu
+ *                                       (tests/test_oracle_fit_pins.py)
  *   Bessel K0/K1/K2                     scipy.special.kv (NR polynomial accuracy 1e-7)
  *   BL metric, connection, RHS          g g^-1 = I, finite differences of gcov,
  *                                       specialised == generic connection
