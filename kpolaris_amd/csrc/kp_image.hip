@@ -332,8 +332,9 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     P.out = *out;
     // an image with an equatorial wedge runs the wedge kernels: persistent
     // split pass B (one ray per thread and unsplit launches give the same
-    // results)
-    const bool sel = P.wedge_hr > 0.0;
+    // results); the slow-light time probe is geometric, without emission
+    // selection (run_slow_light_time_probe_metric, slow_light_driver.hpp:632-780)
+    const bool sel = P.wedge_hr > 0.0 && !probe;
     P.persistent = opt.persistent || sel ? 1 : 0;
     int device = 0;
     KP_HIP(hipGetDevice(&device));
@@ -534,6 +535,7 @@ static kp_status run_image(const kp_pass_a_params* params, const kp_riaf_model* 
     KP_HIP(hipGetLastError());
     if (stats) KP_HIP(hipEventRecord(ev[0], stream));
     auto launch = [&](const void* fn, size_t smem, Params& Q) -> kp_status {
+        if (!fn) return kp_internal::fail(KP_UNSUPPORTED, "no kernel is built for this combination of options");
         int grid = 0;
         const kp_status s2 = grid_of(fn, smem, grid, Q.persistent != 0);
         if (s2 != KP_OK) return s2;
