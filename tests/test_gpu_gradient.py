@@ -162,3 +162,31 @@ def test_gpu_gradient_rejects_mixed_precision():
     p, m, _, _ = _opts(nx=4, ny=4)
     with pytest.raises(ValueError):
         image.run_riaf_image_gradient(p, m, (0.0, 0.0))
+
+
+def test_gpu_gradient_with_the_emission_wedge():
+    """The tangent kernel with the equatorial emission wedge (h/r = 0.1)
+    against the oracle's central differences of wedge images."""
+    import torch
+    from kpolaris_amd import image
+    print("case: gradient 16x16 with the wedge h/r 0.1 vs oracle central differences", flush=True)
+    p, m, _, _ = _opts(nx=16, ny=16)
+    p.emission_selection.equatorial_h_over_r = 0.1
+    out, grad, st = image.run_riaf_image_gradient(p, m, (0.0, 0.0, 0.0))
+    torch.cuda.synchronize()
+    got = {k: v.cpu().numpy() for k, v in out.items()}
+    ref = parity.oracle_with_sensitivity(p, m)
+    print(parity.assert_parity(got, ref, "wedge gradient value", max_ill=parity.WEDGE_MAX_ILL,
+                               step_rel=parity.WEDGE_STEP_REL), flush=True)
+    g = grad.cpu().numpy()
+    smooth = parity.smooth_for_derivatives(ref, ref["_cond"])
+    for k in range(3):
+        fd, same = _fd_oracle(p, m, k + 1)
+        ok = same & smooth
+        print(f"wedge gradient check on {int(ok.sum())}/{ok.size} pixels", flush=True)
+        assert ok.sum() >= parity.MIN_CHECKED * ok.size
+        for s, name in enumerate(parity.STOKES):
+            scale = np.abs(fd[name]).max()
+            err = np.abs(g[k, s][ok] - fd[name][ok]).max() / scale
+            print(f"wedge q{k} {name} max rel err {err:.2e}", flush=True)
+            assert err < 1e-6, (k, name, err)

@@ -100,3 +100,27 @@ def test_gpu_slow_light_wedge_blocks(precision):
     one = _np(out)
     assert 0.0 < one["image_i"].sum() < _np(full)["image_i"].sum()
     _blocks_stream_jobs(p, grids, host, times, one, f"wedge {precision}", precision=precision)
+
+
+def test_gpu_slow_light_three_frequencies():
+    """Three frequencies sharing the slow-light geodesics (the single-pass
+    frequency loop of the windowed kernel), FP64 and mixed: blocks, streamed
+    and jobs equal the one-block image; the FP64 image against the oracle."""
+    import oracle
+    from kpolaris_amd import grmhd_synth as gs, image
+    cells, times, model = _sequence()
+    grids, host = _grids(model, cells)
+    p = gs.pass_a_params(model, nx=10, ny=10, camera_radius=200.0, outer_radius=30.0)
+    F = [86e9, 230e9, 345e9]
+    for precision in (abi.PRECISION_FP64, abi.PRECISION_MIXED):
+        print(f"case: slow light 3 frequencies precision {precision}", flush=True)
+        out, _ = image.run_grmhd_slow_light_image(p, grids, times, T_OBS, F, precision=precision)
+        one = _np(out)
+        if precision == abi.PRECISION_FP64:
+            run = lambda q, t=T_OBS: oracle.run_grmhd_slow_light_image(q, model, host, times, t, F)[0]  # noqa: E731
+            ref = run(p)
+            cond = parity.sensitivity(run, p, ref, parity.time_shifts(lambda t: run(p, t), T_OBS))
+            print(parity.assert_parity_grid(one, ref, 3, "gpu slow light 3 frequencies", cond, time_cut_steps=1),
+                  flush=True)
+        _blocks_stream_jobs(p, grids, host, times, one, f"3 frequencies {precision}", frequencies=F,
+                            precision=precision)
