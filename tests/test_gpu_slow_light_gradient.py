@@ -50,12 +50,24 @@ def _fd(p, model, cells, times, kind, h):
     return {n: (res[0][n] - res[1][n]) / step for n in STOKES}, same
 
 
-def test_gpu_slow_light_gradient_against_oracle(seq):
+@pytest.mark.parametrize("hr", [0.0, 0.1])
+def test_gpu_slow_light_gradient_against_oracle(seq, hr):
+    """hr > 0: with the equatorial emission wedge (its kernel; the wedge
+    images' parity constants for the value).  The wedge keeps the midplane's
+    emission, where the Faraday depth is largest, so Q and U curve more with
+    the density scale and R_high: its central differences use steps ten times
+    smaller (h = 1e-5 and 5e-6; at h = 1e-4 the density derivative's O(h^2)
+    term is 7.5e-4 of Q's scale and the Richardson value's O(h^4) term 2e-7,
+    at 2e-5 the R_high derivative's O(h^2) term is 1.1e-4 and the Richardson
+    value agrees to 1.2e-8)."""
     import oracle
     import torch
-    from kpolaris_amd import image
+    from kpolaris_amd import abi, image
     grids, host, times, model, p = seq
-    print("case: slow-light gradient 12x12 vs oracle central differences", flush=True)
+    p = abi.PassAParams.from_buffer_copy(p)
+    p.emission_selection.equatorial_h_over_r = hr
+    wedge = dict(max_ill=parity.WEDGE_MAX_ILL, step_rel=parity.WEDGE_STEP_REL) if hr > 0.0 else {}
+    print(f"case: slow-light gradient 12x12 wedge {hr} vs oracle central differences", flush=True)
     out, grad, st = image.run_grmhd_slow_light_image_gradient(p, grids, times, T_OBS)
     torch.cuda.synchronize()
     got = {k: v.cpu().numpy() for k, v in out.items()}
@@ -63,14 +75,16 @@ def test_gpu_slow_light_gradient_against_oracle(seq):
     ref, _, _ = oracle.run_grmhd_slow_light_image(p, model, host, times, T_OBS)
     run = lambda q, t=T_OBS: oracle.run_grmhd_slow_light_image(q, model, host, times, t)[0]  # noqa: E731
     cond = parity.sensitivity(run, p, ref, parity.time_shifts(lambda t: run(p, t), T_OBS))
-    print(parity.assert_parity_grid(got, ref, 1, "gpu slow-light tangent value", cond, time_cut_steps=1), flush=True)
+    print(parity.assert_parity_grid(got, ref, 1, "gpu slow-light tangent value", cond, time_cut_steps=1, **wedge),
+          flush=True)
     # the derivatives are compared on the pixels that the oracle's own
     # sensitivity leaves well-conditioned (every component) and whose step
     # counts equal the oracle's
     same_steps = got["steps"] == ref["steps"]
     for kind in range(3):
-        fd, same = _fd(p, model, host, times, kind, 1e-4)
-        fd2, same2 = _fd(p, model, host, times, kind, 5e-5)
+        h = 1e-4 if hr == 0.0 else 1e-5
+        fd, same = _fd(p, model, host, times, kind, h)
+        fd2, same2 = _fd(p, model, host, times, kind, 0.5 * h)
         for s, name in enumerate(STOKES):
             ok = same & same2 & same_steps & parity.smooth_for_derivatives(ref, cond, 1)
             print(f"gradient check of {name} on {int(ok.sum())}/{ok.size} pixels", flush=True)
