@@ -49,6 +49,26 @@ def test_noalloc_equals_allocating_entry_point_twice():
             assert (out["reason"].cpu().numpy() == 1).mean() > 0.9
 
 
+def test_noalloc_with_selection_equals_allocating_entry_point():
+    """The equatorial emission wedge and direct_only (the wedge kernels on the
+    scratch array): equal to the allocating entry point's image bit for bit."""
+    import torch
+    from kpolaris_amd import abi, image
+    for prec in (abi.PRECISION_FP64, abi.PRECISION_MIXED):
+        print(f"case: noalloc wedge 0.1 + direct_only precision {prec} 48x40", flush=True)
+        p, m = _job(48, 40)
+        p.emission_selection.equatorial_h_over_r = 0.1
+        p.direct_only = 1
+        out = image.alloc_device_image(48 * 40)
+        for t in out.values():
+            t.fill_(7)
+        image.run_riaf_image_noalloc(p, m, out, precision=prec)
+        ref, _ = image.run_riaf_image(p, m, precision=prec, with_stats=False)
+        torch.cuda.synchronize()
+        for k in ref:
+            assert np.array_equal(out[k].cpu().numpy(), ref[k].cpu().numpy(), equal_nan=True), (prec, k)
+
+
 def test_noalloc_rejections():
     from kpolaris_amd import abi, image
     n = int(abi.NOALLOC_MAX_PIXELS ** 0.5) + 1
