@@ -178,3 +178,37 @@ def test_gpu_selection_option_rules():
     a, b = _host(a), _host(b)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_gpu_selection_multifrequency_and_mixed_grmhd():
+    """The multi-frequency wedge kernels: RIAF at three frequencies in FP64
+    against the oracle, and in mixed precision against the FP64 wedge image;
+    the mixed GRMHD wedge image against the FP64 one, per frequency, at the
+    mixed bars (net polarisation held in absolute terms on these coarse
+    images)."""
+    import oracle
+    from kpolaris_amd import grmhd_synth as gs, image
+    F = [86e9, 230e9, 345e9]
+    p, m = _riaf(16)
+    p = _with(p, hr=0.1)
+    print("case: RIAF 16x16 3 freqs wedge 0.1 FP64 and mixed", flush=True)
+    out, _ = image.run_riaf_multifrequency_image(p, m, F)
+    got = _host(out)
+    run = lambda q: oracle.run_riaf_multifrequency_image(q, m, F)[0]  # noqa: E731
+    ref = run(p)
+    cond = parity.sensitivity(run, p, ref)
+    print(parity.assert_parity_grid(got, ref, 3, "gpu RIAF 3 freqs wedge", cond, max_ill=parity.WEDGE_MAX_ILL,
+                                    step_rel=parity.WEDGE_STEP_REL), flush=True)
+    mx, _ = image.run_riaf_multifrequency_image(p, m, F, precision=abi.PRECISION_MIXED)
+    g, r = parity.split_frequencies(_host(mx), 3), parity.split_frequencies(got, 3)
+    for f in range(3):
+        print(parity.assert_mixed(g[f], r[f], f"gpu mixed RIAF wedge f{f}", pol="absolute"), flush=True)
+    cells, gm = gs.make_snapshot(40, 24, 24, seed=3)
+    grid = image.GRMHDGrid(gm, cells)
+    q = _with(gs.pass_a_params(gm, nx=16, ny=16), hr=0.1)
+    print("case: GRMHD 16x16 3 freqs wedge 0.1 mixed vs FP64", flush=True)
+    a, _ = image.run_grmhd_image(q, grid, F)
+    b, _ = image.run_grmhd_image(q, grid, F, precision=abi.PRECISION_MIXED)
+    g, r = parity.split_frequencies(_host(b), 3), parity.split_frequencies(_host(a), 3)
+    for f in range(3):
+        print(parity.assert_mixed(g[f], r[f], f"gpu mixed GRMHD wedge f{f}", pol="absolute"), flush=True)
