@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define KP_ABI_VERSION 6
+#define KP_ABI_VERSION 7
 
 /* Largest frequency list of one shared-geodesic image (the reference's
  * KPOLARIS_MAX_FREQUENCIES build option, CMakeLists.txt:14). */
@@ -665,6 +665,18 @@ kp_status kp_run_grmhd_slow_light_stream_jobs(const kp_pass_a_params* params, co
                                               int32_t njobs, const double* frequencies, int32_t nfreq,
                                               const kp_run_options* options, kp_image_data* outs,
                                               kp_image_stats* stats, kp_slow_light_schedule* schedule);
+
+/* The same observation-time jobs over resident device snapshots
+ * (kp_slow_light, as kp_run_grmhd_slow_light_image): blocks of
+ * sequence->windows_per_block pairs, 0 = one block over the sequence;
+ * sequence->observation_time is not read.  Each job equals
+ * kp_run_grmhd_slow_light_image's image over its own snapshots as the
+ * streamed form does, and the streamed form's image of the same jobs. */
+kp_status kp_run_grmhd_slow_light_image_jobs(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                             const kp_slow_light* sequence, const kp_slow_light_job* jobs,
+                                             int32_t njobs, const double* frequencies, int32_t nfreq,
+                                             const kp_run_options* options, kp_image_data* outs,
+                                             kp_image_stats* stats);
 
 /* Slow-light image and its derivatives along the fixed geodesics with
  * respect to (q_density, R_high, q_magnetic): q_density scales n_e by e^q and

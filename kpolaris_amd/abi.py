@@ -64,7 +64,7 @@ PRECISION_MIXED = 1
 FP64_RADIUS_DEFAULT = 2.5
 
 # include/kp_image.h KP_ABI_VERSION this mirror matches
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)

@@ -1304,10 +1304,11 @@ extern "C" kp_status kp_run_grmhd_multifrequency_image(const kp_pass_a_params* p
     return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, model);
 }
 
-extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
-                                                    const kp_slow_light* sequence, const double* frequencies,
-                                                    int32_t nfreq, const kp_run_options* options,
-                                                    kp_image_data* out, kp_image_stats* stats) {
+// The resident runs: one observation time (jobs null) or several.
+static kp_status run_resident(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                              const kp_slow_light* sequence, const SlowJobs* jobs, const double* frequencies,
+                              int32_t nfreq, const kp_run_options* options, kp_image_data* out,
+                              kp_image_stats* stats) {
     kp_internal::clear_error();
     if (!model || !sequence) return kp_internal::fail(KP_INVALID_ARGUMENT, "null GRMHD model or sequence");
     const kp_slow_light& q = *sequence;
@@ -1318,7 +1319,18 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
         return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light step mode");
     if (q.interpolation != KP_SLOW_LIGHT_FLUID && q.interpolation != KP_SLOW_LIGHT_COEFFICIENTS)
         return kp_internal::fail(KP_INVALID_ARGUMENT, "unknown slow-light interpolation");
-    if (!std::isfinite(q.observation_time)) return kp_internal::fail(KP_INVALID_ARGUMENT, "observation time must be finite");
+    if (!jobs && !std::isfinite(q.observation_time))
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "observation time must be finite");
+    if (jobs) {
+        // slow_light_batch_driver.hpp:116, 144-146
+        if (jobs->njobs < 1 || !jobs->jobs || !jobs->outs)
+            return kp_internal::fail(KP_INVALID_ARGUMENT, "no slow-light observation-time jobs");
+        for (int j = 0; j < jobs->njobs; ++j) {
+            const kp_slow_light_job& J = jobs->jobs[j];
+            if (!std::isfinite(J.observation_time) || J.first < 0 || J.first >= J.last || J.last >= q.nsnap)
+                return kp_internal::fail(KP_INVALID_ARGUMENT, "invalid slow-light batch observer time or snapshot support");
+        }
+    }
     for (int k = 0; k < q.nsnap; ++k) {
         if (!q.cells[k]) return kp_internal::fail(KP_INVALID_ARGUMENT, "null snapshot cells");
         if (!std::isfinite(q.times[k]) || (k > 0 && !(q.times[k] > q.times[k - 1])))
@@ -1337,9 +1349,38 @@ extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* param
     // blocks shorter than the sequence run the block scheduler over the
     // resident snapshots
     kp_slow::Resident resident(q.cells);
+    if (jobs) {
+        // jobs run in the block loop: windows_per_block 0 is one block
+        kp_slow_light seq = q;
+        if (seq.windows_per_block == 0 || seq.windows_per_block > q.nsnap - 1) seq.windows_per_block = q.nsnap - 1;
+        return run_image(params, nullptr, fl, options, jobs->outs, stats, 0, 0.0, nullptr, &m, &seq, nullptr, false,
+                         &resident, nullptr, jobs);
+    }
     kp_slow::Source* source = (q.windows_per_block > 0 && q.windows_per_block < q.nsnap - 1) ? &resident : nullptr;
     return run_image(params, nullptr, fl, options, out, stats, 0, 0.0, nullptr, &m, sequence, nullptr, false,
                      source);
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_image(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                    const kp_slow_light* sequence, const double* frequencies,
+                                                    int32_t nfreq, const kp_run_options* options,
+                                                    kp_image_data* out, kp_image_stats* stats) {
+    return run_resident(params, model, sequence, nullptr, frequencies, nfreq, options, out, stats);
+}
+
+extern "C" kp_status kp_run_grmhd_slow_light_image_jobs(const kp_pass_a_params* params, const kp_grmhd_model* model,
+                                                        const kp_slow_light* sequence, const kp_slow_light_job* jobs,
+                                                        int32_t njobs, const double* frequencies, int32_t nfreq,
+                                                        const kp_run_options* options, kp_image_data* outs,
+                                                        kp_image_stats* stats) {
+    kp_internal::clear_error();
+    if (njobs < 1 || !jobs || !outs)
+        return kp_internal::fail(KP_INVALID_ARGUMENT, "no slow-light observation-time jobs");
+    SlowJobs J;
+    J.jobs = jobs;
+    J.njobs = njobs;
+    J.outs = outs;
+    return run_resident(params, model, sequence, &J, frequencies, nfreq, options, nullptr, stats);
 }
 
 // The streamed runs: one observation time (jobs null) or several.

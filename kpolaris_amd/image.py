@@ -111,6 +111,10 @@ def library():
                                                 P(C.c_double), C.c_int32, P(abi.RunOptions), P(abi.ImageData),
                                                 P(abi.ImageStats)]
     L.kp_run_grmhd_slow_light_image.restype = C.c_int32
+    L.kp_run_grmhd_slow_light_image_jobs.argtypes = [P(abi.PassAParams), P(abi.GRMHDModel), P(abi.SlowLight),
+                                                     P(abi.SlowLightJob), C.c_int32, P(C.c_double), C.c_int32,
+                                                     P(abi.RunOptions), P(abi.ImageData), P(abi.ImageStats)]
+    L.kp_run_grmhd_slow_light_image_jobs.restype = C.c_int32
     L.kp_slow_light_default.argtypes = [P(abi.SlowLight)]
     L.kp_slow_light_stream_default.argtypes = [P(abi.SlowLightStream)]
     L.kp_slow_light_stream_default.restype = None
@@ -613,6 +617,35 @@ def run_grmhd_slow_light_image(params: abi.PassAParams, grids, times, observatio
         C.byref(params), C.byref(model), C.byref(seq), f.ctypes.data_as(C.POINTER(C.c_double)), len(f),
         C.byref(opts), C.byref(img), C.byref(st) if with_stats else None))
     return out, (st.as_dict() if with_stats else None)
+
+
+def run_grmhd_slow_light_image_jobs(params: abi.PassAParams, grids, times, jobs, frequencies=None, *,
+                                    step_mode: int = abi.SLOW_LIGHT_DECOUPLED,
+                                    interpolation: int = abi.SLOW_LIGHT_FLUID, windows_per_block: int = 0,
+                                    block_size: int = 0, waves_per_cu: int = 0,
+                                    precision: int = abi.PRECISION_FP64, fields=abi.IMAGE_FIELDS,
+                                    with_stats: bool = True):
+    """Images at several observation times over resident snapshots
+    (kp_run_grmhd_slow_light_image_jobs): `jobs` is a list of
+    (observation_time, first, last).  Returns (list of outputs, one per job,
+    stats summed over the jobs)."""
+    import torch
+    npix = params.camera.nx * params.camera.ny
+    model = grids[0].model
+    f = np.ascontiguousarray([model.freq_cgs] if frequencies is None else frequencies, dtype=np.float64)
+    seq, keep = _sequence(grids, times, 0.0, step_mode, interpolation, windows_per_block)
+    table = (abi.SlowLightJob * max(1, len(jobs)))()
+    for j, (tobs, first, last) in enumerate(jobs):
+        table[j] = abi.SlowLightJob(float(tobs), int(first), int(last))
+    outs = [alloc_device_image(npix, fields=fields, nfreq=len(f)) for _ in jobs]
+    imgs = (abi.ImageData * max(1, len(jobs)))(*[image_struct_from_tensors(o) for o in outs])
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = run_options(precision, True, block_size, waves_per_cu, C.c_void_p(stream))
+    st = abi.ImageStats()
+    _check(library().kp_run_grmhd_slow_light_image_jobs(
+        C.byref(params), C.byref(model), C.byref(seq), table, len(jobs), f.ctypes.data_as(C.POINTER(C.c_double)),
+        len(f), C.byref(opts), imgs, C.byref(st) if with_stats else None))
+    return outs, (st.as_dict() if with_stats else None)
 
 
 def host_cell_loader(snapshots):

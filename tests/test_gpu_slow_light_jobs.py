@@ -118,3 +118,29 @@ def test_gpu_jobs_option_rules(seq):
         image.run_grmhd_slow_light_stream_jobs(p, model, times, [(t, 0, 15), (late, 0, 8)], loader,
                                                windows_per_block=2)
     assert e.value.status == abi.KP_TIME_SUPPORT
+
+
+@pytest.mark.parametrize("wpb", [0, 2])
+def test_gpu_resident_jobs_equal_streamed_jobs(seq, wpb):
+    """kp_run_grmhd_slow_light_image_jobs over resident device snapshots (one
+    block, or blocks of two pairs): each job equals the streamed jobs' image
+    and the resident single-observation-time run over its own snapshots."""
+    from kpolaris_amd import image
+    model, host, times, p, probe = seq
+    grids = [image.GRMHDGrid(model, c) for c in host]
+    jobs = [(_t_obs(times, probe, 0, 15), 0, 15), (_t_obs(times, probe, 3, 14, 0.3), 3, 14)]
+    print(f"case: resident observation jobs windows_per_block {wpb} {jobs}", flush=True)
+    outs, st = image.run_grmhd_slow_light_image_jobs(p, grids, times, jobs, windows_per_block=wpb)
+    assert st["rays"] == 2 * p.camera.nx * p.camera.ny
+    streamed, _, _ = image.run_grmhd_slow_light_stream_jobs(p, model, times, jobs, image.host_cell_loader(host),
+                                                            windows_per_block=max(wpb, 1))
+    for j, (t, first, last) in enumerate(jobs):
+        got = _np(outs[j])
+        single, _ = image.run_grmhd_slow_light_image(p, grids[first:last + 1], times[first:last + 1], t)
+        ref = _np(single)
+        for k in KEYS:
+            assert np.array_equal(got[k], ref[k]), (j, k)
+            assert np.array_equal(got[k], _np(streamed[j])[k]), (j, k)
+    with pytest.raises(image.KPError) as e:
+        image.run_grmhd_slow_light_image_jobs(p, grids, times, [(jobs[0][0], 0, 16)])
+    assert e.value.status == abi.KP_INVALID_ARGUMENT
